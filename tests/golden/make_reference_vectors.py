@@ -1,0 +1,117 @@
+"""Extract the known-answer vectors that hmpc's own tests hold for this path into
+reference_vectors.json (data only: inputs and expected outputs).
+
+Sources (read as text, nothing is compiled or run):
+  tests/crypto/chacha.cpp:9-196        RFC 8439 quarter round, block function, 5 extra vectors
+  tests/ints/mod.cpp:12-40             Montgomery constants for the 150-bit prime p
+  tests/ints/poly_mod.cpp:78-96        root / inverse root and 5 NTT outputs for x_i = i, N = 2^14
+  tests/ints/num/theory/root_of_unity.cpp:8-9   p = 1601, n = 64
+
+Usage: python tests/golden/make_reference_vectors.py [/root/reference]
+"""
+import json
+import os
+import re
+import sys
+
+ROOT = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def hexes(text):
+    return [int(h.replace("'", ""), 16) for h in re.findall(r"0x([0-9a-fA-F']+)", text)]
+
+
+def section(text, start, end=None):
+    i = text.index(start)
+    j = text.index(end, i) if end else len(text)
+    return text[i:j]
+
+
+def chacha_vectors():
+    src = open(os.path.join(ROOT, "tests/crypto/chacha.cpp")).read()
+    qr = hexes(section(src, 'SECTION("Quarter round")', 'SECTION("Block function")'))
+    assert len(qr) == 8
+    blk = section(src, 'SECTION("Block function")', "auto param")
+    nums = hexes(blk)
+    key, nonce, counter = nums[0:8], nums[8:11], nums[11]
+    state_init = nums[12:28]
+    out = nums[28:44]
+    assert len(nums) == 44
+    extra = section(src, 'SECTION("Additional test vectors for the block function")')
+    # key / nonce / counter evolve exactly as in the test body
+    vectors = []
+    k, n, c = [0] * 8, [0] * 3, 0
+    parts = re.split(r"// Test Vector #\d+", extra)[1:]
+    assert len(parts) == 5
+    for idx, part in enumerate(parts):
+        for m in re.finditer(r"counter\[0\] = 0x([0-9a-fA-F']+)", part):
+            c = int(m.group(1).replace("'", ""), 16)
+        for m in re.finditer(r"key\[(\d)\] = 0x([0-9a-fA-F']+)", part):
+            k[int(m.group(1))] = int(m.group(2).replace("'", ""), 16)
+        for m in re.finditer(r"nonce\[(\d)\] = 0x([0-9a-fA-F']+)", part):
+            n[int(m.group(1))] = int(m.group(2).replace("'", ""), 16)
+        expected = [int(v.replace("'", ""), 16) for v in re.findall(r"state\[\d+\] == 0x([0-9a-fA-F']+)", part)]
+        assert len(expected) == 16
+        # vector #2 is the second block of the same stream (counter advanced by the engine)
+        vectors.append({"key": list(k), "nonce": list(n), "counter": c + (1 if idx == 1 else 0), "block": expected})
+    return {
+        "quarter_round": {"in": qr[0:4], "out": qr[4:8]},
+        "block": {"key": key, "nonce": nonce, "counter": counter, "initial_state": state_init, "block": out},
+        "extra": vectors,
+    }
+
+
+def mod_vectors():
+    src = open(os.path.join(ROOT, "tests/ints/mod.cpp")).read()
+    p = int(re.search(r"constexpr auto p = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    r_mod_p = int(re.search(r"reduced_auxiliary_modulus == 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    m512 = int(re.search(r"fivetwelve_montgomery = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    big = int(re.search(r"mod_p\(0x(2faeadbe7a0195c011ac195ad10269830e80010+2a)_int\)\)", src).group(1), 16)
+    return {"p": p, "R_mod_p": r_mod_p, "mont_512": m512, "big_unsigned": big, "big_unsigned_value": 42,
+            "neg_one_value": p - 1}
+
+
+def poly_vectors():
+    src = open(os.path.join(ROOT, "tests/ints/poly_mod.cpp")).read()
+    p = int(re.search(r"constexpr auto p = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    root = int(re.search(r"<integer>\(y\.root\) == 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    iroot = int(re.search(r"<integer>\(ntt_y\.root\) == 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    outs = {int(i): int(v.replace("'", ""), 16) for i, v in
+            re.findall(r"ntt_x_elements\[index_type\{j, (\d+)\}\]\)\) == 0x([0-9a-f']+)_int", src)}
+    assert len(outs) == 5
+    return {"p": p, "log_n": 14, "root": root, "inverse_root": iroot, "ntt_of_iota": {str(k): v for k, v in outs.items()}}
+
+
+def root_vectors():
+    src = open(os.path.join(ROOT, "tests/ints/num/theory/root_of_unity.cpp")).read()
+    p = int(re.search(r"constexpr auto p = 0b([01']+)_int", src).group(1).replace("'", ""), 2)
+    n = 1 << int(re.search(r"n = 1 << (\d+)", src).group(1))
+    return {"p": p, "n": n}
+
+
+def main():
+    data = {
+        "source": "iko4/hmpc tests (see make_reference_vectors.py for file:line)",
+        "chacha20": chacha_vectors(),
+        "mod": mod_vectors(),
+        "poly_mod": poly_vectors(),
+        "root_of_unity": root_vectors(),
+    }
+
+    def hexify(o):
+        if isinstance(o, dict):
+            return {k: hexify(v) for k, v in o.items()}
+        if isinstance(o, list):
+            return [hexify(v) for v in o]
+        if isinstance(o, int) and o >= 1 << 32:
+            return hex(o)
+        return o
+
+    with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
+        json.dump(hexify(data), f, indent=1)
+        f.write("\n")
+
+
+if __name__ == "__main__":
+    main()

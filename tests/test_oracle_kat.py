@@ -1,0 +1,361 @@
+"""Pins the CPU oracle against hmpc's own known-answer tests and against the mathematics.
+
+Each check is chosen so that a plausible mistake in the oracle (wrong word order, a
+dropped carry, a wrong bit-reversal, a transposed operand) fails it.
+"""
+
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+from bigref import (P150, Q64, Q128A, Q128B, brv, chacha20_block, is_probable_prime, limbs_of,
+                    negacyclic_dft, quarter_round, schoolbook_negacyclic, stream_words, words_to_int)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VEC = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+
+
+def _int(x):
+    return int(x, 16) if isinstance(x, str) else int(x)
+
+
+# ------------------------------------------------------------------ ChaCha20 (tests/crypto/chacha.cpp)
+def test_quarter_round_rfc8439():
+    qr = VEC["chacha20"]["quarter_round"]
+    assert list(quarter_round(*qr["in"])) == qr["out"]
+
+
+def test_block_function_rfc8439():
+    b = VEC["chacha20"]["block"]
+    out = oracle.chacha20_block(b["key"], b["counter"], b["nonce"])
+    assert out.tolist() == b["block"]
+    assert chacha20_block(b["key"], b["counter"], b["nonce"]) == b["block"]
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_additional_block_vectors(idx):
+    v = VEC["chacha20"]["extra"][idx]
+    assert oracle.chacha20_block(v["key"], v["counter"], v["nonce"]).tolist() == v["block"]
+
+
+def test_stream_counter_advances_per_block():
+    # hmpc/crypto/chacha.hpp:159-163: counter += 1 after each block
+    v0, v1 = VEC["chacha20"]["extra"][0], VEC["chacha20"]["extra"][1]
+    s = oracle.chacha20_stream(v0["key"], v0["nonce"], 0, 32)
+    assert s[:16].tolist() == v0["block"] and s[16:].tolist() == v1["block"]
+
+
+def test_chacha_random_keys_match_textbook():
+    rng = random.Random(7)
+    for _ in range(20):
+        key = [rng.getrandbits(32) for _ in range(8)]
+        nonce = [rng.getrandbits(32) for _ in range(3)]
+        ctr = rng.getrandbits(32)
+        assert oracle.chacha20_block(key, ctr, nonce).tolist() == chacha20_block(key, ctr, nonce)
+
+
+# ------------------------------------------------------------------ Montgomery (tests/ints/mod.cpp)
+def test_mod_constants_p150():
+    v = VEC["mod"]
+    p = _int(v["p"])
+    m = oracle.Modulus(p)
+    assert m.limbs == 5 and m.bit_size == 150
+    nprime, r1, r2 = m.constants()
+    assert r1 == _int(v["R_mod_p"]) == (1 << 160) % p
+    assert r2 == (1 << 320) % p
+    assert (p * nprime + 1) % (1 << 32) == 0
+    assert m.from_unsigned(0) == 0
+    assert m.from_unsigned(1) == r1
+    assert m.from_unsigned(512) == _int(v["mont_512"])
+    assert m.to_unsigned(_int(v["mont_512"])) == 512
+    x = _int(v["mont_512"])
+    for k in range(1, 6):
+        assert m.to_unsigned(m.pow(x, k)) == 512 ** k
+    assert m.to_unsigned(m.from_unsigned(p)) == 0
+    assert m.to_unsigned(m.from_unsigned(p + 1)) == 1
+    assert m.to_unsigned(m.from_unsigned(_int(v["big_unsigned"]))) == v["big_unsigned_value"]
+    assert m.to_unsigned(m.from_small_signed(-1)) == _int(v["neg_one_value"])
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A, Q128B, P150, 1601, 0x8822D80623320001])
+def test_montgomery_ops_closed_form(q):
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    Rinv = pow(R, -1, q)
+    rng = random.Random(q & 0xFFFF)
+    for _ in range(50):
+        a, b = rng.randrange(q), rng.randrange(q)
+        assert m.mul(a, b) == a * b * Rinv % q
+        assert m.add(a, b) == (a + b) % q
+        assert m.sub(a, b) == (a - b) % q
+        assert m.from_unsigned(a) == a * R % q
+        assert m.to_unsigned(a) == a * Rinv % q
+        e = rng.getrandbits(70)
+        assert m.pow(a, e) == pow(a * Rinv, e, q) * R % q
+        if a:
+            assert m.invert(a) == pow(a * Rinv, -1, q) * R % q
+    for s in (-20, -1, 0, 1, 20):
+        assert m.from_small_signed(s) == (s % q) * R % q
+    # REDC with more rounds than limbs: v * 2^(-32 r) mod q (uniform sampling path)
+    for r in (m.limbs, m.limbs + 3):
+        v = rng.getrandbits(32 * r - 3)
+        assert m.montgomery_reduce(v, r, r) == v * pow(1 << (32 * r), -1, q) % q
+
+
+# ------------------------------------------------------------------ moduli of the workloads
+def test_workload_moduli_are_the_stated_primes():
+    assert is_probable_prime(Q64) and limbs_of(Q64) == 2
+    for q, k, L in ((Q128A, 17, 4), (Q128B, 15, 4)):
+        assert is_probable_prime(q) and q % (1 << k) == 1 and limbs_of(q) == L
+        # smallest prime > 2^124 with q = 1 mod 2^k
+        c = (1 << 124) + 1
+        while c < q:
+            assert not is_probable_prime(c), hex(c)
+            c += 1 << k
+    assert Q128B % (1 << 16) != 1  # cannot host N = 2^15 (needs 2N | q - 1)
+
+
+# ------------------------------------------------------------------ roots (poly_mod.cpp, root_of_unity.cpp)
+def python_root_search(q, log_degree):
+    """Closed form of hmpc/ints/num/theory/root_of_unity.hpp:33-53 from the all-zero ChaCha stream."""
+    L = limbs_of(q)
+    bits = (q - 1).bit_length()
+    R = 1 << (32 * L)
+    words = stream_words([0] * 8, [0] * 3, 0, 64 * L)
+    deg = 1 << log_degree
+    for k in range(64):
+        x = words_to_int(words[k * L:(k + 1) * L], bits)
+        if x > q - 1:
+            x -= q
+        if x == 0:
+            continue
+        a = x * pow(R, -1, q) % q
+        g = pow(a, (q - 1) // deg, q)
+        if pow(g, deg // 2, q) != 1:
+            return g * R % q, k
+    raise AssertionError
+
+
+def test_root_kat_p150():
+    v = VEC["poly_mod"]
+    p = _int(v["p"])
+    m = oracle.Modulus(p)
+    root = m.root_of_unity(v["log_n"] + 1)
+    iroot = m.root_of_unity(v["log_n"] + 1, inverse=True)
+    assert m.to_unsigned(root) == _int(v["root"])
+    assert m.to_unsigned(iroot) == _int(v["inverse_root"])
+    assert python_root_search(p, v["log_n"] + 1)[0] == root
+
+
+def test_root_of_unity_1601():
+    v = VEC["root_of_unity"]
+    m = oracle.Modulus(v["p"])
+    n = v["n"]
+    root = m.root_of_unity(n.bit_length() - 1)
+    one = m.from_unsigned(1)
+    for i in range(1, n):
+        assert m.pow(root, i) != one
+    assert m.pow(root, n) == one
+    assert m.pow(root, 0) == one
+
+
+@pytest.mark.parametrize("q,log_n", [(Q64, 12), (Q64, 16), (Q128A, 16), (Q128A, 12), (Q128B, 14)])
+def test_root_closed_form_workload_moduli(q, log_n):
+    m = oracle.Modulus(q)
+    assert m.root_of_unity(log_n + 1) == python_root_search(q, log_n + 1)[0]
+
+
+def test_roots_table_layout():
+    q, log_n = Q64, 5
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    N = 1 << log_n
+    phi = m.root_of_unity(log_n + 1) * pow(R, -1, q) % q
+    fwd = oracle.planar_to_ints(m.roots(log_n))
+    inv = oracle.planar_to_ints(m.roots(log_n, inverse=True))
+    ninv = pow(N, -1, q)
+    assert fwd[0] == R % q
+    assert inv[0] == ninv * R % q
+    for i in range(1, N):
+        j = brv(i, log_n)
+        assert fwd[j] == pow(phi, i, q) * R % q
+        expect = pow(phi, -i, q) * (ninv if j == 1 else 1) % q
+        assert inv[j] == expect * R % q
+
+
+# ------------------------------------------------------------------ NTT (poly_mod.cpp:92-103 + math)
+def test_ntt_kat_p150():
+    v = VEC["poly_mod"]
+    p = _int(v["p"])
+    m = oracle.Modulus(p)
+    log_n = v["log_n"]
+    N = 1 << log_n
+    R = 1 << (32 * m.limbs)
+    x = oracle.ints_to_planar([(i % N) * R % p for i in range(2 * N)], m.limbs)
+    y = m.ntt(x, log_n, threads=4)
+    vals = oracle.planar_to_ints(y)
+    Rinv = pow(R, -1, p)
+    for j in range(2):
+        for k, want in v["ntt_of_iota"].items():
+            assert vals[j * N + int(k)] * Rinv % p == _int(want)
+    z = m.ntt(y, log_n, inverse=True, threads=4)
+    assert np.array_equal(z, x)
+
+
+def test_ntt_of_zero_is_zero():
+    m = oracle.Modulus(P150)
+    x = np.zeros((m.limbs, 2 << 10), np.uint32)
+    assert not m.ntt(x, 10).any() and not m.ntt(x, 10, inverse=True).any()
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A, Q128B])
+@pytest.mark.parametrize("log_n", [2, 3, 5])
+def test_ntt_closed_form_dft(q, log_n):
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    Rinv = pow(R, -1, q)
+    N = 1 << log_n
+    rng = random.Random(log_n)
+    a = [rng.randrange(q) for _ in range(3 * N)]
+    x = oracle.ints_to_planar([v * R % q for v in a], m.limbs)
+    y = oracle.planar_to_ints(m.ntt(x, log_n))
+    phi = m.root_of_unity(log_n + 1) * Rinv % q
+    for b in range(3):
+        want = negacyclic_dft(a[b * N:(b + 1) * N], phi, q)
+        assert [v * Rinv % q for v in y[b * N:(b + 1) * N]] == want
+
+
+@pytest.mark.parametrize("q,log_n", [(Q64, 6), (Q128A, 6), (Q128B, 7), (P150, 5)])
+def test_convolution_theorem_vs_schoolbook(q, log_n):
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    Rinv = pow(R, -1, q)
+    N = 1 << log_n
+    rng = random.Random(q % 1000 + log_n)
+    a = [rng.randrange(q) for _ in range(N)]
+    b = [rng.randrange(q) for _ in range(N)]
+    xa = oracle.ints_to_planar([v * R % q for v in a], m.limbs)
+    xb = oracle.ints_to_planar([v * R % q for v in b], m.limbs)
+    prod = m.ntt(m.pointwise("mul", m.ntt(xa, log_n), m.ntt(xb, log_n)), log_n, inverse=True)
+    want = schoolbook_negacyclic(a, b, q)
+    assert [v * Rinv % q for v in oracle.planar_to_ints(prod)] == want
+    assert np.array_equal(m.schoolbook_negacyclic(log_n, xa, xb), prod)
+
+
+def test_c1_roundtrip_and_product_n4096():
+    """Config: N=4096, q64, coefficients from the explicit-key stream (key 42)."""
+    m = oracle.Modulus(Q64)
+    log_n = 12
+    key = [42]
+    x = m.sample_uniform(key, 0, 4096)
+    y = m.sample_uniform(key, 1, 4096)
+    fx = m.ntt(x, log_n, threads=4)
+    assert np.array_equal(m.ntt(fx, log_n, inverse=True, threads=4), x)
+    prod = m.ntt(m.pointwise("mul", fx, m.ntt(y, log_n)), log_n, inverse=True)
+    assert np.array_equal(prod, m.schoolbook_negacyclic(log_n, x, y))
+
+
+# ------------------------------------------------------------------ samplers
+@pytest.mark.parametrize("q", [Q64, Q128A])
+def test_uniform_explicit_key_closed_form(q):
+    m = oracle.Modulus(q)
+    key = [42, 1, 2, 3, 4, 5, 6, 7]
+    count, offset = 37, 3
+    out = oracle.planar_to_ints(m.sample_uniform(key, offset, count))
+    bits = m.bit_size + 80
+    W = (bits + 31) // 32
+    r = max(m.limbs, W)
+    for i in range(count):
+        nonce_v = i + count * offset
+        words = chacha20_block(key, 0, [nonce_v & 0xFFFFFFFF, nonce_v >> 32, 0])[:W]
+        v = words_to_int(words, bits)
+        assert out[i] == v * pow(1 << (32 * r), -1, q) % q
+
+
+def test_uniform_is_statistically_uniform():
+    m = oracle.Modulus(Q64)
+    vals = np.array([v >> 60 for v in oracle.planar_to_ints(m.sample_uniform([9], 0, 1 << 14))])
+    counts = np.bincount(vals, minlength=16)
+    assert counts.min() > 850 and counts.max() < 1200
+
+
+@pytest.mark.parametrize("bits", [2, 20, 40])
+def test_cbd_explicit_key_closed_form(bits):
+    q = Q128A
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    key = [42]
+    count = 64
+    out = oracle.planar_to_ints(m.sample_cbd(key, 2, bits, count))
+    W = (bits + 31) // 32
+    for i in range(count):
+        nv = i + count * 2
+        words = chacha20_block(key + [0] * 7, 0, [nv & 0xFFFFFFFF, nv >> 32, 0])[:W]
+        val = bin(words_to_int(words, bits)).count("1") - bits // 2
+        assert abs(val) <= bits // 2
+        assert out[i] == (val % q) * R % q
+
+
+def test_cbd_capability_layout():
+    """Stage-0 fused sampling: one stream per butterfly, counter = i*N + tid, two draws."""
+    q = Q64
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    key = [42, 0, 0, 0, 0, 0, 0, 1]
+    nonce = [5, 0, 0]
+    log_n, n, bits = 4, 3, 40
+    N = 1 << log_n
+    out = oracle.planar_to_ints(m.sample_cbd_capability(key, nonce, bits, log_n, n))
+    for i in range(n):
+        for tid in range(N // 2):
+            words = stream_words(key, nonce, i * N + tid, 4)
+            lo = bin(words_to_int(words[0:2], 40)).count("1") - 20
+            hi = bin(words_to_int(words[2:4], 40)).count("1") - 20
+            assert out[i * N + tid] == (lo % q) * R % q
+            assert out[i * N + tid + N // 2] == (hi % q) * R % q
+
+
+# ------------------------------------------------------------------ BGV
+def _keygen(m, log_n, key, t_mont):
+    """Project-defined key generation: B uniform, s and e centred binomial (variance 1/2, 10)."""
+    N = 1 << log_n
+    B = m.sample_uniform(key, 0, N)
+    s = m.ntt(m.sample_cbd(key, 1, 2, N), log_n)
+    e = m.ntt(m.sample_cbd(key, 2, 40, N), log_n)
+    tcol = oracle.ints_to_planar([t_mont] * N, m.limbs)
+    A = m.pointwise("add", m.pointwise("mul", B, s), m.pointwise("mul", e, tcol))
+    return A, B, s
+
+
+@pytest.mark.parametrize("q,log_n", [(Q128B, 10), (Q64, 8)])
+def test_bgv_encrypt_decrypt_roundtrip(q, log_n):
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    key = [42]
+    t_mont = m.from_unsigned(1 << 32) if q > (1 << 64) else m.from_unsigned(2)
+    A, B, s = _keygen(m, log_n, key, t_mont)
+    n = 3
+    N = 1 << log_n
+    if q > (1 << 64):
+        msgs = oracle.sample_words(key, 3, n * N)
+    else:
+        msgs = (oracle.sample_words(key, 3, n * N) & 1).astype(np.uint32)
+    enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
+    c0, c1, nonce = m.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, [0, 0, 0], threads=4)
+    assert nonce.tolist() == [3 * n * N, 0, 0]
+    dec = m.bgv_decrypt(log_n, s, c0, c1)
+    if q > (1 << 64):
+        assert np.array_equal(dec, msgs)
+    else:
+        assert np.array_equal(dec & 1, msgs)
+    # c1 = B*u + t*w recomputed from the sampled randomness
+    u = m.ntt(m.sample_cbd_capability(enc_key, [0, 0, 0], 2, log_n, n), log_n)
+    w = m.ntt(m.sample_cbd_capability(enc_key, [2 * n * N, 0, 0], 40, log_n, n), log_n)
+    Bb = np.tile(B, (1, n))
+    tcol = oracle.ints_to_planar([t_mont] * (n * N), m.limbs)
+    want = m.pointwise("add", m.pointwise("mul", Bb, u), m.pointwise("mul", w, tcol))
+    assert np.array_equal(c1, want)
