@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the NTT / BGV path on one MI355X (prints ONE JSON line).
+
+Headline (`value`): batched negacyclic polynomial products, 512 pairs, N = 2^16,
+q = 2^124 + 2^20 + 2^18 + 1 (4 x 32-bit limbs): forward NTT of both inputs, point-wise
+Montgomery product, inverse NTT, inputs resident in HBM. Unit: Gcoeff/s = (sum over the
+three transforms of N) / time. Beside it the line carries the BGV encryption rate
+(1024 ciphertexts, N = 2^14, batched vs the plain per-ciphertext path), the N = 4096
+NTT->INTT latency p50/p99, the roofline of the dominant step and a CPU baseline (the
+oracle restatement of hmpc's algorithm on the host cores, bounded sample).
+
+Inputs are synthetic: explicit-key ChaCha20 streams with key {42, 0, ..., 0}.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+KEY = [42, 0, 0, 0, 0, 0, 0, 0]
+ENC_KEY = [42, 0, 0, 0, 0, 0, 0, 1]
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def percentile(xs, p):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), p))
+
+
+def time_steps(ctx, stream, fn, steps, warmup):
+    """W untimed steps, then K timed steps bracketed by synchronisation; per-step events."""
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    ctx.wait()
+    events = [ctx.event() for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    events[0].record(stream)
+    for k in range(steps):
+        fn()
+        events[k + 1].record(stream)
+    stream.synchronize()
+    ctx.wait()
+    wall = time.perf_counter() - t0
+    per = [events[k].elapsed_ms(events[k + 1]) for k in range(steps)]
+    total = events[0].elapsed_ms(events[-1])
+    return total / steps, per, wall
+
+
+def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
+    mod = hm.Modulus(ctx, hm.Q128A)
+    mod.prepare(log_n)
+    count = n << log_n
+    log(f"[bench] C2 polymul: {n} pairs, N=2^{log_n}, L={mod.limbs}")
+    a = mod.sample_uniform(KEY, 0, count)
+    b = mod.sample_uniform(KEY, 1, count)
+    out = mod.alloc(count)
+    ms, per, wall = time_steps(ctx, stream, lambda: mod.polymul(log_n, n, a, b, out, stream=stream), steps, warmup)
+    res = {"mod": mod, "a": a, "b": b, "out": out, "n": n, "log_n": log_n, "ms": ms, "per": per, "wall": wall}
+    # plain per-op path for the same product: per-stage launches, separate point-wise kernel
+    fa, fb = mod.alloc(count), mod.alloc(count)
+
+    def plain():
+        mod.ntt(log_n, n, a, fa, plain=True, stream=stream)
+        mod.ntt(log_n, n, b, fb, plain=True, stream=stream)
+        mod.pointwise(mod.MUL, count, fa, fb, out=fa, stream=stream)
+        mod.ntt(log_n, n, fa, out, inverse=True, plain=True, stream=stream)
+
+    pms, _, _ = time_steps(ctx, stream, plain, max(2, steps // 4), 1)
+    res["plain_ms"] = pms
+    del fa, fb
+    return res
+
+
+def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
+    mod = hm.Modulus(ctx, hm.Q128B)
+    mod.prepare(log_n)
+    N = 1 << log_n
+    log(f"[bench] C3 BGV: {n} ciphertexts, N=2^{log_n}")
+    t_mont = mod.mont(1 << 32)
+    B = mod.sample_uniform(KEY, 0, N)
+    s = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 1, 2, N))
+    e = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 2, 40, N))
+    A = mod.pointwise(mod.ADD, N, mod.pointwise(mod.MUL, N, B, s), mod.pointwise_scalar(mod.MUL, N, e, t_mont))
+    msgs = ctx.sample_words(KEY, 3, n * N)
+    c0, c1 = mod.alloc(n * N), mod.alloc(n * N)
+    ctx.set_rng(ENC_KEY, [0, 0, 0])
+    ms, per, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n, A, B, msgs, t_mont, c0=c0, c1=c1, stream=stream),
+                            steps, warmup)
+    # decrypt-and-compare on the last batch (size-independent property)
+    dec = mod.bgv_decrypt(log_n, n, s, c0, c1, stream=stream)
+    stream.synchronize()
+    ok = bool(np.array_equal(dec.to_numpy(n * N), msgs.to_numpy(n * N)))
+    pc0, pc1 = mod.alloc(n_plain * N), mod.alloc(n_plain * N)
+    pms, _, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n_plain, A, B, msgs, t_mont, c0=pc0, c1=pc1,
+                                                               plain=True, stream=stream), 2, 1)
+    return {"n": n, "ms": ms, "per": per, "enc_per_s": n / (ms / 1e3), "decrypt_ok": ok,
+            "plain_enc_per_s": n_plain / (pms / 1e3), "plain_sample": n_plain}
+
+
+def bench_c1(hm, ctx, stream, iters=200):
+    mod = hm.Modulus(ctx, hm.Q64)
+    mod.prepare(12)
+    x = mod.sample_uniform(KEY, 0, 4096)
+    y, z = mod.alloc(4096), mod.alloc(4096)
+    out = {}
+    for name, plain in (("fused", False), ("plain", True)):
+        def step():
+            mod.ntt(12, 1, x, y, plain=plain, stream=stream)
+            mod.ntt(12, 1, y, z, inverse=True, plain=plain, stream=stream)
+        _, per, _ = time_steps(ctx, stream, step, iters, 10)
+        out[name] = {"p50_us": percentile(per, 50) * 1e3, "p99_us": percentile(per, 99) * 1e3}
+    ok = np.array_equal(z.to_numpy((2, 4096)), x.to_numpy((2, 4096)))
+    out["roundtrip_ok"] = bool(ok)
+    return out
+
+
+def cpu_baseline(pm, seconds_target=10.0):
+    """Oracle restatement of hmpc's polymul on the host cores over a bounded sample of
+    the same inputs (the first pairs of the batch)."""
+    import oracle
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    om = oracle.Modulus(pm["mod"].q)
+    log_n = pm["log_n"]
+    N = 1 << log_n
+    n_total = pm["n"]
+    count = n_total * N
+    a_all = pm["a"].to_numpy((om.limbs, count))
+    b_all = pm["b"].to_numpy((om.limbs, count))
+    pairs = 2
+    _, secs = om.polymul_timed(log_n, np.ascontiguousarray(a_all[:, : pairs * N]),
+                               np.ascontiguousarray(b_all[:, : pairs * N]), threads)
+    pairs = int(min(n_total, max(2, pairs * seconds_target / max(secs, 1e-3))))
+    a = np.ascontiguousarray(a_all[:, : pairs * N])
+    b = np.ascontiguousarray(b_all[:, : pairs * N])
+    got, secs = om.polymul_timed(log_n, a, b, threads)
+    want = pm["out"].to_numpy((om.limbs, count))[:, : pairs * N]
+    return {"value": 3 * pairs * N / secs / 1e9, "unit": "Gcoeff/s", "cores": threads, "kind": "port",
+            "sample": f"{pairs} of {n_total} pairs (N=2^{log_n}, q128a), {secs:.2f} s wall",
+            "matches_gpu": bool(np.array_equal(got, want))}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3 (default all)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the workloads shard by independent batch elements); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import hmpc_amd as hm
+
+    with open(os.path.join(ROOT, "BASELINE.json")) as f:
+        baseline = json.load(f)
+    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3"}
+    ctx = hm.Context(0, key=KEY)
+    stream = ctx.stream()
+    info = ctx.info()
+    log(f"[bench] device {info['name']} {info['arch']} CUs={info['compute_units']}")
+
+    line = {"metric": baseline["metric"], "unit": "Gcoeff/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32", "data": "synthetic (explicit-key ChaCha20, key 42)"}
+    pm = None
+    if "c2" in sections:
+        pm = bench_polymul(hm, ctx, stream, args.steps, args.warmup)
+        n, log_n = pm["n"], pm["log_n"]
+        N = 1 << log_n
+        L = pm["mod"].limbs
+        coeffs = 3 * n * N
+        value = coeffs / (pm["ms"] / 1e3) / 1e9
+        alg_bytes = 3 * N * L * 4 * n  # read a, b and write the product once
+        achieved = alg_bytes / (pm["ms"] / 1e3) / 1e9
+        modmuls = n * (3 * (N // 2) * log_n + N + N // 2)
+        line.update({
+            "value": value, "ms_per_step": pm["ms"],
+            "p50_ms": percentile(pm["per"], 50), "p99_ms": percentile(pm["per"], 99),
+            "config": {"workload": "batched polymul INTT(NTT(a) o NTT(b)): 512 pairs, N=2^16, "
+                                   "q=2^124+2^20+2^18+1 (4x32-bit limbs)", "pairs": n, "N": N, "limbs": L},
+            "plain_path": {"ms_per_step": pm["plain_ms"], "value": coeffs / (pm["plain_ms"] / 1e3) / 1e9,
+                           "speedup_fused": pm["plain_ms"] / pm["ms"]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "polymul step (column pass, row pass, inverse column pass)",
+                         "algorithmic_bytes_per_step": alg_bytes},
+            "int_rate": {"modmul_per_s": modmuls / (pm["ms"] / 1e3), "modmuls_per_step": modmuls},
+        })
+    if "c3" in sections:
+        bg = bench_bgv(hm, ctx, stream, max(3, args.steps // 2), args.warmup)
+        line["bgv"] = {"workload": "BGV pk encryption, 1024 ciphertexts, N=2^14, q=0x1...78001 (4 limbs), t=2^32",
+                       "enc_per_s": bg["enc_per_s"], "ms_per_batch": bg["ms"],
+                       "p50_ms": percentile(bg["per"], 50), "p99_ms": percentile(bg["per"], 99),
+                       "plain_enc_per_s": bg["plain_enc_per_s"], "plain_sample": bg["plain_sample"],
+                       "batched_vs_plain": bg["enc_per_s"] / bg["plain_enc_per_s"], "decrypt_ok": bg["decrypt_ok"]}
+    if "c1" in sections:
+        line["c1_latency"] = bench_c1(hm, ctx, stream)
+    if pm is not None and not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle, host cores)")
+        line["cpu_baseline"] = cpu_baseline(pm, args.cpu_seconds)
+    if "value" not in line:
+        line.update({"value": None, "ms_per_step": None, "config": {"workload": ",".join(sorted(sections))}})
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,981 @@
+// C ABI of libhmpc_hip.so (declared in include/hmpc_hip.h): contexts, registered moduli with
+// resident twiddle tables, device memory, and the dispatch of every compute entry point to
+// the gfx950 kernels in kernels.hip. Host-side number theory (Montgomery constants, the root
+// search of hmpc/ints/num/theory/root_of_unity.hpp, inverses) runs here on the CPU with the
+// same limb templates the kernels use.
+#include "kernels.hpp"
+
+#include "../../include/hmpc_hip.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+using namespace hmpc_amd;
+
+namespace
+{
+    thread_local std::string last_error;
+
+    int fail(int code, std::string const& msg)
+    {
+        last_error = msg;
+        return code;
+    }
+
+    int hip_fail(hipError_t e, char const* where)
+    {
+        return fail(e == hipErrorOutOfMemory ? HMPC_ERR_OUT_OF_MEMORY : HMPC_ERR_DEVICE,
+                    std::string(where) + ": " + hipGetErrorString(e));
+    }
+
+#define HMPC_HIP(call)                               \
+    do                                               \
+    {                                                \
+        hipError_t e_ = (call);                      \
+        if (e_ != hipSuccess)                        \
+            return hip_fail(e_, #call);              \
+    } while (0)
+
+    template<typename F>
+    auto host_limbs(int L, F&& f)
+    {
+        switch (L)
+        {
+        case 1: return f(std::integral_constant<int, 1>{});
+        case 2: return f(std::integral_constant<int, 2>{});
+        case 3: return f(std::integral_constant<int, 3>{});
+        case 4: return f(std::integral_constant<int, 4>{});
+        case 5: return f(std::integral_constant<int, 5>{});
+        case 6: return f(std::integral_constant<int, 6>{});
+        case 7: return f(std::integral_constant<int, 7>{});
+        case 8: return f(std::integral_constant<int, 8>{});
+        default: return f(std::integral_constant<int, 9>{});
+        }
+    }
+
+    // ---------------------------------------------------------------- host number theory
+    int bit_width(u32 const* x, int words)
+    {
+        for (int i = words - 1; i >= 0; --i)
+            if (x[i])
+                return 32 * i + 32 - __builtin_clz(x[i]);
+        return 0;
+    }
+
+    // x = 2 x mod q over L limbs (x < q)
+    void double_mod(u32* x, u32 const* q, int L)
+    {
+        u32 d[max_limbs + 1], s[max_limbs + 1];
+        u32 c = 0;
+        for (int i = 0; i < L; ++i)
+            d[i] = addc(x[i], x[i], c, c);
+        d[L] = c;
+        u32 br = 0;
+        for (int i = 0; i < L; ++i)
+            s[i] = subb(d[i], q[i], br, br);
+        s[L] = subb(d[L], 0u, br, br);
+        for (int i = 0; i < L; ++i)
+            x[i] = br ? d[i] : s[i];
+    }
+
+    template<int L>
+    void pow_mod(u32 (&r)[L], u32 const (&base)[L], u32 const* e, int e_words, u32 const (&one)[L], mod_params const& P)
+    {
+        u32 acc[L], b[L];
+        copy<L>(acc, one);
+        copy<L>(b, base);
+        int bits = bit_width(e, e_words);
+        for (int i = 0; i < bits; ++i)
+        {
+            if ((e[i / 32] >> (i % 32)) & 1u)
+                mod_mul<L>(acc, acc, b, P);
+            mod_mul<L>(b, b, b, P);
+        }
+        copy<L>(r, acc);
+    }
+
+    template<int L>
+    bool equal(u32 const (&a)[L], u32 const (&b)[L])
+    {
+        for (int i = 0; i < L; ++i)
+            if (a[i] != b[i])
+                return false;
+        return true;
+    }
+}
+
+struct hmpc_context_s
+{
+    int device = 0;
+    u32 key[8] = {};
+    u32 nonce[3] = {};
+    hipDeviceProp_t props{};
+    std::mutex ws_mutex;
+    std::map<hipStream_t, std::pair<void*, std::size_t>> workspaces;
+
+    // scratch of at least `bytes` for work ordered on `stream` (grown on demand)
+    int workspace(hipStream_t stream, std::size_t bytes, void** out)
+    {
+        std::lock_guard<std::mutex> lock(ws_mutex);
+        auto& w = workspaces[stream];
+        if (w.second < bytes)
+        {
+            if (w.first)
+            {
+                HMPC_HIP(hipStreamSynchronize(stream));
+                HMPC_HIP(hipFree(w.first));
+                w.first = nullptr;
+                w.second = 0;
+            }
+            HMPC_HIP(hipMalloc(&w.first, bytes));
+            w.second = bytes;
+        }
+        *out = w.first;
+        return HMPC_OK;
+    }
+
+    ~hmpc_context_s()
+    {
+        for (auto& [s, w] : workspaces)
+            if (w.first)
+                (void)hipFree(w.first);
+    }
+};
+
+struct table_pair
+{
+    u32* fwd = nullptr;
+    u32* inv = nullptr;
+    u32 root[max_limbs] = {};
+    u32 inverse_root[max_limbs] = {};
+};
+
+struct hmpc_modulus_s
+{
+    hmpc_context ctx = nullptr;
+    int limbs = 0;
+    int bit_size = 0;
+    mod_params P{};
+    u32 one[max_limbs] = {};
+    u32 r2[max_limbs] = {};
+    std::map<int, table_pair> tables;
+
+    ~hmpc_modulus_s()
+    {
+        for (auto& [k, t] : tables)
+        {
+            if (t.fwd)
+                (void)hipFree(t.fwd);
+            if (t.inv)
+                (void)hipFree(t.inv);
+        }
+    }
+
+    int prepare(int log_n);
+
+    int args(int log_n, transform_args& T)
+    {
+        int rc = prepare(log_n);
+        if (rc)
+            return rc;
+        auto& t = tables[log_n];
+        T.P = P;
+        std::copy(one, one + max_limbs, T.one);
+        std::copy(r2, r2 + max_limbs, T.r2);
+        T.log_n = log_n;
+        T.fwd = t.fwd;
+        T.inv = t.inv;
+        return HMPC_OK;
+    }
+};
+
+namespace
+{
+    // hmpc/ints/num/theory/root_of_unity.hpp:19-54 with the all-zero ChaCha20 stream
+    // (compiletime_number_generator, hmpc/random/number_generator.hpp:106-110).
+    template<int L>
+    bool root_search(hmpc_modulus_s const& m, int log_degree, u32 (&out)[L])
+    {
+        mod_params const& P = m.P;
+        u32 qm1[L];
+        u32 br = 0;
+        for (int i = 0; i < L; ++i)
+            qm1[i] = subb(P.q[i], i == 0 ? 1u : 0u, br, br);
+        for (int b = 0; b < log_degree; ++b)
+            if ((qm1[b / 32] >> (b % 32)) & 1u)
+                return false;
+        u32 exponent[L] = {};
+        for (int b = 0; b + log_degree < 32 * L; ++b)
+        {
+            int src = b + log_degree;
+            if ((qm1[src / 32] >> (src % 32)) & 1u)
+                exponent[b / 32] |= 1u << (b % 32);
+        }
+        u32 one[L];
+        for (int i = 0; i < L; ++i)
+            one[i] = m.one[i];
+        u32 half_degree[1] = {1u << (log_degree - 1)};
+        u32 degree[2] = {log_degree < 32 ? (1u << log_degree) : 0u, log_degree < 32 ? 0u : 1u};
+        chacha_key zero{};
+        u32 block[16];
+        u32 counter = 0;
+        int index = 16;
+        auto next = [&]() {
+            if (index == 16)
+            {
+                chacha20_block(block, zero, counter++, 0u, 0u, 0u);
+                index = 0;
+            }
+            return block[index++];
+        };
+        for (int attempt = 0; attempt < 4096; ++attempt)
+        {
+            u32 x[L];
+            for (int i = 0; i < L; ++i)
+                x[i] = next();
+            if (m.bit_size % 32)
+                x[L - 1] &= (1u << (m.bit_size % 32)) - 1u;
+            // x > q - 1  ->  x -= q
+            bool greater = false;
+            for (int i = L - 1; i >= 0; --i)
+                if (x[i] != qm1[i])
+                {
+                    greater = x[i] > qm1[i];
+                    break;
+                }
+            if (greater)
+            {
+                u32 b2 = 0;
+                for (int i = 0; i < L; ++i)
+                    x[i] = subb(x[i], P.q[i], b2, b2);
+            }
+            bool zero_x = true;
+            for (int i = 0; i < L; ++i)
+                zero_x &= x[i] == 0;
+            if (zero_x)
+                continue;
+            u32 g[L], t1[L], t2[L];
+            pow_mod<L>(g, x, exponent, L, one, P);
+            pow_mod<L>(t1, g, half_degree, 1, one, P);
+            pow_mod<L>(t2, g, degree, 2, one, P);
+            if (!equal<L>(t1, one) && equal<L>(t2, one))
+            {
+                copy<L>(out, g);
+                return true;
+            }
+        }
+        return false;
+    }
+
+    template<int L>
+    void invert_mont(u32 (&r)[L], u32 const (&x)[L], hmpc_modulus_s const& m)
+    {
+        u32 e[L];
+        u32 br = 0;
+        for (int i = 0; i < L; ++i)
+            e[i] = subb(m.P.q[i], i == 0 ? 2u : 0u, br, br);
+        u32 one[L];
+        for (int i = 0; i < L; ++i)
+            one[i] = m.one[i];
+        pow_mod<L>(r, x, e, L, one, m.P);
+    }
+
+    struct device_guard
+    {
+        explicit device_guard(int device) { (void)hipSetDevice(device); }
+    };
+
+    hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+    bool supported_log_n(int log_n) { return log_n >= 2 && log_n <= 20; }
+}
+
+int hmpc_modulus_s::prepare(int log_n)
+{
+    if (!supported_log_n(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
+    if (tables.count(log_n))
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    table_pair t;
+    bool ok = host_limbs(limbs, [&](auto LC) {
+        constexpr int LL = decltype(LC)::value;
+        u32 phi[LL], iphi[LL], nval[LL] = {}, nm[LL], ninv[LL], r2v[LL];
+        if (!root_search<LL>(*this, log_n + 1, phi))
+            return false;
+        invert_mont<LL>(iphi, phi, *this);
+        nval[0] = 1u << log_n;
+        for (int i = 0; i < LL; ++i)
+            r2v[i] = r2[i];
+        mod_mul<LL>(nm, nval, r2v, P);
+        invert_mont<LL>(ninv, nm, *this);
+        std::copy(phi, phi + LL, t.root);
+        std::copy(iphi, iphi + LL, t.inverse_root);
+        std::size_t bytes = (std::size_t{4} * LL) << log_n;
+        if (hipMalloc(&t.fwd, bytes) != hipSuccess || hipMalloc(&t.inv, bytes) != hipSuccess)
+            return false;
+        if (launch_make_twiddles(LL, t.fwd, log_n, phi, ninv, one, false, P, nullptr) != hipSuccess)
+            return false;
+        if (launch_make_twiddles(LL, t.inv, log_n, iphi, ninv, one, true, P, nullptr) != hipSuccess)
+            return false;
+        return hipDeviceSynchronize() == hipSuccess;
+    });
+    if (!ok)
+    {
+        if (t.fwd)
+            (void)hipFree(t.fwd);
+        if (t.inv)
+            (void)hipFree(t.inv);
+        return fail(HMPC_ERR_INVALID_MODULUS, "no primitive 2N-th root of unity for this modulus (2N must divide q - 1)");
+    }
+    tables[log_n] = t;
+    return HMPC_OK;
+}
+
+extern "C" {
+
+const char* hmpc_last_error(void) { return last_error.c_str(); }
+
+// ------------------------------------------------------------------ context
+int hmpc_context_create(int device, const uint32_t* key, hmpc_context* out)
+{
+    if (!out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "out is null");
+    int count = 0;
+    HMPC_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "no such device");
+    auto ctx = std::make_unique<hmpc_context_s>();
+    ctx->device = device;
+    HMPC_HIP(hipSetDevice(device));
+    HMPC_HIP(hipGetDeviceProperties(&ctx->props, device));
+    if (key)
+        std::copy(key, key + 8, ctx->key);
+    else
+    {
+        std::random_device rd;
+        for (auto& k : ctx->key)
+            k = rd();
+    }
+    *out = ctx.release();
+    return HMPC_OK;
+}
+
+int hmpc_context_destroy(hmpc_context ctx)
+{
+    if (ctx)
+    {
+        device_guard g(ctx->device);
+        (void)hipDeviceSynchronize();
+        delete ctx;
+    }
+    return HMPC_OK;
+}
+
+int hmpc_context_get_rng(hmpc_context ctx, uint32_t* key8, uint32_t* nonce3)
+{
+    if (!ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null context");
+    if (key8)
+        std::copy(ctx->key, ctx->key + 8, key8);
+    if (nonce3)
+        std::copy(ctx->nonce, ctx->nonce + 3, nonce3);
+    return HMPC_OK;
+}
+
+int hmpc_context_set_rng(hmpc_context ctx, const uint32_t* key8, const uint32_t* nonce3)
+{
+    if (!ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null context");
+    if (key8)
+        std::copy(key8, key8 + 8, ctx->key);
+    if (nonce3)
+        std::copy(nonce3, nonce3 + 3, ctx->nonce);
+    return HMPC_OK;
+}
+
+int hmpc_context_wait(hmpc_context ctx)
+{
+    if (!ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null context");
+    device_guard g(ctx->device);
+    HMPC_HIP(hipDeviceSynchronize());
+    return HMPC_OK;
+}
+
+int hmpc_context_info(hmpc_context ctx, hmpc_device_info* out)
+{
+    if (!ctx || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->name, sizeof(out->name), "%s", ctx->props.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", ctx->props.gcnArchName);
+    out->compute_units = ctx->props.multiProcessorCount;
+    out->wavefront_size = ctx->props.warpSize;
+    out->clock_khz = ctx->props.clockRate;
+    out->memory_clock_khz = ctx->props.memoryClockRate;
+    out->memory_bus_width = ctx->props.memoryBusWidth;
+    out->global_memory = ctx->props.totalGlobalMem;
+    out->lds_per_workgroup = ctx->props.sharedMemPerBlock;
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ memory
+int hmpc_malloc(hmpc_context ctx, size_t bytes, void** out)
+{
+    if (!ctx || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    device_guard g(ctx->device);
+    HMPC_HIP(hipMalloc(out, std::max<size_t>(bytes, 4)));
+    return HMPC_OK;
+}
+
+int hmpc_free(hmpc_context ctx, void* ptr)
+{
+    if (!ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null context");
+    device_guard g(ctx->device);
+    if (ptr)
+        HMPC_HIP(hipFree(ptr));
+    return HMPC_OK;
+}
+
+int hmpc_memcpy_h2d(hmpc_context ctx, void* dst, const void* src, size_t bytes)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return HMPC_OK;
+}
+
+int hmpc_memcpy_d2h(hmpc_context ctx, void* dst, const void* src, size_t bytes)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return HMPC_OK;
+}
+
+int hmpc_memcpy_d2d(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_stream_create(hmpc_context ctx, void** stream)
+{
+    device_guard g(ctx->device);
+    hipStream_t s;
+    HMPC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return HMPC_OK;
+}
+
+int hmpc_stream_destroy(hmpc_context ctx, void* stream)
+{
+    device_guard g(ctx->device);
+    if (stream)
+    {
+        HMPC_HIP(hipStreamSynchronize(as_stream(stream)));
+        {
+            std::lock_guard<std::mutex> lock(ctx->ws_mutex);
+            auto it = ctx->workspaces.find(as_stream(stream));
+            if (it != ctx->workspaces.end())
+            {
+                if (it->second.first)
+                    HMPC_HIP(hipFree(it->second.first));
+                ctx->workspaces.erase(it);
+            }
+        }
+        HMPC_HIP(hipStreamDestroy(as_stream(stream)));
+    }
+    return HMPC_OK;
+}
+
+int hmpc_stream_synchronize(hmpc_context ctx, void* stream)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipStreamSynchronize(as_stream(stream)));
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ moduli
+int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_modulus* out)
+{
+    if (!ctx || !q || !out || q_words <= 0 || q_words > 64)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad modulus arguments");
+    int qbits = bit_width(q, q_words);
+    if (qbits < 2 || (q[0] & 1u) == 0)
+        return fail(HMPC_ERR_INVALID_MODULUS, "modulus must be odd and > 1");
+    std::vector<u32> qm1(q, q + q_words);
+    {
+        u32 br = 1;
+        for (auto& w : qm1)
+            w = subb(w, 0u, br, br);
+    }
+    int bit_size = bit_width(qm1.data(), q_words);
+    int L = (bit_size + 31) / 32;
+    if (L > max_limbs || qbits > 32 * L)
+        return fail(HMPC_ERR_INVALID_MODULUS, "modulus wider than 288 bits is not supported");
+    auto m = std::make_unique<hmpc_modulus_s>();
+    m->ctx = ctx;
+    m->limbs = L;
+    m->bit_size = bit_size;
+    m->P.limbs = L;
+    std::copy(q, q + L, m->P.q);
+    u32 inv = 1;
+    for (int i = 0; i < 6; ++i)
+        inv *= 2u - q[0] * inv;
+    m->P.nprime = 0u - inv;
+    u32 x[max_limbs] = {1};
+    for (int b = 0; b < 64 * L; ++b)
+    {
+        double_mod(x, m->P.q, L);
+        if (b == 32 * L - 1)
+            std::copy(x, x + L, m->one);
+    }
+    std::copy(x, x + L, m->r2);
+    *out = m.release();
+    return HMPC_OK;
+}
+
+int hmpc_modulus_destroy(hmpc_modulus mod)
+{
+    if (mod)
+    {
+        device_guard g(mod->ctx->device);
+        (void)hipDeviceSynchronize();
+        delete mod;
+    }
+    return HMPC_OK;
+}
+
+int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* nprime, uint32_t* r_mod_q, uint32_t* r2_mod_q)
+{
+    if (!mod)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null modulus");
+    if (limbs)
+        *limbs = mod->limbs;
+    if (bit_size)
+        *bit_size = mod->bit_size;
+    if (nprime)
+        *nprime = mod->P.nprime;
+    if (r_mod_q)
+        std::copy(mod->one, mod->one + mod->limbs, r_mod_q);
+    if (r2_mod_q)
+        std::copy(mod->r2, mod->r2 + mod->limbs, r2_mod_q);
+    return HMPC_OK;
+}
+
+int hmpc_modulus_prepare(hmpc_modulus mod, int log_n)
+{
+    if (!mod)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null modulus");
+    return mod->prepare(log_n);
+}
+
+int hmpc_modulus_root(hmpc_modulus mod, int log_n, int inverse, uint32_t* out)
+{
+    if (!mod || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = mod->prepare(log_n);
+    if (rc)
+        return rc;
+    auto& t = mod->tables[log_n];
+    std::copy(inverse ? t.inverse_root : t.root, (inverse ? t.inverse_root : t.root) + mod->limbs, out);
+    return HMPC_OK;
+}
+
+int hmpc_modulus_twiddles(hmpc_modulus mod, int log_n, int inverse, const uint32_t** out)
+{
+    if (!mod || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = mod->prepare(log_n);
+    if (rc)
+        return rc;
+    auto& t = mod->tables[log_n];
+    *out = inverse ? t.inv : t.fwd;
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ NTT
+int hmpc_ntt(hmpc_modulus mod, int log_n, size_t n, int inverse, const uint32_t* in, uint32_t* out, void* stream)
+{
+    if (!mod || (n && (!in || !out)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0)
+        return HMPC_OK;
+    if (n > 0x7fffffffu)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_ntt_fused(mod->limbs, T, n, inverse != 0, in, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_ntt_plain(hmpc_modulus mod, int log_n, size_t n, int inverse, const uint32_t* in, uint32_t* out, uint32_t* scratch,
+                   void* stream)
+{
+    if (!mod || (n && (!in || !out)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0)
+        return HMPC_OK;
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    device_guard g(mod->ctx->device);
+    hipStream_t s = as_stream(stream);
+    u32* sc = scratch;
+    if (!sc)
+    {
+        void* w = nullptr;
+        rc = mod->ctx->workspace(s, (std::size_t{4} * mod->limbs * n) << log_n, &w);
+        if (rc)
+            return rc;
+        sc = static_cast<u32*>(w);
+    }
+    HMPC_HIP(launch_ntt_plain(mod->limbs, T, n, inverse != 0, in, out, sc, s));
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ point-wise
+int hmpc_pointwise(hmpc_modulus mod, int op, size_t count, const uint32_t* a, const uint32_t* b, size_t b_count,
+                   uint32_t* out, void* stream)
+{
+    if (!mod || op < 0 || op > 2)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad op");
+    if (count == 0)
+        return HMPC_OK;
+    if (!a || !b || !out || b_count == 0 || b_count > count)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_pointwise(mod->limbs, mod->P, op, count, a, b, b_count, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_pointwise_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t* a, const uint32_t* c_host, uint32_t* out,
+                          void* stream)
+{
+    if (!mod || op < 0 || op > 2 || !c_host)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad op");
+    if (count == 0)
+        return HMPC_OK;
+    if (!a || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_pointwise_scalar(mod->limbs, mod->P, op, count, a, c_host, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_from_u32(hmpc_modulus mod, size_t count, const uint32_t* words, uint32_t* out, void* stream)
+{
+    if (!mod)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null modulus");
+    if (count == 0)
+        return HMPC_OK;
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_from_u32(mod->limbs, mod->P, mod->r2, count, words, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_to_unsigned(hmpc_modulus mod, size_t count, const uint32_t* in, uint32_t* out, void* stream)
+{
+    if (!mod)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null modulus");
+    if (count == 0)
+        return HMPC_OK;
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_to_unsigned(mod->limbs, mod->P, count, in, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ sampling
+namespace
+{
+    chacha_key make_key(const uint32_t* k)
+    {
+        chacha_key key{};
+        std::copy(k, k + 8, key.k);
+        return key;
+    }
+
+    void nonce_add(u32* nonce, u64 value)
+    {
+        u32 c = 0;
+        nonce[0] = addc(nonce[0], static_cast<u32>(value), 0u, c);
+        nonce[1] = addc(nonce[1], static_cast<u32>(value >> 32), c, c);
+        nonce[2] = addc(nonce[2], 0u, c, c);
+    }
+
+    bool valid_bits(int bits) { return bits >= 2 && bits <= 64 && bits % 2 == 0; }
+}
+
+int hmpc_sample_uniform(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream)
+{
+    if (!mod || !key8 || (count && !out))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (count == 0)
+        return HMPC_OK;
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_sample_uniform(mod->limbs, mod->P, mod->bit_size, make_key(key8), offset, count, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_sample_cbd(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, int bits, size_t count, uint32_t* out,
+                    void* stream)
+{
+    if (!mod || !key8 || (count && !out))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (!valid_bits(bits))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    if (count == 0)
+        return HMPC_OK;
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_sample_cbd(mod->limbs, mod->P, mod->r2, make_key(key8), offset, bits, count, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_sample_words(hmpc_context ctx, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream)
+{
+    if (!ctx || !key8 || (count && !out))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (count == 0)
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    HMPC_HIP(launch_sample_words(make_key(key8), offset, count, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_sample_cbd_ntt(hmpc_context ctx, hmpc_modulus mod, int bits, int log_n, size_t n, uint32_t* out, void* stream)
+{
+    if (!ctx || !mod || (n && !out))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (!valid_bits(bits))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    HMPC_HIP(launch_sample_cbd_ntt(mod->limbs, T, make_key(ctx->key), ctx->nonce, bits, n, out, as_stream(stream)));
+    nonce_add(ctx->nonce, static_cast<u64>(n) << log_n);
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ fused batches
+int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out, void* stream)
+{
+    if (!mod || (n && (!a || !b || !out)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    if (n > 0x7fffffffu)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
+    device_guard g(mod->ctx->device);
+    hipStream_t s = as_stream(stream);
+    void* w = nullptr;
+    std::size_t words = polymul_workspace_words(mod->limbs, log_n, n);
+    if (words)
+    {
+        rc = mod->ctx->workspace(s, words * 4, &w);
+        if (rc)
+            return rc;
+    }
+    HMPC_HIP(launch_polymul(mod->limbs, T, n, a, b, out, static_cast<u32*>(w), s));
+    return HMPC_OK;
+}
+
+namespace
+{
+    int bgv_check(hmpc_context ctx, hmpc_modulus mod, size_t n, const uint32_t* pk_a, const uint32_t* pk_b,
+                  const uint32_t* messages, const uint32_t* p_mont, int bu, int bv, int bw, uint32_t* c0, uint32_t* c1)
+    {
+        if (!ctx || !mod || !p_mont || (n && (!pk_a || !pk_b || !messages || !c0 || !c1)))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+        if (!valid_bits(bu) || !valid_bits(bv) || !valid_bits(bw))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+        if (n > 0x7fffffffu)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
+        return HMPC_OK;
+    }
+
+    bgv_args make_bgv_args(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, const uint32_t* p_mont, int bu, int bv,
+                           int bw)
+    {
+        bgv_args B{};
+        B.key = make_key(ctx->key);
+        std::copy(ctx->nonce, ctx->nonce + 3, B.nonce_u);
+        std::copy(ctx->nonce, ctx->nonce + 3, B.nonce_v);
+        nonce_add(B.nonce_v, static_cast<u64>(n) << log_n);
+        std::copy(B.nonce_v, B.nonce_v + 3, B.nonce_w);
+        nonce_add(B.nonce_w, static_cast<u64>(n) << log_n);
+        B.bits_u = bu;
+        B.bits_v = bv;
+        B.bits_w = bw;
+        std::copy(p_mont, p_mont + mod->limbs, B.p_mont);
+        return B;
+    }
+}
+
+int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, const uint32_t* pk_a, const uint32_t* pk_b,
+                     const uint32_t* messages, const uint32_t* p_mont, int bits_u, int bits_v, int bits_w, uint32_t* c0,
+                     uint32_t* c1, void* stream)
+{
+    int rc = bgv_check(ctx, mod, n, pk_a, pk_b, messages, p_mont, bits_u, bits_v, bits_w, c0, c1);
+    if (rc)
+        return rc;
+    transform_args T;
+    rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    void* w = nullptr;
+    rc = ctx->workspace(s, bgv_workspace_words(mod->limbs, log_n, n) * 4, &w);
+    if (rc)
+        return rc;
+    bgv_args B = make_bgv_args(ctx, mod, log_n, n, p_mont, bits_u, bits_v, bits_w);
+    HMPC_HIP(launch_bgv_encrypt(mod->limbs, T, B, n, pk_a, pk_b, messages, c0, c1, static_cast<u32*>(w), s));
+    nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
+    return HMPC_OK;
+}
+
+int hmpc_bgv_encrypt_plain(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, const uint32_t* pk_a,
+                           const uint32_t* pk_b, const uint32_t* messages, const uint32_t* p_mont, int bits_u, int bits_v,
+                           int bits_w, uint32_t* c0, uint32_t* c1, void* stream)
+{
+    int rc = bgv_check(ctx, mod, n, pk_a, pk_b, messages, p_mont, bits_u, bits_v, bits_w, c0, c1);
+    if (rc)
+        return rc;
+    transform_args T;
+    rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    int L = mod->limbs;
+    std::size_t N = std::size_t{1} << log_n;
+    std::size_t poly = static_cast<std::size_t>(L) * N;  // words of one polynomial
+    void* w = nullptr;
+    rc = ctx->workspace(s, 6 * poly * 4, &w);
+    if (rc)
+        return rc;
+    u32* coeff = static_cast<u32*>(w);
+    u32* scratch = coeff + poly;
+    u32* U = scratch + poly;
+    u32* V = U + poly;
+    u32* M = V + poly;
+    u32* Wt = M + poly;
+    bgv_args B = make_bgv_args(ctx, mod, log_n, n, p_mont, bits_u, bits_v, bits_w);
+    std::size_t count = n * N;
+    // one ciphertext at a time, hmpc's per-op launches: sample, per-stage transforms, then
+    // element-wise c0 = (A*u + v*p) + m and c1 = B*u + w*p
+    for (std::size_t i = 0; i < n; ++i)
+    {
+        HMPC_HIP(launch_sample_cbd_capability(L, mod->P, mod->r2, B.key, B.nonce_u, bits_u, log_n, 1, coeff, i, s));
+        HMPC_HIP(launch_ntt_plain(L, T, 1, false, coeff, U, scratch, s));
+        HMPC_HIP(launch_sample_cbd_capability(L, mod->P, mod->r2, B.key, B.nonce_v, bits_v, log_n, 1, coeff, i, s));
+        HMPC_HIP(launch_ntt_plain(L, T, 1, false, coeff, V, scratch, s));
+        HMPC_HIP(launch_from_u32(L, mod->P, mod->r2, N, messages + i * N, coeff, s));
+        HMPC_HIP(launch_ntt_plain(L, T, 1, false, coeff, M, scratch, s));
+        // c0 (written limb-planar into the batch layout through a per-ciphertext temporary)
+        HMPC_HIP(launch_pointwise(L, mod->P, HMPC_MUL, N, pk_a, U, N, coeff, s));
+        HMPC_HIP(launch_pointwise_scalar(L, mod->P, HMPC_MUL, N, V, B.p_mont, V, s));
+        HMPC_HIP(launch_pointwise(L, mod->P, HMPC_ADD, N, coeff, V, N, coeff, s));
+        HMPC_HIP(launch_pointwise(L, mod->P, HMPC_ADD, N, coeff, M, N, coeff, s));
+        for (int l = 0; l < L; ++l)
+            HMPC_HIP(hipMemcpyAsync(c0 + l * count + i * N, coeff + l * N, N * 4, hipMemcpyDeviceToDevice, s));
+        HMPC_HIP(launch_sample_cbd_capability(L, mod->P, mod->r2, B.key, B.nonce_w, bits_w, log_n, 1, coeff, i, s));
+        HMPC_HIP(launch_ntt_plain(L, T, 1, false, coeff, Wt, scratch, s));
+        HMPC_HIP(launch_pointwise(L, mod->P, HMPC_MUL, N, pk_b, U, N, coeff, s));
+        HMPC_HIP(launch_pointwise_scalar(L, mod->P, HMPC_MUL, N, Wt, B.p_mont, Wt, s));
+        HMPC_HIP(launch_pointwise(L, mod->P, HMPC_ADD, N, coeff, Wt, N, coeff, s));
+        for (int l = 0; l < L; ++l)
+            HMPC_HIP(hipMemcpyAsync(c1 + l * count + i * N, coeff + l * N, N * 4, hipMemcpyDeviceToDevice, s));
+    }
+    nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
+    return HMPC_OK;
+}
+
+int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_ntt, const uint32_t* c0, const uint32_t* c1,
+                     uint32_t* messages, void* stream)
+{
+    if (!mod || (n && (!s_ntt || !c0 || !c1 || !messages)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    device_guard g(mod->ctx->device);
+    hipStream_t s = as_stream(stream);
+    std::size_t count = n << log_n;
+    void* w = nullptr;
+    rc = mod->ctx->workspace(s, count * mod->limbs * 4, &w);
+    if (rc)
+        return rc;
+    u32* x = static_cast<u32*>(w);
+    int L = mod->limbs;
+    HMPC_HIP(launch_pointwise(L, mod->P, HMPC_MUL, count, c1, s_ntt, std::size_t{1} << log_n, x, s));
+    HMPC_HIP(launch_pointwise(L, mod->P, HMPC_SUB, count, c0, x, count, x, s));
+    HMPC_HIP(launch_ntt_fused(L, T, n, true, x, x, s));
+    HMPC_HIP(launch_centered_low32(L, mod->P, count, x, messages, s));
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ timing
+int hmpc_event_create(hmpc_context ctx, void** event)
+{
+    device_guard g(ctx->device);
+    hipEvent_t e;
+    HMPC_HIP(hipEventCreate(&e));
+    *event = e;
+    return HMPC_OK;
+}
+
+int hmpc_event_destroy(hmpc_context ctx, void* event)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return HMPC_OK;
+}
+
+int hmpc_event_record(hmpc_context ctx, void* event, void* stream)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_event_elapsed_ms(hmpc_context ctx, void* start, void* stop, float* ms)
+{
+    device_guard g(ctx->device);
+    HMPC_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+    HMPC_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return HMPC_OK;
+}
+
+}  // extern "C"

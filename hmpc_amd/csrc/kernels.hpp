@@ -1,0 +1,82 @@
+// Internal launcher interface between the C ABI (api.hip) and the HIP kernels (kernels.hip).
+#pragma once
+
+#include "modarith.hpp"
+#include "chacha.hpp"
+
+#include <cstddef>
+
+namespace hmpc_amd
+{
+    // Everything a kernel needs to know about a modulus at one transform size.
+    struct transform_args
+    {
+        mod_params P;
+        u32 one[max_limbs];    // R mod q
+        u32 r2[max_limbs];     // R^2 mod q
+        int log_n;
+        const u32* fwd;        // [L][N] forward table (hmpc's roots, device)
+        const u32* inv;        // [L][N] inverse table
+    };
+
+    // Decomposition N = N1 * N2 used by the two-pass fused transforms.
+    struct split_plan
+    {
+        int log_n1;  // column transforms of N1 points (first stages of the forward transform)
+        int log_n2;  // row transforms of N2 points
+        int cols;    // columns per workgroup in the column pass
+        int rows;    // rows per workgroup in the row pass
+        bool single; // whole polynomial in one workgroup's LDS
+    };
+
+    split_plan plan_for(int limbs, int log_n, int tiles_in_lds);
+
+    hipError_t launch_make_twiddles(int L, u32* table, int log_n, u32 const* root, u32 const* norm, u32 const* one,
+                                    bool inverse, mod_params const& P, hipStream_t s);
+
+    hipError_t launch_ntt_plain(int L, transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                                u32* scratch, hipStream_t s);
+
+    hipError_t launch_ntt_fused(int L, transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                                hipStream_t s);
+
+    hipError_t launch_polymul(int L, transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                              u32* workspace, hipStream_t s);
+    std::size_t polymul_workspace_words(int L, int log_n, std::size_t n);
+
+    hipError_t launch_pointwise(int L, mod_params const& P, int op, std::size_t count, u32 const* a, u32 const* b,
+                                std::size_t b_count, u32* out, hipStream_t s);
+    hipError_t launch_pointwise_scalar(int L, mod_params const& P, int op, std::size_t count, u32 const* a,
+                                       u32 const* c, u32* out, hipStream_t s);
+    hipError_t launch_from_u32(int L, mod_params const& P, u32 const* r2, std::size_t count, u32 const* words, u32* out,
+                               hipStream_t s);
+    hipError_t launch_to_unsigned(int L, mod_params const& P, std::size_t count, u32 const* in, u32* out, hipStream_t s);
+    hipError_t launch_centered_low32(int L, mod_params const& P, std::size_t count, u32 const* in, u32* out,
+                                     hipStream_t s);
+
+    hipError_t launch_sample_uniform(int L, mod_params const& P, int bit_size, chacha_key const& key, u64 offset,
+                                     std::size_t count, u32* out, hipStream_t s);
+    hipError_t launch_sample_cbd(int L, mod_params const& P, u32 const* r2, chacha_key const& key, u64 offset, int bits,
+                                 std::size_t count, u32* out, hipStream_t s);
+    hipError_t launch_sample_words(chacha_key const& key, u64 offset, std::size_t count, u32* out, hipStream_t s);
+
+    // Capability-path CBD fused with the forward transform's first stages.
+    hipError_t launch_sample_cbd_ntt(int L, transform_args const& T, chacha_key const& key, u32 const* nonce, int bits,
+                                     std::size_t n, u32* out, hipStream_t s);
+
+    struct bgv_args
+    {
+        chacha_key key;
+        u32 nonce_u[3], nonce_v[3], nonce_w[3];
+        int bits_u, bits_v, bits_w;
+        u32 p_mont[max_limbs];
+    };
+    std::size_t bgv_workspace_words(int L, int log_n, std::size_t n);
+    hipError_t launch_bgv_encrypt(int L, transform_args const& T, bgv_args const& B, std::size_t n, u32 const* pk_a,
+                                  u32 const* pk_b, u32 const* messages, u32* c0, u32* c1, u32* workspace,
+                                  hipStream_t s);
+    // per-ciphertext plain path pieces
+    hipError_t launch_sample_cbd_capability(int L, mod_params const& P, u32 const* r2, chacha_key const& key,
+                                            u32 const* nonce, int bits, int log_n, std::size_t n, u32* out,
+                                            std::size_t first_poly, hipStream_t s);
+}

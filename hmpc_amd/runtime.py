@@ -1,0 +1,405 @@
+"""ctypes bindings to libhmpc_hip.so (the C ABI in include/hmpc_hip.h).
+
+This is the Python host side used by the tests and the benchmark. It is plumbing only:
+every computation runs in the HIP library, and loading fails loudly if the library is
+missing (there is no CPU fallback).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhmpc_hip.so")
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+
+# exported symbol -> (restype, argtypes); every symbol declared in include/hmpc_hip.h
+SIGNATURES = {
+    "hmpc_last_error": (ctypes.c_char_p, []),
+    "hmpc_context_create": (_i, [_i, _u32p, ctypes.POINTER(_vp)]),
+    "hmpc_context_destroy": (_i, [_vp]),
+    "hmpc_context_get_rng": (_i, [_vp, _u32p, _u32p]),
+    "hmpc_context_set_rng": (_i, [_vp, _u32p, _u32p]),
+    "hmpc_context_wait": (_i, [_vp]),
+    "hmpc_context_info": (_i, [_vp, _vp]),
+    "hmpc_malloc": (_i, [_vp, _sz, ctypes.POINTER(_vp)]),
+    "hmpc_free": (_i, [_vp, _vp]),
+    "hmpc_memcpy_h2d": (_i, [_vp, _vp, _vp, _sz]),
+    "hmpc_memcpy_d2h": (_i, [_vp, _vp, _vp, _sz]),
+    "hmpc_memcpy_d2d": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "hmpc_stream_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "hmpc_stream_destroy": (_i, [_vp, _vp]),
+    "hmpc_stream_synchronize": (_i, [_vp, _vp]),
+    "hmpc_modulus_create": (_i, [_vp, _u32p, _i, ctypes.POINTER(_vp)]),
+    "hmpc_modulus_destroy": (_i, [_vp]),
+    "hmpc_modulus_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _u32p, _u32p, _u32p]),
+    "hmpc_modulus_prepare": (_i, [_vp, _i]),
+    "hmpc_modulus_root": (_i, [_vp, _i, _i, _u32p]),
+    "hmpc_modulus_twiddles": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
+    "hmpc_ntt": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp]),
+    "hmpc_ntt_plain": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp, _vp]),
+    "hmpc_pointwise": (_i, [_vp, _i, _sz, _vp, _vp, _sz, _vp, _vp]),
+    "hmpc_pointwise_scalar": (_i, [_vp, _i, _sz, _vp, _u32p, _vp, _vp]),
+    "hmpc_from_u32": (_i, [_vp, _sz, _vp, _vp, _vp]),
+    "hmpc_to_unsigned": (_i, [_vp, _sz, _vp, _vp, _vp]),
+    "hmpc_sample_uniform": (_i, [_vp, _u32p, _u64, _sz, _vp, _vp]),
+    "hmpc_sample_cbd": (_i, [_vp, _u32p, _u64, _i, _sz, _vp, _vp]),
+    "hmpc_sample_words": (_i, [_vp, _u32p, _u64, _sz, _vp, _vp]),
+    "hmpc_sample_cbd_ntt": (_i, [_vp, _vp, _i, _i, _sz, _vp, _vp]),
+    "hmpc_polymul": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp]),
+    "hmpc_bgv_encrypt": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
+    "hmpc_bgv_encrypt_plain": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
+    "hmpc_bgv_decrypt": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "hmpc_event_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "hmpc_event_destroy": (_i, [_vp, _vp]),
+    "hmpc_event_record": (_i, [_vp, _vp, _vp]),
+    "hmpc_event_elapsed_ms": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+class HmpcError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"hmpc error {code}: {message}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python -m hmpc_amd.build`")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise HmpcError(rc, load().hmpc_last_error().decode())
+
+
+def _arr(values, n=None) -> np.ndarray:
+    a = np.zeros(n if n is not None else len(values), dtype=np.uint32)
+    a[: len(values)] = values
+    return a
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+def int_to_words(value: int, words: int) -> np.ndarray:
+    return np.array([(int(value) >> (32 * i)) & 0xFFFFFFFF for i in range(words)], dtype=np.uint32)
+
+
+def words_to_int(words) -> int:
+    return sum(int(w) << (32 * i) for i, w in enumerate(words))
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("compute_units", ctypes.c_int),
+        ("wavefront_size", ctypes.c_int),
+        ("clock_khz", ctypes.c_int),
+        ("memory_clock_khz", ctypes.c_int),
+        ("memory_bus_width", ctypes.c_int),
+        ("global_memory", ctypes.c_size_t),
+        ("lds_per_workgroup", ctypes.c_size_t),
+    ]
+
+
+class Buffer:
+    """Raw device allocation owned by a context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(load().hmpc_malloc(ctx.handle, max(self.nbytes, 4), ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, ctx: "Context", a: np.ndarray) -> "Buffer":
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        if a.nbytes:
+            check(load().hmpc_memcpy_h2d(ctx.handle, b.ptr, a.ctypes.data, a.nbytes))
+        return b
+
+    def to_numpy(self, shape, dtype=np.uint32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        if out.nbytes:
+            check(load().hmpc_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and _lib is not None and self.ctx.handle:
+            _lib.hmpc_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """hmpc::comp::queue analogue: one device, one RNG capability (key, nonce)."""
+
+    def __init__(self, device: int = 0, key=None):
+        load()
+        h = _vp()
+        k = _arr(key, 8) if key is not None else None
+        check(_lib.hmpc_context_create(device, _p(k) if k is not None else None, ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if self.handle and _lib is not None:
+            _lib.hmpc_context_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def wait(self):
+        check(_lib.hmpc_context_wait(self.handle))
+
+    def info(self) -> dict:
+        d = DeviceInfo()
+        check(_lib.hmpc_context_info(self.handle, ctypes.byref(d)))
+        return {f: (getattr(d, f).decode() if isinstance(getattr(d, f), bytes) else getattr(d, f)) for f, _ in d._fields_}
+
+    @property
+    def rng(self):
+        k = np.zeros(8, np.uint32)
+        n = np.zeros(3, np.uint32)
+        check(_lib.hmpc_context_get_rng(self.handle, _p(k), _p(n)))
+        return k, n
+
+    def set_rng(self, key=None, nonce=None):
+        k = _arr(key, 8) if key is not None else None
+        n = _arr(nonce, 3) if nonce is not None else None
+        check(_lib.hmpc_context_set_rng(self.handle, _p(k) if k is not None else None, _p(n) if n is not None else None))
+
+    def buffer(self, nbytes: int) -> Buffer:
+        return Buffer(self, nbytes)
+
+    def upload(self, a: np.ndarray) -> Buffer:
+        return Buffer.from_numpy(self, a)
+
+    def stream(self) -> "Stream":
+        return Stream(self)
+
+    def event(self) -> "Event":
+        return Event(self)
+
+    def sample_words(self, key, offset: int, count: int, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or Buffer(self, 4 * count)
+        k = _arr(key, 8)
+        check(_lib.hmpc_sample_words(self.handle, _p(k), offset, count, out.ptr, _sp(stream)))
+        return out
+
+
+def _sp(stream):
+    return None if stream is None else stream.handle
+
+
+class Stream:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _vp()
+        check(_lib.hmpc_stream_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        check(_lib.hmpc_stream_synchronize(self.ctx.handle, self.handle))
+
+    def close(self):
+        if self.handle and _lib is not None and self.ctx.handle:
+            _lib.hmpc_stream_destroy(self.ctx.handle, self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _vp()
+        check(_lib.hmpc_event_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        check(_lib.hmpc_event_record(self.ctx.handle, self.handle, _sp(stream)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = ctypes.c_float()
+        check(_lib.hmpc_event_elapsed_ms(self.ctx.handle, self.handle, end.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle and _lib is not None and self.ctx.handle:
+                _lib.hmpc_event_destroy(self.ctx.handle, self.handle)
+        except Exception:
+            pass
+
+
+class Modulus:
+    """A registered modulus q (hmpc::ints::mod<q> at run time) with resident tables."""
+
+    ADD, SUB, MUL = 0, 1, 2
+
+    def __init__(self, ctx: Context, q: int):
+        self.ctx = ctx
+        self.q = int(q)
+        words = max(1, (self.q.bit_length() + 31) // 32)
+        qa = int_to_words(self.q, words)
+        h = _vp()
+        check(_lib.hmpc_modulus_create(ctx.handle, _p(qa), words, ctypes.byref(h)))
+        self.handle = h.value
+        L, b = _i(), _i()
+        check(_lib.hmpc_modulus_info(self.handle, ctypes.byref(L), ctypes.byref(b), None, None, None))
+        self.limbs, self.bit_size = L.value, b.value
+
+    def close(self):
+        if self.handle and _lib is not None:
+            _lib.hmpc_modulus_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- host helpers
+    def constants(self):
+        npr = np.zeros(1, np.uint32)
+        r1 = np.zeros(self.limbs, np.uint32)
+        r2 = np.zeros(self.limbs, np.uint32)
+        check(_lib.hmpc_modulus_info(self.handle, None, None, _p(npr), _p(r1), _p(r2)))
+        return int(npr[0]), words_to_int(r1), words_to_int(r2)
+
+    def prepare(self, log_n: int):
+        check(_lib.hmpc_modulus_prepare(self.handle, log_n))
+
+    def root(self, log_n: int, inverse: bool = False) -> int:
+        out = np.zeros(self.limbs, np.uint32)
+        check(_lib.hmpc_modulus_root(self.handle, log_n, int(inverse), _p(out)))
+        return words_to_int(out)
+
+    def twiddles(self, log_n: int, inverse: bool = False) -> np.ndarray:
+        p = _vp()
+        check(_lib.hmpc_modulus_twiddles(self.handle, log_n, int(inverse), ctypes.byref(p)))
+        out = np.empty((self.limbs, 1 << log_n), np.uint32)
+        check(_lib.hmpc_memcpy_d2h(self.ctx.handle, out.ctypes.data, p.value, out.nbytes))
+        return out
+
+    def mont(self, value: int) -> np.ndarray:
+        """Montgomery form value*R mod q as L host words."""
+        R = 1 << (32 * self.limbs)
+        return int_to_words(int(value) % self.q * R % self.q, self.limbs)
+
+    # ---------------------------------------------------------------- tensors ([L][count] u32)
+    def upload(self, planes: np.ndarray) -> Buffer:
+        assert planes.shape[0] == self.limbs
+        return Buffer.from_numpy(self.ctx, np.ascontiguousarray(planes, dtype=np.uint32))
+
+    def download(self, buf: Buffer, count: int) -> np.ndarray:
+        return buf.to_numpy((self.limbs, count))
+
+    def alloc(self, count: int) -> Buffer:
+        return Buffer(self.ctx, 4 * self.limbs * count)
+
+    def ntt(self, log_n, n, x: Buffer, out: Buffer | None = None, inverse=False, plain=False, stream=None) -> Buffer:
+        out = out or self.alloc(n << log_n)
+        if plain:
+            check(_lib.hmpc_ntt_plain(self.handle, log_n, n, int(inverse), x.ptr, out.ptr, None, _sp(stream)))
+        else:
+            check(_lib.hmpc_ntt(self.handle, log_n, n, int(inverse), x.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def pointwise(self, op, count, a: Buffer, b: Buffer, b_count=None, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        check(_lib.hmpc_pointwise(self.handle, op, count, a.ptr, b.ptr, b_count or count, out.ptr, _sp(stream)))
+        return out
+
+    def pointwise_scalar(self, op, count, a: Buffer, c_mont: np.ndarray, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        c = np.ascontiguousarray(c_mont, dtype=np.uint32)
+        check(_lib.hmpc_pointwise_scalar(self.handle, op, count, a.ptr, _p(c), out.ptr, _sp(stream)))
+        return out
+
+    def from_u32(self, count, words: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        check(_lib.hmpc_from_u32(self.handle, count, words.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def to_unsigned(self, count, x: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        check(_lib.hmpc_to_unsigned(self.handle, count, x.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def sample_uniform(self, key, offset, count, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        k = _arr(key, 8)
+        check(_lib.hmpc_sample_uniform(self.handle, _p(k), offset, count, out.ptr, _sp(stream)))
+        return out
+
+    def sample_cbd(self, key, offset, bits, count, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        k = _arr(key, 8)
+        check(_lib.hmpc_sample_cbd(self.handle, _p(k), offset, bits, count, out.ptr, _sp(stream)))
+        return out
+
+    def sample_cbd_ntt(self, bits, log_n, n, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(n << log_n)
+        check(_lib.hmpc_sample_cbd_ntt(self.ctx.handle, self.handle, bits, log_n, n, out.ptr, _sp(stream)))
+        return out
+
+    def polymul(self, log_n, n, a: Buffer, b: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(n << log_n)
+        check(_lib.hmpc_polymul(self.handle, log_n, n, a.ptr, b.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def bgv_encrypt(self, log_n, n, pk_a: Buffer, pk_b: Buffer, messages: Buffer, p_mont: np.ndarray,
+                    bits=(2, 40, 40), c0: Buffer | None = None, c1: Buffer | None = None, plain=False, stream=None):
+        c0 = c0 or self.alloc(n << log_n)
+        c1 = c1 or self.alloc(n << log_n)
+        pm = np.ascontiguousarray(p_mont, dtype=np.uint32)
+        fn = _lib.hmpc_bgv_encrypt_plain if plain else _lib.hmpc_bgv_encrypt
+        check(fn(self.ctx.handle, self.handle, log_n, n, pk_a.ptr, pk_b.ptr, messages.ptr, _p(pm),
+                 bits[0], bits[1], bits[2], c0.ptr, c1.ptr, _sp(stream)))
+        return c0, c1
+
+    def bgv_decrypt(self, log_n, n, s_ntt: Buffer, c0: Buffer, c1: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or Buffer(self.ctx, 4 * (n << log_n))
+        check(_lib.hmpc_bgv_decrypt(self.handle, log_n, n, s_ntt.ptr, c0.ptr, c1.ptr, out.ptr, _sp(stream)))
+        return out

@@ -1,0 +1,182 @@
+/*
+ * hmpc_hip.h — C ABI of the MI355X (gfx950) library for hmpc's data-parallel core:
+ * the negacyclic NTT over multi-limb Montgomery integers, the BGV (linear HE) encryption
+ * built on it, and the ChaCha20 sampling that feeds both.
+ *
+ * Conventions shared by every entry point
+ *  - Plain C types only: opaque handles, device pointers (void* / uint32_t*), sizes.
+ *  - Every call returns an hmpc_status; 0 is success. hmpc_last_error() gives a message
+ *    for the calling thread. (Same convention as hmpc-rs's FFI, hmpc-rs/src/ffi/errors.rs:191-216.)
+ *  - Polynomial tensors are limb-planar: limb l of element e lives at buf[l * count + e],
+ *    count = n * N, element index = b * N + k (hmpc/comp/accessor.hpp:191-197,
+ *    hmpc/shape.hpp:204-239). Values are canonical Montgomery residues a*R mod q,
+ *    R = 2^(32 L), L = ceil(bit_width(q - 1) / 32) (hmpc/ints/mod.hpp:90-93).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream). Calls are
+ *    asynchronous on that stream unless documented otherwise.
+ *  - A context is not thread-safe (like hmpc::comp::queue, hmpc/comp/queue.hpp:165-253):
+ *    use one host thread per context.
+ */
+#ifndef HMPC_HIP_H
+#define HMPC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hmpc_status {
+    HMPC_OK = 0,
+    HMPC_ERR_INVALID_ARGUMENT = 1, /* bad size, null pointer, unsupported N or L */
+    HMPC_ERR_INVALID_MODULUS = 2,  /* even, too large, or no 2N-th root of unity */
+    HMPC_ERR_OUT_OF_MEMORY = 3,
+    HMPC_ERR_DEVICE = 4,           /* HIP runtime error */
+    HMPC_ERR_BUDGET = 5,           /* admission refused by the memory budget */
+    HMPC_ERR_NOT_READY = 6,        /* request still running */
+    HMPC_ERR_CANCELLED = 7
+} hmpc_status;
+
+typedef struct hmpc_context_s* hmpc_context;
+typedef struct hmpc_modulus_s* hmpc_modulus;
+
+/* ------------------------------------------------------------------ context
+ * Replaces hmpc::comp::queue's constructors (hmpc/comp/queue.hpp:229-239): `key` is the
+ * 8-word ChaCha20 key of the queue's random-number capability; the 3-word nonce starts at
+ * zero. key == NULL selects a key from the OS entropy source (hmpc/detail/random.hpp:10-29). */
+int hmpc_context_create(int device, const uint32_t* key, hmpc_context* out);
+int hmpc_context_destroy(hmpc_context ctx);
+/* Queue RNG state (hmpc/comp/queue.hpp:175-180): key[8], nonce[3]. */
+int hmpc_context_get_rng(hmpc_context ctx, uint32_t* key8, uint32_t* nonce3);
+int hmpc_context_set_rng(hmpc_context ctx, const uint32_t* key8, const uint32_t* nonce3);
+/* hmpc::comp::queue::wait (hmpc/comp/queue.hpp:531-534): synchronise the device. */
+int hmpc_context_wait(hmpc_context ctx);
+
+typedef struct hmpc_device_info {
+    char name[64];
+    char arch[32];
+    int compute_units;
+    int wavefront_size;
+    int clock_khz;
+    int memory_clock_khz;
+    int memory_bus_width;
+    size_t global_memory;
+    size_t lds_per_workgroup;
+} hmpc_device_info;
+/* hmpc::comp::queue::info (hmpc/comp/queue.hpp:536-597) */
+int hmpc_context_info(hmpc_context ctx, hmpc_device_info* out);
+
+const char* hmpc_last_error(void);
+
+/* ------------------------------------------------------------------ device memory
+ * Stand-ins for hmpc::comp::make_tensor / host_accessor (hmpc/comp/tensor.hpp:86-90,
+ * hmpc/comp/accessor.hpp:282-288): raw device buffers plus blocking copies. */
+int hmpc_malloc(hmpc_context ctx, size_t bytes, void** out);
+int hmpc_free(hmpc_context ctx, void* ptr);
+int hmpc_memcpy_h2d(hmpc_context ctx, void* dst, const void* src, size_t bytes);
+int hmpc_memcpy_d2h(hmpc_context ctx, void* dst, const void* src, size_t bytes);
+int hmpc_memcpy_d2d(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream);
+int hmpc_stream_create(hmpc_context ctx, void** stream);
+int hmpc_stream_destroy(hmpc_context ctx, void* stream);
+int hmpc_stream_synchronize(hmpc_context ctx, void* stream);
+
+/* ------------------------------------------------------------------ moduli
+ * A registered modulus carries what hmpc computes at compile time for mod<q>
+ * (hmpc/ints/mod.hpp:98-127): N', R mod q, R^2 mod q; and, per prepared transform size,
+ * the resident twiddle tables of basic_number_theoretic_transform::get_roots
+ * (hmpc/expr/number_theoretic_transform.hpp:35-88), cached like the queue's extra tensors
+ * (hmpc/comp/queue.hpp:364-381). q must be odd with bit_width(q - 1) <= 288. */
+int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_modulus* out);
+int hmpc_modulus_destroy(hmpc_modulus mod);
+/* limbs L, bit size of q - 1, and constants (each array L words, may be NULL) */
+int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* nprime, uint32_t* r_mod_q, uint32_t* r2_mod_q);
+/* Build (once) and keep resident the forward/inverse tables for N = 2^log_n (2 <= log_n <= 20),
+ * 2N must divide q - 1. Blocking. */
+int hmpc_modulus_prepare(hmpc_modulus mod, int log_n);
+/* Primitive 2N-th root phi (or phi^-1) in Montgomery form, L words on the host
+ * (hmpc/ints/num/theory/root_of_unity.hpp:19-54). */
+int hmpc_modulus_root(hmpc_modulus mod, int log_n, int inverse, uint32_t* out);
+/* Device pointer of the resident table [L][N] (read only). */
+int hmpc_modulus_twiddles(hmpc_modulus mod, int log_n, int inverse, const uint32_t** out);
+
+/* ------------------------------------------------------------------ NTT
+ * Forward: number_theoretic_transform_expression (hmpc/expr/number_theoretic_transform.hpp:
+ * 239-394): natural-order coefficients -> evaluations in hmpc's bit-reversed order,
+ * A[j] = sum_i a_i phi^((2 brv(j) + 1) i). Inverse: the exact inverse (:396-550).
+ * n polynomials of N = 2^log_n coefficients; `in` may equal `out`.
+ * The fused path keeps stages in LDS (one pass for small N, two passes otherwise). */
+int hmpc_ntt(hmpc_modulus mod, int log_n, size_t n, int inverse, const uint32_t* in, uint32_t* out, void* stream);
+/* hmpc's own launch structure: stage 0 into a transposed [N][n] scratch, one launch per
+ * middle stage, last stage back to [n][N] (K2-K7 of the reference). `scratch` holds
+ * L * n * N words (NULL: allocated and freed by the call). */
+int hmpc_ntt_plain(hmpc_modulus mod, int log_n, size_t n, int inverse, const uint32_t* in, uint32_t* out,
+                   uint32_t* scratch, void* stream);
+
+/* ------------------------------------------------------------------ point-wise ops
+ * add/subtract/multiply_expression executed element-wise (hmpc/expr/binary_expression.hpp:
+ * 8-163 via hmpc/comp/queue.hpp:387-412). b is broadcast with period b_count (b index =
+ * e % b_count; b_count == count for no broadcast, the unsqueeze of hmpc/expr/unsqueeze.hpp). */
+typedef enum hmpc_pointwise_op { HMPC_ADD = 0, HMPC_SUB = 1, HMPC_MUL = 2 } hmpc_pointwise_op;
+int hmpc_pointwise(hmpc_modulus mod, int op, size_t count, const uint32_t* a, const uint32_t* b, size_t b_count,
+                   uint32_t* out, void* stream);
+/* out = a * c for a Montgomery constant c (L host words): value/constant leaves
+ * (hmpc/expr/value.hpp, hmpc/expr/constant.hpp). */
+int hmpc_pointwise_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t* a, const uint32_t* c_host,
+                          uint32_t* out, void* stream);
+/* unsigned 32-bit words -> mod q (hmpc/ints/mod.hpp:165-173, the lift used by examples/bgv.cpp:66-69) */
+int hmpc_from_u32(hmpc_modulus mod, size_t count, const uint32_t* words, uint32_t* out, void* stream);
+/* Montgomery -> canonical integers (hmpc/ints/mod.hpp:356-361), limb-planar in and out */
+int hmpc_to_unsigned(hmpc_modulus mod, size_t count, const uint32_t* in, uint32_t* out, void* stream);
+
+/* ------------------------------------------------------------------ sampling
+ * Explicit-key streams (hmpc/expr/random/number_generator.hpp:31-74): element e of a tensor
+ * with `count` elements uses ChaCha20(key, nonce = e + count * offset (96-bit), counter 0).
+ *  uniform: uniform<mod q> with 80-bit drowning (hmpc/random/uniform.hpp:34-66)
+ *  cbd:     centred binomial with `bits` = 4 * variance random bits (hmpc/random/binomial.hpp)
+ *  words:   the first 32-bit word of each element's stream (BGV messages in [0, 2^32)) */
+int hmpc_sample_uniform(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream);
+int hmpc_sample_cbd(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, int bits, size_t count, uint32_t* out, void* stream);
+int hmpc_sample_words(hmpc_context ctx, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream);
+/* Capability-path centred binomial consumed by a forward NTT (hmpc/comp/queue.hpp:185-253,
+ * hmpc/expr/number_theoretic_transform.hpp:320-333): uses the context's key and current
+ * nonce, then advances the nonce by n * N. Output is NTT(sample) (evaluation domain). */
+int hmpc_sample_cbd_ntt(hmpc_context ctx, hmpc_modulus mod, int bits, int log_n, size_t n, uint32_t* out, void* stream);
+
+/* ------------------------------------------------------------------ fused batches
+ * Batched negacyclic products out_i = INTT(NTT(a_i) o NTT(b_i)), i < n, in one fused pass
+ * sequence (LDS-resident stages, no per-stage launches). */
+int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out, void* stream);
+
+/* BGV public-key encryption of n messages (hmpc/expr/crypto/lhe/enc.hpp:19-69 with the
+ * default randomness of hmpc/expr/crypto/lhe/randomness.hpp:80-110):
+ *   u, v, w  centred binomial with bits_u/v/w random bits (hmpc's defaults: 2, 40, 40 —
+ *            variances 1/2, 10, 10), drawn on the context's capability stream with nonces
+ *            nonce, nonce + nN, nonce + 2nN; the context nonce advances by 3nN
+ *   c0 = A*NTT(u) + NTT(v)*p + NTT(m),  c1 = B*NTT(u) + NTT(w)*p
+ * pk_a, pk_b: [L][N] in NTT form; messages: n*N u32 words lifted as unsigned values;
+ * p_mont: plaintext modulus as a Montgomery constant (L host words); c0, c1: [L][n*N]. */
+int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, const uint32_t* pk_a,
+                     const uint32_t* pk_b, const uint32_t* messages, const uint32_t* p_mont, int bits_u,
+                     int bits_v, int bits_w, uint32_t* c0, uint32_t* c1, void* stream);
+/* Same result, hmpc's launch structure: per ciphertext, per-stage NTT launches and
+ * separate element-wise kernels (the plain one-launch-sequence-per-ciphertext path). */
+int hmpc_bgv_encrypt_plain(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, const uint32_t* pk_a,
+                           const uint32_t* pk_b, const uint32_t* messages, const uint32_t* p_mont, int bits_u,
+                           int bits_v, int bits_w, uint32_t* c0, uint32_t* c1, void* stream);
+/* hmpc/expr/crypto/lhe/dec.hpp:13-45 with plaintext modulus 2^32: x = INTT(c0 - s*c1),
+ * centred lift, low 32 bits -> messages (n*N u32). */
+int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_ntt, const uint32_t* c0,
+                     const uint32_t* c1, uint32_t* messages, void* stream);
+
+/* ------------------------------------------------------------------ timing helpers
+ * HIP events on a given stream (used by benchmarks to time kernels on the stream they run on). */
+int hmpc_event_create(hmpc_context ctx, void** event);
+int hmpc_event_destroy(hmpc_context ctx, void* event);
+int hmpc_event_record(hmpc_context ctx, void* event, void* stream);
+int hmpc_event_elapsed_ms(hmpc_context ctx, void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMPC_HIP_H */

@@ -1,0 +1,242 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle, bit for bit.
+
+Inputs are seeded explicit-key ChaCha20 streams (key 42), sizes small enough for the
+oracle to finish in seconds. Every case prints its name before touching the GPU.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import hmpc_amd
+import oracle
+from bigref import P150, Q64, Q128A, Q128B
+
+pytestmark = pytest.mark.gpu
+
+Q279 = 0x591F5B834C0D961F67343BCC8902BDEDA2771F54306FF151162FF8D2B40F4194DC0001  # tests/expr/crypto/lhe/enc.cpp:15
+KEY = [42, 0, 0, 0, 0, 0, 0, 0]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hmpc_amd.Context(0, key=KEY)
+    yield c
+    c.wait()
+
+
+_mods = {}
+
+
+def mods(ctx, q):
+    if q not in _mods:
+        _mods[q] = (hmpc_amd.Modulus(ctx, q), oracle.Modulus(q))
+    return _mods[q]
+
+
+NTT_CASES = [
+    (Q64, 2, 3), (Q64, 3, 2), (Q64, 5, 4), (Q64, 12, 3), (Q64, 13, 2), (Q64, 14, 2), (Q64, 16, 1),
+    (Q128A, 4, 3), (Q128A, 12, 2), (Q128A, 13, 2), (Q128A, 16, 2), (Q128B, 14, 3),
+    (P150, 11, 2), (Q279, 10, 2),
+]
+
+
+@pytest.mark.parametrize("q,log_n,n", NTT_CASES, ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
+def test_ntt_matches_oracle(ctx, q, log_n, n):
+    gm, om = mods(ctx, q)
+    say(f"ntt q={hex(q)[:12]} L={gm.limbs} log_n={log_n} n={n}")
+    count = n << log_n
+    x = om.sample_uniform(KEY, 0, count, threads=8)
+    want_f = om.ntt(x, log_n, threads=8)
+    want_i = om.ntt(x, log_n, inverse=True, threads=8)
+    dx = gm.upload(x)
+    for plain in (False, True):
+        f = gm.download(gm.ntt(log_n, n, dx, plain=plain), count)
+        assert np.array_equal(f, want_f), f"forward plain={plain}"
+        i = gm.download(gm.ntt(log_n, n, dx, inverse=True, plain=plain), count)
+        assert np.array_equal(i, want_i), f"inverse plain={plain}"
+    # in place round trip
+    buf = gm.upload(x)
+    gm.ntt(log_n, n, buf, out=buf)
+    gm.ntt(log_n, n, buf, out=buf, inverse=True)
+    assert np.array_equal(gm.download(buf, count), x)
+
+
+@pytest.mark.parametrize("q,log_n", [(Q64, 12), (Q128A, 16), (Q128B, 14), (P150, 14)])
+def test_twiddles_and_root_match_oracle(ctx, q, log_n):
+    gm, om = mods(ctx, q)
+    say(f"twiddles q={hex(q)[:12]} log_n={log_n}")
+    assert gm.root(log_n) == om.root_of_unity(log_n + 1)
+    assert gm.root(log_n, inverse=True) == om.root_of_unity(log_n + 1, inverse=True)
+    assert np.array_equal(gm.twiddles(log_n), om.roots(log_n))
+    assert np.array_equal(gm.twiddles(log_n, inverse=True), om.roots(log_n, inverse=True))
+    npr, r1, r2 = gm.constants()
+    assert (npr, r1, r2) == om.constants()
+
+
+def test_poly_mod_kat_on_gpu(ctx):
+    """tests/ints/poly_mod.cpp:60-103 on the GPU: x_i = i, two polynomials, N = 2^14."""
+    gm, om = mods(ctx, P150)
+    say("poly_mod KAT p150 N=2^14")
+    log_n, N = 14, 1 << 14
+    R = 1 << (32 * gm.limbs)
+    x = oracle.ints_to_planar([(i % N) * R % P150 for i in range(2 * N)], gm.limbs)
+    y = gm.download(gm.ntt(log_n, 2, gm.upload(x)), 2 * N)
+    vals = oracle.planar_to_ints(y)
+    Rinv = pow(R, -1, P150)
+    kat = {0: 0x7bf0fe81e23706642244f0cf8fb65af112666, 1: 0xb8129497504c45e2fab6f665863b8d0cdb431,
+           2: 0x25b2015ae56339a59389c056ae782f1e58b697, 10: 0x2433793de67b5574558745981a36837082b73e,
+           19: 0x1cdb8c2cce3758d6acd87bf78549948e08faba}
+    for j in range(2):
+        for k, want in kat.items():
+            assert vals[j * N + k] * Rinv % P150 == want
+
+
+POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 16, 2),
+                 (Q128B, 14, 2), (P150, 9, 2)]
+
+
+@pytest.mark.parametrize("q,log_n,n", POLYMUL_CASES, ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
+def test_polymul_matches_oracle(ctx, q, log_n, n):
+    gm, om = mods(ctx, q)
+    say(f"polymul q={hex(q)[:12]} log_n={log_n} n={n}")
+    count = n << log_n
+    a = om.sample_uniform(KEY, 0, count, threads=8)
+    b = om.sample_uniform(KEY, 1, count, threads=8)
+    want, _ = om.polymul_timed(log_n, a, b, threads=8)
+    got = gm.download(gm.polymul(log_n, n, gm.upload(a), gm.upload(b)), count)
+    assert np.array_equal(got, want)
+
+
+def test_polymul_equals_schoolbook_c1(ctx):
+    """N=4096, q64: INTT(NTT(x) o NTT(y)) == schoolbook negacyclic product."""
+    gm, om = mods(ctx, Q64)
+    say("polymul vs schoolbook q64 N=4096")
+    x = om.sample_uniform(KEY, 0, 4096)
+    y = om.sample_uniform(KEY, 1, 4096)
+    got = gm.download(gm.polymul(12, 1, gm.upload(x), gm.upload(y)), 4096)
+    assert np.array_equal(got, om.schoolbook_negacyclic(12, x, y))
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A, P150, Q279])
+def test_samplers_match_oracle(ctx, q):
+    gm, om = mods(ctx, q)
+    say(f"samplers q={hex(q)[:12]}")
+    count = 5000
+    for offset in (0, 3):
+        got = gm.download(gm.sample_uniform(KEY, offset, count), count)
+        assert np.array_equal(got, om.sample_uniform(KEY, offset, count))
+    for bits in (2, 20, 40, 64):
+        got = gm.download(gm.sample_cbd(KEY, 2, bits, count), count)
+        assert np.array_equal(got, om.sample_cbd(KEY, 2, bits, count)), bits
+    words = ctx.sample_words(KEY, 3, count).to_numpy(count)
+    assert np.array_equal(words, oracle.sample_words(KEY, 3, count))
+
+
+def test_from_u32_and_to_unsigned(ctx):
+    gm, om = mods(ctx, Q128A)
+    say("from_u32 / to_unsigned")
+    count = 4096
+    words = oracle.sample_words(KEY, 5, count)
+    dw = ctx.upload(words)
+    m = gm.download(gm.from_u32(count, dw), count)
+    R = 1 << 128
+    assert oracle.planar_to_ints(m) == [int(w) * R % Q128A for w in words]
+    back = gm.download(gm.to_unsigned(count, gm.upload(m)), count)
+    assert oracle.planar_to_ints(back) == [int(w) for w in words]
+
+
+@pytest.mark.parametrize("q,log_n,n,bits", [(Q64, 12, 3, 40), (Q128B, 14, 2, 2), (Q128A, 6, 3, 64), (P150, 10, 2, 40)])
+def test_capability_cbd_ntt(ctx, q, log_n, n, bits):
+    gm, om = mods(ctx, q)
+    say(f"cbd_ntt q={hex(q)[:12]} log_n={log_n} n={n} bits={bits}")
+    key = [42, 0, 0, 0, 0, 0, 0, 1]
+    ctx.set_rng(key, [7, 0, 0])
+    got = gm.download(gm.sample_cbd_ntt(bits, log_n, n), n << log_n)
+    want = om.ntt(om.sample_cbd_capability(key, [7, 0, 0], bits, log_n, n), log_n, threads=8)
+    assert np.array_equal(got, want)
+    assert ctx.rng[1].tolist() == [7 + (n << log_n), 0, 0]
+
+
+def _keygen(om, log_n, t_mont):
+    N = 1 << log_n
+    B = om.sample_uniform(KEY, 0, N)
+    s = om.ntt(om.sample_cbd(KEY, 1, 2, N), log_n)
+    e = om.ntt(om.sample_cbd(KEY, 2, 40, N), log_n)
+    tcol = oracle.ints_to_planar([t_mont] * N, om.limbs)
+    A = om.pointwise("add", om.pointwise("mul", B, s), om.pointwise("mul", e, tcol))
+    return A, B, s
+
+
+@pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q128A, 12, 2), (Q64, 10, 3), (Q128A, 16, 1)])
+def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
+    gm, om = mods(ctx, q)
+    say(f"bgv q={hex(q)[:12]} log_n={log_n} n={n}")
+    N = 1 << log_n
+    t_mont = om.from_unsigned(1 << 32)
+    A, B, s = _keygen(om, log_n, t_mont)
+    msgs = oracle.sample_words(KEY, 3, n * N)
+    enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
+    want0, want1, nonce = om.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, [0, 0, 0], threads=8)
+    dA, dB, dm = gm.upload(A), gm.upload(B), ctx.upload(msgs)
+    pm = oracle.int_to_limbs(t_mont, gm.limbs)
+    for plain in (False, True):
+        ctx.set_rng(enc_key, [0, 0, 0])
+        c0, c1 = gm.bgv_encrypt(log_n, n, dA, dB, dm, pm, plain=plain)
+        assert np.array_equal(gm.download(c0, n * N), want0), f"c0 plain={plain}"
+        assert np.array_equal(gm.download(c1, n * N), want1), f"c1 plain={plain}"
+        assert ctx.rng[1].tolist() == nonce.tolist()
+    dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
+    assert np.array_equal(dec, msgs)
+    assert np.array_equal(dec, om.bgv_decrypt(log_n, s, want0, want1))
+
+
+def test_errors_are_reported(ctx):
+    say("error paths")
+    with pytest.raises(hmpc_amd.HmpcError):
+        hmpc_amd.Modulus(ctx, 1 << 70)  # even
+    gm, _ = mods(ctx, Q128B)
+    with pytest.raises(hmpc_amd.HmpcError):
+        gm.prepare(16)  # 2N does not divide q - 1
+    with pytest.raises(hmpc_amd.HmpcError):
+        gm.prepare(1)
+
+
+def _edge_values(q, limbs, count, seed):
+    rng = np.random.default_rng(seed)
+    R = 1 << (32 * limbs)
+    edges = [0, 1, 2, q - 1, q - 2, (q - 1) // 2, (q + 1) // 2, R % q, (R - 1) % q, (1 << (32 * limbs - 1)) % q]
+    edges += [((1 << (32 * k)) - 1) % q for k in range(1, limbs + 1)]
+    vals = [int.from_bytes(rng.bytes(4 * limbs), "little") % q for _ in range(count - len(edges))]
+    return edges + vals
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A, Q128B, P150, Q279, 1601, 0x8822D80623320001])
+def test_pointwise_ops_edge_values(ctx, q):
+    gm, om = mods(ctx, q)
+    say(f"pointwise edges q={hex(q)[:12]}")
+    count = 20000
+    a_vals = _edge_values(q, gm.limbs, count, 1)
+    b_vals = _edge_values(q, gm.limbs, count, 2)[::-1]
+    # every edge against every edge in the first block
+    k = 20
+    a = oracle.ints_to_planar(a_vals, gm.limbs)
+    b = oracle.ints_to_planar(b_vals, gm.limbs)
+    da, db = gm.upload(a), gm.upload(b)
+    for op, name in ((gm.ADD, "add"), (gm.SUB, "sub"), (gm.MUL, "mul")):
+        got = gm.download(gm.pointwise(op, count, da, db), count)
+        assert np.array_equal(got, om.pointwise(name, a, b)), name
+    # broadcast of a short operand (unsqueeze) and scalar constants
+    got = gm.download(gm.pointwise(gm.MUL, count, da, db, b_count=k), count)
+    assert np.array_equal(got, om.pointwise("mul", a, np.ascontiguousarray(b[:, :k]))), "broadcast"
+    for c in (0, 1, q - 1, a_vals[5000]):
+        cw = oracle.int_to_limbs(c, gm.limbs)
+        got = gm.download(gm.pointwise_scalar(gm.MUL, count, da, cw), count)
+        want = om.pointwise("mul", a, oracle.ints_to_planar([c], gm.limbs))
+        assert np.array_equal(got, want), hex(c)
