@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmpc_hip.so")
-SOURCES = ["kernels.hip", "api.hip"]
+SOURCES = ["kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip", "kernels_fast_l4.hip", "api.hip"]
 ARCH = os.environ.get("HMPC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -4,6 +4,7 @@
 // search of hmpc/ints/num/theory/root_of_unity.hpp, inverses) runs here on the CPU with the
 // same limb templates the kernels use.
 #include "kernels.hpp"
+#include "fastmath.hpp"
 
 #include "../../include/hmpc_hip.h"
 
@@ -155,6 +156,8 @@ struct table_pair
 {
     u32* fwd = nullptr;
     u32* inv = nullptr;
+    u32* fwd_em = nullptr;
+    u32* inv_em = nullptr;
     u32 root[max_limbs] = {};
     u32 inverse_root[max_limbs] = {};
 };
@@ -167,18 +170,23 @@ struct hmpc_modulus_s
     mod_params P{};
     u32 one[max_limbs] = {};
     u32 r2[max_limbs] = {};
+    u32 q2[max_limbs] = {};
+    int form = form_generic;  // arithmetic form of the fast path
+    int variant = 0;          // 0: fast path where supported, 1: portable LDS radix-2 kernels
     std::map<int, table_pair> tables;
+    std::map<int, int> elems; // fast path: elements per thread chosen per log_n (tuner)
+
+    int default_elems(int) const { return limbs == 4 ? 4 : 16; }
 
     ~hmpc_modulus_s()
     {
         for (auto& [k, t] : tables)
-        {
-            if (t.fwd)
-                (void)hipFree(t.fwd);
-            if (t.inv)
-                (void)hipFree(t.inv);
-        }
+            for (u32* p : {t.fwd, t.inv, t.fwd_em, t.inv_em})
+                if (p)
+                    (void)hipFree(p);
     }
+
+    bool fast(int log_n) const { return variant == 0 && fast_path_supported(limbs, form, log_n); }
 
     int prepare(int log_n);
 
@@ -194,12 +202,26 @@ struct hmpc_modulus_s
         T.log_n = log_n;
         T.fwd = t.fwd;
         T.inv = t.inv;
+        T.fwd_em = t.fwd_em;
+        T.inv_em = t.inv_em;
+        std::copy(q2, q2 + max_limbs, T.q2);
+        T.form = form;
+        auto e = elems.find(log_n);
+        T.elems = e != elems.end() ? e->second : default_elems(log_n);
         return HMPC_OK;
     }
 };
 
 namespace
 {
+    hipError_t dispatch_ntt(hmpc_modulus_s const& m, transform_args const& T, std::size_t n, bool inverse, u32 const* in,
+                            u32* out, hipStream_t s)
+    {
+        if (m.fast(T.log_n))
+            return launch_fast_ntt(m.limbs, T, n, inverse, in, out, s);
+        return launch_ntt_fused(m.limbs, T, n, inverse, in, out, s);
+    }
+
     // hmpc/ints/num/theory/root_of_unity.hpp:19-54 with the all-zero ChaCha20 stream
     // (compiletime_number_generator, hmpc/random/number_generator.hpp:106-110).
     template<int L>
@@ -321,20 +343,24 @@ int hmpc_modulus_s::prepare(int log_n)
         std::copy(phi, phi + LL, t.root);
         std::copy(iphi, iphi + LL, t.inverse_root);
         std::size_t bytes = (std::size_t{4} * LL) << log_n;
-        if (hipMalloc(&t.fwd, bytes) != hipSuccess || hipMalloc(&t.inv, bytes) != hipSuccess)
-            return false;
+        for (u32** p : {&t.fwd, &t.inv, &t.fwd_em, &t.inv_em})
+            if (hipMalloc(p, bytes) != hipSuccess)
+                return false;
         if (launch_make_twiddles(LL, t.fwd, log_n, phi, ninv, one, false, P, nullptr) != hipSuccess)
             return false;
         if (launch_make_twiddles(LL, t.inv, log_n, iphi, ninv, one, true, P, nullptr) != hipSuccess)
+            return false;
+        if (launch_to_element_major(LL, t.fwd, t.fwd_em, std::size_t{1} << log_n, nullptr) != hipSuccess)
+            return false;
+        if (launch_to_element_major(LL, t.inv, t.inv_em, std::size_t{1} << log_n, nullptr) != hipSuccess)
             return false;
         return hipDeviceSynchronize() == hipSuccess;
     });
     if (!ok)
     {
-        if (t.fwd)
-            (void)hipFree(t.fwd);
-        if (t.inv)
-            (void)hipFree(t.inv);
+        for (u32* p : {t.fwd, t.inv, t.fwd_em, t.inv_em})
+            if (p)
+                (void)hipFree(p);
         return fail(HMPC_ERR_INVALID_MODULUS, "no primitive 2N-th root of unity for this modulus (2N must divide q - 1)");
     }
     tables[log_n] = t;
@@ -543,6 +569,23 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
             std::copy(x, x + L, m->one);
     }
     std::copy(x, x + L, m->r2);
+    {
+        u32 c = 0;
+        for (int i = 0; i < L; ++i)
+            m->q2[i] = addc(m->P.q[i], m->P.q[i], c, c);
+    }
+    bool lazy_ok = (m->P.q[L - 1] >> 30) == 0;  // 4q < R
+    bool middle_zero = L >= 3;
+    for (int i = 1; i + 1 < L; ++i)
+        middle_zero &= m->P.q[i] == 0;
+    if (L == 2 && m->P.q[0] == 1u && m->P.q[1] == 0xffffffffu)
+        m->form = form_goldilocks;
+    else if (lazy_ok && middle_zero)
+        m->form = form_sparse;
+    else if (lazy_ok)
+        m->form = form_lazy;
+    else
+        m->form = form_generic;
     *out = m.release();
     return HMPC_OK;
 }
@@ -572,6 +615,41 @@ int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* npr
         std::copy(mod->one, mod->one + mod->limbs, r_mod_q);
     if (r2_mod_q)
         std::copy(mod->r2, mod->r2 + mod->limbs, r2_mod_q);
+    return HMPC_OK;
+}
+
+int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant)
+{
+    if (!mod || variant < 0 || variant > 1)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "variant must be 0 (fast) or 1 (portable)");
+    mod->variant = variant;
+    return HMPC_OK;
+}
+
+int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread)
+{
+    if (!mod || log_n < 2 || log_n > 20)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
+    bool ok = elems_per_thread == 0 || (mod->limbs == 4 && (elems_per_thread == 4 || elems_per_thread == 8)) ||
+              (mod->limbs == 2 && (elems_per_thread == 8 || elems_per_thread == 16));
+    if (!ok)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "elements per thread: 4 or 8 (4 limbs), 8 or 16 (2 limbs), 0 = default");
+    if (elems_per_thread == 0)
+        mod->elems.erase(log_n);
+    else
+        mod->elems[log_n] = elems_per_thread;
+    return HMPC_OK;
+}
+
+int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* fast)
+{
+    if (!mod || log_n < 2 || log_n > 20)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
+    auto e = mod->elems.find(log_n);
+    if (elems_per_thread)
+        *elems_per_thread = e != mod->elems.end() ? e->second : mod->default_elems(log_n);
+    if (fast)
+        *fast = mod->fast(log_n) ? 1 : 0;
     return HMPC_OK;
 }
 
@@ -620,7 +698,7 @@ int hmpc_ntt(hmpc_modulus mod, int log_n, size_t n, int inverse, const uint32_t*
     if (rc)
         return rc;
     device_guard g(mod->ctx->device);
-    HMPC_HIP(launch_ntt_fused(mod->limbs, T, n, inverse != 0, in, out, as_stream(stream)));
+    HMPC_HIP(dispatch_ntt(*mod, T, n, inverse != 0, in, out, as_stream(stream)));
     return HMPC_OK;
 }
 
@@ -792,14 +870,18 @@ int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const
     device_guard g(mod->ctx->device);
     hipStream_t s = as_stream(stream);
     void* w = nullptr;
-    std::size_t words = polymul_workspace_words(mod->limbs, log_n, n);
+    bool fast = mod->fast(log_n);
+    std::size_t words = fast ? fast_polymul_workspace_words(mod->limbs, log_n, n) : polymul_workspace_words(mod->limbs, log_n, n);
     if (words)
     {
         rc = mod->ctx->workspace(s, words * 4, &w);
         if (rc)
             return rc;
     }
-    HMPC_HIP(launch_polymul(mod->limbs, T, n, a, b, out, static_cast<u32*>(w), s));
+    if (fast)
+        HMPC_HIP(launch_fast_polymul(mod->limbs, T, n, a, b, out, static_cast<u32*>(w), s));
+    else
+        HMPC_HIP(launch_polymul(mod->limbs, T, n, a, b, out, static_cast<u32*>(w), s));
     return HMPC_OK;
 }
 
@@ -941,7 +1023,7 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
     int L = mod->limbs;
     HMPC_HIP(launch_pointwise(L, mod->P, HMPC_MUL, count, c1, s_ntt, std::size_t{1} << log_n, x, s));
     HMPC_HIP(launch_pointwise(L, mod->P, HMPC_SUB, count, c0, x, count, x, s));
-    HMPC_HIP(launch_ntt_fused(L, T, n, true, x, x, s));
+    HMPC_HIP(dispatch_ntt(*mod, T, n, true, x, x, s));
     HMPC_HIP(launch_centered_low32(L, mod->P, count, x, messages, s));
     return HMPC_OK;
 }

@@ -1,0 +1,766 @@
+// Fast transform path for L = 2 and L = 4 limbs (the 64-bit and 125-bit moduli of the
+// workloads), gfx950. Included by kernels_fast_l2.hip and kernels_fast_l4.hip, which
+// instantiate it for one limb count each (separate translation units build in parallel).
+//
+// A workgroup owns a 4096-element tile in LDS (element-major, one ds_read_b128 / ds_read_b64
+// per element, XOR-swizzled so that strided lanes hit distinct banks). Each of its 4096 / E
+// threads keeps E elements (E = 4, 8 or 16, a tuning knob) in VGPRs and runs up to log2(E)
+// radix-2 stages on them between two LDS exchanges ("rounds"), so an S-point sub-transform
+// costs ceil(log S / log E) exchanges instead of log S. At the start of a round every thread
+// issues the loads of all twiddles the round needs (element-major copies of hmpc's tables,
+// one global_load_dwordx4 per twiddle), so their latencies overlap.
+//
+// Round geometry (s = log S, round = stages [a, b), w = b - a). A "unit" is the group of
+// 2^w elements that interact in the round; a thread handles E / 2^w units.
+//   forward (CT, stage i pairs bit s-1-i):  j = F << (s-a) | k << (s-b) | Lo,  F: top a bits
+//       twiddle index  (base << i) + (F << (i-a) | k >> (b-i))
+//   inverse (GS, stage i pairs bit i):      j = F << b | k << a | Lo,          Lo: low a bits
+//       twiddle index  base * (S >> (i+1)) + (F << (b-i-1) | k >> (i-a+1))
+// base = 1 for column passes and whole-polynomial tiles, N1 + hi for the row pass of row hi
+// (the two-pass decomposition is described in kernels.hip). Forward rounds are full-width
+// first and partial last, inverse rounds partial first, so the forward's last round and the
+// inverse's first round group the same elements: a product stays in registers between them.
+#pragma once
+
+#include "kernels.hpp"
+#include "fastmath.hpp"
+
+#include <type_traits>
+
+namespace hmpc_amd
+{
+    namespace fast
+    {
+        constexpr int tile_log = 12;
+        constexpr int tile = 1 << tile_log;
+
+        constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+        struct tile_desc
+        {
+            int s;         // log2 of the sub-transform length
+            int log_seqs;  // s + log_seqs = 12
+            bool rows;     // row layout (c << s) + j, else column layout (j << log_seqs) + c
+        };
+
+        struct unit_t
+        {
+            u32 c, F, Lo;
+        };
+
+        __device__ __forceinline__ unit_t unit_of(tile_desc const& td, int a, int w, bool inv, int u)
+        {
+            int fbits = td.s - w;
+            u32 f, c;
+            if (td.rows)
+            {
+                f = static_cast<u32>(u) & ((1u << fbits) - 1u);
+                c = static_cast<u32>(u) >> fbits;
+            }
+            else
+            {
+                c = static_cast<u32>(u) & ((1u << td.log_seqs) - 1u);
+                f = static_cast<u32>(u) >> td.log_seqs;
+            }
+            unit_t r;
+            r.c = c;
+            if (!inv)
+            {
+                int lo_bits = td.s - a - w;
+                r.F = f >> lo_bits;
+                r.Lo = f & ((1u << lo_bits) - 1u);
+            }
+            else
+            {
+                r.F = f >> a;
+                r.Lo = f & ((1u << a) - 1u);
+            }
+            return r;
+        }
+
+        __device__ __forceinline__ u32 j_of(unit_t const& un, u32 k, int a, int w, int s, bool inv)
+        {
+            return inv ? ((un.F << (a + w)) | (k << a) | un.Lo) : ((un.F << (s - a)) | (k << (s - a - w)) | un.Lo);
+        }
+
+        __device__ __forceinline__ int elem_of(tile_desc const& td, u32 c, u32 j)
+        {
+            return td.rows ? static_cast<int>((c << td.s) + j) : static_cast<int>((j << td.log_seqs) + c);
+        }
+
+        template<int L>
+        __device__ __forceinline__ int phys(int e)
+        {
+            constexpr int M = (32 / L) < 16 ? (32 / L) : 16;
+            return e ^ ((e >> 4) & (M - 1));
+        }
+
+        template<int L>
+        __device__ __forceinline__ void lds_get(u32 (&x)[L], u32 const* lds, int e)
+        {
+            int p = phys<L>(e);
+            if constexpr (L == 4)
+            {
+                uint4 v = reinterpret_cast<uint4 const*>(lds)[p];
+                x[0] = v.x;
+                x[1] = v.y;
+                x[2] = v.z;
+                x[3] = v.w;
+            }
+            else
+            {
+                static_assert(L == 2);
+                uint2 v = reinterpret_cast<uint2 const*>(lds)[p];
+                x[0] = v.x;
+                x[1] = v.y;
+            }
+        }
+
+        template<int L>
+        __device__ __forceinline__ void lds_put(u32* lds, int e, u32 const (&x)[L])
+        {
+            int p = phys<L>(e);
+            if constexpr (L == 4)
+                reinterpret_cast<uint4*>(lds)[p] = make_uint4(x[0], x[1], x[2], x[3]);
+            else
+                reinterpret_cast<uint2*>(lds)[p] = make_uint2(x[0], x[1]);
+        }
+
+        // twiddle from an element-major table [N][L]
+        template<int L>
+        __device__ __forceinline__ void tw_get(u32 (&w)[L], u32 const* __restrict__ tw, u32 idx)
+        {
+            if constexpr (L == 4)
+            {
+                uint4 v = reinterpret_cast<uint4 const*>(tw)[idx];
+                w[0] = v.x;
+                w[1] = v.y;
+                w[2] = v.z;
+                w[3] = v.w;
+            }
+            else
+            {
+                uint2 v = reinterpret_cast<uint2 const*>(tw)[idx];
+                w[0] = v.x;
+                w[1] = v.y;
+            }
+        }
+
+        // round schedule: forward full rounds first, inverse partial round first
+        template<int WMAX>
+        __device__ __forceinline__ int round_count(int s) { return (s + WMAX - 1) / WMAX; }
+        template<int WMAX>
+        __device__ __forceinline__ int round_start(int s, int r, bool inv)
+        {
+            int part = s % WMAX;
+            if (!inv || part == 0)
+                return WMAX * r;
+            return r == 0 ? 0 : part + WMAX * (r - 1);
+        }
+        template<int WMAX>
+        __host__ __device__ __forceinline__ int round_width(int s, int r, bool inv)
+        {
+            int part = s % WMAX;
+            int n = (s + WMAX - 1) / WMAX;
+            if (part == 0)
+                return WMAX;
+            if (!inv)
+                return r == n - 1 ? part : WMAX;
+            return r == 0 ? part : WMAX;
+        }
+
+        template<int E>
+        struct shape
+        {
+            static constexpr int elems = E;
+            static constexpr int threads = tile / E;
+            static constexpr int wmax = ilog2(E);
+        };
+
+        template<int L, int E, int W>
+        __device__ __forceinline__ void gather(u32 (&x)[E][L], u32 const* lds, tile_desc const& td, int a, bool inv)
+        {
+            constexpr int K = 1 << W;
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+                unit_t un = unit_of(td, a, W, inv, threadIdx.x + shape<E>::threads * g);
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    lds_get<L>(x[g * K + k], lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)));
+            }
+        }
+
+        template<int L, int E, int W>
+        __device__ __forceinline__ void scatter(u32 (&x)[E][L], u32* lds, tile_desc const& td, int a, bool inv)
+        {
+            constexpr int K = 1 << W;
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+                unit_t un = unit_of(td, a, W, inv, threadIdx.x + shape<E>::threads * g);
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    lds_put<L>(lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)), x[g * K + k]);
+            }
+        }
+
+        // twiddles of one round: per unit g, stage II occupies entries
+        //   forward: [2^II - 1, 2^(II+1) - 1), entry (2^II - 1) + (k >> (W - II))
+        //   inverse: [2^W - 2^(W-II), 2^W - 2^(W-II-1)), entry 2^W - 2^(W-II) + (k >> (II + 1))
+        template<int W>
+        constexpr int tw_slot(bool inv, int ii, int k)
+        {
+            return inv ? ((1 << W) - (1 << (W - ii)) + (k >> (ii + 1))) : ((1 << ii) - 1 + (k >> (W - ii)));
+        }
+
+        template<int L, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void prefetch_stage(u32 (&tws)[E >> W][(1 << W) - 1][L], tile_desc const& td, int a,
+                                                       u32 base0, u32 base_step, u32 const* __restrict__ tw)
+        {
+            constexpr int K = 1 << W;
+            constexpr int count = Inv ? (1 << (W - II - 1)) : (1 << II);
+            int const i = a + II;
+            u32 const h = 1u << (td.s - i - 1);
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+                unit_t un = unit_of(td, a, W, Inv, threadIdx.x + shape<E>::threads * g);
+                u32 base = base0 + un.c * base_step;
+#pragma unroll
+                for (int m = 0; m < count; ++m)
+                {
+                    u32 idx = Inv ? base * h + ((un.F << (W - II - 1)) | static_cast<u32>(m))
+                                  : (base << i) + ((un.F << II) | static_cast<u32>(m));
+                    constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
+                    tw_get<L>(tws[g][first + m], tw, idx);
+                }
+            }
+        }
+
+        template<int L, int Form, int E, bool Inv, int W, int II, bool Last>
+        __device__ __forceinline__ void stage(u32 (&x)[E][L], u32 const (&tws)[E >> W][(1 << W) - 1][L],
+                                              u32 const (&w0)[L], u32 const (&w1)[L], fast_params const& P)
+        {
+            using ops = fast_ops<L, Form>;
+            constexpr int K = 1 << W;
+            constexpr int lb = Inv ? II : (W - 1 - II);
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    if (k & (1 << lb))
+                        continue;
+                    int const slot = tw_slot<W>(Inv, II, k);
+                    if constexpr (!Inv)
+                        ops::fwd(x[g * K + k], x[g * K + (k | (1 << lb))], tws[g][slot], P);
+                    else if constexpr (Last)
+                        ops::inv_last(x[g * K + k], x[g * K + (k | (1 << lb))], w0, w1, P);
+                    else
+                        ops::inv(x[g * K + k], x[g * K + (k | (1 << lb))], tws[g][slot], P);
+                }
+            }
+        }
+
+        template<int L, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void prefetch_upto(u32 (&tws)[E >> W][(1 << W) - 1][L], tile_desc const& td, int a,
+                                                      u32 base0, u32 base_step, u32 const* __restrict__ tw)
+        {
+            if constexpr (II < W - 1)
+            {
+                prefetch_stage<L, E, Inv, W, II>(tws, td, a, base0, base_step, tw);
+                prefetch_upto<L, E, Inv, W, II + 1>(tws, td, a, base0, base_step, tw);
+            }
+        }
+
+        template<int L, int Form, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void stages_upto(u32 (&x)[E][L], u32 const (&tws)[E >> W][(1 << W) - 1][L],
+                                                    u32 const (&w0)[L], u32 const (&w1)[L], fast_params const& P)
+        {
+            if constexpr (II < W - 1)
+            {
+                stage<L, Form, E, Inv, W, II, false>(x, tws, w0, w1, P);
+                stages_upto<L, Form, E, Inv, W, II + 1>(x, tws, w0, w1, P);
+            }
+        }
+
+        // butterflies of one round: all twiddle loads first, then the stages. The round's last
+        // stage is the inverse's scaled final stage when the round ends the transform.
+        template<int L, int Form, int E, bool Inv, int W>
+        __device__ __forceinline__ void compute_round(u32 (&x)[E][L], tile_desc const& td, int a, u32 base0, u32 base_step,
+                                                      u32 const* __restrict__ tw, fast_params const& P, bool final_scale)
+        {
+            u32 tws[E >> W][(1 << W) - 1][L];
+            bool const scaled = Inv && final_scale && (a + W == td.s);
+            prefetch_upto<L, E, Inv, W, 0>(tws, td, a, base0, base_step, tw);
+            u32 w0[L] = {}, w1[L] = {};
+            if (scaled)
+            {
+                tw_get<L>(w0, tw, 0);
+                tw_get<L>(w1, tw, 1);
+            }
+            else
+                prefetch_stage<L, E, Inv, W, W - 1>(tws, td, a, base0, base_step, tw);
+            stages_upto<L, Form, E, Inv, W, 0>(x, tws, w0, w1, P);
+            if (scaled)
+                stage<L, Form, E, Inv, W, W - 1, true>(x, tws, w0, w1, P);
+            else
+                stage<L, Form, E, Inv, W, W - 1, false>(x, tws, w0, w1, P);
+        }
+
+        template<int L, int Form, int E, bool Inv, int W>
+        __device__ __forceinline__ void one_round(u32 (&x)[E][L], u32* lds, tile_desc const& td, int a, u32 base0,
+                                                  u32 base_step, u32 const* __restrict__ tw, fast_params const& P,
+                                                  bool final_scale, bool keep)
+        {
+            gather<L, E, W>(x, lds, td, a, Inv);
+            compute_round<L, Form, E, Inv, W>(x, td, a, base0, base_step, tw, P, final_scale);
+            if (!keep)
+                scatter<L, E, W>(x, lds, td, a, Inv);
+        }
+
+        template<int L, int Form, int E, bool Inv>
+        __device__ __forceinline__ void dispatch_round(int w, u32 (&x)[E][L], u32* lds, tile_desc const& td, int a, u32 base0,
+                                                       u32 base_step, u32 const* __restrict__ tw, fast_params const& P,
+                                                       bool final_scale, bool keep)
+        {
+            constexpr int WM = shape<E>::wmax;
+            if (w == WM)
+                one_round<L, Form, E, Inv, WM>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+            else if (WM > 1 && w == 1)
+                one_round<L, Form, E, Inv, 1>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+            else if constexpr (WM > 2)
+            {
+                if (w == 2)
+                    one_round<L, Form, E, Inv, 2>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                else if constexpr (WM > 3)
+                {
+                    if (w == 3)
+                        one_round<L, Form, E, Inv, 3>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                }
+            }
+        }
+
+        // All rounds of the tile's sub-transform, LDS -> LDS. With keep_last (forward only) the
+        // last round's results stay in x instead of being written back. (Width dispatch by
+        // explicit branches: a lambda over x would force the array into scratch memory.)
+        template<int L, int Form, int E, bool Inv>
+        __device__ __forceinline__ void tile_rounds(u32 (&x)[E][L], u32* lds, tile_desc const& td, u32 base0, u32 base_step,
+                                                    u32 const* __restrict__ tw, fast_params const& P, bool final_scale,
+                                                    int first_round, bool keep_last)
+        {
+            constexpr int WM = shape<E>::wmax;
+            int n = round_count<WM>(td.s);
+            for (int r = first_round; r < n; ++r)
+            {
+                int a = round_start<WM>(td.s, r, Inv);
+                int w = round_width<WM>(td.s, r, Inv);
+                bool keep = keep_last && r == n - 1;
+                dispatch_round<L, Form, E, Inv>(w, x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                if (!keep)
+                    __syncthreads();
+            }
+        }
+
+        // inverse round 0 computed on registers that already hold its elements, then written to LDS
+        template<int L, int Form, int E, int W>
+        __device__ __forceinline__ void regs_round(u32 (&x)[E][L], u32* lds, tile_desc const& td, u32 base0, u32 base_step,
+                                                   u32 const* __restrict__ tw, fast_params const& P, bool final_scale)
+        {
+            compute_round<L, Form, E, true, W>(x, td, 0, base0, base_step, tw, P, final_scale);
+            scatter<L, E, W>(x, lds, td, 0, true);
+        }
+
+        template<int L, int Form, int E>
+        __device__ __forceinline__ void dispatch_round_regs(int w, u32 (&x)[E][L], u32* lds, tile_desc const& td, u32 base0,
+                                                            u32 base_step, u32 const* __restrict__ tw, fast_params const& P,
+                                                            bool final_scale)
+        {
+            constexpr int WM = shape<E>::wmax;
+            if (w == WM)
+                regs_round<L, Form, E, WM>(x, lds, td, base0, base_step, tw, P, final_scale);
+            else if (WM > 1 && w == 1)
+                regs_round<L, Form, E, 1>(x, lds, td, base0, base_step, tw, P, final_scale);
+            else if constexpr (WM > 2)
+            {
+                if (w == 2)
+                    regs_round<L, Form, E, 2>(x, lds, td, base0, base_step, tw, P, final_scale);
+                else if constexpr (WM > 3)
+                {
+                    if (w == 3)
+                        regs_round<L, Form, E, 3>(x, lds, td, base0, base_step, tw, P, final_scale);
+                }
+            }
+        }
+
+        // global (limb-planar) <-> LDS tile. Element e of the tile maps to global index
+        // base + e (rows) or base + (e >> log_seqs) * stride + (e & (C - 1)) (columns).
+        struct io_desc
+        {
+            std::size_t base;   // global element index of tile element 0
+            std::size_t count;  // plane size (elements per limb plane)
+            u32 stride;         // column layout: distance between rows (N2)
+            int log_seqs;
+            bool rows;
+            int valid;          // tile elements backed by real data (tail tiles)
+        };
+
+        __device__ __forceinline__ std::size_t gidx(io_desc const& io, int e)
+        {
+            if (io.rows)
+                return io.base + e;
+            return io.base + static_cast<std::size_t>(e >> io.log_seqs) * io.stride + (e & ((1 << io.log_seqs) - 1));
+        }
+
+        template<int L, int T>
+        __device__ __forceinline__ void tile_load(u32* lds, u32 const* __restrict__ src, io_desc const& io)
+        {
+#pragma unroll 4
+            for (int e = threadIdx.x; e < tile; e += T)
+            {
+                u32 x[L];
+                if (e < io.valid)
+                {
+                    std::size_t g = gidx(io, e);
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        x[l] = src[l * io.count + g];
+                }
+                else
+                {
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        x[l] = 0;
+                }
+                lds_put<L>(lds, e, x);
+            }
+        }
+
+        // canonical: reduce lazy values to [0, q) on the way out
+        template<int L, int Form, int T>
+        __device__ __forceinline__ void tile_store(u32 const* lds, u32* __restrict__ dst, io_desc const& io, bool canonical,
+                                                   fast_params const& P)
+        {
+#pragma unroll 4
+            for (int e = threadIdx.x; e < io.valid; e += T)
+            {
+                u32 x[L];
+                lds_get<L>(x, lds, e);
+                if (canonical)
+                    fast_ops<L, Form>::canonical(x, P);
+                std::size_t g = gidx(io, e);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    dst[l * io.count + g] = x[l];
+            }
+        }
+
+        struct pass_geom
+        {
+            int log_n, log_n1, log_n2;
+            int log_seqs;  // sequences per tile (columns, rows or polynomials)
+            bool single;
+            u32 blocks_per_poly;
+        };
+
+        __device__ __forceinline__ void cols_setup(pass_geom const& G, u32 bid, std::size_t count, tile_desc& td, io_desc& io)
+        {
+            u32 p = bid / G.blocks_per_poly;
+            u32 cb = bid - p * G.blocks_per_poly;
+            td = tile_desc{G.log_n1, G.log_seqs, false};
+            io.base = (static_cast<std::size_t>(p) << G.log_n) + (static_cast<std::size_t>(cb) << G.log_seqs);
+            io.count = count;
+            io.stride = 1u << G.log_n2;
+            io.log_seqs = G.log_seqs;
+            io.rows = false;
+            io.valid = tile;
+        }
+
+        __device__ __forceinline__ u32 rows_setup(pass_geom const& G, u32 bid, std::size_t count, tile_desc& td, io_desc& io)
+        {
+            u32 p = bid / G.blocks_per_poly;
+            u32 rb = bid - p * G.blocks_per_poly;
+            u32 hi0 = rb << G.log_seqs;
+            td = tile_desc{G.log_n2, G.log_seqs, true};
+            io.base = (static_cast<std::size_t>(p) << G.log_n) + (static_cast<std::size_t>(hi0) << G.log_n2);
+            io.count = count;
+            io.stride = 0;
+            io.log_seqs = G.log_seqs;
+            io.rows = true;
+            io.valid = tile;
+            return hi0;
+        }
+
+        __device__ __forceinline__ void single_setup(pass_geom const& G, u32 bid, std::size_t n, tile_desc& td, io_desc& io)
+        {
+            td = tile_desc{G.log_n, G.log_seqs, true};
+            std::size_t first = static_cast<std::size_t>(bid) << G.log_seqs;
+            std::size_t polys = n - first < (std::size_t{1} << G.log_seqs) ? n - first : (std::size_t{1} << G.log_seqs);
+            io.base = first << G.log_n;
+            io.count = n << G.log_n;
+            io.stride = 0;
+            io.log_seqs = G.log_seqs;
+            io.rows = true;
+            io.valid = static_cast<int>(polys << G.log_n);
+        }
+
+        inline pass_geom geom_for(int log_n, bool rows_pass)
+        {
+            pass_geom G{};
+            G.log_n = log_n;
+            if (log_n <= tile_log)
+            {
+                G.single = true;
+                G.log_n1 = log_n;
+                G.log_n2 = 0;
+                G.log_seqs = tile_log - log_n;
+                G.blocks_per_poly = 1;
+                return G;
+            }
+            G.single = false;
+            G.log_n1 = log_n / 2;
+            G.log_n2 = log_n - G.log_n1;
+            if (rows_pass)
+            {
+                G.log_seqs = tile_log - G.log_n2;
+                G.blocks_per_poly = 1u << (G.log_n1 - G.log_seqs);
+            }
+            else
+            {
+                G.log_seqs = tile_log - G.log_n1;
+                G.blocks_per_poly = 1u << (G.log_n2 - G.log_seqs);
+            }
+            return G;
+        }
+
+        constexpr std::size_t tile_bytes(int L) { return std::size_t{4} * L * tile; }
+
+        inline fast_params make_fast(transform_args const& T)
+        {
+            fast_params P{};
+            for (int l = 0; l < max_limbs; ++l)
+            {
+                P.q[l] = T.P.q[l];
+                P.q2[l] = T.q2[l];
+            }
+            P.nprime = T.P.nprime;
+            return P;
+        }
+
+        // ------------------------------------------------------------------ kernels
+        // mode: 0 forward column pass, 1 forward row pass (final), 2 inverse row pass,
+        //       3 inverse column pass (final), 4 forward whole-poly, 5 inverse whole-poly
+        template<int L, int Form, int E, int Mode>
+        __global__ void __launch_bounds__(tile / E, E >= 8 ? 2 : 4) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
+                                                           u32 const* __restrict__ tw, pass_geom G, u64 n, fast_params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr bool Inv = Mode == 2 || Mode == 3 || Mode == 5;
+            constexpr int T = tile / E;
+            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
+            tile_desc td;
+            io_desc io;
+            u32 base0 = 1, base_step = 0;
+            bool final_scale = false, canonical = false;
+            if constexpr (Mode == 0 || Mode == 3)
+            {
+                cols_setup(G, blockIdx.x, count, td, io);
+                final_scale = Mode == 3;
+                canonical = Mode == 3;
+            }
+            else if constexpr (Mode == 1 || Mode == 2)
+            {
+                u32 hi0 = rows_setup(G, blockIdx.x, count, td, io);
+                base0 = (1u << G.log_n1) + hi0;
+                base_step = 1;
+                canonical = Mode == 1;
+            }
+            else
+            {
+                single_setup(G, blockIdx.x, n, td, io);
+                final_scale = Mode == 5;
+                canonical = true;
+            }
+            tile_load<L, T>(lds, in, io);
+            __syncthreads();
+            u32 x[E][L];
+            tile_rounds<L, Form, E, Inv>(x, lds, td, base0, base_step, tw, P, final_scale, 0, false);
+            tile_store<L, Form, T>(lds, out, io, canonical, P);
+        }
+
+        // polynomial product, whole polynomials per tile (N <= 4096) or the row pass of the
+        // two-pass product: forward(A), forward(B), A o B in registers, inverse.
+        template<int L, int Form, int E, bool Single>
+        __global__ void __launch_bounds__(tile / E, E >= 8 ? 2 : 4) k_polymul(u32 const* __restrict__ a, u32 const* __restrict__ b,
+                                                              u32* __restrict__ out, u32 const* __restrict__ twf,
+                                                              u32 const* __restrict__ twi, pass_geom G, u64 n, fast_params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            using ops = fast_ops<L, Form>;
+            constexpr int T = tile / E;
+            constexpr int WM = shape<E>::wmax;
+            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
+            tile_desc td;
+            io_desc io;
+            u32 base0 = 1, base_step = 0;
+            if constexpr (Single)
+            {
+                single_setup(G, blockIdx.x, n, td, io);
+            }
+            else
+            {
+                u32 hi0 = rows_setup(G, blockIdx.x, count, td, io);
+                base0 = (1u << G.log_n1) + hi0;
+                base_step = 1;
+            }
+            u32 xa[E][L];
+            u32 xb[E][L];
+            tile_load<L, T>(lds, a, io);
+            __syncthreads();
+            tile_rounds<L, Form, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
+            __syncthreads();
+            tile_load<L, T>(lds, b, io);
+            __syncthreads();
+            tile_rounds<L, Form, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+            {
+                ops::half_reduce(xa[k], P);
+                ops::half_reduce(xb[k], P);
+                u32 t[L];
+                ops::mul(t, xa[k], xb[k], P);
+                copy<L>(xa[k], t);
+            }
+            // inverse round 0 groups the same elements as the forward's last round
+            int w0 = round_width<WM>(td.s, 0, true);
+            dispatch_round_regs<L, Form, E>(w0, xa, lds, td, base0, base_step, twi, P, Single);
+            __syncthreads();
+            tile_rounds<L, Form, E, true>(xa, lds, td, base0, base_step, twi, P, Single, 1, false);
+            tile_store<L, Form, T>(lds, out, io, Single, P);
+        }
+
+        // forward column pass of both inputs of the two-pass product (blockIdx.x = 2 * tile + which)
+        template<int L, int Form, int E>
+        __global__ void __launch_bounds__(tile / E, E >= 8 ? 2 : 4) k_polymul_cols(u32 const* __restrict__ a, u32 const* __restrict__ b,
+                                                                   u32* __restrict__ wa, u32* __restrict__ wb,
+                                                                   u32 const* __restrict__ tw, pass_geom G, u64 n,
+                                                                   fast_params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
+            u32 which = blockIdx.x & 1u;
+            tile_desc td;
+            io_desc io;
+            cols_setup(G, blockIdx.x >> 1, count, td, io);
+            tile_load<L, T>(lds, which ? b : a, io);
+            __syncthreads();
+            u32 x[E][L];
+            tile_rounds<L, Form, E, false>(x, lds, td, 1u, 0u, tw, P, false, 0, false);
+            tile_store<L, Form, T>(lds, which ? wb : wa, io, false, P);
+        }
+    }
+}
+
+namespace hmpc_amd
+{
+    namespace fast
+    {
+        // ------------------------------------------------------------------ host launchers
+        // The fast kernels are instantiated for the special-form moduli only (sparse 4-limb,
+        // Goldilocks 2-limb); every other modulus runs the portable kernels of kernels.hip.
+        template<int L>
+        constexpr int fast_form = L == 4 ? form_sparse : form_goldilocks;
+
+        template<int L, typename F>
+        hipError_t with_form_elems(int form, int elems, F&& f)
+        {
+            if (form != fast_form<L>)
+                return hipErrorInvalidValue;
+            using FC = std::integral_constant<int, fast_form<L>>;
+            if constexpr (L == 4)
+            {
+                if (elems == 4)
+                    return f(FC{}, std::integral_constant<int, 4>{});
+                return f(FC{}, std::integral_constant<int, 8>{});
+            }
+            else
+            {
+                if (elems == 8)
+                    return f(FC{}, std::integral_constant<int, 8>{});
+                return f(FC{}, std::integral_constant<int, 16>{});
+            }
+        }
+
+        template<int L>
+        hipError_t launch_ntt(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out, hipStream_t s)
+        {
+            fast_params P = make_fast(T);
+            return with_form_elems<L>(T.form, T.elems, [&](auto FC, auto EC) -> hipError_t {
+                constexpr int FF = decltype(FC)::value;
+                constexpr int E = decltype(EC)::value;
+                constexpr int TH = tile / E;
+                std::size_t bytes = tile_bytes(L);
+                u32 const* tw = inverse ? T.inv_em : T.fwd_em;
+                if (T.log_n <= tile_log)
+                {
+                    pass_geom G = geom_for(T.log_n, true);
+                    unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
+                    if (!inverse)
+                        k_pass<L, FF, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                    else
+                        k_pass<L, FF, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                    return hipGetLastError();
+                }
+                pass_geom gc = geom_for(T.log_n, false);
+                pass_geom gr = geom_for(T.log_n, true);
+                unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
+                unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
+                if (!inverse)
+                {
+                    k_pass<L, FF, E, 0><<<grid_c, TH, bytes, s>>>(in, out, tw, gc, n, P);
+                    k_pass<L, FF, E, 1><<<grid_r, TH, bytes, s>>>(out, out, tw, gr, n, P);
+                }
+                else
+                {
+                    k_pass<L, FF, E, 2><<<grid_r, TH, bytes, s>>>(in, out, tw, gr, n, P);
+                    k_pass<L, FF, E, 3><<<grid_c, TH, bytes, s>>>(out, out, tw, gc, n, P);
+                }
+                return hipGetLastError();
+            });
+        }
+
+        template<int L>
+        hipError_t launch_polymul(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                  u32* workspace, hipStream_t s)
+        {
+            fast_params P = make_fast(T);
+            return with_form_elems<L>(T.form, T.elems, [&](auto FC, auto EC) -> hipError_t {
+                constexpr int FF = decltype(FC)::value;
+                constexpr int E = decltype(EC)::value;
+                constexpr int TH = tile / E;
+                std::size_t bytes = tile_bytes(L);
+                if (T.log_n <= tile_log)
+                {
+                    pass_geom G = geom_for(T.log_n, true);
+                    unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
+                    k_polymul<L, FF, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_em, T.inv_em, G, n, P);
+                    return hipGetLastError();
+                }
+                std::size_t count = n << T.log_n;
+                u32* wa = workspace;
+                u32* wb = workspace + L * count;
+                pass_geom gc = geom_for(T.log_n, false);
+                pass_geom gr = geom_for(T.log_n, true);
+                unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
+                unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
+                k_polymul_cols<L, FF, E><<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_em, gc, n, P);
+                k_polymul<L, FF, E, false><<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_em, T.inv_em, gr, n, P);
+                k_pass<L, FF, E, 3><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_em, gc, n, P);
+                return hipGetLastError();
+            });
+        }
+    }
+}

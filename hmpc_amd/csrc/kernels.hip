@@ -132,6 +132,27 @@ namespace hmpc_amd
         });
     }
 
+    template<int L>
+    __global__ void __launch_bounds__(block_threads) k_to_element_major(u32 const* __restrict__ planar, u32* __restrict__ em,
+                                                                        u64 count)
+    {
+        u64 e = u64{blockIdx.x} * blockDim.x + threadIdx.x;
+        if (e >= count)
+            return;
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+            em[e * L + l] = planar[l * count + e];
+    }
+
+    hipError_t launch_to_element_major(int L, u32 const* planar, u32* em, std::size_t count, hipStream_t s)
+    {
+        return with_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            k_to_element_major<LL><<<grid_for(count), block_threads, 0, s>>>(planar, em, count);
+            return hipGetLastError();
+        });
+    }
+
     // ------------------------------------------------------------------ plain path (hmpc's launch structure)
     // K2 / K5: first stage, [n][N] -> transposed scratch [N][n]
     template<int L, bool Inv>

@@ -17,6 +17,11 @@ namespace hmpc_amd
         int log_n;
         const u32* fwd;        // [L][N] forward table (hmpc's roots, device)
         const u32* inv;        // [L][N] inverse table
+        const u32* fwd_em;     // [N][L] element-major copies for vector loads
+        const u32* inv_em;
+        u32 q2[max_limbs];     // 2q
+        int form;              // arith_form of the fast path
+        int elems;             // fast path: elements per thread (4, 8 or 16)
     };
 
     // Decomposition N = N1 * N2 used by the two-pass fused transforms.
@@ -30,6 +35,16 @@ namespace hmpc_amd
     };
 
     split_plan plan_for(int limbs, int log_n, int tiles_in_lds);
+
+    hipError_t launch_to_element_major(int L, u32 const* planar, u32* em, std::size_t count, hipStream_t s);
+
+    // fast path (L = 2, 4): register-resident radix-16 rounds, see kernels_fast.hip
+    bool fast_path_supported(int L, int form, int log_n);
+    hipError_t launch_fast_ntt(int L, transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                               hipStream_t s);
+    std::size_t fast_polymul_workspace_words(int L, int log_n, std::size_t n);
+    hipError_t launch_fast_polymul(int L, transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                   u32* workspace, hipStream_t s);
 
     hipError_t launch_make_twiddles(int L, u32* table, int log_n, u32 const* root, u32 const* norm, u32 const* one,
                                     bool inverse, mod_params const& P, hipStream_t s);

@@ -1,0 +1,49 @@
+// Dispatch of the fast transform path by limb count (instantiations in kernels_fast_l2.hip
+// and kernels_fast_l4.hip, device code in fast_impl.hpp).
+#include "kernels.hpp"
+#include "fastmath.hpp"
+
+namespace hmpc_amd
+{
+    hipError_t launch_fast_ntt_l2(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                                  hipStream_t s);
+    hipError_t launch_fast_ntt_l4(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                                  hipStream_t s);
+    hipError_t launch_fast_polymul_l2(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                      u32* workspace, hipStream_t s);
+    hipError_t launch_fast_polymul_l4(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                      u32* workspace, hipStream_t s);
+
+    bool fast_path_supported(int L, int form, int log_n)
+    {
+        // special-form moduli only; whole-polynomial tiles up to 2^12, two passes of at most
+        // 2^12 points up to 2^20
+        bool special = (L == 4 && form == form_sparse) || (L == 2 && form == form_goldilocks);
+        return special && log_n >= 2 && log_n <= 20;
+    }
+
+    hipError_t launch_fast_ntt(int L, transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                               hipStream_t s)
+    {
+        if (L == 4)
+            return launch_fast_ntt_l4(T, n, inverse, in, out, s);
+        if (L == 2)
+            return launch_fast_ntt_l2(T, n, inverse, in, out, s);
+        return hipErrorInvalidValue;
+    }
+
+    std::size_t fast_polymul_workspace_words(int L, int log_n, std::size_t n)
+    {
+        return log_n <= 12 ? 0 : 2 * static_cast<std::size_t>(L) * (n << log_n);
+    }
+
+    hipError_t launch_fast_polymul(int L, transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                   u32* workspace, hipStream_t s)
+    {
+        if (L == 4)
+            return launch_fast_polymul_l4(T, n, a, b, out, workspace, s);
+        if (L == 2)
+            return launch_fast_polymul_l2(T, n, a, b, out, workspace, s);
+        return hipErrorInvalidValue;
+    }
+}

@@ -42,6 +42,9 @@ SIGNATURES = {
     "hmpc_modulus_destroy": (_i, [_vp]),
     "hmpc_modulus_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _u32p, _u32p, _u32p]),
     "hmpc_modulus_prepare": (_i, [_vp, _i]),
+    "hmpc_modulus_set_kernels": (_i, [_vp, _i]),
+    "hmpc_modulus_set_schedule": (_i, [_vp, _i, _i]),
+    "hmpc_modulus_get_schedule": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "hmpc_modulus_root": (_i, [_vp, _i, _i, _u32p]),
     "hmpc_modulus_twiddles": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
     "hmpc_ntt": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp]),
@@ -306,6 +309,18 @@ class Modulus:
         r2 = np.zeros(self.limbs, np.uint32)
         check(_lib.hmpc_modulus_info(self.handle, None, None, _p(npr), _p(r1), _p(r2)))
         return int(npr[0]), words_to_int(r1), words_to_int(r2)
+
+    def set_kernels(self, variant: int):
+        """0: fast register-radix kernels where supported, 1: portable LDS radix-2 kernels."""
+        check(_lib.hmpc_modulus_set_kernels(self.handle, variant))
+
+    def set_schedule(self, log_n: int, elems_per_thread: int):
+        check(_lib.hmpc_modulus_set_schedule(self.handle, log_n, elems_per_thread))
+
+    def schedule(self, log_n: int):
+        e, f = _i(), _i()
+        check(_lib.hmpc_modulus_get_schedule(self.handle, log_n, ctypes.byref(e), ctypes.byref(f)))
+        return {"elems_per_thread": e.value, "fast": bool(f.value)}
 
     def prepare(self, log_n: int):
         check(_lib.hmpc_modulus_prepare(self.handle, log_n))

@@ -90,6 +90,15 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
 int hmpc_modulus_destroy(hmpc_modulus mod);
 /* limbs L, bit size of q - 1, and constants (each array L words, may be NULL) */
 int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* nprime, uint32_t* r_mod_q, uint32_t* r2_mod_q);
+/* Kernel family for this modulus: 0 (default) = register-resident radix-16 rounds where
+ * supported (L = 2 or 4; Goldilocks / sparse / lazy reduction chosen from q), else the
+ * portable LDS radix-2 kernels; 1 = always the portable kernels. Results are identical. */
+int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
+/* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 limbs, 8 or
+ * 16 for 2 limbs; 0 restores the default). Chosen by the autotuner; results are identical. */
+int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread);
+/* Current schedule and whether the fast kernels serve this (modulus, N) at all. */
+int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* fast);
 /* Build (once) and keep resident the forward/inverse tables for N = 2^log_n (2 <= log_n <= 20),
  * 2N must divide q - 1. Blocking. */
 int hmpc_modulus_prepare(hmpc_modulus mod, int log_n);

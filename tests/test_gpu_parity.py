@@ -16,6 +16,11 @@ from bigref import P150, Q64, Q128A, Q128B
 pytestmark = pytest.mark.gpu
 
 Q279 = 0x591F5B834C0D961F67343BCC8902BDEDA2771F54306FF151162FF8D2B40F4194DC0001  # tests/expr/crypto/lhe/enc.cpp:15
+# primes = 1 mod 2^14 that select each arithmetic form of the fast kernels
+Q_GENERIC4 = 0x80CD613ED8F16ADF91B7584A226B8001  # 128-bit: no 4q < R headroom
+Q_LAZY4 = 0x201E2FEB414C343C1027C4D1C3924001     # 126-bit, dense limbs
+Q_LAZY2 = 0x20030B397ED50001                     # 62-bit
+Q_GENERIC2 = 0xC78E5106731EC001                  # 64-bit, not Goldilocks
 KEY = [42, 0, 0, 0, 0, 0, 0, 0]
 
 
@@ -34,23 +39,31 @@ def ctx():
 _mods = {}
 
 
-def mods(ctx, q):
-    if q not in _mods:
-        _mods[q] = (hmpc_amd.Modulus(ctx, q), oracle.Modulus(q))
-    return _mods[q]
+def mods(ctx, q, variant=0):
+    """(GPU modulus, oracle modulus); variant 0 = fast kernels, 1 = portable kernels."""
+    if (q, variant) not in _mods:
+        gm = hmpc_amd.Modulus(ctx, q)
+        gm.set_kernels(variant)
+        _mods[(q, variant)] = (gm, oracle.Modulus(q))
+    return _mods[(q, variant)]
 
 
 NTT_CASES = [
-    (Q64, 2, 3), (Q64, 3, 2), (Q64, 5, 4), (Q64, 12, 3), (Q64, 13, 2), (Q64, 14, 2), (Q64, 16, 1),
-    (Q128A, 4, 3), (Q128A, 12, 2), (Q128A, 13, 2), (Q128A, 16, 2), (Q128B, 14, 3),
+    (Q64, 2, 3), (Q64, 3, 2), (Q64, 5, 4), (Q64, 9, 13), (Q64, 12, 3), (Q64, 13, 2), (Q64, 14, 2), (Q64, 15, 1),
+    (Q64, 16, 1), (Q64, 17, 1),
+    (Q128A, 2, 1), (Q128A, 4, 3), (Q128A, 7, 33), (Q128A, 12, 2), (Q128A, 13, 2), (Q128A, 15, 1), (Q128A, 16, 2),
+    (Q128A, 17, 1), (Q128B, 14, 3),
+    (Q_GENERIC4, 12, 2), (Q_GENERIC4, 14, 2), (Q_LAZY4, 12, 2), (Q_LAZY4, 13, 3), (Q_LAZY2, 11, 3), (Q_LAZY2, 14, 1),
+    (Q_GENERIC2, 12, 2), (Q_GENERIC2, 13, 1),
     (P150, 11, 2), (Q279, 10, 2),
 ]
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("q,log_n,n", NTT_CASES, ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
-def test_ntt_matches_oracle(ctx, q, log_n, n):
-    gm, om = mods(ctx, q)
-    say(f"ntt q={hex(q)[:12]} L={gm.limbs} log_n={log_n} n={n}")
+def test_ntt_matches_oracle(ctx, q, log_n, n, variant):
+    gm, om = mods(ctx, q, variant)
+    say(f"ntt q={hex(q)[:12]} L={gm.limbs} log_n={log_n} n={n} variant={variant}")
     count = n << log_n
     x = om.sample_uniform(KEY, 0, count, threads=8)
     want_f = om.ntt(x, log_n, threads=8)
@@ -99,13 +112,15 @@ def test_poly_mod_kat_on_gpu(ctx):
 
 
 POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 16, 2),
-                 (Q128B, 14, 2), (P150, 9, 2)]
+                 (Q128A, 5, 7), (Q128B, 14, 2), (Q128B, 13, 3), (Q_GENERIC4, 12, 2), (Q_GENERIC4, 14, 1),
+                 (Q_LAZY4, 13, 2), (Q_LAZY2, 12, 2), (Q_GENERIC2, 13, 2), (P150, 9, 2)]
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("q,log_n,n", POLYMUL_CASES, ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
-def test_polymul_matches_oracle(ctx, q, log_n, n):
-    gm, om = mods(ctx, q)
-    say(f"polymul q={hex(q)[:12]} log_n={log_n} n={n}")
+def test_polymul_matches_oracle(ctx, q, log_n, n, variant):
+    gm, om = mods(ctx, q, variant)
+    say(f"polymul q={hex(q)[:12]} log_n={log_n} n={n} variant={variant}")
     count = n << log_n
     a = om.sample_uniform(KEY, 0, count, threads=8)
     b = om.sample_uniform(KEY, 1, count, threads=8)
@@ -217,7 +232,7 @@ def _edge_values(q, limbs, count, seed):
     return edges + vals
 
 
-@pytest.mark.parametrize("q", [Q64, Q128A, Q128B, P150, Q279, 1601, 0x8822D80623320001])
+@pytest.mark.parametrize("q", [Q64, Q128A, Q128B, P150, Q279, 1601, 0x8822D80623320001, Q_GENERIC4, Q_LAZY2])
 def test_pointwise_ops_edge_values(ctx, q):
     gm, om = mods(ctx, q)
     say(f"pointwise edges q={hex(q)[:12]}")
@@ -233,10 +248,36 @@ def test_pointwise_ops_edge_values(ctx, q):
         got = gm.download(gm.pointwise(op, count, da, db), count)
         assert np.array_equal(got, om.pointwise(name, a, b)), name
     # broadcast of a short operand (unsqueeze) and scalar constants
-    got = gm.download(gm.pointwise(gm.MUL, count, da, db, b_count=k), count)
-    assert np.array_equal(got, om.pointwise("mul", a, np.ascontiguousarray(b[:, :k]))), "broadcast"
+    bk = np.ascontiguousarray(b[:, :k])
+    got = gm.download(gm.pointwise(gm.MUL, count, da, gm.upload(bk), b_count=k), count)
+    assert np.array_equal(got, om.pointwise("mul", a, bk)), "broadcast"
     for c in (0, 1, q - 1, a_vals[5000]):
         cw = oracle.int_to_limbs(c, gm.limbs)
         got = gm.download(gm.pointwise_scalar(gm.MUL, count, da, cw), count)
         want = om.pointwise("mul", a, oracle.ints_to_planar([c], gm.limbs))
         assert np.array_equal(got, want), hex(c)
+
+
+FAST_CASES = [(Q128A, 4, 5), (Q128A, 12, 3), (Q128A, 13, 2), (Q128A, 16, 2), (Q128B, 14, 2), (Q64, 3, 9), (Q64, 12, 3),
+              (Q64, 14, 2), (Q64, 16, 1)]
+
+
+@pytest.mark.parametrize("q,log_n,n", FAST_CASES, ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
+def test_fast_kernel_schedules(ctx, q, log_n, n):
+    """Every elements-per-thread schedule of the fast kernels gives hmpc's results."""
+    gm, om = mods(ctx, q)
+    assert gm.schedule(log_n)["fast"]
+    count = n << log_n
+    a = om.sample_uniform(KEY, 0, count, threads=8)
+    b = om.sample_uniform(KEY, 1, count, threads=8)
+    want_f = om.ntt(a, log_n, threads=8)
+    want_i = om.ntt(a, log_n, inverse=True, threads=8)
+    want_p, _ = om.polymul_timed(log_n, a, b, threads=8)
+    da, db = gm.upload(a), gm.upload(b)
+    for elems in ((4, 8) if gm.limbs == 4 else (8, 16)):
+        say(f"fast schedule q={hex(q)[:12]} log_n={log_n} n={n} elems={elems}")
+        gm.set_schedule(log_n, elems)
+        assert np.array_equal(gm.download(gm.ntt(log_n, n, da), count), want_f), f"forward elems={elems}"
+        assert np.array_equal(gm.download(gm.ntt(log_n, n, da, inverse=True), count), want_i), f"inverse elems={elems}"
+        assert np.array_equal(gm.download(gm.polymul(log_n, n, da, db), count), want_p), f"polymul elems={elems}"
+    gm.set_schedule(log_n, 0)
