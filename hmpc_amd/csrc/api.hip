@@ -158,6 +158,9 @@ struct table_pair
     u32* inv = nullptr;
     u32* fwd_em = nullptr;
     u32* inv_em = nullptr;
+    u32* fwd_fast = nullptr;  // r26 tables (owned) or aliases of the em tables
+    u32* inv_fast = nullptr;
+    bool fast_owned = false;
     u32 root[max_limbs] = {};
     u32 inverse_root[max_limbs] = {};
 };
@@ -173,6 +176,7 @@ struct hmpc_modulus_s
     u32 q2[max_limbs] = {};
     int form = form_generic;  // arithmetic form of the fast path
     int variant = 0;          // 0: fast path where supported, 1: portable LDS radix-2 kernels
+    p26 r26{};                // constants of the radix-2^26 form
     std::map<int, table_pair> tables;
     std::map<int, int> elems; // fast path: elements per thread chosen per log_n (tuner)
 
@@ -181,9 +185,15 @@ struct hmpc_modulus_s
     ~hmpc_modulus_s()
     {
         for (auto& [k, t] : tables)
+        {
             for (u32* p : {t.fwd, t.inv, t.fwd_em, t.inv_em})
                 if (p)
                     (void)hipFree(p);
+            if (t.fast_owned)
+                for (u32* p : {t.fwd_fast, t.inv_fast})
+                    if (p)
+                        (void)hipFree(p);
+        }
     }
 
     bool fast(int log_n) const { return variant == 0 && fast_path_supported(limbs, form, log_n); }
@@ -204,6 +214,9 @@ struct hmpc_modulus_s
         T.inv = t.inv;
         T.fwd_em = t.fwd_em;
         T.inv_em = t.inv_em;
+        T.fwd_fast = t.fwd_fast;
+        T.inv_fast = t.inv_fast;
+        T.r26 = r26;
         std::copy(q2, q2 + max_limbs, T.q2);
         T.form = form;
         auto e = elems.find(log_n);
@@ -354,6 +367,21 @@ int hmpc_modulus_s::prepare(int log_n)
             return false;
         if (launch_to_element_major(LL, t.inv, t.inv_em, std::size_t{1} << log_n, nullptr) != hipSuccess)
             return false;
+        if (form == form_r26)
+        {
+            std::size_t fb = (std::size_t{4} * 8) << log_n;
+            if (hipMalloc(&t.fwd_fast, fb) != hipSuccess || hipMalloc(&t.inv_fast, fb) != hipSuccess)
+                return false;
+            t.fast_owned = true;
+            if (launch_r26_table(t.fwd, t.fwd_fast, std::size_t{1} << log_n, r26, nullptr) != hipSuccess ||
+                launch_r26_table(t.inv, t.inv_fast, std::size_t{1} << log_n, r26, nullptr) != hipSuccess)
+                return false;
+        }
+        else
+        {
+            t.fwd_fast = t.fwd_em;
+            t.inv_fast = t.inv_em;
+        }
         return hipDeviceSynchronize() == hipSuccess;
     });
     if (!ok)
@@ -361,6 +389,10 @@ int hmpc_modulus_s::prepare(int log_n)
         for (u32* p : {t.fwd, t.inv, t.fwd_em, t.inv_em})
             if (p)
                 (void)hipFree(p);
+        if (t.fast_owned)
+            for (u32* p : {t.fwd_fast, t.inv_fast})
+                if (p)
+                    (void)hipFree(p);
         return fail(HMPC_ERR_INVALID_MODULUS, "no primitive 2N-th root of unity for this modulus (2N must divide q - 1)");
     }
     tables[log_n] = t;
@@ -580,6 +612,22 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
         middle_zero &= m->P.q[i] == 0;
     if (L == 2 && m->P.q[0] == 1u && m->P.q[1] == 0xffffffffu)
         m->form = form_goldilocks;
+    else if (L == 4 && m->P.q[3] == 0x10000000u && m->P.q[2] == 0 && m->P.q[1] == 0 && m->P.q[0] < (1u << 24))
+    {
+        // q = 2^124 + c: radix-2^26 constants (see r26.hpp for the bounds they satisfy)
+        m->form = form_r26;
+        u32 c = m->P.q[0];
+        p26& R = m->r26;
+        R.c = c;
+        R.n26 = m->P.nprime & m26;
+        u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
+        u32 const k32[5] = {32 * c + (1u << 29), (1u << 29) - 8, (1u << 29) - 8, (1u << 29) - 8, (1u << 25) - 8};
+        u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
+        std::copy(k4, k4 + 5, R.k4);
+        std::copy(k32, k32 + 5, R.k32);
+        std::copy(qb, qb + 5, R.qb);
+        std::copy(m->P.q, m->P.q + 4, R.q32);
+    }
     else if (lazy_ok && middle_zero)
         m->form = form_sparse;
     else if (lazy_ok)

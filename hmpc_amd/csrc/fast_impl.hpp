@@ -1,14 +1,15 @@
-// Fast transform path for L = 2 and L = 4 limbs (the 64-bit and 125-bit moduli of the
-// workloads), gfx950. Included by kernels_fast_l2.hip and kernels_fast_l4.hip, which
-// instantiate it for one limb count each (separate translation units build in parallel).
+// Fast transform path for the special-form moduli of the workloads, gfx950:
+//   arith_r26   q = 2^124 + c (4 limbs in memory), radix-2^26 redundant arithmetic (r26.hpp)
+//   arith_gold  q = 2^64 - 2^32 + 1 (2 limbs), Goldilocks Montgomery reduction (fastmath.hpp)
+// Included by kernels_fast_l2.hip and kernels_fast_l4.hip, which instantiate one policy each
+// (separate translation units build in parallel).
 //
-// A workgroup owns a 4096-element tile in LDS (element-major, one ds_read_b128 / ds_read_b64
-// per element, XOR-swizzled so that strided lanes hit distinct banks). Each of its 4096 / E
-// threads keeps E elements (E = 4, 8 or 16, a tuning knob) in VGPRs and runs up to log2(E)
-// radix-2 stages on them between two LDS exchanges ("rounds"), so an S-point sub-transform
-// costs ceil(log S / log E) exchanges instead of log S. At the start of a round every thread
-// issues the loads of all twiddles the round needs (element-major copies of hmpc's tables,
-// one global_load_dwordx4 per twiddle), so their latencies overlap.
+// A workgroup owns a 4096-element tile in LDS (element-major, XOR-swizzled so that strided
+// lanes hit distinct banks). Each of its 4096 / E threads keeps E elements (E = 4, 8 or 16, a
+// tuning knob) in VGPRs and runs up to log2(E) radix-2 stages on them between two LDS
+// exchanges ("rounds"), so an S-point sub-transform costs ceil(log S / log E) exchanges instead
+// of log S. At the start of a round every thread issues the loads of all twiddles the round
+// needs (element-major tables in the policy's format), so their latencies overlap.
 //
 // Round geometry (s = log S, round = stages [a, b), w = b - a). A "unit" is the group of
 // 2^w elements that interact in the round; a thread handles E / 2^w units.
@@ -24,6 +25,7 @@
 
 #include "kernels.hpp"
 #include "fastmath.hpp"
+#include "r26.hpp"
 
 #include <type_traits>
 
@@ -36,6 +38,193 @@ namespace hmpc_amd
 
         constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
+        // ------------------------------------------------------------------ arithmetic policies
+        // r26: registers and LDS hold 5 limbs of 26 bits per element; global memory keeps hmpc's
+        // 4 x 32-bit limb-planar layout (converted once on load and once on store per kernel).
+        struct arith_r26
+        {
+            static constexpr int rl = 5;    // limbs per element in registers and LDS
+            static constexpr int iol = 4;   // limbs per element in global memory (hmpc layout)
+            static constexpr int tws = 8;   // twiddle table stride in words (5 used)
+            using params = p26;
+
+            __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
+                                                               std::size_t g)
+            {
+                r26_unpack(x, src[g], src[count + g], src[2 * count + g], src[3 * count + g]);
+            }
+
+            // canonical: final output; otherwise a lazy value < 2q for the next pass
+            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, std::size_t g,
+                                                                u32 const (&xin)[rl], bool canonical, params const& P)
+            {
+                u32 x[rl] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
+                r26_fold(x, P);
+                u32 w[4];
+                if (canonical)
+                    r26_canonical(w, x, P);
+                else
+                {
+                    r26_normalize(x);
+                    r26_pack(w, x);
+                }
+#pragma unroll
+                for (int l = 0; l < 4; ++l)
+                    dst[l * count + g] = w[l];
+            }
+
+            __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 4) & 15); }
+
+            __device__ __forceinline__ static void lds_get(u32 (&x)[rl], u32 const* lds, int e)
+            {
+                int p = phys(e) * rl;
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    x[l] = lds[p + l];
+            }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl])
+            {
+                int p = phys(e) * rl;
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    lds[p + l] = x[l];
+            }
+
+            __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
+            {
+                uint4 v = reinterpret_cast<uint4 const*>(tw)[2 * idx];
+                w[0] = v.x;
+                w[1] = v.y;
+                w[2] = v.z;
+                w[3] = v.w;
+                w[4] = tw[8 * static_cast<std::size_t>(idx) + 4];
+            }
+
+            // (x, y) -> (x + t, x - t + 4q), t = mont(y, w)
+            __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            {
+                u32 t[rl];
+                r26_mont(t, y, w, P);
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                {
+                    u32 xl = x[l];
+                    x[l] = xl + t[l];
+                    y[l] = xl + (P.k4[l] - t[l]);
+                }
+            }
+
+            // (x, y) -> (fold(x + y), mont(x - y + 32q, w))
+            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            {
+                u32 d[rl];
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                {
+                    d[l] = x[l] + (P.k32[l] - y[l]);
+                    x[l] = x[l] + y[l];
+                }
+                r26_fold(x, P);
+                r26_mont(y, d, w, P);
+            }
+
+            __device__ __forceinline__ static void inv_last(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w0)[rl],
+                                                            u32 const (&w1)[rl], params const& P)
+            {
+                u32 s[rl], d[rl];
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                {
+                    d[l] = x[l] + (P.k32[l] - y[l]);
+                    s[l] = x[l] + y[l];
+                }
+                r26_mont(x, s, w0, P);
+                r26_mont(y, d, w1, P);
+            }
+
+            // point-wise product of two data values a R, b R -> a b R (R' / R = 4 correction)
+            __device__ __forceinline__ static void point(u32 (&a)[rl], u32 (&b)[rl], params const& P)
+            {
+                r26_fold(a, P);
+                r26_fold(b, P);
+                u32 t[rl];
+                r26_mont(t, a, b, P);
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    a[l] = t[l] << 2;
+            }
+        };
+
+        struct arith_gold
+        {
+            static constexpr int rl = 2;
+            static constexpr int iol = 2;
+            static constexpr int tws = 2;
+            using params = fast_params;
+            using ops = fast_ops<2, form_goldilocks>;
+
+            __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
+                                                               std::size_t g)
+            {
+                x[0] = src[g];
+                x[1] = src[count + g];
+            }
+
+            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, std::size_t g,
+                                                                u32 const (&x)[rl], bool, params const&)
+            {
+                dst[g] = x[0];
+                dst[count + g] = x[1];
+            }
+
+            __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 4) & 15); }
+
+            __device__ __forceinline__ static void lds_get(u32 (&x)[rl], u32 const* lds, int e)
+            {
+                uint2 v = reinterpret_cast<uint2 const*>(lds)[phys(e)];
+                x[0] = v.x;
+                x[1] = v.y;
+            }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl])
+            {
+                reinterpret_cast<uint2*>(lds)[phys(e)] = make_uint2(x[0], x[1]);
+            }
+
+            __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
+            {
+                uint2 v = reinterpret_cast<uint2 const*>(tw)[idx];
+                w[0] = v.x;
+                w[1] = v.y;
+            }
+
+            __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            {
+                ops::fwd(x, y, w, P);
+            }
+
+            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            {
+                ops::inv(x, y, w, P);
+            }
+
+            __device__ __forceinline__ static void inv_last(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w0)[rl],
+                                                            u32 const (&w1)[rl], params const& P)
+            {
+                ops::inv_last(x, y, w0, w1, P);
+            }
+
+            __device__ __forceinline__ static void point(u32 (&a)[rl], u32 (&b)[rl], params const& P)
+            {
+                u32 t[rl];
+                ops::mul(t, a, b, P);
+                a[0] = t[0];
+                a[1] = t[1];
+            }
+        };
+
+        // ------------------------------------------------------------------ tile geometry
         struct tile_desc
         {
             int s;         // log2 of the sub-transform length
@@ -88,64 +277,6 @@ namespace hmpc_amd
             return td.rows ? static_cast<int>((c << td.s) + j) : static_cast<int>((j << td.log_seqs) + c);
         }
 
-        template<int L>
-        __device__ __forceinline__ int phys(int e)
-        {
-            constexpr int M = (32 / L) < 16 ? (32 / L) : 16;
-            return e ^ ((e >> 4) & (M - 1));
-        }
-
-        template<int L>
-        __device__ __forceinline__ void lds_get(u32 (&x)[L], u32 const* lds, int e)
-        {
-            int p = phys<L>(e);
-            if constexpr (L == 4)
-            {
-                uint4 v = reinterpret_cast<uint4 const*>(lds)[p];
-                x[0] = v.x;
-                x[1] = v.y;
-                x[2] = v.z;
-                x[3] = v.w;
-            }
-            else
-            {
-                static_assert(L == 2);
-                uint2 v = reinterpret_cast<uint2 const*>(lds)[p];
-                x[0] = v.x;
-                x[1] = v.y;
-            }
-        }
-
-        template<int L>
-        __device__ __forceinline__ void lds_put(u32* lds, int e, u32 const (&x)[L])
-        {
-            int p = phys<L>(e);
-            if constexpr (L == 4)
-                reinterpret_cast<uint4*>(lds)[p] = make_uint4(x[0], x[1], x[2], x[3]);
-            else
-                reinterpret_cast<uint2*>(lds)[p] = make_uint2(x[0], x[1]);
-        }
-
-        // twiddle from an element-major table [N][L]
-        template<int L>
-        __device__ __forceinline__ void tw_get(u32 (&w)[L], u32 const* __restrict__ tw, u32 idx)
-        {
-            if constexpr (L == 4)
-            {
-                uint4 v = reinterpret_cast<uint4 const*>(tw)[idx];
-                w[0] = v.x;
-                w[1] = v.y;
-                w[2] = v.z;
-                w[3] = v.w;
-            }
-            else
-            {
-                uint2 v = reinterpret_cast<uint2 const*>(tw)[idx];
-                w[0] = v.x;
-                w[1] = v.y;
-            }
-        }
-
         // round schedule: forward full rounds first, inverse partial round first
         template<int WMAX>
         __device__ __forceinline__ int round_count(int s) { return (s + WMAX - 1) / WMAX; }
@@ -172,13 +303,12 @@ namespace hmpc_amd
         template<int E>
         struct shape
         {
-            static constexpr int elems = E;
             static constexpr int threads = tile / E;
             static constexpr int wmax = ilog2(E);
         };
 
-        template<int L, int E, int W>
-        __device__ __forceinline__ void gather(u32 (&x)[E][L], u32 const* lds, tile_desc const& td, int a, bool inv)
+        template<typename A, int E, int W>
+        __device__ __forceinline__ void gather(u32 (&x)[E][A::rl], u32 const* lds, tile_desc const& td, int a, bool inv)
         {
             constexpr int K = 1 << W;
 #pragma unroll
@@ -187,12 +317,12 @@ namespace hmpc_amd
                 unit_t un = unit_of(td, a, W, inv, threadIdx.x + shape<E>::threads * g);
 #pragma unroll
                 for (int k = 0; k < K; ++k)
-                    lds_get<L>(x[g * K + k], lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)));
+                    A::lds_get(x[g * K + k], lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)));
             }
         }
 
-        template<int L, int E, int W>
-        __device__ __forceinline__ void scatter(u32 (&x)[E][L], u32* lds, tile_desc const& td, int a, bool inv)
+        template<typename A, int E, int W>
+        __device__ __forceinline__ void scatter(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a, bool inv)
         {
             constexpr int K = 1 << W;
 #pragma unroll
@@ -201,7 +331,7 @@ namespace hmpc_amd
                 unit_t un = unit_of(td, a, W, inv, threadIdx.x + shape<E>::threads * g);
 #pragma unroll
                 for (int k = 0; k < K; ++k)
-                    lds_put<L>(lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)), x[g * K + k]);
+                    A::lds_put(lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)), x[g * K + k]);
             }
         }
 
@@ -214,12 +344,13 @@ namespace hmpc_amd
             return inv ? ((1 << W) - (1 << (W - ii)) + (k >> (ii + 1))) : ((1 << ii) - 1 + (k >> (W - ii)));
         }
 
-        template<int L, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void prefetch_stage(u32 (&tws)[E >> W][(1 << W) - 1][L], tile_desc const& td, int a,
-                                                       u32 base0, u32 base_step, u32 const* __restrict__ tw)
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void prefetch_stage(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], tile_desc const& td,
+                                                       int a, u32 base0, u32 base_step, u32 const* __restrict__ tw)
         {
             constexpr int K = 1 << W;
             constexpr int count = Inv ? (1 << (W - II - 1)) : (1 << II);
+            constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
             int const i = a + II;
             u32 const h = 1u << (td.s - i - 1);
 #pragma unroll
@@ -232,17 +363,26 @@ namespace hmpc_amd
                 {
                     u32 idx = Inv ? base * h + ((un.F << (W - II - 1)) | static_cast<u32>(m))
                                   : (base << i) + ((un.F << II) | static_cast<u32>(m));
-                    constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
-                    tw_get<L>(tws[g][first + m], tw, idx);
+                    A::tw_get(tw_regs[g][first + m], tw, idx);
                 }
             }
         }
 
-        template<int L, int Form, int E, bool Inv, int W, int II, bool Last>
-        __device__ __forceinline__ void stage(u32 (&x)[E][L], u32 const (&tws)[E >> W][(1 << W) - 1][L],
-                                              u32 const (&w0)[L], u32 const (&w1)[L], fast_params const& P)
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void prefetch_upto(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], tile_desc const& td,
+                                                      int a, u32 base0, u32 base_step, u32 const* __restrict__ tw)
         {
-            using ops = fast_ops<L, Form>;
+            if constexpr (II < W - 1)
+            {
+                prefetch_stage<A, E, Inv, W, II>(tw_regs, td, a, base0, base_step, tw);
+                prefetch_upto<A, E, Inv, W, II + 1>(tw_regs, td, a, base0, base_step, tw);
+            }
+        }
+
+        template<typename A, int E, bool Inv, int W, int II, bool Last>
+        __device__ __forceinline__ void stage(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
+                                              u32 const (&w0)[A::rl], u32 const (&w1)[A::rl], typename A::params const& P)
+        {
             constexpr int K = 1 << W;
             constexpr int lb = Inv ? II : (W - 1 - II);
 #pragma unroll
@@ -255,90 +395,81 @@ namespace hmpc_amd
                         continue;
                     int const slot = tw_slot<W>(Inv, II, k);
                     if constexpr (!Inv)
-                        ops::fwd(x[g * K + k], x[g * K + (k | (1 << lb))], tws[g][slot], P);
+                        A::fwd(x[g * K + k], x[g * K + (k | (1 << lb))], tw_regs[g][slot], P);
                     else if constexpr (Last)
-                        ops::inv_last(x[g * K + k], x[g * K + (k | (1 << lb))], w0, w1, P);
+                        A::inv_last(x[g * K + k], x[g * K + (k | (1 << lb))], w0, w1, P);
                     else
-                        ops::inv(x[g * K + k], x[g * K + (k | (1 << lb))], tws[g][slot], P);
+                        A::inv(x[g * K + k], x[g * K + (k | (1 << lb))], tw_regs[g][slot], P);
                 }
             }
         }
 
-        template<int L, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void prefetch_upto(u32 (&tws)[E >> W][(1 << W) - 1][L], tile_desc const& td, int a,
-                                                      u32 base0, u32 base_step, u32 const* __restrict__ tw)
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void stages_upto(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
+                                                    u32 const (&w0)[A::rl], u32 const (&w1)[A::rl],
+                                                    typename A::params const& P)
         {
             if constexpr (II < W - 1)
             {
-                prefetch_stage<L, E, Inv, W, II>(tws, td, a, base0, base_step, tw);
-                prefetch_upto<L, E, Inv, W, II + 1>(tws, td, a, base0, base_step, tw);
-            }
-        }
-
-        template<int L, int Form, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void stages_upto(u32 (&x)[E][L], u32 const (&tws)[E >> W][(1 << W) - 1][L],
-                                                    u32 const (&w0)[L], u32 const (&w1)[L], fast_params const& P)
-        {
-            if constexpr (II < W - 1)
-            {
-                stage<L, Form, E, Inv, W, II, false>(x, tws, w0, w1, P);
-                stages_upto<L, Form, E, Inv, W, II + 1>(x, tws, w0, w1, P);
+                stage<A, E, Inv, W, II, false>(x, tw_regs, w0, w1, P);
+                stages_upto<A, E, Inv, W, II + 1>(x, tw_regs, w0, w1, P);
             }
         }
 
         // butterflies of one round: all twiddle loads first, then the stages. The round's last
         // stage is the inverse's scaled final stage when the round ends the transform.
-        template<int L, int Form, int E, bool Inv, int W>
-        __device__ __forceinline__ void compute_round(u32 (&x)[E][L], tile_desc const& td, int a, u32 base0, u32 base_step,
-                                                      u32 const* __restrict__ tw, fast_params const& P, bool final_scale)
+        template<typename A, int E, bool Inv, int W>
+        __device__ __forceinline__ void compute_round(u32 (&x)[E][A::rl], tile_desc const& td, int a, u32 base0,
+                                                      u32 base_step, u32 const* __restrict__ tw,
+                                                      typename A::params const& P, bool final_scale)
         {
-            u32 tws[E >> W][(1 << W) - 1][L];
+            u32 tw_regs[E >> W][(1 << W) - 1][A::rl];
             bool const scaled = Inv && final_scale && (a + W == td.s);
-            prefetch_upto<L, E, Inv, W, 0>(tws, td, a, base0, base_step, tw);
-            u32 w0[L] = {}, w1[L] = {};
+            prefetch_upto<A, E, Inv, W, 0>(tw_regs, td, a, base0, base_step, tw);
+            u32 w0[A::rl] = {}, w1[A::rl] = {};
             if (scaled)
             {
-                tw_get<L>(w0, tw, 0);
-                tw_get<L>(w1, tw, 1);
+                A::tw_get(w0, tw, 0);
+                A::tw_get(w1, tw, 1);
             }
             else
-                prefetch_stage<L, E, Inv, W, W - 1>(tws, td, a, base0, base_step, tw);
-            stages_upto<L, Form, E, Inv, W, 0>(x, tws, w0, w1, P);
+                prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, td, a, base0, base_step, tw);
+            stages_upto<A, E, Inv, W, 0>(x, tw_regs, w0, w1, P);
             if (scaled)
-                stage<L, Form, E, Inv, W, W - 1, true>(x, tws, w0, w1, P);
+                stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P);
             else
-                stage<L, Form, E, Inv, W, W - 1, false>(x, tws, w0, w1, P);
+                stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P);
         }
 
-        template<int L, int Form, int E, bool Inv, int W>
-        __device__ __forceinline__ void one_round(u32 (&x)[E][L], u32* lds, tile_desc const& td, int a, u32 base0,
-                                                  u32 base_step, u32 const* __restrict__ tw, fast_params const& P,
+        template<typename A, int E, bool Inv, int W>
+        __device__ __forceinline__ void one_round(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a, u32 base0,
+                                                  u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
                                                   bool final_scale, bool keep)
         {
-            gather<L, E, W>(x, lds, td, a, Inv);
-            compute_round<L, Form, E, Inv, W>(x, td, a, base0, base_step, tw, P, final_scale);
+            gather<A, E, W>(x, lds, td, a, Inv);
+            compute_round<A, E, Inv, W>(x, td, a, base0, base_step, tw, P, final_scale);
             if (!keep)
-                scatter<L, E, W>(x, lds, td, a, Inv);
+                scatter<A, E, W>(x, lds, td, a, Inv);
         }
 
-        template<int L, int Form, int E, bool Inv>
-        __device__ __forceinline__ void dispatch_round(int w, u32 (&x)[E][L], u32* lds, tile_desc const& td, int a, u32 base0,
-                                                       u32 base_step, u32 const* __restrict__ tw, fast_params const& P,
-                                                       bool final_scale, bool keep)
+        template<typename A, int E, bool Inv>
+        __device__ __forceinline__ void dispatch_round(int w, u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a,
+                                                       u32 base0, u32 base_step, u32 const* __restrict__ tw,
+                                                       typename A::params const& P, bool final_scale, bool keep)
         {
             constexpr int WM = shape<E>::wmax;
             if (w == WM)
-                one_round<L, Form, E, Inv, WM>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                one_round<A, E, Inv, WM>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
             else if (WM > 1 && w == 1)
-                one_round<L, Form, E, Inv, 1>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                one_round<A, E, Inv, 1>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
             else if constexpr (WM > 2)
             {
                 if (w == 2)
-                    one_round<L, Form, E, Inv, 2>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                    one_round<A, E, Inv, 2>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
                 else if constexpr (WM > 3)
                 {
                     if (w == 3)
-                        one_round<L, Form, E, Inv, 3>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                        one_round<A, E, Inv, 3>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
                 }
             }
         }
@@ -346,10 +477,10 @@ namespace hmpc_amd
         // All rounds of the tile's sub-transform, LDS -> LDS. With keep_last (forward only) the
         // last round's results stay in x instead of being written back. (Width dispatch by
         // explicit branches: a lambda over x would force the array into scratch memory.)
-        template<int L, int Form, int E, bool Inv>
-        __device__ __forceinline__ void tile_rounds(u32 (&x)[E][L], u32* lds, tile_desc const& td, u32 base0, u32 base_step,
-                                                    u32 const* __restrict__ tw, fast_params const& P, bool final_scale,
-                                                    int first_round, bool keep_last)
+        template<typename A, int E, bool Inv>
+        __device__ __forceinline__ void tile_rounds(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
+                                                    u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
+                                                    bool final_scale, int first_round, bool keep_last)
         {
             constexpr int WM = shape<E>::wmax;
             int n = round_count<WM>(td.s);
@@ -358,39 +489,40 @@ namespace hmpc_amd
                 int a = round_start<WM>(td.s, r, Inv);
                 int w = round_width<WM>(td.s, r, Inv);
                 bool keep = keep_last && r == n - 1;
-                dispatch_round<L, Form, E, Inv>(w, x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                dispatch_round<A, E, Inv>(w, x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
                 if (!keep)
                     __syncthreads();
             }
         }
 
         // inverse round 0 computed on registers that already hold its elements, then written to LDS
-        template<int L, int Form, int E, int W>
-        __device__ __forceinline__ void regs_round(u32 (&x)[E][L], u32* lds, tile_desc const& td, u32 base0, u32 base_step,
-                                                   u32 const* __restrict__ tw, fast_params const& P, bool final_scale)
+        template<typename A, int E, int W>
+        __device__ __forceinline__ void regs_round(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
+                                                   u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
+                                                   bool final_scale)
         {
-            compute_round<L, Form, E, true, W>(x, td, 0, base0, base_step, tw, P, final_scale);
-            scatter<L, E, W>(x, lds, td, 0, true);
+            compute_round<A, E, true, W>(x, td, 0, base0, base_step, tw, P, final_scale);
+            scatter<A, E, W>(x, lds, td, 0, true);
         }
 
-        template<int L, int Form, int E>
-        __device__ __forceinline__ void dispatch_round_regs(int w, u32 (&x)[E][L], u32* lds, tile_desc const& td, u32 base0,
-                                                            u32 base_step, u32 const* __restrict__ tw, fast_params const& P,
-                                                            bool final_scale)
+        template<typename A, int E>
+        __device__ __forceinline__ void dispatch_round_regs(int w, u32 (&x)[E][A::rl], u32* lds, tile_desc const& td,
+                                                            u32 base0, u32 base_step, u32 const* __restrict__ tw,
+                                                            typename A::params const& P, bool final_scale)
         {
             constexpr int WM = shape<E>::wmax;
             if (w == WM)
-                regs_round<L, Form, E, WM>(x, lds, td, base0, base_step, tw, P, final_scale);
+                regs_round<A, E, WM>(x, lds, td, base0, base_step, tw, P, final_scale);
             else if (WM > 1 && w == 1)
-                regs_round<L, Form, E, 1>(x, lds, td, base0, base_step, tw, P, final_scale);
+                regs_round<A, E, 1>(x, lds, td, base0, base_step, tw, P, final_scale);
             else if constexpr (WM > 2)
             {
                 if (w == 2)
-                    regs_round<L, Form, E, 2>(x, lds, td, base0, base_step, tw, P, final_scale);
+                    regs_round<A, E, 2>(x, lds, td, base0, base_step, tw, P, final_scale);
                 else if constexpr (WM > 3)
                 {
                     if (w == 3)
-                        regs_round<L, Form, E, 3>(x, lds, td, base0, base_step, tw, P, final_scale);
+                        regs_round<A, E, 3>(x, lds, td, base0, base_step, tw, P, final_scale);
                 }
             }
         }
@@ -414,46 +546,35 @@ namespace hmpc_amd
             return io.base + static_cast<std::size_t>(e >> io.log_seqs) * io.stride + (e & ((1 << io.log_seqs) - 1));
         }
 
-        template<int L, int T>
+        template<typename A, int T>
         __device__ __forceinline__ void tile_load(u32* lds, u32 const* __restrict__ src, io_desc const& io)
         {
 #pragma unroll 4
             for (int e = threadIdx.x; e < tile; e += T)
             {
-                u32 x[L];
+                u32 x[A::rl];
                 if (e < io.valid)
-                {
-                    std::size_t g = gidx(io, e);
-#pragma unroll
-                    for (int l = 0; l < L; ++l)
-                        x[l] = src[l * io.count + g];
-                }
+                    A::load_global(x, src, io.count, gidx(io, e));
                 else
                 {
 #pragma unroll
-                    for (int l = 0; l < L; ++l)
+                    for (int l = 0; l < A::rl; ++l)
                         x[l] = 0;
                 }
-                lds_put<L>(lds, e, x);
+                A::lds_put(lds, e, x);
             }
         }
 
-        // canonical: reduce lazy values to [0, q) on the way out
-        template<int L, int Form, int T>
+        template<typename A, int T>
         __device__ __forceinline__ void tile_store(u32 const* lds, u32* __restrict__ dst, io_desc const& io, bool canonical,
-                                                   fast_params const& P)
+                                                   typename A::params const& P)
         {
 #pragma unroll 4
             for (int e = threadIdx.x; e < io.valid; e += T)
             {
-                u32 x[L];
-                lds_get<L>(x, lds, e);
-                if (canonical)
-                    fast_ops<L, Form>::canonical(x, P);
-                std::size_t g = gidx(io, e);
-#pragma unroll
-                for (int l = 0; l < L; ++l)
-                    dst[l * io.count + g] = x[l];
+                u32 x[A::rl];
+                A::lds_get(x, lds, e);
+                A::store_global(dst, io.count, gidx(io, e), x, canonical, P);
             }
         }
 
@@ -535,26 +656,16 @@ namespace hmpc_amd
             return G;
         }
 
-        constexpr std::size_t tile_bytes(int L) { return std::size_t{4} * L * tile; }
-
-        inline fast_params make_fast(transform_args const& T)
-        {
-            fast_params P{};
-            for (int l = 0; l < max_limbs; ++l)
-            {
-                P.q[l] = T.P.q[l];
-                P.q2[l] = T.q2[l];
-            }
-            P.nprime = T.P.nprime;
-            return P;
-        }
+        template<typename A>
+        constexpr std::size_t tile_bytes() { return std::size_t{4} * A::rl * tile; }
 
         // ------------------------------------------------------------------ kernels
         // mode: 0 forward column pass, 1 forward row pass (final), 2 inverse row pass,
         //       3 inverse column pass (final), 4 forward whole-poly, 5 inverse whole-poly
-        template<int L, int Form, int E, int Mode>
-        __global__ void __launch_bounds__(tile / E, E >= 8 ? 2 : 4) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
-                                                           u32 const* __restrict__ tw, pass_geom G, u64 n, fast_params P)
+        template<typename A, int E, int Mode>
+        __global__ void __launch_bounds__(tile / E) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
+                                                           u32 const* __restrict__ tw, pass_geom G, u64 n,
+                                                           typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
             constexpr bool Inv = Mode == 2 || Mode == 3 || Mode == 5;
@@ -583,22 +694,22 @@ namespace hmpc_amd
                 final_scale = Mode == 5;
                 canonical = true;
             }
-            tile_load<L, T>(lds, in, io);
+            tile_load<A, T>(lds, in, io);
             __syncthreads();
-            u32 x[E][L];
-            tile_rounds<L, Form, E, Inv>(x, lds, td, base0, base_step, tw, P, final_scale, 0, false);
-            tile_store<L, Form, T>(lds, out, io, canonical, P);
+            u32 x[E][A::rl];
+            tile_rounds<A, E, Inv>(x, lds, td, base0, base_step, tw, P, final_scale, 0, false);
+            tile_store<A, T>(lds, out, io, canonical, P);
         }
 
         // polynomial product, whole polynomials per tile (N <= 4096) or the row pass of the
         // two-pass product: forward(A), forward(B), A o B in registers, inverse.
-        template<int L, int Form, int E, bool Single>
-        __global__ void __launch_bounds__(tile / E, E >= 8 ? 2 : 4) k_polymul(u32 const* __restrict__ a, u32 const* __restrict__ b,
+        template<typename A, int E, bool Single>
+        __global__ void __launch_bounds__(tile / E) k_polymul(u32 const* __restrict__ a, u32 const* __restrict__ b,
                                                               u32* __restrict__ out, u32 const* __restrict__ twf,
-                                                              u32 const* __restrict__ twi, pass_geom G, u64 n, fast_params P)
+                                                              u32 const* __restrict__ twi, pass_geom G, u64 n,
+                                                              typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
-            using ops = fast_ops<L, Form>;
             constexpr int T = tile / E;
             constexpr int WM = shape<E>::wmax;
             std::size_t count = static_cast<std::size_t>(n) << G.log_n;
@@ -615,38 +726,32 @@ namespace hmpc_amd
                 base0 = (1u << G.log_n1) + hi0;
                 base_step = 1;
             }
-            u32 xa[E][L];
-            u32 xb[E][L];
-            tile_load<L, T>(lds, a, io);
+            u32 xa[E][A::rl];
+            u32 xb[E][A::rl];
+            tile_load<A, T>(lds, a, io);
             __syncthreads();
-            tile_rounds<L, Form, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
+            tile_rounds<A, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
             __syncthreads();
-            tile_load<L, T>(lds, b, io);
+            tile_load<A, T>(lds, b, io);
             __syncthreads();
-            tile_rounds<L, Form, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
+            tile_rounds<A, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
 #pragma unroll
             for (int k = 0; k < E; ++k)
-            {
-                ops::half_reduce(xa[k], P);
-                ops::half_reduce(xb[k], P);
-                u32 t[L];
-                ops::mul(t, xa[k], xb[k], P);
-                copy<L>(xa[k], t);
-            }
+                A::point(xa[k], xb[k], P);
             // inverse round 0 groups the same elements as the forward's last round
             int w0 = round_width<WM>(td.s, 0, true);
-            dispatch_round_regs<L, Form, E>(w0, xa, lds, td, base0, base_step, twi, P, Single);
+            dispatch_round_regs<A, E>(w0, xa, lds, td, base0, base_step, twi, P, Single);
             __syncthreads();
-            tile_rounds<L, Form, E, true>(xa, lds, td, base0, base_step, twi, P, Single, 1, false);
-            tile_store<L, Form, T>(lds, out, io, Single, P);
+            tile_rounds<A, E, true>(xa, lds, td, base0, base_step, twi, P, Single, 1, false);
+            tile_store<A, T>(lds, out, io, Single, P);
         }
 
         // forward column pass of both inputs of the two-pass product (blockIdx.x = 2 * tile + which)
-        template<int L, int Form, int E>
-        __global__ void __launch_bounds__(tile / E, E >= 8 ? 2 : 4) k_polymul_cols(u32 const* __restrict__ a, u32 const* __restrict__ b,
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E) k_polymul_cols(u32 const* __restrict__ a, u32 const* __restrict__ b,
                                                                    u32* __restrict__ wa, u32* __restrict__ wb,
                                                                    u32 const* __restrict__ tw, pass_geom G, u64 n,
-                                                                   fast_params P)
+                                                                   typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
             constexpr int T = tile / E;
@@ -655,63 +760,67 @@ namespace hmpc_amd
             tile_desc td;
             io_desc io;
             cols_setup(G, blockIdx.x >> 1, count, td, io);
-            tile_load<L, T>(lds, which ? b : a, io);
+            tile_load<A, T>(lds, which ? b : a, io);
             __syncthreads();
-            u32 x[E][L];
-            tile_rounds<L, Form, E, false>(x, lds, td, 1u, 0u, tw, P, false, 0, false);
-            tile_store<L, Form, T>(lds, which ? wb : wa, io, false, P);
+            u32 x[E][A::rl];
+            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, tw, P, false, 0, false);
+            tile_store<A, T>(lds, which ? wb : wa, io, false, P);
         }
-    }
-}
 
-namespace hmpc_amd
-{
-    namespace fast
-    {
         // ------------------------------------------------------------------ host launchers
-        // The fast kernels are instantiated for the special-form moduli only (sparse 4-limb,
-        // Goldilocks 2-limb); every other modulus runs the portable kernels of kernels.hip.
-        template<int L>
-        constexpr int fast_form = L == 4 ? form_sparse : form_goldilocks;
-
-        template<int L, typename F>
-        hipError_t with_form_elems(int form, int elems, F&& f)
+        template<typename A, typename F>
+        hipError_t with_elems(int elems, F&& f)
         {
-            if (form != fast_form<L>)
-                return hipErrorInvalidValue;
-            using FC = std::integral_constant<int, fast_form<L>>;
-            if constexpr (L == 4)
+            if constexpr (A::rl == 5)
             {
                 if (elems == 4)
-                    return f(FC{}, std::integral_constant<int, 4>{});
-                return f(FC{}, std::integral_constant<int, 8>{});
+                    return f(std::integral_constant<int, 4>{});
+                return f(std::integral_constant<int, 8>{});
             }
             else
             {
                 if (elems == 8)
-                    return f(FC{}, std::integral_constant<int, 8>{});
-                return f(FC{}, std::integral_constant<int, 16>{});
+                    return f(std::integral_constant<int, 8>{});
+                return f(std::integral_constant<int, 16>{});
             }
         }
 
-        template<int L>
-        hipError_t launch_ntt(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out, hipStream_t s)
+        // dynamic LDS above the 64 KiB default needs the kernel attribute raised
+        template<typename K>
+        hipError_t allow_lds(K kernel, std::size_t bytes)
         {
-            fast_params P = make_fast(T);
-            return with_form_elems<L>(T.form, T.elems, [&](auto FC, auto EC) -> hipError_t {
-                constexpr int FF = decltype(FC)::value;
+            if (bytes <= 65536)
+                return hipSuccess;
+            return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(bytes));
+        }
+
+        template<typename A>
+        hipError_t launch_ntt(transform_args const& T, typename A::params const& P, std::size_t n, bool inverse,
+                              u32 const* in, u32* out, hipStream_t s)
+        {
+            return with_elems<A>(T.elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
                 constexpr int TH = tile / E;
-                std::size_t bytes = tile_bytes(L);
-                u32 const* tw = inverse ? T.inv_em : T.fwd_em;
+                constexpr std::size_t bytes = tile_bytes<A>();
+                u32 const* tw = inverse ? T.inv_fast : T.fwd_fast;
+                hipError_t e = hipSuccess;
                 if (T.log_n <= tile_log)
                 {
                     pass_geom G = geom_for(T.log_n, true);
                     unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
                     if (!inverse)
-                        k_pass<L, FF, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                    {
+                        if ((e = allow_lds(k_pass<A, E, 4>, bytes)) != hipSuccess)
+                            return e;
+                        k_pass<A, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                    }
                     else
-                        k_pass<L, FF, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                    {
+                        if ((e = allow_lds(k_pass<A, E, 5>, bytes)) != hipSuccess)
+                            return e;
+                        k_pass<A, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                    }
                     return hipGetLastError();
                 }
                 pass_geom gc = geom_for(T.log_n, false);
@@ -720,45 +829,53 @@ namespace hmpc_amd
                 unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
                 if (!inverse)
                 {
-                    k_pass<L, FF, E, 0><<<grid_c, TH, bytes, s>>>(in, out, tw, gc, n, P);
-                    k_pass<L, FF, E, 1><<<grid_r, TH, bytes, s>>>(out, out, tw, gr, n, P);
+                    if ((e = allow_lds(k_pass<A, E, 0>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 1>, bytes)) != hipSuccess)
+                        return e;
+                    k_pass<A, E, 0><<<grid_c, TH, bytes, s>>>(in, out, tw, gc, n, P);
+                    k_pass<A, E, 1><<<grid_r, TH, bytes, s>>>(out, out, tw, gr, n, P);
                 }
                 else
                 {
-                    k_pass<L, FF, E, 2><<<grid_r, TH, bytes, s>>>(in, out, tw, gr, n, P);
-                    k_pass<L, FF, E, 3><<<grid_c, TH, bytes, s>>>(out, out, tw, gc, n, P);
+                    if ((e = allow_lds(k_pass<A, E, 2>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 3>, bytes)) != hipSuccess)
+                        return e;
+                    k_pass<A, E, 2><<<grid_r, TH, bytes, s>>>(in, out, tw, gr, n, P);
+                    k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(out, out, tw, gc, n, P);
                 }
                 return hipGetLastError();
             });
         }
 
-        template<int L>
-        hipError_t launch_polymul(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
-                                  u32* workspace, hipStream_t s)
+        template<typename A>
+        hipError_t launch_polymul(transform_args const& T, typename A::params const& P, std::size_t n, u32 const* a,
+                                  u32 const* b, u32* out, u32* workspace, hipStream_t s)
         {
-            fast_params P = make_fast(T);
-            return with_form_elems<L>(T.form, T.elems, [&](auto FC, auto EC) -> hipError_t {
-                constexpr int FF = decltype(FC)::value;
+            return with_elems<A>(T.elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
                 constexpr int TH = tile / E;
-                std::size_t bytes = tile_bytes(L);
+                constexpr std::size_t bytes = tile_bytes<A>();
+                hipError_t e = hipSuccess;
                 if (T.log_n <= tile_log)
                 {
                     pass_geom G = geom_for(T.log_n, true);
                     unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
-                    k_polymul<L, FF, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_em, T.inv_em, G, n, P);
+                    if ((e = allow_lds(k_polymul<A, E, true>, bytes)) != hipSuccess)
+                        return e;
+                    k_polymul<A, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_fast, T.inv_fast, G, n, P);
                     return hipGetLastError();
                 }
                 std::size_t count = n << T.log_n;
                 u32* wa = workspace;
-                u32* wb = workspace + L * count;
+                u32* wb = workspace + A::iol * count;
                 pass_geom gc = geom_for(T.log_n, false);
                 pass_geom gr = geom_for(T.log_n, true);
                 unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
                 unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
-                k_polymul_cols<L, FF, E><<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_em, gc, n, P);
-                k_polymul<L, FF, E, false><<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_em, T.inv_em, gr, n, P);
-                k_pass<L, FF, E, 3><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_em, gc, n, P);
+                if ((e = allow_lds(k_polymul_cols<A, E>, bytes)) != hipSuccess ||
+                    (e = allow_lds(k_polymul<A, E, false>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 3>, bytes)) != hipSuccess)
+                    return e;
+                k_polymul_cols<A, E><<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
+                k_polymul<A, E, false><<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
+                k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
                 return hipGetLastError();
             });
         }

@@ -23,6 +23,7 @@ namespace hmpc_amd
         form_lazy = 1,
         form_sparse = 2,
         form_goldilocks = 3,
+        form_r26 = 4,  // q = 2^124 + c, c < 2^24: radix-2^26 redundant limbs (r26.hpp)
     };
 
     struct fast_params

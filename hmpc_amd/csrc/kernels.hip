@@ -153,6 +153,45 @@ namespace hmpc_amd
         });
     }
 
+    __global__ void __launch_bounds__(block_threads) k_r26_table(u32 const* __restrict__ planar, u32* __restrict__ out, u64 count,
+                                                                 p26 P)
+    {
+        u64 e = u64{blockIdx.x} * blockDim.x + threadIdx.x;
+        if (e >= count)
+            return;
+        u32 w[4];
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+            w[l] = planar[l * count + e];
+        // 4 w R mod q (w R canonical, 4 q < 2^127): shift, then three conditional subtractions
+        u32 t[4] = {w[0] << 2, (w[1] << 2) | (w[0] >> 30), (w[2] << 2) | (w[1] >> 30), (w[3] << 2) | (w[2] >> 30)};
+        for (int rep = 0; rep < 3; ++rep)
+        {
+            u32 d[4];
+            u32 br = 0;
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+                d[l] = subb(t[l], P.q32[l], br, br);
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+                t[l] = br ? t[l] : d[l];
+        }
+        u32 x[5];
+        r26_unpack(x, t[0], t[1], t[2], t[3]);
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+            out[e * 8 + l] = x[l];
+#pragma unroll
+        for (int l = 5; l < 8; ++l)
+            out[e * 8 + l] = 0;
+    }
+
+    hipError_t launch_r26_table(u32 const* planar, u32* out, std::size_t count, p26 const& P, hipStream_t s)
+    {
+        k_r26_table<<<grid_for(count), block_threads, 0, s>>>(planar, out, count, P);
+        return hipGetLastError();
+    }
+
     // ------------------------------------------------------------------ plain path (hmpc's launch structure)
     // K2 / K5: first stage, [n][N] -> transposed scratch [N][n]
     template<int L, bool Inv>

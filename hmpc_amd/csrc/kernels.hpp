@@ -3,6 +3,7 @@
 
 #include "modarith.hpp"
 #include "chacha.hpp"
+#include "r26.hpp"
 
 #include <cstddef>
 
@@ -19,9 +20,12 @@ namespace hmpc_amd
         const u32* inv;        // [L][N] inverse table
         const u32* fwd_em;     // [N][L] element-major copies for vector loads
         const u32* inv_em;
+        const u32* fwd_fast;   // tables in the fast path's format (r26: [N][8] words)
+        const u32* inv_fast;
         u32 q2[max_limbs];     // 2q
         int form;              // arith_form of the fast path
         int elems;             // fast path: elements per thread (4, 8 or 16)
+        p26 r26;               // constants of the radix-2^26 form (q = 2^124 + c)
     };
 
     // Decomposition N = N1 * N2 used by the two-pass fused transforms.
@@ -37,6 +41,8 @@ namespace hmpc_amd
     split_plan plan_for(int limbs, int log_n, int tiles_in_lds);
 
     hipError_t launch_to_element_major(int L, u32 const* planar, u32* em, std::size_t count, hipStream_t s);
+    // [4][N] canonical table of w R -> [N][8] radix-2^26 limbs of w 2^130 mod q
+    hipError_t launch_r26_table(u32 const* planar, u32* out, std::size_t count, p26 const& P, hipStream_t s);
 
     // fast path (L = 2, 4): register-resident radix-16 rounds, see kernels_fast.hip
     bool fast_path_supported(int L, int form, int log_n);

@@ -18,7 +18,7 @@ namespace hmpc_amd
     {
         // special-form moduli only; whole-polynomial tiles up to 2^12, two passes of at most
         // 2^12 points up to 2^20
-        bool special = (L == 4 && form == form_sparse) || (L == 2 && form == form_goldilocks);
+        bool special = (L == 4 && form == form_r26) || (L == 2 && form == form_goldilocks);
         return special && log_n >= 2 && log_n <= 20;
     }
 

@@ -1,4 +1,4 @@
-// Fast transform kernels instantiated for L = 2 limbs (the code lives in fast_impl.hpp).
+// Fast transform kernels instantiated for the Goldilocks 2-limb policy (code in fast_impl.hpp).
 #include "fast_impl.hpp"
 
 namespace hmpc_amd
@@ -6,12 +6,26 @@ namespace hmpc_amd
     hipError_t launch_fast_ntt_l2(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
                                   hipStream_t s)
     {
-        return fast::launch_ntt<2>(T, n, inverse, in, out, s);
+        fast_params P{};
+        for (int l = 0; l < max_limbs; ++l)
+        {
+            P.q[l] = T.P.q[l];
+            P.q2[l] = T.q2[l];
+        }
+        P.nprime = T.P.nprime;
+        return fast::launch_ntt<fast::arith_gold>(T, P, n, inverse, in, out, s);
     }
 
     hipError_t launch_fast_polymul_l2(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                       u32* workspace, hipStream_t s)
     {
-        return fast::launch_polymul<2>(T, n, a, b, out, workspace, s);
+        fast_params P{};
+        for (int l = 0; l < max_limbs; ++l)
+        {
+            P.q[l] = T.P.q[l];
+            P.q2[l] = T.q2[l];
+        }
+        P.nprime = T.P.nprime;
+        return fast::launch_polymul<fast::arith_gold>(T, P, n, a, b, out, workspace, s);
     }
 }

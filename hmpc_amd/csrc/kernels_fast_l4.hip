@@ -1,4 +1,4 @@
-// Fast transform kernels instantiated for L = 4 limbs (the code lives in fast_impl.hpp).
+// Fast transform kernels instantiated for the radix-2^26 4-limb policy (code in fast_impl.hpp).
 #include "fast_impl.hpp"
 
 namespace hmpc_amd
@@ -6,12 +6,14 @@ namespace hmpc_amd
     hipError_t launch_fast_ntt_l4(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
                                   hipStream_t s)
     {
-        return fast::launch_ntt<4>(T, n, inverse, in, out, s);
+        p26 const& P = T.r26;
+        return fast::launch_ntt<fast::arith_r26>(T, P, n, inverse, in, out, s);
     }
 
     hipError_t launch_fast_polymul_l4(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                       u32* workspace, hipStream_t s)
     {
-        return fast::launch_polymul<4>(T, n, a, b, out, workspace, s);
+        p26 const& P = T.r26;
+        return fast::launch_polymul<fast::arith_r26>(T, P, n, a, b, out, workspace, s);
     }
 }

@@ -1,0 +1,138 @@
+// Radix-2^26 redundant arithmetic for moduli q = 2^124 + c, 0 < c < 2^24 (the 125-bit primes
+// of the workloads: 2^124 + 2^20 + 2^18 + 1 and 2^124 + 0x78001).
+//
+// Why: on gfx950 v_mad_u64_u32 runs at full rate, but 32-bit-limb Montgomery code spends most
+// of its instructions on 64-bit register-pair moves and carry adds around each mad. With
+// 26-bit limbs every partial product accumulates in place into a 64-bit column sum
+// (acc = a * b + acc: one v_mad_u64_u32, no carries), and additions are limb-wise.
+//
+// Representation: x = sum_j x_j 2^(26 j), j = 0..4, limbs x_j < 2^32 (redundant). Internal
+// Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in hmpc's
+// representation a*R mod q (R = 2^128); twiddles are stored as w * 2^130 mod q so that
+// mont(a R, w 2^130) = (a w) R. A product of two data values is corrected by 4 = R' / R.
+//
+// Bounds (q > 2^124, so 64 q > 2^130):
+//   mont(a, b): column sums < 2^61 when a_j < 2^31 and b_j < 2^27; result < 2q with normalized
+//               limbs (r_4 < 2^22) whenever v(a) v(b) < 2^130 q
+//   forward butterfly (x + t, x - t + K4), t = mont(y, w) < 2q: value +4q and limbs +2^27 per
+//               stage; from inputs < 2q with limbs < 2^26 this stays < 2^130 and < 2^31 for
+//               up to 12 stages, so a whole pass runs without reductions
+//   fold(x):    normalize, then x_4 >= 2^20 carries h = x_4 >> 20 copies of 2^124 = q - c:
+//               replace by -h c (plus one q in borrowed digits to keep limbs >= 0);
+//               result < 2q with limbs < 2^28, for any input limbs < 2^32
+//   inverse butterfly (fold(x + y), mont(x - y + K32, w)) keeps every value < 2q (< 8q for
+//               the first stage after a x4 product correction)
+#pragma once
+
+#include "modarith.hpp"
+
+namespace hmpc_amd
+{
+    constexpr u32 m26 = (1u << 26) - 1u;
+
+    struct p26
+    {
+        u32 c;         // q = 2^124 + c
+        u32 n26;       // -q^-1 mod 2^26
+        u32 k4[5];     // 4q with limbs k4_0 >= 2^26, k4_{1..3} = 2^26 - 1 (>= mont output limbs)
+        u32 k32[5];    // 32q with limbs >= 2^29 - 8 (>= folded limbs)
+        u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
+        u32 q32[4];    // q as 32-bit limbs (final canonical subtraction)
+    };
+
+    HMPC_HD void r26_unpack(u32 (&x)[5], u32 w0, u32 w1, u32 w2, u32 w3)
+    {
+        x[0] = w0 & m26;
+        x[1] = ((w0 >> 26) | (w1 << 6)) & m26;
+        x[2] = ((w1 >> 20) | (w2 << 12)) & m26;
+        x[3] = ((w2 >> 14) | (w3 << 18)) & m26;
+        x[4] = w3 >> 8;
+    }
+
+    // normalized limbs (x_j < 2^26 for j < 4) and value < 2^128 -> 4 words
+    HMPC_HD void r26_pack(u32 (&w)[4], u32 const (&x)[5])
+    {
+        w[0] = x[0] | (x[1] << 26);
+        w[1] = (x[1] >> 6) | (x[2] << 20);
+        w[2] = (x[2] >> 12) | (x[3] << 14);
+        w[3] = (x[3] >> 18) | (x[4] << 8);
+    }
+
+    // carry-propagate limbs 0..3 into limb 4 (limb 4 keeps the excess)
+    HMPC_HD void r26_normalize(u32 (&x)[5])
+    {
+        x[1] += x[0] >> 26;
+        x[0] &= m26;
+        x[2] += x[1] >> 26;
+        x[1] &= m26;
+        x[3] += x[2] >> 26;
+        x[2] &= m26;
+        x[4] += x[3] >> 26;
+        x[3] &= m26;
+    }
+
+    // value -> congruent value < 2q, limbs < 2^28, for any limbs < 2^31
+    HMPC_HD void r26_fold(u32 (&x)[5], p26 const& P)
+    {
+        r26_normalize(x);  // x_4 < 2^31 + 2^5
+        u32 h = x[4] >> 20;
+        x[4] &= (1u << 20) - 1u;
+        u64 hc = static_cast<u64>(h) * P.c;  // < 2^12 * 2^24
+        u32 lo = static_cast<u32>(hc) & m26;
+        u32 hi = static_cast<u32>(hc >> 26);
+        x[0] = x[0] + P.qb[0] - lo;
+        x[1] = x[1] + P.qb[1] - hi;
+        x[2] += P.qb[2];
+        x[3] += P.qb[3];
+        x[4] += P.qb[4];
+    }
+
+    // Montgomery product a b 2^-130 mod q (lazy: < 2q, normalized limbs)
+    HMPC_HD void r26_mont(u32 (&r)[5], u32 const (&a)[5], u32 const (&b)[5], p26 const& P)
+    {
+        u64 T[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            T[k] = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                T[i + j] = static_cast<u64>(a[i]) * b[j] + T[i + j];
+        // q in radix 2^26: limb 0 = c, limb 4 = 2^20, others 0
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            u32 m = (static_cast<u32>(T[i]) * P.n26) & m26;
+            T[i] = static_cast<u64>(m) * P.c + T[i];
+            T[i + 4] = static_cast<u64>(m) * (1u << 20) + T[i + 4];
+            T[i + 1] += T[i] >> 26;
+        }
+        u64 c = T[5];
+        r[0] = static_cast<u32>(c) & m26;
+        c = T[6] + (c >> 26);
+        r[1] = static_cast<u32>(c) & m26;
+        c = T[7] + (c >> 26);
+        r[2] = static_cast<u32>(c) & m26;
+        c = T[8] + (c >> 26);
+        r[3] = static_cast<u32>(c) & m26;
+        r[4] = static_cast<u32>(c >> 26);
+    }
+
+    // x (value < 2q after a fold, limbs < 2^28) -> canonical 32-bit limbs
+    HMPC_HD void r26_canonical(u32 (&w)[4], u32 const (&xin)[5], p26 const& P)
+    {
+        u32 x[5] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
+        r26_normalize(x);
+        r26_pack(w, x);
+        u32 d[4];
+        u32 br = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            d[i] = subb(w[i], P.q32[i], br, br);
+        bool take = br == 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            w[i] = take ? d[i] : w[i];
+    }
+}

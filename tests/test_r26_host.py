@@ -1,0 +1,125 @@
+"""CPU checks of the radix-2^26 device arithmetic (hmpc_amd/csrc/r26.hpp) built for the host.
+
+The same header is compiled into the gfx950 kernels; here it runs on the CPU against python
+big integers, including the lazy bounds the fast transform relies on (values < 2^130 and
+limbs < 2^31 across 12 forward stages; inverse stages stay below 2q).
+"""
+
+import os
+import random
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "cpp", "r26_check.cpp")
+BIN = os.path.join(ROOT, "tests", "cpp", "r26_check")
+
+PRIMES = [(1 << 124) + (1 << 20) + (1 << 18) + 1, 0x10000000000000000000000000078001]
+M26 = (1 << 26) - 1
+
+
+@pytest.fixture(scope="module")
+def tool():
+    if not os.path.exists(BIN) or os.path.getmtime(BIN) < os.path.getmtime(SRC) or \
+            os.path.getmtime(BIN) < os.path.getmtime(os.path.join(ROOT, "hmpc_amd", "csrc", "r26.hpp")):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", SRC, "-o", BIN],
+                       check=True)
+    p = subprocess.Popen([BIN], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+    yield p
+    p.stdin.close()
+    p.wait()
+
+
+def run(p, line):
+    p.stdin.write(line + "\n")
+    p.stdin.flush()
+    return [int(v) for v in p.stdout.readline().split()]
+
+
+def limbs(v, top_bits=32):
+    out = [(v >> (26 * j)) & M26 for j in range(4)]
+    out.append(v >> 104)
+    return out
+
+
+def value(l):
+    return sum(int(x) << (26 * j) for j, x in enumerate(l))
+
+
+def redundant(v, rng, max_bits):
+    """a random redundant representation of v with limbs < 2^max_bits (v small enough)"""
+    l = limbs(v)
+    # move random multiples of 2^26 down from higher limbs
+    for j in range(4, 0, -1):
+        room = (1 << max_bits) - 1 - l[j - 1]
+        k = min(l[j], room >> 26)
+        k = rng.randrange(k + 1) if k else 0
+        l[j] -= k
+        l[j - 1] += k << 26
+    assert value(l) == v and all(0 <= x < (1 << 32) for x in l)
+    return l
+
+
+@pytest.mark.parametrize("q", PRIMES)
+def test_mont_fold_canonical(tool, q):
+    c = q - (1 << 124)
+    rng = random.Random(q & 0xFFFF)
+    R = 1 << 130
+    for _ in range(400):
+        a = rng.randrange(50 * q)
+        b = rng.randrange(q)
+        la = redundant(a, rng, 31)
+        lb = limbs(b)
+        r = run(tool, f"M {c} " + " ".join(map(str, la + lb)))
+        v = value(r)
+        assert v % q == a * b * pow(R, -1, q) % q
+        assert v < 2 * q and all(x < (1 << 26) for x in r[:4]) and r[4] < (1 << 22)
+        x = rng.randrange(1 << 135)
+        lx = redundant(x, rng, 31) if x < (1 << 133) else limbs(x)
+        if max(lx) >= (1 << 31):
+            continue
+        f = run(tool, f"F {c} " + " ".join(map(str, lx)))
+        assert value(f) % q == x % q and value(f) < 2 * q and all(v < (1 << 28) for v in f)
+        w = run(tool, f"C {c} " + " ".join(map(str, f)))
+        assert sum(v << (32 * i) for i, v in enumerate(w)) == x % q
+
+
+@pytest.mark.parametrize("q", PRIMES)
+def test_forward_chain_bounds(tool, q):
+    """12 lazy forward stages: the value and limb bounds hold and the result is exact mod q."""
+    c = q - (1 << 124)
+    rng = random.Random(7)
+    R = 1 << 130
+    for _ in range(200):
+        x, y = rng.randrange(2 * q), rng.randrange(2 * q)
+        ws = [rng.randrange(q) for _ in range(12)]
+        line = f"B {c} 12 " + " ".join(map(str, limbs(x) + limbs(y))) + " " + \
+               " ".join(" ".join(map(str, limbs(w))) for w in ws)
+        out = run(tool, line)
+        X, Y = value(out[:5]), value(out[5:])
+        ex, ey = x, y
+        for w in ws:
+            t = ey * w * pow(R, -1, q) % q
+            ex, ey = (ex + t) % q, (ex - t) % q
+        assert X % q == ex and Y % q == ey
+        assert X < (1 << 130) and Y < (1 << 130) and max(out) < (1 << 31)
+
+
+@pytest.mark.parametrize("q", PRIMES)
+def test_inverse_chain_bounds(tool, q):
+    c = q - (1 << 124)
+    rng = random.Random(11)
+    R = 1 << 130
+    for _ in range(200):
+        x, y = rng.randrange(8 * q), rng.randrange(8 * q)
+        lx, ly = redundant(x, rng, 28), redundant(y, rng, 28)
+        ws = [rng.randrange(q) for _ in range(12)]
+        line = f"I {c} 12 " + " ".join(map(str, lx + ly)) + " " + " ".join(" ".join(map(str, limbs(w))) for w in ws)
+        out = run(tool, line)
+        X, Y = value(out[:5]), value(out[5:])
+        ex, ey = x, y
+        for w in ws:
+            ex, ey = (ex + ey) % q, (ex - ey) * w * pow(R, -1, q) % q
+        assert X % q == ex and Y % q == ey
+        assert X < 2 * q and Y < 2 * q and max(out) < (1 << 28)
