@@ -7,13 +7,15 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import time
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmpc_hip.so")
-SOURCES = ["kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip", "kernels_fast_l4.hip", "api.hip"]
+SOURCES = ["kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip", "kernels_fast_l4.hip", "kernels_jobs_l2.hip",
+           "kernels_jobs_l4.hip", "api.hip"]
 ARCH = os.environ.get("HMPC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -31,10 +33,13 @@ CFLAGS = [
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    t0 = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError(f"hipcc failed for {src}")
+    if os.environ.get("HMPC_BUILD_TIMES"):
+        print(f"  {src}: {time.time() - t0:.1f} s", flush=True)
     return obj
 
 

@@ -179,8 +179,15 @@ struct hmpc_modulus_s
     p26 r26{};                // constants of the radix-2^26 form
     std::map<int, table_pair> tables;
     std::map<int, int> elems; // fast path: elements per thread chosen per log_n (tuner)
+    std::map<int, int> passes; // fast path: 1 = one launch per pass instead of the persistent job kernel
+    int occupancy[17] = {};   // persistent job kernel: workgroups per CU by elements per thread
 
     int default_elems(int) const { return limbs == 4 ? 4 : 16; }
+    bool persistent(int log_n) const
+    {
+        auto p = passes.find(log_n);
+        return p == passes.end() || p->second == 0;
+    }
 
     ~hmpc_modulus_s()
     {
@@ -627,6 +634,12 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
         std::copy(k32, k32 + 5, R.k32);
         std::copy(qb, qb + 5, R.qb);
         std::copy(m->P.q, m->P.q + 4, R.q32);
+        R.t20 = 1u << 20;
+        // R 2^130 mod q = 4 R^2 mod q, as normalized radix-2^26 limbs
+        u32 rr[4] = {m->r2[0], m->r2[1], m->r2[2], m->r2[3]};
+        double_mod(rr, m->P.q, 4);
+        double_mod(rr, m->P.q, 4);
+        r26_unpack(R.rr, rr[0], rr[1], rr[2], rr[3]);
     }
     else if (lazy_ok && middle_zero)
         m->form = form_sparse;
@@ -674,7 +687,7 @@ int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant)
     return HMPC_OK;
 }
 
-int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread)
+int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int persistent)
 {
     if (!mod || log_n < 2 || log_n > 20)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
@@ -682,20 +695,28 @@ int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread)
               (mod->limbs == 2 && (elems_per_thread == 8 || elems_per_thread == 16));
     if (!ok)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "elements per thread: 4 or 8 (4 limbs), 8 or 16 (2 limbs), 0 = default");
+    if (persistent < -1 || persistent > 1)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "persistent: 1 (job kernel), 0 (one launch per pass), -1 (unchanged)");
     if (elems_per_thread == 0)
         mod->elems.erase(log_n);
     else
         mod->elems[log_n] = elems_per_thread;
+    if (persistent == 1)
+        mod->passes.erase(log_n);
+    else if (persistent == 0)
+        mod->passes[log_n] = 1;
     return HMPC_OK;
 }
 
-int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* fast)
+int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* persistent, int* fast)
 {
     if (!mod || log_n < 2 || log_n > 20)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
     auto e = mod->elems.find(log_n);
     if (elems_per_thread)
         *elems_per_thread = e != mod->elems.end() ? e->second : mod->default_elems(log_n);
+    if (persistent)
+        *persistent = mod->persistent(log_n) ? 1 : 0;
     if (fast)
         *fast = mod->fast(log_n) ? 1 : 0;
     return HMPC_OK;
@@ -902,6 +923,126 @@ int hmpc_sample_cbd_ntt(hmpc_context ctx, hmpc_modulus mod, int bits, int log_n,
     return HMPC_OK;
 }
 
+// ------------------------------------------------------------------ persistent job kernel
+namespace
+{
+    // workgroups of the persistent kernel the device holds at once, capped by the job count
+    int jobs_grid(hmpc_modulus_s& m, int elems, std::size_t njobs, unsigned* grid)
+    {
+        int& occ = m.occupancy[elems];
+        if (occ == 0)
+        {
+            device_guard g(m.ctx->device);
+            hipError_t e = fast_jobs_occupancy(m.limbs, elems, &occ);
+            if (e != hipSuccess)
+                return hip_fail(e, "fast_jobs_occupancy");
+            occ = std::max(occ, 1);
+        }
+        std::size_t g = static_cast<std::size_t>(occ) * static_cast<std::size_t>(m.ctx->props.multiProcessorCount);
+        *grid = static_cast<unsigned>(std::max<std::size_t>(1, std::min(g, njobs)));
+        return HMPC_OK;
+    }
+
+    constexpr std::size_t counter_bytes = 256;
+
+    // stream workspace laid out as [job counter, 256 B][grid scratch slots]
+    int jobs_workspace(hmpc_context ctx, hipStream_t s, unsigned grid, u64 slot_words, u32** counter, u32** scratch)
+    {
+        void* w = nullptr;
+        int rc = ctx->workspace(s, counter_bytes + static_cast<std::size_t>(grid) * slot_words * 4, &w);
+        if (rc)
+            return rc;
+        *counter = static_cast<u32*>(w);
+        *scratch = static_cast<u32*>(w) + counter_bytes / 4;
+        return HMPC_OK;
+    }
+
+    // plaintext modulus given in Montgomery form -> integer, when it is below 2^40 (the fused
+    // samplers form p v + m as a signed 64-bit integer before converting it)
+    bool small_plaintext(hmpc_modulus_s const& m, u32 const* p_mont, u64* out)
+    {
+        return host_limbs(m.limbs, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            u32 a[LL], b[LL] = {}, r[LL];
+            for (int i = 0; i < LL; ++i)
+                a[i] = p_mont[i];
+            b[0] = 1;
+            mod_mul<LL>(r, a, b, m.P);
+            u64 v = r[0];
+            if constexpr (LL >= 2)
+                v |= static_cast<u64>(r[1]) << 32;
+            for (int i = 2; i < LL; ++i)
+                if (r[i])
+                    return false;
+            *out = v;
+            return v < (u64{1} << 40);
+        });
+    }
+
+    job_desc polymul_job(transform_args const& T, int log_n, u32 const* a, u32 const* b, u32* out, u64 count, u64 off)
+    {
+        job_desc J{};
+        J.type = job_polymul;
+        J.log_n = static_cast<u32>(log_n);
+        J.twf = T.fwd_fast;
+        J.twi = T.inv_fast;
+        J.in0 = a;
+        J.in1 = b;
+        J.in_count = count;
+        J.in_off = off;
+        J.out0 = out;
+        J.out_count = count;
+        J.out_off = off;
+        return J;
+    }
+
+    job_desc bgv_job(transform_args const& T, bgv_args const& B, u64 p_int, int log_n, u32 const* pk_a, u32 const* pk_b,
+                     u32 const* messages, u32* c0, u32* c1, u64 count, u64 off, u32 poly)
+    {
+        job_desc J{};
+        J.type = job_bgv;
+        J.log_n = static_cast<u32>(log_n);
+        J.poly = poly;
+        J.bits[0] = B.bits_u;
+        J.bits[1] = B.bits_v;
+        J.bits[2] = B.bits_w;
+        J.key = B.key;
+        std::copy(B.nonce_u, B.nonce_u + 3, J.nonce[0]);
+        std::copy(B.nonce_v, B.nonce_v + 3, J.nonce[1]);
+        std::copy(B.nonce_w, B.nonce_w + 3, J.nonce[2]);
+        J.p_int = p_int;
+        J.twf = T.fwd_fast;
+        J.twi = T.inv_fast;
+        J.in0 = pk_a;
+        J.in1 = pk_b;
+        J.in_count = u64{1} << log_n;
+        J.in_off = 0;
+        J.out0 = c0;
+        J.out1 = c1;
+        J.out_count = count;
+        J.out_off = off;
+        J.msg = messages;
+        J.msg_off = off;
+        return J;
+    }
+
+    // uniform batch of n jobs (job j at element offset j N) on the persistent kernel
+    int run_uniform_jobs(hmpc_modulus_s& m, transform_args const& T, job_desc const& base, std::size_t n, hipStream_t s)
+    {
+        unsigned grid = 0;
+        int rc = jobs_grid(m, T.elems, n, &grid);
+        if (rc)
+            return rc;
+        u64 slot = job_scratch_words(base.type, T.log_n, m.limbs);
+        u32 *counter = nullptr, *scratch = nullptr;
+        rc = jobs_workspace(m.ctx, s, grid, slot, &counter, &scratch);
+        if (rc)
+            return rc;
+        HMPC_HIP(launch_fast_jobs(m.limbs, T, nullptr, base, static_cast<u32>(n), grid, counter, scratch, slot, s));
+        return HMPC_OK;
+    }
+}
+
 // ------------------------------------------------------------------ fused batches
 int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out, void* stream)
 {
@@ -919,6 +1060,8 @@ int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const
     hipStream_t s = as_stream(stream);
     void* w = nullptr;
     bool fast = mod->fast(log_n);
+    if (fast && mod->persistent(log_n))
+        return run_uniform_jobs(*mod, T, polymul_job(T, log_n, a, b, out, static_cast<u64>(n) << log_n, 0), n, s);
     std::size_t words = fast ? fast_polymul_workspace_words(mod->limbs, log_n, n) : polymul_workspace_words(mod->limbs, log_n, n);
     if (words)
     {
@@ -980,11 +1123,21 @@ int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, co
         return HMPC_OK;
     device_guard g(ctx->device);
     hipStream_t s = as_stream(stream);
+    bgv_args B = make_bgv_args(ctx, mod, log_n, n, p_mont, bits_u, bits_v, bits_w);
+    u64 p_int = 0;
+    if (mod->fast(log_n) && mod->persistent(log_n) && small_plaintext(*mod, p_mont, &p_int))
+    {
+        job_desc J = bgv_job(T, B, p_int, log_n, pk_a, pk_b, messages, c0, c1, static_cast<u64>(n) << log_n, 0, 0);
+        rc = run_uniform_jobs(*mod, T, J, n, s);
+        if (rc)
+            return rc;
+        nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
+        return HMPC_OK;
+    }
     void* w = nullptr;
     rc = ctx->workspace(s, bgv_workspace_words(mod->limbs, log_n, n) * 4, &w);
     if (rc)
         return rc;
-    bgv_args B = make_bgv_args(ctx, mod, log_n, n, p_mont, bits_u, bits_v, bits_w);
     HMPC_HIP(launch_bgv_encrypt(mod->limbs, T, B, n, pk_a, pk_b, messages, c0, c1, static_cast<u32*>(w), s));
     nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
     return HMPC_OK;
@@ -1073,6 +1226,367 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
     HMPC_HIP(launch_pointwise(L, mod->P, HMPC_SUB, count, c0, x, count, x, s));
     HMPC_HIP(dispatch_ntt(*mod, T, n, true, x, x, s));
     HMPC_HIP(launch_centered_low32(L, mod->P, count, x, messages, s));
+    return HMPC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ job batches
+struct hmpc_batch_s
+{
+    struct entry
+    {
+        hmpc_modulus mod;
+        job_desc d;
+        u64 cost;
+    };
+    struct job_class
+    {
+        hmpc_modulus mod = nullptr;
+        int elems = 0;
+        int log_n = 0;                // any prepared size of the class (kernel parameters)
+        std::vector<job_desc> jobs;   // longest first
+        std::size_t table_off = 0;    // byte offsets into mem
+        std::size_t counter_off = 0;
+        unsigned grid = 0;
+        u64 slot_words = 0;
+    };
+
+    hmpc_context ctx = nullptr;
+    std::vector<entry> entries;
+    std::vector<job_class> classes;
+    bool built = false;
+    void* mem = nullptr;
+    std::size_t mem_bytes = 0;
+    std::size_t scratch_off = 0;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void release()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        if (mem)
+            (void)hipFree(mem);
+        mem = nullptr;
+        mem_bytes = 0;
+        classes.clear();
+        built = false;
+    }
+
+    ~hmpc_batch_s()
+    {
+        device_guard g(ctx->device);
+        (void)hipDeviceSynchronize();
+        release();
+        if (capture_stream)
+            (void)hipStreamDestroy(capture_stream);
+    }
+
+    // group by (modulus, elements per thread), order each class longest job first (LPT), size
+    // the grids and the shared scratch, upload the job tables
+    int build()
+    {
+        device_guard g(ctx->device);
+        std::map<std::pair<hmpc_modulus, int>, std::size_t> index;
+        std::vector<std::vector<entry const*>> members;
+        for (auto const& e : entries)
+        {
+            int log_n = static_cast<int>(e.d.log_n);
+            auto it = e.mod->elems.find(log_n);
+            int elems = it != e.mod->elems.end() ? it->second : e.mod->default_elems(log_n);
+            auto key = std::make_pair(e.mod, elems);
+            auto f = index.find(key);
+            if (f == index.end())
+            {
+                f = index.emplace(key, classes.size()).first;
+                job_class c;
+                c.mod = e.mod;
+                c.elems = elems;
+                c.log_n = log_n;
+                classes.push_back(c);
+                members.emplace_back();
+            }
+            members[f->second].push_back(&e);
+        }
+        std::size_t off = 0;
+        u64 scratch_words = 0;
+        for (std::size_t k = 0; k < classes.size(); ++k)
+        {
+            auto& c = classes[k];
+            auto& mem_list = members[k];
+            std::stable_sort(mem_list.begin(), mem_list.end(), [](entry const* x, entry const* y) { return x->cost > y->cost; });
+            for (auto const* e : mem_list)
+            {
+                c.jobs.push_back(e->d);
+                c.slot_words = std::max(c.slot_words, job_scratch_words(e->d.type, static_cast<int>(e->d.log_n), c.mod->limbs));
+            }
+            int rc = jobs_grid(*c.mod, c.elems, c.jobs.size(), &c.grid);
+            if (rc)
+                return rc;
+            scratch_words = std::max(scratch_words, static_cast<u64>(c.grid) * c.slot_words);
+            c.table_off = off;
+            off += (c.jobs.size() * sizeof(job_desc) + 255) / 256 * 256;
+            c.counter_off = off;
+            off += counter_bytes;
+        }
+        scratch_off = off;
+        mem_bytes = off + scratch_words * 4;
+        HMPC_HIP(hipMalloc(&mem, mem_bytes));
+        for (auto const& c : classes)
+            HMPC_HIP(hipMemcpy(static_cast<char*>(mem) + c.table_off, c.jobs.data(), c.jobs.size() * sizeof(job_desc),
+                               hipMemcpyHostToDevice));
+        built = true;
+        return HMPC_OK;
+    }
+
+    // one persistent launch per class, in sequence on `s` (they share the scratch)
+    int launch(hipStream_t s)
+    {
+        for (auto const& c : classes)
+        {
+            transform_args T;
+            int rc = c.mod->args(c.log_n, T);
+            if (rc)
+                return rc;
+            T.elems = c.elems;
+            auto* table = reinterpret_cast<job_desc const*>(static_cast<char*>(mem) + c.table_off);
+            auto* counter = reinterpret_cast<u32*>(static_cast<char*>(mem) + c.counter_off);
+            auto* scratch = reinterpret_cast<u32*>(static_cast<char*>(mem) + scratch_off);
+            HMPC_HIP(launch_fast_jobs(c.mod->limbs, T, table, job_desc{}, static_cast<u32>(c.jobs.size()), c.grid, counter,
+                                      scratch, c.slot_words, s));
+        }
+        return HMPC_OK;
+    }
+};
+
+extern "C" {
+
+int hmpc_batch_create(hmpc_context ctx, hmpc_batch* out)
+{
+    if (!ctx || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    auto b = std::make_unique<hmpc_batch_s>();
+    b->ctx = ctx;
+    *out = b.release();
+    return HMPC_OK;
+}
+
+int hmpc_batch_destroy(hmpc_batch batch)
+{
+    delete batch;
+    return HMPC_OK;
+}
+
+namespace
+{
+    int batch_check(hmpc_batch batch, hmpc_modulus mod, int log_n, transform_args& T)
+    {
+        if (!batch || !mod || mod->ctx != batch->ctx)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "batch and modulus must belong to one context");
+        if (batch->built)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "batch already ran; create a new batch to add jobs");
+        int rc = mod->args(log_n, T);
+        if (rc)
+            return rc;
+        if (!mod->fast(log_n))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "job batches need a modulus served by the fast kernels "
+                                                   "(q = 2^124 + c with c < 2^24, or 2^64 - 2^32 + 1)");
+        if (batch->entries.size() >= 0x7fffffffu)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
+        return HMPC_OK;
+    }
+}
+
+int hmpc_batch_add_polymul(hmpc_batch batch, hmpc_modulus mod, int log_n, const uint32_t* a, const uint32_t* b,
+                           uint32_t* out)
+{
+    transform_args T;
+    int rc = batch_check(batch, mod, log_n, T);
+    if (rc)
+        return rc;
+    if (!a || !b || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    u64 N = u64{1} << log_n;
+    batch->entries.push_back({mod, polymul_job(T, log_n, a, b, out, N, 0), 3 * N * static_cast<u64>(log_n)});
+    return HMPC_OK;
+}
+
+int hmpc_batch_add_bgv_encrypt(hmpc_batch batch, hmpc_modulus mod, int log_n, const uint32_t* pk_a, const uint32_t* pk_b,
+                               const uint32_t* messages, const uint32_t* p_mont, int bits_u, int bits_v, int bits_w,
+                               uint32_t* c0, uint32_t* c1)
+{
+    transform_args T;
+    int rc = batch_check(batch, mod, log_n, T);
+    if (rc)
+        return rc;
+    rc = bgv_check(batch->ctx, mod, 1, pk_a, pk_b, messages, p_mont, bits_u, bits_v, bits_w, c0, c1);
+    if (rc)
+        return rc;
+    u64 p_int = 0;
+    if (!small_plaintext(*mod, p_mont, &p_int))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "job batches need a plaintext modulus below 2^40");
+    hmpc_context ctx = batch->ctx;
+    bgv_args B = make_bgv_args(ctx, mod, log_n, 1, p_mont, bits_u, bits_v, bits_w);
+    u64 N = u64{1} << log_n;
+    batch->entries.push_back({mod, bgv_job(T, B, p_int, log_n, pk_a, pk_b, messages, c0, c1, N, 0, 0),
+                              4 * N * static_cast<u64>(log_n)});
+    nonce_add(ctx->nonce, 3 * N);
+    return HMPC_OK;
+}
+
+int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream)
+{
+    if (!batch)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null batch");
+    if (batch->entries.empty())
+        return HMPC_OK;
+    device_guard g(batch->ctx->device);
+    hipStream_t s = as_stream(stream);
+    if (!batch->built)
+    {
+        int rc = batch->build();
+        if (rc)
+        {
+            batch->release();
+            return rc;
+        }
+    }
+    if (!use_graph)
+        return batch->launch(s);
+    if (!batch->exec)
+    {
+        // capture on a private stream (the null stream cannot be captured); counters are reset
+        // by a kernel node, never a memset node
+        if (!batch->capture_stream)
+            HMPC_HIP(hipStreamCreateWithFlags(&batch->capture_stream, hipStreamNonBlocking));
+        HMPC_HIP(hipStreamBeginCapture(batch->capture_stream, hipStreamCaptureModeRelaxed));
+        int rc = batch->launch(batch->capture_stream);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(batch->capture_stream, &graph);
+        if (rc)
+        {
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return hip_fail(e, "hipStreamEndCapture");
+        batch->graph = graph;
+        HMPC_HIP(hipGraphInstantiate(&batch->exec, graph, nullptr, nullptr, 0));
+    }
+    HMPC_HIP(hipGraphLaunch(batch->exec, s));
+    return HMPC_OK;
+}
+
+int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* device_bytes)
+{
+    if (!batch)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null batch");
+    if (jobs)
+        *jobs = batch->entries.size();
+    if (classes)
+        *classes = batch->classes.size();
+    if (device_bytes)
+        *device_bytes = batch->mem_bytes;
+    return HMPC_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* elems_out, int* persistent_out,
+                  float* ms_out)
+{
+    if (!mod || pairs == 0 || pairs > (size_t{1} << 20))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (!mod->fast(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "no tunable schedule: this modulus uses the portable kernels");
+    device_guard g(mod->ctx->device);
+    hipStream_t s = as_stream(stream);
+    std::size_t count = pairs << log_n;
+    std::size_t bytes = count * mod->limbs * 4;
+    u32* buf = nullptr;
+    HMPC_HIP(hipMalloc(&buf, 3 * bytes));
+    u32* a = buf;
+    u32* b = buf + count * mod->limbs;
+    u32* out = b + count * mod->limbs;
+    chacha_key key{};
+    key.k[0] = 0x7475u;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best_e = 0, best_p = 1;
+    float best = 0.0f;
+    auto cleanup = [&]() {
+        if (e0)
+            (void)hipEventDestroy(e0);
+        if (e1)
+            (void)hipEventDestroy(e1);
+        (void)hipFree(buf);
+    };
+    if (launch_sample_uniform(mod->limbs, mod->P, mod->bit_size, key, 0, count, a, s) != hipSuccess ||
+        launch_sample_uniform(mod->limbs, mod->P, mod->bit_size, key, 1, count, b, s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    {
+        cleanup();
+        return fail(HMPC_ERR_DEVICE, "autotune set-up failed");
+    }
+    int const cand_l4[2] = {4, 8}, cand_l2[2] = {8, 16};
+    int const* cand = mod->limbs == 4 ? cand_l4 : cand_l2;
+    auto saved_e = mod->elems;
+    auto saved_p = mod->passes;
+    for (int ci = 0; ci < 2; ++ci)
+        for (int persistent = 1; persistent >= 0; --persistent)
+        {
+            mod->elems[log_n] = cand[ci];
+            if (persistent)
+                mod->passes.erase(log_n);
+            else
+                mod->passes[log_n] = 1;
+            rc = hmpc_polymul(mod, log_n, pairs, a, b, out, s);  // warm-up (workspace, code objects)
+            for (int r = 0; rc == HMPC_OK && r < 3; ++r)
+            {
+                if (r == 0)
+                    (void)hipEventRecord(e0, s);
+                rc = hmpc_polymul(mod, log_n, pairs, a, b, out, s);
+            }
+            float ms = 0.0f;
+            if (rc == HMPC_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(HMPC_ERR_DEVICE, "autotune timing failed");
+            if (rc)
+            {
+                mod->elems = saved_e;
+                mod->passes = saved_p;
+                cleanup();
+                return rc;
+            }
+            ms /= 3.0f;
+            if (best_e == 0 || ms < best)
+            {
+                best = ms;
+                best_e = cand[ci];
+                best_p = persistent;
+            }
+        }
+    mod->elems[log_n] = best_e;
+    if (best_p)
+        mod->passes.erase(log_n);
+    else
+        mod->passes[log_n] = 1;
+    cleanup();
+    if (elems_out)
+        *elems_out = best_e;
+    if (persistent_out)
+        *persistent_out = best_p;
+    if (ms_out)
+        *ms_out = best;
     return HMPC_OK;
 }
 

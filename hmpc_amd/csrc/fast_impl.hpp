@@ -36,6 +36,17 @@ namespace hmpc_amd
         constexpr int tile_log = 12;
         constexpr int tile = 1 << tile_log;
 
+        // Thread index as an opaque value per use. Every per-thread LDS / twiddle address below is
+        // a function of it; kept opaque, that arithmetic is recomputed where it is used (a few
+        // integer ops) instead of being hoisted out of the job kernel's tile loops and held in
+        // VGPRs across them, which spilled registers.
+        __device__ __forceinline__ int tid_x()
+        {
+            int t;
+            asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(static_cast<int>(threadIdx.x)));
+            return t;
+        }
+
         constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
         // ------------------------------------------------------------------ arithmetic policies
@@ -154,6 +165,34 @@ namespace hmpc_amd
                 for (int l = 0; l < rl; ++l)
                     a[l] = t[l] << 2;
             }
+
+            // x = k R + x for k canonical (limbs < 2^26) and u, x forward-transform outputs
+            // (< 52 q, limbs < 2^31): mont(u, k) < 2q needs v(u) v(k) < 2^130 q, which holds
+            // without folding u; 4 mont(u, k) adds limbs < 2^28
+            __device__ __forceinline__ static void point_add(u32 (&x)[rl], u32 const (&u)[rl], u32 const (&k)[rl],
+                                                             params const& P)
+            {
+                u32 t[rl];
+                r26_mont(t, u, k, P);
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    x[l] += t[l] << 2;
+            }
+
+            // signed integer |z| < 2^78 -> z R mod q as a transform input (< 4q, limbs < 2^27):
+            // mont(|z|, R 2^130) = |z| R (normalized, < 2q), negated as 4q - t with the
+            // borrowed-digit constant k4 (limbs of k4 dominate normalized limbs)
+            __device__ __forceinline__ static void from_signed(u32 (&x)[rl], long long z, params const& P)
+            {
+                u64 az = z < 0 ? static_cast<u64>(-z) : static_cast<u64>(z);
+                u32 a[rl] = {static_cast<u32>(az) & m26, static_cast<u32>(az >> 26) & m26, static_cast<u32>(az >> 52), 0u, 0u};
+                u32 t[rl];
+                r26_mont(t, a, P.rr, P);
+                bool neg = z < 0;
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    x[l] = neg ? P.k4[l] - t[l] : t[l];
+            }
         };
 
         struct arith_gold
@@ -221,6 +260,32 @@ namespace hmpc_amd
                 ops::mul(t, a, b, P);
                 a[0] = t[0];
                 a[1] = t[1];
+            }
+
+            // x = k u + x (canonical)
+            __device__ __forceinline__ static void point_add(u32 (&x)[rl], u32 const (&u)[rl], u32 const (&k)[rl],
+                                                             params const& P)
+            {
+                u32 t[rl], s[rl];
+                ops::mul(t, k, u, P);
+                ops::add(s, t, x, P);
+                x[0] = s[0];
+                x[1] = s[1];
+            }
+
+            // signed integer |z| < 2^63 -> z R mod q (canonical): |z| < q is canonical, so the
+            // Montgomery product with R^2 is |z| R; negatives are q - that (0 stays 0)
+            __device__ __forceinline__ static void from_signed(u32 (&x)[rl], long long z, params const& P)
+            {
+                u64 az = z < 0 ? static_cast<u64>(-z) : static_cast<u64>(z);
+                u32 a[rl] = {static_cast<u32>(az), static_cast<u32>(az >> 32)};
+                u32 r2[rl] = {P.r2[0], P.r2[1]};
+                u32 t[rl], zero[rl] = {0u, 0u}, n[rl];
+                ops::mul(t, a, r2, P);
+                ops::sub(n, zero, t, P);
+                bool neg = z < 0;
+                x[0] = neg ? n[0] : t[0];
+                x[1] = neg ? n[1] : t[1];
             }
         };
 
@@ -314,7 +379,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                unit_t un = unit_of(td, a, W, inv, threadIdx.x + shape<E>::threads * g);
+                unit_t un = unit_of(td, a, W, inv, tid_x() + shape<E>::threads * g);
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                     A::lds_get(x[g * K + k], lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)));
@@ -328,7 +393,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                unit_t un = unit_of(td, a, W, inv, threadIdx.x + shape<E>::threads * g);
+                unit_t un = unit_of(td, a, W, inv, tid_x() + shape<E>::threads * g);
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                     A::lds_put(lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)), x[g * K + k]);
@@ -356,7 +421,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                unit_t un = unit_of(td, a, W, Inv, threadIdx.x + shape<E>::threads * g);
+                unit_t un = unit_of(td, a, W, Inv, tid_x() + shape<E>::threads * g);
                 u32 base = base0 + un.c * base_step;
 #pragma unroll
                 for (int m = 0; m < count; ++m)
@@ -550,7 +615,7 @@ namespace hmpc_amd
         __device__ __forceinline__ void tile_load(u32* lds, u32 const* __restrict__ src, io_desc const& io)
         {
 #pragma unroll 4
-            for (int e = threadIdx.x; e < tile; e += T)
+            for (int e = tid_x(); e < tile; e += T)
             {
                 u32 x[A::rl];
                 if (e < io.valid)
@@ -570,7 +635,7 @@ namespace hmpc_amd
                                                    typename A::params const& P)
         {
 #pragma unroll 4
-            for (int e = threadIdx.x; e < io.valid; e += T)
+            for (int e = tid_x(); e < io.valid; e += T)
             {
                 u32 x[A::rl];
                 A::lds_get(x, lds, e);
@@ -627,7 +692,7 @@ namespace hmpc_amd
             io.valid = static_cast<int>(polys << G.log_n);
         }
 
-        inline pass_geom geom_for(int log_n, bool rows_pass)
+        __host__ __device__ inline pass_geom geom_for(int log_n, bool rows_pass)
         {
             pass_geom G{};
             G.log_n = log_n;

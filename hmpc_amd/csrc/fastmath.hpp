@@ -30,6 +30,7 @@ namespace hmpc_amd
     {
         u32 q[max_limbs];
         u32 q2[max_limbs];  // 2q (lazy forms)
+        u32 r2[max_limbs];  // R^2 mod q (conversion of integers to Montgomery form)
         u32 nprime;
     };
 

@@ -1,0 +1,475 @@
+// Persistent job kernel of the fast path, gfx950.
+//
+// One launch runs a list of independent jobs (jobs.hpp). The grid is sized to the number of
+// workgroups the device holds at once; every workgroup claims the next job with one atomic on a
+// counter and runs it to completion, from inputs in HBM to outputs in HBM, then claims the next,
+// until the list is exhausted (every workgroup exits once the counter passes the job count). A
+// job never waits for another workgroup, so there is no inter-workgroup synchronisation and no
+// cross-XCD coherence traffic: the only state shared between the passes of one job is the
+// workgroup's own scratch slot, written and read by the same workgroup (visible after its
+// barrier).
+//
+// Per job, for N = 2^log_n:
+//   N <= 4096: the whole polynomial is one LDS tile (fast_impl.hpp); forward rounds, product
+//              and inverse rounds run back to back.
+//   N  > 4096: the two-pass decomposition N = N1 * N2 of fast_impl.hpp, pass by pass over the
+//              job's tiles: forward column tiles -> scratch; row tiles (forward, product,
+//              inverse) -> scratch; inverse column tiles -> output.
+// BGV jobs sample u, p v + m and p w on the capability streams straight into the column tiles
+// (three transforms instead of four: NTT(v) p + NTT(m) = NTT(p v + m)), and combine with the
+// public key in the row pass: c0 = A o U + X0, c1 = B o U + X1
+// (hmpc/expr/crypto/lhe/enc.hpp:19-69, hmpc/expr/crypto/lhe/randomness.hpp:80).
+#pragma once
+
+#include "fast_impl.hpp"
+#include "jobs.hpp"
+
+namespace hmpc_amd
+{
+    namespace fast
+    {
+        // ------------------------------------------------------------------ tile placement
+        __device__ __forceinline__ void job_cols_tile(pass_geom const& G, u32 t, std::size_t count, std::size_t off,
+                                                      tile_desc& td, io_desc& io)
+        {
+            cols_setup(G, t, count, td, io);
+            io.base += off;
+        }
+
+        __device__ __forceinline__ u32 job_rows_tile(pass_geom const& G, u32 t, std::size_t count, std::size_t off,
+                                                     tile_desc& td, io_desc& io)
+        {
+            u32 hi0 = rows_setup(G, t, count, td, io);
+            io.base += off;
+            return hi0;
+        }
+
+        __device__ __forceinline__ void job_single_tile(int log_n, std::size_t count, std::size_t off, tile_desc& td,
+                                                        io_desc& io)
+        {
+            td = tile_desc{log_n, tile_log - log_n, true};
+            io.base = off;
+            io.count = count;
+            io.stride = 0;
+            io.log_seqs = tile_log - log_n;
+            io.rows = true;
+            io.valid = 1 << log_n;
+        }
+
+        // ------------------------------------------------------------------ second operands
+        // After the forward transform's last round (stages [s - W, s)) of a row-layout tile, the
+        // thread's unit g holds the K = 2^W consecutive tile elements ((tid + T g) << W) + k, so a
+        // second operand of the same elements is read straight from global memory (K
+        // consecutive words per limb plane; the workgroup wrote it itself just before, so it is
+        // served from L2) instead of occupying a second register tile.
+        //   Mode 0: x = a o x            (a = NTT(a) of a product, lazy)
+        //   Mode 1: x = k o u + x        (u = NTT(u), k = public-key polynomial, canonical)
+        // Only the tile's first `valid` elements are backed by memory (a whole-tile job of degree
+        // N < 4096 fills one of the tile's 4096 / N sequences); the others are left untouched.
+        template<typename A, int E, int W, int Mode>
+        __device__ __forceinline__ void with_operands(u32 (&x)[E][A::rl], int valid, u32 const* __restrict__ src,
+                                                      std::size_t src_count, std::size_t src_base, u32 const* __restrict__ key,
+                                                      std::size_t key_count, std::size_t key_base, typename A::params const& P)
+        {
+            constexpr int K = 1 << W;
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+                int const e0 = (tid_x() + shape<E>::threads * g) << W;
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    if (e0 + k >= valid)
+                        continue;
+                    u32 v[A::rl];
+                    A::load_global(v, src, src_count, src_base + e0 + k);
+                    if constexpr (Mode == 0)
+                    {
+                        A::point(v, x[g * K + k], P);
+#pragma unroll
+                        for (int l = 0; l < A::rl; ++l)
+                            x[g * K + k][l] = v[l];
+                    }
+                    else
+                    {
+                        u32 kv[A::rl];
+                        A::load_global(kv, key, key_count, key_base + e0 + k);
+                        A::point_add(x[g * K + k], v, kv, P);
+                    }
+                }
+            }
+        }
+
+        template<typename A, int E, int Mode>
+        __device__ __forceinline__ void with_operands_last(u32 (&x)[E][A::rl], tile_desc const& td, int valid, u32 const* src,
+                                                           std::size_t src_count, std::size_t src_base, u32 const* key,
+                                                           std::size_t key_count, std::size_t key_base,
+                                                           typename A::params const& P)
+        {
+            constexpr int WM = shape<E>::wmax;
+            int w = round_width<WM>(td.s, round_count<WM>(td.s) - 1, false);
+            if (w == WM)
+                with_operands<A, E, WM, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+            else if (WM > 1 && w == 1)
+                with_operands<A, E, 1, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+            else if constexpr (WM > 2)
+            {
+                if (w == 2)
+                    with_operands<A, E, 2, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+                else if constexpr (WM > 3)
+                {
+                    if (w == 3)
+                        with_operands<A, E, 3, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+                }
+            }
+        }
+
+        template<typename A, int E>
+        __device__ __forceinline__ void scatter_last(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td)
+        {
+            constexpr int WM = shape<E>::wmax;
+            int n = round_count<WM>(td.s);
+            int a = round_start<WM>(td.s, n - 1, false);
+            int w = round_width<WM>(td.s, n - 1, false);
+            if (w == WM)
+                scatter<A, E, WM>(x, lds, td, a, false);
+            else if (WM > 1 && w == 1)
+                scatter<A, E, 1>(x, lds, td, a, false);
+            else if constexpr (WM > 2)
+            {
+                if (w == 2)
+                    scatter<A, E, 2>(x, lds, td, a, false);
+                else if constexpr (WM > 3)
+                {
+                    if (w == 3)
+                        scatter<A, E, 3>(x, lds, td, a, false);
+                }
+            }
+        }
+
+        // ------------------------------------------------------------------ capability sampling
+        // Fills the tile with stream `which` (0: u, 1: p v + m, 2: p w) of a BGV job. Butterfly
+        // pair k < N/2 draws both of its coefficients (k, k + N/2) from one ChaCha20 block with
+        // counter low32(poly N + k) (hmpc/expr/number_theoretic_transform.hpp:326-329,
+        // hmpc/comp/queue.hpp:191-201). Column tiles hold rows r and r + N1/2 of each column,
+        // so both coefficients of a pair land in the same tile.
+        template<typename A, int T, int Which>
+        __device__ __forceinline__ void sample_tile(u32* lds, job_desc const& J, bool cols, int log_n1, int log_seqs, u32 col0,
+                                                    typename A::params const& P)
+        {
+            constexpr int which = Which;
+            int const log_n = static_cast<int>(J.log_n);
+            u32 const half = 1u << (log_n - 1);
+            int const half_rows = cols ? (1 << (log_n1 - 1)) : 0;
+            int const pairs = cols ? (half_rows << log_seqs) : static_cast<int>(half);
+            int const bits = J.bits[which];
+            u32 const* nonce = J.nonce[which];
+            long long const p = static_cast<long long>(J.p_int);
+            for (int e = tid_x(); e < pairs; e += T)
+            {
+                u32 k;
+                int elo, ehi;
+                if (cols)
+                {
+                    int r = e >> log_seqs;
+                    int c = e & ((1 << log_seqs) - 1);
+                    k = (static_cast<u32>(r) << (log_n - log_n1)) + col0 + static_cast<u32>(c);
+                    elo = e;
+                    ehi = e + (half_rows << log_seqs);
+                }
+                else
+                {
+                    k = static_cast<u32>(e);
+                    elo = e;
+                    ehi = e + static_cast<int>(half);
+                }
+                u32 blk[16];
+                chacha20_block(blk, J.key, static_cast<u32>((static_cast<u64>(J.poly) << log_n) + k), nonce[0], nonce[1],
+                               nonce[2]);
+                long long zl = centered_binomial_words(blk[0], blk[1], bits);
+                long long zh = bits <= 32 ? centered_binomial_words(blk[1], 0u, bits) : centered_binomial_words(blk[2], blk[3], bits);
+                if constexpr (which != 0)
+                {
+                    zl *= p;
+                    zh *= p;
+                }
+                if constexpr (which == 1)
+                {
+                    zl += J.msg[J.msg_off + k];
+                    zh += J.msg[J.msg_off + k + half];
+                }
+                u32 xl[A::rl], xh[A::rl];
+                A::from_signed(xl, zl, P);
+                A::from_signed(xh, zh, P);
+                A::lds_put(lds, elo, xl);
+                A::lds_put(lds, ehi, xh);
+            }
+        }
+
+        // ------------------------------------------------------------------ jobs
+        // Only one register tile is live at any time (the other operand streams from scratch),
+        // which keeps the kernel within 128 VGPRs.
+        template<typename A, int E>
+        __device__ __forceinline__ void run_polymul_job(job_desc const& J, u32* lds, u32* scratch, typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            constexpr int WM = shape<E>::wmax;
+            int const log_n = static_cast<int>(J.log_n);
+            std::size_t const N = std::size_t{1} << log_n;
+            u32 x[E][A::rl];
+            if (log_n <= tile_log)
+            {
+                tile_desc td;
+                io_desc ia, io, is;
+                job_single_tile(log_n, J.in_count, J.in_off, td, ia);
+                job_single_tile(log_n, J.out_count, J.out_off, td, io);
+                job_single_tile(log_n, N, 0, td, is);
+                // NTT(a) -> scratch
+                tile_load<A, T>(lds, J.in0, ia);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+                tile_store<A, T>(lds, scratch, is, false, P);
+                __syncthreads();
+                // NTT(b) in registers, times NTT(a), inverse
+                tile_load<A, T>(lds, J.in1, ia);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, true);
+                with_operands_last<A, E, 0>(x, td, is.valid, scratch, N, 0, nullptr, 0, 0, P);
+                dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, 1u, 0u, J.twi, P, true);
+                __syncthreads();
+                tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 1, false);
+                tile_store<A, T>(lds, J.out0, io, true, P);
+                __syncthreads();
+                return;
+            }
+            u32* sa = scratch;
+            u32* sb = scratch + A::iol * N;
+            pass_geom const gc = geom_for(log_n, false);
+            pass_geom const gr = geom_for(log_n, true);
+            // forward column pass of a and b
+            for (u32 t = 0; t < 2 * gc.blocks_per_poly; ++t)
+            {
+                u32 which = t >= gc.blocks_per_poly ? 1u : 0u;
+                u32 ct = t - which * gc.blocks_per_poly;
+                tile_desc td;
+                io_desc ii, is;
+                job_cols_tile(gc, ct, J.in_count, J.in_off, td, ii);
+                job_cols_tile(gc, ct, N, 0, td, is);
+                tile_load<A, T>(lds, which ? J.in1 : J.in0, ii);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+                tile_store<A, T>(lds, which ? sb : sa, is, false, P);
+                __syncthreads();
+            }
+            // row pass: NTT(a) rows back into scratch; NTT(b) rows in registers, product,
+            // inverse rows (scaled later) into scratch
+            for (u32 t = 0; t < gr.blocks_per_poly; ++t)
+            {
+                tile_desc td;
+                io_desc is;
+                u32 hi0 = job_rows_tile(gr, t, N, 0, td, is);
+                u32 base0 = (1u << gr.log_n1) + hi0;
+                tile_load<A, T>(lds, sa, is);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+                tile_store<A, T>(lds, sa, is, false, P);
+                __syncthreads();
+                tile_load<A, T>(lds, sb, is);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
+                dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
+                __syncthreads();
+                tile_rounds<A, E, true>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
+                tile_store<A, T>(lds, sa, is, false, P);
+                __syncthreads();
+            }
+            // inverse column pass with the final scaling, canonical output
+            for (u32 t = 0; t < gc.blocks_per_poly; ++t)
+            {
+                tile_desc td;
+                io_desc is, io;
+                job_cols_tile(gc, t, N, 0, td, is);
+                job_cols_tile(gc, t, J.out_count, J.out_off, td, io);
+                tile_load<A, T>(lds, sa, is);
+                __syncthreads();
+                tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false);
+                tile_store<A, T>(lds, J.out0, io, true, P);
+                __syncthreads();
+            }
+        }
+
+        // c = k o U + X: X in registers after its forward transform, U and k read per element,
+        // result written through LDS with coalesced stores (canonical)
+        template<typename A, int E>
+        __device__ __forceinline__ void bgv_emit(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 const* U,
+                                                 std::size_t u_count, std::size_t u_base, u32 const* key,
+                                                 io_desc const& ik, u32* out, io_desc const& io,
+                                                 typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            with_operands_last<A, E, 1>(x, td, ik.valid, U, u_count, u_base, key, ik.count, ik.base, P);
+            scatter_last<A, E>(x, lds, td);
+            __syncthreads();
+            tile_store<A, T>(lds, out, io, true, P);
+            __syncthreads();
+        }
+
+        template<typename A, int E, int Which>
+        __device__ __forceinline__ void bgv_columns(job_desc const& J, u32* lds, u32* dst, pass_geom const& gc,
+                                                    u32 (&x)[E][A::rl], typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            std::size_t const N = std::size_t{1} << J.log_n;
+            for (u32 ct = 0; ct < gc.blocks_per_poly; ++ct)
+            {
+                tile_desc td;
+                io_desc is;
+                job_cols_tile(gc, ct, N, 0, td, is);
+                sample_tile<A, T, Which>(lds, J, true, gc.log_n1, gc.log_seqs, ct << gc.log_seqs, P);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+                tile_store<A, T>(lds, dst, is, false, P);
+                __syncthreads();
+            }
+        }
+
+        template<typename A, int E>
+        __device__ __forceinline__ void run_bgv_job(job_desc const& J, u32* lds, u32* scratch, typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            int const log_n = static_cast<int>(J.log_n);
+            std::size_t const N = std::size_t{1} << log_n;
+            u32 x[E][A::rl];
+            if (log_n <= tile_log)
+            {
+                // public key and scratch planes hold N elements: in_count == N, in_off == 0
+                tile_desc td;
+                io_desc ik, io, is;
+                job_single_tile(log_n, J.in_count, J.in_off, td, ik);
+                job_single_tile(log_n, J.out_count, J.out_off, td, io);
+                job_single_tile(log_n, N, 0, td, is);
+                sample_tile<A, T, 0>(lds, J, false, 0, 0, 0u, P);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+                tile_store<A, T>(lds, scratch, is, false, P);
+                __syncthreads();
+                sample_tile<A, T, 1>(lds, J, false, 0, 0, 0u, P);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, true);
+                bgv_emit<A, E>(x, lds, td, scratch, N, 0, J.in0, ik, J.out0, io, P);
+                sample_tile<A, T, 2>(lds, J, false, 0, 0, 0u, P);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, true);
+                bgv_emit<A, E>(x, lds, td, scratch, N, 0, J.in1, ik, J.out1, io, P);
+                return;
+            }
+            pass_geom const gc = geom_for(log_n, false);
+            pass_geom const gr = geom_for(log_n, true);
+            u32* su = scratch;
+            // sample and run the column stages of u, p v + m, p w
+            bgv_columns<A, E, 0>(J, lds, su, gc, x, P);
+            bgv_columns<A, E, 1>(J, lds, scratch + A::iol * N, gc, x, P);
+            bgv_columns<A, E, 2>(J, lds, scratch + 2 * A::iol * N, gc, x, P);
+            // row stages: U back into scratch; X0, X1 in registers combined with the key
+            for (u32 t = 0; t < gr.blocks_per_poly; ++t)
+            {
+                tile_desc td;
+                io_desc is, ik, io;
+                u32 hi0 = job_rows_tile(gr, t, N, 0, td, is);
+                job_rows_tile(gr, t, J.in_count, J.in_off, td, ik);
+                job_rows_tile(gr, t, J.out_count, J.out_off, td, io);
+                u32 base0 = (1u << gr.log_n1) + hi0;
+                tile_load<A, T>(lds, su, is);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+                tile_store<A, T>(lds, su, is, false, P);
+                __syncthreads();
+                for (int which = 1; which < 3; ++which)
+                {
+                    tile_load<A, T>(lds, scratch + which * A::iol * N, is);
+                    __syncthreads();
+                    tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                    bool first = which == 1;
+                    bgv_emit<A, E>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+                }
+            }
+        }
+
+        // ------------------------------------------------------------------ the persistent kernel
+        template<typename A>
+        __global__ void k_reset_counter(u32* counter)
+        {
+            if (threadIdx.x == 0)
+                counter[0] = 0;
+        }
+
+        // Jobs come from `table` or, when it is null, from `base` with per-job strides (job j of
+        // a uniform batch: element offsets + j N, capability polynomial index + j).
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E) k_jobs(job_desc const* __restrict__ table, job_desc base, u32 njobs,
+                                                           u32* __restrict__ counter, u32* __restrict__ scratch,
+                                                           u64 slot_words, typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            u32* slot = scratch + static_cast<std::size_t>(blockIdx.x) * slot_words;
+            for (;;)
+            {
+                // the tile is free here (every job ends with a barrier after its last LDS read)
+                if (threadIdx.x == 0)
+                    lds[0] = atomicAdd(counter, 1u);
+                __syncthreads();
+                u32 j = __builtin_amdgcn_readfirstlane(lds[0]);
+                __syncthreads();
+                if (j >= njobs)
+                    break;
+                job_desc J;
+                if (table)
+                    J = table[j];
+                else
+                {
+                    J = base;
+                    u64 step = static_cast<u64>(j) << J.log_n;
+                    if (J.type == job_polymul)
+                        J.in_off += step;
+                    J.out_off += step;
+                    J.msg_off += step;
+                    J.poly += j;
+                }
+                if (J.type == job_polymul)
+                    run_polymul_job<A, E>(J, lds, slot, P);
+                else
+                    run_bgv_job<A, E>(J, lds, slot, P);
+            }
+        }
+
+        template<typename A>
+        hipError_t jobs_occupancy(int elems, int* blocks_per_cu)
+        {
+            return with_elems<A>(elems, [&](auto EC) -> hipError_t {
+                constexpr int E = decltype(EC)::value;
+                constexpr std::size_t bytes = tile_bytes<A>();
+                hipError_t e = allow_lds(k_jobs<A, E>, bytes);
+                if (e != hipSuccess)
+                    return e;
+                return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_jobs<A, E>, tile / E, bytes);
+            });
+        }
+
+        template<typename A>
+        hipError_t launch_jobs(int elems, job_desc const* table, job_desc const& base, u32 njobs, unsigned grid,
+                               u32* counter, u32* scratch, u64 slot_words, typename A::params const& P, hipStream_t s)
+        {
+            return with_elems<A>(elems, [&](auto EC) -> hipError_t {
+                constexpr int E = decltype(EC)::value;
+                constexpr std::size_t bytes = tile_bytes<A>();
+                hipError_t e = allow_lds(k_jobs<A, E>, bytes);
+                if (e != hipSuccess)
+                    return e;
+                k_reset_counter<A><<<1, 64, 0, s>>>(counter);
+                k_jobs<A, E><<<grid, tile / E, bytes, s>>>(table, base, njobs, counter, scratch, slot_words, P);
+                return hipGetLastError();
+            });
+        }
+    }
+}

@@ -4,6 +4,7 @@
 #include "modarith.hpp"
 #include "chacha.hpp"
 #include "r26.hpp"
+#include "jobs.hpp"
 
 #include <cstddef>
 
@@ -51,6 +52,11 @@ namespace hmpc_amd
     std::size_t fast_polymul_workspace_words(int L, int log_n, std::size_t n);
     hipError_t launch_fast_polymul(int L, transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                    u32* workspace, hipStream_t s);
+    // persistent job kernel (jobs_impl.hpp): workgroups the device holds at once per CU, and
+    // the launch (counter reset + kernel) of `njobs` jobs from `table` or the uniform `base`
+    hipError_t fast_jobs_occupancy(int L, int elems, int* blocks_per_cu);
+    hipError_t launch_fast_jobs(int L, transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
+                                unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
 
     hipError_t launch_make_twiddles(int L, u32* table, int log_n, u32 const* root, u32 const* norm, u32 const* one,
                                     bool inverse, mod_params const& P, hipStream_t s);

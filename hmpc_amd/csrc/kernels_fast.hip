@@ -2,6 +2,7 @@
 // and kernels_fast_l4.hip, device code in fast_impl.hpp).
 #include "kernels.hpp"
 #include "fastmath.hpp"
+#include "jobs.hpp"
 
 namespace hmpc_amd
 {
@@ -13,6 +14,13 @@ namespace hmpc_amd
                                       u32* workspace, hipStream_t s);
     hipError_t launch_fast_polymul_l4(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                       u32* workspace, hipStream_t s);
+
+    hipError_t fast_jobs_occupancy_l2(int elems, int* blocks_per_cu);
+    hipError_t fast_jobs_occupancy_l4(int elems, int* blocks_per_cu);
+    hipError_t launch_fast_jobs_l2(transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
+                                   unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
+    hipError_t launch_fast_jobs_l4(transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
+                                   unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
 
     bool fast_path_supported(int L, int form, int log_n)
     {
@@ -44,6 +52,25 @@ namespace hmpc_amd
             return launch_fast_polymul_l4(T, n, a, b, out, workspace, s);
         if (L == 2)
             return launch_fast_polymul_l2(T, n, a, b, out, workspace, s);
+        return hipErrorInvalidValue;
+    }
+
+    hipError_t fast_jobs_occupancy(int L, int elems, int* blocks_per_cu)
+    {
+        if (L == 4)
+            return fast_jobs_occupancy_l4(elems, blocks_per_cu);
+        if (L == 2)
+            return fast_jobs_occupancy_l2(elems, blocks_per_cu);
+        return hipErrorInvalidValue;
+    }
+
+    hipError_t launch_fast_jobs(int L, transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
+                                unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s)
+    {
+        if (L == 4)
+            return launch_fast_jobs_l4(T, table, base, njobs, grid, counter, scratch, slot_words, s);
+        if (L == 2)
+            return launch_fast_jobs_l2(T, table, base, njobs, grid, counter, scratch, slot_words, s);
         return hipErrorInvalidValue;
     }
 }

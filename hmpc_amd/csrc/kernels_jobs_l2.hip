@@ -1,0 +1,33 @@
+// Persistent job kernel instantiated for the Goldilocks 2-limb policy (code in jobs_impl.hpp).
+#include "jobs_impl.hpp"
+
+namespace hmpc_amd
+{
+    namespace
+    {
+        fast_params gold_params(transform_args const& T)
+        {
+            fast_params P{};
+            for (int l = 0; l < max_limbs; ++l)
+            {
+                P.q[l] = T.P.q[l];
+                P.q2[l] = T.q2[l];
+                P.r2[l] = T.r2[l];
+            }
+            P.nprime = T.P.nprime;
+            return P;
+        }
+    }
+
+    hipError_t fast_jobs_occupancy_l2(int elems, int* blocks_per_cu)
+    {
+        return fast::jobs_occupancy<fast::arith_gold>(elems, blocks_per_cu);
+    }
+
+    hipError_t launch_fast_jobs_l2(transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
+                                   unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s)
+    {
+        return fast::launch_jobs<fast::arith_gold>(T.elems, table, base, njobs, grid, counter, scratch, slot_words,
+                                                   gold_params(T), s);
+    }
+}

@@ -38,6 +38,9 @@ namespace hmpc_amd
         u32 k32[5];    // 32q with limbs >= 2^29 - 8 (>= folded limbs)
         u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
         u32 q32[4];    // q as 32-bit limbs (final canonical subtraction)
+        u32 rr[5];     // R 2^130 mod q (R = 2^128), normalized: mont(z, rr) = z R mod q
+        u32 t20;       // 2^20 (top radix-2^26 limb of q), kept in a register so that m * 2^20 + T
+                       // stays one v_mad_u64_u32 instead of a 64-bit shift and a 64-bit add
     };
 
     HMPC_HD void r26_unpack(u32 (&x)[5], u32 w0, u32 w1, u32 w2, u32 w3)
@@ -105,7 +108,7 @@ namespace hmpc_amd
         {
             u32 m = (static_cast<u32>(T[i]) * P.n26) & m26;
             T[i] = static_cast<u64>(m) * P.c + T[i];
-            T[i + 4] = static_cast<u64>(m) * (1u << 20) + T[i + 4];
+            T[i + 4] = static_cast<u64>(m) * P.t20 + T[i + 4];
             T[i + 1] += T[i] >> 26;
         }
         u64 c = T[5];

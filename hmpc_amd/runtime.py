@@ -43,8 +43,9 @@ SIGNATURES = {
     "hmpc_modulus_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _u32p, _u32p, _u32p]),
     "hmpc_modulus_prepare": (_i, [_vp, _i]),
     "hmpc_modulus_set_kernels": (_i, [_vp, _i]),
-    "hmpc_modulus_set_schedule": (_i, [_vp, _i, _i]),
-    "hmpc_modulus_get_schedule": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "hmpc_modulus_set_schedule": (_i, [_vp, _i, _i, _i]),
+    "hmpc_modulus_get_schedule": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "hmpc_autotune": (_i, [_vp, _i, _sz, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
     "hmpc_modulus_root": (_i, [_vp, _i, _i, _u32p]),
     "hmpc_modulus_twiddles": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
     "hmpc_ntt": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp]),
@@ -61,6 +62,12 @@ SIGNATURES = {
     "hmpc_bgv_encrypt": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
     "hmpc_bgv_encrypt_plain": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
     "hmpc_bgv_decrypt": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "hmpc_batch_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "hmpc_batch_destroy": (_i, [_vp]),
+    "hmpc_batch_add_polymul": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "hmpc_batch_add_bgv_encrypt": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp]),
+    "hmpc_batch_run": (_i, [_vp, _i, _vp]),
+    "hmpc_batch_info": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
     "hmpc_event_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "hmpc_event_destroy": (_i, [_vp, _vp]),
     "hmpc_event_record": (_i, [_vp, _vp, _vp]),
@@ -314,13 +321,20 @@ class Modulus:
         """0: fast register-radix kernels where supported, 1: portable LDS radix-2 kernels."""
         check(_lib.hmpc_modulus_set_kernels(self.handle, variant))
 
-    def set_schedule(self, log_n: int, elems_per_thread: int):
-        check(_lib.hmpc_modulus_set_schedule(self.handle, log_n, elems_per_thread))
+    def set_schedule(self, log_n: int, elems_per_thread: int, persistent: int = -1):
+        """elements per thread (0 = default); persistent 1 = job kernel, 0 = one launch per pass,
+        -1 = unchanged"""
+        check(_lib.hmpc_modulus_set_schedule(self.handle, log_n, elems_per_thread, persistent))
 
     def schedule(self, log_n: int):
-        e, f = _i(), _i()
-        check(_lib.hmpc_modulus_get_schedule(self.handle, log_n, ctypes.byref(e), ctypes.byref(f)))
-        return {"elems_per_thread": e.value, "fast": bool(f.value)}
+        e, p, f = _i(), _i(), _i()
+        check(_lib.hmpc_modulus_get_schedule(self.handle, log_n, ctypes.byref(e), ctypes.byref(p), ctypes.byref(f)))
+        return {"elems_per_thread": e.value, "persistent": bool(p.value), "fast": bool(f.value)}
+
+    def autotune(self, log_n: int, pairs: int, stream=None) -> dict:
+        e, p, ms = _i(), _i(), ctypes.c_float()
+        check(_lib.hmpc_autotune(self.handle, log_n, pairs, _sp(stream), ctypes.byref(e), ctypes.byref(p), ctypes.byref(ms)))
+        return {"elems_per_thread": e.value, "persistent": bool(p.value), "ms": ms.value}
 
     def prepare(self, log_n: int):
         check(_lib.hmpc_modulus_prepare(self.handle, log_n))
@@ -418,3 +432,45 @@ class Modulus:
         out = out or Buffer(self.ctx, 4 * (n << log_n))
         check(_lib.hmpc_bgv_decrypt(self.handle, log_n, n, s_ntt.ptr, c0.ptr, c1.ptr, out.ptr, _sp(stream)))
         return out
+
+
+class Batch:
+    """A list of independent jobs (products, encryptions) of uneven degree and modulus run by
+    persistent kernels, optionally replayed from a captured hipGraph."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _vp()
+        check(_lib.hmpc_batch_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h.value
+        self._keep = []
+
+    def add_polymul(self, mod: Modulus, log_n: int, a: Buffer, b: Buffer, out: Buffer):
+        check(_lib.hmpc_batch_add_polymul(self.handle, mod.handle, log_n, a.ptr, b.ptr, out.ptr))
+        self._keep += [a, b, out]
+
+    def add_bgv_encrypt(self, mod: Modulus, log_n: int, pk_a: Buffer, pk_b: Buffer, messages: Buffer, p_mont,
+                        c0: Buffer, c1: Buffer, bits=(2, 40, 40)):
+        pm = np.ascontiguousarray(p_mont, dtype=np.uint32)
+        check(_lib.hmpc_batch_add_bgv_encrypt(self.handle, mod.handle, log_n, pk_a.ptr, pk_b.ptr, messages.ptr, _p(pm),
+                                              bits[0], bits[1], bits[2], c0.ptr, c1.ptr))
+        self._keep += [pk_a, pk_b, messages, c0, c1]
+
+    def run(self, graph: bool = True, stream=None):
+        check(_lib.hmpc_batch_run(self.handle, int(graph), _sp(stream)))
+
+    def info(self) -> dict:
+        j, c, b = _sz(), _sz(), _sz()
+        check(_lib.hmpc_batch_info(self.handle, ctypes.byref(j), ctypes.byref(c), ctypes.byref(b)))
+        return {"jobs": j.value, "classes": c.value, "device_bytes": b.value}
+
+    def close(self):
+        if self.handle and _lib is not None:
+            _lib.hmpc_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

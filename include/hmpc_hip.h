@@ -95,10 +95,18 @@ int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* npr
  * portable LDS radix-2 kernels; 1 = always the portable kernels. Results are identical. */
 int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
 /* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 limbs, 8 or
- * 16 for 2 limbs; 0 restores the default). Chosen by the autotuner; results are identical. */
-int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread);
+ * 16 for 2 limbs; 0 restores the default) and launch structure (persistent = 1: one persistent
+ * job kernel in which each workgroup runs whole products / encryptions; 0: one launch per
+ * transform pass; -1: unchanged). Chosen by hmpc_autotune; results are identical. */
+int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int persistent);
 /* Current schedule and whether the fast kernels serve this (modulus, N) at all. */
-int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* fast);
+int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* persistent, int* fast);
+/* Autotuner: times batched products of `pairs` synthetic polynomial pairs of N = 2^log_n under
+ * every schedule of the fast kernels, installs the fastest with hmpc_modulus_set_schedule and
+ * reports it (each output may be NULL; ms = time of one batch). Blocking. No counterpart in
+ * hmpc, whose kernels have no schedule. */
+int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* elems_per_thread, int* persistent,
+                  float* ms);
 /* Build (once) and keep resident the forward/inverse tables for N = 2^log_n (2 <= log_n <= 20),
  * 2N must divide q - 1. Blocking. */
 int hmpc_modulus_prepare(hmpc_modulus mod, int log_n);
@@ -176,6 +184,31 @@ int hmpc_bgv_encrypt_plain(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t
  * centred lift, low 32 bits -> messages (n*N u32). */
 int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_ntt, const uint32_t* c0,
                      const uint32_t* c1, uint32_t* messages, void* stream);
+
+/* ------------------------------------------------------------------ job batches
+ * Many independent jobs of uneven degree (2^12 .. 2^16 and beyond) and modulus in persistent
+ * launches: one launch per (modulus, schedule) class, in which each workgroup claims jobs
+ * longest first and runs each from inputs to outputs. Each job's polynomials are single
+ * polynomials ([L][N] buffers). hmpc has no batch object: a job list stands for the sequence of
+ * queue(...) calls it replaces (hmpc/comp/queue.hpp:416-529), with the same results.
+ * Moduli must be served by the fast kernels (hmpc_modulus_get_schedule: fast = 1). */
+typedef struct hmpc_batch_s* hmpc_batch;
+int hmpc_batch_create(hmpc_context ctx, hmpc_batch* out);
+int hmpc_batch_destroy(hmpc_batch batch);
+/* out = INTT(NTT(a) o NTT(b)) */
+int hmpc_batch_add_polymul(hmpc_batch batch, hmpc_modulus mod, int log_n, const uint32_t* a, const uint32_t* b,
+                           uint32_t* out);
+/* one BGV encryption (hmpc_bgv_encrypt with n = 1): the context nonce is taken now and
+ * advanced by 3N, as a queue(enc(...)) call would; p_mont must encode an integer < 2^40 */
+int hmpc_batch_add_bgv_encrypt(hmpc_batch batch, hmpc_modulus mod, int log_n, const uint32_t* pk_a, const uint32_t* pk_b,
+                               const uint32_t* messages, const uint32_t* p_mont, int bits_u, int bits_v, int bits_w,
+                               uint32_t* c0, uint32_t* c1);
+/* Run every job. The first run freezes the batch (job tables uploaded, schedules fixed); with
+ * use_graph the launches are captured once into a hipGraph and later runs replay it (same
+ * inputs, nonces and outputs). Asynchronous on `stream`. */
+int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream);
+/* jobs, kernel classes (after the first run), device bytes held (tables, counters, scratch) */
+int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* device_bytes);
 
 /* ------------------------------------------------------------------ timing helpers
  * HIP events on a given stream (used by benchmarks to time kernels on the stream they run on). */

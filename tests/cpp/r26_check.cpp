@@ -34,6 +34,7 @@ static p26 make(u32 c, u32 q_unused)
     R.q32[1] = 0;
     R.q32[2] = 0;
     R.q32[3] = 0x10000000u;
+    R.t20 = 1u << 20;
     return R;
 }
 

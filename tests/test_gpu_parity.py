@@ -201,12 +201,15 @@ def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     want0, want1, nonce = om.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, [0, 0, 0], threads=8)
     dA, dB, dm = gm.upload(A), gm.upload(B), ctx.upload(msgs)
     pm = oracle.int_to_limbs(t_mont, gm.limbs)
-    for plain in (False, True):
+    for mode in ("jobs", "passes", "plain"):
+        say(f"  bgv mode={mode}")
+        gm.set_schedule(log_n, 0, 0 if mode == "passes" else 1)
         ctx.set_rng(enc_key, [0, 0, 0])
-        c0, c1 = gm.bgv_encrypt(log_n, n, dA, dB, dm, pm, plain=plain)
-        assert np.array_equal(gm.download(c0, n * N), want0), f"c0 plain={plain}"
-        assert np.array_equal(gm.download(c1, n * N), want1), f"c1 plain={plain}"
+        c0, c1 = gm.bgv_encrypt(log_n, n, dA, dB, dm, pm, plain=mode == "plain")
+        assert np.array_equal(gm.download(c0, n * N), want0), f"c0 mode={mode}"
+        assert np.array_equal(gm.download(c1, n * N), want1), f"c1 mode={mode}"
         assert ctx.rng[1].tolist() == nonce.tolist()
+    gm.set_schedule(log_n, 0, 1)
     dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
     assert np.array_equal(dec, msgs)
     assert np.array_equal(dec, om.bgv_decrypt(log_n, s, want0, want1))
@@ -275,9 +278,78 @@ def test_fast_kernel_schedules(ctx, q, log_n, n):
     want_p, _ = om.polymul_timed(log_n, a, b, threads=8)
     da, db = gm.upload(a), gm.upload(b)
     for elems in ((4, 8) if gm.limbs == 4 else (8, 16)):
-        say(f"fast schedule q={hex(q)[:12]} log_n={log_n} n={n} elems={elems}")
-        gm.set_schedule(log_n, elems)
-        assert np.array_equal(gm.download(gm.ntt(log_n, n, da), count), want_f), f"forward elems={elems}"
-        assert np.array_equal(gm.download(gm.ntt(log_n, n, da, inverse=True), count), want_i), f"inverse elems={elems}"
-        assert np.array_equal(gm.download(gm.polymul(log_n, n, da, db), count), want_p), f"polymul elems={elems}"
-    gm.set_schedule(log_n, 0)
+        for persistent in (1, 0):
+            say(f"fast schedule q={hex(q)[:12]} log_n={log_n} n={n} elems={elems} persistent={persistent}")
+            gm.set_schedule(log_n, elems, persistent)
+            assert gm.schedule(log_n)["persistent"] == bool(persistent)
+            assert np.array_equal(gm.download(gm.ntt(log_n, n, da), count), want_f), f"forward elems={elems}"
+            assert np.array_equal(gm.download(gm.ntt(log_n, n, da, inverse=True), count), want_i), f"inverse elems={elems}"
+            got = gm.download(gm.polymul(log_n, n, da, db), count)
+            assert np.array_equal(got, want_p), f"polymul elems={elems} persistent={persistent}"
+    gm.set_schedule(log_n, 0, 1)
+
+
+def test_autotune_installs_a_correct_schedule(ctx):
+    gm, om = mods(ctx, Q128A)
+    log_n, n = 14, 8
+    say(f"autotune q={hex(Q128A)[:12]} log_n={log_n} pairs={n}")
+    best = gm.autotune(log_n, n)
+    sched = gm.schedule(log_n)
+    assert sched["elems_per_thread"] == best["elems_per_thread"] and sched["persistent"] == best["persistent"]
+    assert best["ms"] > 0
+    count = n << log_n
+    a = om.sample_uniform(KEY, 0, count, threads=8)
+    b = om.sample_uniform(KEY, 1, count, threads=8)
+    want, _ = om.polymul_timed(log_n, a, b, threads=8)
+    assert np.array_equal(gm.download(gm.polymul(log_n, n, gm.upload(a), gm.upload(b)), count), want)
+    gm.set_schedule(log_n, 0, 1)
+
+
+# uneven job list: products and encryptions, N from 2^12 to 2^16, both fast moduli
+BATCH_JOBS = [("mul", Q64, 12), ("bgv", Q128A, 13), ("mul", Q128A, 16), ("bgv", Q64, 14), ("mul", Q128A, 12),
+              ("bgv", Q128A, 16), ("mul", Q64, 15), ("bgv", Q64, 12), ("mul", Q128A, 14), ("mul", Q64, 16),
+              ("bgv", Q128A, 12), ("mul", Q128A, 13)]
+
+
+def test_batch_of_uneven_jobs_matches_oracle(ctx):
+    """Persistent job kernels (one per modulus class) over an uneven job list, eager and
+    replayed from a hipGraph three times with host work between replays; every job's output
+    equals the oracle's, and each encryption consumes the nonces a queue call would."""
+    enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
+    ctx.set_rng(enc_key, [0, 0, 0])
+    batch = hmpc_amd.Batch(ctx)
+    checks = []
+    nonce = np.zeros(3, np.uint32)
+    for i, (kind, q, log_n) in enumerate(BATCH_JOBS):
+        gm, om = mods(ctx, q)
+        N = 1 << log_n
+        if kind == "mul":
+            a = om.sample_uniform(KEY, 10 + 2 * i, N)
+            b = om.sample_uniform(KEY, 11 + 2 * i, N)
+            out = gm.alloc(N)
+            batch.add_polymul(gm, log_n, gm.upload(a), gm.upload(b), out)
+            want, _ = om.polymul_timed(log_n, a, b, threads=8)
+            checks.append((gm, N, [(out, want)]))
+        else:
+            t_mont = om.from_unsigned(1 << 32)
+            A, B, _ = _keygen(om, log_n, t_mont)
+            msgs = oracle.sample_words(KEY, 100 + i, N)
+            c0, c1 = gm.alloc(N), gm.alloc(N)
+            batch.add_bgv_encrypt(gm, log_n, gm.upload(A), gm.upload(B), ctx.upload(msgs),
+                                  oracle.int_to_limbs(t_mont, gm.limbs), c0, c1)
+            want0, want1, nonce = om.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, nonce, threads=8)
+            checks.append((gm, N, [(c0, want0), (c1, want1)]))
+    assert ctx.rng[1].tolist() == list(nonce)
+    for run, graph in enumerate((False, True, True, True)):
+        say(f"batch run={run} graph={graph} jobs={len(BATCH_JOBS)}")
+        for gm, N, outs in checks:
+            for buf, _ in outs:  # eager work between replays: clear every output
+                gm.pointwise_scalar(gm.MUL, N, buf, np.zeros(gm.limbs, np.uint32), out=buf)
+        ctx.wait()
+        batch.run(graph=graph)
+        ctx.wait()
+        for k, (gm, N, outs) in enumerate(checks):
+            for buf, want in outs:
+                assert np.array_equal(gm.download(buf, N), want), f"job {k} {BATCH_JOBS[k]} run {run}"
+    info = batch.info()
+    assert info["jobs"] == len(BATCH_JOBS) and info["classes"] >= 2
