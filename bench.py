@@ -56,6 +56,17 @@ def time_steps(ctx, stream, fn, steps, warmup):
     return total / steps, per, wall
 
 
+# algorithmic HBM bytes per launch of each fast kernel, in units of (elements x 4L bytes): what
+# the kernel must read and write at least (a, b in, product out; column-pass results in between)
+KERNEL_PASSES = {
+    "fast persistent jobs": 3,           # polymul: a, b read, product written
+    "fast polymul forward columns": 4,   # a, b read; both column results written
+    "fast polymul rows": 3,              # both column results read, row result written
+    "fast polymul inverse columns": 2,   # row result read, product written
+    "fast polymul (whole tile)": 3,
+}
+
+
 def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
     mod = hm.Modulus(ctx, hm.Q128A)
     mod.prepare(log_n)
@@ -64,8 +75,14 @@ def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
     a = mod.sample_uniform(KEY, 0, count)
     b = mod.sample_uniform(KEY, 1, count)
     out = mod.alloc(count)
+    tuned = mod.autotune(log_n, 64, stream=stream)
+    log(f"[bench] autotuned schedule: {tuned}")
+    ctx.set_timing(True)
     ms, per, wall = time_steps(ctx, stream, lambda: mod.polymul(log_n, n, a, b, out, stream=stream), steps, warmup)
-    res = {"mod": mod, "a": a, "b": b, "out": out, "n": n, "log_n": log_n, "ms": ms, "per": per, "wall": wall}
+    kernels = ctx.kernel_times()
+    ctx.set_timing(False)
+    res = {"mod": mod, "a": a, "b": b, "out": out, "n": n, "log_n": log_n, "ms": ms, "per": per, "wall": wall,
+           "schedule": tuned, "kernels": kernels}
     # plain per-op path for the same product: per-stage launches, separate point-wise kernel
     fa, fb = mod.alloc(count), mod.alloc(count)
 
@@ -77,8 +94,21 @@ def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
 
     pms, _, _ = time_steps(ctx, stream, plain, max(2, steps // 4), 1)
     res["plain_ms"] = pms
+    # the timed product again (the plain path wrote `out` last)
+    mod.polymul(log_n, n, a, b, out, stream=stream)
+    stream.synchronize()
     del fa, fb
     return res
+
+
+def keygen(mod, log_n, t_mont):
+    """pk = (A, B) with A = B s + t e in NTT form (project-defined key generation: hmpc has none)"""
+    N = 1 << log_n
+    B = mod.sample_uniform(KEY, 0, N)
+    s = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 1, 2, N))
+    e = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 2, 40, N))
+    A = mod.pointwise(mod.ADD, N, mod.pointwise(mod.MUL, N, B, s), mod.pointwise_scalar(mod.MUL, N, e, t_mont))
+    return A, B, s
 
 
 def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
@@ -87,15 +117,17 @@ def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
     N = 1 << log_n
     log(f"[bench] C3 BGV: {n} ciphertexts, N=2^{log_n}")
     t_mont = mod.mont(1 << 32)
-    B = mod.sample_uniform(KEY, 0, N)
-    s = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 1, 2, N))
-    e = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 2, 40, N))
-    A = mod.pointwise(mod.ADD, N, mod.pointwise(mod.MUL, N, B, s), mod.pointwise_scalar(mod.MUL, N, e, t_mont))
+    A, B, s = keygen(mod, log_n, t_mont)
     msgs = ctx.sample_words(KEY, 3, n * N)
     c0, c1 = mod.alloc(n * N), mod.alloc(n * N)
+    tuned = mod.autotune_bgv(log_n, 128, stream=stream)  # 128 < resident workgroups: its launches have their own grid
+    log(f"[bench] autotuned BGV schedule: {tuned}")
     ctx.set_rng(ENC_KEY, [0, 0, 0])
+    ctx.set_timing(True)
     ms, per, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n, A, B, msgs, t_mont, c0=c0, c1=c1, stream=stream),
                             steps, warmup)
+    kernels = ctx.kernel_times()
+    ctx.set_timing(False)
     # decrypt-and-compare on the last batch (size-independent property)
     dec = mod.bgv_decrypt(log_n, n, s, c0, c1, stream=stream)
     stream.synchronize()
@@ -103,8 +135,8 @@ def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
     pc0, pc1 = mod.alloc(n_plain * N), mod.alloc(n_plain * N)
     pms, _, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n_plain, A, B, msgs, t_mont, c0=pc0, c1=pc1,
                                                                plain=True, stream=stream), 2, 1)
-    return {"n": n, "ms": ms, "per": per, "enc_per_s": n / (ms / 1e3), "decrypt_ok": ok,
-            "plain_enc_per_s": n_plain / (pms / 1e3), "plain_sample": n_plain}
+    return {"n": n, "ms": ms, "per": per, "enc_per_s": n / (ms / 1e3), "decrypt_ok": ok, "schedule": tuned,
+            "kernels": kernels, "plain_enc_per_s": n_plain / (pms / 1e3), "plain_sample": n_plain}
 
 
 def bench_c1(hm, ctx, stream, iters=200):
@@ -122,6 +154,94 @@ def bench_c1(hm, ctx, stream, iters=200):
     ok = np.array_equal(z.to_numpy((2, 4096)), x.to_numpy((2, 4096)))
     out["roundtrip_ok"] = bool(ok)
     return out
+
+
+def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
+    """Uneven batch: N from 2^12..2^16 (p = .3 .25 .2 .15 .1), q64 / q128a (50/50), product /
+    encryption (50/50); autotuned persistent launches replayed from a hipGraph vs the same jobs
+    launched one by one (hmpc's per-stage plain path, and the fused kernels per job)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([12, 13, 14, 15, 16], size=jobs, p=[0.3, 0.25, 0.2, 0.15, 0.1])
+    qsel = rng.integers(0, 2, size=jobs)
+    kind = rng.integers(0, 2, size=jobs)
+    mods = {0: hm.Modulus(ctx, hm.Q64), 1: hm.Modulus(ctx, hm.Q128A)}
+    log(f"[bench] C4 uneven batch: {jobs} jobs")
+    tuned = {}
+    keys = {}
+    for qi, m in mods.items():
+        for ln in range(12, 17):
+            m.prepare(ln)
+            tuned[f"{'q64' if qi == 0 else 'q128a'}/2^{ln}"] = {"polymul": m.autotune(ln, 16, stream=stream),
+                                                               "bgv": m.autotune_bgv(ln, 16, stream=stream)}
+            t_mont = m.mont(1 << 32)
+            A, B, _ = keygen(m, ln, t_mont)
+            keys[(qi, ln)] = (A, B, t_mont)
+    bufs = []
+    ctx.set_rng(ENC_KEY, [0, 0, 0])
+    batch = hm.Batch(ctx)
+    coeffs = 0
+    for j in range(jobs):
+        m, ln, N = mods[int(qsel[j])], int(sizes[j]), 1 << int(sizes[j])
+        if kind[j] == 0:
+            a = m.sample_uniform(KEY, 1000 + 2 * j, N)
+            b = m.sample_uniform(KEY, 1001 + 2 * j, N)
+            out = m.alloc(N)
+            batch.add_polymul(m, ln, a, b, out)
+            bufs.append(("mul", m, ln, (a, b, out)))
+            coeffs += 3 * N
+        else:
+            A, B, t_mont = keys[(int(qsel[j]), ln)]
+            msg = ctx.sample_words(KEY, 5000 + j, N)
+            c0, c1 = m.alloc(N), m.alloc(N)
+            batch.add_bgv_encrypt(m, ln, A, B, msg, t_mont, c0, c1)
+            bufs.append(("bgv", m, ln, (A, B, msg, t_mont, c0, c1)))
+            coeffs += 3 * N
+    batch.run(graph=True, stream=stream)  # build + capture
+    stream.synchronize()
+    g_ms, g_per, _ = time_steps(ctx, stream, lambda: batch.run(graph=True, stream=stream), 5, 1)
+    e_ms, _, _ = time_steps(ctx, stream, lambda: batch.run(graph=False, stream=stream), 3, 1)
+    info = batch.info()
+
+    def per_job(plain):
+        for kind_j, m, ln, t in bufs:
+            if kind_j == "mul":
+                a, b, out = t
+                if plain:
+                    N = 1 << ln
+                    fa, fb = out, scratch[m]
+                    m.ntt(ln, 1, a, fa, plain=True, stream=stream)
+                    m.ntt(ln, 1, b, fb, plain=True, stream=stream)
+                    m.pointwise(m.MUL, N, fa, fb, out=fa, stream=stream)
+                    m.ntt(ln, 1, fa, out, inverse=True, plain=True, stream=stream)
+                else:
+                    m.polymul(ln, 1, a, b, out, stream=stream)
+            else:
+                A, B, msg, t_mont, c0, c1 = t
+                m.bgv_encrypt(ln, 1, A, B, msg, t_mont, c0=c0, c1=c1, plain=plain, stream=stream)
+
+    scratch = {qi: m.alloc(1 << 16) for qi, m in mods.items()}
+    scratch = {m: scratch[qi] for qi, m in mods.items()}
+    f_ms, _, _ = time_steps(ctx, stream, lambda: per_job(False), 2, 1)
+    p_ms, _, _ = time_steps(ctx, stream, lambda: per_job(True), 1, 0)
+    return {"jobs": jobs, "graph_ms": g_ms, "graph_p50_ms": percentile(g_per, 50), "eager_ms": e_ms,
+            "per_job_fused_ms": f_ms, "plain_per_job_ms": p_ms, "coeffs": coeffs,
+            "gcoeff_per_s": coeffs / (g_ms / 1e3) / 1e9, "batched_vs_plain": p_ms / g_ms,
+            "batched_vs_per_job_fused": f_ms / g_ms, "classes": info["classes"],
+            "device_bytes": info["device_bytes"], "schedules": tuned}
+
+
+def bench_copy(hm, ctx, stream, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of a known size: the achievable-bandwidth reference for this box and
+    the calibration of the HBM byte counters (FETCH_SIZE / WRITE_SIZE of this copy are compared
+    with its 1 GiB read + 1 GiB write)."""
+    src, dst = ctx.buffer(nbytes), ctx.buffer(nbytes)
+    lib = hm.load()
+
+    def step():
+        hm.runtime.check(lib.hmpc_memcpy_d2d(ctx.handle, dst.ptr, src.ptr, nbytes, stream.handle))
+
+    ms, _, _ = time_steps(ctx, stream, step, reps, 1)
+    return {"bytes": nbytes, "ms": ms, "gbs": 2 * nbytes / (ms / 1e3) / 1e9}
 
 
 def cpu_baseline(pm, seconds_target=10.0):
@@ -150,6 +270,20 @@ def cpu_baseline(pm, seconds_target=10.0):
             "matches_gpu": bool(np.array_equal(got, want))}
 
 
+def traffic_for(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy kernel of known
+    size), or None when no summary covers this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(workload, {}).get(kernel)
+    return None if e is None else e.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +291,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3 (default all)")
+    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3,c4,copy (default all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the workloads shard by independent batch elements); "
@@ -168,7 +302,7 @@ def main():
 
     with open(os.path.join(ROOT, "BASELINE.json")) as f:
         baseline = json.load(f)
-    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3"}
+    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "copy"}
     ctx = hm.Context(0, key=KEY)
     stream = ctx.stream()
     info = ctx.info()
@@ -185,20 +319,26 @@ def main():
         L = pm["mod"].limbs
         coeffs = 3 * n * N
         value = coeffs / (pm["ms"] / 1e3) / 1e9
-        alg_bytes = 3 * N * L * 4 * n  # read a, b and write the product once
-        achieved = alg_bytes / (pm["ms"] / 1e3) / 1e9
+        unit_bytes = N * L * 4 * n  # one pass over the batch
         modmuls = n * (3 * (N // 2) * log_n + N + N // 2)
+        # dominant kernel: largest total time among the launches recorded in the timed region
+        kname, (klaunch, kavg) = max(pm["kernels"].items(), key=lambda kv: kv[1][0] * kv[1][1])
+        kbytes = KERNEL_PASSES.get(kname, 3) * unit_bytes
+        achieved = kbytes / (kavg / 1e3) / 1e9
         line.update({
             "value": value, "ms_per_step": pm["ms"],
             "p50_ms": percentile(pm["per"], 50), "p99_ms": percentile(pm["per"], 99),
             "config": {"workload": "batched polymul INTT(NTT(a) o NTT(b)): 512 pairs, N=2^16, "
-                                   "q=2^124+2^20+2^18+1 (4x32-bit limbs)", "pairs": n, "N": N, "limbs": L},
+                                   "q=2^124+2^20+2^18+1 (4x32-bit limbs)", "pairs": n, "N": N, "limbs": L,
+                       "schedule": pm["schedule"]},
             "plain_path": {"ms_per_step": pm["plain_ms"], "value": coeffs / (pm["plain_ms"] / 1e3) / 1e9,
                            "speedup_fused": pm["plain_ms"] / pm["ms"]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "polymul step (column pass, row pass, inverse column pass)",
-                         "algorithmic_bytes_per_step": alg_bytes},
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kname, "c2"),
+                         "kernel": kname, "kernel_avg_ms": kavg, "launches": klaunch,
+                         "algorithmic_bytes_per_launch": kbytes,
+                         "step_hbm_gbs": 3 * unit_bytes / (pm["ms"] / 1e3) / 1e9},
+            "kernels_ms": {k: v[1] for k, v in pm["kernels"].items()},
             "int_rate": {"modmul_per_s": modmuls / (pm["ms"] / 1e3), "modmuls_per_step": modmuls},
         })
     if "c3" in sections:
@@ -207,9 +347,14 @@ def main():
                        "enc_per_s": bg["enc_per_s"], "ms_per_batch": bg["ms"],
                        "p50_ms": percentile(bg["per"], 50), "p99_ms": percentile(bg["per"], 99),
                        "plain_enc_per_s": bg["plain_enc_per_s"], "plain_sample": bg["plain_sample"],
-                       "batched_vs_plain": bg["enc_per_s"] / bg["plain_enc_per_s"], "decrypt_ok": bg["decrypt_ok"]}
+                       "batched_vs_plain": bg["enc_per_s"] / bg["plain_enc_per_s"], "decrypt_ok": bg["decrypt_ok"],
+                       "schedule": bg["schedule"], "kernels_ms": {k: v[1] for k, v in bg["kernels"].items()}}
+    if "c4" in sections:
+        line["uneven_batch"] = bench_c4(hm, ctx, stream)
     if "c1" in sections:
         line["c1_latency"] = bench_c1(hm, ctx, stream)
+    if "copy" in sections:
+        line["copy_reference"] = bench_copy(hm, ctx, stream)
     if pm is not None and not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle, host cores)")
         line["cpu_baseline"] = cpu_baseline(pm, args.cpu_seconds)

@@ -114,9 +114,103 @@ namespace
     }
 }
 
+namespace
+{
+    // HIP events around each fast-path launch (launch_probe), aggregated per kernel name
+    struct event_probe final : launch_probe
+    {
+        struct record
+        {
+            char const* kernel;
+            hipEvent_t start, stop;
+        };
+        std::vector<record> open, done;
+        std::vector<hipEvent_t> pool;
+        std::vector<std::pair<std::string, std::pair<std::size_t, double>>> stats;
+
+        hipEvent_t take()
+        {
+            if (!pool.empty())
+            {
+                hipEvent_t e = pool.back();
+                pool.pop_back();
+                return e;
+            }
+            hipEvent_t e = nullptr;
+            return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+        }
+
+        void begin(hipStream_t s, char const* kernel) override
+        {
+            hipEvent_t e = take();
+            if (e && hipEventRecord(e, s) == hipSuccess)
+                open.push_back({kernel, e, nullptr});
+            else
+                open.push_back({kernel, nullptr, nullptr});
+        }
+
+        void end(hipStream_t s, char const* kernel) override
+        {
+            if (open.empty())
+                return;
+            record r = open.back();
+            open.pop_back();
+            hipEvent_t e = take();
+            if (r.start && e && hipEventRecord(e, s) == hipSuccess)
+            {
+                r.stop = e;
+                done.push_back(r);
+            }
+            (void)kernel;
+        }
+
+        // wait for the recorded launches and fold them into the per-kernel statistics
+        void collect()
+        {
+            for (auto& r : done)
+            {
+                float ms = 0.0f;
+                if (hipEventSynchronize(r.stop) == hipSuccess && hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess)
+                {
+                    auto it = std::find_if(stats.begin(), stats.end(), [&](auto const& x) { return x.first == r.kernel; });
+                    if (it == stats.end())
+                    {
+                        stats.push_back({r.kernel, {0, 0.0}});
+                        it = stats.end() - 1;
+                    }
+                    it->second.first += 1;
+                    it->second.second += ms;
+                }
+                pool.push_back(r.start);
+                pool.push_back(r.stop);
+            }
+            done.clear();
+        }
+
+        void reset()
+        {
+            collect();
+            stats.clear();
+        }
+
+        ~event_probe() override
+        {
+            for (auto& r : done)
+            {
+                (void)hipEventDestroy(r.start);
+                (void)hipEventDestroy(r.stop);
+            }
+            for (hipEvent_t e : pool)
+                (void)hipEventDestroy(e);
+        }
+    };
+}
+
 struct hmpc_context_s
 {
     int device = 0;
+    bool timing = false;
+    event_probe probe;
     u32 key[8] = {};
     u32 nonce[3] = {};
     hipDeviceProp_t props{};
@@ -180,9 +274,15 @@ struct hmpc_modulus_s
     std::map<int, table_pair> tables;
     std::map<int, int> elems; // fast path: elements per thread chosen per log_n (tuner)
     std::map<int, int> passes; // fast path: 1 = one launch per pass instead of the persistent job kernel
+    std::map<int, int> bgv_elems; // persistent BGV jobs: elements per thread per log_n (tuner)
     int occupancy[17] = {};   // persistent job kernel: workgroups per CU by elements per thread
 
     int default_elems(int) const { return limbs == 4 ? 4 : 16; }
+    int bgv_elems_for(int log_n) const
+    {
+        auto e = bgv_elems.find(log_n);
+        return e != bgv_elems.end() ? e->second : (limbs == 4 ? 4 : 8);
+    }
     bool persistent(int log_n) const
     {
         auto p = passes.find(log_n);
@@ -334,6 +434,17 @@ namespace
     struct device_guard
     {
         explicit device_guard(int device) { (void)hipSetDevice(device); }
+    };
+
+    // installs the context's timing probe for the duration of one entry point
+    struct probe_guard
+    {
+        launch_probe* previous;
+        explicit probe_guard(hmpc_context ctx) : previous(current_probe())
+        {
+            current_probe() = ctx && ctx->timing ? &ctx->probe : nullptr;
+        }
+        ~probe_guard() { current_probe() = previous; }
     };
 
     hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
@@ -767,6 +878,7 @@ int hmpc_ntt(hmpc_modulus mod, int log_n, size_t n, int inverse, const uint32_t*
     if (rc)
         return rc;
     device_guard g(mod->ctx->device);
+    probe_guard pg(mod->ctx);
     HMPC_HIP(dispatch_ntt(*mod, T, n, inverse != 0, in, out, as_stream(stream)));
     return HMPC_OK;
 }
@@ -1057,6 +1169,7 @@ int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const
     if (n > 0x7fffffffu)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
     device_guard g(mod->ctx->device);
+    probe_guard pg(mod->ctx);
     hipStream_t s = as_stream(stream);
     void* w = nullptr;
     bool fast = mod->fast(log_n);
@@ -1122,11 +1235,13 @@ int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, co
     if (n == 0)
         return HMPC_OK;
     device_guard g(ctx->device);
+    probe_guard pg(ctx);
     hipStream_t s = as_stream(stream);
     bgv_args B = make_bgv_args(ctx, mod, log_n, n, p_mont, bits_u, bits_v, bits_w);
     u64 p_int = 0;
-    if (mod->fast(log_n) && mod->persistent(log_n) && small_plaintext(*mod, p_mont, &p_int))
+    if (mod->fast(log_n) && small_plaintext(*mod, p_mont, &p_int))
     {
+        T.elems = mod->bgv_elems_for(log_n);
         job_desc J = bgv_job(T, B, p_int, log_n, pk_a, pk_b, messages, c0, c1, static_cast<u64>(n) << log_n, 0, 0);
         rc = run_uniform_jobs(*mod, T, J, n, s);
         if (rc)
@@ -1229,6 +1344,39 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
     return HMPC_OK;
 }
 
+// ------------------------------------------------------------------ kernel timing
+int hmpc_context_set_timing(hmpc_context ctx, int enabled)
+{
+    if (!ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null context");
+    device_guard g(ctx->device);
+    ctx->probe.reset();
+    ctx->timing = enabled != 0;
+    return HMPC_OK;
+}
+
+int hmpc_context_timing(hmpc_context ctx, int index, char* name, size_t name_len, size_t* launches, double* avg_ms)
+{
+    if (!ctx || index < 0)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
+    device_guard g(ctx->device);
+    ctx->probe.collect();
+    if (static_cast<std::size_t>(index) >= ctx->probe.stats.size())
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "no such kernel record");
+    auto const& st = ctx->probe.stats[static_cast<std::size_t>(index)];
+    if (name && name_len)
+    {
+        std::size_t n = std::min(name_len - 1, st.first.size());
+        std::memcpy(name, st.first.data(), n);
+        name[n] = 0;
+    }
+    if (launches)
+        *launches = st.second.first;
+    if (avg_ms)
+        *avg_ms = st.second.first ? st.second.second / static_cast<double>(st.second.first) : 0.0;
+    return HMPC_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ job batches
@@ -1299,7 +1447,8 @@ struct hmpc_batch_s
         {
             int log_n = static_cast<int>(e.d.log_n);
             auto it = e.mod->elems.find(log_n);
-            int elems = it != e.mod->elems.end() ? it->second : e.mod->default_elems(log_n);
+            int elems = e.d.type == job_bgv ? e.mod->bgv_elems_for(log_n)
+                                            : (it != e.mod->elems.end() ? it->second : e.mod->default_elems(log_n));
             auto key = std::make_pair(e.mod, elems);
             auto f = index.find(key);
             if (f == index.end())
@@ -1458,7 +1607,10 @@ int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream)
         }
     }
     if (!use_graph)
+    {
+        probe_guard pg(batch->ctx);
         return batch->launch(s);
+    }
     if (!batch->exec)
     {
         // capture on a private stream (the null stream cannot be captured); counters are reset
@@ -1585,6 +1737,103 @@ int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* 
         *elems_out = best_e;
     if (persistent_out)
         *persistent_out = best_p;
+    if (ms_out)
+        *ms_out = best;
+    return HMPC_OK;
+}
+
+int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, void* stream, int* elems_out, float* ms_out)
+{
+    if (!ctx || !mod || n == 0 || n > (size_t{1} << 16))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
+    transform_args T;
+    int rc = mod->args(log_n, T);
+    if (rc)
+        return rc;
+    if (!mod->fast(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "no tunable schedule: this modulus uses the portable kernels");
+    device_guard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    std::size_t N = std::size_t{1} << log_n;
+    std::size_t L = static_cast<std::size_t>(mod->limbs);
+    u32* buf = nullptr;
+    HMPC_HIP(hipMalloc(&buf, (2 * L * N + n * N + 2 * L * n * N) * 4));
+    u32* pk_a = buf;
+    u32* pk_b = pk_a + L * N;
+    u32* msg = pk_b + L * N;
+    u32* c0 = msg + n * N;
+    u32* c1 = c0 + L * n * N;
+    chacha_key key{};
+    key.k[0] = 0x7476u;
+    u32 p_mont[max_limbs] = {};
+    host_limbs(mod->limbs, [&](auto LC) {
+        constexpr int LL = decltype(LC)::value;
+        u32 v[LL] = {}, r2v[LL], r[LL];
+        v[0] = 0;
+        if constexpr (LL >= 2)
+            v[1] = 1;  // 2^32
+        for (int i = 0; i < LL; ++i)
+            r2v[i] = mod->r2[i];
+        mod_mul<LL>(r, v, r2v, mod->P);
+        std::copy(r, r + LL, p_mont);
+        return 0;
+    });
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    u32 saved_nonce[3];
+    std::copy(ctx->nonce, ctx->nonce + 3, saved_nonce);
+    auto saved = mod->bgv_elems;
+    auto cleanup = [&]() {
+        std::copy(saved_nonce, saved_nonce + 3, ctx->nonce);
+        if (e0)
+            (void)hipEventDestroy(e0);
+        if (e1)
+            (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(buf);
+    };
+    if (launch_sample_uniform(mod->limbs, mod->P, mod->bit_size, key, 0, N, pk_a, s) != hipSuccess ||
+        launch_sample_uniform(mod->limbs, mod->P, mod->bit_size, key, 1, N, pk_b, s) != hipSuccess ||
+        launch_sample_words(key, 2, n * N, msg, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+    {
+        cleanup();
+        return fail(HMPC_ERR_DEVICE, "autotune set-up failed");
+    }
+    int const cand_l4[2] = {4, 8}, cand_l2[2] = {8, 16};
+    int const* cand = mod->limbs == 4 ? cand_l4 : cand_l2;
+    int best_e = 0;
+    float best = 0.0f;
+    for (int ci = 0; ci < 2; ++ci)
+    {
+        mod->bgv_elems[log_n] = cand[ci];
+        rc = hmpc_bgv_encrypt(ctx, mod, log_n, n, pk_a, pk_b, msg, p_mont, 2, 40, 40, c0, c1, s);
+        for (int r = 0; rc == HMPC_OK && r < 3; ++r)
+        {
+            if (r == 0)
+                (void)hipEventRecord(e0, s);
+            rc = hmpc_bgv_encrypt(ctx, mod, log_n, n, pk_a, pk_b, msg, p_mont, 2, 40, 40, c0, c1, s);
+        }
+        float ms = 0.0f;
+        if (rc == HMPC_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(HMPC_ERR_DEVICE, "autotune timing failed");
+        if (rc)
+        {
+            mod->bgv_elems = saved;
+            cleanup();
+            return rc;
+        }
+        ms /= 3.0f;
+        if (best_e == 0 || ms < best)
+        {
+            best = ms;
+            best_e = cand[ci];
+        }
+    }
+    mod->bgv_elems[log_n] = best_e;
+    cleanup();
+    if (elems_out)
+        *elems_out = best_e;
     if (ms_out)
         *ms_out = best;
     return HMPC_OK;

@@ -27,7 +27,10 @@
 #include "fastmath.hpp"
 #include "r26.hpp"
 
+#include <mutex>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace hmpc_amd
 {
@@ -706,7 +709,11 @@ namespace hmpc_amd
                 return G;
             }
             G.single = false;
-            G.log_n1 = log_n / 2;
+            // column tiles of at least 32 columns: each tile row is then 128 contiguous bytes per
+            // limb plane (whole cache lines; 16-column tiles fetched every line twice)
+            G.log_n1 = log_n / 2 < 7 ? log_n / 2 : 7;
+            if (log_n - G.log_n1 > tile_log)
+                G.log_n1 = log_n - tile_log;
             G.log_n2 = log_n - G.log_n1;
             if (rows_pass)
             {
@@ -850,14 +857,24 @@ namespace hmpc_amd
             }
         }
 
-        // dynamic LDS above the 64 KiB default needs the kernel attribute raised
+        // dynamic LDS above the 64 KiB default needs the kernel attribute raised (once per kernel
+        // and process; the attribute call is not free)
         template<typename K>
         hipError_t allow_lds(K kernel, std::size_t bytes)
         {
             if (bytes <= 65536)
                 return hipSuccess;
-            return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       static_cast<int>(bytes));
+            static std::mutex m;
+            static std::vector<std::pair<const void*, std::size_t>> done;
+            const void* f = reinterpret_cast<const void*>(kernel);
+            std::lock_guard<std::mutex> lock(m);
+            for (auto const& d : done)
+                if (d.first == f && d.second >= bytes)
+                    return hipSuccess;
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+            if (e == hipSuccess)
+                done.push_back({f, bytes});
+            return e;
         }
 
         template<typename A>
@@ -878,13 +895,19 @@ namespace hmpc_amd
                     {
                         if ((e = allow_lds(k_pass<A, E, 4>, bytes)) != hipSuccess)
                             return e;
-                        k_pass<A, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                        {
+                            probe_scope ps(s, "fast ntt forward (whole tile)");
+                            k_pass<A, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                        }
                     }
                     else
                     {
                         if ((e = allow_lds(k_pass<A, E, 5>, bytes)) != hipSuccess)
                             return e;
-                        k_pass<A, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                        {
+                            probe_scope ps(s, "fast ntt inverse (whole tile)");
+                            k_pass<A, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+                        }
                     }
                     return hipGetLastError();
                 }
@@ -896,15 +919,27 @@ namespace hmpc_amd
                 {
                     if ((e = allow_lds(k_pass<A, E, 0>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 1>, bytes)) != hipSuccess)
                         return e;
-                    k_pass<A, E, 0><<<grid_c, TH, bytes, s>>>(in, out, tw, gc, n, P);
-                    k_pass<A, E, 1><<<grid_r, TH, bytes, s>>>(out, out, tw, gr, n, P);
+                    {
+                        probe_scope ps(s, "fast ntt forward columns");
+                        k_pass<A, E, 0><<<grid_c, TH, bytes, s>>>(in, out, tw, gc, n, P);
+                    }
+                    {
+                        probe_scope ps(s, "fast ntt forward rows");
+                        k_pass<A, E, 1><<<grid_r, TH, bytes, s>>>(out, out, tw, gr, n, P);
+                    }
                 }
                 else
                 {
                     if ((e = allow_lds(k_pass<A, E, 2>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 3>, bytes)) != hipSuccess)
                         return e;
-                    k_pass<A, E, 2><<<grid_r, TH, bytes, s>>>(in, out, tw, gr, n, P);
-                    k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(out, out, tw, gc, n, P);
+                    {
+                        probe_scope ps(s, "fast ntt inverse rows");
+                        k_pass<A, E, 2><<<grid_r, TH, bytes, s>>>(in, out, tw, gr, n, P);
+                    }
+                    {
+                        probe_scope ps(s, "fast ntt inverse columns");
+                        k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(out, out, tw, gc, n, P);
+                    }
                 }
                 return hipGetLastError();
             });
@@ -925,7 +960,10 @@ namespace hmpc_amd
                     unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
                     if ((e = allow_lds(k_polymul<A, E, true>, bytes)) != hipSuccess)
                         return e;
-                    k_polymul<A, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_fast, T.inv_fast, G, n, P);
+                    {
+                        probe_scope ps(s, "fast polymul (whole tile)");
+                        k_polymul<A, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_fast, T.inv_fast, G, n, P);
+                    }
                     return hipGetLastError();
                 }
                 std::size_t count = n << T.log_n;
@@ -938,9 +976,18 @@ namespace hmpc_amd
                 if ((e = allow_lds(k_polymul_cols<A, E>, bytes)) != hipSuccess ||
                     (e = allow_lds(k_polymul<A, E, false>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 3>, bytes)) != hipSuccess)
                     return e;
-                k_polymul_cols<A, E><<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
-                k_polymul<A, E, false><<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
-                k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
+                {
+                    probe_scope ps(s, "fast polymul forward columns");
+                    k_polymul_cols<A, E><<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
+                }
+                {
+                    probe_scope ps(s, "fast polymul rows");
+                    k_polymul<A, E, false><<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
+                }
+                {
+                    probe_scope ps(s, "fast polymul inverse columns");
+                    k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
+                }
                 return hipGetLastError();
             });
         }

@@ -467,7 +467,10 @@ namespace hmpc_amd
                 if (e != hipSuccess)
                     return e;
                 k_reset_counter<A><<<1, 64, 0, s>>>(counter);
-                k_jobs<A, E><<<grid, tile / E, bytes, s>>>(table, base, njobs, counter, scratch, slot_words, P);
+                {
+                    probe_scope ps(s, "fast persistent jobs");
+                    k_jobs<A, E><<<grid, tile / E, bytes, s>>>(table, base, njobs, counter, scratch, slot_words, P);
+                }
                 return hipGetLastError();
             });
         }

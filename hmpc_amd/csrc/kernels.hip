@@ -19,6 +19,15 @@
 
 namespace hmpc_amd
 {
+    launch_probe*& current_probe()
+    {
+        thread_local launch_probe* probe = nullptr;
+        return probe;
+    }
+}
+
+namespace hmpc_amd
+{
     namespace
     {
         constexpr int block_threads = 256;

@@ -10,6 +10,33 @@
 
 namespace hmpc_amd
 {
+    // Optional per-launch timing. While a probe is installed for the calling thread, the fast
+    // path brackets each kernel launch with begin/end callbacks on the launch stream (the
+    // library's timing probe records HIP events there).
+    struct launch_probe
+    {
+        virtual void begin(hipStream_t s, char const* kernel) = 0;
+        virtual void end(hipStream_t s, char const* kernel) = 0;
+        virtual ~launch_probe() = default;
+    };
+    launch_probe*& current_probe();
+
+    struct probe_scope
+    {
+        hipStream_t s;
+        char const* kernel;
+        probe_scope(hipStream_t stream, char const* name) : s(stream), kernel(name)
+        {
+            if (launch_probe* p = current_probe())
+                p->begin(s, kernel);
+        }
+        ~probe_scope()
+        {
+            if (launch_probe* p = current_probe())
+                p->end(s, kernel);
+        }
+    };
+
     // Everything a kernel needs to know about a modulus at one transform size.
     struct transform_args
     {

@@ -1,0 +1,134 @@
+"""Summaries of rocprofv3 result databases (rocpd SQLite files) for the benchmark workloads.
+
+    python -m hmpc_amd.profiling kernels RESULTS.db [--symbols a,b] > summary.txt
+        per kernel symbol and grid size: dispatches, average / min / max duration
+
+    python -m hmpc_amd.profiling traffic FETCH.db WRITE.db --workload c2 --copy-bytes N [--out traffic.json]
+        HBM bytes per dispatch of the workload's kernels from a FETCH_SIZE run and a WRITE_SIZE
+        run, each scaled by what the same counter reads for the device copy of N bytes in the
+        same run (the copy reads N and writes N bytes)
+
+The traffic file maps bench kernel names (the names of the library's launch probes) to bytes
+per launch; bench.py reports it as roofline.traffic.
+"""
+
+from __future__ import annotations
+
+import argparse
+import collections
+import json
+import sqlite3
+import sys
+
+# launch-probe name (bench.py) -> kernel symbol prefix of the workload's dispatches
+WORKLOAD_KERNELS = {
+    "c2": {
+        "fast polymul forward columns": "hmpc_amd::fast::k_polymul_cols<",
+        "fast polymul rows": "hmpc_amd::fast::k_polymul<",        # the two-pass product (N > 4096)
+        "fast polymul inverse columns": "hmpc_amd::fast::k_pass<",  # the only per-pass transform of the step
+    },
+    "c3": {
+        "fast persistent jobs": "hmpc_amd::fast::k_jobs<",
+    },
+}
+COPY_SYMBOL = "__amd_rocclr_copyBuffer"
+
+
+def _short(name: str) -> str:
+    name = name.split("(")[0]
+    return name[5:] if name.startswith("void ") else name
+
+
+def kernel_rows(db_path: str):
+    cur = sqlite3.connect(db_path).cursor()
+    rows = cur.execute("select name, grid_x, workgroup_x, duration from kernels").fetchall()
+    groups = collections.defaultdict(list)
+    for name, grid, wg, dur in rows:
+        groups[(_short(name), int(grid), int(wg))].append(float(dur))
+    return groups
+
+
+def counter_rows(db_path: str, counter: str):
+    cur = sqlite3.connect(db_path).cursor()
+    rows = cur.execute("select kernel_name, grid_size, value from counters_collection where counter_name = ?",
+                       (counter,)).fetchall()
+    groups = collections.defaultdict(list)
+    for name, grid, value in rows:
+        groups[(_short(name), int(grid))].append(float(value))
+    return groups
+
+
+def cmd_kernels(args):
+    groups = kernel_rows(args.db)
+    wanted = [s for s in args.symbols.split(",") if s] if args.symbols else None
+    print(f"{'kernel':<78} {'grid':>10} {'wg':>5} {'calls':>6} {'avg us':>10} {'min us':>10} {'max us':>10}")
+    for (name, grid, wg), d in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        if wanted and not any(name.startswith(w) for w in wanted):
+            continue
+        print(f"{name[:78]:<78} {grid:>10} {wg:>5} {len(d):>6} {sum(d) / len(d) / 1e3:>10.1f} {min(d) / 1e3:>10.1f} "
+              f"{max(d) / 1e3:>10.1f}")
+
+
+def _largest(groups, prefix):
+    cands = [(g, v) for (n, g), v in groups.items() if n.startswith(prefix)]
+    if not cands:
+        return None, None
+    g, v = max(cands, key=lambda c: c[0])
+    return g, sum(v) / len(v)
+
+
+def cmd_traffic(args):
+    fetch = counter_rows(args.fetch_db, "FETCH_SIZE")
+    write = counter_rows(args.write_db, "WRITE_SIZE")
+    _, copy_fetch = _largest(fetch, COPY_SYMBOL)
+    _, copy_write = _largest(write, COPY_SYMBOL)
+    if not copy_fetch or not copy_write:
+        sys.exit("no copy dispatch in the counter runs")
+    # counters are KiB per dispatch; the copy moves copy_bytes in each direction
+    fscale = args.copy_bytes / (copy_fetch * 1024.0)
+    wscale = args.copy_bytes / (copy_write * 1024.0)
+    out = {}
+    try:
+        with open(args.out) as f:
+            out = json.load(f)
+    except (OSError, ValueError):
+        pass
+    entries = {}
+    lines = [f"copy calibration: {args.copy_bytes} B copy reads FETCH_SIZE {copy_fetch:.0f} KiB, WRITE_SIZE "
+             f"{copy_write:.0f} KiB -> scale fetch x{fscale:.3f}, write x{wscale:.3f}",
+             f"{'bench kernel':<32} {'grid':>10} {'fetch B':>14} {'write B':>14} {'total B':>14}"]
+    for bench_name, prefix in WORKLOAD_KERNELS[args.workload].items():
+        g, f = _largest(fetch, prefix)
+        _, w = _largest(write, prefix)
+        if f is None or w is None:
+            continue
+        fb, wb = f * 1024.0 * fscale, w * 1024.0 * wscale
+        entries[bench_name] = {"bytes_per_launch": fb + wb, "fetch_bytes": fb, "write_bytes": wb, "grid": g,
+                               "fetch_scale": fscale, "write_scale": wscale}
+        lines.append(f"{bench_name:<32} {g:>10} {fb:>14.0f} {wb:>14.0f} {fb + wb:>14.0f}")
+    out[args.workload] = entries
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+            f.write("\n")
+    print("\n".join(lines))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("db")
+    k.add_argument("--symbols", default="")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_db")
+    t.add_argument("write_db")
+    t.add_argument("--workload", required=True, choices=sorted(WORKLOAD_KERNELS))
+    t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--out", default="")
+    args = ap.parse_args(argv)
+    (cmd_kernels if args.cmd == "kernels" else cmd_traffic)(args)
+
+
+if __name__ == "__main__":
+    main()

@@ -46,6 +46,7 @@ SIGNATURES = {
     "hmpc_modulus_set_schedule": (_i, [_vp, _i, _i, _i]),
     "hmpc_modulus_get_schedule": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "hmpc_autotune": (_i, [_vp, _i, _sz, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
+    "hmpc_autotune_bgv": (_i, [_vp, _vp, _i, _sz, _vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
     "hmpc_modulus_root": (_i, [_vp, _i, _i, _u32p]),
     "hmpc_modulus_twiddles": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
     "hmpc_ntt": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp]),
@@ -68,6 +69,8 @@ SIGNATURES = {
     "hmpc_batch_add_bgv_encrypt": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp]),
     "hmpc_batch_run": (_i, [_vp, _i, _vp]),
     "hmpc_batch_info": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "hmpc_context_set_timing": (_i, [_vp, _i]),
+    "hmpc_context_timing": (_i, [_vp, _i, ctypes.c_char_p, _sz, ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_double)]),
     "hmpc_event_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "hmpc_event_destroy": (_i, [_vp, _vp]),
     "hmpc_event_record": (_i, [_vp, _vp, _vp]),
@@ -219,6 +222,23 @@ class Context:
     def upload(self, a: np.ndarray) -> Buffer:
         return Buffer.from_numpy(self, a)
 
+    def set_timing(self, enabled: bool):
+        """Record HIP events around every fast-kernel launch (clears earlier records)."""
+        check(_lib.hmpc_context_set_timing(self.handle, int(enabled)))
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (launches, average ms)} of the launches recorded since set_timing."""
+        out = {}
+        name = ctypes.create_string_buffer(128)
+        i = 0
+        while True:
+            n, ms = _sz(), ctypes.c_double()
+            if _lib.hmpc_context_timing(self.handle, i, name, 128, ctypes.byref(n), ctypes.byref(ms)) != 0:
+                break
+            out[name.value.decode()] = (n.value, ms.value)
+            i += 1
+        return out
+
     def stream(self) -> "Stream":
         return Stream(self)
 
@@ -330,6 +350,11 @@ class Modulus:
         e, p, f = _i(), _i(), _i()
         check(_lib.hmpc_modulus_get_schedule(self.handle, log_n, ctypes.byref(e), ctypes.byref(p), ctypes.byref(f)))
         return {"elems_per_thread": e.value, "persistent": bool(p.value), "fast": bool(f.value)}
+
+    def autotune_bgv(self, log_n: int, n: int, stream=None) -> dict:
+        e, ms = _i(), ctypes.c_float()
+        check(_lib.hmpc_autotune_bgv(self.ctx.handle, self.handle, log_n, n, _sp(stream), ctypes.byref(e), ctypes.byref(ms)))
+        return {"elems_per_thread": e.value, "ms": ms.value}
 
     def autotune(self, log_n: int, pairs: int, stream=None) -> dict:
         e, p, ms = _i(), _i(), ctypes.c_float()

@@ -107,6 +107,12 @@ int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread
  * hmpc, whose kernels have no schedule. */
 int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* elems_per_thread, int* persistent,
                   float* ms);
+/* Autotuner of batched BGV encryption (always the persistent job kernel on the fast moduli):
+ * times n synthetic encryptions of N = 2^log_n per elements-per-thread choice and installs the
+ * fastest for hmpc_bgv_encrypt and batch jobs of this modulus and N. The context's RNG state is
+ * restored afterwards. Blocking. */
+int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, void* stream, int* elems_per_thread,
+                      float* ms);
 /* Build (once) and keep resident the forward/inverse tables for N = 2^log_n (2 <= log_n <= 20),
  * 2N must divide q - 1. Blocking. */
 int hmpc_modulus_prepare(hmpc_modulus mod, int log_n);
@@ -209,6 +215,16 @@ int hmpc_batch_add_bgv_encrypt(hmpc_batch batch, hmpc_modulus mod, int log_n, co
 int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream);
 /* jobs, kernel classes (after the first run), device bytes held (tables, counters, scratch) */
 int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* device_bytes);
+
+/* ------------------------------------------------------------------ kernel timing
+ * Per-kernel durations of the fast kernels, measured with HIP events recorded around every launch
+ * on the stream it runs on (not inside graph capture). Enabling (or disabling) clears the
+ * records. hmpc_context_timing reports the index-th kernel name seen so far (name_len bytes with
+ * the terminating zero), its launch count and average duration in ms; it synchronises on the
+ * recorded events and fails with HMPC_ERR_INVALID_ARGUMENT past the last kernel. No counterpart
+ * in hmpc (profiling of the SYCL queue is left to the SYCL runtime there). */
+int hmpc_context_set_timing(hmpc_context ctx, int enabled);
+int hmpc_context_timing(hmpc_context ctx, int index, char* name, size_t name_len, size_t* launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ timing helpers
  * HIP events on a given stream (used by benchmarks to time kernels on the stream they run on). */

@@ -201,15 +201,15 @@ def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     want0, want1, nonce = om.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, [0, 0, 0], threads=8)
     dA, dB, dm = gm.upload(A), gm.upload(B), ctx.upload(msgs)
     pm = oracle.int_to_limbs(t_mont, gm.limbs)
-    for mode in ("jobs", "passes", "plain"):
+    gp, _ = mods(ctx, q, 1)
+    for mode in ("jobs", "portable", "plain"):
         say(f"  bgv mode={mode}")
-        gm.set_schedule(log_n, 0, 0 if mode == "passes" else 1)
         ctx.set_rng(enc_key, [0, 0, 0])
-        c0, c1 = gm.bgv_encrypt(log_n, n, dA, dB, dm, pm, plain=mode == "plain")
+        g = gp if mode == "portable" else gm
+        c0, c1 = g.bgv_encrypt(log_n, n, dA, dB, dm, pm, plain=mode == "plain")
         assert np.array_equal(gm.download(c0, n * N), want0), f"c0 mode={mode}"
         assert np.array_equal(gm.download(c1, n * N), want1), f"c1 mode={mode}"
         assert ctx.rng[1].tolist() == nonce.tolist()
-    gm.set_schedule(log_n, 0, 1)
     dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
     assert np.array_equal(dec, msgs)
     assert np.array_equal(dec, om.bgv_decrypt(log_n, s, want0, want1))
@@ -287,6 +287,24 @@ def test_fast_kernel_schedules(ctx, q, log_n, n):
             got = gm.download(gm.polymul(log_n, n, da, db), count)
             assert np.array_equal(got, want_p), f"polymul elems={elems} persistent={persistent}"
     gm.set_schedule(log_n, 0, 1)
+
+
+def test_autotune_bgv_keeps_results_and_rng(ctx):
+    gm, om = mods(ctx, Q128B)
+    log_n, n = 14, 4
+    say(f"autotune bgv q={hex(Q128B)[:12]} log_n={log_n} n={n}")
+    enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
+    ctx.set_rng(enc_key, [5, 0, 0])
+    best = gm.autotune_bgv(log_n, n)
+    assert best["elems_per_thread"] in (4, 8) and best["ms"] > 0
+    assert ctx.rng[1].tolist() == [5, 0, 0]
+    N = 1 << log_n
+    t_mont = om.from_unsigned(1 << 32)
+    A, B, _ = _keygen(om, log_n, t_mont)
+    msgs = oracle.sample_words(KEY, 3, n * N)
+    want0, want1, _ = om.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, [5, 0, 0], threads=8)
+    c0, c1 = gm.bgv_encrypt(log_n, n, gm.upload(A), gm.upload(B), ctx.upload(msgs), oracle.int_to_limbs(t_mont, gm.limbs))
+    assert np.array_equal(gm.download(c0, n * N), want0) and np.array_equal(gm.download(c1, n * N), want1)
 
 
 def test_autotune_installs_a_correct_schedule(ctx):
