@@ -230,6 +230,99 @@ def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
             "device_bytes": info["device_bytes"], "schedules": tuned}
 
 
+def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0)):
+    """Serving: 16 resident BGV keys (q128a, N = 2^13..2^16, four keys per N); each request
+    encrypts k ~ U[1, 64] messages under a uniformly chosen key. Peak = closed-loop throughput with
+    every request queued at once; then open-loop Poisson arrivals at each fraction of the peak
+    rate. Latency runs from the scheduled arrival to completion (client lateness included)."""
+    mod = hm.Modulus(ctx, hm.Q128A)
+    t_mont = mod.mont(1 << 32)
+    log(f"[bench] C5 serving: {requests} requests per load point")
+    keys = []
+    for ln in (13, 14, 15, 16):
+        mod.prepare(ln)
+        mod.autotune_bgv(ln, 64, stream=stream)
+    srv = hm.Server(ctx, budget_bytes=200 << 30, slots=3)
+    pk = {}
+    for i in range(16):
+        ln = 13 + i % 4
+        N = 1 << ln
+        B = mod.sample_uniform(KEY, 100 + 3 * i, N)
+        s = mod.ntt(ln, 1, mod.sample_cbd(KEY, 101 + 3 * i, 2, N))
+        e = mod.ntt(ln, 1, mod.sample_cbd(KEY, 102 + 3 * i, 40, N))
+        A = mod.pointwise(mod.ADD, N, mod.pointwise(mod.MUL, N, B, s), mod.pointwise_scalar(mod.MUL, N, e, t_mont))
+        pk[i] = (A, B, ln)
+        keys.append(srv.add_key(mod, ln, A, B, t_mont))
+    pool = ctx.sample_words(KEY, 7, 64 << 16)
+    rng = np.random.default_rng(seed)
+
+    def draw(n):
+        return [(int(rng.integers(0, 16)), int(rng.integers(1, 65))) for _ in range(n)]
+
+    tag = [0]
+
+    def run(reqs, arrivals):
+        tag[0] += 1
+        lateness, lat, pending = {}, {}, set()
+        t0 = time.perf_counter()
+        i = 0
+        while i < len(reqs) or pending:
+            now = time.perf_counter() - t0
+            while i < len(reqs) and arrivals[i] <= now:
+                key, k = reqs[i]
+                rid = srv.submit_encrypt(keys[key], k, pool.ptr, [0, i, tag[0]])  # one nonce per request
+                lateness[rid] = now - arrivals[i]
+                pending.add(rid)
+                i += 1
+                now = time.perf_counter() - t0
+            got = srv.completed()
+            for rid, us in got:
+                lat[rid] = us / 1e3 + lateness[rid] * 1e3
+                srv.release(rid)
+                pending.discard(rid)
+            if not got and (i >= len(reqs) or arrivals[i] > now + 50e-6):
+                time.sleep(20e-6)
+        wall = time.perf_counter() - t0
+        return wall, np.array(list(lat.values()))
+
+    # warm-up (pool growth, code objects)
+    reqs = draw(200)
+    run(reqs, [0.0] * len(reqs))
+    reqs = draw(requests)
+    cts = sum(k for _, k in reqs)
+    wall, _ = run(reqs, [0.0] * len(reqs))
+    peak_rps = len(reqs) / wall
+    points = []
+    for f in fractions:
+        rate = f * peak_rps
+        reqs = draw(requests)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+        wall, lat = run(reqs, arrivals)
+        done_rate = len(reqs) / max(wall, arrivals[-1])
+        points.append({"fraction": f, "offered_req_per_s": rate, "achieved_req_per_s": done_rate,
+                       "ct_per_s": sum(k for _, k in reqs) / wall, "p50_ms": percentile(lat, 50),
+                       "p99_ms": percentile(lat, 99)})
+    ok = [p for p in points if p["achieved_req_per_s"] >= 0.8 * peak_rps]
+    op = max(ok, key=lambda p: p["fraction"]) if ok else points[-1]
+    st = srv.stats()
+    srv.close()
+    # plain path: the same kind of requests, one per call through hmpc's launch structure
+    sample = draw(8)
+    c0, c1 = mod.alloc(64 << 16), mod.alloc(64 << 16)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    n_ct = 0
+    for key, k in sample:
+        A, B, ln = pk[key]
+        mod.bgv_encrypt(ln, k, A, B, pool, t_mont, c0=c0, c1=c1, plain=True, stream=stream)
+        n_ct += k
+    stream.synchronize()
+    plain_ct = n_ct / (time.perf_counter() - t0)
+    return {"requests_per_point": requests, "peak_req_per_s": peak_rps, "peak_ct_per_s": cts / (len(reqs) / peak_rps),
+            "operating_point": op, "points": points, "plain_ct_per_s": plain_ct, "plain_sample_requests": len(sample),
+            "launches": st["launches"], "budget_bytes": 200 << 30, "slots": 3}
+
+
 def bench_copy(hm, ctx, stream, nbytes=1 << 30, reps=5):
     """Device-to-device copy of a known size: the achievable-bandwidth reference for this box and
     the calibration of the HBM byte counters (FETCH_SIZE / WRITE_SIZE of this copy are compared
@@ -291,7 +384,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3,c4,copy (default all)")
+    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3,c4,c5,copy (default all)")
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="requests per load point of the serving sweep (the full workload uses 100000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the workloads shard by independent batch elements); "
@@ -302,7 +397,7 @@ def main():
 
     with open(os.path.join(ROOT, "BASELINE.json")) as f:
         baseline = json.load(f)
-    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "copy"}
+    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "c5", "copy"}
     ctx = hm.Context(0, key=KEY)
     stream = ctx.stream()
     info = ctx.info()
@@ -351,6 +446,8 @@ def main():
                        "schedule": bg["schedule"], "kernels_ms": {k: v[1] for k, v in bg["kernels"].items()}}
     if "c4" in sections:
         line["uneven_batch"] = bench_c4(hm, ctx, stream)
+    if "c5" in sections:
+        line["serving"] = bench_c5(hm, ctx, stream, requests=args.serving_requests)
     if "c1" in sections:
         line["c1_latency"] = bench_c1(hm, ctx, stream)
     if "copy" in sections:

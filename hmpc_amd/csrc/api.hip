@@ -9,6 +9,9 @@
 #include "../../include/hmpc_hip.h"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -18,6 +21,9 @@
 #include <random>
 #include <string>
 #include <type_traits>
+#include <deque>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 using namespace hmpc_amd;
@@ -1868,6 +1874,660 @@ int hmpc_event_elapsed_ms(hmpc_context ctx, void* start, void* stop, float* ms)
     device_guard g(ctx->device);
     HMPC_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
     HMPC_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return HMPC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving loop
+// A worker thread owns a few streams ("slots"). Requests queue in arrival order; whenever a
+// slot is free the worker admits queued requests while their outputs fit the HBM budget and
+// packs their ciphertexts / pairs into one persistent-kernel launch on that slot. Launches on
+// different slots overlap on the device (the next launch's workgroups take CUs as the previous
+// launch's workgroups finish: continuous batching at workgroup granularity). A host function
+// queued behind each launch timestamps its completion. Memory comes back when the client
+// releases a finished request.
+namespace
+{
+    using server_clock = std::chrono::steady_clock;
+
+    // request outputs: power-of-two blocks (>= 1 MiB) carved from 1 GiB chunks and recycled
+    struct block_pool
+    {
+        std::vector<void*> chunks;
+        char* cur = nullptr;
+        std::size_t left = 0;
+        std::map<std::size_t, std::vector<void*>> free_lists;
+
+        static std::size_t class_of(std::size_t bytes)
+        {
+            std::size_t c = std::size_t{1} << 20;
+            while (c < bytes)
+                c <<= 1;
+            return c;
+        }
+
+        hipError_t take(std::size_t cls, void** out)
+        {
+            auto& fl = free_lists[cls];
+            if (!fl.empty())
+            {
+                *out = fl.back();
+                fl.pop_back();
+                return hipSuccess;
+            }
+            if (left < cls)
+            {
+                std::size_t bytes = std::max(std::size_t{1} << 30, cls);
+                void* p = nullptr;
+                hipError_t e = hipMalloc(&p, bytes);
+                if (e != hipSuccess)
+                    return e;
+                chunks.push_back(p);
+                cur = static_cast<char*>(p);
+                left = bytes;
+            }
+            *out = cur;
+            cur += cls;
+            left -= cls;
+            return hipSuccess;
+        }
+
+        void give(std::size_t cls, void* p) { free_lists[cls].push_back(p); }
+
+        void release_all()
+        {
+            for (void* c : chunks)
+                (void)hipFree(c);
+            chunks.clear();
+            free_lists.clear();
+            cur = nullptr;
+            left = 0;
+        }
+    };
+
+    struct server_key
+    {
+        hmpc_modulus mod;
+        int log_n;
+        u32 const* pk_a;
+        u32 const* pk_b;
+        u64 p_int;
+        int bits[3];
+    };
+
+    enum request_state : int
+    {
+        req_queued = 0,
+        req_running = 1,
+        req_done = 2,
+        req_failed = 3,
+    };
+
+    struct server_request
+    {
+        int kind = 0;  // 0 encrypt, 1 product
+        hmpc_modulus mod = nullptr;
+        int log_n = 0;
+        std::size_t n = 0;  // ciphertexts or pairs
+        int key = -1;
+        u32 const* messages = nullptr;
+        u32 nonce[3] = {};
+        u32 const* a = nullptr;
+        u32 const* b = nullptr;
+        std::size_t bytes = 0;
+        std::size_t cls = 0;
+        void* out = nullptr;
+        int state = req_queued;
+        server_clock::time_point t_submit{}, t_done{};
+    };
+
+    struct server_slot
+    {
+        hipStream_t stream = nullptr;
+        job_desc* d_table = nullptr;
+        job_desc* h_table = nullptr;  // pinned staging of the job table
+        std::size_t table_cap = 0;
+        u32* counters = nullptr;      // one 256-byte counter per class
+        u32* scratch = nullptr;
+        std::size_t scratch_words = 0;
+        std::vector<u64> requests;
+        bool busy = false;
+        std::atomic<bool> done{false};
+        std::atomic<long long> done_ns{0};
+        std::condition_variable* wake = nullptr;
+    };
+
+    // host function queued behind a slot's launch (runs on the runtime's callback thread: no HIP
+    // calls, no server lock)
+    void slot_finished(void* p)
+    {
+        auto* s = static_cast<server_slot*>(p);
+        s->done_ns.store(std::chrono::duration_cast<std::chrono::nanoseconds>(server_clock::now().time_since_epoch()).count());
+        s->done.store(true);
+        s->wake->notify_one();
+    }
+
+    constexpr int max_classes_per_launch = 16;
+}
+
+struct hmpc_server_s
+{
+    hmpc_context ctx = nullptr;
+    std::size_t budget = 0;
+    std::size_t in_use = 0;
+    std::size_t max_jobs = 4096;
+    chacha_key key{};
+    std::vector<server_key> keys;
+    std::mutex m;
+    std::condition_variable work_cv, done_cv;
+    std::deque<u64> queue;
+    std::unordered_map<u64, server_request> reqs;
+    std::vector<std::unique_ptr<server_slot>> slots;
+    block_pool pool;
+    std::thread worker;
+    bool stop = false;
+    u64 next_id = 1;
+    std::vector<std::pair<u64, double>> completed;  // finished, not yet reported: (request, latency us)
+    std::string error;
+    u64 launches = 0, jobs_run = 0;
+
+    bool idle() const
+    {
+        for (auto const& s : slots)
+            if (s->busy)
+                return false;
+        return true;
+    }
+
+    // worker: completions, then admission into free slots
+    void run()
+    {
+        (void)hipSetDevice(ctx->device);
+        std::unique_lock<std::mutex> lk(m);
+        for (;;)
+        {
+            bool finished = false;
+            for (auto& sp : slots)
+            {
+                server_slot& s = *sp;
+                if (s.busy && s.done.load())
+                {
+                    auto t = server_clock::time_point(std::chrono::nanoseconds(s.done_ns.load()));
+                    for (u64 id : s.requests)
+                    {
+                        auto& r = reqs[id];
+                        r.state = req_done;
+                        r.t_done = t;
+                        completed.push_back({id, std::chrono::duration<double, std::micro>(t - r.t_submit).count()});
+                    }
+                    s.requests.clear();
+                    s.busy = false;
+                    s.done.store(false);
+                    finished = true;
+                }
+            }
+            if (finished)
+                done_cv.notify_all();
+            if (stop)
+            {
+                // shutting down: queued requests are dropped, running ones drained
+                for (u64 id : queue)
+                    reqs[id].state = req_failed;
+                queue.clear();
+                if (idle())
+                    break;
+            }
+            for (auto& sp : slots)
+            {
+                if (sp->busy || queue.empty())
+                    continue;
+                if (!admit(*sp))
+                    break;
+            }
+            work_cv.wait_for(lk, std::chrono::microseconds(200));
+        }
+    }
+
+    // moves queued requests that fit into slot s and launches them; false if nothing fit
+    bool admit(server_slot& s)
+    {
+        std::vector<u64> take;
+        std::size_t jobs = 0;
+        while (!queue.empty())
+        {
+            auto& r = reqs[queue.front()];
+            if (!take.empty() && jobs + r.n > max_jobs)
+                break;
+            if (in_use + r.bytes > budget)
+                break;  // arrival order: the head waits for memory
+            void* out = nullptr;
+            if (pool.take(r.cls, &out) != hipSuccess)
+            {
+                error = "out of device memory for request outputs";
+                break;
+            }
+            r.out = out;
+            in_use += r.bytes;
+            r.state = req_running;
+            take.push_back(queue.front());
+            jobs += r.n;
+            queue.pop_front();
+        }
+        if (take.empty())
+            return false;
+        if (launch(s, take, jobs) != HMPC_OK)
+        {
+            if (error.empty())
+                error = last_error;
+            for (u64 id : take)
+            {
+                auto& r = reqs[id];
+                r.state = req_failed;
+                completed.push_back({id, -1.0});
+            }
+            done_cv.notify_all();
+            return false;
+        }
+        return true;
+    }
+
+    int launch(server_slot& s, std::vector<u64> const& take, std::size_t jobs)
+    {
+        struct cls_info
+        {
+            hmpc_modulus mod;
+            int elems;
+            int log_n;
+            std::vector<std::pair<u64, job_desc>> jobs;  // (cost, job)
+        };
+        std::vector<cls_info> classes;
+        for (u64 id : take)
+        {
+            auto& r = reqs[id];
+            transform_args T;
+            int rc = r.mod->args(r.log_n, T);
+            if (rc)
+            {
+                error = last_error;
+                return rc;
+            }
+            int elems = r.kind == 0 ? r.mod->bgv_elems_for(r.log_n) : (r.mod->elems.count(r.log_n) ? r.mod->elems[r.log_n] : r.mod->default_elems(r.log_n));
+            auto it = std::find_if(classes.begin(), classes.end(), [&](cls_info const& c) { return c.mod == r.mod && c.elems == elems; });
+            if (it == classes.end())
+            {
+                if (classes.size() == max_classes_per_launch)
+                {
+                    error = "too many kernel classes in one launch";
+                    return HMPC_ERR_INVALID_ARGUMENT;
+                }
+                classes.push_back({r.mod, elems, r.log_n, {}});
+                it = classes.end() - 1;
+            }
+            u64 N = u64{1} << r.log_n;
+            u64 count = static_cast<u64>(r.n) << r.log_n;
+            auto* out = static_cast<u32*>(r.out);
+            if (r.kind == 0)
+            {
+                server_key const& k = keys[static_cast<std::size_t>(r.key)];
+                bgv_args B{};
+                B.key = key;
+                std::copy(r.nonce, r.nonce + 3, B.nonce_u);
+                std::copy(r.nonce, r.nonce + 3, B.nonce_v);
+                nonce_add(B.nonce_v, count);
+                std::copy(B.nonce_v, B.nonce_v + 3, B.nonce_w);
+                nonce_add(B.nonce_w, count);
+                B.bits_u = k.bits[0];
+                B.bits_v = k.bits[1];
+                B.bits_w = k.bits[2];
+                u32* c1 = out + static_cast<std::size_t>(r.mod->limbs) * count;
+                for (std::size_t i = 0; i < r.n; ++i)
+                    it->jobs.push_back({4 * N * r.log_n, bgv_job(T, B, k.p_int, r.log_n, k.pk_a, k.pk_b, r.messages, out, c1,
+                                                                 count, i * N, static_cast<u32>(i))});
+            }
+            else
+            {
+                for (std::size_t i = 0; i < r.n; ++i)
+                {
+                    job_desc J = polymul_job(T, r.log_n, r.a, r.b, out, count, i * N);
+                    it->jobs.push_back({3 * N * r.log_n, J});
+                }
+            }
+        }
+        // the slot is idle: its table, counters and scratch may be resized
+        if (s.table_cap < jobs)
+        {
+            if (s.d_table)
+                HMPC_HIP(hipFree(s.d_table));
+            if (s.h_table)
+                HMPC_HIP(hipHostFree(s.h_table));
+            s.d_table = nullptr;
+            s.h_table = nullptr;
+            std::size_t cap = std::max(jobs, max_jobs);
+            HMPC_HIP(hipMalloc(&s.d_table, cap * sizeof(job_desc)));
+            HMPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_table), cap * sizeof(job_desc)));
+            s.table_cap = cap;
+        }
+        std::size_t need = 0;
+        std::vector<unsigned> grids(classes.size());
+        std::vector<u64> slot_words(classes.size());
+        for (std::size_t c = 0; c < classes.size(); ++c)
+        {
+            auto& cl = classes[c];
+            std::stable_sort(cl.jobs.begin(), cl.jobs.end(), [](auto const& x, auto const& y) { return x.first > y.first; });
+            u64 w = 0;
+            for (auto const& j : cl.jobs)
+                w = std::max(w, job_scratch_words(j.second.type, static_cast<int>(j.second.log_n), cl.mod->limbs));
+            int rc = jobs_grid(*cl.mod, cl.elems, cl.jobs.size(), &grids[c]);
+            if (rc)
+            {
+                error = last_error;
+                return rc;
+            }
+            slot_words[c] = w;
+            need = std::max<std::size_t>(need, static_cast<std::size_t>(grids[c]) * w);
+        }
+        if (s.scratch_words < need)
+        {
+            if (s.scratch)
+                HMPC_HIP(hipFree(s.scratch));
+            s.scratch = nullptr;
+            HMPC_HIP(hipMalloc(&s.scratch, need * 4));
+            s.scratch_words = need;
+        }
+        std::size_t off = 0;
+        for (auto const& cl : classes)
+            for (auto const& j : cl.jobs)
+                s.h_table[off++] = j.second;
+        HMPC_HIP(hipMemcpyAsync(s.d_table, s.h_table, off * sizeof(job_desc), hipMemcpyHostToDevice, s.stream));
+        off = 0;
+        for (std::size_t c = 0; c < classes.size(); ++c)
+        {
+            auto& cl = classes[c];
+            transform_args T;
+            int rc = cl.mod->args(cl.log_n, T);
+            if (rc)
+                return rc;
+            T.elems = cl.elems;
+            HMPC_HIP(launch_fast_jobs(cl.mod->limbs, T, s.d_table + off, job_desc{}, static_cast<u32>(cl.jobs.size()), grids[c],
+                                      s.counters + 64 * c, s.scratch, slot_words[c], s.stream));
+            off += cl.jobs.size();
+        }
+        s.done.store(false);
+        s.requests = take;
+        s.busy = true;
+        HMPC_HIP(hipLaunchHostFunc(s.stream, slot_finished, &s));
+        launches += 1;
+        jobs_run += jobs;
+        return HMPC_OK;
+    }
+};
+
+extern "C" {
+
+int hmpc_server_create(hmpc_context ctx, size_t budget_bytes, int slots, hmpc_server* out)
+{
+    if (!ctx || !out || budget_bytes == 0 || slots < 1 || slots > 8)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "server: context, budget > 0 and 1..8 slots required");
+    device_guard g(ctx->device);
+    auto srv = std::make_unique<hmpc_server_s>();
+    srv->ctx = ctx;
+    srv->budget = budget_bytes;
+    std::copy(ctx->key, ctx->key + 8, srv->key.k);
+    for (int i = 0; i < slots; ++i)
+    {
+        auto s = std::make_unique<server_slot>();
+        s->wake = &srv->work_cv;
+        HMPC_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HMPC_HIP(hipMalloc(&s->counters, 256 * max_classes_per_launch));
+        srv->slots.push_back(std::move(s));
+    }
+    hmpc_server_s* raw = srv.get();
+    srv->worker = std::thread([raw]() { raw->run(); });
+    *out = srv.release();
+    return HMPC_OK;
+}
+
+int hmpc_server_destroy(hmpc_server srv)
+{
+    if (!srv)
+        return HMPC_OK;
+    {
+        std::lock_guard<std::mutex> lk(srv->m);
+        srv->stop = true;
+    }
+    srv->work_cv.notify_all();
+    if (srv->worker.joinable())
+        srv->worker.join();
+    device_guard g(srv->ctx->device);
+    for (auto& s : srv->slots)
+    {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamDestroy(s->stream);
+        if (s->d_table)
+            (void)hipFree(s->d_table);
+        if (s->h_table)
+            (void)hipHostFree(s->h_table);
+        if (s->counters)
+            (void)hipFree(s->counters);
+        if (s->scratch)
+            (void)hipFree(s->scratch);
+    }
+    srv->pool.release_all();
+    delete srv;
+    return HMPC_OK;
+}
+
+int hmpc_server_add_key(hmpc_server srv, hmpc_modulus mod, int log_n, const uint32_t* pk_a, const uint32_t* pk_b,
+                        const uint32_t* p_mont, int bits_u, int bits_v, int bits_w, int* key_id)
+{
+    if (!srv || !mod || !pk_a || !pk_b || !p_mont || !key_id || mod->ctx != srv->ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument or foreign modulus");
+    if (!valid_bits(bits_u) || !valid_bits(bits_v) || !valid_bits(bits_w))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    int rc = mod->prepare(log_n);
+    if (rc)
+        return rc;
+    if (!mod->fast(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "serving needs a modulus served by the fast kernels");
+    u64 p_int = 0;
+    if (!small_plaintext(*mod, p_mont, &p_int))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "serving needs a plaintext modulus below 2^40");
+    std::lock_guard<std::mutex> lk(srv->m);
+    srv->keys.push_back({mod, log_n, pk_a, pk_b, p_int, {bits_u, bits_v, bits_w}});
+    *key_id = static_cast<int>(srv->keys.size() - 1);
+    return HMPC_OK;
+}
+
+namespace
+{
+    int server_enqueue(hmpc_server srv, server_request r, uint64_t* request)
+    {
+        if (r.bytes > srv->budget)
+            return fail(HMPC_ERR_BUDGET, "request outputs exceed the server's memory budget");
+        {
+            std::lock_guard<std::mutex> lk(srv->m);
+            if (!srv->error.empty())
+                return fail(HMPC_ERR_DEVICE, "server: " + srv->error);
+            u64 id = srv->next_id++;
+            r.t_submit = server_clock::now();
+            srv->reqs.emplace(id, r);
+            srv->queue.push_back(id);
+            *request = id;
+        }
+        srv->work_cv.notify_one();
+        return HMPC_OK;
+    }
+}
+
+int hmpc_server_submit_encrypt(hmpc_server srv, int key_id, size_t k, const uint32_t* messages, const uint32_t* nonce3,
+                               uint64_t* request)
+{
+    if (!srv || !messages || !nonce3 || !request || k == 0 || k > 65536)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
+    server_request r;
+    {
+        std::lock_guard<std::mutex> lk(srv->m);
+        if (key_id < 0 || static_cast<std::size_t>(key_id) >= srv->keys.size())
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "no such key");
+        server_key const& key = srv->keys[static_cast<std::size_t>(key_id)];
+        r.mod = key.mod;
+        r.log_n = key.log_n;
+    }
+    r.kind = 0;
+    r.key = key_id;
+    r.n = k;
+    r.messages = messages;
+    std::copy(nonce3, nonce3 + 3, r.nonce);
+    std::size_t bytes = 2 * (k << r.log_n) * static_cast<std::size_t>(r.mod->limbs) * 4;
+    r.cls = block_pool::class_of(bytes);
+    r.bytes = r.cls;
+    return server_enqueue(srv, r, request);
+}
+
+int hmpc_server_submit_polymul(hmpc_server srv, hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b,
+                               uint64_t* request)
+{
+    if (!srv || !mod || !a || !b || !request || n == 0 || n > 65536 || mod->ctx != srv->ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
+    int rc = mod->prepare(log_n);
+    if (rc)
+        return rc;
+    if (!mod->fast(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "serving needs a modulus served by the fast kernels");
+    server_request r;
+    r.kind = 1;
+    r.mod = mod;
+    r.log_n = log_n;
+    r.n = n;
+    r.a = a;
+    r.b = b;
+    std::size_t bytes = (n << log_n) * static_cast<std::size_t>(mod->limbs) * 4;
+    r.cls = block_pool::class_of(bytes);
+    r.bytes = r.cls;
+    return server_enqueue(srv, r, request);
+}
+
+int hmpc_server_poll(hmpc_server srv, uint64_t request, int* state, double* latency_us)
+{
+    if (!srv)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null server");
+    std::lock_guard<std::mutex> lk(srv->m);
+    auto it = srv->reqs.find(request);
+    if (it == srv->reqs.end())
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "no such request");
+    if (state)
+        *state = it->second.state;
+    if (latency_us)
+        *latency_us = it->second.state == req_done
+                          ? std::chrono::duration<double, std::micro>(it->second.t_done - it->second.t_submit).count()
+                          : 0.0;
+    return it->second.state == req_failed ? fail(HMPC_ERR_DEVICE, "server: " + srv->error) : HMPC_OK;
+}
+
+int hmpc_server_wait(hmpc_server srv, uint64_t request)
+{
+    if (!srv)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null server");
+    std::unique_lock<std::mutex> lk(srv->m);
+    for (;;)
+    {
+        auto it = srv->reqs.find(request);
+        if (it == srv->reqs.end())
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "no such request");
+        if (it->second.state == req_done)
+            return HMPC_OK;
+        if (it->second.state == req_failed)
+            return fail(HMPC_ERR_DEVICE, "server: " + srv->error);
+        srv->done_cv.wait_for(lk, std::chrono::milliseconds(10));
+    }
+}
+
+int hmpc_server_result(hmpc_server srv, uint64_t request, uint32_t** c0, uint32_t** c1)
+{
+    if (!srv)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null server");
+    std::lock_guard<std::mutex> lk(srv->m);
+    auto it = srv->reqs.find(request);
+    if (it == srv->reqs.end())
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "no such request");
+    if (it->second.state != req_done)
+        return fail(HMPC_ERR_NOT_READY, "request not finished");
+    server_request const& r = it->second;
+    auto* out = static_cast<u32*>(r.out);
+    if (c0)
+        *c0 = out;
+    if (c1)
+        *c1 = r.kind == 0 ? out + static_cast<std::size_t>(r.mod->limbs) * (r.n << r.log_n) : nullptr;
+    return HMPC_OK;
+}
+
+int hmpc_server_release(hmpc_server srv, uint64_t request)
+{
+    if (!srv)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null server");
+    {
+        std::lock_guard<std::mutex> lk(srv->m);
+        auto it = srv->reqs.find(request);
+        if (it == srv->reqs.end())
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "no such request");
+        server_request& r = it->second;
+        if (r.state == req_running)
+            return fail(HMPC_ERR_NOT_READY, "request still running");
+        if (r.state == req_queued)
+        {
+            srv->queue.erase(std::find(srv->queue.begin(), srv->queue.end(), request));
+            srv->reqs.erase(it);
+            return HMPC_OK;
+        }
+        if (r.out)
+        {
+            srv->pool.give(r.cls, r.out);
+            srv->in_use -= r.bytes;
+        }
+        srv->reqs.erase(it);
+    }
+    srv->work_cv.notify_one();
+    return HMPC_OK;
+}
+
+int hmpc_server_completed(hmpc_server srv, uint64_t* requests, double* latency_us, size_t max, size_t* count)
+{
+    if (!srv || !count || (max && (!requests || !latency_us)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
+    std::lock_guard<std::mutex> lk(srv->m);
+    std::size_t n = std::min(max, srv->completed.size());
+    for (std::size_t i = 0; i < n; ++i)
+    {
+        requests[i] = srv->completed[i].first;
+        latency_us[i] = srv->completed[i].second;
+    }
+    srv->completed.erase(srv->completed.begin(), srv->completed.begin() + static_cast<std::ptrdiff_t>(n));
+    *count = n;
+    return HMPC_OK;
+}
+
+int hmpc_server_stats(hmpc_server srv, size_t* queued, size_t* running, size_t* bytes_in_use, uint64_t* launches,
+                      uint64_t* jobs)
+{
+    if (!srv)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null server");
+    std::lock_guard<std::mutex> lk(srv->m);
+    if (queued)
+        *queued = srv->queue.size();
+    if (running)
+    {
+        std::size_t r = 0;
+        for (auto const& s : srv->slots)
+            r += s->requests.size();
+        *running = r;
+    }
+    if (bytes_in_use)
+        *bytes_in_use = srv->in_use;
+    if (launches)
+        *launches = srv->launches;
+    if (jobs)
+        *jobs = srv->jobs_run;
     return HMPC_OK;
 }
 

@@ -69,6 +69,18 @@ SIGNATURES = {
     "hmpc_batch_add_bgv_encrypt": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp]),
     "hmpc_batch_run": (_i, [_vp, _i, _vp]),
     "hmpc_batch_info": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "hmpc_server_create": (_i, [_vp, _sz, _i, ctypes.POINTER(_vp)]),
+    "hmpc_server_destroy": (_i, [_vp]),
+    "hmpc_server_add_key": (_i, [_vp, _vp, _i, _vp, _vp, _u32p, _i, _i, _i, ctypes.POINTER(_i)]),
+    "hmpc_server_submit_encrypt": (_i, [_vp, _i, _sz, _vp, _u32p, ctypes.POINTER(_u64)]),
+    "hmpc_server_submit_polymul": (_i, [_vp, _vp, _i, _sz, _vp, _vp, ctypes.POINTER(_u64)]),
+    "hmpc_server_poll": (_i, [_vp, _u64, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
+    "hmpc_server_wait": (_i, [_vp, _u64]),
+    "hmpc_server_result": (_i, [_vp, _u64, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "hmpc_server_release": (_i, [_vp, _u64]),
+    "hmpc_server_completed": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_double), _sz, ctypes.POINTER(_sz)]),
+    "hmpc_server_stats": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_u64),
+                               ctypes.POINTER(_u64)]),
     "hmpc_context_set_timing": (_i, [_vp, _i]),
     "hmpc_context_timing": (_i, [_vp, _i, ctypes.c_char_p, _sz, ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_double)]),
     "hmpc_event_create": (_i, [_vp, ctypes.POINTER(_vp)]),
@@ -492,6 +504,99 @@ class Batch:
     def close(self):
         if self.handle and _lib is not None:
             _lib.hmpc_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class View:
+    """Non-owning device pointer (offset into a buffer, or memory owned by the server)."""
+
+    def __init__(self, ctx: Context, ptr: int):
+        self.ctx = ctx
+        self.ptr = ptr
+
+    def to_numpy(self, shape, dtype=np.uint32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            check(load().hmpc_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+
+class Server:
+    """Continuous-batching request loop (hmpc_server_*): encryption and product requests admitted
+    within an HBM budget and packed into overlapping persistent launches."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, ctx: Context, budget_bytes: int, slots: int = 3):
+        self.ctx = ctx
+        h = _vp()
+        check(_lib.hmpc_server_create(ctx.handle, budget_bytes, slots, ctypes.byref(h)))
+        self.handle = h.value
+        self._keep = []
+        self._ids = (_u64 * 4096)()
+        self._lat = (ctypes.c_double * 4096)()
+
+    def add_key(self, mod: Modulus, log_n: int, pk_a: Buffer, pk_b: Buffer, p_mont, bits=(2, 40, 40)) -> int:
+        k = _i()
+        pm = np.ascontiguousarray(p_mont, dtype=np.uint32)
+        check(_lib.hmpc_server_add_key(self.handle, mod.handle, log_n, pk_a.ptr, pk_b.ptr, _p(pm), bits[0], bits[1], bits[2],
+                                       ctypes.byref(k)))
+        self._keep += [pk_a, pk_b]
+        return k.value
+
+    def submit_encrypt(self, key: int, k: int, messages_ptr: int, nonce) -> int:
+        r = _u64()
+        nz = _arr(nonce, 3)
+        check(_lib.hmpc_server_submit_encrypt(self.handle, key, k, messages_ptr, _p(nz), ctypes.byref(r)))
+        return r.value
+
+    def submit_polymul(self, mod: Modulus, log_n: int, n: int, a_ptr: int, b_ptr: int) -> int:
+        r = _u64()
+        check(_lib.hmpc_server_submit_polymul(self.handle, mod.handle, log_n, n, a_ptr, b_ptr, ctypes.byref(r)))
+        return r.value
+
+    def poll(self, request: int):
+        st, lat = _i(), ctypes.c_double()
+        check(_lib.hmpc_server_poll(self.handle, request, ctypes.byref(st), ctypes.byref(lat)))
+        return st.value, lat.value
+
+    def wait(self, request: int):
+        check(_lib.hmpc_server_wait(self.handle, request))
+
+    def result(self, request: int):
+        c0, c1 = _vp(), _vp()
+        check(_lib.hmpc_server_result(self.handle, request, ctypes.byref(c0), ctypes.byref(c1)))
+        return View(self.ctx, c0.value), (View(self.ctx, c1.value) if c1.value else None)
+
+    def release(self, request: int):
+        check(_lib.hmpc_server_release(self.handle, request))
+
+    def completed(self):
+        """[(request, latency_us)] finished since the last call"""
+        out = []
+        while True:
+            n = _sz()
+            check(_lib.hmpc_server_completed(self.handle, self._ids, self._lat, 4096, ctypes.byref(n)))
+            out += [(self._ids[i], self._lat[i]) for i in range(n.value)]
+            if n.value < 4096:
+                return out
+
+    def stats(self) -> dict:
+        q, r, b = _sz(), _sz(), _sz()
+        la, jo = _u64(), _u64()
+        check(_lib.hmpc_server_stats(self.handle, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b), ctypes.byref(la),
+                                     ctypes.byref(jo)))
+        return {"queued": q.value, "running": r.value, "bytes_in_use": b.value, "launches": la.value, "jobs": jo.value}
+
+    def close(self):
+        if self.handle and _lib is not None:
+            _lib.hmpc_server_destroy(self.handle)
         self.handle = None
 
     def __del__(self):

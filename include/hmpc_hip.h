@@ -216,6 +216,44 @@ int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream);
 /* jobs, kernel classes (after the first run), device bytes held (tables, counters, scratch) */
 int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* device_bytes);
 
+/* ------------------------------------------------------------------ serving
+ * A request loop on a native worker thread that admits encryption and product requests as they
+ * arrive (continuous batching): whenever one of `slots` streams is free, queued requests whose
+ * outputs fit the HBM budget are packed, in arrival order, into one persistent-kernel launch on
+ * it; launches on different slots overlap on the device. Keys, twiddles, messages and outputs
+ * stay in HBM. Outputs are allocated by the server (counted against `budget_bytes`) and held
+ * until hmpc_server_release. A request larger than the budget fails with HMPC_ERR_BUDGET;
+ * others wait for memory. Randomness: the server's ChaCha key is the context key at creation;
+ * an encryption request of k messages with nonce n gives exactly the ciphertexts of
+ * hmpc_bgv_encrypt(n = k) on a context with that key and nonce (u, v, w on nonces n, n + kN,
+ * n + 2kN), whatever else shares the GPU. Input buffers must stay valid until the request is
+ * done. Thread safety: one client thread; the calls below lock the server. No counterpart in
+ * hmpc, whose queue executes synchronously per call (hmpc/comp/queue.hpp:416-529). */
+typedef struct hmpc_server_s* hmpc_server;
+int hmpc_server_create(hmpc_context ctx, size_t budget_bytes, int slots, hmpc_server* out);
+/* drops queued requests, drains running ones, frees all server memory */
+int hmpc_server_destroy(hmpc_server server);
+/* resident public key (caller-owned device buffers [L][N] in NTT form) and its sampler widths */
+int hmpc_server_add_key(hmpc_server server, hmpc_modulus mod, int log_n, const uint32_t* pk_a, const uint32_t* pk_b,
+                        const uint32_t* p_mont, int bits_u, int bits_v, int bits_w, int* key_id);
+/* k messages (device, k*N u32 words) under key_id; outputs c0, c1 each [L][k*N] */
+int hmpc_server_submit_encrypt(hmpc_server server, int key_id, size_t k, const uint32_t* messages, const uint32_t* nonce3,
+                               uint64_t* request);
+/* n products INTT(NTT(a_i) o NTT(b_i)) of [L][n*N] inputs; output [L][n*N] (returned as c0) */
+int hmpc_server_submit_polymul(hmpc_server server, hmpc_modulus mod, int log_n, size_t n, const uint32_t* a,
+                               const uint32_t* b, uint64_t* request);
+/* state 0 queued, 1 running, 2 done, 3 failed; latency = submission to completion (us, when done) */
+int hmpc_server_poll(hmpc_server server, uint64_t request, int* state, double* latency_us);
+int hmpc_server_wait(hmpc_server server, uint64_t request);
+/* device pointers of a finished request's outputs (c1 is NULL for products), valid until release */
+int hmpc_server_result(hmpc_server server, uint64_t request, uint32_t** c0, uint32_t** c1);
+/* returns the outputs' memory to the budget (or cancels a queued request) */
+int hmpc_server_release(hmpc_server server, uint64_t request);
+/* requests finished since the last call (up to max), with their latencies in us (-1: failed) */
+int hmpc_server_completed(hmpc_server server, uint64_t* requests, double* latency_us, size_t max, size_t* count);
+int hmpc_server_stats(hmpc_server server, size_t* queued, size_t* running, size_t* bytes_in_use, uint64_t* launches,
+                      uint64_t* jobs);
+
 /* ------------------------------------------------------------------ kernel timing
  * Per-kernel durations of the fast kernels, measured with HIP events recorded around every launch
  * on the stream it runs on (not inside graph capture). Enabling (or disabling) clears the
