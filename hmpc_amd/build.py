@@ -54,9 +54,26 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_facade_test()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+FACADE_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_test.cpp")
+FACADE_TEST_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_test")
+
+
+def build_facade_test() -> str:
+    """The C++ program that drives include/hmpc_amd/comp.hpp (run by tests/test_gpu_facade.py)."""
+    root = os.path.dirname(HERE)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{os.path.join(root, 'include')}", FACADE_TEST_SRC,
+           f"-L{HERE}", "-lhmpc_hip", "-Wl,-rpath,$ORIGIN/../../hmpc_amd", "-o", FACADE_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("facade test build failed")
+    return FACADE_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -43,3 +43,13 @@ def test_load_and_error_path_without_gpu():
     rc = lib.hmpc_context_create(0, None, None)
     assert rc == 1
     assert b"null" in lib.hmpc_last_error()
+
+
+def test_cpp_interface_compiles(tmp_path):
+    """include/hmpc_amd/comp.hpp (the hmpc::comp mirror) compiles as C++17 with warnings as errors."""
+    import subprocess
+
+    src = os.path.join(ROOT, "tests", "cpp", "facade_test.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        f"-I{os.path.join(ROOT, 'include')}", src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
