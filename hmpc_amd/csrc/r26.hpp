@@ -12,8 +12,8 @@
 // mont(a R, w 2^130) = (a w) R. A product of two data values is corrected by 4 = R' / R.
 //
 // Bounds (q > 2^124, so 64 q > 2^130):
-//   mont(a, b): column sums < 2^61 when a_j < 2^31 and b_j < 2^27; result < 2q with normalized
-//               limbs (r_4 < 2^22) whenever v(a) v(b) < 2^130 q
+//   mont(a, b): column sums < 2^61 when a_j < 2^31 and b_j < 2^27; result < 2q (1 + 2^-19) with
+//               normalized limbs (r_4 < 2^22) whenever v(a) v(b) < 2^130 q
 //   forward butterfly (x + t, x - t + K4), t = mont(y, w) < 2q: value +4q and limbs +2^27 per
 //               stage; from inputs < 2q with limbs < 2^26 this stays < 2^130 and < 2^31 for
 //               up to 12 stages, so a whole pass runs without reductions
@@ -102,11 +102,16 @@ namespace hmpc_amd
 #pragma unroll
             for (int j = 0; j < 5; ++j)
                 T[i + j] = static_cast<u64>(a[i]) * b[j] + T[i + j];
-        // q in radix 2^26: limb 0 = c, limb 4 = 2^20, others 0
+        // q in radix 2^26: limb 0 = c, limb 4 = 2^20, others 0. Only the last quotient digit is
+        // masked to 26 bits: with m_0..m_3 < 2^32 the quotient M = sum m_i 2^(26 i) stays below
+        // 2^130 + 2^111, so the result (T + M q) / 2^130 keeps its bound (< 2q (1 + 2^-19),
+        // normalized limbs, r_4 < 2^22) and each round saves the mask
 #pragma unroll
         for (int i = 0; i < 5; ++i)
         {
-            u32 m = (static_cast<u32>(T[i]) * P.n26) & m26;
+            u32 m = static_cast<u32>(T[i]) * P.n26;
+            if (i == 4)
+                m &= m26;
             T[i] = static_cast<u64>(m) * P.c + T[i];
             T[i + 4] = static_cast<u64>(m) * P.t20 + T[i + 4];
             T[i + 1] += T[i] >> 26;
