@@ -60,6 +60,7 @@ namespace hmpc_amd
             static constexpr int rl = 5;    // limbs per element in registers and LDS
             static constexpr int iol = 4;   // limbs per element in global memory (hmpc layout)
             static constexpr int tws = 8;   // twiddle table stride in words (5 used)
+            static constexpr bool stage_prefetch = true;
             using params = p26;
 
             __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
@@ -198,11 +199,20 @@ namespace hmpc_amd
             }
         };
 
+        // the same arithmetic with all of a round's twiddles loaded up front: for kernels that run
+        // one workgroup per CU anyway (the persistent job kernel), where the loads' latency is not
+        // covered by another workgroup
+        struct arith_r26_round_prefetch : arith_r26
+        {
+            static constexpr bool stage_prefetch = false;
+        };
+
         struct arith_gold
         {
             static constexpr int rl = 2;
             static constexpr int iol = 2;
             static constexpr int tws = 2;
+            static constexpr bool stage_prefetch = false;
             using params = fast_params;
             using ops = fast_ops<2, form_goldilocks>;
 
@@ -484,6 +494,21 @@ namespace hmpc_amd
             }
         }
 
+        // stages II .. W-2 of a round, each preceded by the loads of its own twiddles
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void stages_prefetched(u32 (&x)[E][A::rl], u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
+                                                          tile_desc const& td, int a, u32 base0, u32 base_step,
+                                                          u32 const* __restrict__ tw, typename A::params const& P)
+        {
+            if constexpr (II < W - 1)
+            {
+                u32 const z[A::rl] = {};
+                prefetch_stage<A, E, Inv, W, II>(tw_regs, td, a, base0, base_step, tw);
+                stage<A, E, Inv, W, II, false>(x, tw_regs, z, z, P);
+                stages_prefetched<A, E, Inv, W, II + 1>(x, tw_regs, td, a, base0, base_step, tw, P);
+            }
+        }
+
         // butterflies of one round: all twiddle loads first, then the stages. The round's last
         // stage is the inverse's scaled final stage when the round ends the transform.
         template<typename A, int E, bool Inv, int W>
@@ -493,8 +518,26 @@ namespace hmpc_amd
         {
             u32 tw_regs[E >> W][(1 << W) - 1][A::rl];
             bool const scaled = Inv && final_scale && (a + W == td.s);
-            prefetch_upto<A, E, Inv, W, 0>(tw_regs, td, a, base0, base_step, tw);
             u32 w0[A::rl] = {}, w1[A::rl] = {};
+            if constexpr (A::stage_prefetch)
+            {
+                // twiddles loaded stage by stage: at most one stage's twiddles live (fewer VGPRs,
+                // the latency is covered by the other workgroup on the CU)
+                stages_prefetched<A, E, Inv, W, 0>(x, tw_regs, td, a, base0, base_step, tw, P);
+                if (scaled)
+                {
+                    A::tw_get(w0, tw, 0);
+                    A::tw_get(w1, tw, 1);
+                    stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P);
+                }
+                else
+                {
+                    prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, td, a, base0, base_step, tw);
+                    stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P);
+                }
+                return;
+            }
+            prefetch_upto<A, E, Inv, W, 0>(tw_regs, td, a, base0, base_step, tw);
             if (scaled)
             {
                 A::tw_get(w0, tw, 0);
@@ -731,11 +774,79 @@ namespace hmpc_amd
         template<typename A>
         constexpr std::size_t tile_bytes() { return std::size_t{4} * A::rl * tile; }
 
+        // ------------------------------------------------------------------ second operands
+        // After the forward transform's last round (stages [s - W, s)) of a row-layout tile, the
+        // thread's unit g holds the K = 2^W consecutive tile elements ((tid + T g) << W) + k, so a
+        // second operand of the same elements is read straight from global memory (K
+        // consecutive words per limb plane; the workgroup wrote it itself just before, so it is
+        // served from L2) instead of occupying a second register tile.
+        //   Mode 0: x = a o x            (a = NTT(a) of a product, lazy)
+        //   Mode 1: x = k o u + x        (u = NTT(u), k = public-key polynomial, canonical)
+        // Only the tile's first `valid` elements are backed by memory (a whole-tile job of degree
+        // N < 4096 fills one of the tile's 4096 / N sequences); the others are left untouched.
+        template<typename A, int E, int W, int Mode>
+        __device__ __forceinline__ void with_operands(u32 (&x)[E][A::rl], int valid, u32 const* __restrict__ src,
+                                                      std::size_t src_count, std::size_t src_base, u32 const* __restrict__ key,
+                                                      std::size_t key_count, std::size_t key_base, typename A::params const& P)
+        {
+            constexpr int K = 1 << W;
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+                int const e0 = (tid_x() + shape<E>::threads * g) << W;
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    if (e0 + k >= valid)
+                        continue;
+                    u32 v[A::rl];
+                    A::load_global(v, src, src_count, src_base + e0 + k);
+                    if constexpr (Mode == 0)
+                    {
+                        A::point(v, x[g * K + k], P);
+#pragma unroll
+                        for (int l = 0; l < A::rl; ++l)
+                            x[g * K + k][l] = v[l];
+                    }
+                    else
+                    {
+                        u32 kv[A::rl];
+                        A::load_global(kv, key, key_count, key_base + e0 + k);
+                        A::point_add(x[g * K + k], v, kv, P);
+                    }
+                }
+            }
+        }
+
+        template<typename A, int E, int Mode>
+        __device__ __forceinline__ void with_operands_last(u32 (&x)[E][A::rl], tile_desc const& td, int valid, u32 const* src,
+                                                           std::size_t src_count, std::size_t src_base, u32 const* key,
+                                                           std::size_t key_count, std::size_t key_base,
+                                                           typename A::params const& P)
+        {
+            constexpr int WM = shape<E>::wmax;
+            int w = round_width<WM>(td.s, round_count<WM>(td.s) - 1, false);
+            if (w == WM)
+                with_operands<A, E, WM, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+            else if (WM > 1 && w == 1)
+                with_operands<A, E, 1, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+            else if constexpr (WM > 2)
+            {
+                if (w == 2)
+                    with_operands<A, E, 2, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+                else if constexpr (WM > 3)
+                {
+                    if (w == 3)
+                        with_operands<A, E, 3, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
+                }
+            }
+        }
+
         // ------------------------------------------------------------------ kernels
         // mode: 0 forward column pass, 1 forward row pass (final), 2 inverse row pass,
         //       3 inverse column pass (final), 4 forward whole-poly, 5 inverse whole-poly
         template<typename A, int E, int Mode>
-        __global__ void __launch_bounds__(tile / E) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
                                                            u32 const* __restrict__ tw, pass_geom G, u64 n,
                                                            typename A::params P)
         {
@@ -774,10 +885,11 @@ namespace hmpc_amd
         }
 
         // polynomial product, whole polynomials per tile (N <= 4096) or the row pass of the
-        // two-pass product: forward(A), forward(B), A o B in registers, inverse.
+        // two-pass product: forward(A), forward(B), A o B, inverse. The row pass works in place
+        // (a == out) and asks for 4 waves per SIMD (2 workgroups per CU with the 80 KiB tile).
         template<typename A, int E, bool Single>
-        __global__ void __launch_bounds__(tile / E) k_polymul(u32 const* __restrict__ a, u32 const* __restrict__ b,
-                                                              u32* __restrict__ out, u32 const* __restrict__ twf,
+        __global__ void __launch_bounds__(tile / E) k_polymul(u32 const* a, u32 const* __restrict__ b,
+                                                              u32* out, u32 const* __restrict__ twf,
                                                               u32 const* __restrict__ twi, pass_geom G, u64 n,
                                                               typename A::params P)
         {
@@ -798,24 +910,46 @@ namespace hmpc_amd
                 base0 = (1u << G.log_n1) + hi0;
                 base_step = 1;
             }
-            u32 xa[E][A::rl];
-            u32 xb[E][A::rl];
-            tile_load<A, T>(lds, a, io);
-            __syncthreads();
-            tile_rounds<A, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
-            __syncthreads();
-            tile_load<A, T>(lds, b, io);
-            __syncthreads();
-            tile_rounds<A, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
-#pragma unroll
-            for (int k = 0; k < E; ++k)
-                A::point(xa[k], xb[k], P);
-            // inverse round 0 groups the same elements as the forward's last round
             int w0 = round_width<WM>(td.s, 0, true);
-            dispatch_round_regs<A, E>(w0, xa, lds, td, base0, base_step, twi, P, Single);
-            __syncthreads();
-            tile_rounds<A, E, true>(xa, lds, td, base0, base_step, twi, P, Single, 1, false);
-            tile_store<A, T>(lds, out, io, Single, P);
+            if constexpr (Single)
+            {
+                u32 xa[E][A::rl];
+                u32 xb[E][A::rl];
+                tile_load<A, T>(lds, a, io);
+                __syncthreads();
+                tile_rounds<A, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
+                __syncthreads();
+                tile_load<A, T>(lds, b, io);
+                __syncthreads();
+                tile_rounds<A, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                    A::point(xa[k], xb[k], P);
+                // inverse round 0 groups the same elements as the forward's last round
+                dispatch_round_regs<A, E>(w0, xa, lds, td, base0, base_step, twi, P, true);
+                __syncthreads();
+                tile_rounds<A, E, true>(xa, lds, td, base0, base_step, twi, P, true, 1, false);
+                tile_store<A, T>(lds, out, io, true, P);
+            }
+            else
+            {
+                // two-pass row tile: NTT(a) rows go back to the workspace (a == out, in place) and
+                // are read per element next to NTT(b) in registers, so one register tile is live
+                u32 x[E][A::rl];
+                tile_load<A, T>(lds, a, io);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, base_step, twf, P, false, 0, false);
+                tile_store<A, T>(lds, out, io, false, P);
+                __syncthreads();
+                tile_load<A, T>(lds, b, io);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, base_step, twf, P, false, 0, true);
+                with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
+                dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
+                __syncthreads();
+                tile_rounds<A, E, true>(x, lds, td, base0, base_step, twi, P, false, 1, false);
+                tile_store<A, T>(lds, out, io, false, P);
+            }
         }
 
         // forward column pass of both inputs of the two-pass product (blockIdx.x = 2 * tile + which)

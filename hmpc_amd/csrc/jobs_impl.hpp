@@ -56,74 +56,6 @@ namespace hmpc_amd
             io.valid = 1 << log_n;
         }
 
-        // ------------------------------------------------------------------ second operands
-        // After the forward transform's last round (stages [s - W, s)) of a row-layout tile, the
-        // thread's unit g holds the K = 2^W consecutive tile elements ((tid + T g) << W) + k, so a
-        // second operand of the same elements is read straight from global memory (K
-        // consecutive words per limb plane; the workgroup wrote it itself just before, so it is
-        // served from L2) instead of occupying a second register tile.
-        //   Mode 0: x = a o x            (a = NTT(a) of a product, lazy)
-        //   Mode 1: x = k o u + x        (u = NTT(u), k = public-key polynomial, canonical)
-        // Only the tile's first `valid` elements are backed by memory (a whole-tile job of degree
-        // N < 4096 fills one of the tile's 4096 / N sequences); the others are left untouched.
-        template<typename A, int E, int W, int Mode>
-        __device__ __forceinline__ void with_operands(u32 (&x)[E][A::rl], int valid, u32 const* __restrict__ src,
-                                                      std::size_t src_count, std::size_t src_base, u32 const* __restrict__ key,
-                                                      std::size_t key_count, std::size_t key_base, typename A::params const& P)
-        {
-            constexpr int K = 1 << W;
-#pragma unroll
-            for (int g = 0; g < E / K; ++g)
-            {
-                int const e0 = (tid_x() + shape<E>::threads * g) << W;
-#pragma unroll
-                for (int k = 0; k < K; ++k)
-                {
-                    if (e0 + k >= valid)
-                        continue;
-                    u32 v[A::rl];
-                    A::load_global(v, src, src_count, src_base + e0 + k);
-                    if constexpr (Mode == 0)
-                    {
-                        A::point(v, x[g * K + k], P);
-#pragma unroll
-                        for (int l = 0; l < A::rl; ++l)
-                            x[g * K + k][l] = v[l];
-                    }
-                    else
-                    {
-                        u32 kv[A::rl];
-                        A::load_global(kv, key, key_count, key_base + e0 + k);
-                        A::point_add(x[g * K + k], v, kv, P);
-                    }
-                }
-            }
-        }
-
-        template<typename A, int E, int Mode>
-        __device__ __forceinline__ void with_operands_last(u32 (&x)[E][A::rl], tile_desc const& td, int valid, u32 const* src,
-                                                           std::size_t src_count, std::size_t src_base, u32 const* key,
-                                                           std::size_t key_count, std::size_t key_base,
-                                                           typename A::params const& P)
-        {
-            constexpr int WM = shape<E>::wmax;
-            int w = round_width<WM>(td.s, round_count<WM>(td.s) - 1, false);
-            if (w == WM)
-                with_operands<A, E, WM, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
-            else if (WM > 1 && w == 1)
-                with_operands<A, E, 1, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
-            else if constexpr (WM > 2)
-            {
-                if (w == 2)
-                    with_operands<A, E, 2, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
-                else if constexpr (WM > 3)
-                {
-                    if (w == 3)
-                        with_operands<A, E, 3, Mode>(x, valid, src, src_count, src_base, key, key_count, key_base, P);
-                }
-            }
-        }
-
         template<typename A, int E>
         __device__ __forceinline__ void scatter_last(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td)
         {
