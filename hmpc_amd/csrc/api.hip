@@ -280,10 +280,21 @@ struct hmpc_modulus_s
     std::map<int, table_pair> tables;
     std::map<int, int> elems; // fast path: elements per thread chosen per log_n (tuner)
     std::map<int, int> passes; // fast path: 1 = one launch per pass instead of the persistent job kernel
-    std::map<int, int> bgv_elems; // persistent BGV jobs: elements per thread per log_n (tuner)
+    std::map<int, int> bgv_elems; // fast BGV: elements per thread per log_n (tuner)
+    std::map<int, int> bgv_passes; // fast BGV, N > 4096: 1 = two launches over tiles, 0 = persistent jobs (tuner)
     int occupancy[17] = {};   // persistent job kernel: workgroups per CU by elements per thread
 
     int default_elems(int) const { return limbs == 4 ? 4 : 16; }
+    // tuned choice, else tiles for batches smaller than the device's resident workgroups
+    bool bgv_tiles(int log_n, std::size_t n) const
+    {
+        if (log_n <= 12)
+            return false;
+        auto p = bgv_passes.find(log_n);
+        if (p != bgv_passes.end())
+            return p->second != 0;
+        return n < 256;
+    }
     int bgv_elems_for(int log_n) const
     {
         auto e = bgv_elems.find(log_n);
@@ -825,6 +836,27 @@ int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread,
     return HMPC_OK;
 }
 
+int hmpc_modulus_set_bgv_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int tiles)
+{
+    if (!mod || log_n < 2 || log_n > 20)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
+    bool ok = elems_per_thread == 0 || (mod->limbs == 4 && (elems_per_thread == 4 || elems_per_thread == 8)) ||
+              (mod->limbs == 2 && (elems_per_thread == 8 || elems_per_thread == 16));
+    if (!ok)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "elements per thread: 4 or 8 (4 limbs), 8 or 16 (2 limbs), 0 = default");
+    if (tiles < -1 || tiles > 1)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "tiles: 1 (two launches over tiles), 0 (persistent jobs), -1 (default)");
+    if (elems_per_thread == 0)
+        mod->bgv_elems.erase(log_n);
+    else
+        mod->bgv_elems[log_n] = elems_per_thread;
+    if (tiles < 0)
+        mod->bgv_passes.erase(log_n);
+    else
+        mod->bgv_passes[log_n] = tiles;
+    return HMPC_OK;
+}
+
 int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* persistent, int* fast)
 {
     if (!mod || log_n < 2 || log_n > 20)
@@ -1249,7 +1281,16 @@ int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, co
     {
         T.elems = mod->bgv_elems_for(log_n);
         job_desc J = bgv_job(T, B, p_int, log_n, pk_a, pk_b, messages, c0, c1, static_cast<u64>(n) << log_n, 0, 0);
-        rc = run_uniform_jobs(*mod, T, J, n, s);
+        if (mod->bgv_tiles(log_n, n))
+        {
+            void* w = nullptr;
+            rc = ctx->workspace(s, 3 * static_cast<std::size_t>(mod->limbs) * (n << log_n) * 4, &w);
+            if (rc)
+                return rc;
+            HMPC_HIP(launch_fast_bgv_passes(mod->limbs, T, J, n, static_cast<u32*>(w), s));
+        }
+        else
+            rc = run_uniform_jobs(*mod, T, J, n, s);
         if (rc)
             return rc;
         nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
@@ -1748,7 +1789,8 @@ int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* 
     return HMPC_OK;
 }
 
-int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, void* stream, int* elems_out, float* ms_out)
+int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, void* stream, int* elems_out, int* tiles_out,
+                      float* ms_out)
 {
     if (!ctx || !mod || n == 0 || n > (size_t{1} << 16))
         return fail(HMPC_ERR_INVALID_ARGUMENT, "bad arguments");
@@ -1807,11 +1849,15 @@ int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, v
     }
     int const cand_l4[2] = {4, 8}, cand_l2[2] = {8, 16};
     int const* cand = mod->limbs == 4 ? cand_l4 : cand_l2;
-    int best_e = 0;
+    int best_e = 0, best_tiles = 0;
     float best = 0.0f;
-    for (int ci = 0; ci < 2; ++ci)
+    auto saved_passes = mod->bgv_passes;
+    for (int trial = 0; trial < (log_n > 12 ? 4 : 2); ++trial)
     {
+        int ci = trial & 1;
+        int tiles = trial >> 1;
         mod->bgv_elems[log_n] = cand[ci];
+        mod->bgv_passes[log_n] = tiles;
         rc = hmpc_bgv_encrypt(ctx, mod, log_n, n, pk_a, pk_b, msg, p_mont, 2, 40, 40, c0, c1, s);
         for (int r = 0; rc == HMPC_OK && r < 3; ++r)
         {
@@ -1826,6 +1872,7 @@ int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, v
         if (rc)
         {
             mod->bgv_elems = saved;
+            mod->bgv_passes = saved_passes;
             cleanup();
             return rc;
         }
@@ -1834,12 +1881,16 @@ int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, v
         {
             best = ms;
             best_e = cand[ci];
+            best_tiles = tiles;
         }
     }
     mod->bgv_elems[log_n] = best_e;
+    mod->bgv_passes[log_n] = best_tiles;
     cleanup();
     if (elems_out)
         *elems_out = best_e;
+    if (tiles_out)
+        *tiles_out = best_tiles;
     if (ms_out)
         *ms_out = best;
     return HMPC_OK;

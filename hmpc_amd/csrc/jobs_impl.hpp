@@ -375,6 +375,112 @@ namespace hmpc_amd
             }
         }
 
+        // ------------------------------------------------------------------ two-pass BGV (N > 4096)
+        // The batched encryption of n ciphertexts as two launches over tiles instead of one
+        // workgroup per ciphertext: every ciphertext's tiles run in parallel (short latency for
+        // small batches). Workspace: the three column-pass results [3][iol][n N].
+        //   column pass: blockIdx.x = (p * 3 + which) * tiles + t  (sample stream `which` of ciphertext p)
+        //   row pass:    blockIdx.x = p * tiles + t
+        template<typename A, int E, int Which>
+        __device__ __forceinline__ void bgv_column_tile(job_desc const& J, u32* lds, u32* dst, pass_geom const& gc, u32 ct,
+                                                        std::size_t count, u32 p, typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            u32 x[E][A::rl];
+            tile_desc td;
+            io_desc is;
+            cols_setup(gc, p * gc.blocks_per_poly + ct, count, td, is);
+            sample_tile<A, T, Which>(lds, J, true, gc.log_n1, gc.log_seqs, ct << gc.log_seqs, P);
+            __syncthreads();
+            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+            tile_store<A, T>(lds, dst, is, false, P);
+        }
+
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_cols(job_desc base, u32* __restrict__ ws,
+                                                                              pass_geom gc, u64 n, typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            u32 ct = blockIdx.x % gc.blocks_per_poly;
+            u32 r = blockIdx.x / gc.blocks_per_poly;
+            u32 which = r % 3;
+            u32 p = r / 3;
+            job_desc J = base;
+            J.poly += p;
+            J.msg_off += static_cast<u64>(p) << J.log_n;
+            std::size_t count = static_cast<std::size_t>(n) << J.log_n;
+            u32* dst = ws + which * A::iol * count;
+            if (which == 0)
+                bgv_column_tile<A, E, 0>(J, lds, dst, gc, ct, count, p, P);
+            else if (which == 1)
+                bgv_column_tile<A, E, 1>(J, lds, dst, gc, ct, count, p, P);
+            else
+                bgv_column_tile<A, E, 2>(J, lds, dst, gc, ct, count, p, P);
+        }
+
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows(job_desc base, u32* __restrict__ ws,
+                                                                              pass_geom gr, u64 n, typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            std::size_t count = static_cast<std::size_t>(n) << base.log_n;
+            u32 p = blockIdx.x / gr.blocks_per_poly;
+            u32 rt = blockIdx.x - p * gr.blocks_per_poly;
+            tile_desc td;
+            io_desc is, ik, io;
+            u32 hi0 = rows_setup(gr, blockIdx.x, count, td, is);  // workspace planes of n N
+            job_rows_tile(gr, rt, base.in_count, base.in_off, td, ik);  // public key (one polynomial)
+            io = is;
+            io.count = base.out_count;
+            io.base = base.out_off + is.base;
+            u32 base0 = (1u << gr.log_n1) + hi0;
+            u32* su = ws;
+            u32 x[E][A::rl];
+            // NTT(u) rows back into the workspace, read per element below
+            tile_load<A, T>(lds, su, is);
+            __syncthreads();
+            tile_rounds<A, E, false>(x, lds, td, base0, 1u, base.twf, P, false, 0, false);
+            tile_store<A, T>(lds, su, is, false, P);
+            __syncthreads();
+            for (int which = 1; which < 3; ++which)
+            {
+                tile_load<A, T>(lds, ws + which * A::iol * count, is);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                bool first = which == 1;
+                bgv_emit<A, E>(x, lds, td, su, count, is.base, first ? base.in0 : base.in1, ik, first ? base.out0 : base.out1,
+                               io, P);
+            }
+        }
+
+        template<typename A>
+        hipError_t launch_bgv_passes(int elems, job_desc const& base, u64 n, u32* ws, typename A::params const& P,
+                                     hipStream_t s)
+        {
+            return with_elems<A>(elems, [&](auto EC) -> hipError_t {
+                constexpr int E = decltype(EC)::value;
+                constexpr std::size_t bytes = tile_bytes<A>();
+                int log_n = static_cast<int>(base.log_n);
+                if (log_n <= tile_log)
+                    return hipErrorInvalidValue;
+                pass_geom gc = geom_for(log_n, false);
+                pass_geom gr = geom_for(log_n, true);
+                hipError_t e = hipSuccess;
+                if ((e = allow_lds(k_bgv_cols<A, E>, bytes)) != hipSuccess || (e = allow_lds(k_bgv_rows<A, E>, bytes)) != hipSuccess)
+                    return e;
+                {
+                    probe_scope ps(s, "fast bgv columns");
+                    k_bgv_cols<A, E><<<static_cast<unsigned>(3 * n * gc.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gc, n, P);
+                }
+                {
+                    probe_scope ps(s, "fast bgv rows");
+                    k_bgv_rows<A, E><<<static_cast<unsigned>(n * gr.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gr, n, P);
+                }
+                return hipGetLastError();
+            });
+        }
+
         template<typename A>
         hipError_t jobs_occupancy(int elems, int* blocks_per_cu)
         {

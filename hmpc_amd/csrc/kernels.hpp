@@ -84,6 +84,8 @@ namespace hmpc_amd
     hipError_t fast_jobs_occupancy(int L, int elems, int* blocks_per_cu);
     hipError_t launch_fast_jobs(int L, transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
                                 unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
+    // batched BGV of n ciphertexts, N > 4096, as two launches over tiles (workspace [3][L][n N])
+    hipError_t launch_fast_bgv_passes(int L, transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
 
     hipError_t launch_make_twiddles(int L, u32* table, int log_n, u32 const* root, u32 const* norm, u32 const* one,
                                     bool inverse, mod_params const& P, hipStream_t s);

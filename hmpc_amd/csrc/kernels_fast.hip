@@ -22,6 +22,9 @@ namespace hmpc_amd
     hipError_t launch_fast_jobs_l4(transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
                                    unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
 
+    hipError_t launch_fast_bgv_passes_l2(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
+    hipError_t launch_fast_bgv_passes_l4(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
+
     bool fast_path_supported(int L, int form, int log_n)
     {
         // special-form moduli only; whole-polynomial tiles up to 2^12, two passes of at most
@@ -71,6 +74,15 @@ namespace hmpc_amd
             return launch_fast_jobs_l4(T, table, base, njobs, grid, counter, scratch, slot_words, s);
         if (L == 2)
             return launch_fast_jobs_l2(T, table, base, njobs, grid, counter, scratch, slot_words, s);
+        return hipErrorInvalidValue;
+    }
+
+    hipError_t launch_fast_bgv_passes(int L, transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s)
+    {
+        if (L == 4)
+            return launch_fast_bgv_passes_l4(T, base, n, ws, s);
+        if (L == 2)
+            return launch_fast_bgv_passes_l2(T, base, n, ws, s);
         return hipErrorInvalidValue;
     }
 }

@@ -30,4 +30,9 @@ namespace hmpc_amd
         return fast::launch_jobs<fast::arith_gold>(T.elems, table, base, njobs, grid, counter, scratch, slot_words,
                                                    gold_params(T), s);
     }
+
+    hipError_t launch_fast_bgv_passes_l2(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s)
+    {
+        return fast::launch_bgv_passes<fast::arith_gold>(T.elems, base, n, ws, gold_params(T), s);
+    }
 }

@@ -13,4 +13,9 @@ namespace hmpc_amd
     {
         return fast::launch_jobs<fast::arith_r26_round_prefetch>(T.elems, table, base, njobs, grid, counter, scratch, slot_words, T.r26, s);
     }
+
+    hipError_t launch_fast_bgv_passes_l4(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s)
+    {
+        return fast::launch_bgv_passes<fast::arith_r26>(T.elems, base, n, ws, T.r26, s);
+    }
 }

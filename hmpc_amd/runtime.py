@@ -44,9 +44,10 @@ SIGNATURES = {
     "hmpc_modulus_prepare": (_i, [_vp, _i]),
     "hmpc_modulus_set_kernels": (_i, [_vp, _i]),
     "hmpc_modulus_set_schedule": (_i, [_vp, _i, _i, _i]),
+    "hmpc_modulus_set_bgv_schedule": (_i, [_vp, _i, _i, _i]),
     "hmpc_modulus_get_schedule": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "hmpc_autotune": (_i, [_vp, _i, _sz, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
-    "hmpc_autotune_bgv": (_i, [_vp, _vp, _i, _sz, _vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
+    "hmpc_autotune_bgv": (_i, [_vp, _vp, _i, _sz, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
     "hmpc_modulus_root": (_i, [_vp, _i, _i, _u32p]),
     "hmpc_modulus_twiddles": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
     "hmpc_ntt": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp]),
@@ -358,15 +359,21 @@ class Modulus:
         -1 = unchanged"""
         check(_lib.hmpc_modulus_set_schedule(self.handle, log_n, elems_per_thread, persistent))
 
+    def set_bgv_schedule(self, log_n: int, elems_per_thread: int = 0, tiles: int = -1):
+        """fast BGV: elements per thread (0 = default); tiles 1 = two launches over tiles,
+        0 = persistent jobs, -1 = default"""
+        check(_lib.hmpc_modulus_set_bgv_schedule(self.handle, log_n, elems_per_thread, tiles))
+
     def schedule(self, log_n: int):
         e, p, f = _i(), _i(), _i()
         check(_lib.hmpc_modulus_get_schedule(self.handle, log_n, ctypes.byref(e), ctypes.byref(p), ctypes.byref(f)))
         return {"elems_per_thread": e.value, "persistent": bool(p.value), "fast": bool(f.value)}
 
     def autotune_bgv(self, log_n: int, n: int, stream=None) -> dict:
-        e, ms = _i(), ctypes.c_float()
-        check(_lib.hmpc_autotune_bgv(self.ctx.handle, self.handle, log_n, n, _sp(stream), ctypes.byref(e), ctypes.byref(ms)))
-        return {"elems_per_thread": e.value, "ms": ms.value}
+        e, t, ms = _i(), _i(), ctypes.c_float()
+        check(_lib.hmpc_autotune_bgv(self.ctx.handle, self.handle, log_n, n, _sp(stream), ctypes.byref(e), ctypes.byref(t),
+                                     ctypes.byref(ms)))
+        return {"elems_per_thread": e.value, "tiles": bool(t.value), "ms": ms.value}
 
     def autotune(self, log_n: int, pairs: int, stream=None) -> dict:
         e, p, ms = _i(), _i(), ctypes.c_float()

@@ -99,6 +99,11 @@ int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
  * job kernel in which each workgroup runs whole products / encryptions; 0: one launch per
  * transform pass; -1: unchanged). Chosen by hmpc_autotune; results are identical. */
 int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int persistent);
+/* Schedule of fast-path BGV encryption for N = 2^log_n: elements per thread (0 = default) and, for
+ * N > 4096, tiles = 1 (a column launch and a row launch over all ciphertexts' tiles), 0 (the
+ * persistent job kernel, one workgroup per ciphertext) or -1 (default: tiles below 256
+ * ciphertexts). Chosen by hmpc_autotune_bgv; results are identical. */
+int hmpc_modulus_set_bgv_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int tiles);
 /* Current schedule and whether the fast kernels serve this (modulus, N) at all. */
 int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* persistent, int* fast);
 /* Autotuner: times batched products of `pairs` synthetic polynomial pairs of N = 2^log_n under
@@ -107,12 +112,15 @@ int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread
  * hmpc, whose kernels have no schedule. */
 int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* elems_per_thread, int* persistent,
                   float* ms);
-/* Autotuner of batched BGV encryption (always the persistent job kernel on the fast moduli):
- * times n synthetic encryptions of N = 2^log_n per elements-per-thread choice and installs the
- * fastest for hmpc_bgv_encrypt and batch jobs of this modulus and N. The context's RNG state is
- * restored afterwards. Blocking. */
+/* Autotuner of batched BGV encryption on the fast moduli: times n synthetic encryptions of
+ * N = 2^log_n per elements-per-thread choice and, for N > 4096, per launch structure (tiles = 1: a
+ * column launch and a row launch over every ciphertext's tiles; 0: the persistent job kernel, one
+ * workgroup per ciphertext) and installs the fastest for hmpc_bgv_encrypt (batch and serving jobs
+ * use the persistent kernel with the chosen elements per thread). Untuned, hmpc_bgv_encrypt uses
+ * tiles for batches below 256 ciphertexts. The context's RNG state is restored afterwards.
+ * Blocking. */
 int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, void* stream, int* elems_per_thread,
-                      float* ms);
+                      int* tiles, float* ms);
 /* Build (once) and keep resident the forward/inverse tables for N = 2^log_n (2 <= log_n <= 20),
  * 2N must divide q - 1. Blocking. */
 int hmpc_modulus_prepare(hmpc_modulus mod, int log_n);

@@ -189,7 +189,8 @@ def _keygen(om, log_n, t_mont):
     return A, B, s
 
 
-@pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q128A, 12, 2), (Q64, 10, 3), (Q128A, 16, 1)])
+@pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q128A, 12, 2), (Q64, 10, 3), (Q128A, 16, 1), (Q64, 15, 2),
+                                       (Q128A, 13, 4)])
 def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     gm, om = mods(ctx, q)
     say(f"bgv q={hex(q)[:12]} log_n={log_n} n={n}")
@@ -202,14 +203,19 @@ def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     dA, dB, dm = gm.upload(A), gm.upload(B), ctx.upload(msgs)
     pm = oracle.int_to_limbs(t_mont, gm.limbs)
     gp, _ = mods(ctx, q, 1)
-    for mode in ("jobs", "portable", "plain"):
-        say(f"  bgv mode={mode}")
+    fast_elems = (4, 8) if gm.limbs == 4 else (8, 16)
+    modes = [("jobs", e) for e in fast_elems] + ([("tiles", e) for e in fast_elems] if log_n > 12 else [])
+    for mode, elems in modes + [("portable", 0), ("plain", 0)]:
+        say(f"  bgv mode={mode} elems={elems}")
+        if mode in ("jobs", "tiles"):
+            gm.set_bgv_schedule(log_n, elems, 1 if mode == "tiles" else 0)
         ctx.set_rng(enc_key, [0, 0, 0])
         g = gp if mode == "portable" else gm
         c0, c1 = g.bgv_encrypt(log_n, n, dA, dB, dm, pm, plain=mode == "plain")
-        assert np.array_equal(gm.download(c0, n * N), want0), f"c0 mode={mode}"
-        assert np.array_equal(gm.download(c1, n * N), want1), f"c1 mode={mode}"
+        assert np.array_equal(gm.download(c0, n * N), want0), f"c0 mode={mode} elems={elems}"
+        assert np.array_equal(gm.download(c1, n * N), want1), f"c1 mode={mode} elems={elems}"
         assert ctx.rng[1].tolist() == nonce.tolist()
+    gm.set_bgv_schedule(log_n, 0, -1)
     dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
     assert np.array_equal(dec, msgs)
     assert np.array_equal(dec, om.bgv_decrypt(log_n, s, want0, want1))
@@ -296,7 +302,7 @@ def test_autotune_bgv_keeps_results_and_rng(ctx):
     enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
     ctx.set_rng(enc_key, [5, 0, 0])
     best = gm.autotune_bgv(log_n, n)
-    assert best["elems_per_thread"] in (4, 8) and best["ms"] > 0
+    assert best["elems_per_thread"] in (4, 8) and best["ms"] > 0 and best["tiles"] in (True, False)
     assert ctx.rng[1].tolist() == [5, 0, 0]
     N = 1 << log_n
     t_mont = om.from_unsigned(1 << 32)
