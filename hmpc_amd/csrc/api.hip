@@ -23,6 +23,7 @@
 #include <type_traits>
 #include <deque>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -299,6 +300,21 @@ struct hmpc_modulus_s
     {
         auto e = bgv_elems.find(log_n);
         return e != bgv_elems.end() ? e->second : (limbs == 4 ? 4 : 8);
+    }
+    // tile launches over job tables (batches, serving): every two-pass BGV job unless the tuner
+    // chose the persistent kernel for its degree; 8 elements per thread unless tuned
+    bool table_tiles(int log_n) const
+    {
+        if (log_n <= 12)
+            return false;
+        auto p = bgv_passes.find(log_n);
+        return p == bgv_passes.end() || p->second != 0;
+    }
+    int table_elems_for(int log_n) const
+    {
+        auto p = bgv_passes.find(log_n);
+        auto e = bgv_elems.find(log_n);
+        return p != bgv_passes.end() && p->second != 0 && e != bgv_elems.end() ? e->second : 8;
     }
     bool persistent(int log_n) const
     {
@@ -1193,6 +1209,113 @@ namespace
     }
 }
 
+namespace
+{
+    // Tile launches over tables of BGV jobs of two-pass degrees (one ciphertext each), grouped by
+    // (modulus, elements per thread) and cut into chunks whose column workspaces stay within
+    // tile_ws_cap_words; the chunks of one stream run in order and share one workspace.
+    constexpr u64 tile_ws_cap_words = u64{1} << 27;  // 512 MiB
+
+    struct tile_chunk
+    {
+        hmpc_modulus mod = nullptr;
+        int elems = 0;
+        int log_n = 0;  // a degree of the chunk (the arithmetic parameters do not depend on it)
+        std::size_t job_off = 0, njobs = 0;
+        std::size_t cols_off = 0, cols_blocks = 0, rows_off = 0, rows_blocks = 0;  // into the map
+        u64 ws_words = 0;
+    };
+
+    struct tile_plan
+    {
+        std::vector<job_desc> jobs;
+        std::vector<u32> map;  // per block: job index << 10 | local tile
+        std::vector<tile_chunk> chunks;
+        u64 ws_words = 0;      // the largest chunk's workspace
+
+        bool empty() const { return jobs.empty(); }
+
+        // in: (modulus, elements per thread, job); order is kept within a group
+        void build(std::vector<std::tuple<hmpc_modulus, int, job_desc>> const& in)
+        {
+            jobs.clear();
+            map.clear();
+            chunks.clear();
+            ws_words = 0;
+            std::vector<std::pair<hmpc_modulus, int>> groups;
+            for (auto const& t : in)
+            {
+                auto g = std::make_pair(std::get<0>(t), std::get<1>(t));
+                if (std::find(groups.begin(), groups.end(), g) == groups.end())
+                    groups.push_back(g);
+            }
+            for (auto const& g : groups)
+            {
+                tile_chunk c;
+                auto close = [&]() {
+                    if (c.njobs)
+                    {
+                        ws_words = std::max(ws_words, c.ws_words);
+                        chunks.push_back(c);
+                    }
+                    c = tile_chunk{};
+                };
+                for (auto const& t : in)
+                {
+                    if (std::get<0>(t) != g.first || std::get<1>(t) != g.second)
+                        continue;
+                    job_desc J = std::get<2>(t);
+                    u64 words = 3 * static_cast<u64>(g.first->limbs) << J.log_n;
+                    if (c.njobs && (c.ws_words + words > tile_ws_cap_words || c.njobs == (1u << 22)))
+                        close();
+                    if (!c.njobs)
+                    {
+                        c.mod = g.first;
+                        c.elems = g.second;
+                        c.log_n = static_cast<int>(J.log_n);
+                        c.job_off = jobs.size();
+                    }
+                    J.ws_off = c.ws_words;
+                    c.ws_words += words;
+                    c.njobs += 1;
+                    jobs.push_back(J);
+                }
+                close();
+            }
+            for (auto& c : chunks)
+            {
+                c.cols_off = map.size();
+                for (std::size_t j = 0; j < c.njobs; ++j)
+                    for (u32 l = 0, e = bgv_tiles_per_job(static_cast<int>(jobs[c.job_off + j].log_n), false); l < e; ++l)
+                        map.push_back(static_cast<u32>(j) << 10 | l);
+                c.cols_blocks = map.size() - c.cols_off;
+                c.rows_off = map.size();
+                for (std::size_t j = 0; j < c.njobs; ++j)
+                    for (u32 l = 0, e = bgv_tiles_per_job(static_cast<int>(jobs[c.job_off + j].log_n), true); l < e; ++l)
+                        map.push_back(static_cast<u32>(j) << 10 | l);
+                c.rows_blocks = map.size() - c.rows_off;
+            }
+        }
+
+        // device copies of `jobs` and `map` at d_jobs / d_map; ws holds ws_words
+        int launch(job_desc const* d_jobs, u32 const* d_map, u32* ws, hipStream_t s) const
+        {
+            for (auto const& c : chunks)
+            {
+                transform_args T;
+                int rc = c.mod->args(c.log_n, T);
+                if (rc)
+                    return rc;
+                T.elems = c.elems;
+                HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, d_jobs + c.job_off, d_map + c.cols_off,
+                                               static_cast<u32>(c.cols_blocks), d_map + c.rows_off,
+                                               static_cast<u32>(c.rows_blocks), ws, s));
+            }
+            return HMPC_OK;
+        }
+    };
+}
+
 // ------------------------------------------------------------------ fused batches
 int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out, void* stream)
 {
@@ -1450,6 +1573,9 @@ struct hmpc_batch_s
     hmpc_context ctx = nullptr;
     std::vector<entry> entries;
     std::vector<job_class> classes;
+    tile_plan tiles;               // BGV jobs of two-pass degrees
+    std::size_t tile_jobs_off = 0; // byte offsets into mem
+    std::size_t tile_map_off = 0;
     bool built = false;
     void* mem = nullptr;
     std::size_t mem_bytes = 0;
@@ -1471,6 +1597,7 @@ struct hmpc_batch_s
         mem = nullptr;
         mem_bytes = 0;
         classes.clear();
+        tiles = tile_plan{};
         built = false;
     }
 
@@ -1483,16 +1610,23 @@ struct hmpc_batch_s
             (void)hipStreamDestroy(capture_stream);
     }
 
-    // group by (modulus, elements per thread), order each class longest job first (LPT), size
-    // the grids and the shared scratch, upload the job tables
+    // BGV jobs of two-pass degrees go to tile launches over a job table; the others are grouped
+    // by (modulus, elements per thread), each class ordered longest job first (LPT) for the
+    // persistent kernel. Sizes the grids and the shared scratch, uploads the tables.
     int build()
     {
         device_guard g(ctx->device);
         std::map<std::pair<hmpc_modulus, int>, std::size_t> index;
         std::vector<std::vector<entry const*>> members;
+        std::vector<std::tuple<hmpc_modulus, int, job_desc>> tiled;
         for (auto const& e : entries)
         {
             int log_n = static_cast<int>(e.d.log_n);
+            if (e.d.type == job_bgv && e.mod->table_tiles(log_n))
+            {
+                tiled.emplace_back(e.mod, e.mod->table_elems_for(log_n), e.d);
+                continue;
+            }
             auto it = e.mod->elems.find(log_n);
             int elems = e.d.type == job_bgv ? e.mod->bgv_elems_for(log_n)
                                             : (it != e.mod->elems.end() ? it->second : e.mod->default_elems(log_n));
@@ -1531,19 +1665,41 @@ struct hmpc_batch_s
             c.counter_off = off;
             off += counter_bytes;
         }
+        tiles.build(tiled);
+        tile_jobs_off = off;
+        off += (tiles.jobs.size() * sizeof(job_desc) + 255) / 256 * 256;
+        tile_map_off = off;
+        off += (tiles.map.size() * 4 + 255) / 256 * 256;
+        scratch_words = std::max(scratch_words, tiles.ws_words);
         scratch_off = off;
         mem_bytes = off + scratch_words * 4;
         HMPC_HIP(hipMalloc(&mem, mem_bytes));
         for (auto const& c : classes)
             HMPC_HIP(hipMemcpy(static_cast<char*>(mem) + c.table_off, c.jobs.data(), c.jobs.size() * sizeof(job_desc),
                                hipMemcpyHostToDevice));
+        if (!tiles.empty())
+        {
+            HMPC_HIP(hipMemcpy(static_cast<char*>(mem) + tile_jobs_off, tiles.jobs.data(), tiles.jobs.size() * sizeof(job_desc),
+                               hipMemcpyHostToDevice));
+            HMPC_HIP(hipMemcpy(static_cast<char*>(mem) + tile_map_off, tiles.map.data(), tiles.map.size() * 4,
+                               hipMemcpyHostToDevice));
+        }
         built = true;
         return HMPC_OK;
     }
 
-    // one persistent launch per class, in sequence on `s` (they share the scratch)
+    // the tile launches, then one persistent launch per class, in sequence on `s` (all share the
+    // scratch)
     int launch(hipStream_t s)
     {
+        if (!tiles.empty())
+        {
+            int rc = tiles.launch(reinterpret_cast<job_desc const*>(static_cast<char*>(mem) + tile_jobs_off),
+                                  reinterpret_cast<u32 const*>(static_cast<char*>(mem) + tile_map_off),
+                                  reinterpret_cast<u32*>(static_cast<char*>(mem) + scratch_off), s);
+            if (rc)
+                return rc;
+        }
         for (auto const& c : classes)
         {
             transform_args T;
@@ -1690,7 +1846,7 @@ int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* dev
     if (jobs)
         *jobs = batch->entries.size();
     if (classes)
-        *classes = batch->classes.size();
+        *classes = batch->classes.size() + batch->tiles.chunks.size();
     if (device_bytes)
         *device_bytes = batch->mem_bytes;
     return HMPC_OK;
@@ -2042,6 +2198,10 @@ namespace
         u32* counters = nullptr;      // one 256-byte counter per class
         u32* scratch = nullptr;
         std::size_t scratch_words = 0;
+        tile_plan tiles;              // the current launch's encryptions at two-pass degrees
+        u32* d_map = nullptr;
+        u32* h_map = nullptr;         // pinned staging of the tile map
+        std::size_t map_cap = 0;
         std::vector<u64> requests;
         bool busy = false;
         std::atomic<bool> done{false};
@@ -2183,6 +2343,8 @@ struct hmpc_server_s
         return true;
     }
 
+    // Encryptions at two-pass degrees run as tile launches over a table of all their
+    // ciphertexts; products and N = 4096 encryptions as persistent launches per class.
     int launch(server_slot& s, std::vector<u64> const& take, std::size_t jobs)
     {
         struct cls_info
@@ -2193,6 +2355,7 @@ struct hmpc_server_s
             std::vector<std::pair<u64, job_desc>> jobs;  // (cost, job)
         };
         std::vector<cls_info> classes;
+        std::vector<std::tuple<hmpc_modulus, int, job_desc>> tiled;
         for (u64 id : take)
         {
             auto& r = reqs[id];
@@ -2203,17 +2366,22 @@ struct hmpc_server_s
                 error = last_error;
                 return rc;
             }
+            bool tile = r.kind == 0 && r.mod->table_tiles(r.log_n);
             int elems = r.kind == 0 ? r.mod->bgv_elems_for(r.log_n) : (r.mod->elems.count(r.log_n) ? r.mod->elems[r.log_n] : r.mod->default_elems(r.log_n));
-            auto it = std::find_if(classes.begin(), classes.end(), [&](cls_info const& c) { return c.mod == r.mod && c.elems == elems; });
-            if (it == classes.end())
+            auto it = classes.end();
+            if (!tile)
             {
-                if (classes.size() == max_classes_per_launch)
+                it = std::find_if(classes.begin(), classes.end(), [&](cls_info const& c) { return c.mod == r.mod && c.elems == elems; });
+                if (it == classes.end())
                 {
-                    error = "too many kernel classes in one launch";
-                    return HMPC_ERR_INVALID_ARGUMENT;
+                    if (classes.size() == max_classes_per_launch)
+                    {
+                        error = "too many kernel classes in one launch";
+                        return HMPC_ERR_INVALID_ARGUMENT;
+                    }
+                    classes.push_back({r.mod, elems, r.log_n, {}});
+                    it = classes.end() - 1;
                 }
-                classes.push_back({r.mod, elems, r.log_n, {}});
-                it = classes.end() - 1;
             }
             u64 N = u64{1} << r.log_n;
             u64 count = static_cast<u64>(r.n) << r.log_n;
@@ -2233,8 +2401,14 @@ struct hmpc_server_s
                 B.bits_w = k.bits[2];
                 u32* c1 = out + static_cast<std::size_t>(r.mod->limbs) * count;
                 for (std::size_t i = 0; i < r.n; ++i)
-                    it->jobs.push_back({4 * N * r.log_n, bgv_job(T, B, k.p_int, r.log_n, k.pk_a, k.pk_b, r.messages, out, c1,
-                                                                 count, i * N, static_cast<u32>(i))});
+                {
+                    job_desc J = bgv_job(T, B, k.p_int, r.log_n, k.pk_a, k.pk_b, r.messages, out, c1, count, i * N,
+                                         static_cast<u32>(i));
+                    if (tile)
+                        tiled.emplace_back(r.mod, r.mod->table_elems_for(r.log_n), J);
+                    else
+                        it->jobs.push_back({4 * N * r.log_n, J});
+                }
             }
             else
             {
@@ -2278,6 +2452,21 @@ struct hmpc_server_s
             slot_words[c] = w;
             need = std::max<std::size_t>(need, static_cast<std::size_t>(grids[c]) * w);
         }
+        s.tiles.build(tiled);
+        need = std::max<std::size_t>(need, s.tiles.ws_words);
+        if (s.map_cap < s.tiles.map.size())
+        {
+            if (s.d_map)
+                HMPC_HIP(hipFree(s.d_map));
+            if (s.h_map)
+                HMPC_HIP(hipHostFree(s.h_map));
+            s.d_map = nullptr;
+            s.h_map = nullptr;
+            std::size_t cap = std::max<std::size_t>(s.tiles.map.size(), 1 << 16);
+            HMPC_HIP(hipMalloc(&s.d_map, cap * 4));
+            HMPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_map), cap * 4));
+            s.map_cap = cap;
+        }
         if (s.scratch_words < need)
         {
             if (s.scratch)
@@ -2290,7 +2479,21 @@ struct hmpc_server_s
         for (auto const& cl : classes)
             for (auto const& j : cl.jobs)
                 s.h_table[off++] = j.second;
+        std::size_t tile_off = off;
+        std::copy(s.tiles.jobs.begin(), s.tiles.jobs.end(), s.h_table + off);
+        off += s.tiles.jobs.size();
         HMPC_HIP(hipMemcpyAsync(s.d_table, s.h_table, off * sizeof(job_desc), hipMemcpyHostToDevice, s.stream));
+        if (!s.tiles.empty())
+        {
+            std::copy(s.tiles.map.begin(), s.tiles.map.end(), s.h_map);
+            HMPC_HIP(hipMemcpyAsync(s.d_map, s.h_map, s.tiles.map.size() * 4, hipMemcpyHostToDevice, s.stream));
+            int rc = s.tiles.launch(s.d_table + tile_off, s.d_map, s.scratch, s.stream);
+            if (rc)
+            {
+                error = last_error;
+                return rc;
+            }
+        }
         off = 0;
         for (std::size_t c = 0; c < classes.size(); ++c)
         {
@@ -2359,6 +2562,10 @@ int hmpc_server_destroy(hmpc_server srv)
             (void)hipFree(s->d_table);
         if (s->h_table)
             (void)hipHostFree(s->h_table);
+        if (s->d_map)
+            (void)hipFree(s->d_map);
+        if (s->h_map)
+            (void)hipHostFree(s->h_map);
         if (s->counters)
             (void)hipFree(s->counters);
         if (s->scratch)

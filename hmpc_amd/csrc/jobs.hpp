@@ -39,6 +39,7 @@ namespace hmpc_amd
         u64 out_count, out_off;
         u32 const* msg;    // bgv: messages as u32 words, msg[msg_off + k]
         u64 msg_off;
+        u64 ws_off;        // tile launches over a job table: the job's workspace region (words)
     };
 
     // Scratch words one workgroup needs for a job of degree 2^log_n: the transformed first

@@ -481,6 +481,93 @@ namespace hmpc_amd
             });
         }
 
+        // ------------------------------------------------------------------ tiles over a job table
+        // The two launches above for a table of BGV jobs of any two-pass degrees, keys and nonces
+        // (one ciphertext each): block b of a launch works on job map[b] >> 10, local tile
+        // map[b] & 1023 (at most 3 x 256 tiles per job, at N = 2^20). Each job keeps its three
+        // column results in its own workspace region [3][iol][N] at ws + ws_off.
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_cols_table(job_desc const* __restrict__ jobs,
+                                                                                    u32 const* __restrict__ map,
+                                                                                    u32* __restrict__ ws,
+                                                                                    typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            u32 local = m & 1023u;
+            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
+            u32 which = local / gc.blocks_per_poly;
+            u32 ct = local - which * gc.blocks_per_poly;
+            std::size_t N = std::size_t{1} << J.log_n;
+            u32* dst = ws + J.ws_off + which * A::iol * N;
+            if (which == 0)
+                bgv_column_tile<A, E, 0>(J, lds, dst, gc, ct, N, 0, P);
+            else if (which == 1)
+                bgv_column_tile<A, E, 1>(J, lds, dst, gc, ct, N, 0, P);
+            else
+                bgv_column_tile<A, E, 2>(J, lds, dst, gc, ct, N, 0, P);
+        }
+
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows_table(job_desc const* __restrict__ jobs,
+                                                                                    u32 const* __restrict__ map,
+                                                                                    u32* __restrict__ ws,
+                                                                                    typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            u32 rt = m & 1023u;
+            pass_geom gr = geom_for(static_cast<int>(J.log_n), true);
+            std::size_t N = std::size_t{1} << J.log_n;
+            u32* su = ws + J.ws_off;
+            tile_desc td;
+            io_desc is, ik, io;
+            u32 hi0 = job_rows_tile(gr, rt, N, 0, td, is);
+            job_rows_tile(gr, rt, J.in_count, J.in_off, td, ik);
+            job_rows_tile(gr, rt, J.out_count, J.out_off, td, io);
+            u32 base0 = (1u << gr.log_n1) + hi0;
+            u32 x[E][A::rl];
+            tile_load<A, T>(lds, su, is);
+            __syncthreads();
+            tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+            tile_store<A, T>(lds, su, is, false, P);
+            __syncthreads();
+            for (int which = 1; which < 3; ++which)
+            {
+                tile_load<A, T>(lds, su + which * A::iol * N, is);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                bool first = which == 1;
+                bgv_emit<A, E>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+            }
+        }
+
+        template<typename A>
+        hipError_t launch_bgv_table(int elems, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
+                                    u32 const* map_rows, u32 rows_blocks, u32* ws, typename A::params const& P, hipStream_t s)
+        {
+            return with_elems<A>(elems, [&](auto EC) -> hipError_t {
+                constexpr int E = decltype(EC)::value;
+                constexpr std::size_t bytes = tile_bytes<A>();
+                hipError_t e = hipSuccess;
+                if ((e = allow_lds(k_bgv_cols_table<A, E>, bytes)) != hipSuccess ||
+                    (e = allow_lds(k_bgv_rows_table<A, E>, bytes)) != hipSuccess)
+                    return e;
+                {
+                    probe_scope ps(s, "fast bgv columns");
+                    k_bgv_cols_table<A, E><<<cols_blocks, tile / E, bytes, s>>>(jobs, map_cols, ws, P);
+                }
+                {
+                    probe_scope ps(s, "fast bgv rows");
+                    k_bgv_rows_table<A, E><<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
+                }
+                return hipGetLastError();
+            });
+        }
+
         template<typename A>
         hipError_t jobs_occupancy(int elems, int* blocks_per_cu)
         {

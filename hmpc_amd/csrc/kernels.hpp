@@ -86,6 +86,13 @@ namespace hmpc_amd
                                 unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
     // batched BGV of n ciphertexts, N > 4096, as two launches over tiles (workspace [3][L][n N])
     hipError_t launch_fast_bgv_passes(int L, transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
+    // the same over a table of BGV jobs of two-pass degrees (one ciphertext each): block b runs
+    // job map[b] >> 10, local tile map[b] & 1023; job j's column results live at
+    // ws + jobs[j].ws_off ([3][L][N])
+    hipError_t launch_fast_bgv_table(int L, transform_args const& T, job_desc const* jobs, u32 const* map_cols,
+                                     u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s);
+    // blocks per job of the tile launches (column launch: 3 streams x column tiles; row launch: row tiles)
+    unsigned bgv_tiles_per_job(int log_n, bool rows);
 
     hipError_t launch_make_twiddles(int L, u32* table, int log_n, u32 const* root, u32 const* norm, u32 const* one,
                                     bool inverse, mod_params const& P, hipStream_t s);

@@ -202,7 +202,9 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
 /* ------------------------------------------------------------------ job batches
  * Many independent jobs of uneven degree (2^12 .. 2^16 and beyond) and modulus in persistent
  * launches: one launch per (modulus, schedule) class, in which each workgroup claims jobs
- * longest first and runs each from inputs to outputs. Each job's polynomials are single
+ * longest first and runs each from inputs to outputs. BGV jobs at N > 4096 instead run as two
+ * launches over a table of all their tiles (column pass, row pass), unless the tuner chose the
+ * persistent kernel for that degree (hmpc_autotune_bgv). Each job's polynomials are single
  * polynomials ([L][N] buffers). hmpc has no batch object: a job list stands for the sequence of
  * queue(...) calls it replaces (hmpc/comp/queue.hpp:416-529), with the same results.
  * Moduli must be served by the fast kernels (hmpc_modulus_get_schedule: fast = 1). */
@@ -221,7 +223,8 @@ int hmpc_batch_add_bgv_encrypt(hmpc_batch batch, hmpc_modulus mod, int log_n, co
  * use_graph the launches are captured once into a hipGraph and later runs replay it (same
  * inputs, nonces and outputs). Asynchronous on `stream`. */
 int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream);
-/* jobs, kernel classes (after the first run), device bytes held (tables, counters, scratch) */
+/* jobs, launch groups after the first run (persistent-kernel classes plus tile-launch chunks of
+ * BGV jobs at N > 4096), device bytes held (tables, counters, scratch) */
 int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* device_bytes);
 
 /* ------------------------------------------------------------------ serving

@@ -332,13 +332,27 @@ def test_autotune_installs_a_correct_schedule(ctx):
 # uneven job list: products and encryptions, N from 2^12 to 2^16, both fast moduli
 BATCH_JOBS = [("mul", Q64, 12), ("bgv", Q128A, 13), ("mul", Q128A, 16), ("bgv", Q64, 14), ("mul", Q128A, 12),
               ("bgv", Q128A, 16), ("mul", Q64, 15), ("bgv", Q64, 12), ("mul", Q128A, 14), ("mul", Q64, 16),
-              ("bgv", Q128A, 12), ("mul", Q128A, 13)]
+              ("bgv", Q128A, 12), ("mul", Q128A, 13), ("bgv", Q128A, 14), ("bgv", Q64, 16), ("bgv", Q128A, 13)]
 
 
-def test_batch_of_uneven_jobs_matches_oracle(ctx):
-    """Persistent job kernels (one per modulus class) over an uneven job list, eager and
-    replayed from a hipGraph three times with host work between replays; every job's output
-    equals the oracle's, and each encryption consumes the nonces a queue call would."""
+@pytest.mark.parametrize("bgv_structure", ["tiles", "jobs"])
+def test_batch_of_uneven_jobs_matches_oracle(ctx, bgv_structure):
+    """Persistent job kernels (one per modulus class) and, for encryptions at N > 4096, tile
+    launches over a job table (or, when so scheduled, the persistent kernel) over an uneven job
+    list, eager and replayed from a hipGraph three times with host work between replays; every
+    job's output equals the oracle's, and each encryption consumes the nonces a queue call would."""
+    two_pass = sorted({(q, log_n) for kind, q, log_n in BATCH_JOBS if kind == "bgv" and log_n > 12})
+    if bgv_structure == "jobs":
+        for q, log_n in two_pass:
+            mods(ctx, q)[0].set_bgv_schedule(log_n, 0, 0)
+    try:
+        _run_uneven_batch(ctx)
+    finally:
+        for q, log_n in two_pass:
+            mods(ctx, q)[0].set_bgv_schedule(log_n, 0, -1)
+
+
+def _run_uneven_batch(ctx):
     enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
     ctx.set_rng(enc_key, [0, 0, 0])
     batch = hmpc_amd.Batch(ctx)

@@ -138,3 +138,27 @@ def test_budget_admission_and_release(env):
     srv.release(r2)
     srv.release(r3)
     srv.close()
+
+
+def test_large_request_spans_tile_chunks(env):
+    """700 ciphertexts at N = 2^14 (q128a) hold 700 x 3 x 4 x 2^14 words of column results, more
+    than one tile launch's 2^27-word workspace: the request runs as two chunks in sequence and
+    equals the direct batched encryption with its nonce (the persistent kernel at this count)."""
+    ctx, gm, om, t_mont, keys = env
+    pm = oracle.int_to_limbs(t_mont, gm.limbs)
+    log_n, k, nonce = 14, 700, [77, 1, 0]
+    N = 1 << log_n
+    srv = hmpc_amd.Server(ctx, budget_bytes=4 << 30, slots=1)
+    kid = srv.add_key(gm, log_n, keys[log_n][2], keys[log_n][3], pm)
+    msgs = ctx.upload(oracle.sample_words(KEY, 80, k * N))
+    say("serving: 700 ciphertexts in two tile chunks")
+    r = srv.submit_encrypt(kid, k, msgs.ptr, nonce)
+    srv.wait(r)
+    c0, c1 = srv.result(r)
+    ctx.set_rng(KEY, nonce)
+    d0, d1 = gm.bgv_encrypt(log_n, k, keys[log_n][2], keys[log_n][3], msgs, pm)
+    assert gm.schedule(log_n)["fast"]
+    assert np.array_equal(c0.to_numpy((gm.limbs, k * N)), gm.download(d0, k * N)), "c0"
+    assert np.array_equal(c1.to_numpy((gm.limbs, k * N)), gm.download(d1, k * N)), "c1"
+    srv.release(r)
+    srv.close()
