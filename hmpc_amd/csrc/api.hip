@@ -280,7 +280,7 @@ struct hmpc_modulus_s
     p26 r26{};                // constants of the radix-2^26 form
     std::map<int, table_pair> tables;
     std::map<int, int> elems; // fast path: elements per thread chosen per log_n (tuner)
-    std::map<int, int> passes; // fast path: 1 = one launch per pass instead of the persistent job kernel
+    std::map<int, int> passes; // fast path: 1 = one launch per pass, 0 = the persistent job kernel (absent: default)
     std::map<int, int> bgv_elems; // fast BGV: elements per thread per log_n (tuner)
     std::map<int, int> bgv_passes; // fast BGV, N > 4096: 1 = two launches over tiles, 0 = persistent jobs (tuner)
     int occupancy[17] = {};   // persistent job kernel: workgroups per CU by elements per thread
@@ -316,6 +316,28 @@ struct hmpc_modulus_s
         auto e = bgv_elems.find(log_n);
         return p != bgv_passes.end() && p->second != 0 && e != bgv_elems.end() ? e->second : 8;
     }
+    // the same for products: the three tile launches unless the tuner chose the persistent kernel
+    bool table_polymul(int log_n) const
+    {
+        if (log_n <= 12)
+            return false;
+        auto p = passes.find(log_n);
+        return p == passes.end() || p->second != 0;
+    }
+    int polymul_elems_for(int log_n) const
+    {
+        auto e = elems.find(log_n);
+        return e != elems.end() ? e->second : default_elems(log_n);
+    }
+    // tile launches over a job table for this job, and with which elements per thread
+    bool table_job(job_desc const& J, int* e) const
+    {
+        int log_n = static_cast<int>(J.log_n);
+        bool bgv = J.type == job_bgv;
+        *e = bgv ? table_elems_for(log_n) : polymul_elems_for(log_n);
+        return bgv ? table_tiles(log_n) : table_polymul(log_n);
+    }
+    // uniform products: the persistent job kernel unless the passes are chosen
     bool persistent(int log_n) const
     {
         auto p = passes.find(log_n);
@@ -839,16 +861,17 @@ int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread,
               (mod->limbs == 2 && (elems_per_thread == 8 || elems_per_thread == 16));
     if (!ok)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "elements per thread: 4 or 8 (4 limbs), 8 or 16 (2 limbs), 0 = default");
-    if (persistent < -1 || persistent > 1)
-        return fail(HMPC_ERR_INVALID_ARGUMENT, "persistent: 1 (job kernel), 0 (one launch per pass), -1 (unchanged)");
+    if (persistent < -2 || persistent > 1)
+        return fail(HMPC_ERR_INVALID_ARGUMENT,
+                    "persistent: 1 (job kernel), 0 (one launch per pass), -1 (unchanged), -2 (default)");
     if (elems_per_thread == 0)
         mod->elems.erase(log_n);
     else
         mod->elems[log_n] = elems_per_thread;
-    if (persistent == 1)
+    if (persistent == -2)
         mod->passes.erase(log_n);
-    else if (persistent == 0)
-        mod->passes[log_n] = 1;
+    else if (persistent >= 0)
+        mod->passes[log_n] = persistent ? 0 : 1;
     return HMPC_OK;
 }
 
@@ -1211,18 +1234,21 @@ namespace
 
 namespace
 {
-    // Tile launches over tables of BGV jobs of two-pass degrees (one ciphertext each), grouped by
-    // (modulus, elements per thread) and cut into chunks whose column workspaces stay within
-    // tile_ws_cap_words; the chunks of one stream run in order and share one workspace.
+    // Tile launches over tables of jobs of two-pass degrees (BGV: one ciphertext each, two
+    // launches; products: one pair each, three launches), grouped by (modulus, elements per
+    // thread, kind) and cut into chunks whose workspaces stay within tile_ws_cap_words; the chunks
+    // of one stream run in order and share one workspace.
     constexpr u64 tile_ws_cap_words = u64{1} << 27;  // 512 MiB
 
     struct tile_chunk
     {
         hmpc_modulus mod = nullptr;
         int elems = 0;
+        u32 type = 0;
         int log_n = 0;  // a degree of the chunk (the arithmetic parameters do not depend on it)
         std::size_t job_off = 0, njobs = 0;
-        std::size_t cols_off = 0, cols_blocks = 0, rows_off = 0, rows_blocks = 0;  // into the map
+        // into the map: forward column pass, row pass, inverse column pass (products)
+        std::size_t cols_off = 0, cols_blocks = 0, rows_off = 0, rows_blocks = 0, icols_off = 0, icols_blocks = 0;
         u64 ws_words = 0;
     };
 
@@ -1242,15 +1268,18 @@ namespace
             map.clear();
             chunks.clear();
             ws_words = 0;
-            std::vector<std::pair<hmpc_modulus, int>> groups;
+            std::vector<std::tuple<hmpc_modulus, int, u32>> groups;
             for (auto const& t : in)
             {
-                auto g = std::make_pair(std::get<0>(t), std::get<1>(t));
+                auto g = std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type);
                 if (std::find(groups.begin(), groups.end(), g) == groups.end())
                     groups.push_back(g);
             }
             for (auto const& g : groups)
             {
+                hmpc_modulus mod = std::get<0>(g);
+                u32 type = std::get<2>(g);
+                u64 streams = type == job_bgv ? 3 : 2;
                 tile_chunk c;
                 auto close = [&]() {
                     if (c.njobs)
@@ -1262,16 +1291,17 @@ namespace
                 };
                 for (auto const& t : in)
                 {
-                    if (std::get<0>(t) != g.first || std::get<1>(t) != g.second)
+                    if (std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type) != g)
                         continue;
                     job_desc J = std::get<2>(t);
-                    u64 words = 3 * static_cast<u64>(g.first->limbs) << J.log_n;
+                    u64 words = streams * static_cast<u64>(mod->limbs) << J.log_n;
                     if (c.njobs && (c.ws_words + words > tile_ws_cap_words || c.njobs == (1u << 22)))
                         close();
                     if (!c.njobs)
                     {
-                        c.mod = g.first;
-                        c.elems = g.second;
+                        c.mod = mod;
+                        c.elems = std::get<1>(g);
+                        c.type = type;
                         c.log_n = static_cast<int>(J.log_n);
                         c.job_off = jobs.size();
                     }
@@ -1282,18 +1312,20 @@ namespace
                 }
                 close();
             }
+            auto section = [&](tile_chunk const& c, bool rows, u32 per_poly, std::size_t& off, std::size_t& blocks) {
+                off = map.size();
+                for (std::size_t j = 0; j < c.njobs; ++j)
+                    for (u32 l = 0, e = per_poly * two_pass_tiles(static_cast<int>(jobs[c.job_off + j].log_n), rows); l < e; ++l)
+                        map.push_back(static_cast<u32>(j) << 10 | l);
+                blocks = map.size() - off;
+            };
             for (auto& c : chunks)
             {
-                c.cols_off = map.size();
-                for (std::size_t j = 0; j < c.njobs; ++j)
-                    for (u32 l = 0, e = bgv_tiles_per_job(static_cast<int>(jobs[c.job_off + j].log_n), false); l < e; ++l)
-                        map.push_back(static_cast<u32>(j) << 10 | l);
-                c.cols_blocks = map.size() - c.cols_off;
-                c.rows_off = map.size();
-                for (std::size_t j = 0; j < c.njobs; ++j)
-                    for (u32 l = 0, e = bgv_tiles_per_job(static_cast<int>(jobs[c.job_off + j].log_n), true); l < e; ++l)
-                        map.push_back(static_cast<u32>(j) << 10 | l);
-                c.rows_blocks = map.size() - c.rows_off;
+                bool bgv = c.type == job_bgv;
+                section(c, false, bgv ? 3 : 2, c.cols_off, c.cols_blocks);
+                section(c, true, 1, c.rows_off, c.rows_blocks);
+                if (!bgv)
+                    section(c, false, 1, c.icols_off, c.icols_blocks);
             }
         }
 
@@ -1307,9 +1339,14 @@ namespace
                 if (rc)
                     return rc;
                 T.elems = c.elems;
-                HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, d_jobs + c.job_off, d_map + c.cols_off,
-                                               static_cast<u32>(c.cols_blocks), d_map + c.rows_off,
-                                               static_cast<u32>(c.rows_blocks), ws, s));
+                auto n = [](std::size_t v) { return static_cast<u32>(v); };
+                if (c.type == job_bgv)
+                    HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, d_jobs + c.job_off, d_map + c.cols_off, n(c.cols_blocks),
+                                                   d_map + c.rows_off, n(c.rows_blocks), ws, s));
+                else
+                    HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, d_jobs + c.job_off, d_map + c.cols_off, n(c.cols_blocks),
+                                                       d_map + c.rows_off, n(c.rows_blocks), d_map + c.icols_off,
+                                                       n(c.icols_blocks), ws, s));
             }
             return HMPC_OK;
         }
@@ -1573,7 +1610,7 @@ struct hmpc_batch_s
     hmpc_context ctx = nullptr;
     std::vector<entry> entries;
     std::vector<job_class> classes;
-    tile_plan tiles;               // BGV jobs of two-pass degrees
+    tile_plan tiles;               // jobs of two-pass degrees
     std::size_t tile_jobs_off = 0; // byte offsets into mem
     std::size_t tile_map_off = 0;
     bool built = false;
@@ -1610,9 +1647,10 @@ struct hmpc_batch_s
             (void)hipStreamDestroy(capture_stream);
     }
 
-    // BGV jobs of two-pass degrees go to tile launches over a job table; the others are grouped
-    // by (modulus, elements per thread), each class ordered longest job first (LPT) for the
-    // persistent kernel. Sizes the grids and the shared scratch, uploads the tables.
+    // Jobs of two-pass degrees go to tile launches over job tables (unless the tuner chose the
+    // persistent kernel for their degree); the others are grouped by (modulus, elements per
+    // thread), each class ordered longest job first (LPT) for the persistent kernel. Sizes the
+    // grids and the shared scratch, uploads the tables.
     int build()
     {
         device_guard g(ctx->device);
@@ -1622,9 +1660,10 @@ struct hmpc_batch_s
         for (auto const& e : entries)
         {
             int log_n = static_cast<int>(e.d.log_n);
-            if (e.d.type == job_bgv && e.mod->table_tiles(log_n))
+            int te = 0;
+            if (e.mod->table_job(e.d, &te))
             {
-                tiled.emplace_back(e.mod, e.mod->table_elems_for(log_n), e.d);
+                tiled.emplace_back(e.mod, te, e.d);
                 continue;
             }
             auto it = e.mod->elems.find(log_n);
@@ -1900,10 +1939,7 @@ int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* 
         for (int persistent = 1; persistent >= 0; --persistent)
         {
             mod->elems[log_n] = cand[ci];
-            if (persistent)
-                mod->passes.erase(log_n);
-            else
-                mod->passes[log_n] = 1;
+            mod->passes[log_n] = persistent ? 0 : 1;
             rc = hmpc_polymul(mod, log_n, pairs, a, b, out, s);  // warm-up (workspace, code objects)
             for (int r = 0; rc == HMPC_OK && r < 3; ++r)
             {
@@ -1931,10 +1967,7 @@ int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* 
             }
         }
     mod->elems[log_n] = best_e;
-    if (best_p)
-        mod->passes.erase(log_n);
-    else
-        mod->passes[log_n] = 1;
+    mod->passes[log_n] = best_p ? 0 : 1;
     cleanup();
     if (elems_out)
         *elems_out = best_e;
@@ -2198,7 +2231,7 @@ namespace
         u32* counters = nullptr;      // one 256-byte counter per class
         u32* scratch = nullptr;
         std::size_t scratch_words = 0;
-        tile_plan tiles;              // the current launch's encryptions at two-pass degrees
+        tile_plan tiles;              // the current launch's jobs at two-pass degrees
         u32* d_map = nullptr;
         u32* h_map = nullptr;         // pinned staging of the tile map
         std::size_t map_cap = 0;
@@ -2343,8 +2376,8 @@ struct hmpc_server_s
         return true;
     }
 
-    // Encryptions at two-pass degrees run as tile launches over a table of all their
-    // ciphertexts; products and N = 4096 encryptions as persistent launches per class.
+    // Requests at two-pass degrees run as tile launches over a table of all their ciphertexts
+    // or pairs; N <= 4096 (and degrees tuned to it) as persistent launches per class.
     int launch(server_slot& s, std::vector<u64> const& take, std::size_t jobs)
     {
         struct cls_info
@@ -2366,8 +2399,8 @@ struct hmpc_server_s
                 error = last_error;
                 return rc;
             }
-            bool tile = r.kind == 0 && r.mod->table_tiles(r.log_n);
-            int elems = r.kind == 0 ? r.mod->bgv_elems_for(r.log_n) : (r.mod->elems.count(r.log_n) ? r.mod->elems[r.log_n] : r.mod->default_elems(r.log_n));
+            bool tile = r.kind == 0 ? r.mod->table_tiles(r.log_n) : r.mod->table_polymul(r.log_n);
+            int elems = r.kind == 0 ? r.mod->bgv_elems_for(r.log_n) : r.mod->polymul_elems_for(r.log_n);
             auto it = classes.end();
             if (!tile)
             {
@@ -2415,7 +2448,10 @@ struct hmpc_server_s
                 for (std::size_t i = 0; i < r.n; ++i)
                 {
                     job_desc J = polymul_job(T, r.log_n, r.a, r.b, out, count, i * N);
-                    it->jobs.push_back({3 * N * r.log_n, J});
+                    if (tile)
+                        tiled.emplace_back(r.mod, r.mod->polymul_elems_for(r.log_n), J);
+                    else
+                        it->jobs.push_back({3 * N * r.log_n, J});
                 }
             }
         }

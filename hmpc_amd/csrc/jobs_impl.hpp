@@ -568,6 +568,124 @@ namespace hmpc_amd
             });
         }
 
+        // The two-pass product over a table of jobs of any two-pass degrees (one polynomial pair
+        // each), in three launches: forward columns of a and b (2 x column tiles per job), rows
+        // (forward, product, inverse), inverse columns with the final scaling. Job j keeps
+        // NTT(a), NTT(b) and the row results in its own workspace region [2][iol][N] at
+        // ws + ws_off. Blocks find their job and tile through map as above.
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E) k_polymul_cols_table(job_desc const* __restrict__ jobs,
+                                                                         u32 const* __restrict__ map, u32* __restrict__ ws,
+                                                                         typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            u32 local = m & 1023u;
+            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
+            u32 which = local >= gc.blocks_per_poly ? 1u : 0u;
+            u32 ct = local - which * gc.blocks_per_poly;
+            std::size_t N = std::size_t{1} << J.log_n;
+            tile_desc td;
+            io_desc ii, is;
+            job_cols_tile(gc, ct, J.in_count, J.in_off, td, ii);
+            job_cols_tile(gc, ct, N, 0, td, is);
+            tile_load<A, T>(lds, which ? J.in1 : J.in0, ii);
+            __syncthreads();
+            u32 x[E][A::rl];
+            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+            tile_store<A, T>(lds, ws + J.ws_off + which * A::iol * N, is, false, P);
+        }
+
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E) k_polymul_rows_table(job_desc const* __restrict__ jobs,
+                                                                         u32 const* __restrict__ map, u32* __restrict__ ws,
+                                                                         typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            constexpr int WM = shape<E>::wmax;
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            u32 rt = m & 1023u;
+            pass_geom gr = geom_for(static_cast<int>(J.log_n), true);
+            std::size_t N = std::size_t{1} << J.log_n;
+            u32* sa = ws + J.ws_off;
+            u32* sb = sa + A::iol * N;
+            tile_desc td;
+            io_desc is;
+            u32 hi0 = job_rows_tile(gr, rt, N, 0, td, is);
+            u32 base0 = (1u << gr.log_n1) + hi0;
+            u32 x[E][A::rl];
+            tile_load<A, T>(lds, sa, is);
+            __syncthreads();
+            tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+            tile_store<A, T>(lds, sa, is, false, P);
+            __syncthreads();
+            tile_load<A, T>(lds, sb, is);
+            __syncthreads();
+            tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+            with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
+            dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
+            __syncthreads();
+            tile_rounds<A, E, true>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
+            tile_store<A, T>(lds, sa, is, false, P);
+        }
+
+        template<typename A, int E>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_icols_table(job_desc const* __restrict__ jobs,
+                                                                                          u32 const* __restrict__ map,
+                                                                                          u32 const* __restrict__ ws,
+                                                                                          typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            u32 ct = m & 1023u;
+            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
+            std::size_t N = std::size_t{1} << J.log_n;
+            tile_desc td;
+            io_desc is, io;
+            job_cols_tile(gc, ct, N, 0, td, is);
+            job_cols_tile(gc, ct, J.out_count, J.out_off, td, io);
+            tile_load<A, T>(lds, ws + J.ws_off, is);
+            __syncthreads();
+            u32 x[E][A::rl];
+            tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false);
+            tile_store<A, T>(lds, J.out0, io, true, P);
+        }
+
+        template<typename A>
+        hipError_t launch_polymul_table(int elems, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
+                                        u32 const* map_rows, u32 rows_blocks, u32 const* map_icols, u32 icols_blocks,
+                                        u32* ws, typename A::params const& P, hipStream_t s)
+        {
+            return with_elems<A>(elems, [&](auto EC) -> hipError_t {
+                constexpr int E = decltype(EC)::value;
+                constexpr std::size_t bytes = tile_bytes<A>();
+                hipError_t e = hipSuccess;
+                if ((e = allow_lds(k_polymul_cols_table<A, E>, bytes)) != hipSuccess ||
+                    (e = allow_lds(k_polymul_rows_table<A, E>, bytes)) != hipSuccess ||
+                    (e = allow_lds(k_polymul_icols_table<A, E>, bytes)) != hipSuccess)
+                    return e;
+                {
+                    probe_scope ps(s, "fast polymul forward columns");
+                    k_polymul_cols_table<A, E><<<cols_blocks, tile / E, bytes, s>>>(jobs, map_cols, ws, P);
+                }
+                {
+                    probe_scope ps(s, "fast polymul rows");
+                    k_polymul_rows_table<A, E><<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
+                }
+                {
+                    probe_scope ps(s, "fast polymul inverse columns");
+                    k_polymul_icols_table<A, E><<<icols_blocks, tile / E, bytes, s>>>(jobs, map_icols, ws, P);
+                }
+                return hipGetLastError();
+            });
+        }
+
         template<typename A>
         hipError_t jobs_occupancy(int elems, int* blocks_per_cu)
         {

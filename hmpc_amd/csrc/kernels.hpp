@@ -91,8 +91,14 @@ namespace hmpc_amd
     // ws + jobs[j].ws_off ([3][L][N])
     hipError_t launch_fast_bgv_table(int L, transform_args const& T, job_desc const* jobs, u32 const* map_cols,
                                      u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s);
-    // blocks per job of the tile launches (column launch: 3 streams x column tiles; row launch: row tiles)
-    unsigned bgv_tiles_per_job(int log_n, bool rows);
+    // the two-pass product over a table of jobs of two-pass degrees (one pair each): forward
+    // columns (2 x column tiles per job), rows, inverse columns; job j's workspace region is
+    // ws + jobs[j].ws_off ([2][L][N]); maps as above
+    hipError_t launch_fast_polymul_table(int L, transform_args const& T, job_desc const* jobs, u32 const* map_cols,
+                                         u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
+                                         u32 icols_blocks, u32* ws, hipStream_t s);
+    // tiles per polynomial of the column (rows = false) or row pass at N = 2^log_n > 4096
+    unsigned two_pass_tiles(int log_n, bool rows);
 
     hipError_t launch_make_twiddles(int L, u32* table, int log_n, u32 const* root, u32 const* norm, u32 const* one,
                                     bool inverse, mod_params const& P, hipStream_t s);

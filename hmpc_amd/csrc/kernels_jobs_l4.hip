@@ -26,9 +26,16 @@ namespace hmpc_amd
                                                     T.r26, s);
     }
 
-    unsigned bgv_tiles_per_job(int log_n, bool rows)
+    unsigned two_pass_tiles(int log_n, bool rows)
     {
-        fast::pass_geom g = fast::geom_for(log_n, rows);
-        return rows ? g.blocks_per_poly : 3 * g.blocks_per_poly;
+        return fast::geom_for(log_n, rows).blocks_per_poly;
+    }
+
+    hipError_t launch_fast_polymul_table_l4(transform_args const& T, job_desc const* jobs, u32 const* map_cols,
+                                             u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
+                                             u32 icols_blocks, u32* ws, hipStream_t s)
+    {
+        return fast::launch_polymul_table<fast::arith_r26>(T.elems, jobs, map_cols, cols_blocks, map_rows, rows_blocks, map_icols,
+                                                        icols_blocks, ws, T.r26, s);
     }
 }

@@ -356,7 +356,7 @@ class Modulus:
 
     def set_schedule(self, log_n: int, elems_per_thread: int, persistent: int = -1):
         """elements per thread (0 = default); persistent 1 = job kernel, 0 = one launch per pass,
-        -1 = unchanged"""
+        -1 = unchanged, -2 = default"""
         check(_lib.hmpc_modulus_set_schedule(self.handle, log_n, elems_per_thread, persistent))
 
     def set_bgv_schedule(self, log_n: int, elems_per_thread: int = 0, tiles: int = -1):

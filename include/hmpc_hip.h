@@ -97,7 +97,9 @@ int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
 /* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 limbs, 8 or
  * 16 for 2 limbs; 0 restores the default) and launch structure (persistent = 1: one persistent
  * job kernel in which each workgroup runs whole products / encryptions; 0: one launch per
- * transform pass; -1: unchanged). Chosen by hmpc_autotune; results are identical. */
+ * transform pass; -1: unchanged; -2: the default, which is the persistent kernel for uniform
+ * products and the tile launches in job batches and serving). Chosen by hmpc_autotune; results
+ * are identical. */
 int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int persistent);
 /* Schedule of fast-path BGV encryption for N = 2^log_n: elements per thread (0 = default) and, for
  * N > 4096, tiles = 1 (a column launch and a row launch over all ciphertexts' tiles), 0 (the
@@ -202,9 +204,10 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
 /* ------------------------------------------------------------------ job batches
  * Many independent jobs of uneven degree (2^12 .. 2^16 and beyond) and modulus in persistent
  * launches: one launch per (modulus, schedule) class, in which each workgroup claims jobs
- * longest first and runs each from inputs to outputs. BGV jobs at N > 4096 instead run as two
- * launches over a table of all their tiles (column pass, row pass), unless the tuner chose the
- * persistent kernel for that degree (hmpc_autotune_bgv). Each job's polynomials are single
+ * longest first and runs each from inputs to outputs. Jobs at N > 4096 instead run as launches
+ * over a table of all their tiles (BGV: column pass, row pass; products: forward columns, rows,
+ * inverse columns), unless the tuner chose the persistent kernel for that degree
+ * (hmpc_autotune_bgv, hmpc_autotune). Each job's polynomials are single
  * polynomials ([L][N] buffers). hmpc has no batch object: a job list stands for the sequence of
  * queue(...) calls it replaces (hmpc/comp/queue.hpp:416-529), with the same results.
  * Moduli must be served by the fast kernels (hmpc_modulus_get_schedule: fast = 1). */
@@ -224,7 +227,7 @@ int hmpc_batch_add_bgv_encrypt(hmpc_batch batch, hmpc_modulus mod, int log_n, co
  * inputs, nonces and outputs). Asynchronous on `stream`. */
 int hmpc_batch_run(hmpc_batch batch, int use_graph, void* stream);
 /* jobs, launch groups after the first run (persistent-kernel classes plus tile-launch chunks of
- * BGV jobs at N > 4096), device bytes held (tables, counters, scratch) */
+ * jobs at N > 4096), device bytes held (tables, counters, scratch) */
 int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* device_bytes);
 
 /* ------------------------------------------------------------------ serving

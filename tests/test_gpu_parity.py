@@ -292,7 +292,7 @@ def test_fast_kernel_schedules(ctx, q, log_n, n):
             assert np.array_equal(gm.download(gm.ntt(log_n, n, da, inverse=True), count), want_i), f"inverse elems={elems}"
             got = gm.download(gm.polymul(log_n, n, da, db), count)
             assert np.array_equal(got, want_p), f"polymul elems={elems} persistent={persistent}"
-    gm.set_schedule(log_n, 0, 1)
+    gm.set_schedule(log_n, 0, -2)
 
 
 def test_autotune_bgv_keeps_results_and_rng(ctx):
@@ -326,7 +326,7 @@ def test_autotune_installs_a_correct_schedule(ctx):
     b = om.sample_uniform(KEY, 1, count, threads=8)
     want, _ = om.polymul_timed(log_n, a, b, threads=8)
     assert np.array_equal(gm.download(gm.polymul(log_n, n, gm.upload(a), gm.upload(b)), count), want)
-    gm.set_schedule(log_n, 0, 1)
+    gm.set_schedule(log_n, 0, -2)
 
 
 # uneven job list: products and encryptions, N from 2^12 to 2^16, both fast moduli
@@ -335,21 +335,31 @@ BATCH_JOBS = [("mul", Q64, 12), ("bgv", Q128A, 13), ("mul", Q128A, 16), ("bgv", 
               ("bgv", Q128A, 12), ("mul", Q128A, 13), ("bgv", Q128A, 14), ("bgv", Q64, 16), ("bgv", Q128A, 13)]
 
 
-@pytest.mark.parametrize("bgv_structure", ["tiles", "jobs"])
-def test_batch_of_uneven_jobs_matches_oracle(ctx, bgv_structure):
-    """Persistent job kernels (one per modulus class) and, for encryptions at N > 4096, tile
-    launches over a job table (or, when so scheduled, the persistent kernel) over an uneven job
-    list, eager and replayed from a hipGraph three times with host work between replays; every
-    job's output equals the oracle's, and each encryption consumes the nonces a queue call would."""
-    two_pass = sorted({(q, log_n) for kind, q, log_n in BATCH_JOBS if kind == "bgv" and log_n > 12})
-    if bgv_structure == "jobs":
-        for q, log_n in two_pass:
-            mods(ctx, q)[0].set_bgv_schedule(log_n, 0, 0)
+@pytest.mark.parametrize("structure", ["tiles", "jobs"])
+def test_batch_of_uneven_jobs_matches_oracle(ctx, structure):
+    """Persistent job kernels (one per modulus class) and, for jobs at N > 4096, tile launches
+    over job tables (or, when so scheduled, the persistent kernel) over an uneven job list, eager
+    and replayed from a hipGraph three times with host work between replays; every job's output
+    equals the oracle's, and each encryption consumes the nonces a queue call would."""
+    two_pass = sorted({(kind, q, log_n) for kind, q, log_n in BATCH_JOBS if log_n > 12})
+    if structure == "jobs":
+        for kind, q, log_n in two_pass:
+            gm = mods(ctx, q)[0]
+            if kind == "bgv":
+                gm.set_bgv_schedule(log_n, 0, 0)
+            else:
+                gm.set_schedule(log_n, 0, 1)
     try:
-        _run_uneven_batch(ctx)
+        info = _run_uneven_batch(ctx)
+        tiled = len({(kind, q) for kind, q, _ in two_pass})
+        assert info["classes"] >= (tiled if structure == "tiles" else 2)
     finally:
-        for q, log_n in two_pass:
-            mods(ctx, q)[0].set_bgv_schedule(log_n, 0, -1)
+        for kind, q, log_n in two_pass:
+            gm = mods(ctx, q)[0]
+            if kind == "bgv":
+                gm.set_bgv_schedule(log_n, 0, -1)
+            else:
+                gm.set_schedule(log_n, 0, -2)
 
 
 def _run_uneven_batch(ctx):
@@ -391,3 +401,4 @@ def _run_uneven_batch(ctx):
                 assert np.array_equal(gm.download(buf, N), want), f"job {k} {BATCH_JOBS[k]} run {run}"
     info = batch.info()
     assert info["jobs"] == len(BATCH_JOBS) and info["classes"] >= 2
+    return info
