@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 KEY = [42, 0, 0, 0, 0, 0, 0, 0]
 ENC_KEY = [42, 0, 0, 0, 0, 0, 0, 1]
 HBM_PEAK_GBS = 8000.0
+VALU_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def log(msg):
@@ -377,6 +378,25 @@ def traffic_for(kernel, workload):
     return None if e is None else e.get("bytes_per_launch")
 
 
+def valu_issue_for(kernel, workload, kernel_ms, cus):
+    """VALU issue bound of `kernel` from the committed counter summary (profiles/valu.json:
+    SQ_INSTS_VALU wave instructions per dispatch): the time those instructions need at one wave
+    instruction per SIMD every 4 cycles at 2.4 GHz, and its fraction of the measured duration."""
+    path = os.path.join(ROOT, "profiles", "valu.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(workload, {}).get(kernel)
+    if e is None or "SQ_INSTS_VALU" not in e:
+        return None
+    rate = cus * 4 * VALU_CLOCK_GHZ * 1e9 / 4
+    issue_ms = e["SQ_INSTS_VALU"] / rate * 1e3
+    return {"wave_instructions": e["SQ_INSTS_VALU"], "issue_rate_per_s": rate, "issue_ms": issue_ms,
+            "frac": issue_ms / kernel_ms}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,7 +452,8 @@ def main():
                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kname, "c2"),
                          "kernel": kname, "kernel_avg_ms": kavg, "launches": klaunch,
                          "algorithmic_bytes_per_launch": kbytes,
-                         "step_hbm_gbs": 3 * unit_bytes / (pm["ms"] / 1e3) / 1e9},
+                         "step_hbm_gbs": 3 * unit_bytes / (pm["ms"] / 1e3) / 1e9,
+                         "valu_issue": valu_issue_for(kname, "c2", kavg, info["compute_units"])},
             "kernels_ms": {k: v[1] for k, v in pm["kernels"].items()},
             "int_rate": {"modmul_per_s": modmuls / (pm["ms"] / 1e3), "modmuls_per_step": modmuls},
         })

@@ -789,18 +789,8 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
     {
         // q = 2^124 + c: radix-2^26 constants (see r26.hpp for the bounds they satisfy)
         m->form = form_r26;
-        u32 c = m->P.q[0];
         p26& R = m->r26;
-        R.c = c;
-        R.n26 = m->P.nprime & m26;
-        u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
-        u32 const k32[5] = {32 * c + (1u << 29), (1u << 29) - 8, (1u << 29) - 8, (1u << 29) - 8, (1u << 25) - 8};
-        u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
-        std::copy(k4, k4 + 5, R.k4);
-        std::copy(k32, k32 + 5, R.k32);
-        std::copy(qb, qb + 5, R.qb);
-        std::copy(m->P.q, m->P.q + 4, R.q32);
-        R.t20 = 1u << 20;
+        r26_constants(R, m->P.q[0]);
         // R 2^130 mod q = 4 R^2 mod q, as normalized radix-2^26 limbs
         u32 rr[4] = {m->r2[0], m->r2[1], m->r2[2], m->r2[3]};
         double_mod(rr, m->P.q, 4);
@@ -1294,7 +1284,7 @@ namespace
                     if (std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type) != g)
                         continue;
                     job_desc J = std::get<2>(t);
-                    u64 words = streams * static_cast<u64>(mod->limbs) << J.log_n;
+                    u64 words = streams * static_cast<u64>(ws_limbs(mod->limbs)) << J.log_n;
                     if (c.njobs && (c.ws_words + words > tile_ws_cap_words || c.njobs == (1u << 22)))
                         close();
                     if (!c.njobs)
@@ -1444,7 +1434,7 @@ int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, co
         if (mod->bgv_tiles(log_n, n))
         {
             void* w = nullptr;
-            rc = ctx->workspace(s, 3 * static_cast<std::size_t>(mod->limbs) * (n << log_n) * 4, &w);
+            rc = ctx->workspace(s, 3 * static_cast<std::size_t>(ws_limbs(mod->limbs)) * (n << log_n) * 4, &w);
             if (rc)
                 return rc;
             HMPC_HIP(launch_fast_bgv_passes(mod->limbs, T, J, n, static_cast<u32*>(w), s));

@@ -59,6 +59,7 @@ namespace hmpc_amd
         {
             static constexpr int rl = 5;    // limbs per element in registers and LDS
             static constexpr int iol = 4;   // limbs per element in global memory (hmpc layout)
+            static constexpr int wsl = 5;   // limbs per element in private workspaces (raw limbs)
             static constexpr int tws = 8;   // twiddle table stride in words (5 used)
             static constexpr bool stage_prefetch = true;
             using params = p26;
@@ -86,6 +87,29 @@ namespace hmpc_amd
 #pragma unroll
                 for (int l = 0; l < 4; ++l)
                     dst[l * count + g] = w[l];
+            }
+
+            // private workspaces between passes hold the raw redundant limbs (no reduction, no
+            // repacking): 25% more bytes, about 50 fewer VALU operations per element and pass. A
+            // forward column result is folded first only when the whole transform is longer
+            // than the lazy forward bound allows (fold = true).
+            __device__ __forceinline__ static void load_ws(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
+                                                           std::size_t g)
+            {
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    x[l] = src[l * count + g];
+            }
+
+            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, std::size_t g,
+                                                            u32 const (&xin)[rl], bool fold, params const& P)
+            {
+                u32 x[rl] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
+                if (fold)
+                    r26_fold(x, P);
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    dst[l * count + g] = x[l];
             }
 
             __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 4) & 15); }
@@ -116,7 +140,7 @@ namespace hmpc_amd
                 w[4] = tw[8 * static_cast<std::size_t>(idx) + 4];
             }
 
-            // (x, y) -> (x + t, x - t + 4q), t = mont(y, w)
+            // (x, y) -> (x + t, x - t + 3q), t = mont(y, w)
             __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
             {
                 u32 t[rl];
@@ -126,7 +150,7 @@ namespace hmpc_amd
                 {
                     u32 xl = x[l];
                     x[l] = xl + t[l];
-                    y[l] = xl + (P.k4[l] - t[l]);
+                    y[l] = xl + (P.k3[l] - t[l]);
                 }
             }
 
@@ -211,6 +235,7 @@ namespace hmpc_amd
         {
             static constexpr int rl = 2;
             static constexpr int iol = 2;
+            static constexpr int wsl = 2;
             static constexpr int tws = 2;
             static constexpr bool stage_prefetch = false;
             using params = fast_params;
@@ -228,6 +253,19 @@ namespace hmpc_amd
             {
                 dst[g] = x[0];
                 dst[count + g] = x[1];
+            }
+
+            // values are canonical throughout: workspaces use the global format
+            __device__ __forceinline__ static void load_ws(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
+                                                           std::size_t g)
+            {
+                load_global(x, src, count, g);
+            }
+
+            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, std::size_t g,
+                                                            u32 const (&x)[rl], bool, params const& P)
+            {
+                store_global(dst, count, g, x, false, P);
             }
 
             __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 4) & 15); }
@@ -657,7 +695,9 @@ namespace hmpc_amd
             return io.base + static_cast<std::size_t>(e >> io.log_seqs) * io.stride + (e & ((1 << io.log_seqs) - 1));
         }
 
-        template<typename A, int T>
+        // Ws: src is a private workspace (A::wsl limb planes, raw format) rather than data in the
+        // hmpc layout
+        template<typename A, int T, bool Ws = false>
         __device__ __forceinline__ void tile_load(u32* lds, u32 const* __restrict__ src, io_desc const& io)
         {
 #pragma unroll 4
@@ -665,7 +705,12 @@ namespace hmpc_amd
             {
                 u32 x[A::rl];
                 if (e < io.valid)
-                    A::load_global(x, src, io.count, gidx(io, e));
+                {
+                    if constexpr (Ws)
+                        A::load_ws(x, src, io.count, gidx(io, e));
+                    else
+                        A::load_global(x, src, io.count, gidx(io, e));
+                }
                 else
                 {
 #pragma unroll
@@ -676,7 +721,8 @@ namespace hmpc_amd
             }
         }
 
-        template<typename A, int T>
+        // Ws: dst is a private workspace (raw format; canonical is then the fold flag of store_ws)
+        template<typename A, int T, bool Ws = false>
         __device__ __forceinline__ void tile_store(u32 const* lds, u32* __restrict__ dst, io_desc const& io, bool canonical,
                                                    typename A::params const& P)
         {
@@ -685,9 +731,16 @@ namespace hmpc_amd
             {
                 u32 x[A::rl];
                 A::lds_get(x, lds, e);
-                A::store_global(dst, io.count, gidx(io, e), x, canonical, P);
+                if constexpr (Ws)
+                    A::store_ws(dst, io.count, gidx(io, e), x, canonical, P);
+                else
+                    A::store_global(dst, io.count, gidx(io, e), x, canonical, P);
             }
         }
+
+        // The lazy forward transform runs up to 19 stages without reductions (r26.hpp); a
+        // forward column result of a longer transform is folded before it goes to the workspace.
+        __host__ __device__ __forceinline__ bool ws_fold_columns(int log_n) { return log_n > 18; }
 
         struct pass_geom
         {
@@ -777,9 +830,9 @@ namespace hmpc_amd
         // ------------------------------------------------------------------ second operands
         // After the forward transform's last round (stages [s - W, s)) of a row-layout tile, the
         // thread's unit g holds the K = 2^W consecutive tile elements ((tid + T g) << W) + k, so a
-        // second operand of the same elements is read straight from global memory (K
-        // consecutive words per limb plane; the workgroup wrote it itself just before, so it is
-        // served from L2) instead of occupying a second register tile.
+        // second operand of the same elements is read straight from its private workspace (K
+        // consecutive words per limb plane, raw format; the workgroup wrote it itself just
+        // before, so it is served from L2) instead of occupying a second register tile.
         //   Mode 0: x = a o x            (a = NTT(a) of a product, lazy)
         //   Mode 1: x = k o u + x        (u = NTT(u), k = public-key polynomial, canonical)
         // Only the tile's first `valid` elements are backed by memory (a whole-tile job of degree
@@ -800,7 +853,7 @@ namespace hmpc_amd
                     if (e0 + k >= valid)
                         continue;
                     u32 v[A::rl];
-                    A::load_global(v, src, src_count, src_base + e0 + k);
+                    A::load_ws(v, src, src_count, src_base + e0 + k);
                     if constexpr (Mode == 0)
                     {
                         A::point(v, x[g * K + k], P);
@@ -845,7 +898,8 @@ namespace hmpc_amd
         // ------------------------------------------------------------------ kernels
         // mode: 0 forward column pass, 1 forward row pass (final), 2 inverse row pass,
         //       3 inverse column pass (final), 4 forward whole-poly, 5 inverse whole-poly
-        template<typename A, int E, int Mode>
+        // WsIn: `in` is a private workspace (the row results of the two-pass product)
+        template<typename A, int E, int Mode, bool WsIn = false>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
                                                            u32 const* __restrict__ tw, pass_geom G, u64 n,
                                                            typename A::params P)
@@ -877,7 +931,7 @@ namespace hmpc_amd
                 final_scale = Mode == 5;
                 canonical = true;
             }
-            tile_load<A, T>(lds, in, io);
+            tile_load<A, T, WsIn>(lds, in, io);
             __syncthreads();
             u32 x[E][A::rl];
             tile_rounds<A, E, Inv>(x, lds, td, base0, base_step, tw, P, final_scale, 0, false);
@@ -936,19 +990,19 @@ namespace hmpc_amd
                 // two-pass row tile: NTT(a) rows go back to the workspace (a == out, in place) and
                 // are read per element next to NTT(b) in registers, so one register tile is live
                 u32 x[E][A::rl];
-                tile_load<A, T>(lds, a, io);
+                tile_load<A, T, true>(lds, a, io);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, base_step, twf, P, false, 0, false);
-                tile_store<A, T>(lds, out, io, false, P);
+                tile_store<A, T, true>(lds, out, io, false, P);
                 __syncthreads();
-                tile_load<A, T>(lds, b, io);
+                tile_load<A, T, true>(lds, b, io);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, base_step, twf, P, false, 0, true);
                 with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
                 dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
                 __syncthreads();
                 tile_rounds<A, E, true>(x, lds, td, base0, base_step, twi, P, false, 1, false);
-                tile_store<A, T>(lds, out, io, false, P);
+                tile_store<A, T, true>(lds, out, io, false, P);
             }
         }
 
@@ -970,7 +1024,7 @@ namespace hmpc_amd
             __syncthreads();
             u32 x[E][A::rl];
             tile_rounds<A, E, false>(x, lds, td, 1u, 0u, tw, P, false, 0, false);
-            tile_store<A, T>(lds, which ? wb : wa, io, false, P);
+            tile_store<A, T, true>(lds, which ? wb : wa, io, ws_fold_columns(G.log_n), P);
         }
 
         // ------------------------------------------------------------------ host launchers
@@ -1102,13 +1156,14 @@ namespace hmpc_amd
                 }
                 std::size_t count = n << T.log_n;
                 u32* wa = workspace;
-                u32* wb = workspace + A::iol * count;
+                u32* wb = workspace + A::wsl * count;
                 pass_geom gc = geom_for(T.log_n, false);
                 pass_geom gr = geom_for(T.log_n, true);
                 unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
                 unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
                 if ((e = allow_lds(k_polymul_cols<A, E>, bytes)) != hipSuccess ||
-                    (e = allow_lds(k_polymul<A, E, false>, bytes)) != hipSuccess || (e = allow_lds(k_pass<A, E, 3>, bytes)) != hipSuccess)
+                    (e = allow_lds(k_polymul<A, E, false>, bytes)) != hipSuccess ||
+                    (e = allow_lds(k_pass<A, E, 3, true>, bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast polymul forward columns");
@@ -1120,7 +1175,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");
-                    k_pass<A, E, 3><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
+                    k_pass<A, E, 3, true><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
                 }
                 return hipGetLastError();
             });

@@ -42,12 +42,16 @@ namespace hmpc_amd
         u64 ws_off;        // tile launches over a job table: the job's workspace region (words)
     };
 
+    // Words per element of the fast kernels' private workspaces: the raw radix-2^26 limbs (5)
+    // for 4-limb moduli, the data format otherwise.
+    inline int ws_limbs(int io_limbs) { return io_limbs == 4 ? 5 : io_limbs; }
+
     // Scratch words one workgroup needs for a job of degree 2^log_n: the transformed first
     // operand (NTT(a), NTT(u)) of whole-tile degrees, and the column-pass results of every
     // polynomial of the two-pass transform.
     inline u64 job_scratch_words(u32 type, int log_n, int io_limbs)
     {
         u64 polys = log_n <= 12 ? 1 : (type == job_bgv ? 3 : 2);
-        return polys * static_cast<u64>(io_limbs) << log_n;
+        return polys * static_cast<u64>(ws_limbs(io_limbs)) << log_n;
     }
 }

@@ -160,7 +160,7 @@ namespace hmpc_amd
                 tile_load<A, T>(lds, J.in0, ia);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
-                tile_store<A, T>(lds, scratch, is, false, P);
+                tile_store<A, T, true>(lds, scratch, is, false, P);
                 __syncthreads();
                 // NTT(b) in registers, times NTT(a), inverse
                 tile_load<A, T>(lds, J.in1, ia);
@@ -175,7 +175,7 @@ namespace hmpc_amd
                 return;
             }
             u32* sa = scratch;
-            u32* sb = scratch + A::iol * N;
+            u32* sb = scratch + A::wsl * N;
             pass_geom const gc = geom_for(log_n, false);
             pass_geom const gr = geom_for(log_n, true);
             // forward column pass of a and b
@@ -190,7 +190,7 @@ namespace hmpc_amd
                 tile_load<A, T>(lds, which ? J.in1 : J.in0, ii);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
-                tile_store<A, T>(lds, which ? sb : sa, is, false, P);
+                tile_store<A, T, true>(lds, which ? sb : sa, is, ws_fold_columns(log_n), P);
                 __syncthreads();
             }
             // row pass: NTT(a) rows back into scratch; NTT(b) rows in registers, product,
@@ -201,19 +201,19 @@ namespace hmpc_amd
                 io_desc is;
                 u32 hi0 = job_rows_tile(gr, t, N, 0, td, is);
                 u32 base0 = (1u << gr.log_n1) + hi0;
-                tile_load<A, T>(lds, sa, is);
+                tile_load<A, T, true>(lds, sa, is);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-                tile_store<A, T>(lds, sa, is, false, P);
+                tile_store<A, T, true>(lds, sa, is, false, P);
                 __syncthreads();
-                tile_load<A, T>(lds, sb, is);
+                tile_load<A, T, true>(lds, sb, is);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
                 with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
                 dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
                 __syncthreads();
                 tile_rounds<A, E, true>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
-                tile_store<A, T>(lds, sa, is, false, P);
+                tile_store<A, T, true>(lds, sa, is, false, P);
                 __syncthreads();
             }
             // inverse column pass with the final scaling, canonical output
@@ -223,7 +223,7 @@ namespace hmpc_amd
                 io_desc is, io;
                 job_cols_tile(gc, t, N, 0, td, is);
                 job_cols_tile(gc, t, J.out_count, J.out_off, td, io);
-                tile_load<A, T>(lds, sa, is);
+                tile_load<A, T, true>(lds, sa, is);
                 __syncthreads();
                 tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false);
                 tile_store<A, T>(lds, J.out0, io, true, P);
@@ -261,7 +261,7 @@ namespace hmpc_amd
                 sample_tile<A, T, Which>(lds, J, true, gc.log_n1, gc.log_seqs, ct << gc.log_seqs, P);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
-                tile_store<A, T>(lds, dst, is, false, P);
+                tile_store<A, T, true>(lds, dst, is, ws_fold_columns(gc.log_n), P);
                 __syncthreads();
             }
         }
@@ -284,7 +284,7 @@ namespace hmpc_amd
                 sample_tile<A, T, 0>(lds, J, false, 0, 0, 0u, P);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
-                tile_store<A, T>(lds, scratch, is, false, P);
+                tile_store<A, T, true>(lds, scratch, is, false, P);
                 __syncthreads();
                 sample_tile<A, T, 1>(lds, J, false, 0, 0, 0u, P);
                 __syncthreads();
@@ -301,8 +301,8 @@ namespace hmpc_amd
             u32* su = scratch;
             // sample and run the column stages of u, p v + m, p w
             bgv_columns<A, E, 0>(J, lds, su, gc, x, P);
-            bgv_columns<A, E, 1>(J, lds, scratch + A::iol * N, gc, x, P);
-            bgv_columns<A, E, 2>(J, lds, scratch + 2 * A::iol * N, gc, x, P);
+            bgv_columns<A, E, 1>(J, lds, scratch + A::wsl * N, gc, x, P);
+            bgv_columns<A, E, 2>(J, lds, scratch + 2 * A::wsl * N, gc, x, P);
             // row stages: U back into scratch; X0, X1 in registers combined with the key
             for (u32 t = 0; t < gr.blocks_per_poly; ++t)
             {
@@ -312,14 +312,14 @@ namespace hmpc_amd
                 job_rows_tile(gr, t, J.in_count, J.in_off, td, ik);
                 job_rows_tile(gr, t, J.out_count, J.out_off, td, io);
                 u32 base0 = (1u << gr.log_n1) + hi0;
-                tile_load<A, T>(lds, su, is);
+                tile_load<A, T, true>(lds, su, is);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-                tile_store<A, T>(lds, su, is, false, P);
+                tile_store<A, T, true>(lds, su, is, false, P);
                 __syncthreads();
                 for (int which = 1; which < 3; ++which)
                 {
-                    tile_load<A, T>(lds, scratch + which * A::iol * N, is);
+                    tile_load<A, T, true>(lds, scratch + which * A::wsl * N, is);
                     __syncthreads();
                     tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
                     bool first = which == 1;
@@ -378,7 +378,7 @@ namespace hmpc_amd
         // ------------------------------------------------------------------ two-pass BGV (N > 4096)
         // The batched encryption of n ciphertexts as two launches over tiles instead of one
         // workgroup per ciphertext: every ciphertext's tiles run in parallel (short latency for
-        // small batches). Workspace: the three column-pass results [3][iol][n N].
+        // small batches). Workspace: the three column-pass results [3][wsl][n N].
         //   column pass: blockIdx.x = (p * 3 + which) * tiles + t  (sample stream `which` of ciphertext p)
         //   row pass:    blockIdx.x = p * tiles + t
         template<typename A, int E, int Which>
@@ -393,7 +393,7 @@ namespace hmpc_amd
             sample_tile<A, T, Which>(lds, J, true, gc.log_n1, gc.log_seqs, ct << gc.log_seqs, P);
             __syncthreads();
             tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
-            tile_store<A, T>(lds, dst, is, false, P);
+            tile_store<A, T, true>(lds, dst, is, ws_fold_columns(gc.log_n), P);
         }
 
         template<typename A, int E>
@@ -409,7 +409,7 @@ namespace hmpc_amd
             J.poly += p;
             J.msg_off += static_cast<u64>(p) << J.log_n;
             std::size_t count = static_cast<std::size_t>(n) << J.log_n;
-            u32* dst = ws + which * A::iol * count;
+            u32* dst = ws + which * A::wsl * count;
             if (which == 0)
                 bgv_column_tile<A, E, 0>(J, lds, dst, gc, ct, count, p, P);
             else if (which == 1)
@@ -438,14 +438,14 @@ namespace hmpc_amd
             u32* su = ws;
             u32 x[E][A::rl];
             // NTT(u) rows back into the workspace, read per element below
-            tile_load<A, T>(lds, su, is);
+            tile_load<A, T, true>(lds, su, is);
             __syncthreads();
             tile_rounds<A, E, false>(x, lds, td, base0, 1u, base.twf, P, false, 0, false);
-            tile_store<A, T>(lds, su, is, false, P);
+            tile_store<A, T, true>(lds, su, is, false, P);
             __syncthreads();
             for (int which = 1; which < 3; ++which)
             {
-                tile_load<A, T>(lds, ws + which * A::iol * count, is);
+                tile_load<A, T, true>(lds, ws + which * A::wsl * count, is);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
                 bool first = which == 1;
@@ -485,7 +485,7 @@ namespace hmpc_amd
         // The two launches above for a table of BGV jobs of any two-pass degrees, keys and nonces
         // (one ciphertext each): block b of a launch works on job map[b] >> 10, local tile
         // map[b] & 1023 (at most 3 x 256 tiles per job, at N = 2^20). Each job keeps its three
-        // column results in its own workspace region [3][iol][N] at ws + ws_off.
+        // column results in its own workspace region [3][wsl][N] at ws + ws_off.
         template<typename A, int E>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_cols_table(job_desc const* __restrict__ jobs,
                                                                                     u32 const* __restrict__ map,
@@ -500,7 +500,7 @@ namespace hmpc_amd
             u32 which = local / gc.blocks_per_poly;
             u32 ct = local - which * gc.blocks_per_poly;
             std::size_t N = std::size_t{1} << J.log_n;
-            u32* dst = ws + J.ws_off + which * A::iol * N;
+            u32* dst = ws + J.ws_off + which * A::wsl * N;
             if (which == 0)
                 bgv_column_tile<A, E, 0>(J, lds, dst, gc, ct, N, 0, P);
             else if (which == 1)
@@ -530,14 +530,14 @@ namespace hmpc_amd
             job_rows_tile(gr, rt, J.out_count, J.out_off, td, io);
             u32 base0 = (1u << gr.log_n1) + hi0;
             u32 x[E][A::rl];
-            tile_load<A, T>(lds, su, is);
+            tile_load<A, T, true>(lds, su, is);
             __syncthreads();
             tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-            tile_store<A, T>(lds, su, is, false, P);
+            tile_store<A, T, true>(lds, su, is, false, P);
             __syncthreads();
             for (int which = 1; which < 3; ++which)
             {
-                tile_load<A, T>(lds, su + which * A::iol * N, is);
+                tile_load<A, T, true>(lds, su + which * A::wsl * N, is);
                 __syncthreads();
                 tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
                 bool first = which == 1;
@@ -571,7 +571,7 @@ namespace hmpc_amd
         // The two-pass product over a table of jobs of any two-pass degrees (one polynomial pair
         // each), in three launches: forward columns of a and b (2 x column tiles per job), rows
         // (forward, product, inverse), inverse columns with the final scaling. Job j keeps
-        // NTT(a), NTT(b) and the row results in its own workspace region [2][iol][N] at
+        // NTT(a), NTT(b) and the row results in its own workspace region [2][wsl][N] at
         // ws + ws_off. Blocks find their job and tile through map as above.
         template<typename A, int E>
         __global__ void __launch_bounds__(tile / E) k_polymul_cols_table(job_desc const* __restrict__ jobs,
@@ -595,7 +595,7 @@ namespace hmpc_amd
             __syncthreads();
             u32 x[E][A::rl];
             tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
-            tile_store<A, T>(lds, ws + J.ws_off + which * A::iol * N, is, false, P);
+            tile_store<A, T, true>(lds, ws + J.ws_off + which * A::wsl * N, is, ws_fold_columns(static_cast<int>(J.log_n)), P);
         }
 
         template<typename A, int E>
@@ -612,25 +612,25 @@ namespace hmpc_amd
             pass_geom gr = geom_for(static_cast<int>(J.log_n), true);
             std::size_t N = std::size_t{1} << J.log_n;
             u32* sa = ws + J.ws_off;
-            u32* sb = sa + A::iol * N;
+            u32* sb = sa + A::wsl * N;
             tile_desc td;
             io_desc is;
             u32 hi0 = job_rows_tile(gr, rt, N, 0, td, is);
             u32 base0 = (1u << gr.log_n1) + hi0;
             u32 x[E][A::rl];
-            tile_load<A, T>(lds, sa, is);
+            tile_load<A, T, true>(lds, sa, is);
             __syncthreads();
             tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-            tile_store<A, T>(lds, sa, is, false, P);
+            tile_store<A, T, true>(lds, sa, is, false, P);
             __syncthreads();
-            tile_load<A, T>(lds, sb, is);
+            tile_load<A, T, true>(lds, sb, is);
             __syncthreads();
             tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
             with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
             dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
             __syncthreads();
             tile_rounds<A, E, true>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
-            tile_store<A, T>(lds, sa, is, false, P);
+            tile_store<A, T, true>(lds, sa, is, false, P);
         }
 
         template<typename A, int E>
@@ -650,7 +650,7 @@ namespace hmpc_amd
             io_desc is, io;
             job_cols_tile(gc, ct, N, 0, td, is);
             job_cols_tile(gc, ct, J.out_count, J.out_off, td, io);
-            tile_load<A, T>(lds, ws + J.ws_off, is);
+            tile_load<A, T, true>(lds, ws + J.ws_off, is);
             __syncthreads();
             u32 x[E][A::rl];
             tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false);

@@ -84,16 +84,17 @@ namespace hmpc_amd
     hipError_t fast_jobs_occupancy(int L, int elems, int* blocks_per_cu);
     hipError_t launch_fast_jobs(int L, transform_args const& T, job_desc const* table, job_desc const& base, u32 njobs,
                                 unsigned grid, u32* counter, u32* scratch, u64 slot_words, hipStream_t s);
-    // batched BGV of n ciphertexts, N > 4096, as two launches over tiles (workspace [3][L][n N])
+    // batched BGV of n ciphertexts, N > 4096, as two launches over tiles (workspace
+    // [3][ws_limbs(L)][n N])
     hipError_t launch_fast_bgv_passes(int L, transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
     // the same over a table of BGV jobs of two-pass degrees (one ciphertext each): block b runs
     // job map[b] >> 10, local tile map[b] & 1023; job j's column results live at
-    // ws + jobs[j].ws_off ([3][L][N])
+    // ws + jobs[j].ws_off ([3][ws_limbs(L)][N])
     hipError_t launch_fast_bgv_table(int L, transform_args const& T, job_desc const* jobs, u32 const* map_cols,
                                      u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s);
     // the two-pass product over a table of jobs of two-pass degrees (one pair each): forward
     // columns (2 x column tiles per job), rows, inverse columns; job j's workspace region is
-    // ws + jobs[j].ws_off ([2][L][N]); maps as above
+    // ws + jobs[j].ws_off ([2][ws_limbs(L)][N]); maps as above
     hipError_t launch_fast_polymul_table(int L, transform_args const& T, job_desc const* jobs, u32 const* map_cols,
                                          u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
                                          u32 icols_blocks, u32* ws, hipStream_t s);

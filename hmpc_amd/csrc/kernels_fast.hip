@@ -57,7 +57,7 @@ namespace hmpc_amd
 
     std::size_t fast_polymul_workspace_words(int L, int log_n, std::size_t n)
     {
-        return log_n <= 12 ? 0 : 2 * static_cast<std::size_t>(L) * (n << log_n);
+        return log_n <= 12 ? 0 : 2 * static_cast<std::size_t>(ws_limbs(L)) * (n << log_n);
     }
 
     hipError_t launch_fast_polymul(int L, transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,

@@ -14,9 +14,12 @@
 // Bounds (q > 2^124, so 64 q > 2^130):
 //   mont(a, b): column sums < 2^61 when a_j < 2^31 and b_j < 2^27; result < 2q (1 + 2^-19) with
 //               normalized limbs (r_4 < 2^22) whenever v(a) v(b) < 2^130 q
-//   forward butterfly (x + t, x - t + K4), t = mont(y, w) < 2q: value +4q and limbs +2^27 per
-//               stage; from inputs < 2q with limbs < 2^26 this stays < 2^130 and < 2^31 for
-//               up to 12 stages, so a whole pass runs without reductions
+//   forward butterfly (x + t, x - t + K3), t = mont(y, w) < 2q (1 + 2^-19): value +3q per stage,
+//               limb 0 +2^27, limbs 1..3 +2^26, limb 4 +2^22. From inputs < 4q with limbs
+//               (2^27, 2^26, 2^26, 2^26, 2^22) (canonical data, or from_signed samples) this
+//               stays below 4q + 3q S and limbs below (S + 1) x those after S stages: a mont
+//               input needs a value < 2^130 > 63.9 q and limbs < 2^32, so up to 19 stages (a
+//               whole transform of N <= 2^19, both of its passes) run without reductions
 //   fold(x):    normalize, then x_4 >= 2^20 carries h = x_4 >> 20 copies of 2^124 = q - c:
 //               replace by -h c (plus one q in borrowed digits to keep limbs >= 0);
 //               result < 2q with limbs < 2^28, for any input limbs < 2^32
@@ -34,6 +37,8 @@ namespace hmpc_amd
     {
         u32 c;         // q = 2^124 + c
         u32 n26;       // -q^-1 mod 2^26
+        u32 k3[5];     // 3q with limbs k3_0 >= 2^26, k3_{1..3} = 2^26 - 1, k3_4 = 3 2^20 - 1
+                       // (>= mont output limbs: r_4 <= 2^21 + 4 for r < 2q (1 + 2^-19))
         u32 k4[5];     // 4q with limbs k4_0 >= 2^26, k4_{1..3} = 2^26 - 1 (>= mont output limbs)
         u32 k32[5];    // 32q with limbs >= 2^29 - 8 (>= folded limbs)
         u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
@@ -42,6 +47,32 @@ namespace hmpc_amd
         u32 t20;       // 2^20 (top radix-2^26 limb of q), kept in a register so that m * 2^20 + T
                        // stays one v_mad_u64_u32 instead of a 64-bit shift and a 64-bit add
     };
+
+    // every constant but rr (which needs R^2 mod q) for q = 2^124 + c, 0 < c < 2^24 odd
+    inline void r26_constants(p26& R, u32 c)
+    {
+        R.c = c;
+        u32 inv = 1;
+        for (int i = 0; i < 6; ++i)
+            inv *= 2u - c * inv;  // c^-1 mod 2^32 (q = c mod 2^32)
+        R.n26 = (0u - inv) & m26;
+        u32 const k3[5] = {3 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, 3 * (1u << 20) - 1};
+        u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
+        u32 const k32[5] = {32 * c + (1u << 29), (1u << 29) - 8, (1u << 29) - 8, (1u << 29) - 8, (1u << 25) - 8};
+        u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
+        for (int i = 0; i < 5; ++i)
+        {
+            R.k3[i] = k3[i];
+            R.k4[i] = k4[i];
+            R.k32[i] = k32[i];
+            R.qb[i] = qb[i];
+        }
+        R.q32[0] = c;
+        R.q32[1] = 0;
+        R.q32[2] = 0;
+        R.q32[3] = 1u << 28;
+        R.t20 = 1u << 20;
+    }
 
     HMPC_HD void r26_unpack(u32 (&x)[5], u32 w0, u32 w1, u32 w2, u32 w3)
     {

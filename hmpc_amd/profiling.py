@@ -8,6 +8,11 @@
         run, each scaled by what the same counter reads for the device copy of N bytes in the
         same run (the copy reads N and writes N bytes)
 
+    python -m hmpc_amd.profiling counters RESULTS.db --workload c2 [--clock-ghz 2.4] [--out valu.json]
+        instruction counters (SQ_INSTS_*) per dispatch of the workload's kernels, and the time
+        their VALU instructions need at the issue rate of one wave instruction per SIMD every
+        4 cycles (CUs x 4 SIMDs x clock / 4)
+
 The traffic file maps bench kernel names (the names of the library's launch probes) to bytes
 per launch; bench.py reports it as roofline.traffic.
 """
@@ -28,7 +33,8 @@ WORKLOAD_KERNELS = {
         "fast polymul inverse columns": "hmpc_amd::fast::k_pass<",  # the only per-pass transform of the step
     },
     "c3": {
-        "fast persistent jobs": "hmpc_amd::fast::k_jobs<",
+        "fast bgv columns": "hmpc_amd::fast::k_bgv_cols<",
+        "fast bgv rows": "hmpc_amd::fast::k_bgv_rows<",
     },
 }
 COPY_SYMBOL = "__amd_rocclr_copyBuffer"
@@ -114,6 +120,40 @@ def cmd_traffic(args):
     print("\n".join(lines))
 
 
+def cmd_counters(args):
+    cur = sqlite3.connect(args.db).cursor()
+    rows = cur.execute("select kernel_name, grid_size, counter_name, value from counters_collection").fetchall()
+    groups = collections.defaultdict(lambda: collections.defaultdict(list))
+    for name, grid, counter, value in rows:
+        groups[(_short(name), int(grid))][counter].append(float(value))
+    issue = args.cus * 4 * args.clock_ghz * 1e9 / 4  # wave instructions per second
+    entries = {}
+    print(f"VALU issue rate: {args.cus} CUs x 4 SIMDs x {args.clock_ghz} GHz / 4 cycles = {issue / 1e9:.1f} G wave-instr/s")
+    for bench_name, prefix in WORKLOAD_KERNELS[args.workload].items():
+        cands = [(g, c) for (n, g), c in groups.items() if n.startswith(prefix)]
+        if not cands:
+            continue
+        grid, counters = max(cands, key=lambda c: c[0])
+        avg = {k: sum(v) / len(v) for k, v in counters.items()}
+        print(f"{bench_name} (grid {grid}, {len(next(iter(counters.values())))} dispatches):")
+        for k in sorted(avg):
+            print(f"  {k:<24} {avg[k]:>16.6g}")
+        if "SQ_INSTS_VALU" in avg:
+            print(f"  VALU issue time          {avg['SQ_INSTS_VALU'] / issue * 1e3:>13.3f} ms")
+        entries[bench_name] = dict(avg, grid=grid)
+    if args.out:
+        out = {}
+        try:
+            with open(args.out) as f:
+                out = json.load(f)
+        except (OSError, ValueError):
+            pass
+        out[args.workload] = entries
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+            f.write("\n")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -126,8 +166,14 @@ def main(argv=None):
     t.add_argument("--workload", required=True, choices=sorted(WORKLOAD_KERNELS))
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--out", default="")
+    c = sub.add_parser("counters")
+    c.add_argument("db")
+    c.add_argument("--workload", required=True, choices=sorted(WORKLOAD_KERNELS))
+    c.add_argument("--cus", type=int, default=256)
+    c.add_argument("--clock-ghz", type=float, default=2.4)
+    c.add_argument("--out", default="")
     args = ap.parse_args(argv)
-    (cmd_kernels if args.cmd == "kernels" else cmd_traffic)(args)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -12,29 +12,10 @@
 
 using namespace hmpc_amd;
 
-static p26 make(u32 c, u32 q_unused)
+static p26 make(u32 c)
 {
-    (void)q_unused;
     p26 R{};
-    R.c = c;
-    u32 inv = 1;
-    for (int i = 0; i < 6; ++i)
-        inv *= 2u - c * inv;  // q mod 2^32 = c (q = 2^124 + c)
-    R.n26 = (0u - inv) & m26;
-    u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
-    u32 const k32[5] = {32 * c + (1u << 29), (1u << 29) - 8, (1u << 29) - 8, (1u << 29) - 8, (1u << 25) - 8};
-    u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
-    for (int i = 0; i < 5; ++i)
-    {
-        R.k4[i] = k4[i];
-        R.k32[i] = k32[i];
-        R.qb[i] = qb[i];
-    }
-    R.q32[0] = c;
-    R.q32[1] = 0;
-    R.q32[2] = 0;
-    R.q32[3] = 0x10000000u;
-    R.t20 = 1u << 20;
+    r26_constants(R, c);
     return R;
 }
 
@@ -45,7 +26,7 @@ int main()
     {
         u32 c;
         std::cin >> c;
-        p26 P = make(c, 0);
+        p26 P = make(c);
         if (cmd == "M")
         {
             u32 a[5], b[5], r[5];
@@ -70,8 +51,10 @@ int main()
         }
         else if (cmd == "B" || cmd == "I")
         {
-            // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t + 4q) keeping the
-            // first output and the second alternately, or inverse (fold(x + y), mont(x - y + 32q, w))
+            // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t + 3q), t = mont(y, w),
+            // or inverse (fold(x + y), mont(x - y + 32q, w)). Per stage: the twiddle limbs, then
+            // for "B" a selector of the next stage's pair (0: (x', y'), 1: (y', y'), 2: (x', x'));
+            // prints the pair after every stage
             int stages;
             std::cin >> stages;
             u32 x[5], y[5];
@@ -83,13 +66,22 @@ int main()
                 for (auto& v : w) std::cin >> v;
                 if (cmd == "B")
                 {
+                    int sel;
+                    std::cin >> sel;
                     u32 t[5];
                     r26_mont(t, y, w, P);
                     for (int l = 0; l < 5; ++l)
                     {
                         u32 xl = x[l];
                         x[l] = xl + t[l];
-                        y[l] = xl + (P.k4[l] - t[l]);
+                        y[l] = xl + (P.k3[l] - t[l]);
+                    }
+                    for (int l = 0; l < 5; ++l)
+                    {
+                        if (sel == 1)
+                            x[l] = y[l];
+                        else if (sel == 2)
+                            y[l] = x[l];
                     }
                 }
                 else
@@ -103,8 +95,9 @@ int main()
                     r26_fold(x, P);
                     r26_mont(y, d, w, P);
                 }
+                printf("%u %u %u %u %u %u %u %u %u %u%s", x[0], x[1], x[2], x[3], x[4], y[0], y[1], y[2], y[3], y[4],
+                       s + 1 < stages ? " " : "\n");
             }
-            printf("%u %u %u %u %u %u %u %u %u %u\n", x[0], x[1], x[2], x[3], x[4], y[0], y[1], y[2], y[3], y[4]);
         }
         fflush(stdout);
     }

@@ -1,8 +1,9 @@
 """CPU checks of the radix-2^26 device arithmetic (hmpc_amd/csrc/r26.hpp) built for the host.
 
 The same header is compiled into the gfx950 kernels; here it runs on the CPU against python
-big integers, including the lazy bounds the fast transform relies on (values < 2^130 and
-limbs < 2^31 across 12 forward stages; inverse stages stay below 2q).
+big integers, including the lazy bounds the fast transform relies on: forward stages add at
+most 3q and at most one input-limb bound per limb, so 19 stages from inputs < 4q keep every mont
+input below 2^130 with limbs < 2^32; inverse stages stay below 2q.
 """
 
 import os
@@ -86,25 +87,59 @@ def test_mont_fold_canonical(tool, q):
         assert sum(v << (32 * i) for i, v in enumerate(w)) == x % q
 
 
-@pytest.mark.parametrize("q", PRIMES)
-def test_forward_chain_bounds(tool, q):
-    """12 lazy forward stages: the value and limb bounds hold and the result is exact mod q."""
+LIMB_STEP = (1 << 27, 1 << 26, 1 << 26, 1 << 26, 1 << 22)  # per-stage limb growth bound (and input bound)
+
+
+def _signed_sample(q, rng):
+    """a from_signed-shaped input: z R mod q for |z| small, negatives as 4q - t (limbs of k4 - t)"""
+    v = rng.randrange(2 * q)
+    if rng.random() < 0.5:
+        return limbs(v)
     c = q - (1 << 124)
-    rng = random.Random(7)
+    k4 = [4 * c + (1 << 26), (1 << 26) - 1, (1 << 26) - 1, (1 << 26) - 1, (1 << 22) - 1]
+    t = limbs(v)
+    return [k - x for k, x in zip(k4, t)]
+
+
+@pytest.mark.parametrize("q", PRIMES)
+@pytest.mark.parametrize("stages", [12, 16, 19])
+def test_forward_chain_bounds(tool, q, stages):
+    """Lazy forward stages from inputs < 4q (canonical or from_signed-shaped), each stage keeping
+    either output pair or a doubled output (the +3q branch twice, the worst value growth): after
+    s stages every value is < 4q + 3q s with limbs < (s + 1) x LIMB_STEP, every mont input is
+    < 2^130, and the results are exact mod q."""
+    c = q - (1 << 124)
+    rng = random.Random(7 + stages)
     R = 1 << 130
-    for _ in range(200):
-        x, y = rng.randrange(2 * q), rng.randrange(2 * q)
-        ws = [rng.randrange(q) for _ in range(12)]
-        line = f"B {c} 12 " + " ".join(map(str, limbs(x) + limbs(y))) + " " + \
-               " ".join(" ".join(map(str, limbs(w))) for w in ws)
+    for it in range(150):
+        lx, ly = _signed_sample(q, rng), _signed_sample(q, rng)
+        ws = [rng.randrange(q) if rng.random() < 0.8 else rng.randrange(4) for _ in range(stages)]
+        sels = [1 if it % 3 == 0 else rng.randrange(3) for _ in range(stages)]
+        line = f"B {c} {stages} " + " ".join(map(str, lx + ly)) + " " + \
+               " ".join(" ".join(map(str, limbs(w))) + f" {sel}" for w, sel in zip(ws, sels))
         out = run(tool, line)
-        X, Y = value(out[:5]), value(out[5:])
-        ex, ey = x, y
-        for w in ws:
+        assert len(out) == 10 * stages
+        ex, ey = value(lx) % q, value(ly) % q
+        prev_y = value(ly)
+        for s, (w, sel) in enumerate(zip(ws, sels)):
+            assert prev_y < R, "mont input >= 2^130"
             t = ey * w * pow(R, -1, q) % q
             ex, ey = (ex + t) % q, (ex - t) % q
-        assert X % q == ex and Y % q == ey
-        assert X < (1 << 130) and Y < (1 << 130) and max(out) < (1 << 31)
+            if sel == 1:
+                ex = ey
+            elif sel == 2:
+                ey = ex
+            st = out[10 * s:10 * s + 10]
+            X, Y = value(st[:5]), value(st[5:])
+            assert X % q == ex and Y % q == ey, f"stage {s}"
+            for v in (X, Y):
+                assert v < 4 * q + 3 * q * (s + 1)
+            for half in (st[:5], st[5:]):
+                assert all(half[j] < (s + 2) * LIMB_STEP[j] for j in range(5)), f"limbs at stage {s}"
+            prev_y = Y
+        # a forward output folds (point-wise products, stores) to < 2q with limbs < 2^28
+        f = run(tool, f"F {c} " + " ".join(map(str, out[-5:])))
+        assert value(f) % q == ey and value(f) < 2 * q and all(v < (1 << 28) for v in f)
 
 
 @pytest.mark.parametrize("q", PRIMES)
@@ -117,7 +152,7 @@ def test_inverse_chain_bounds(tool, q):
         lx, ly = redundant(x, rng, 28), redundant(y, rng, 28)
         ws = [rng.randrange(q) for _ in range(12)]
         line = f"I {c} 12 " + " ".join(map(str, lx + ly)) + " " + " ".join(" ".join(map(str, limbs(w))) for w in ws)
-        out = run(tool, line)
+        out = run(tool, line)[-10:]
         X, Y = value(out[:5]), value(out[5:])
         ex, ey = x, y
         for w in ws:
