@@ -154,8 +154,9 @@ namespace hmpc_amd
                 }
             }
 
-            // (x, y) -> (fold(x + y), mont(x - y + 32q, w))
-            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            // (x, y) -> (x + y, mont(x - y + 32q, w)), x + y folded when `fold` (odd stages)
+            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P,
+                                                       bool fold)
             {
                 u32 d[rl];
 #pragma unroll
@@ -164,7 +165,8 @@ namespace hmpc_amd
                     d[l] = x[l] + (P.k32[l] - y[l]);
                     x[l] = x[l] + y[l];
                 }
-                r26_fold(x, P);
+                if (fold)
+                    r26_fold(x, P);
                 r26_mont(y, d, w, P);
             }
 
@@ -207,19 +209,10 @@ namespace hmpc_amd
                     x[l] += t[l] << 2;
             }
 
-            // signed integer |z| < 2^78 -> z R mod q as a transform input (< 4q, limbs < 2^27):
-            // mont(|z|, R 2^130) = |z| R (normalized, < 2q), negated as 4q - t with the
-            // borrowed-digit constant k4 (limbs of k4 dominate normalized limbs)
+            // signed integer -> z R mod q as a transform input (< q, limbs < 2^27)
             __device__ __forceinline__ static void from_signed(u32 (&x)[rl], long long z, params const& P)
             {
-                u64 az = z < 0 ? static_cast<u64>(-z) : static_cast<u64>(z);
-                u32 a[rl] = {static_cast<u32>(az) & m26, static_cast<u32>(az >> 26) & m26, static_cast<u32>(az >> 52), 0u, 0u};
-                u32 t[rl];
-                r26_mont(t, a, P.rr, P);
-                bool neg = z < 0;
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                    x[l] = neg ? P.k4[l] - t[l] : t[l];
+                r26_from_signed(x, z, P);
             }
         };
 
@@ -294,7 +287,8 @@ namespace hmpc_amd
                 ops::fwd(x, y, w, P);
             }
 
-            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P,
+                                                       bool)
             {
                 ops::inv(x, y, w, P);
             }
@@ -495,12 +489,16 @@ namespace hmpc_amd
             }
         }
 
+        // g0: index of the round's first stage within the whole inverse transform (its parity
+        // selects the stages that fold)
         template<typename A, int E, bool Inv, int W, int II, bool Last>
         __device__ __forceinline__ void stage(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
-                                              u32 const (&w0)[A::rl], u32 const (&w1)[A::rl], typename A::params const& P)
+                                              u32 const (&w0)[A::rl], u32 const (&w1)[A::rl], typename A::params const& P,
+                                              int g0)
         {
             constexpr int K = 1 << W;
             constexpr int lb = Inv ? II : (W - 1 - II);
+            bool const fold = ((g0 + II) & 1) != 0;
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
@@ -515,7 +513,7 @@ namespace hmpc_amd
                     else if constexpr (Last)
                         A::inv_last(x[g * K + k], x[g * K + (k | (1 << lb))], w0, w1, P);
                     else
-                        A::inv(x[g * K + k], x[g * K + (k | (1 << lb))], tw_regs[g][slot], P);
+                        A::inv(x[g * K + k], x[g * K + (k | (1 << lb))], tw_regs[g][slot], P, fold);
                 }
             }
         }
@@ -523,12 +521,12 @@ namespace hmpc_amd
         template<typename A, int E, bool Inv, int W, int II>
         __device__ __forceinline__ void stages_upto(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
                                                     u32 const (&w0)[A::rl], u32 const (&w1)[A::rl],
-                                                    typename A::params const& P)
+                                                    typename A::params const& P, int g0)
         {
             if constexpr (II < W - 1)
             {
-                stage<A, E, Inv, W, II, false>(x, tw_regs, w0, w1, P);
-                stages_upto<A, E, Inv, W, II + 1>(x, tw_regs, w0, w1, P);
+                stage<A, E, Inv, W, II, false>(x, tw_regs, w0, w1, P, g0);
+                stages_upto<A, E, Inv, W, II + 1>(x, tw_regs, w0, w1, P, g0);
             }
         }
 
@@ -536,42 +534,45 @@ namespace hmpc_amd
         template<typename A, int E, bool Inv, int W, int II>
         __device__ __forceinline__ void stages_prefetched(u32 (&x)[E][A::rl], u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
                                                           tile_desc const& td, int a, u32 base0, u32 base_step,
-                                                          u32 const* __restrict__ tw, typename A::params const& P)
+                                                          u32 const* __restrict__ tw, typename A::params const& P, int g0)
         {
             if constexpr (II < W - 1)
             {
                 u32 const z[A::rl] = {};
                 prefetch_stage<A, E, Inv, W, II>(tw_regs, td, a, base0, base_step, tw);
-                stage<A, E, Inv, W, II, false>(x, tw_regs, z, z, P);
-                stages_prefetched<A, E, Inv, W, II + 1>(x, tw_regs, td, a, base0, base_step, tw, P);
+                stage<A, E, Inv, W, II, false>(x, tw_regs, z, z, P, g0);
+                stages_prefetched<A, E, Inv, W, II + 1>(x, tw_regs, td, a, base0, base_step, tw, P, g0);
             }
         }
 
         // butterflies of one round: all twiddle loads first, then the stages. The round's last
-        // stage is the inverse's scaled final stage when the round ends the transform.
+        // stage is the inverse's scaled final stage when the round ends the transform. off: the
+        // inverse stages before this tile's sub-transform (a column pass after a row pass whose
+        // results were not folded); stage a + II of the tile is stage off + a + II of the inverse.
         template<typename A, int E, bool Inv, int W>
         __device__ __forceinline__ void compute_round(u32 (&x)[E][A::rl], tile_desc const& td, int a, u32 base0,
                                                       u32 base_step, u32 const* __restrict__ tw,
-                                                      typename A::params const& P, bool final_scale)
+                                                      typename A::params const& P, bool final_scale, int off)
         {
             u32 tw_regs[E >> W][(1 << W) - 1][A::rl];
             bool const scaled = Inv && final_scale && (a + W == td.s);
             u32 w0[A::rl] = {}, w1[A::rl] = {};
+            int const g0 = off + a;
             if constexpr (A::stage_prefetch)
             {
                 // twiddles loaded stage by stage: at most one stage's twiddles live (fewer VGPRs,
                 // the latency is covered by the other workgroup on the CU)
-                stages_prefetched<A, E, Inv, W, 0>(x, tw_regs, td, a, base0, base_step, tw, P);
+                stages_prefetched<A, E, Inv, W, 0>(x, tw_regs, td, a, base0, base_step, tw, P, g0);
                 if (scaled)
                 {
                     A::tw_get(w0, tw, 0);
                     A::tw_get(w1, tw, 1);
-                    stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P);
+                    stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P, g0);
                 }
                 else
                 {
                     prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, td, a, base0, base_step, tw);
-                    stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P);
+                    stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P, g0);
                 }
                 return;
             }
@@ -583,20 +584,20 @@ namespace hmpc_amd
             }
             else
                 prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, td, a, base0, base_step, tw);
-            stages_upto<A, E, Inv, W, 0>(x, tw_regs, w0, w1, P);
+            stages_upto<A, E, Inv, W, 0>(x, tw_regs, w0, w1, P, g0);
             if (scaled)
-                stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P);
+                stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P, g0);
             else
-                stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P);
+                stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P, g0);
         }
 
         template<typename A, int E, bool Inv, int W>
         __device__ __forceinline__ void one_round(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a, u32 base0,
                                                   u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
-                                                  bool final_scale, bool keep)
+                                                  bool final_scale, bool keep, int off)
         {
             gather<A, E, W>(x, lds, td, a, Inv);
-            compute_round<A, E, Inv, W>(x, td, a, base0, base_step, tw, P, final_scale);
+            compute_round<A, E, Inv, W>(x, td, a, base0, base_step, tw, P, final_scale, off);
             if (!keep)
                 scatter<A, E, W>(x, lds, td, a, Inv);
         }
@@ -604,21 +605,21 @@ namespace hmpc_amd
         template<typename A, int E, bool Inv>
         __device__ __forceinline__ void dispatch_round(int w, u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a,
                                                        u32 base0, u32 base_step, u32 const* __restrict__ tw,
-                                                       typename A::params const& P, bool final_scale, bool keep)
+                                                       typename A::params const& P, bool final_scale, bool keep, int off)
         {
             constexpr int WM = shape<E>::wmax;
             if (w == WM)
-                one_round<A, E, Inv, WM>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                one_round<A, E, Inv, WM>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep, off);
             else if (WM > 1 && w == 1)
-                one_round<A, E, Inv, 1>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                one_round<A, E, Inv, 1>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep, off);
             else if constexpr (WM > 2)
             {
                 if (w == 2)
-                    one_round<A, E, Inv, 2>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                    one_round<A, E, Inv, 2>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep, off);
                 else if constexpr (WM > 3)
                 {
                     if (w == 3)
-                        one_round<A, E, Inv, 3>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                        one_round<A, E, Inv, 3>(x, lds, td, a, base0, base_step, tw, P, final_scale, keep, off);
                 }
             }
         }
@@ -629,7 +630,7 @@ namespace hmpc_amd
         template<typename A, int E, bool Inv>
         __device__ __forceinline__ void tile_rounds(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
                                                     u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
-                                                    bool final_scale, int first_round, bool keep_last)
+                                                    bool final_scale, int first_round, bool keep_last, int off = 0)
         {
             constexpr int WM = shape<E>::wmax;
             int n = round_count<WM>(td.s);
@@ -638,7 +639,7 @@ namespace hmpc_amd
                 int a = round_start<WM>(td.s, r, Inv);
                 int w = round_width<WM>(td.s, r, Inv);
                 bool keep = keep_last && r == n - 1;
-                dispatch_round<A, E, Inv>(w, x, lds, td, a, base0, base_step, tw, P, final_scale, keep);
+                dispatch_round<A, E, Inv>(w, x, lds, td, a, base0, base_step, tw, P, final_scale, keep, off);
                 if (!keep)
                     __syncthreads();
             }
@@ -650,7 +651,7 @@ namespace hmpc_amd
                                                    u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
                                                    bool final_scale)
         {
-            compute_round<A, E, true, W>(x, td, 0, base0, base_step, tw, P, final_scale);
+            compute_round<A, E, true, W>(x, td, 0, base0, base_step, tw, P, final_scale, 0);
             scatter<A, E, W>(x, lds, td, 0, true);
         }
 
@@ -934,7 +935,9 @@ namespace hmpc_amd
             tile_load<A, T, WsIn>(lds, in, io);
             __syncthreads();
             u32 x[E][A::rl];
-            tile_rounds<A, E, Inv>(x, lds, td, base0, base_step, tw, P, final_scale, 0, false);
+            // a workspace input holds the row pass's unfolded results: the column stages continue
+            // the inverse's stage numbering after the row pass's log_n2 stages
+            tile_rounds<A, E, Inv>(x, lds, td, base0, base_step, tw, P, final_scale, 0, false, WsIn ? G.log_n2 : 0);
             tile_store<A, T>(lds, out, io, canonical, P);
         }
 

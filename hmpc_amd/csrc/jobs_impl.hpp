@@ -225,7 +225,7 @@ namespace hmpc_amd
                 job_cols_tile(gc, t, J.out_count, J.out_off, td, io);
                 tile_load<A, T, true>(lds, sa, is);
                 __syncthreads();
-                tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false);
+                tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false, gc.log_n2);
                 tile_store<A, T>(lds, J.out0, io, true, P);
                 __syncthreads();
             }
@@ -653,7 +653,7 @@ namespace hmpc_amd
             tile_load<A, T, true>(lds, ws + J.ws_off, is);
             __syncthreads();
             u32 x[E][A::rl];
-            tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false);
+            tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false, gc.log_n2);
             tile_store<A, T>(lds, J.out0, io, true, P);
         }
 

@@ -23,8 +23,11 @@
 //   fold(x):    normalize, then x_4 >= 2^20 carries h = x_4 >> 20 copies of 2^124 = q - c:
 //               replace by -h c (plus one q in borrowed digits to keep limbs >= 0);
 //               result < 2q with limbs < 2^28, for any input limbs < 2^32
-//   inverse butterfly (fold(x + y), mont(x - y + K32, w)) keeps every value < 2q (< 8q for
-//               the first stage after a x4 product correction)
+//   inverse butterfly (x + y, mont(x - y + K32, w)), with x + y folded at odd stages of the
+//               transform: at odd stages the inputs are < 4q (1 + 2^-19) with limbs < 2^29 (one
+//               unfolded sum of folded values), at even stages < 2q (1 + 2^-19) with limbs
+//               < 2^28; from products (< 8q, limbs < 2^28 after the x4 correction) stage 0
+//               gives < 16q, limbs < 2^29. Every mont input stays < 48q with limbs < 2^31.
 #pragma once
 
 #include "modarith.hpp"
@@ -40,7 +43,8 @@ namespace hmpc_amd
         u32 k3[5];     // 3q with limbs k3_0 >= 2^26, k3_{1..3} = 2^26 - 1, k3_4 = 3 2^20 - 1
                        // (>= mont output limbs: r_4 <= 2^21 + 4 for r < 2q (1 + 2^-19))
         u32 k4[5];     // 4q with limbs k4_0 >= 2^26, k4_{1..3} = 2^26 - 1 (>= mont output limbs)
-        u32 k32[5];    // 32q with limbs >= 2^29 - 8 (>= folded limbs)
+        u32 k32[5];    // 32q with limbs >= 12 2^26 - 12 > 2^29 (>= the limbs of a folded value and
+                       // of one unfolded sum of two)
         u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
         u32 q32[4];    // q as 32-bit limbs (final canonical subtraction)
         u32 rr[5];     // R 2^130 mod q (R = 2^128), normalized: mont(z, rr) = z R mod q
@@ -58,7 +62,8 @@ namespace hmpc_amd
         R.n26 = (0u - inv) & m26;
         u32 const k3[5] = {3 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, 3 * (1u << 20) - 1};
         u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
-        u32 const k32[5] = {32 * c + (1u << 29), (1u << 29) - 8, (1u << 29) - 8, (1u << 29) - 8, (1u << 25) - 8};
+        u32 const k32[5] = {32 * c + 12 * (1u << 26), 12 * (1u << 26) - 12, 12 * (1u << 26) - 12, 12 * (1u << 26) - 12,
+                            (1u << 25) - 12};
         u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
         for (int i = 0; i < 5; ++i)
         {
@@ -156,6 +161,26 @@ namespace hmpc_amd
         c = T[8] + (c >> 26);
         r[3] = static_cast<u32>(c) & m26;
         r[4] = static_cast<u32>(c >> 26);
+    }
+
+    // signed integer z -> a value congruent to z R mod q (R = 2^128) with limbs
+    // (< 2^27, < 2^26, < 2^26, < 2^26, < 2^20) and value < q, without a Montgomery product:
+    // 2^128 = 16 (q - c) = -16 c mod q, so z R = -16 c z, and t = 16 c |z| < 2^92 < q for any
+    // |z| < 2^64. z < 0: t itself; z >= 0: q - t, formed limb-wise from q's borrowed digits qb
+    // (qb_j >= t_j for every limb: no borrows).
+    HMPC_HD void r26_from_signed(u32 (&x)[5], long long z, p26 const& P)
+    {
+        u64 az = z < 0 ? static_cast<u64>(0) - static_cast<u64>(z) : static_cast<u64>(z);
+        u32 c16 = P.c << 4;  // < 2^28
+        u64 t0 = static_cast<u64>(static_cast<u32>(az) & m26) * c16;
+        u64 t1 = static_cast<u64>(static_cast<u32>(az >> 26) & m26) * c16 + (t0 >> 26);
+        u64 t2 = static_cast<u64>(static_cast<u32>(az >> 52)) * c16 + (t1 >> 26);
+        u32 t[5] = {static_cast<u32>(t0) & m26, static_cast<u32>(t1) & m26, static_cast<u32>(t2) & m26,
+                    static_cast<u32>(t2 >> 26), 0u};
+        bool neg = z < 0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+            x[l] = neg ? t[l] : P.qb[l] - t[l];
     }
 
     // x (value < 2q after a fold, limbs < 2^28) -> canonical 32-bit limbs

@@ -3,6 +3,7 @@
 //   M c a0..a4 b0..b4      -> r26_mont limbs
 //   F c x0..x4             -> r26_fold limbs
 //   C c x0..x4             -> r26_canonical 32-bit words
+//   S c z                  -> r26_from_signed limbs (z a signed 64-bit integer)
 //   B c stages x0..x4 y0..y4 w0..w4 (per stage: w limbs)  -> forward butterfly chain
 #include "../../hmpc_amd/csrc/r26.hpp"
 
@@ -42,6 +43,14 @@ int main()
             r26_fold(x, P);
             printf("%u %u %u %u %u\n", x[0], x[1], x[2], x[3], x[4]);
         }
+        else if (cmd == "S")
+        {
+            long long z;
+            std::cin >> z;
+            u32 x[5];
+            r26_from_signed(x, z, P);
+            printf("%u %u %u %u %u\n", x[0], x[1], x[2], x[3], x[4]);
+        }
         else if (cmd == "C")
         {
             u32 x[5], w[4];
@@ -52,9 +61,10 @@ int main()
         else if (cmd == "B" || cmd == "I")
         {
             // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t + 3q), t = mont(y, w),
-            // or inverse (fold(x + y), mont(x - y + 32q, w)). Per stage: the twiddle limbs, then
-            // for "B" a selector of the next stage's pair (0: (x', y'), 1: (y', y'), 2: (x', x'));
-            // prints the pair after every stage
+            // or inverse (x + y, mont(x - y + 32q, w)) with x + y folded on request. Per stage: the
+            // twiddle limbs, then for "B" a selector of the next stage's pair (0: (x', y'),
+            // 1: (y', y'), 2: (x', x')), for "I" the fold flag and a selector (0: (x', y'),
+            // 1: (x', x'), 2: (y', y')); prints the pair after every stage
             int stages;
             std::cin >> stages;
             u32 x[5], y[5];
@@ -86,14 +96,24 @@ int main()
                 }
                 else
                 {
+                    int fold, sel;
+                    std::cin >> fold >> sel;
                     u32 d[5];
                     for (int l = 0; l < 5; ++l)
                     {
                         d[l] = x[l] + (P.k32[l] - y[l]);
                         x[l] = x[l] + y[l];
                     }
-                    r26_fold(x, P);
+                    if (fold)
+                        r26_fold(x, P);
                     r26_mont(y, d, w, P);
+                    for (int l = 0; l < 5; ++l)
+                    {
+                        if (sel == 1)
+                            y[l] = x[l];
+                        else if (sel == 2)
+                            x[l] = y[l];
+                    }
                 }
                 printf("%u %u %u %u %u %u %u %u %u %u%s", x[0], x[1], x[2], x[3], x[4], y[0], y[1], y[2], y[3], y[4],
                        s + 1 < stages ? " " : "\n");

@@ -87,6 +87,21 @@ def test_mont_fold_canonical(tool, q):
         assert sum(v << (32 * i) for i, v in enumerate(w)) == x % q
 
 
+@pytest.mark.parametrize("q", PRIMES)
+def test_from_signed(tool, q):
+    """z -> z R mod q (R = 2^128) without a product: exact, value < q, limbs within the forward
+    transform's input bounds; edge values of the signed 64-bit range included."""
+    c = q - (1 << 124)
+    rng = random.Random(5)
+    zs = [0, 1, -1, 2, -2, (1 << 63) - 1, -(1 << 63), (1 << 45) + 12345, -(1 << 45) - 7]
+    zs += [rng.randrange(-(1 << 63), 1 << 63) for _ in range(300)] + [rng.randrange(-(1 << 46), 1 << 46) for _ in range(300)]
+    for z in zs:
+        x = run(tool, f"S {c} {z}")
+        v = value(x)
+        assert v % q == z * (1 << 128) % q, z
+        assert v <= q and x[0] < (1 << 27) and all(l < (1 << 26) for l in x[1:4]) and x[4] < (1 << 20)
+
+
 LIMB_STEP = (1 << 27, 1 << 26, 1 << 26, 1 << 26, 1 << 22)  # per-stage limb growth bound (and input bound)
 
 
@@ -143,19 +158,47 @@ def test_forward_chain_bounds(tool, q, stages):
 
 
 @pytest.mark.parametrize("q", PRIMES)
-def test_inverse_chain_bounds(tool, q):
+@pytest.mark.parametrize("stages", [12, 16])
+def test_inverse_chain_bounds(tool, q, stages):
+    """Lazy inverse stages, x + y folded at odd stages only, from product-shaped inputs (< 8q,
+    limbs < 2^28), each stage keeping either output pair or a doubled output: every mont input
+    x - y + 32q stays < 48q with limbs < 2^31 (no limb of y exceeds 32q's), odd-stage inputs
+    stay < 4q (1 + 2^-19) with limbs < 2^29, the folded outputs < 2q with limbs < 2^28, and the
+    results are exact mod q."""
     c = q - (1 << 124)
-    rng = random.Random(11)
+    rng = random.Random(11 + stages)
     R = 1 << 130
-    for _ in range(200):
+    k32 = [32 * c + 12 * (1 << 26), 12 * (1 << 26) - 12, 12 * (1 << 26) - 12, 12 * (1 << 26) - 12, (1 << 25) - 12]
+    assert value(k32) == 32 * q
+    for it in range(150):
         x, y = rng.randrange(8 * q), rng.randrange(8 * q)
         lx, ly = redundant(x, rng, 28), redundant(y, rng, 28)
-        ws = [rng.randrange(q) for _ in range(12)]
-        line = f"I {c} 12 " + " ".join(map(str, lx + ly)) + " " + " ".join(" ".join(map(str, limbs(w))) for w in ws)
-        out = run(tool, line)[-10:]
-        X, Y = value(out[:5]), value(out[5:])
-        ex, ey = x, y
-        for w in ws:
+        ws = [rng.randrange(q) for _ in range(stages)]
+        sels = [1 if it % 3 == 0 else rng.randrange(3) for _ in range(stages)]
+        folds = [s & 1 for s in range(stages)]
+        line = f"I {c} {stages} " + " ".join(map(str, lx + ly)) + " " + \
+               " ".join(" ".join(map(str, limbs(w))) + f" {f} {sel}" for w, f, sel in zip(ws, folds, sels))
+        out = run(tool, line)
+        assert len(out) == 10 * stages
+        ex, ey = x % q, y % q
+        px, py = lx, ly
+        for s, (w, f, sel) in enumerate(zip(ws, folds, sels)):
+            d = [a + (k - b) for a, k, b in zip(px, k32, py)]
+            assert all(0 <= k - b for k, b in zip(k32, py)), f"limb of y above 32q's at stage {s}"
+            assert value(d) < 48 * q and max(d) < (1 << 31), f"mont input at stage {s}"
+            if f:  # odd stage: inputs are one unfolded sum (or the stage-0 sum of products)
+                bound = 16 * q if s == 1 else 4 * q + (q >> 17)
+                assert value(px) < bound and value(py) < bound and max(px + py) < (1 << 29) + (1 << 27)
             ex, ey = (ex + ey) % q, (ex - ey) * w * pow(R, -1, q) % q
-        assert X % q == ex and Y % q == ey
-        assert X < 2 * q and Y < 2 * q and max(out) < (1 << 28)
+            if sel == 1:
+                ey = ex
+            elif sel == 2:
+                ex = ey
+            st = out[10 * s:10 * s + 10]
+            X, Y = value(st[:5]), value(st[5:])
+            assert X % q == ex and Y % q == ey, f"stage {s}"
+            if f and sel != 2:
+                assert X < 2 * q and max(st[:5]) < (1 << 28)
+            px, py = st[:5], st[5:]
+
+
