@@ -5,7 +5,7 @@ package holds its sources (``csrc/``), its build step (``build.py``) and thin ct
 bindings (``runtime.py``).
 """
 
-from .runtime import (Batch, Buffer, Context, Event, HmpcError, Modulus, Server, Stream, View, LIB_PATH,  # noqa: F401
+from .runtime import (Batch, Buffer, Context, Event, HmpcError, HostBuffer, Modulus, Server, Stream, View, LIB_PATH,  # noqa: F401
                       SIGNATURES,
                       int_to_words, load, words_to_int)
 

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmpc_hip.so")
 SOURCES = ["kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip", "kernels_fast_l4.hip", "kernels_jobs_l2.hip",
-           "kernels_jobs_l4.hip", "api.hip"]
+           "kernels_jobs_l4.hip", "kernels_ext.hip", "api.hip"]
 ARCH = os.environ.get("HMPC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

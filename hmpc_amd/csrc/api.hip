@@ -701,6 +701,48 @@ int hmpc_memcpy_d2d(hmpc_context ctx, void* dst, const void* src, size_t bytes, 
     return HMPC_OK;
 }
 
+int hmpc_host_alloc(hmpc_context ctx, size_t bytes, void** out)
+{
+    if (!ctx || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    device_guard g(ctx->device);
+    *out = nullptr;
+    if (bytes == 0)
+        return HMPC_OK;
+    HMPC_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    return HMPC_OK;
+}
+
+int hmpc_host_free(hmpc_context ctx, void* ptr)
+{
+    if (!ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null context");
+    device_guard g(ctx->device);
+    if (ptr)
+        HMPC_HIP(hipHostFree(ptr));
+    return HMPC_OK;
+}
+
+int hmpc_memcpy_h2d_async(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!ctx || (bytes && (!dst || !src)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    device_guard g(ctx->device);
+    if (bytes)
+        HMPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_memcpy_d2h_async(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!ctx || (bytes && (!dst || !src)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    device_guard g(ctx->device);
+    if (bytes)
+        HMPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    return HMPC_OK;
+}
+
 int hmpc_stream_create(hmpc_context ctx, void** stream)
 {
     device_guard g(ctx->device);
@@ -1003,6 +1045,63 @@ int hmpc_pointwise_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t
     device_guard g(mod->ctx->device);
     HMPC_HIP(launch_pointwise_scalar(mod->limbs, mod->P, op, count, a, c_host, out, as_stream(stream)));
     return HMPC_OK;
+}
+
+int hmpc_ciphertext_op(hmpc_modulus mod, int op, size_t count, const uint32_t* a0, const uint32_t* a1, const uint32_t* b0,
+                       const uint32_t* b1, size_t b_count, uint32_t* out0, uint32_t* out1, void* stream)
+{
+    if (!mod || op < 0 || op > 2)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad op");
+    if (count == 0)
+        return HMPC_OK;
+    if (!a0 || !a1 || !b0 || !b1 || !out0 || !out1 || b_count == 0 || b_count > count)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_ct_binary(mod->limbs, mod->P, op, count, a0, a1, b0, b1, b_count, out0, out1, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_ciphertext_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t* a0, const uint32_t* a1,
+                           const uint32_t* c_host, uint32_t* out0, uint32_t* out1, void* stream)
+{
+    if (!mod || op < 0 || op > 2 || !c_host)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad op");
+    if (count == 0)
+        return HMPC_OK;
+    if (!a0 || !a1 || !out0 || !out1)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_ct_scalar(mod->limbs, mod->P, op, count, a0, a1, c_host, out0, out1, as_stream(stream)));
+    return HMPC_OK;
+}
+
+namespace
+{
+    int cipher_call(hmpc_context ctx, bool encrypt, int limbs, size_t count, const uint32_t* key8, const uint32_t* nonce3,
+                    uint32_t counter0, const uint32_t* in, uint32_t* out, void* stream)
+    {
+        if (!ctx || !key8 || !nonce3 || limbs < 1 || limbs > 9)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "cipher: context, key, nonce and 1..9 limbs required");
+        if (count == 0)
+            return HMPC_OK;
+        if (!in || !out || in == out)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "cipher: distinct input and output buffers required");
+        device_guard g(ctx->device);
+        HMPC_HIP(launch_cipher(limbs, encrypt, count, key8, nonce3, counter0, in, out, as_stream(stream)));
+        return HMPC_OK;
+    }
+}
+
+int hmpc_cipher_encrypt(hmpc_context ctx, int limbs, size_t count, const uint32_t* key8, const uint32_t* nonce3,
+                        uint32_t counter0, const uint32_t* in, uint32_t* out, void* stream)
+{
+    return cipher_call(ctx, true, limbs, count, key8, nonce3, counter0, in, out, stream);
+}
+
+int hmpc_cipher_decrypt(hmpc_context ctx, int limbs, size_t count, const uint32_t* key8, const uint32_t* nonce3,
+                        uint32_t counter0, const uint32_t* in, uint32_t* out, void* stream)
+{
+    return cipher_call(ctx, false, limbs, count, key8, nonce3, counter0, in, out, stream);
 }
 
 int hmpc_from_u32(hmpc_modulus mod, size_t count, const uint32_t* words, uint32_t* out, void* stream)

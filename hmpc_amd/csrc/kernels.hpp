@@ -116,6 +116,16 @@ namespace hmpc_amd
 
     hipError_t launch_pointwise(int L, mod_params const& P, int op, std::size_t count, u32 const* a, u32 const* b,
                                 std::size_t b_count, u32* out, hipStream_t s);
+    // both parts of a ciphertext in one launch (kernels_ext.hip): out_p = a_p op b_p with b
+    // broadcast with period b_count, or out_p = a_p op c for a constant c (L words)
+    hipError_t launch_ct_binary(int L, mod_params const& P, int op, std::size_t count, u32 const* a0, u32 const* a1,
+                                u32 const* b0, u32 const* b1, std::size_t b_count, u32* out0, u32* out1, hipStream_t s);
+    hipError_t launch_ct_scalar(int L, mod_params const& P, int op, std::size_t count, u32 const* a0, u32 const* a1,
+                                u32 const* c, u32* out0, u32* out1, hipStream_t s);
+    // tensor cipher: limb-planar [L][count] <-> element-major words [count][L] XOR the ChaCha20
+    // keystream (key, nonce, first block counter0)
+    hipError_t launch_cipher(int L, bool encrypt, std::size_t count, u32 const* key8, u32 const* nonce3, u32 counter0,
+                             u32 const* in, u32* out, hipStream_t s);
     hipError_t launch_pointwise_scalar(int L, mod_params const& P, int op, std::size_t count, u32 const* a,
                                        u32 const* c, u32* out, hipStream_t s);
     hipError_t launch_from_u32(int L, mod_params const& P, u32 const* r2, std::size_t count, u32 const* words, u32* out,

@@ -1,0 +1,163 @@
+// Element-wise kernels around the hot path, gfx950:
+//   - homomorphic operations on BGV ciphertexts (c0 and c1 in one launch);
+//   - the ChaCha20 tensor cipher (keystream XOR over the limbs of a tensor).
+// All are HBM-bound streaming kernels (the cipher: ChaCha-bound at one block per 16 words).
+#include "kernels.hpp"
+#include "chacha.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace hmpc_amd
+{
+    namespace
+    {
+        constexpr int ext_threads = 256;
+
+        unsigned ext_grid(std::size_t work)
+        {
+            return static_cast<unsigned>((work + ext_threads - 1) / ext_threads);
+        }
+
+        template<typename F>
+        hipError_t ext_limbs(int L, F&& f)
+        {
+            switch (L)
+            {
+            case 1: return f(std::integral_constant<int, 1>{});
+            case 2: return f(std::integral_constant<int, 2>{});
+            case 3: return f(std::integral_constant<int, 3>{});
+            case 4: return f(std::integral_constant<int, 4>{});
+            case 5: return f(std::integral_constant<int, 5>{});
+            case 6: return f(std::integral_constant<int, 6>{});
+            case 7: return f(std::integral_constant<int, 7>{});
+            case 8: return f(std::integral_constant<int, 8>{});
+            case 9: return f(std::integral_constant<int, 9>{});
+            default: return hipErrorInvalidValue;
+            }
+        }
+
+        template<int L>
+        __device__ __forceinline__ void ct_apply(int op, u32 (&r)[L], u32 const (&x)[L], u32 const (&y)[L],
+                                                 mod_params const& P)
+        {
+            if (op == 0)
+                mod_add<L>(r, x, y, P);
+            else if (op == 1)
+                mod_sub<L>(r, x, y, P);
+            else
+                mod_mul<L>(r, x, y, P);
+        }
+
+        // blockIdx.y selects the ciphertext part: out_p = a_p op b_p, b broadcast with period b_count
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_ct_binary(int op, u64 count, u32 const* __restrict__ a0,
+                                                                   u32 const* __restrict__ a1, u32 const* __restrict__ b0,
+                                                                   u32 const* __restrict__ b1, u64 b_count,
+                                                                   u32* __restrict__ out0, u32* __restrict__ out1,
+                                                                   mod_params P)
+        {
+            u64 e = u64{blockIdx.x} * blockDim.x + threadIdx.x;
+            if (e >= count)
+                return;
+            bool second = blockIdx.y != 0;
+            u64 eb = b_count == count ? e : e % b_count;
+            u32 x[L], y[L], r[L];
+            load_planar<L>(x, second ? a1 : a0, count, e);
+            load_planar<L>(y, second ? b1 : b0, b_count, eb);
+            ct_apply<L>(op, r, x, y, P);
+            store_planar<L>(second ? out1 : out0, count, e, r);
+        }
+
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_ct_scalar(int op, u64 count, u32 const* __restrict__ a0,
+                                                                   u32 const* __restrict__ a1, residue<L> c,
+                                                                   u32* __restrict__ out0, u32* __restrict__ out1,
+                                                                   mod_params P)
+        {
+            u64 e = u64{blockIdx.x} * blockDim.x + threadIdx.x;
+            if (e >= count)
+                return;
+            bool second = blockIdx.y != 0;
+            u32 x[L], r[L];
+            load_planar<L>(x, second ? a1 : a0, count, e);
+            ct_apply<L>(op, r, x, c.v, P);
+            store_planar<L>(second ? out1 : out0, count, e, r);
+        }
+
+        // Tensor cipher, one ChaCha20 block per thread: block t covers keystream words
+        // [16 t, 16 t + 16), and keystream word w masks limb w % L of element w / L
+        // (hmpc/expr/crypto/cipher.hpp:124-160: element i takes words [i L, i L + L) of one
+        // stream with counter = block index). Encryption reads the limb-planar tensor and writes
+        // element-major words; decryption the reverse.
+        template<int L, bool Enc>
+        __global__ void __launch_bounds__(ext_threads) k_cipher(u64 count, u32 const* __restrict__ in,
+                                                                u32* __restrict__ out, chacha_key key, u32 counter0,
+                                                                u32 n0, u32 n1, u32 n2)
+        {
+            u64 t = u64{blockIdx.x} * blockDim.x + threadIdx.x;
+            u64 words = count * L;
+            u64 w0 = t * 16;
+            if (w0 >= words)
+                return;
+            u32 ks[16];
+            chacha20_block(ks, key, counter0 + static_cast<u32>(t), n0, n1, n2);
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+            {
+                u64 w = w0 + k;
+                if (w >= words)
+                    break;
+                u64 i = w / L;
+                u64 j = w - i * L;
+                if constexpr (Enc)
+                    out[w] = in[j * count + i] ^ ks[k];
+                else
+                    out[j * count + i] = in[w] ^ ks[k];
+            }
+        }
+    }
+
+    hipError_t launch_ct_binary(int L, mod_params const& P, int op, std::size_t count, u32 const* a0, u32 const* a1,
+                                u32 const* b0, u32 const* b1, std::size_t b_count, u32* out0, u32* out1, hipStream_t s)
+    {
+        return ext_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            dim3 grid(ext_grid(count), 2);
+            k_ct_binary<LL><<<grid, ext_threads, 0, s>>>(op, count, a0, a1, b0, b1, b_count, out0, out1, P);
+            return hipGetLastError();
+        });
+    }
+
+    hipError_t launch_ct_scalar(int L, mod_params const& P, int op, std::size_t count, u32 const* a0, u32 const* a1,
+                                u32 const* c, u32* out0, u32* out1, hipStream_t s)
+    {
+        return ext_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            residue<LL> cv;
+            for (int l = 0; l < LL; ++l)
+                cv.v[l] = c[l];
+            dim3 grid(ext_grid(count), 2);
+            k_ct_scalar<LL><<<grid, ext_threads, 0, s>>>(op, count, a0, a1, cv, out0, out1, P);
+            return hipGetLastError();
+        });
+    }
+
+    hipError_t launch_cipher(int L, bool encrypt, std::size_t count, u32 const* key8, u32 const* nonce3, u32 counter0,
+                             u32 const* in, u32* out, hipStream_t s)
+    {
+        return ext_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            chacha_key key;
+            for (int i = 0; i < 8; ++i)
+                key.k[i] = key8[i];
+            std::size_t blocks = (count * LL + 15) / 16;
+            if (encrypt)
+                k_cipher<LL, true><<<ext_grid(blocks), ext_threads, 0, s>>>(count, in, out, key, counter0, nonce3[0],
+                                                                            nonce3[1], nonce3[2]);
+            else
+                k_cipher<LL, false><<<ext_grid(blocks), ext_threads, 0, s>>>(count, in, out, key, counter0, nonce3[0],
+                                                                             nonce3[1], nonce3[2]);
+            return hipGetLastError();
+        });
+    }
+}

@@ -35,6 +35,10 @@ SIGNATURES = {
     "hmpc_memcpy_h2d": (_i, [_vp, _vp, _vp, _sz]),
     "hmpc_memcpy_d2h": (_i, [_vp, _vp, _vp, _sz]),
     "hmpc_memcpy_d2d": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "hmpc_host_alloc": (_i, [_vp, _sz, ctypes.POINTER(_vp)]),
+    "hmpc_host_free": (_i, [_vp, _vp]),
+    "hmpc_memcpy_h2d_async": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "hmpc_memcpy_d2h_async": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "hmpc_stream_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "hmpc_stream_destroy": (_i, [_vp, _vp]),
     "hmpc_stream_synchronize": (_i, [_vp, _vp]),
@@ -54,6 +58,10 @@ SIGNATURES = {
     "hmpc_ntt_plain": (_i, [_vp, _i, _sz, _i, _vp, _vp, _vp, _vp]),
     "hmpc_pointwise": (_i, [_vp, _i, _sz, _vp, _vp, _sz, _vp, _vp]),
     "hmpc_pointwise_scalar": (_i, [_vp, _i, _sz, _vp, _u32p, _vp, _vp]),
+    "hmpc_ciphertext_op": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp]),
+    "hmpc_ciphertext_scalar": (_i, [_vp, _i, _sz, _vp, _vp, _u32p, _vp, _vp, _vp]),
+    "hmpc_cipher_encrypt": (_i, [_vp, _i, _sz, _u32p, _u32p, ctypes.c_uint32, _vp, _vp, _vp]),
+    "hmpc_cipher_decrypt": (_i, [_vp, _i, _sz, _u32p, _u32p, ctypes.c_uint32, _vp, _vp, _vp]),
     "hmpc_from_u32": (_i, [_vp, _sz, _vp, _vp, _vp]),
     "hmpc_to_unsigned": (_i, [_vp, _sz, _vp, _vp, _vp]),
     "hmpc_sample_uniform": (_i, [_vp, _u32p, _u64, _sz, _vp, _vp]),
@@ -264,6 +272,61 @@ class Context:
         check(_lib.hmpc_sample_words(self.handle, _p(k), offset, count, out.ptr, _sp(stream)))
         return out
 
+    def cipher_encrypt(self, limbs: int, count: int, key, nonce, src: Buffer, out: Buffer | None = None, counter: int = 0,
+                       stream=None) -> Buffer:
+        """limb-planar u32[limbs][count] -> element-major u32[count][limbs] XOR ChaCha20(key, nonce)"""
+        out = out or Buffer(self, 4 * limbs * count)
+        k, n = _arr(key, 8), _arr(nonce, 3)
+        check(_lib.hmpc_cipher_encrypt(self.handle, limbs, count, _p(k), _p(n), counter, src.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def cipher_decrypt(self, limbs: int, count: int, key, nonce, src: Buffer, out: Buffer | None = None, counter: int = 0,
+                       stream=None) -> Buffer:
+        """element-major u32[count][limbs] -> limb-planar u32[limbs][count]"""
+        out = out or Buffer(self, 4 * limbs * count)
+        k, n = _arr(key, 8), _arr(nonce, 3)
+        check(_lib.hmpc_cipher_decrypt(self.handle, limbs, count, _p(k), _p(n), counter, src.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def host_buffer(self, nbytes: int) -> "HostBuffer":
+        return HostBuffer(self, nbytes)
+
+
+class HostBuffer:
+    """Page-locked host memory (hmpc_host_alloc) with asynchronous copies to and from the device."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(load().hmpc_host_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def array(self, dtype=np.uint32) -> np.ndarray:
+        """a numpy view of the pinned memory (valid while this object lives)"""
+        n = self.nbytes // np.dtype(dtype).itemsize
+        buf = (ctypes.c_char * self.nbytes).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=dtype, count=n)
+
+    def to_device(self, dst: Buffer, nbytes: int | None = None, stream=None):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        check(_lib.hmpc_memcpy_h2d_async(self.ctx.handle, dst.ptr, self.ptr, n, _sp(stream)))
+
+    def from_device(self, src: Buffer, nbytes: int | None = None, stream=None):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        check(_lib.hmpc_memcpy_d2h_async(self.ctx.handle, self.ptr, src.ptr, n, _sp(stream)))
+
+    def free(self):
+        if self.ptr is not None and _lib is not None and self.ctx.handle:
+            _lib.hmpc_host_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
 
 def _sp(stream):
     return None if stream is None else stream.handle
@@ -429,6 +492,23 @@ class Modulus:
         c = np.ascontiguousarray(c_mont, dtype=np.uint32)
         check(_lib.hmpc_pointwise_scalar(self.handle, op, count, a.ptr, _p(c), out.ptr, _sp(stream)))
         return out
+
+    def ciphertext_op(self, op, count, a0: Buffer, a1: Buffer, b0: Buffer, b1: Buffer, b_count=None,
+                      out0: Buffer | None = None, out1: Buffer | None = None, stream=None):
+        """(a0 op b0, a1 op b1) for ciphertexts a, b (or a plaintext b0 = b1 broadcast with period b_count)"""
+        out0 = out0 or self.alloc(count)
+        out1 = out1 or self.alloc(count)
+        check(_lib.hmpc_ciphertext_op(self.handle, op, count, a0.ptr, a1.ptr, b0.ptr, b1.ptr,
+                                      count if b_count is None else b_count, out0.ptr, out1.ptr, _sp(stream)))
+        return out0, out1
+
+    def ciphertext_scalar(self, op, count, a0: Buffer, a1: Buffer, c_mont: np.ndarray, out0: Buffer | None = None,
+                          out1: Buffer | None = None, stream=None):
+        out0 = out0 or self.alloc(count)
+        out1 = out1 or self.alloc(count)
+        c = np.ascontiguousarray(c_mont, dtype=np.uint32)
+        check(_lib.hmpc_ciphertext_scalar(self.handle, op, count, a0.ptr, a1.ptr, _p(c), out0.ptr, out1.ptr, _sp(stream)))
+        return out0, out1
 
     def from_u32(self, count, words: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
         out = out or self.alloc(count)

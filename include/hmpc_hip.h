@@ -76,6 +76,15 @@ int hmpc_free(hmpc_context ctx, void* ptr);
 int hmpc_memcpy_h2d(hmpc_context ctx, void* dst, const void* src, size_t bytes);
 int hmpc_memcpy_d2h(hmpc_context ctx, void* dst, const void* src, size_t bytes);
 int hmpc_memcpy_d2d(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* Pinned host staging for the wire format (hmpc-rs sends limb-planar Montgomery bytes,
+ * hmpc/net/queue.hpp:79-80, hmpc/net/structure.hpp:292-300): page-locked host memory and
+ * asynchronous copies on a stream, so ciphertexts go to and from the network buffers without a
+ * relayout or a pageable bounce. The async copies return at once; synchronize the stream (or
+ * record an event) before the host touches the buffer. */
+int hmpc_host_alloc(hmpc_context ctx, size_t bytes, void** out);
+int hmpc_host_free(hmpc_context ctx, void* ptr);
+int hmpc_memcpy_h2d_async(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream);
+int hmpc_memcpy_d2h_async(hmpc_context ctx, void* dst, const void* src, size_t bytes, void* stream);
 int hmpc_stream_create(hmpc_context ctx, void** stream);
 int hmpc_stream_destroy(hmpc_context ctx, void* stream);
 int hmpc_stream_synchronize(hmpc_context ctx, void* stream);
@@ -156,10 +165,33 @@ int hmpc_pointwise(hmpc_modulus mod, int op, size_t count, const uint32_t* a, co
  * (hmpc/expr/value.hpp, hmpc/expr/constant.hpp). */
 int hmpc_pointwise_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t* a, const uint32_t* c_host,
                           uint32_t* out, void* stream);
+/* Homomorphic operations on BGV ciphertexts (c0, c1) of `count` elements in NTT form, both
+ * parts in one launch: ciphertext_expression operator+ / operator- and operator* with an
+ * expression (hmpc/expr/crypto/lhe/ciphertext.hpp:48-81). hmpc_ciphertext_op: out_p = a_p op b_p
+ * (op = HMPC_ADD / HMPC_SUB with another ciphertext b; HMPC_MUL with b0 = b1 = a plaintext
+ * polynomial in NTT form, broadcast with period b_count); hmpc_ciphertext_scalar: out_p = a_p op c
+ * for a Montgomery constant c (L host words). Outputs may alias inputs. */
+int hmpc_ciphertext_op(hmpc_modulus mod, int op, size_t count, const uint32_t* a0, const uint32_t* a1, const uint32_t* b0,
+                       const uint32_t* b1, size_t b_count, uint32_t* out0, uint32_t* out1, void* stream);
+int hmpc_ciphertext_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t* a0, const uint32_t* a1,
+                           const uint32_t* c_host, uint32_t* out0, uint32_t* out1, void* stream);
 /* unsigned 32-bit words -> mod q (hmpc/ints/mod.hpp:165-173, the lift used by examples/bgv.cpp:66-69) */
 int hmpc_from_u32(hmpc_modulus mod, size_t count, const uint32_t* words, uint32_t* out, void* stream);
 /* Montgomery -> canonical integers (hmpc/ints/mod.hpp:356-361), limb-planar in and out */
 int hmpc_to_unsigned(hmpc_modulus mod, size_t count, const uint32_t* in, uint32_t* out, void* stream);
+
+/* ------------------------------------------------------------------ tensor cipher
+ * hmpc::expr::crypto::enc / dec with cipher(key, nonce) (hmpc/expr/crypto/cipher.hpp:63-318,
+ * hmpc/crypto/cipher.hpp:30-45): element i of a tensor of `limbs`-word values takes words
+ * [i limbs, (i + 1) limbs) of one ChaCha20 keystream (key8, nonce3, block counter starting at
+ * counter0; hmpc's expressions use 0) and is XORed with them limb by limb. Encryption reads the
+ * limb-planar tensor u32[limbs][count] and writes the element-major word tensor
+ * u32[count][limbs] (hmpc's result shape {..., limbs}); decryption the reverse. The block
+ * counter is 32 bits wide and wraps, as in hmpc. */
+int hmpc_cipher_encrypt(hmpc_context ctx, int limbs, size_t count, const uint32_t* key8, const uint32_t* nonce3,
+                        uint32_t counter0, const uint32_t* in, uint32_t* out, void* stream);
+int hmpc_cipher_decrypt(hmpc_context ctx, int limbs, size_t count, const uint32_t* key8, const uint32_t* nonce3,
+                        uint32_t counter0, const uint32_t* in, uint32_t* out, void* stream);
 
 /* ------------------------------------------------------------------ sampling
  * Explicit-key streams (hmpc/expr/random/number_generator.hpp:31-74): element e of a tensor

@@ -59,6 +59,8 @@ def lib():
         L.oracle_roots.argtypes = [vp, i, i, _u32p]
         L.oracle_chacha20_block.argtypes = [_u32p, ctypes.c_uint32, _u32p, _u32p]
         L.oracle_chacha20_stream.argtypes = [_u32p, _u32p, ctypes.c_uint32, sz, _u32p]
+        L.oracle_cipher_encrypt.argtypes = [i, sz, _u32p, _u32p, ctypes.c_uint32, _u32p, _u32p]
+        L.oracle_cipher_decrypt.argtypes = [i, sz, _u32p, _u32p, ctypes.c_uint32, _u32p, _u32p]
         L.oracle_ntt.argtypes = [vp, i, sz, i, _u32p, _u32p, i]
         L.oracle_pointwise.argtypes = [vp, i, sz, _u32p, _u32p, sz, _u32p, i]
         L.oracle_sample_uniform.argtypes = [vp, _u32p, u64, sz, _u32p, i]
@@ -295,6 +297,26 @@ def chacha20_stream(key, nonce, counter: int, words: int) -> np.ndarray:
     nz = np.array(nonce, dtype=np.uint32)
     out = np.empty(words, np.uint32)
     lib().oracle_chacha20_stream(_p(k), _p(nz), counter, words, _p(out))
+    return out
+
+
+def cipher_encrypt(planar: np.ndarray, key, nonce, counter: int = 0) -> np.ndarray:
+    """hmpc's tensor cipher: limb-planar u32[L][count] -> element-major u32[count][L]"""
+    planar = np.ascontiguousarray(planar, dtype=np.uint32)
+    limbs, count = planar.shape
+    k, nz = key_array(key), np.array(list(nonce) + [0] * (3 - len(nonce)), dtype=np.uint32)
+    out = np.empty((count, limbs), np.uint32)
+    lib().oracle_cipher_encrypt(limbs, count, _p(k), _p(nz), counter, _p(planar), _p(out))
+    return out
+
+
+def cipher_decrypt(words: np.ndarray, key, nonce, counter: int = 0) -> np.ndarray:
+    """element-major u32[count][L] -> limb-planar u32[L][count]"""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    count, limbs = words.shape
+    k, nz = key_array(key), np.array(list(nonce) + [0] * (3 - len(nonce)), dtype=np.uint32)
+    out = np.empty((limbs, count), np.uint32)
+    lib().oracle_cipher_decrypt(limbs, count, _p(k), _p(nz), counter, _p(words), _p(out))
     return out
 
 

@@ -22,6 +22,9 @@
 //                             hmpc/expr/number_theoretic_transform.hpp:320-333
 //   bgv encrypt / decrypt     decrypt round trip (tests/expr/crypto/lhe/enc.cpp:65-72 shape),
 //                             linear-algebra recomputation in python big ints
+//   cipher_encrypt / decrypt  RFC 8439 section 2.4.2 vector held in tests/crypto/cipher.cpp:8-36,
+//                             the round trip of tests/expr/crypto/cipher.cpp, and the continuous
+//                             keystream of chacha20_stream (one stream across hmpc's batches)
 
 #include <algorithm>
 #include <atomic>
@@ -836,6 +839,62 @@ extern "C"
         number_generator g(key, nonce, counter);
         for (size_t i = 0; i < words; ++i)
             out[i] = g.next();
+    }
+
+    // Tensor cipher, hmpc/expr/crypto/cipher.hpp:121-161 (enc) and :253-300 (dec) over
+    // hmpc/crypto/cipher.hpp:30-45: work item b owns batch_size = lcm(16, L) words, i.e.
+    // elements_per_batch = batch_size / L elements, and a generator started at block counter
+    // counter0 + b * (batch_size / 16); each element takes the next L words as its mask and is
+    // XORed with it. `planar` is u32[L][count], `words` u32[count][L].
+    static size_t gcd_size(size_t a, size_t b)
+    {
+        while (b)
+        {
+            size_t t = a % b;
+            a = b;
+            b = t;
+        }
+        return a;
+    }
+
+    static void cipher_batches(int limbs, size_t count, u32 const* key, u32 const* nonce, u32 counter0, bool encrypt,
+                               u32 const* in, u32* out)
+    {
+        size_t L = static_cast<size_t>(limbs);
+        size_t batch_size = 16 / gcd_size(16, L) * L;
+        size_t per_batch = batch_size / L;
+        size_t blocks_per_batch = batch_size / 16;
+        size_t batches = (count * L + batch_size - 1) / batch_size;
+        for (size_t b = 0; b < batches; ++b)
+        {
+            number_generator g(key, nonce, static_cast<u32>(counter0 + b * blocks_per_batch));
+            for (size_t e = 0; e < per_batch; ++e)
+            {
+                size_t i = b * per_batch + e;
+                if (i >= count)
+                    break;
+                for (size_t j = 0; j < L; ++j)
+                {
+                    u32 mask = g.next();
+                    if (encrypt)
+                        out[i * L + j] = in[j * count + i] ^ mask;
+                    else
+                        out[j * count + i] = in[i * L + j] ^ mask;
+                }
+            }
+        }
+    }
+
+    void oracle_cipher_encrypt(int limbs, size_t count, u32 const* key, u32 const* nonce, u32 counter0, u32 const* planar,
+                               u32* words)
+    {
+        cipher_batches(limbs, count, key, nonce, counter0, true, planar, words);
+    }
+
+    void oracle_cipher_decrypt(int limbs, size_t count, u32 const* key, u32 const* nonce, u32 counter0, u32 const* words,
+                               u32* planar)
+    {
+        cipher_batches(limbs, count, key, nonce, counter0, false, words, planar);
     }
 
     // ------------------------------------------------------------------ NTT

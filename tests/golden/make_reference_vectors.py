@@ -6,6 +6,8 @@ Sources (read as text, nothing is compiled or run):
   tests/ints/mod.cpp:12-40             Montgomery constants for the 150-bit prime p
   tests/ints/poly_mod.cpp:78-96        root / inverse root and 5 NTT outputs for x_i = i, N = 2^14
   tests/ints/num/theory/root_of_unity.cpp:8-9   p = 1601, n = 64
+  tests/crypto/cipher.cpp:8-36         RFC 8439 section 2.4.2 encryption (key, nonce, counter 1, 114 bytes)
+  tests/expr/crypto/cipher.cpp:8-30    tensor cipher inputs: ubigint<127> x_i = i, shape {4, 2}, key {42}, nonce {1}
 
 Usage: python tests/golden/make_reference_vectors.py [/root/reference]
 """
@@ -62,6 +64,26 @@ def chacha_vectors():
     }
 
 
+def cipher_vectors():
+    src = open(os.path.join(ROOT, "tests/crypto/cipher.cpp")).read()
+    key = hexes(section(src, "key[] =", ";"))
+    nonce = hexes(section(src, "nonce[] =", ";"))
+    counter = hexes(section(src, "counter[] =", ";"))
+    plain = hexes(section(src, "plaintext_bytes[] =", "};"))
+    cipher = hexes(section(src, "expected_ciphertext_bytes[] =", "};"))
+    assert len(key) == 8 and len(nonce) == 3 and len(counter) == 1 and len(plain) == 114 and len(cipher) == 114
+    tsrc = open(os.path.join(ROOT, "tests/expr/crypto/cipher.cpp")).read()
+    bits = int(re.search(r"ubigint<(\d+)>", tsrc).group(1))
+    shape = [int(v) for v in re.search(r"hmpc::shape\{(\d+), (\d+)\}", tsrc).groups()]
+    tkey = int(re.search(r"key = \{(\d+)\}", tsrc).group(1))
+    tnonce = int(re.search(r"nonce = \{(\d+)\}", tsrc).group(1))
+    return {
+        "rfc8439_2_4_2": {"key": key, "nonce": nonce, "counter": counter[0], "plaintext": plain, "ciphertext": cipher},
+        "tensor": {"bits": bits, "limbs": (bits + 31) // 32, "shape": shape, "key": [tkey] + [0] * 7,
+                   "nonce": [tnonce, 0, 0], "values": "x_i = i"},
+    }
+
+
 def mod_vectors():
     src = open(os.path.join(ROOT, "tests/ints/mod.cpp")).read()
     p = int(re.search(r"constexpr auto p = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
@@ -97,6 +119,7 @@ def main():
         "mod": mod_vectors(),
         "poly_mod": poly_vectors(),
         "root_of_unity": root_vectors(),
+        "cipher": cipher_vectors(),
     }
 
     def hexify(o):

@@ -359,3 +359,75 @@ def test_bgv_encrypt_decrypt_roundtrip(q, log_n):
     tcol = oracle.ints_to_planar([t_mont] * (n * N), m.limbs)
     want = m.pointwise("add", m.pointwise("mul", Bb, u), m.pointwise("mul", w, tcol))
     assert np.array_equal(c1, want)
+
+
+# ---------------------------------------------------------------- tensor cipher
+def _words_of(byte_list):
+    b = bytes(byte_list)
+    return np.frombuffer(b[: len(b) // 4 * 4], dtype="<u4").astype(np.uint32)
+
+
+def test_cipher_rfc8439_encryption_vector():
+    """RFC 8439 section 2.4.2 (tests/crypto/cipher.cpp:8-36): the first 112 bytes = 28 words,
+    as 7 elements of 4 limbs (element-major words are the byte stream), counter 1."""
+    v = VEC["cipher"]["rfc8439_2_4_2"]
+    plain, want = _words_of(v["plaintext"]), _words_of(v["ciphertext"])
+    planar = plain.reshape(7, 4).T.copy()
+    got = oracle.cipher_encrypt(planar, v["key"], v["nonce"], v["counter"])
+    assert np.array_equal(got.reshape(-1), want)
+    assert np.array_equal(oracle.cipher_decrypt(got, v["key"], v["nonce"], v["counter"]), planar)
+    # 2 limbs per element: the same byte stream, another batch structure (lcm(16, 2) = 16)
+    got2 = oracle.cipher_encrypt(plain.reshape(14, 2).T.copy(), v["key"], v["nonce"], v["counter"])
+    assert np.array_equal(got2.reshape(-1), want)
+
+
+@pytest.mark.parametrize("limbs", [1, 3, 4, 5, 7])
+def test_cipher_is_one_keystream_across_batches(limbs):
+    """hmpc's per-batch generators (lcm(16, L) words each) add up to one continuous keystream:
+    word w of the element-major output is input ^ stream[w], for every limb count."""
+    rng = np.random.default_rng(limbs)
+    count = 37
+    planar = rng.integers(0, 1 << 32, size=(limbs, count), dtype=np.uint64).astype(np.uint32)
+    key, nonce = [7, 1, 2, 3, 4, 5, 6, 8], [9, 10, 11]
+    got = oracle.cipher_encrypt(planar, key, nonce)
+    stream = oracle.chacha20_stream(key, nonce, 0, count * limbs)
+    assert np.array_equal(got.reshape(-1), planar.T.reshape(-1) ^ stream)
+
+
+def test_cipher_tensor_roundtrip_reference_inputs():
+    """tests/expr/crypto/cipher.cpp: ubigint<127> x_i = i over shape {4, 2}, key {42}, nonce {1}:
+    decryption returns the inputs and the ciphertext has shape {4, 2, 4}."""
+    t = VEC["cipher"]["tensor"]
+    count = t["shape"][0] * t["shape"][1]
+    planar = np.zeros((t["limbs"], count), np.uint32)
+    planar[0] = np.arange(count, dtype=np.uint32)
+    ct = oracle.cipher_encrypt(planar, t["key"], t["nonce"])
+    assert ct.shape == (count, t["limbs"])
+    assert not np.array_equal(ct.T, planar)
+    assert np.array_equal(oracle.cipher_decrypt(ct, t["key"], t["nonce"]), planar)
+
+
+def test_ciphertext_ops_under_decryption_oracle():
+    """hmpc's ciphertext operator+ / operator- / operator* (expr/crypto/lhe/ciphertext.hpp:48-81)
+    restated with the oracle's point-wise ops: decryption gives m_a + m_b, m_a - m_b and k m_a
+    mod t = 2^32."""
+    om = oracle.Modulus(Q128A)
+    log_n, N = 10, 1 << 10
+    t_mont = om.from_unsigned(1 << 32)
+    key = [42, 0, 0, 0, 0, 0, 0, 0]
+    B = om.sample_uniform(key, 10, N)
+    s = om.ntt(om.sample_cbd(key, 11, 2, N), log_n)
+    e = om.ntt(om.sample_cbd(key, 12, 40, N), log_n)
+    A = om.pointwise("add", om.pointwise("mul", B, s), om.pointwise("mul", e, oracle.ints_to_planar([t_mont] * N, om.limbs)))
+    ma, mb = oracle.sample_words(key, 13, N), oracle.sample_words(key, 14, N)
+    enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
+    a0, a1, nonce = om.bgv_encrypt(log_n, A, B, ma, t_mont, enc_key, [0, 0, 0])
+    b0, b1, _ = om.bgv_encrypt(log_n, A, B, mb, t_mont, enc_key, nonce)
+    m64a, m64b = ma.astype(np.int64), mb.astype(np.int64)
+    add = om.bgv_decrypt(log_n, s, om.pointwise("add", a0, b0), om.pointwise("add", a1, b1))
+    assert np.array_equal(add, ((m64a + m64b) % (1 << 32)).astype(np.uint32))
+    sub = om.bgv_decrypt(log_n, s, om.pointwise("sub", a0, b0), om.pointwise("sub", a1, b1))
+    assert np.array_equal(sub, ((m64a - m64b) % (1 << 32)).astype(np.uint32))
+    k = oracle.ints_to_planar([om.from_unsigned(7)], om.limbs)
+    mul = om.bgv_decrypt(log_n, s, om.pointwise("mul", a0, k), om.pointwise("mul", a1, k))
+    assert np.array_equal(mul, ((m64a * 7) % (1 << 32)).astype(np.uint32))
