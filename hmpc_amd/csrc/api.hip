@@ -1075,6 +1075,54 @@ int hmpc_ciphertext_scalar(hmpc_modulus mod, int op, size_t count, const uint32_
     return HMPC_OK;
 }
 
+int hmpc_fma(hmpc_modulus mod, size_t count, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out,
+             void* stream)
+{
+    if (!mod)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null modulus");
+    if (count == 0)
+        return HMPC_OK;
+    if (!a || !b || !c || !out)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    HMPC_HIP(launch_fma(mod->limbs, mod->P, count, a, b, c, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_reduce(hmpc_modulus mod, int op, size_t count, const uint32_t* in, uint32_t* out, void* stream)
+{
+    if (!mod || (op != HMPC_ADD && op != HMPC_MUL))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "reduce: HMPC_ADD or HMPC_MUL");
+    if (!out || (count && !in))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    hipStream_t s = as_stream(stream);
+    void* w = nullptr;
+    int rc = mod->ctx->workspace(s, static_cast<std::size_t>(reduce_parts(count)) * mod->limbs * 4, &w);
+    if (rc)
+        return rc;
+    HMPC_HIP(launch_reduce(mod->limbs, mod->P, mod->one, op == HMPC_ADD ? 0 : 1, count, in, static_cast<u32*>(w), out, s));
+    return HMPC_OK;
+}
+
+int hmpc_matvec(hmpc_modulus mod, size_t batch, size_t rows, size_t cols, const uint32_t* M, int broadcast_matrix,
+                const uint32_t* v, int broadcast_vector, uint32_t* out, void* stream)
+{
+    if (!mod)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null modulus");
+    if (batch == 0 || rows == 0)
+        return HMPC_OK;
+    if (!out || (cols && (!M || !v)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad operands");
+    device_guard g(mod->ctx->device);
+    std::size_t m_batches = broadcast_matrix ? 1 : batch;
+    std::size_t v_batches = broadcast_vector ? 1 : batch;
+    HMPC_HIP(launch_matvec(mod->limbs, mod->P, batch, rows, cols, M, m_batches * rows * cols,
+                           broadcast_matrix ? 0 : rows * cols, v, v_batches * cols, broadcast_vector ? 0 : cols, out,
+                           as_stream(stream)));
+    return HMPC_OK;
+}
+
 namespace
 {
     int cipher_call(hmpc_context ctx, bool encrypt, int limbs, size_t count, const uint32_t* key8, const uint32_t* nonce3,

@@ -122,6 +122,18 @@ namespace hmpc_amd
                                 u32 const* b0, u32 const* b1, std::size_t b_count, u32* out0, u32* out1, hipStream_t s);
     hipError_t launch_ct_scalar(int L, mod_params const& P, int op, std::size_t count, u32 const* a0, u32 const* a1,
                                 u32 const* c, u32* out0, u32* out1, hipStream_t s);
+    // out = a b + c
+    hipError_t launch_fma(int L, mod_params const& P, std::size_t count, u32 const* a, u32 const* b, u32 const* c, u32* out,
+                          hipStream_t s);
+    // full reduction (op 0: sum, 1: product; one = R mod q) of `count` elements to out (L words);
+    // partials holds reduce_parts(count) x L words
+    unsigned reduce_parts(std::size_t count);
+    hipError_t launch_reduce(int L, mod_params const& P, u32 const* one, int op, std::size_t count, u32 const* in,
+                             u32* partials, u32* out, hipStream_t s);
+    // out[b][r] = sum_c M[b m_stride + r cols + c] v[b v_stride + c] (strides 0 broadcast)
+    hipError_t launch_matvec(int L, mod_params const& P, std::size_t batch, std::size_t rows, std::size_t cols,
+                             u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v, std::size_t v_count,
+                             std::size_t v_stride, u32* out, hipStream_t s);
     // tensor cipher: limb-planar [L][count] <-> element-major words [count][L] XOR the ChaCha20
     // keystream (key, nonce, first block counter0)
     hipError_t launch_cipher(int L, bool encrypt, std::size_t count, u32 const* key8, u32 const* nonce3, u32 counter0,

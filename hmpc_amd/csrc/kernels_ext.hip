@@ -1,6 +1,7 @@
 // Element-wise kernels around the hot path, gfx950:
 //   - homomorphic operations on BGV ciphertexts (c0 and c1 in one launch);
-//   - the ChaCha20 tensor cipher (keystream XOR over the limbs of a tensor).
+//   - the ChaCha20 tensor cipher (keystream XOR over the limbs of a tensor);
+//   - fused multiply-add, full reductions (sum, product) and matrix-vector products over mod q.
 // All are HBM-bound streaming kernels (the cipher: ChaCha-bound at one block per 16 words).
 #include "kernels.hpp"
 #include "chacha.hpp"
@@ -115,6 +116,195 @@ namespace hmpc_amd
                     out[j * count + i] = in[w] ^ ks[k];
             }
         }
+    }
+
+    namespace
+    {
+        // out = a b + c (examples/fma.cpp:56-67: x[i] = a[i] * b[i] + c[i])
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_fma(u64 count, u32 const* __restrict__ a, u32 const* __restrict__ b,
+                                                             u32 const* __restrict__ c, u32* __restrict__ out, mod_params P)
+        {
+            u64 e = u64{blockIdx.x} * blockDim.x + threadIdx.x;
+            if (e >= count)
+                return;
+            u32 x[L], y[L], z[L], t[L], r[L];
+            load_planar<L>(x, a, count, e);
+            load_planar<L>(y, b, count, e);
+            load_planar<L>(z, c, count, e);
+            mod_mul<L>(t, x, y, P);
+            mod_add<L>(r, t, z, P);
+            store_planar<L>(out, count, e, r);
+        }
+
+        template<int L>
+        __device__ __forceinline__ void combine(int op, u32 (&acc)[L], u32 const (&v)[L], mod_params const& P)
+        {
+            u32 r[L];
+            if (op == 0)
+                mod_add<L>(r, acc, v, P);
+            else
+                mod_mul<L>(r, acc, v, P);
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+                acc[l] = r[l];
+        }
+
+        template<int L>
+        __device__ __forceinline__ void identity(int op, u32 (&acc)[L], u32 const (&one)[L])
+        {
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+                acc[l] = op == 0 ? 0u : one[l];
+        }
+
+        // workgroup tree reduction through LDS (ext_threads x L words); result in thread 0
+        template<int L>
+        __device__ __forceinline__ void block_reduce(int op, u32 (&acc)[L], u32* lds, mod_params const& P)
+        {
+            int t = threadIdx.x;
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+                lds[l * ext_threads + t] = acc[l];
+            __syncthreads();
+            for (int h = ext_threads / 2; h > 0; h >>= 1)
+            {
+                if (t < h)
+                {
+                    u32 v[L];
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        v[l] = lds[l * ext_threads + t + h];
+                    combine<L>(op, acc, v, P);
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        lds[l * ext_threads + t] = acc[l];
+                }
+                __syncthreads();
+            }
+        }
+
+        // pass 1: grid-stride partial reductions, one partial per workgroup into partials[L][grid];
+        // pass 2 (one workgroup): the partials to out (L words). Sums and products mod q are
+        // associative and commutative, so the result does not depend on the order.
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_reduce(int op, u64 count, u32 const* __restrict__ in,
+                                                                u32* __restrict__ partials, residue<L> one, mod_params P)
+        {
+            __shared__ u32 lds[L * ext_threads];
+            u32 acc[L];
+            identity<L>(op, acc, one.v);
+            for (u64 e = u64{blockIdx.x} * blockDim.x + threadIdx.x; e < count; e += u64{gridDim.x} * blockDim.x)
+            {
+                u32 v[L];
+                load_planar<L>(v, in, count, e);
+                combine<L>(op, acc, v, P);
+            }
+            block_reduce<L>(op, acc, lds, P);
+            if (threadIdx.x == 0)
+                store_planar<L>(partials, gridDim.x, blockIdx.x, acc);
+        }
+
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_reduce_final(int op, u32 parts, u32 const* __restrict__ partials,
+                                                                      u32* __restrict__ out, residue<L> one, mod_params P)
+        {
+            __shared__ u32 lds[L * ext_threads];
+            u32 acc[L];
+            identity<L>(op, acc, one.v);
+            for (u32 e = threadIdx.x; e < parts; e += ext_threads)
+            {
+                u32 v[L];
+                load_planar<L>(v, partials, parts, e);
+                combine<L>(op, acc, v, P);
+            }
+            block_reduce<L>(op, acc, lds, P);
+            if (threadIdx.x == 0)
+                store_planar<L>(out, 1, 0, acc);
+        }
+
+        // out[b][r] = sum_c M[b][r][c] v[b][c] (hmpc/expr/matrix_vector_product.hpp:92-133), one wave
+        // per output row: lanes stride over the columns (coalesced per limb plane), then a
+        // cross-lane tree. Matrix and vector strides of 0 broadcast them over the batch.
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_matvec(u64 batch, u64 rows, u64 cols, u32 const* __restrict__ M,
+                                                                u64 m_count, u64 m_stride, u32 const* __restrict__ v,
+                                                                u64 v_count, u64 v_stride, u32* __restrict__ out, mod_params P)
+        {
+            constexpr int waves = ext_threads / 64;
+            u64 row = u64{blockIdx.x} * waves + threadIdx.x / 64;
+            int lane = threadIdx.x & 63;
+            if (row >= batch * rows)
+                return;
+            u64 b = row / rows;
+            u64 r = row - b * rows;
+            u64 m0 = b * m_stride + r * cols;
+            u64 v0 = b * v_stride;
+            u32 acc[L] = {};
+            for (u64 c = lane; c < cols; c += 64)
+            {
+                u32 x[L], y[L], t[L];
+                load_planar<L>(x, M, m_count, m0 + c);
+                load_planar<L>(y, v, v_count, v0 + c);
+                mod_mul<L>(t, x, y, P);
+                combine<L>(0, acc, t, P);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+            {
+                u32 o[L];
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    o[l] = static_cast<u32>(__shfl_xor(static_cast<int>(acc[l]), off, 64));
+                combine<L>(0, acc, o, P);
+            }
+            if (lane == 0)
+                store_planar<L>(out, batch * rows, row, acc);
+        }
+    }
+
+    hipError_t launch_fma(int L, mod_params const& P, std::size_t count, u32 const* a, u32 const* b, u32 const* c, u32* out,
+                          hipStream_t s)
+    {
+        return ext_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            k_fma<LL><<<ext_grid(count), ext_threads, 0, s>>>(count, a, b, c, out, P);
+            return hipGetLastError();
+        });
+    }
+
+    unsigned reduce_parts(std::size_t count)
+    {
+        std::size_t g = (count + ext_threads - 1) / ext_threads;
+        return static_cast<unsigned>(g < 1 ? 1 : (g > 1024 ? 1024 : g));
+    }
+
+    hipError_t launch_reduce(int L, mod_params const& P, u32 const* one, int op, std::size_t count, u32 const* in,
+                             u32* partials, u32* out, hipStream_t s)
+    {
+        return ext_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            residue<LL> o;
+            for (int l = 0; l < LL; ++l)
+                o.v[l] = one[l];
+            unsigned parts = reduce_parts(count);
+            k_reduce<LL><<<parts, ext_threads, 0, s>>>(op, count, in, partials, o, P);
+            k_reduce_final<LL><<<1, ext_threads, 0, s>>>(op, parts, partials, out, o, P);
+            return hipGetLastError();
+        });
+    }
+
+    hipError_t launch_matvec(int L, mod_params const& P, std::size_t batch, std::size_t rows, std::size_t cols,
+                             u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v, std::size_t v_count,
+                             std::size_t v_stride, u32* out, hipStream_t s)
+    {
+        return ext_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            std::size_t waves = ext_threads / 64;
+            unsigned grid = static_cast<unsigned>((batch * rows + waves - 1) / waves);
+            k_matvec<LL><<<grid, ext_threads, 0, s>>>(batch, rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, P);
+            return hipGetLastError();
+        });
     }
 
     hipError_t launch_ct_binary(int L, mod_params const& P, int op, std::size_t count, u32 const* a0, u32 const* a1,

@@ -60,6 +60,9 @@ SIGNATURES = {
     "hmpc_pointwise_scalar": (_i, [_vp, _i, _sz, _vp, _u32p, _vp, _vp]),
     "hmpc_ciphertext_op": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "hmpc_ciphertext_scalar": (_i, [_vp, _i, _sz, _vp, _vp, _u32p, _vp, _vp, _vp]),
+    "hmpc_fma": (_i, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "hmpc_reduce": (_i, [_vp, _i, _sz, _vp, _vp, _vp]),
+    "hmpc_matvec": (_i, [_vp, _sz, _sz, _sz, _vp, _i, _vp, _i, _vp, _vp]),
     "hmpc_cipher_encrypt": (_i, [_vp, _i, _sz, _u32p, _u32p, ctypes.c_uint32, _vp, _vp, _vp]),
     "hmpc_cipher_decrypt": (_i, [_vp, _i, _sz, _u32p, _u32p, ctypes.c_uint32, _vp, _vp, _vp]),
     "hmpc_from_u32": (_i, [_vp, _sz, _vp, _vp, _vp]),
@@ -509,6 +512,24 @@ class Modulus:
         c = np.ascontiguousarray(c_mont, dtype=np.uint32)
         check(_lib.hmpc_ciphertext_scalar(self.handle, op, count, a0.ptr, a1.ptr, _p(c), out0.ptr, out1.ptr, _sp(stream)))
         return out0, out1
+
+    def fma(self, count, a: Buffer, b: Buffer, c: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(count)
+        check(_lib.hmpc_fma(self.handle, count, a.ptr, b.ptr, c.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def reduce(self, op, count, x: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        """sum (op ADD) or product (op MUL) of `count` elements, as one element"""
+        out = out or self.alloc(1)
+        check(_lib.hmpc_reduce(self.handle, op, count, x.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def matvec(self, batch, rows, cols, M: Buffer, v: Buffer, broadcast_matrix=False, broadcast_vector=False,
+               out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or self.alloc(batch * rows)
+        check(_lib.hmpc_matvec(self.handle, batch, rows, cols, M.ptr, int(broadcast_matrix), v.ptr, int(broadcast_vector),
+                               out.ptr, _sp(stream)))
+        return out
 
     def from_u32(self, count, words: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
         out = out or self.alloc(count)

@@ -175,6 +175,19 @@ int hmpc_ciphertext_op(hmpc_modulus mod, int op, size_t count, const uint32_t* a
                        const uint32_t* b1, size_t b_count, uint32_t* out0, uint32_t* out1, void* stream);
 int hmpc_ciphertext_scalar(hmpc_modulus mod, int op, size_t count, const uint32_t* a0, const uint32_t* a1,
                            const uint32_t* c_host, uint32_t* out0, uint32_t* out1, void* stream);
+/* out = a * b + c element-wise, fused (examples/fma.cpp:56-67) */
+int hmpc_fma(hmpc_modulus mod, size_t count, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out,
+             void* stream);
+/* Full reduction of `count` elements to one (out: L device words): op = HMPC_ADD (hmpc::expr::sum)
+ * or HMPC_MUL (hmpc::expr::product), hmpc/expr/reduce.hpp:229-274 with sycl::plus / multiplies
+ * over mod q. The result is exact whatever the reduction order. */
+int hmpc_reduce(hmpc_modulus mod, int op, size_t count, const uint32_t* in, uint32_t* out, void* stream);
+/* Matrix-vector product over mod q (hmpc/expr/matrix_vector_product.hpp:92-133): for b < batch,
+ * out[b][r] = sum_c M[b][r][c] * v[b][c], with M row-major [batch][rows][cols] and v [batch][cols]
+ * (limb-planar); broadcast_matrix / broadcast_vector = 1 uses one matrix / vector for every b
+ * (hmpc's common_extent broadcast). out is [batch][rows]. */
+int hmpc_matvec(hmpc_modulus mod, size_t batch, size_t rows, size_t cols, const uint32_t* M, int broadcast_matrix,
+                const uint32_t* v, int broadcast_vector, uint32_t* out, void* stream);
 /* unsigned 32-bit words -> mod q (hmpc/ints/mod.hpp:165-173, the lift used by examples/bgv.cpp:66-69) */
 int hmpc_from_u32(hmpc_modulus mod, size_t count, const uint32_t* words, uint32_t* out, void* stream);
 /* Montgomery -> canonical integers (hmpc/ints/mod.hpp:356-361), limb-planar in and out */

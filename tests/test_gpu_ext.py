@@ -4,7 +4,8 @@
   the RFC 8439 section 2.4.2 vector and hmpc's own round-trip test inputs;
 - homomorphic operations on BGV ciphertexts (sum, difference, scalar and plaintext products)
   against the oracle's point-wise ops, and the properties decryption fixes for them;
-- pinned host staging with asynchronous copies (the wire-format bridge).
+- pinned host staging with asynchronous copies (the wire-format bridge);
+- fused multiply-add, sum / product reductions and matrix-vector products over mod q.
 """
 
 import json
@@ -172,3 +173,45 @@ def test_pinned_host_staging_roundtrip(ctx):
     assert np.array_equal(h2.array(), data)
     h.free()
     h2.free()
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A])
+def test_fma_reduce_matvec_match_oracle(ctx, q):
+    """fused a b + c against the oracle's point-wise ops; sum and product reductions and the
+    matrix-vector product against Python big-integer arithmetic on the Montgomery values
+    (a R + b R = (a + b) R; mont(a R, b R) = a b R)"""
+    gm, om = hmpc_amd.Modulus(ctx, q), oracle.Modulus(q)
+    say(f"fma / reduce / matvec q={hex(q)[:12]}")
+    R = 1 << (32 * gm.limbs)
+    Rinv = pow(R, -1, q)
+    count = 100003
+    a, b, c = (om.sample_uniform(KEY, 20 + i, count) for i in range(3))
+    got = gm.download(gm.fma(count, gm.upload(a), gm.upload(b), gm.upload(c)), count)
+    assert np.array_equal(got, om.pointwise("add", om.pointwise("mul", a, b), c))
+    xs = oracle.planar_to_ints(a)
+    for n in (1, 255, 256, 257, count):
+        s = oracle.planar_to_ints(gm.download(gm.reduce(gm.ADD, n, gm.upload(a[:, :n].copy())), 1))[0]
+        assert s == sum(xs[:n]) % q, n
+    p_want = R % q
+    for x in xs[:5000]:
+        p_want = p_want * x * Rinv % q
+    p = oracle.planar_to_ints(gm.download(gm.reduce(gm.MUL, 5000, gm.upload(a[:, :5000].copy())), 1))[0]
+    assert p == p_want
+    z = oracle.planar_to_ints(gm.download(gm.reduce(gm.ADD, 0, gm.upload(a[:, :1].copy())), 1))[0]
+    assert z == 0
+    # matvec: batch 3 of 17 x 70 matrices, then one matrix broadcast over 3 vectors
+    batch, rows, cols = 3, 17, 70
+    M = om.sample_uniform(KEY, 30, batch * rows * cols)
+    v = om.sample_uniform(KEY, 31, batch * cols)
+    mi, vi = oracle.planar_to_ints(M), oracle.planar_to_ints(v)
+    out = oracle.planar_to_ints(gm.download(gm.matvec(batch, rows, cols, gm.upload(M), gm.upload(v)), batch * rows))
+    for bb in range(batch):
+        for r in range(rows):
+            want = sum(mi[(bb * rows + r) * cols + k] * vi[bb * cols + k] for k in range(cols)) * Rinv % q
+            assert out[bb * rows + r] == want, (bb, r)
+    out = oracle.planar_to_ints(gm.download(gm.matvec(batch, rows, cols, gm.upload(M[:, :rows * cols].copy()), gm.upload(v),
+                                                      broadcast_matrix=True), batch * rows))
+    for bb in range(batch):
+        for r in range(rows):
+            want = sum(mi[r * cols + k] * vi[bb * cols + k] for k in range(cols)) * Rinv % q
+            assert out[bb * rows + r] == want, (bb, r)
