@@ -96,7 +96,8 @@ namespace hmpc_amd
         {
             coefficient,
             number_theoretic_transform,
-            words,  // plain u32 words (messages, decrypted plaintexts)
+            words,         // plain u32 words (messages, decrypted plaintexts)
+            cipher_words,  // tensor-cipher output: element-major u32[size][L] (hmpc's shape {..., L})
         };
 
         // n polynomials of N = 2^log_n coefficients mod q, limb-planar [L][n N] on the device; or
@@ -162,6 +163,9 @@ namespace hmpc_amd
             sub,
             mul,
             scalar_mul,
+            sum,            // hmpc::expr::sum (reduce with add) to one element
+            product,        // hmpc::expr::product (reduce with multiply)
+            matvec,         // hmpc::expr::matrix_vector_product
             uniform,        // explicit-key uniform<mod q>
             cbd,            // explicit-key centred binomial
             cbd_capability, // queue-capability centred binomial (only under an NTT)
@@ -233,6 +237,29 @@ namespace hmpc_amd
             n.a = std::move(x);
             n.c = std::move(constant_mont);
             return make(std::move(n));
+        }
+
+        // hmpc/expr/reduce.hpp:264-274: sum / product of every element, a one-element tensor
+        inline expression sum(expression x)
+        {
+            node n;
+            n.kind = op::sum;
+            n.a = std::move(x);
+            return make(std::move(n));
+        }
+        inline expression product(expression x)
+        {
+            node n;
+            n.kind = op::product;
+            n.a = std::move(x);
+            return make(std::move(n));
+        }
+
+        // hmpc/expr/matrix_vector_product.hpp: M with `batch()` rows of N = cols coefficients, v one
+        // row of cols; the result holds one element per row of M (rows a power of two)
+        inline expression matrix_vector_product(expression matrix, expression vector)
+        {
+            return binary(op::matvec, std::move(matrix), std::move(vector));
         }
 
         namespace random
@@ -312,6 +339,55 @@ namespace hmpc_amd
                 return {std::move(k), std::move(messages), std::move(t_mont), r};
             }
             inline dec_expression dec(comp::tensor s, ciphertext ct) { return {std::move(s), std::move(ct)}; }
+
+            // ciphertext_expression operators (hmpc/expr/crypto/lhe/ciphertext.hpp:48-81): with another
+            // ciphertext (+, -), a plaintext polynomial in NTT form broadcast over the batch (*), or a
+            // Montgomery constant (*)
+            struct ciphertext_expression
+            {
+                int op = HMPC_ADD;
+                ciphertext a, b;
+                comp::tensor plain;
+                std::vector<std::uint32_t> constant;
+            };
+            inline ciphertext_expression operator+(ciphertext a, ciphertext b) { return {HMPC_ADD, std::move(a), std::move(b), {}, {}}; }
+            inline ciphertext_expression operator-(ciphertext a, ciphertext b) { return {HMPC_SUB, std::move(a), std::move(b), {}, {}}; }
+            inline ciphertext_expression operator*(ciphertext a, comp::tensor plain)
+            {
+                return {HMPC_MUL, std::move(a), {}, std::move(plain), {}};
+            }
+            inline ciphertext_expression operator*(ciphertext a, std::vector<std::uint32_t> constant_mont)
+            {
+                return {HMPC_MUL, std::move(a), {}, {}, std::move(constant_mont)};
+            }
+        }
+
+        // hmpc::expr::crypto::cipher / enc / dec (hmpc/expr/crypto/cipher.hpp): ChaCha20 keystream XOR
+        namespace crypto
+        {
+            struct cipher
+            {
+                std::array<std::uint32_t, 8> key{};
+                std::array<std::uint32_t, 3> nonce{};
+            };
+            struct cipher_enc_expression
+            {
+                cipher c;
+                comp::tensor value;
+            };
+            struct cipher_dec_expression
+            {
+                cipher c;
+                comp::tensor words;
+                comp::representation rep;
+            };
+            inline cipher_enc_expression enc(cipher c, comp::tensor value) { return {c, std::move(value)}; }
+            // dec<T>: the representation of the decrypted tensor stands for hmpc's value type T
+            inline cipher_dec_expression dec(cipher c, comp::tensor words,
+                                             comp::representation rep = comp::representation::coefficient)
+            {
+                return {c, std::move(words), rep};
+            }
         }
     }
 
@@ -360,6 +436,42 @@ namespace hmpc_amd
                                        e.messages.data(), e.plaintext_modulus.data(), e.r.bits_u, e.r.bits_v, e.r.bits_w,
                                        c0.data(), c1.data(), nullptr));
                 return {c0, c1};
+            }
+            // q(ct op ...): a new ciphertext, both parts in one launch
+            expr::crypto::lhe::ciphertext operator()(expr::crypto::lhe::ciphertext_expression const& e)
+            {
+                auto const& m = e.a.c0.mod();
+                std::size_t count = e.a.c0.size();
+                tensor c0(*this, m, e.a.c0.batch(), e.a.c0.log_n(), representation::number_theoretic_transform);
+                tensor c1(*this, m, e.a.c0.batch(), e.a.c0.log_n(), representation::number_theoretic_transform);
+                if (!e.constant.empty())
+                    check(hmpc_ciphertext_scalar(m.handle(), e.op, count, e.a.c0.data(), e.a.c1.data(), e.constant.data(),
+                                                 c0.data(), c1.data(), nullptr));
+                else if (e.plain)
+                    check(hmpc_ciphertext_op(m.handle(), e.op, count, e.a.c0.data(), e.a.c1.data(), e.plain.data(),
+                                             e.plain.data(), e.plain.size(), c0.data(), c1.data(), nullptr));
+                else
+                    check(hmpc_ciphertext_op(m.handle(), e.op, count, e.a.c0.data(), e.a.c1.data(), e.b.c0.data(),
+                                             e.b.c1.data(), count, c0.data(), c1.data(), nullptr));
+                return {c0, c1};
+            }
+            // q(enc(cipher, x)): element-major cipher words
+            tensor operator()(expr::crypto::cipher_enc_expression const& e)
+            {
+                auto const& x = e.value;
+                tensor t(*this, x.mod(), x.batch(), x.log_n(), representation::cipher_words);
+                check(hmpc_cipher_encrypt(ctx_, x.limbs(), x.size(), e.c.key.data(), e.c.nonce.data(), 0, x.data(), t.data(),
+                                          nullptr));
+                return t;
+            }
+            // q(dec<T>(cipher, y)): the limb-planar tensor back
+            tensor operator()(expr::crypto::cipher_dec_expression const& e)
+            {
+                auto const& y = e.words;
+                tensor t(*this, y.mod(), y.batch(), y.log_n(), e.rep);
+                check(hmpc_cipher_decrypt(ctx_, y.limbs(), y.size(), e.c.key.data(), e.c.nonce.data(), 0, y.data(), t.data(),
+                                          nullptr));
+                return t;
             }
             // q(dec(...)): plaintext words
             tensor operator()(expr::crypto::lhe::dec_expression const& e)
@@ -531,6 +643,19 @@ namespace hmpc_amd
             case op::sub:
             case op::mul:
             {
+                // a * b + c in one kernel, as hmpc fuses an element-wise tree into one (queue.hpp:387-412)
+                if (e->kind == op::add && e->a->kind == op::mul)
+                {
+                    tensor x = eval(e->a->a);
+                    tensor y = eval(e->a->b);
+                    tensor z = eval(e->b);
+                    if (x.mod() == y.mod() && x.mod() == z.mod() && x.size() == y.size() && x.size() == z.size())
+                    {
+                        tensor t(*this, x.mod(), x.batch(), x.log_n(), x.rep());
+                        check(hmpc_fma(x.mod().handle(), x.size(), x.data(), y.data(), z.data(), t.data(), nullptr));
+                        return t;
+                    }
+                }
                 tensor a = eval(e->a);
                 tensor b = eval(e->b);
                 if (!(a.mod() == b.mod()) || b.size() == 0 || a.size() % b.size() != 0)
@@ -545,6 +670,29 @@ namespace hmpc_amd
                 tensor a = eval(e->a);
                 tensor t(*this, a.mod(), a.batch(), a.log_n(), a.rep());
                 check(hmpc_pointwise_scalar(a.mod().handle(), HMPC_MUL, a.size(), a.data(), e->c.data(), t.data(), nullptr));
+                return t;
+            }
+            case op::sum:
+            case op::product:
+            {
+                tensor a = eval(e->a);
+                tensor t(*this, a.mod(), 1, 0, a.rep());
+                check(hmpc_reduce(a.mod().handle(), e->kind == op::sum ? HMPC_ADD : HMPC_MUL, a.size(), a.data(), t.data(),
+                                  nullptr));
+                return t;
+            }
+            case op::matvec:
+            {
+                tensor m = eval(e->a);
+                tensor v = eval(e->b);
+                std::size_t rows = m.batch(), cols = m.size() / m.batch();
+                int log_rows = 0;
+                while ((std::size_t{1} << log_rows) < rows)
+                    ++log_rows;
+                if (!(m.mod() == v.mod()) || v.size() != cols || (std::size_t{1} << log_rows) != rows)
+                    throw error(HMPC_ERR_INVALID_ARGUMENT, "matrix_vector_product: one modulus, v of M's row length, 2^k rows");
+                tensor t(*this, m.mod(), 1, log_rows, m.rep());
+                check(hmpc_matvec(m.mod().handle(), 1, rows, cols, m.data(), 0, v.data(), 0, t.data(), nullptr));
                 return t;
             }
             }

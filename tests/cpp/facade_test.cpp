@@ -1,5 +1,6 @@
 // Uses the C++ interface (include/hmpc_amd/comp.hpp) the way hmpc code uses hmpc::comp: a queue,
-// tensors, NTT expressions with hmpc's rewrites, explicit-key samplers, lhe::enc / dec. Writes
+// tensors, NTT expressions with hmpc's rewrites, explicit-key samplers, lhe::enc / dec, ciphertext
+// operators, the tensor cipher, sum and matrix_vector_product. Writes
 // its results (limb-planar u32 words) to the file named by argv[1] for tests/test_gpu_facade.py,
 // which recomputes them with the oracle. Exit code 0 when its own checks pass.
 #include "../../include/hmpc_amd/comp.hpp"
@@ -71,6 +72,31 @@ int main(int argc, char** argv)
         auto plain = q(crypto::lhe::dec(s, ct));
         if (plain.to_host() != words)
             return fail("decryption differs from the messages");
+        // ciphertext operators: a second encryption, the sum, 7 times the first
+        auto msgs2 = comp::make_tensor(q, m, k, log_n, comp::representation::words);
+        std::vector<std::uint32_t> words2(k << log_n);
+        for (std::size_t i = 0; i < words2.size(); ++i)
+            words2[i] = static_cast<std::uint32_t>(i * 40503u + 7u);
+        msgs2.from_host(words2.data(), words2.size());
+        auto ct2 = q(crypto::lhe::enc({A, B}, msgs2, t));
+        auto ct_sum = q(ct + ct2);
+        auto ct_7 = q(ct * m.from_unsigned(7));
+        auto dec_sum = q(crypto::lhe::dec(s, ct_sum)).to_host();
+        auto dec_7 = q(crypto::lhe::dec(s, ct_7)).to_host();
+        for (std::size_t i = 0; i < words.size(); ++i)
+            if (dec_sum[i] != words[i] + words2[i] || dec_7[i] != 7u * words[i])
+                return fail("ciphertext operators do not decrypt to the plaintext operations");
+        // tensor cipher round trip
+        crypto::cipher ch{key, {1, 0, 0}};
+        auto y = q(crypto::enc(ch, a));
+        auto a_again = q(crypto::dec(ch, y));
+        if (a_again.to_host() != a.to_host())
+            return fail("tensor cipher round trip");
+        // reductions and a matrix-vector product
+        auto total = q(sum(tensor(a)));
+        auto M = q(random::uniform(m, key, 20, 4, log_n));
+        auto v = q(random::uniform(m, key, 21, 1, log_n));
+        auto Mv = q(matrix_vector_product(tensor(M), tensor(v)));
         q.wait();
         std::ofstream f(argv[1], std::ios::binary);
         put(f, a.to_host());
@@ -84,6 +110,12 @@ int main(int argc, char** argv)
         put(f, words);
         put(f, ct.c0.to_host());
         put(f, ct.c1.to_host());
+        put(f, words2);
+        put(f, ct_sum.c0.to_host());
+        put(f, ct_sum.c1.to_host());
+        put(f, y.to_host());
+        put(f, total.to_host());
+        put(f, Mv.to_host());
         std::printf("facade ok\n");
         return 0;
     }
