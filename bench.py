@@ -338,6 +338,74 @@ def bench_copy(hm, ctx, stream, nbytes=1 << 30, reps=5):
     return {"bytes": nbytes, "ms": ms, "gbs": 2 * nbytes / (ms / 1e3) / 1e9}
 
 
+def bench_ext(hm, ctx, stream, reps=5):
+    """The element-wise operations around the path, each against the HBM roofline with its
+    algorithmic bytes (q128a, L = 4, 16 B per element): tensor cipher, ciphertext sum and
+    plaintext product over 1024 x 2^14 ciphertexts, fused multiply-add, sum reduction and a
+    4096 x 4096 matrix-vector product; and BGV through pinned host buffers (messages up, c0 and c1
+    down: the PCIe-inclusive rate)."""
+    mod = hm.Modulus(ctx, hm.Q128A)
+    L = mod.limbs
+    out = {}
+
+    def rate(name, fn, nbytes, extra=None):
+        ms, _, _ = time_steps(ctx, stream, fn, reps, 1)
+        gbs = nbytes / (ms / 1e3) / 1e9
+        out[name] = dict({"ms": ms, "bytes": nbytes, "gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}, **(extra or {}))
+
+    log("[bench] element-wise operations (q128a)")
+    count = 1 << 24
+    x = mod.sample_uniform(KEY, 40, count)
+    y = ctx.buffer(4 * L * count)
+    rate("cipher_encrypt", lambda: ctx.cipher_encrypt(L, count, KEY, [1, 0, 0], x, out=y, stream=stream), 2 * 4 * L * count,
+         {"elements": count, "chacha_blocks": count * L // 16})
+    n_ct = 1024 << 14
+    a0, a1 = mod.sample_uniform(KEY, 41, n_ct), mod.sample_uniform(KEY, 42, n_ct)
+    b0, b1 = mod.sample_uniform(KEY, 43, n_ct), mod.sample_uniform(KEY, 44, n_ct)
+    o0, o1 = mod.alloc(n_ct), mod.alloc(n_ct)
+    rate("ciphertext_add", lambda: mod.ciphertext_op(mod.ADD, n_ct, a0, a1, b0, b1, out0=o0, out1=o1, stream=stream),
+         3 * 2 * 16 * n_ct, {"ciphertexts": 1024, "N": 1 << 14})
+    p = mod.sample_uniform(KEY, 45, 1 << 14)
+    rate("ciphertext_mul_plain", lambda: mod.ciphertext_op(mod.MUL, n_ct, a0, a1, p, p, b_count=1 << 14, out0=o0, out1=o1,
+                                                           stream=stream), 2 * 2 * 16 * n_ct, {"ciphertexts": 1024})
+    del b0, b1
+    c = mod.sample_uniform(KEY, 46, count)
+    d = mod.sample_uniform(KEY, 47, count)
+    f = mod.alloc(count)
+    rate("fma", lambda: mod.fma(count, x, c, d, out=f, stream=stream), 4 * 16 * count, {"elements": count})
+    r = mod.alloc(1)
+    rate("sum", lambda: mod.reduce(mod.ADD, count, x, out=r, stream=stream), 16 * count, {"elements": count})
+    rows = cols = 4096
+    M = mod.sample_uniform(KEY, 48, rows * cols)
+    v = mod.sample_uniform(KEY, 49, cols)
+    mv = mod.alloc(rows)
+    rate("matvec", lambda: mod.matvec(1, rows, cols, M, v, out=mv, stream=stream), 16 * rows * cols,
+         {"rows": rows, "cols": cols})
+    del M, a0, a1, o0, o1, c, d, f, x, y
+    # BGV with host-resident messages and ciphertexts (pinned staging, async copies on the stream)
+    bmod = hm.Modulus(ctx, hm.Q128B)
+    log_n, n = 14, 1024
+    N = 1 << log_n
+    t_mont = bmod.mont(1 << 32)
+    A, B, _ = keygen(bmod, log_n, t_mont)
+    msgs_h = ctx.host_buffer(4 * n * N)
+    msgs_h.array()[:] = ctx.sample_words(KEY, 3, n * N).to_numpy(n * N)
+    c0_h, c1_h = ctx.host_buffer(16 * n * N), ctx.host_buffer(16 * n * N)
+    msgs_d, c0, c1 = ctx.buffer(4 * n * N), bmod.alloc(n * N), bmod.alloc(n * N)
+    bmod.autotune_bgv(log_n, 128, stream=stream)
+
+    def host_step():
+        msgs_h.to_device(msgs_d, stream=stream)
+        bmod.bgv_encrypt(log_n, n, A, B, msgs_d, t_mont, c0=c0, c1=c1, stream=stream)
+        c0_h.from_device(c0, stream=stream)
+        c1_h.from_device(c1, stream=stream)
+
+    ms, _, _ = time_steps(ctx, stream, host_step, 3, 1)
+    out["bgv_host_buffers"] = {"ms": ms, "enc_per_s": n / (ms / 1e3), "pcie_bytes": 4 * n * N + 2 * 16 * n * N,
+                               "pcie_gbs": (4 * n * N + 2 * 16 * n * N) / (ms / 1e3) / 1e9}
+    return out
+
+
 def cpu_baseline(pm, seconds_target=10.0):
     """Oracle restatement of hmpc's polymul on the host cores over a bounded sample of
     the same inputs (the first pairs of the batch)."""
@@ -404,7 +472,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3,c4,c5,copy (default all)")
+    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3,c4,c5,ext,copy (default all)")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests per load point of the serving sweep (the full workload uses 100000)")
     args = ap.parse_args()
@@ -417,7 +485,7 @@ def main():
 
     with open(os.path.join(ROOT, "BASELINE.json")) as f:
         baseline = json.load(f)
-    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "c5", "copy"}
+    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "c5", "ext", "copy"}
     ctx = hm.Context(0, key=KEY)
     stream = ctx.stream()
     info = ctx.info()
@@ -471,6 +539,8 @@ def main():
         line["serving"] = bench_c5(hm, ctx, stream, requests=args.serving_requests)
     if "c1" in sections:
         line["c1_latency"] = bench_c1(hm, ctx, stream)
+    if "ext" in sections:
+        line["elementwise"] = bench_ext(hm, ctx, stream)
     if "copy" in sections:
         line["copy_reference"] = bench_copy(hm, ctx, stream)
     if pm is not None and not args.no_cpu_baseline:

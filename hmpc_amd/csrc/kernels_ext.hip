@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace hmpc_amd
 {
     namespace
@@ -93,7 +95,7 @@ namespace hmpc_amd
         template<int L, bool Enc>
         __global__ void __launch_bounds__(ext_threads) k_cipher(u64 count, u32 const* __restrict__ in,
                                                                 u32* __restrict__ out, chacha_key key, u32 counter0,
-                                                                u32 n0, u32 n1, u32 n2)
+                                                                u32 n0, u32 n1, u32 n2, bool aligned)
         {
             u64 t = u64{blockIdx.x} * blockDim.x + threadIdx.x;
             u64 words = count * L;
@@ -102,6 +104,55 @@ namespace hmpc_amd
                 return;
             u32 ks[16];
             chacha20_block(ks, key, counter0 + static_cast<u32>(t), n0, n1, n2);
+            if constexpr (L == 1 || L == 2 || L == 4)
+            {
+                // a whole block of whole elements (16 / L of them, a multiple of 4): 16-byte loads
+                // per limb plane and four 16-byte stores of the element-major words; the planes and
+                // the output are 16-byte aligned when the buffers are and count is a multiple of 4
+                constexpr int per = 16 / L;
+                u64 e0 = t * per;
+                if (aligned && e0 + per <= count && count % 4 == 0)
+                {
+                    u32 v[16];
+                    if constexpr (Enc)
+                    {
+#pragma unroll
+                        for (int j = 0; j < L; ++j)
+#pragma unroll
+                            for (int q = 0; q < per; q += 4)
+                            {
+                                uint4 x = *reinterpret_cast<uint4 const*>(in + j * count + e0 + q);
+                                v[(q + 0) * L + j] = x.x;
+                                v[(q + 1) * L + j] = x.y;
+                                v[(q + 2) * L + j] = x.z;
+                                v[(q + 3) * L + j] = x.w;
+                            }
+#pragma unroll
+                        for (int k = 0; k < 16; k += 4)
+                            *reinterpret_cast<uint4*>(out + w0 + k) =
+                                make_uint4(v[k] ^ ks[k], v[k + 1] ^ ks[k + 1], v[k + 2] ^ ks[k + 2], v[k + 3] ^ ks[k + 3]);
+                    }
+                    else
+                    {
+#pragma unroll
+                        for (int k = 0; k < 16; k += 4)
+                        {
+                            uint4 x = *reinterpret_cast<uint4 const*>(in + w0 + k);
+                            v[k] = x.x ^ ks[k];
+                            v[k + 1] = x.y ^ ks[k + 1];
+                            v[k + 2] = x.z ^ ks[k + 2];
+                            v[k + 3] = x.w ^ ks[k + 3];
+                        }
+#pragma unroll
+                        for (int j = 0; j < L; ++j)
+#pragma unroll
+                            for (int q = 0; q < per; q += 4)
+                                *reinterpret_cast<uint4*>(out + j * count + e0 + q) =
+                                    make_uint4(v[(q + 0) * L + j], v[(q + 1) * L + j], v[(q + 2) * L + j], v[(q + 3) * L + j]);
+                    }
+                    return;
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 16; ++k)
             {
@@ -240,8 +291,28 @@ namespace hmpc_amd
             u64 r = row - b * rows;
             u64 m0 = b * m_stride + r * cols;
             u64 v0 = b * v_stride;
+            // four columns in flight per lane (independent loads and accumulators)
             u32 acc[L] = {};
-            for (u64 c = lane; c < cols; c += 64)
+            u32 acc2[L] = {};
+            u64 c = lane;
+            for (; c + 192 < cols; c += 256)
+            {
+                u32 x[4][L], y[4][L];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    load_planar<L>(x[u], M, m_count, m0 + c + 64 * u);
+                    load_planar<L>(y[u], v, v_count, v0 + c + 64 * u);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    u32 t[L];
+                    mod_mul<L>(t, x[u], y[u], P);
+                    combine<L>(0, (u & 1) ? acc2 : acc, t, P);
+                }
+            }
+            for (; c < cols; c += 64)
             {
                 u32 x[L], y[L], t[L];
                 load_planar<L>(x, M, m_count, m0 + c);
@@ -249,6 +320,7 @@ namespace hmpc_amd
                 mod_mul<L>(t, x, y, P);
                 combine<L>(0, acc, t, P);
             }
+            combine<L>(0, acc, acc2, P);
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1)
             {
@@ -260,6 +332,58 @@ namespace hmpc_amd
             }
             if (lane == 0)
                 store_planar<L>(out, batch * rows, row, acc);
+        }
+    }
+
+    namespace
+    {
+        // long rows: one workgroup per output row, the columns strided over all its threads (more
+        // loads in flight per row), then the workgroup tree
+        template<int L>
+        __global__ void __launch_bounds__(ext_threads) k_matvec_wide(u64 rows, u64 cols, u32 const* __restrict__ M,
+                                                                     u64 m_count, u64 m_stride, u32 const* __restrict__ v,
+                                                                     u64 v_count, u64 v_stride, u32* __restrict__ out,
+                                                                     u64 out_count, mod_params P)
+        {
+            __shared__ u32 lds[L * ext_threads];
+            u64 row = blockIdx.x;
+            u64 b = row / rows;
+            u64 r = row - b * rows;
+            u64 m0 = b * m_stride + r * cols;
+            u64 v0 = b * v_stride;
+            u32 acc[L] = {};
+            u32 acc2[L] = {};
+            u64 c = threadIdx.x;
+            constexpr u64 T = ext_threads;
+            for (; c + 3 * T < cols; c += 4 * T)
+            {
+                u32 x[4][L], y[4][L];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    load_planar<L>(x[u], M, m_count, m0 + c + T * u);
+                    load_planar<L>(y[u], v, v_count, v0 + c + T * u);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    u32 t[L];
+                    mod_mul<L>(t, x[u], y[u], P);
+                    combine<L>(0, (u & 1) ? acc2 : acc, t, P);
+                }
+            }
+            for (; c < cols; c += T)
+            {
+                u32 x[L], y[L], t[L];
+                load_planar<L>(x, M, m_count, m0 + c);
+                load_planar<L>(y, v, v_count, v0 + c);
+                mod_mul<L>(t, x, y, P);
+                combine<L>(0, acc, t, P);
+            }
+            combine<L>(0, acc, acc2, P);
+            block_reduce<L>(0, acc, lds, P);
+            if (threadIdx.x == 0)
+                store_planar<L>(out, out_count, row, acc);
         }
     }
 
@@ -300,6 +424,12 @@ namespace hmpc_amd
     {
         return ext_limbs(L, [&](auto LC) {
             constexpr int LL = decltype(LC)::value;
+            if (cols >= 1024)
+            {
+                k_matvec_wide<LL><<<static_cast<unsigned>(batch * rows), ext_threads, 0, s>>>(
+                    rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, batch * rows, P);
+                return hipGetLastError();
+            }
             std::size_t waves = ext_threads / 64;
             unsigned grid = static_cast<unsigned>((batch * rows + waves - 1) / waves);
             k_matvec<LL><<<grid, ext_threads, 0, s>>>(batch, rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, P);
@@ -341,12 +471,13 @@ namespace hmpc_amd
             for (int i = 0; i < 8; ++i)
                 key.k[i] = key8[i];
             std::size_t blocks = (count * LL + 15) / 16;
+            bool aligned = ((reinterpret_cast<std::uintptr_t>(in) | reinterpret_cast<std::uintptr_t>(out)) & 15u) == 0;
             if (encrypt)
                 k_cipher<LL, true><<<ext_grid(blocks), ext_threads, 0, s>>>(count, in, out, key, counter0, nonce3[0],
-                                                                            nonce3[1], nonce3[2]);
+                                                                            nonce3[1], nonce3[2], aligned);
             else
                 k_cipher<LL, false><<<ext_grid(blocks), ext_threads, 0, s>>>(count, in, out, key, counter0, nonce3[0],
-                                                                             nonce3[1], nonce3[2]);
+                                                                             nonce3[1], nonce3[2], aligned);
             return hipGetLastError();
         });
     }

@@ -54,7 +54,8 @@ def test_cipher_rfc8439_vector(ctx):
     assert np.array_equal(back.to_numpy((4, 7)), planar)
 
 
-@pytest.mark.parametrize("limbs,count", [(1, 1), (2, 37), (4, 4099), (5, 1000), (9, 333), (4, 1 << 18)])
+@pytest.mark.parametrize("limbs,count", [(1, 1), (1, 4096), (2, 37), (2, 1 << 16), (4, 4099), (4, 1 << 18), (5, 1000),
+                                         (8, 1000), (9, 333)])
 def test_cipher_matches_oracle(ctx, limbs, count):
     say(f"cipher limbs={limbs} count={count}")
     rng = np.random.default_rng(limbs * 1000 + count)
@@ -65,6 +66,32 @@ def test_cipher_matches_oracle(ctx, limbs, count):
     assert np.array_equal(ct.to_numpy((count, limbs)), want)
     back = ctx.cipher_decrypt(limbs, count, key, nonce, ct)
     assert np.array_equal(back.to_numpy((limbs, count)), planar)
+
+
+class _At:
+    """a device address inside a buffer (to pass unaligned pointers through the C ABI)"""
+
+    def __init__(self, buf, byte_offset):
+        self.ptr = buf.ptr + byte_offset
+
+
+@pytest.mark.parametrize("limbs", [1, 2, 4])
+def test_cipher_unaligned_buffers(ctx, limbs):
+    """the 16-byte fast path is taken only for aligned buffers; offset pointers give the same words"""
+    say(f"cipher unaligned limbs={limbs}")
+    count = 4096
+    rng = np.random.default_rng(77 + limbs)
+    planar = rng.integers(0, 1 << 32, size=(limbs, count), dtype=np.uint64).astype(np.uint32)
+    key, nonce = [1, 2, 3, 4, 5, 6, 7, 8], [0, 0, 9]
+    want = oracle.cipher_encrypt(planar, key, nonce)
+    src = ctx.upload(np.concatenate([np.zeros(1, np.uint32), planar.reshape(-1)]))
+    dst = ctx.buffer(4 * (limbs * count + 1))
+    ctx.cipher_encrypt(limbs, count, key, nonce, _At(src, 4), out=_At(dst, 4))
+    got = dst.to_numpy(limbs * count + 1)[1:]
+    assert np.array_equal(got, want.reshape(-1))
+    back = ctx.buffer(4 * (limbs * count + 1))
+    ctx.cipher_decrypt(limbs, count, key, nonce, _At(dst, 4), out=_At(back, 4))
+    assert np.array_equal(back.to_numpy(limbs * count + 1)[1:], planar.reshape(-1))
 
 
 def test_cipher_reference_tensor_roundtrip(ctx):
@@ -199,8 +226,13 @@ def test_fma_reduce_matvec_match_oracle(ctx, q):
     assert p == p_want
     z = oracle.planar_to_ints(gm.download(gm.reduce(gm.ADD, 0, gm.upload(a[:, :1].copy())), 1))[0]
     assert z == 0
-    # matvec: batch 3 of 17 x 70 matrices, then one matrix broadcast over 3 vectors
-    batch, rows, cols = 3, 17, 70
+    # matvec: batch 3 of 17 x 70 matrices, then one matrix broadcast over 3 vectors; the same
+    # with rows of 1500 (one workgroup per row)
+    for batch, rows, cols in ((3, 17, 70), (2, 5, 1500)):
+        _check_matvec(gm, om, batch, rows, cols, Rinv, q)
+
+
+def _check_matvec(gm, om, batch, rows, cols, Rinv, q):
     M = om.sample_uniform(KEY, 30, batch * rows * cols)
     v = om.sample_uniform(KEY, 31, batch * cols)
     mi, vi = oracle.planar_to_ints(M), oracle.planar_to_ints(v)
