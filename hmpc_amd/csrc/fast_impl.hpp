@@ -741,6 +741,8 @@ namespace hmpc_amd
 
         // The lazy forward transform runs up to 19 stages without reductions (r26.hpp); a
         // forward column result of a longer transform is folded before it goes to the workspace.
+        // (No prime 2^124 + c with c < 2^24 has 2^19 | q - 1, so radix-2^26 transforms stop at
+        // N = 2^17 and the fold is a guard only.)
         __host__ __device__ __forceinline__ bool ws_fold_columns(int log_n) { return log_n > 18; }
 
         struct pass_geom

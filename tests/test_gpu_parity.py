@@ -111,7 +111,8 @@ def test_poly_mod_kat_on_gpu(ctx):
             assert vals[j * N + k] * Rinv % P150 == want
 
 
-POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 16, 2),
+POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 16, 2), (Q128A, 17, 1),
+                 (Q64, 18, 1),
                  (Q128A, 5, 7), (Q128B, 14, 2), (Q128B, 13, 3), (Q_GENERIC4, 12, 2), (Q_GENERIC4, 14, 1),
                  (Q_LAZY4, 13, 2), (Q_LAZY2, 12, 2), (Q_GENERIC2, 13, 2), (P150, 9, 2)]
 
@@ -189,8 +190,10 @@ def _keygen(om, log_n, t_mont):
     return A, B, s
 
 
+# (Q128A, 17): the largest degree of any q = 2^124 + c with c < 2^24 (v2(q - 1) <= 18), so the longest
+# lazy chain of the radix-2^26 transforms
 @pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q128A, 12, 2), (Q64, 10, 3), (Q128A, 16, 1), (Q64, 15, 2),
-                                       (Q128A, 13, 4)])
+                                       (Q128A, 13, 4), (Q128A, 17, 1)])
 def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     gm, om = mods(ctx, q)
     say(f"bgv q={hex(q)[:12]} log_n={log_n} n={n}")
@@ -219,6 +222,27 @@ def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
     assert np.array_equal(dec, msgs)
     assert np.array_equal(dec, om.bgv_decrypt(log_n, s, want0, want1))
+
+
+def test_empty_inputs_are_no_ops(ctx):
+    """n = 0 / count = 0 through every batched entry point: no launch, no error, outputs untouched"""
+    gm, om = mods(ctx, Q128A)
+    say("empty inputs")
+    buf = gm.upload(om.sample_uniform(KEY, 0, 16))
+    before = gm.download(buf, 16)
+    gm.ntt(12, 0, buf, out=buf)
+    gm.polymul(14, 0, buf, buf, out=buf)
+    gm.pointwise(gm.ADD, 0, buf, buf, out=buf)
+    gm.ciphertext_op(gm.ADD, 0, buf, buf, buf, buf, b_count=1, out0=buf, out1=buf)
+    gm.fma(0, buf, buf, buf, out=buf)
+    gm.matvec(0, 4, 4, buf, buf, out=buf)
+    ctx.cipher_encrypt(4, 0, KEY, [0, 0, 0], buf, out=ctx.buffer(64))
+    pm = oracle.int_to_limbs(om.from_unsigned(1 << 32), gm.limbs)
+    nonce = ctx.rng[1].tolist()
+    gm.bgv_encrypt(14, 0, buf, buf, buf, pm, c0=buf, c1=buf)
+    assert ctx.rng[1].tolist() == nonce
+    ctx.wait()
+    assert np.array_equal(gm.download(buf, 16), before)
 
 
 def test_errors_are_reported(ctx):
