@@ -946,7 +946,10 @@ namespace hmpc_amd
         // polynomial product, whole polynomials per tile (N <= 4096) or the row pass of the
         // two-pass product: forward(A), forward(B), A o B, inverse. The row pass works in place
         // (a == out) and asks for 4 waves per SIMD (2 workgroups per CU with the 80 KiB tile).
-        template<typename A, int E, bool Single>
+        // S > 0: the row length log2 (G.log_n2) as a compile-time constant, so that the round
+        // structure and every LDS address are resolved at compile time (and only the round widths
+        // this length uses are emitted)
+        template<typename A, int E, bool Single, int S = 0>
         __global__ void __launch_bounds__(tile / E) k_polymul(u32 const* a, u32 const* __restrict__ b,
                                                               u32* out, u32 const* __restrict__ twf,
                                                               u32 const* __restrict__ twi, pass_geom G, u64 n,
@@ -966,6 +969,11 @@ namespace hmpc_amd
             else
             {
                 u32 hi0 = rows_setup(G, blockIdx.x, count, td, io);
+                if constexpr (S > 0)
+                {
+                    td.s = S;
+                    td.log_seqs = tile_log - S;
+                }
                 base0 = (1u << G.log_n1) + hi0;
                 base_step = 1;
             }
@@ -1167,8 +1175,18 @@ namespace hmpc_amd
                 unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
                 unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
                 if ((e = allow_lds(k_polymul_cols<A, E>, bytes)) != hipSuccess ||
-                    (e = allow_lds(k_polymul<A, E, false>, bytes)) != hipSuccess ||
                     (e = allow_lds(k_pass<A, E, 3, true>, bytes)) != hipSuccess)
+                    return e;
+                auto rows_kernel = k_polymul<A, E, false, 0>;
+                switch (gr.log_n2)
+                {
+                case 7: rows_kernel = k_polymul<A, E, false, 7>; break;
+                case 8: rows_kernel = k_polymul<A, E, false, 8>; break;
+                case 9: rows_kernel = k_polymul<A, E, false, 9>; break;
+                case 10: rows_kernel = k_polymul<A, E, false, 10>; break;
+                default: break;
+                }
+                if ((e = allow_lds(rows_kernel, bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast polymul forward columns");
@@ -1176,7 +1194,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
-                    k_polymul<A, E, false><<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
+                    rows_kernel<<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");
