@@ -224,6 +224,19 @@ namespace hmpc_amd
             static constexpr bool stage_prefetch = false;
         };
 
+        // the arithmetic of the product's row kernel: with its row length fixed at compile time the
+        // radix-2^26 row kernel holds a whole round's twiddles within the 128-VGPR budget
+        template<typename A>
+        struct rows_arith
+        {
+            using type = A;
+        };
+        template<>
+        struct rows_arith<arith_r26>
+        {
+            using type = arith_r26_round_prefetch;
+        };
+
         struct arith_gold
         {
             static constexpr int rl = 2;
@@ -950,7 +963,7 @@ namespace hmpc_amd
         // structure and every LDS address are resolved at compile time (and only the round widths
         // this length uses are emitted)
         template<typename A, int E, bool Single, int S = 0>
-        __global__ void __launch_bounds__(tile / E) k_polymul(u32 const* a, u32 const* __restrict__ b,
+        __global__ void __launch_bounds__(tile / E, (E == 8 && !Single) ? 4 : 1) k_polymul(u32 const* a, u32 const* __restrict__ b,
                                                               u32* out, u32 const* __restrict__ twf,
                                                               u32 const* __restrict__ twi, pass_geom G, u64 n,
                                                               typename A::params P)
@@ -1177,13 +1190,14 @@ namespace hmpc_amd
                 if ((e = allow_lds(k_polymul_cols<A, E>, bytes)) != hipSuccess ||
                     (e = allow_lds(k_pass<A, E, 3, true>, bytes)) != hipSuccess)
                     return e;
+                using AR = typename rows_arith<A>::type;
                 auto rows_kernel = k_polymul<A, E, false, 0>;
                 switch (gr.log_n2)
                 {
-                case 7: rows_kernel = k_polymul<A, E, false, 7>; break;
-                case 8: rows_kernel = k_polymul<A, E, false, 8>; break;
-                case 9: rows_kernel = k_polymul<A, E, false, 9>; break;
-                case 10: rows_kernel = k_polymul<A, E, false, 10>; break;
+                case 7: rows_kernel = k_polymul<AR, E, false, 7>; break;
+                case 8: rows_kernel = k_polymul<AR, E, false, 8>; break;
+                case 9: rows_kernel = k_polymul<AR, E, false, 9>; break;
+                case 10: rows_kernel = k_polymul<AR, E, false, 10>; break;
                 default: break;
                 }
                 if ((e = allow_lds(rows_kernel, bytes)) != hipSuccess)
