@@ -915,7 +915,7 @@ namespace hmpc_amd
         // mode: 0 forward column pass, 1 forward row pass (final), 2 inverse row pass,
         //       3 inverse column pass (final), 4 forward whole-poly, 5 inverse whole-poly
         // WsIn: `in` is a private workspace (the row results of the two-pass product)
-        template<typename A, int E, int Mode, bool WsIn = false>
+        template<typename A, int E, int Mode, bool WsIn = false, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_pass(u32 const* __restrict__ in, u32* __restrict__ out,
                                                            u32 const* __restrict__ tw, pass_geom G, u64 n,
                                                            typename A::params P)
@@ -931,6 +931,12 @@ namespace hmpc_amd
             if constexpr (Mode == 0 || Mode == 3)
             {
                 cols_setup(G, blockIdx.x, count, td, io);
+                if constexpr (S > 0)
+                {
+                    td.s = S;
+                    td.log_seqs = tile_log - S;
+                    io.log_seqs = tile_log - S;
+                }
                 final_scale = Mode == 3;
                 canonical = Mode == 3;
             }
@@ -1033,7 +1039,7 @@ namespace hmpc_amd
         }
 
         // forward column pass of both inputs of the two-pass product (blockIdx.x = 2 * tile + which)
-        template<typename A, int E>
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E) k_polymul_cols(u32 const* __restrict__ a, u32 const* __restrict__ b,
                                                                    u32* __restrict__ wa, u32* __restrict__ wb,
                                                                    u32 const* __restrict__ tw, pass_geom G, u64 n,
@@ -1046,6 +1052,12 @@ namespace hmpc_amd
             tile_desc td;
             io_desc io;
             cols_setup(G, blockIdx.x >> 1, count, td, io);
+            if constexpr (S > 0)
+            {
+                td.s = S;
+                td.log_seqs = tile_log - S;
+                io.log_seqs = tile_log - S;
+            }
             tile_load<A, T>(lds, which ? b : a, io);
             __syncthreads();
             u32 x[E][A::rl];
@@ -1187,8 +1199,10 @@ namespace hmpc_amd
                 pass_geom gr = geom_for(T.log_n, true);
                 unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
                 unsigned grid_r = static_cast<unsigned>(n * gr.blocks_per_poly);
-                if ((e = allow_lds(k_polymul_cols<A, E>, bytes)) != hipSuccess ||
-                    (e = allow_lds(k_pass<A, E, 3, true>, bytes)) != hipSuccess)
+                // the column length is 2^7 from N = 2^14 on (2^6 at N = 2^13)
+                auto cols_kernel = gc.log_n1 == 7 ? k_polymul_cols<A, E, 7> : k_polymul_cols<A, E, 0>;
+                auto icols_kernel = gc.log_n1 == 7 ? k_pass<A, E, 3, true, 7> : k_pass<A, E, 3, true, 0>;
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(icols_kernel, bytes)) != hipSuccess)
                     return e;
                 using AR = typename rows_arith<A>::type;
                 auto rows_kernel = k_polymul<A, E, false, 0>;
@@ -1204,7 +1218,7 @@ namespace hmpc_amd
                     return e;
                 {
                     probe_scope ps(s, "fast polymul forward columns");
-                    k_polymul_cols<A, E><<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
+                    cols_kernel<<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
@@ -1212,7 +1226,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");
-                    k_pass<A, E, 3, true><<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
+                    icols_kernel<<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
                 }
                 return hipGetLastError();
             });
