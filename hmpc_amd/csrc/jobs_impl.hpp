@@ -381,7 +381,7 @@ namespace hmpc_amd
         // small batches). Workspace: the three column-pass results [3][wsl][n N].
         //   column pass: blockIdx.x = (p * 3 + which) * tiles + t  (sample stream `which` of ciphertext p)
         //   row pass:    blockIdx.x = p * tiles + t
-        template<typename A, int E, int Which>
+        template<typename A, int E, int Which, int S = 0>
         __device__ __forceinline__ void bgv_column_tile(job_desc const& J, u32* lds, u32* dst, pass_geom const& gc, u32 ct,
                                                         std::size_t count, u32 p, typename A::params const& P)
         {
@@ -390,13 +390,23 @@ namespace hmpc_amd
             tile_desc td;
             io_desc is;
             cols_setup(gc, p * gc.blocks_per_poly + ct, count, td, is);
-            sample_tile<A, T, Which>(lds, J, true, gc.log_n1, gc.log_seqs, ct << gc.log_seqs, P);
+            int log_n1 = gc.log_n1, log_seqs = gc.log_seqs;
+            if constexpr (S > 0)
+            {
+                // the column length as a compile-time constant
+                log_n1 = S;
+                log_seqs = tile_log - S;
+                td.s = S;
+                td.log_seqs = log_seqs;
+                is.log_seqs = log_seqs;
+            }
+            sample_tile<A, T, Which>(lds, J, true, log_n1, log_seqs, ct << log_seqs, P);
             __syncthreads();
             tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
             tile_store<A, T, true>(lds, dst, is, ws_fold_columns(gc.log_n), P);
         }
 
-        template<typename A, int E>
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_cols(job_desc base, u32* __restrict__ ws,
                                                                               pass_geom gc, u64 n, typename A::params P)
         {
@@ -411,14 +421,14 @@ namespace hmpc_amd
             std::size_t count = static_cast<std::size_t>(n) << J.log_n;
             u32* dst = ws + which * A::wsl * count;
             if (which == 0)
-                bgv_column_tile<A, E, 0>(J, lds, dst, gc, ct, count, p, P);
+                bgv_column_tile<A, E, 0, S>(J, lds, dst, gc, ct, count, p, P);
             else if (which == 1)
-                bgv_column_tile<A, E, 1>(J, lds, dst, gc, ct, count, p, P);
+                bgv_column_tile<A, E, 1, S>(J, lds, dst, gc, ct, count, p, P);
             else
-                bgv_column_tile<A, E, 2>(J, lds, dst, gc, ct, count, p, P);
+                bgv_column_tile<A, E, 2, S>(J, lds, dst, gc, ct, count, p, P);
         }
 
-        template<typename A, int E>
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows(job_desc base, u32* __restrict__ ws,
                                                                               pass_geom gr, u64 n, typename A::params P)
         {
@@ -431,6 +441,11 @@ namespace hmpc_amd
             io_desc is, ik, io;
             u32 hi0 = rows_setup(gr, blockIdx.x, count, td, is);  // workspace planes of n N
             job_rows_tile(gr, rt, base.in_count, base.in_off, td, ik);  // public key (one polynomial)
+            if constexpr (S > 0)
+            {
+                td.s = S;  // the row length as a compile-time constant
+                td.log_seqs = tile_log - S;
+            }
             io = is;
             io.count = base.out_count;
             io.base = base.out_off + is.base;
@@ -467,15 +482,26 @@ namespace hmpc_amd
                 pass_geom gc = geom_for(log_n, false);
                 pass_geom gr = geom_for(log_n, true);
                 hipError_t e = hipSuccess;
-                if ((e = allow_lds(k_bgv_cols<A, E>, bytes)) != hipSuccess || (e = allow_lds(k_bgv_rows<A, E>, bytes)) != hipSuccess)
+                // column length 2^7 from N = 2^14 on; row lengths 2^7 .. 2^10 at N = 2^14 .. 2^17
+                auto cols_kernel = gc.log_n1 == 7 ? k_bgv_cols<A, E, 7> : k_bgv_cols<A, E, 0>;
+                auto rows_kernel = k_bgv_rows<A, E, 0>;
+                switch (gr.log_n2)
+                {
+                case 7: rows_kernel = k_bgv_rows<A, E, 7>; break;
+                case 8: rows_kernel = k_bgv_rows<A, E, 8>; break;
+                case 9: rows_kernel = k_bgv_rows<A, E, 9>; break;
+                case 10: rows_kernel = k_bgv_rows<A, E, 10>; break;
+                default: break;
+                }
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast bgv columns");
-                    k_bgv_cols<A, E><<<static_cast<unsigned>(3 * n * gc.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gc, n, P);
+                    cols_kernel<<<static_cast<unsigned>(3 * n * gc.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gc, n, P);
                 }
                 {
                     probe_scope ps(s, "fast bgv rows");
-                    k_bgv_rows<A, E><<<static_cast<unsigned>(n * gr.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gr, n, P);
+                    rows_kernel<<<static_cast<unsigned>(n * gr.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gr, n, P);
                 }
                 return hipGetLastError();
             });
