@@ -172,8 +172,9 @@ def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
     for qi, m in mods.items():
         for ln in range(12, 17):
             m.prepare(ln)
-            tuned[f"{'q64' if qi == 0 else 'q128a'}/2^{ln}"] = {"polymul": m.autotune(ln, 16, stream=stream),
-                                                               "bgv": m.autotune_bgv(ln, 16, stream=stream)}
+            # trial sizes near the batch's per-class job counts (about 2000 / 20 classes)
+            tuned[f"{'q64' if qi == 0 else 'q128a'}/2^{ln}"] = {"polymul": m.autotune(ln, 64, stream=stream),
+                                                               "bgv": m.autotune_bgv(ln, 128, stream=stream)}
             t_mont = m.mont(1 << 32)
             A, B, _ = keygen(m, ln, t_mont)
             keys[(qi, ln)] = (A, B, t_mont)
@@ -201,6 +202,12 @@ def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
     stream.synchronize()
     g_ms, g_per, _ = time_steps(ctx, stream, lambda: batch.run(graph=True, stream=stream), 5, 1)
     e_ms, _, _ = time_steps(ctx, stream, lambda: batch.run(graph=False, stream=stream), 3, 1)
+    # per-launch event timing of one eager run (launch probes are not part of the captured graph)
+    ctx.set_timing(True)
+    batch.run(graph=False, stream=stream)
+    stream.synchronize()
+    kernels = ctx.kernel_times()
+    ctx.set_timing(False)
     info = batch.info()
 
     def per_job(plain):
@@ -228,6 +235,7 @@ def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
             "per_job_fused_ms": f_ms, "plain_per_job_ms": p_ms, "coeffs": coeffs,
             "gcoeff_per_s": coeffs / (g_ms / 1e3) / 1e9, "batched_vs_plain": p_ms / g_ms,
             "batched_vs_per_job_fused": f_ms / g_ms, "classes": info["classes"],
+            "eager_kernels_ms": {k: v[0] * v[1] for k, v in kernels.items()},
             "device_bytes": info["device_bytes"], "schedules": tuned}
 
 
