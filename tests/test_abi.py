@@ -53,3 +53,30 @@ def test_cpp_interface_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
                         f"-I{os.path.join(ROOT, 'include')}", src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_argument_validation_without_gpu():
+    """null handles and out-of-range arguments are rejected by the host side of every newer entry
+    point before any device work (status codes, messages through hmpc_last_error)"""
+    if not os.path.exists(hmpc_amd.LIB_PATH):
+        pytest.skip("library not built")
+    lib = hmpc_amd.load()
+    key = (ctypes.c_uint32 * 8)()
+    nonce = (ctypes.c_uint32 * 3)()
+    one = (ctypes.c_uint32 * 4)(1)
+    calls = {
+        "hmpc_ciphertext_op": lambda: lib.hmpc_ciphertext_op(None, 0, 16, None, None, None, None, 16, None, None, None),
+        "hmpc_ciphertext_scalar": lambda: lib.hmpc_ciphertext_scalar(None, 2, 16, None, None, one, None, None, None),
+        "hmpc_cipher_encrypt": lambda: lib.hmpc_cipher_encrypt(None, 4, 16, key, nonce, 0, None, None, None),
+        "hmpc_cipher_decrypt": lambda: lib.hmpc_cipher_decrypt(None, 4, 16, key, nonce, 0, None, None, None),
+        "hmpc_fma": lambda: lib.hmpc_fma(None, 16, None, None, None, None, None),
+        "hmpc_reduce": lambda: lib.hmpc_reduce(None, 0, 16, None, None, None),
+        "hmpc_matvec": lambda: lib.hmpc_matvec(None, 1, 4, 4, None, 0, None, 0, None, None),
+        "hmpc_host_alloc": lambda: lib.hmpc_host_alloc(None, 64, None),
+        "hmpc_memcpy_h2d_async": lambda: lib.hmpc_memcpy_h2d_async(None, None, None, 64, None),
+        "hmpc_memcpy_d2h_async": lambda: lib.hmpc_memcpy_d2h_async(None, None, None, 64, None),
+    }
+    for name, call in calls.items():
+        rc = call()
+        assert rc != 0, name
+        assert lib.hmpc_last_error(), name
