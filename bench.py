@@ -457,7 +457,9 @@ def traffic_for(kernel, workload):
 def valu_issue_for(kernel, workload, kernel_ms, cus):
     """VALU issue bound of `kernel` from the committed counter summary (profiles/valu.json:
     SQ_INSTS_VALU wave instructions per dispatch): the time those instructions need at one wave
-    instruction per SIMD every 4 cycles at 2.4 GHz, and its fraction of the measured duration."""
+    instruction per SIMD every 4 cycles at 2.4 GHz, and its fraction of the measured duration;
+    where the summary holds the engine clock the kernel ran at under the counters
+    (GRBM_GUI_ACTIVE / duration), the same at that clock."""
     path = os.path.join(ROOT, "profiles", "valu.json")
     try:
         with open(path) as f:
@@ -469,8 +471,13 @@ def valu_issue_for(kernel, workload, kernel_ms, cus):
         return None
     rate = cus * 4 * VALU_CLOCK_GHZ * 1e9 / 4
     issue_ms = e["SQ_INSTS_VALU"] / rate * 1e3
-    return {"wave_instructions": e["SQ_INSTS_VALU"], "issue_rate_per_s": rate, "issue_ms": issue_ms,
-            "frac": issue_ms / kernel_ms}
+    out = {"wave_instructions": e["SQ_INSTS_VALU"], "issue_rate_per_s": rate, "issue_ms": issue_ms,
+           "frac": issue_ms / kernel_ms}
+    if e.get("clock_ghz"):
+        ms = e["SQ_INSTS_VALU"] / (cus * e["clock_ghz"] * 1e9) * 1e3
+        out.update({"clock_ghz_measured": e["clock_ghz"], "issue_ms_at_measured_clock": ms,
+                    "frac_at_measured_clock": ms / kernel_ms})
+    return out
 
 
 def main():

@@ -62,6 +62,7 @@ namespace hmpc_amd
             static constexpr int wsl = 5;   // limbs per element in private workspaces (raw limbs)
             static constexpr int tws = 8;   // twiddle table stride in words (5 used)
             static constexpr bool stage_prefetch = true;
+            static constexpr bool wave_units = false;  // unit order (unit_index)
             using params = p26;
 
             __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
@@ -224,8 +225,42 @@ namespace hmpc_amd
             static constexpr bool stage_prefetch = false;
         };
 
+        // The LDS slot order of the row kernels (rows of 2^7 .. 2^10 elements). With 64 banks of 4
+        // bytes, element slots of 5 words and E = 8 every gather and scatter of every round of these
+        // rows, and the 64 consecutive elements of a tile load or store, reach 64 distinct banks
+        // (e ^ ((e >> 4) & 15) leaves 2- and 4-way conflicts in most of their rounds); at E = 4 and
+        // 16 it is at most 2-way.
+        struct arith_r26_rows : arith_r26
+        {
+            static constexpr bool wave_units = true;  // wave-local row tiles (rows of <= 64 E elements)
+
+            __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 3) & 63) ^ ((e >> 10) & 15); }
+
+            __device__ __forceinline__ static void lds_get(u32 (&x)[rl], u32 const* lds, int e)
+            {
+                int p = phys(e) * rl;
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    x[l] = lds[p + l];
+            }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl])
+            {
+                int p = phys(e) * rl;
+#pragma unroll
+                for (int l = 0; l < rl; ++l)
+                    lds[p + l] = x[l];
+            }
+        };
+
+        struct arith_r26_rows_prefetch : arith_r26_rows
+        {
+            static constexpr bool stage_prefetch = false;
+        };
+
         // the arithmetic of the product's row kernel: with its row length fixed at compile time the
-        // radix-2^26 row kernel holds a whole round's twiddles within the 128-VGPR budget
+        // radix-2^26 row kernel holds a whole round's twiddles within the 128-VGPR budget, and its
+        // rows are wave-local in the row kernels' LDS slot order
         template<typename A>
         struct rows_arith
         {
@@ -234,7 +269,7 @@ namespace hmpc_amd
         template<>
         struct rows_arith<arith_r26>
         {
-            using type = arith_r26_round_prefetch;
+            using type = arith_r26_rows_prefetch;
         };
 
         struct arith_gold
@@ -244,6 +279,7 @@ namespace hmpc_amd
             static constexpr int wsl = 2;
             static constexpr int tws = 2;
             static constexpr bool stage_prefetch = false;
+            static constexpr bool wave_units = false;
             using params = fast_params;
             using ops = fast_ops<2, form_goldilocks>;
 
@@ -430,6 +466,44 @@ namespace hmpc_amd
             static constexpr int wmax = ilog2(E);
         };
 
+        // Unit g of the thread in a round of width W: tid + (tile / E) g, or with the policy's
+        // wave_units the wave-major order in which each wave owns the 64 E / 2^W consecutive units
+        // [wave 64 E / 2^W, (wave + 1) 64 E / 2^W), lane-consecutive within each g. In a row-layout
+        // tile whose rows are at most 64 E elements long every round of a wave then touches only
+        // the wave's own rows (tile elements [wave 64 E, (wave + 1) 64 E)). Both orders are the same
+        // for full-width rounds (2^W = E, one unit per thread).
+        template<typename A, int E, int W>
+        __device__ __forceinline__ int unit_index(int g)
+        {
+            int const t = tid_x();
+            if constexpr ((1 << W) == E)
+                return t;
+            else if constexpr (A::wave_units)
+                return ((((t >> 6) * (E >> W)) + g) << 6) | (t & 63);
+            else
+                return t + shape<E>::threads * g;
+        }
+
+        // Wave-local tiles (every exchange of a wave stays within its own tile elements) replace
+        // the workgroup barrier by a wave barrier with wavefront-scope fences: a wave's LDS and
+        // vector memory operations complete in order, so lanes of one wave see each other's
+        // stores after it without waiting for the other waves.
+        __device__ __forceinline__ void wave_sync()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
+        template<bool WL>
+        __device__ __forceinline__ void tile_sync()
+        {
+            if constexpr (WL)
+                wave_sync();
+            else
+                __syncthreads();
+        }
+
         template<typename A, int E, int W>
         __device__ __forceinline__ void gather(u32 (&x)[E][A::rl], u32 const* lds, tile_desc const& td, int a, bool inv)
         {
@@ -437,7 +511,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                unit_t un = unit_of(td, a, W, inv, tid_x() + shape<E>::threads * g);
+                unit_t un = unit_of(td, a, W, inv, unit_index<A, E, W>(g));
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                     A::lds_get(x[g * K + k], lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)));
@@ -451,7 +525,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                unit_t un = unit_of(td, a, W, inv, tid_x() + shape<E>::threads * g);
+                unit_t un = unit_of(td, a, W, inv, unit_index<A, E, W>(g));
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                     A::lds_put(lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)), x[g * K + k]);
@@ -479,7 +553,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                unit_t un = unit_of(td, a, W, Inv, tid_x() + shape<E>::threads * g);
+                unit_t un = unit_of(td, a, W, Inv, unit_index<A, E, W>(g));
                 u32 base = base0 + un.c * base_step;
 #pragma unroll
                 for (int m = 0; m < count; ++m)
@@ -640,7 +714,8 @@ namespace hmpc_amd
         // All rounds of the tile's sub-transform, LDS -> LDS. With keep_last (forward only) the
         // last round's results stay in x instead of being written back. (Width dispatch by
         // explicit branches: a lambda over x would force the array into scratch memory.)
-        template<typename A, int E, bool Inv>
+        // WL: a wave-local tile (rows of at most 64 E elements), synchronized per wave.
+        template<typename A, int E, bool Inv, bool WL = false>
         __device__ __forceinline__ void tile_rounds(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
                                                     u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
                                                     bool final_scale, int first_round, bool keep_last, int off = 0)
@@ -654,7 +729,7 @@ namespace hmpc_amd
                 bool keep = keep_last && r == n - 1;
                 dispatch_round<A, E, Inv>(w, x, lds, td, a, base0, base_step, tw, P, final_scale, keep, off);
                 if (!keep)
-                    __syncthreads();
+                    tile_sync<WL>();
             }
         }
 
@@ -710,12 +785,16 @@ namespace hmpc_amd
         }
 
         // Ws: src is a private workspace (A::wsl limb planes, raw format) rather than data in the
-        // hmpc layout
-        template<typename A, int T, bool Ws = false>
+        // hmpc layout. WL: each wave moves its own elements wave 64 E + 64 i + lane (E = tile / T),
+        // the ones its rounds work on in a wave-local tile.
+        template<typename A, int T, bool Ws = false, bool WL = false>
         __device__ __forceinline__ void tile_load(u32* lds, u32 const* __restrict__ src, io_desc const& io)
         {
+            int const t = tid_x();
+            int const e0 = WL ? (t & ~63) * (tile / T) + (t & 63) : t;
+            int const end = WL ? e0 + tile / T * 64 : tile;
 #pragma unroll 4
-            for (int e = tid_x(); e < tile; e += T)
+            for (int e = e0; e < end; e += WL ? 64 : T)
             {
                 u32 x[A::rl];
                 if (e < io.valid)
@@ -736,12 +815,15 @@ namespace hmpc_amd
         }
 
         // Ws: dst is a private workspace (raw format; canonical is then the fold flag of store_ws)
-        template<typename A, int T, bool Ws = false>
+        template<typename A, int T, bool Ws = false, bool WL = false>
         __device__ __forceinline__ void tile_store(u32 const* lds, u32* __restrict__ dst, io_desc const& io, bool canonical,
                                                    typename A::params const& P)
         {
+            int const t = tid_x();
+            int const e0 = WL ? (t & ~63) * (tile / T) + (t & 63) : t;
+            int const end = WL ? (e0 + tile / T * 64 < io.valid ? e0 + tile / T * 64 : io.valid) : io.valid;
 #pragma unroll 4
-            for (int e = tid_x(); e < io.valid; e += T)
+            for (int e = e0; e < end; e += WL ? 64 : T)
             {
                 u32 x[A::rl];
                 A::lds_get(x, lds, e);
@@ -845,7 +927,7 @@ namespace hmpc_amd
 
         // ------------------------------------------------------------------ second operands
         // After the forward transform's last round (stages [s - W, s)) of a row-layout tile, the
-        // thread's unit g holds the K = 2^W consecutive tile elements ((tid + T g) << W) + k, so a
+        // thread's unit g holds the K = 2^W consecutive tile elements (unit_index(g) << W) + k, so a
         // second operand of the same elements is read straight from its private workspace (K
         // consecutive words per limb plane, raw format; the workgroup wrote it itself just
         // before, so it is served from L2) instead of occupying a second register tile.
@@ -862,7 +944,7 @@ namespace hmpc_amd
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {
-                int const e0 = (tid_x() + shape<E>::threads * g) << W;
+                int const e0 = unit_index<A, E, W>(g) << W;
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                 {
@@ -1020,21 +1102,25 @@ namespace hmpc_amd
             else
             {
                 // two-pass row tile: NTT(a) rows go back to the workspace (a == out, in place) and
-                // are read per element next to NTT(b) in registers, so one register tile is live
+                // are read per element next to NTT(b) in registers, so one register tile is live.
+                // With the wave-major unit order, rows of at most 64 E elements (S <= 9 at E = 8) are
+                // wave-local: each wave loads, transforms, multiplies and stores whole rows of its
+                // own, with no workgroup barrier.
+                constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
                 u32 x[E][A::rl];
-                tile_load<A, T, true>(lds, a, io);
-                __syncthreads();
-                tile_rounds<A, E, false>(x, lds, td, base0, base_step, twf, P, false, 0, false);
-                tile_store<A, T, true>(lds, out, io, false, P);
-                __syncthreads();
-                tile_load<A, T, true>(lds, b, io);
-                __syncthreads();
-                tile_rounds<A, E, false>(x, lds, td, base0, base_step, twf, P, false, 0, true);
+                tile_load<A, T, true, WL>(lds, a, io);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, false);
+                tile_store<A, T, true, WL>(lds, out, io, false, P);
+                tile_sync<WL>();
+                tile_load<A, T, true, WL>(lds, b, io);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
                 with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
                 dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
-                __syncthreads();
-                tile_rounds<A, E, true>(x, lds, td, base0, base_step, twi, P, false, 1, false);
-                tile_store<A, T, true>(lds, out, io, false, P);
+                tile_sync<WL>();
+                tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false);
+                tile_store<A, T, true, WL>(lds, out, io, false, P);
             }
         }
 

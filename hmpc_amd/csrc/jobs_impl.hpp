@@ -233,7 +233,7 @@ namespace hmpc_amd
 
         // c = k o U + X: X in registers after its forward transform, U and k read per element,
         // result written through LDS with coalesced stores (canonical)
-        template<typename A, int E>
+        template<typename A, int E, bool WL = false>
         __device__ __forceinline__ void bgv_emit(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 const* U,
                                                  std::size_t u_count, std::size_t u_base, u32 const* key,
                                                  io_desc const& ik, u32* out, io_desc const& io,
@@ -242,9 +242,9 @@ namespace hmpc_amd
             constexpr int T = tile / E;
             with_operands_last<A, E, 1>(x, td, ik.valid, U, u_count, u_base, key, ik.count, ik.base, P);
             scatter_last<A, E>(x, lds, td);
-            __syncthreads();
-            tile_store<A, T>(lds, out, io, true, P);
-            __syncthreads();
+            tile_sync<WL>();
+            tile_store<A, T, false, WL>(lds, out, io, true, P);
+            tile_sync<WL>();
         }
 
         template<typename A, int E, int Which>
@@ -452,20 +452,23 @@ namespace hmpc_amd
             u32 base0 = (1u << gr.log_n1) + hi0;
             u32* su = ws;
             u32 x[E][A::rl];
+            // with the wave-major unit order rows of at most 64 E elements are wave-local: no
+            // workgroup barrier
+            constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
             // NTT(u) rows back into the workspace, read per element below
-            tile_load<A, T, true>(lds, su, is);
-            __syncthreads();
-            tile_rounds<A, E, false>(x, lds, td, base0, 1u, base.twf, P, false, 0, false);
-            tile_store<A, T, true>(lds, su, is, false, P);
-            __syncthreads();
+            tile_load<A, T, true, WL>(lds, su, is);
+            tile_sync<WL>();
+            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, false);
+            tile_store<A, T, true, WL>(lds, su, is, false, P);
+            tile_sync<WL>();
             for (int which = 1; which < 3; ++which)
             {
-                tile_load<A, T, true>(lds, ws + which * A::wsl * count, is);
-                __syncthreads();
-                tile_rounds<A, E, false>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                tile_load<A, T, true, WL>(lds, ws + which * A::wsl * count, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
                 bool first = which == 1;
-                bgv_emit<A, E>(x, lds, td, su, count, is.base, first ? base.in0 : base.in1, ik, first ? base.out0 : base.out1,
-                               io, P);
+                bgv_emit<A, E, WL>(x, lds, td, su, count, is.base, first ? base.in0 : base.in1, ik,
+                                   first ? base.out0 : base.out1, io, P);
             }
         }
 

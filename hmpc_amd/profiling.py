@@ -11,7 +11,9 @@
     python -m hmpc_amd.profiling counters RESULTS.db --workload c2 [--clock-ghz 2.4] [--out valu.json]
         instruction counters (SQ_INSTS_*) per dispatch of the workload's kernels, and the time
         their VALU instructions need at the issue rate of one wave instruction per SIMD every
-        4 cycles (CUs x 4 SIMDs x clock / 4)
+        4 cycles (CUs x 4 SIMDs x clock / 4); with GRBM_GUI_ACTIVE in the run, also the engine
+        clock each kernel ran at (busy cycles per XCD / dispatch duration). Several runs (databases of
+        different counters) merge into one entry per kernel in --out.
 
 The traffic file maps bench kernel names (the names of the library's launch probes) to bytes
 per launch; bench.py reports it as roofline.traffic.
@@ -122,10 +124,13 @@ def cmd_traffic(args):
 
 def cmd_counters(args):
     cur = sqlite3.connect(args.db).cursor()
-    rows = cur.execute("select kernel_name, grid_size, counter_name, value from counters_collection").fetchall()
+    rows = cur.execute("select kernel_name, grid_size, counter_name, value, start, end from counters_collection").fetchall()
     groups = collections.defaultdict(lambda: collections.defaultdict(list))
-    for name, grid, counter, value in rows:
+    for name, grid, counter, value, start, end in rows:
         groups[(_short(name), int(grid))][counter].append(float(value))
+        if counter == "GRBM_GUI_ACTIVE":
+            # GPU busy cycles over the dispatch: with its duration, the engine clock it ran at
+            groups[(_short(name), int(grid))]["duration_ns"].append(float(end - start))
     issue = args.cus * 4 * args.clock_ghz * 1e9 / 4  # wave instructions per second
     entries = {}
     print(f"VALU issue rate: {args.cus} CUs x 4 SIMDs x {args.clock_ghz} GHz / 4 cycles = {issue / 1e9:.1f} G wave-instr/s")
@@ -140,6 +145,10 @@ def cmd_counters(args):
             print(f"  {k:<24} {avg[k]:>16.6g}")
         if "SQ_INSTS_VALU" in avg:
             print(f"  VALU issue time          {avg['SQ_INSTS_VALU'] / issue * 1e3:>13.3f} ms")
+        if avg.get("duration_ns"):
+            # rocprofv3 sums GRBM_GUI_ACTIVE over the XCDs (8 on MI355X)
+            avg["clock_ghz"] = avg["GRBM_GUI_ACTIVE"] / args.xcds / avg["duration_ns"]
+            print(f"  engine clock             {avg['clock_ghz']:>13.3f} GHz (GRBM_GUI_ACTIVE / {args.xcds} XCDs / duration)")
         entries[bench_name] = dict(avg, grid=grid)
     if args.out:
         out = {}
@@ -148,7 +157,10 @@ def cmd_counters(args):
                 out = json.load(f)
         except (OSError, ValueError):
             pass
-        out[args.workload] = entries
+        merged = out.get(args.workload, {})
+        for k, v in entries.items():
+            merged.setdefault(k, {}).update(v)
+        out[args.workload] = merged
         with open(args.out, "w") as f:
             json.dump(out, f, indent=1, sort_keys=True)
             f.write("\n")
@@ -171,6 +183,7 @@ def main(argv=None):
     c.add_argument("--workload", required=True, choices=sorted(WORKLOAD_KERNELS))
     c.add_argument("--cus", type=int, default=256)
     c.add_argument("--clock-ghz", type=float, default=2.4)
+    c.add_argument("--xcds", type=int, default=8)
     c.add_argument("--out", default="")
     args = ap.parse_args(argv)
     {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters}[args.cmd](args)

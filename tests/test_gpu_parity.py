@@ -111,7 +111,8 @@ def test_poly_mod_kat_on_gpu(ctx):
             assert vals[j * N + k] * Rinv % P150 == want
 
 
-POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 16, 2), (Q128A, 17, 1),
+POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 15, 2), (Q128A, 16, 2),
+                 (Q128A, 17, 1),
                  (Q64, 18, 1),
                  (Q128A, 5, 7), (Q128B, 14, 2), (Q128B, 13, 3), (Q_GENERIC4, 12, 2), (Q_GENERIC4, 14, 1),
                  (Q_LAZY4, 13, 2), (Q_LAZY2, 12, 2), (Q_GENERIC2, 13, 2), (P150, 9, 2)]
@@ -192,8 +193,8 @@ def _keygen(om, log_n, t_mont):
 
 # (Q128A, 17): the largest degree of any q = 2^124 + c with c < 2^24 (v2(q - 1) <= 18), so the longest
 # lazy chain of the radix-2^26 transforms
-@pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q128A, 12, 2), (Q64, 10, 3), (Q128A, 16, 1), (Q64, 15, 2),
-                                       (Q128A, 13, 4), (Q128A, 17, 1)])
+@pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q128A, 12, 2), (Q64, 10, 3), (Q128A, 15, 2), (Q128A, 16, 1),
+                                       (Q64, 15, 2), (Q128A, 13, 4), (Q128A, 17, 1)])
 def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     gm, om = mods(ctx, q)
     say(f"bgv q={hex(q)[:12]} log_n={log_n} n={n}")
@@ -291,8 +292,8 @@ def test_pointwise_ops_edge_values(ctx, q):
         assert np.array_equal(got, want), hex(c)
 
 
-FAST_CASES = [(Q128A, 4, 5), (Q128A, 12, 3), (Q128A, 13, 2), (Q128A, 16, 2), (Q128B, 14, 2), (Q64, 3, 9), (Q64, 12, 3),
-              (Q64, 14, 2), (Q64, 16, 1)]
+FAST_CASES = [(Q128A, 4, 5), (Q128A, 12, 3), (Q128A, 13, 2), (Q128A, 15, 2), (Q128A, 16, 2), (Q128A, 17, 1), (Q128B, 14, 2),
+              (Q64, 3, 9), (Q64, 12, 3), (Q64, 14, 2), (Q64, 15, 2), (Q64, 16, 1)]
 
 
 @pytest.mark.parametrize("q,log_n,n", FAST_CASES, ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
