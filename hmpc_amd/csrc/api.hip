@@ -1117,7 +1117,8 @@ int hmpc_matvec(hmpc_modulus mod, size_t batch, size_t rows, size_t cols, const 
     device_guard g(mod->ctx->device);
     std::size_t m_batches = broadcast_matrix ? 1 : batch;
     std::size_t v_batches = broadcast_vector ? 1 : batch;
-    HMPC_HIP(launch_matvec(mod->limbs, mod->P, batch, rows, cols, M, m_batches * rows * cols,
+    p26 const* r26 = mod->form == form_r26 && mod->variant == 0 ? &mod->r26 : nullptr;
+    HMPC_HIP(launch_matvec(mod->limbs, mod->P, r26, batch, rows, cols, M, m_batches * rows * cols,
                            broadcast_matrix ? 0 : rows * cols, v, v_batches * cols, broadcast_vector ? 0 : cols, out,
                            as_stream(stream)));
     return HMPC_OK;

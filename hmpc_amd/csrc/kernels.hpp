@@ -130,10 +130,11 @@ namespace hmpc_amd
     unsigned reduce_parts(std::size_t count);
     hipError_t launch_reduce(int L, mod_params const& P, u32 const* one, int op, std::size_t count, u32 const* in,
                              u32* partials, u32* out, hipStream_t s);
-    // out[b][r] = sum_c M[b m_stride + r cols + c] v[b v_stride + c] (strides 0 broadcast)
-    hipError_t launch_matvec(int L, mod_params const& P, std::size_t batch, std::size_t rows, std::size_t cols,
-                             u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v, std::size_t v_count,
-                             std::size_t v_stride, u32* out, hipStream_t s);
+    // out[b][r] = sum_c M[b m_stride + r cols + c] v[b v_stride + c] (strides 0 broadcast); r26: the
+    // radix-2^26 constants when q = 2^124 + c (4 limbs), else null
+    hipError_t launch_matvec(int L, mod_params const& P, p26 const* r26, std::size_t batch, std::size_t rows,
+                             std::size_t cols, u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v,
+                             std::size_t v_count, std::size_t v_stride, u32* out, hipStream_t s);
     // tensor cipher: limb-planar [L][count] <-> element-major words [count][L] XOR the ChaCha20
     // keystream (key, nonce, first block counter0)
     hipError_t launch_cipher(int L, bool encrypt, std::size_t count, u32 const* key8, u32 const* nonce3, u32 counter0,

@@ -274,15 +274,115 @@ namespace hmpc_amd
                 store_planar<L>(out, 1, 0, acc);
         }
 
+        // Arithmetic of the matrix-vector products, as a policy: accumulator words W, a product
+        // added into an accumulator, the sum of two accumulators, `settle` (called at least every
+        // 16 products into one accumulator and before `merge`) and the canonical result.
+        //   mv_mont<L>: hmpc's Montgomery form for any odd q: each product reduced, each sum mod q
+        //   mv_r26:     q = 2^124 + c (4 limbs): radix-2^26 products (r26.hpp) summed without
+        //               reductions; a fold every 16 products keeps the limbs below 2^31
+        template<int L>
+        struct mv_mont
+        {
+            static constexpr int W = L;
+            using params = mod_params;
+            __device__ __forceinline__ static void product_add(u32 (&acc)[W], u32 const (&x)[L], u32 const (&y)[L],
+                                                               params const& P)
+            {
+                u32 t[L];
+                mod_mul<L>(t, x, y, P);
+                combine<L>(0, acc, t, P);
+            }
+            __device__ __forceinline__ static void settle(u32 (&)[W], params const&) {}
+            __device__ __forceinline__ static void merge(u32 (&acc)[W], u32 const (&o)[W], params const& P)
+            {
+                combine<L>(0, acc, o, P);
+            }
+            __device__ __forceinline__ static void finish(u32 (&out)[L], u32 const (&acc)[W], params const&)
+            {
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    out[l] = acc[l];
+            }
+        };
+
+        // Bounds: canonical inputs unpack to limbs < 2^26 (x_4 < 2^21), so each r26_mont product is
+        // < 2q (1 + 2^-19) with limbs < 2^26 (limb 4 < 2^22). A folded accumulator (< 2q, limbs
+        // < 2^28) plus 16 products keeps limbs < 2^28 + 2^30 < 2^31, the fold's input bound. The sum
+        // S of products a R b R 2^-130 is a b R / 4 summed: the result is 4 S, folded and made
+        // canonical.
+        struct mv_r26
+        {
+            static constexpr int W = 5;
+            using params = p26;
+            __device__ __forceinline__ static void product_add(u32 (&acc)[W], u32 const (&x)[4], u32 const (&y)[4],
+                                                               params const& P)
+            {
+                u32 a[5], b[5], t[5];
+                r26_unpack(a, x[0], x[1], x[2], x[3]);
+                r26_unpack(b, y[0], y[1], y[2], y[3]);
+                r26_mont(t, a, b, P);
+#pragma unroll
+                for (int l = 0; l < 5; ++l)
+                    acc[l] += t[l];
+            }
+            __device__ __forceinline__ static void settle(u32 (&acc)[W], params const& P) { r26_fold(acc, P); }
+            // both settled: limbs < 2^29, folded again
+            __device__ __forceinline__ static void merge(u32 (&acc)[W], u32 const (&o)[W], params const& P)
+            {
+#pragma unroll
+                for (int l = 0; l < 5; ++l)
+                    acc[l] += o[l];
+                r26_fold(acc, P);
+            }
+            __device__ __forceinline__ static void finish(u32 (&out)[4], u32 const (&acc)[W], params const& P)
+            {
+                u32 x[5];
+#pragma unroll
+                for (int l = 0; l < 5; ++l)
+                    x[l] = acc[l] << 2;  // settled limbs < 2^28
+                r26_fold(x, P);
+                r26_canonical(out, x, P);
+            }
+        };
+
+        // workgroup tree of settled accumulators through LDS; result in thread 0
+        template<typename V>
+        __device__ __forceinline__ void block_merge(u32 (&acc)[V::W], u32* lds, typename V::params const& P)
+        {
+            constexpr int W = V::W;
+            int t = threadIdx.x;
+#pragma unroll
+            for (int l = 0; l < W; ++l)
+                lds[l * ext_threads + t] = acc[l];
+            __syncthreads();
+            for (int h = ext_threads / 2; h > 0; h >>= 1)
+            {
+                if (t < h)
+                {
+                    u32 v[W];
+#pragma unroll
+                    for (int l = 0; l < W; ++l)
+                        v[l] = lds[l * ext_threads + t + h];
+                    V::merge(acc, v, P);
+#pragma unroll
+                    for (int l = 0; l < W; ++l)
+                        lds[l * ext_threads + t] = acc[l];
+                }
+                __syncthreads();
+            }
+        }
+
         // out[b][r] = sum_c M[b][r][c] v[b][c] (hmpc/expr/matrix_vector_product.hpp:92-133), one wave
         // per output row: lanes stride over the columns (coalesced per limb plane), then a
         // cross-lane tree. Matrix and vector strides of 0 broadcast them over the batch.
-        template<int L>
+        template<int L, typename V>
         __global__ void __launch_bounds__(ext_threads) k_matvec(u64 batch, u64 rows, u64 cols, u32 const* __restrict__ M,
                                                                 u64 m_count, u64 m_stride, u32 const* __restrict__ v,
-                                                                u64 v_count, u64 v_stride, u32* __restrict__ out, mod_params P)
+                                                                u64 v_count, u64 v_stride, u32* __restrict__ out,
+                                                                typename V::params P)
         {
             constexpr int waves = ext_threads / 64;
+            constexpr int W = V::W;
             u64 row = u64{blockIdx.x} * waves + threadIdx.x / 64;
             int lane = threadIdx.x & 63;
             if (row >= batch * rows)
@@ -291,11 +391,12 @@ namespace hmpc_amd
             u64 r = row - b * rows;
             u64 m0 = b * m_stride + r * cols;
             u64 v0 = b * v_stride;
-            // four columns in flight per lane (independent loads and accumulators)
-            u32 acc[L] = {};
-            u32 acc2[L] = {};
+            // four columns in flight per lane (independent loads and two accumulators, 2 products
+            // each per iteration: settled every 8 iterations)
+            u32 acc[W] = {};
+            u32 acc2[W] = {};
             u64 c = lane;
-            for (; c + 192 < cols; c += 256)
+            for (int it = 1; c + 192 < cols; c += 256, ++it)
             {
                 u32 x[4][L], y[4][L];
 #pragma unroll
@@ -306,32 +407,40 @@ namespace hmpc_amd
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
+                    V::product_add((u & 1) ? acc2 : acc, x[u], y[u], P);
+                if ((it & 7) == 0)
                 {
-                    u32 t[L];
-                    mod_mul<L>(t, x[u], y[u], P);
-                    combine<L>(0, (u & 1) ? acc2 : acc, t, P);
+                    V::settle(acc, P);
+                    V::settle(acc2, P);
                 }
             }
+            V::settle(acc, P);
+            V::settle(acc2, P);
+            // at most 3 more products per lane before the next settle
             for (; c < cols; c += 64)
             {
-                u32 x[L], y[L], t[L];
+                u32 x[L], y[L];
                 load_planar<L>(x, M, m_count, m0 + c);
                 load_planar<L>(y, v, v_count, v0 + c);
-                mod_mul<L>(t, x, y, P);
-                combine<L>(0, acc, t, P);
+                V::product_add(acc, x, y, P);
             }
-            combine<L>(0, acc, acc2, P);
+            V::settle(acc, P);
+            V::merge(acc, acc2, P);
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1)
             {
-                u32 o[L];
+                u32 o[W];
 #pragma unroll
-                for (int l = 0; l < L; ++l)
+                for (int l = 0; l < W; ++l)
                     o[l] = static_cast<u32>(__shfl_xor(static_cast<int>(acc[l]), off, 64));
-                combine<L>(0, acc, o, P);
+                V::merge(acc, o, P);
             }
             if (lane == 0)
-                store_planar<L>(out, batch * rows, row, acc);
+            {
+                u32 res[L];
+                V::finish(res, acc, P);
+                store_planar<L>(out, batch * rows, row, res);
+            }
         }
     }
 
@@ -339,23 +448,24 @@ namespace hmpc_amd
     {
         // long rows: one workgroup per output row, the columns strided over all its threads (more
         // loads in flight per row), then the workgroup tree
-        template<int L>
+        template<int L, typename V>
         __global__ void __launch_bounds__(ext_threads) k_matvec_wide(u64 rows, u64 cols, u32 const* __restrict__ M,
                                                                      u64 m_count, u64 m_stride, u32 const* __restrict__ v,
                                                                      u64 v_count, u64 v_stride, u32* __restrict__ out,
-                                                                     u64 out_count, mod_params P)
+                                                                     u64 out_count, typename V::params P)
         {
-            __shared__ u32 lds[L * ext_threads];
+            constexpr int W = V::W;
+            __shared__ u32 lds[W * ext_threads];
             u64 row = blockIdx.x;
             u64 b = row / rows;
             u64 r = row - b * rows;
             u64 m0 = b * m_stride + r * cols;
             u64 v0 = b * v_stride;
-            u32 acc[L] = {};
-            u32 acc2[L] = {};
+            u32 acc[W] = {};
+            u32 acc2[W] = {};
             u64 c = threadIdx.x;
             constexpr u64 T = ext_threads;
-            for (; c + 3 * T < cols; c += 4 * T)
+            for (int it = 1; c + 3 * T < cols; c += 4 * T, ++it)
             {
                 u32 x[4][L], y[4][L];
 #pragma unroll
@@ -366,24 +476,48 @@ namespace hmpc_amd
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
+                    V::product_add((u & 1) ? acc2 : acc, x[u], y[u], P);
+                if ((it & 7) == 0)
                 {
-                    u32 t[L];
-                    mod_mul<L>(t, x[u], y[u], P);
-                    combine<L>(0, (u & 1) ? acc2 : acc, t, P);
+                    V::settle(acc, P);
+                    V::settle(acc2, P);
                 }
             }
+            V::settle(acc, P);
+            V::settle(acc2, P);
             for (; c < cols; c += T)
             {
-                u32 x[L], y[L], t[L];
+                u32 x[L], y[L];
                 load_planar<L>(x, M, m_count, m0 + c);
                 load_planar<L>(y, v, v_count, v0 + c);
-                mod_mul<L>(t, x, y, P);
-                combine<L>(0, acc, t, P);
+                V::product_add(acc, x, y, P);
             }
-            combine<L>(0, acc, acc2, P);
-            block_reduce<L>(0, acc, lds, P);
+            V::settle(acc, P);
+            V::merge(acc, acc2, P);
+            block_merge<V>(acc, lds, P);
             if (threadIdx.x == 0)
-                store_planar<L>(out, out_count, row, acc);
+            {
+                u32 res[L];
+                V::finish(res, acc, P);
+                store_planar<L>(out, out_count, row, res);
+            }
+        }
+
+        template<int L, typename V>
+        hipError_t matvec_with(typename V::params const& P, std::size_t batch, std::size_t rows, std::size_t cols,
+                               u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v, std::size_t v_count,
+                               std::size_t v_stride, u32* out, hipStream_t s)
+        {
+            if (cols >= 1024)
+            {
+                k_matvec_wide<L, V><<<static_cast<unsigned>(batch * rows), ext_threads, 0, s>>>(
+                    rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, batch * rows, P);
+                return hipGetLastError();
+            }
+            std::size_t waves = ext_threads / 64;
+            unsigned grid = static_cast<unsigned>((batch * rows + waves - 1) / waves);
+            k_matvec<L, V><<<grid, ext_threads, 0, s>>>(batch, rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, P);
+            return hipGetLastError();
         }
     }
 
@@ -418,22 +552,15 @@ namespace hmpc_amd
         });
     }
 
-    hipError_t launch_matvec(int L, mod_params const& P, std::size_t batch, std::size_t rows, std::size_t cols,
-                             u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v, std::size_t v_count,
-                             std::size_t v_stride, u32* out, hipStream_t s)
+    hipError_t launch_matvec(int L, mod_params const& P, p26 const* r26, std::size_t batch, std::size_t rows,
+                             std::size_t cols, u32 const* M, std::size_t m_count, std::size_t m_stride, u32 const* v,
+                             std::size_t v_count, std::size_t v_stride, u32* out, hipStream_t s)
     {
+        if (r26 && L == 4)
+            return matvec_with<4, mv_r26>(*r26, batch, rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, s);
         return ext_limbs(L, [&](auto LC) {
             constexpr int LL = decltype(LC)::value;
-            if (cols >= 1024)
-            {
-                k_matvec_wide<LL><<<static_cast<unsigned>(batch * rows), ext_threads, 0, s>>>(
-                    rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, batch * rows, P);
-                return hipGetLastError();
-            }
-            std::size_t waves = ext_threads / 64;
-            unsigned grid = static_cast<unsigned>((batch * rows + waves - 1) / waves);
-            k_matvec<LL><<<grid, ext_threads, 0, s>>>(batch, rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, P);
-            return hipGetLastError();
+            return matvec_with<LL, mv_mont<LL>>(P, batch, rows, cols, M, m_count, m_stride, v, v_count, v_stride, out, s);
         });
     }
 

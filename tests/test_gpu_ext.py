@@ -247,3 +247,23 @@ def _check_matvec(gm, om, batch, rows, cols, Rinv, q):
         for r in range(rows):
             want = sum(mi[r * cols + k] * vi[bb * cols + k] for k in range(cols)) * Rinv % q
             assert out[bb * rows + r] == want, (bb, r)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_matvec_long_rows_and_largest_values(ctx, variant):
+    """q = 2^124 + c: the radix-2^26 product sums (variant 0: reductions only every 16 products per
+    accumulator) and the portable Montgomery kernels (variant 1) give the same canonical results:
+    rows long enough for many settles in both kernels (one wave and one workgroup per row), and
+    matrices of q - 1 only (the largest canonical value in every product and partial sum)"""
+    q = Q128A
+    gm, om = hmpc_amd.Modulus(ctx, q), oracle.Modulus(q)
+    gm.set_kernels(variant)
+    Rinv = pow(1 << 128, -1, q)
+    say(f"matvec long rows variant={variant}")
+    for batch, rows, cols in ((2, 3, 1000), (1, 2, 40000)):
+        _check_matvec(gm, om, batch, rows, cols, Rinv, q)
+    for rows, cols in ((4, 1000), (2, 40000)):
+        top = oracle.ints_to_planar([q - 1] * (rows * cols), 4)
+        vec = oracle.ints_to_planar([q - 1] * cols, 4)
+        out = oracle.planar_to_ints(gm.download(gm.matvec(1, rows, cols, gm.upload(top), gm.upload(vec)), rows))
+        assert out == [cols * (q - 1) * (q - 1) * Rinv % q] * rows, (rows, cols)
