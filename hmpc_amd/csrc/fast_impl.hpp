@@ -76,7 +76,7 @@ namespace hmpc_amd
                                                                 u32 const (&xin)[rl], bool canonical, params const& P)
             {
                 u32 x[rl] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
-                r26_fold(x, P);
+                r26_fold_signed(x, P);  // forward results are signed digits
                 u32 w[4];
                 if (canonical)
                     r26_canonical(w, x, P);
@@ -107,7 +107,7 @@ namespace hmpc_amd
             {
                 u32 x[rl] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
                 if (fold)
-                    r26_fold(x, P);
+                    r26_fold_signed(x, P);
 #pragma unroll
                 for (int l = 0; l < rl; ++l)
                     dst[l * count + g] = x[l];
@@ -142,17 +142,10 @@ namespace hmpc_amd
             }
 
             // (x, y) -> (x + t, x - t + 3q), t = mont(y, w)
+            // (x, y) -> (x + t, x - t), t = mont(y, w), on signed digits
             __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
             {
-                u32 t[rl];
-                r26_mont(t, y, w, P);
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                {
-                    u32 xl = x[l];
-                    x[l] = xl + t[l];
-                    y[l] = xl + (P.k3[l] - t[l]);
-                }
+                r26_forward(x, y, w, P);
             }
 
             // (x, y) -> (x + y, mont(x - y + 32q, w)), x + y folded when `fold` (odd stages)
@@ -185,11 +178,12 @@ namespace hmpc_amd
                 r26_mont(y, d, w1, P);
             }
 
-            // point-wise product of two data values a R, b R -> a b R (R' / R = 4 correction)
+            // point-wise product of two forward results a R, b R (signed digits) -> a b R (R' / R = 4
+            // correction)
             __device__ __forceinline__ static void point(u32 (&a)[rl], u32 (&b)[rl], params const& P)
             {
-                r26_fold(a, P);
-                r26_fold(b, P);
+                r26_fold_signed(a, P);
+                r26_fold_signed(b, P);
                 u32 t[rl];
                 r26_mont(t, a, b, P);
 #pragma unroll
@@ -197,14 +191,14 @@ namespace hmpc_amd
                     a[l] = t[l] << 2;
             }
 
-            // x = k R + x for k canonical (limbs < 2^26) and u, x forward-transform outputs
-            // (< 52 q, limbs < 2^31): mont(u, k) < 2q needs v(u) v(k) < 2^130 q, which holds
-            // without folding u; 4 mont(u, k) adds limbs < 2^28
+            // x = k R + x for k canonical (limbs < 2^26) and u, x forward-transform outputs (signed
+            // digits, |value| < 35 q, |limbs| < 19 2^26): mont(u, k) lies in (-q, 2q) without folding
+            // u; 4 mont(u, k) adds digits < 2^28 (a signed top limb < 2^24 in magnitude)
             __device__ __forceinline__ static void point_add(u32 (&x)[rl], u32 const (&u)[rl], u32 const (&k)[rl],
                                                              params const& P)
             {
                 u32 t[rl];
-                r26_mont(t, u, k, P);
+                r26_mont_signed(t, u, k, P);
 #pragma unroll
                 for (int l = 0; l < rl; ++l)
                     x[l] += t[l] << 2;
@@ -834,10 +828,10 @@ namespace hmpc_amd
             }
         }
 
-        // The lazy forward transform runs up to 19 stages without reductions (r26.hpp); a
-        // forward column result of a longer transform is folded before it goes to the workspace.
-        // (No prime 2^124 + c with c < 2^24 has 2^19 | q - 1, so radix-2^26 transforms stop at
-        // N = 2^17 and the fold is a guard only.)
+        // The lazy forward transform runs up to 29 stages without reductions (r26.hpp); a
+        // forward column result of a transform longer than 18 stages is folded before it goes to
+        // the workspace. (No prime 2^124 + c with c < 2^24 has 2^19 | q - 1, so radix-2^26
+        // transforms stop at N = 2^17 and the fold is a guard only.)
         __host__ __device__ __forceinline__ bool ws_fold_columns(int log_n) { return log_n > 18; }
 
         struct pass_geom

@@ -4,33 +4,43 @@
 // Why: on gfx950 v_mad_u64_u32 runs at full rate, but 32-bit-limb Montgomery code spends most
 // of its instructions on 64-bit register-pair moves and carry adds around each mad. With
 // 26-bit limbs every partial product accumulates in place into a 64-bit column sum
-// (acc = a * b + acc: one v_mad_u64_u32, no carries), and additions are limb-wise.
+// (acc = a * b + acc: one 64-bit mad, no carries), and additions are limb-wise.
 //
-// Representation: x = sum_j x_j 2^(26 j), j = 0..4, limbs x_j < 2^32 (redundant). Internal
-// Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in hmpc's
-// representation a*R mod q (R = 2^128); twiddles are stored as w * 2^130 mod q so that
+// Representation: x = sum_j x_j 2^(26 j), j = 0..4, limbs x_j held in 32 bits (redundant). The
+// forward transform's values are signed digits: two's-complement limbs |x_j| < 2^31, so that
+// its butterfly (x + t, x - t) needs no added multiple of q; every other value has limbs in
+// [0, 2^31). Internal Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in
+// hmpc's representation a*R mod q (R = 2^128); twiddles are stored as w * 2^130 mod q so that
 // mont(a R, w 2^130) = (a w) R. A product of two data values is corrected by 4 = R' / R.
 //
 // Bounds (q > 2^124, so 64 q > 2^130):
-//   mont(a, b): column sums < 2^61 when a_j < 2^31 and b_j < 2^27; result < 2q (1 + 2^-19) with
-//               normalized limbs (r_4 < 2^22) whenever v(a) v(b) < 2^130 q
-//   forward butterfly (x + t, x - t + K3), t = mont(y, w) < 2q (1 + 2^-19): value +3q per stage,
-//               limb 0 +2^27, limbs 1..3 +2^26, limb 4 +2^22. From inputs < 4q with limbs
-//               (2^27, 2^26, 2^26, 2^26, 2^22) (canonical data, or from_signed samples) this
-//               stays below 4q + 3q S and limbs below (S + 1) x those after S stages: a mont
-//               input needs a value < 2^130 > 63.9 q and limbs < 2^32, so up to 19 stages (a
-//               whole transform of N <= 2^19, both of its passes) run without reductions
+//   mont(a, b): a signed-digit (|a_j| < 2^31), b in [0, q) with limbs < 2^27: column sums stay
+//               below 2^61 in magnitude; the result (T + M q) / 2^130 has digits r_0..r_3 in
+//               [0, 2^26) and a signed top limb, and lies in (-|a| q / 2^130, |a| q / 2^130 +
+//               q (1 + 2^-19)): within (-q, 2q (1 + 2^-19)) for |a| < 64 q, in [0, 2q (1 + 2^-19))
+//               with r_4 < 2^22 for a >= 0, v(a) v(b) < 2^130 q
+//   forward butterfly (x + t, x - t), t = mont(y, w): |value| grows by < 2q (1 + 2^-18) per stage,
+//               limbs 0..3 by < 2^26 and limb 4 by < 2^22. From inputs in [0, q) with limbs
+//               (< 2^27, < 2^26, < 2^26, < 2^26, < 2^22) (canonical data, or from_signed
+//               samples) S stages stay below q + 2.0001 q S in magnitude with |limbs| < (S + 2) x
+//               those: a mont input needs |value| < 64 q and |limbs| < 2^31, so up to 29 stages
+//               (a whole transform of N <= 2^29, both of its passes) run without reductions
 //   fold(x):    normalize, then x_4 >= 2^20 carries h = x_4 >> 20 copies of 2^124 = q - c:
-//               replace by -h c (plus one q in borrowed digits to keep limbs >= 0);
-//               result < 2q with limbs < 2^28, for any input limbs < 2^32
+//               replace by -h c (plus one q in borrowed digits to keep limbs >= 0); result < 2q
+//               with limbs < 2^28, for any limbs < 2^31. fold_signed adds 64 q first, so that a
+//               signed-digit value (|value| < 64 q, |x_j| < 2^31 - 2^26, |x_4| < 2^29) folds the
+//               same way
 //   inverse butterfly (x + y, mont(x - y + K32, w)), with x + y folded at odd stages of the
-//               transform: at odd stages the inputs are < 4q (1 + 2^-19) with limbs < 2^29 (one
-//               unfolded sum of folded values), at even stages < 2q (1 + 2^-19) with limbs
-//               < 2^28; from products (< 8q, limbs < 2^28 after the x4 correction) stage 0
-//               gives < 16q, limbs < 2^29. Every mont input stays < 48q with limbs < 2^31.
+//               transform (inputs are products or canonical data, never signed): at odd stages
+//               the inputs are < 4q (1 + 2^-19) with limbs < 2^29 (one unfolded sum of folded
+//               values), at even stages < 2q (1 + 2^-19) with limbs < 2^28; from products (< 8q,
+//               limbs < 2^28 after the x4 correction) stage 0 gives < 16q, limbs < 2^29. Every
+//               mont input stays < 48q with limbs < 2^31.
 #pragma once
 
 #include "modarith.hpp"
+
+#include <type_traits>
 
 namespace hmpc_amd
 {
@@ -40,12 +50,11 @@ namespace hmpc_amd
     {
         u32 c;         // q = 2^124 + c
         u32 n26;       // -q^-1 mod 2^26
-        u32 k3[5];     // 3q with limbs k3_0 >= 2^26, k3_{1..3} = 2^26 - 1, k3_4 = 3 2^20 - 1
-                       // (>= mont output limbs: r_4 <= 2^21 + 4 for r < 2q (1 + 2^-19))
         u32 k4[5];     // 4q with limbs k4_0 >= 2^26, k4_{1..3} = 2^26 - 1 (>= mont output limbs)
         u32 k32[5];    // 32q with limbs >= 12 2^26 - 12 > 2^29 (>= the limbs of a folded value and
                        // of one unfolded sum of two)
         u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
+        u32 b64[2];    // 64 c as two radix-2^26 limbs (with 2^26 in limb 4: 64 q, the signed fold's bias)
         u32 q32[4];    // q as 32-bit limbs (final canonical subtraction)
         u32 rr[5];     // R 2^130 mod q (R = 2^128), normalized: mont(z, rr) = z R mod q
         u32 t20;       // 2^20 (top radix-2^26 limb of q), kept in a register so that m * 2^20 + T
@@ -60,18 +69,18 @@ namespace hmpc_amd
         for (int i = 0; i < 6; ++i)
             inv *= 2u - c * inv;  // c^-1 mod 2^32 (q = c mod 2^32)
         R.n26 = (0u - inv) & m26;
-        u32 const k3[5] = {3 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, 3 * (1u << 20) - 1};
         u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
         u32 const k32[5] = {32 * c + 12 * (1u << 26), 12 * (1u << 26) - 12, 12 * (1u << 26) - 12, 12 * (1u << 26) - 12,
                             (1u << 25) - 12};
         u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
         for (int i = 0; i < 5; ++i)
         {
-            R.k3[i] = k3[i];
             R.k4[i] = k4[i];
             R.k32[i] = k32[i];
             R.qb[i] = qb[i];
         }
+        R.b64[0] = (64u * c) & m26;
+        R.b64[1] = (64u * c) >> 26;
         R.q32[0] = c;
         R.q32[1] = 0;
         R.q32[2] = 0;
@@ -97,7 +106,7 @@ namespace hmpc_amd
         w[3] = (x[3] >> 18) | (x[4] << 8);
     }
 
-    // carry-propagate limbs 0..3 into limb 4 (limb 4 keeps the excess)
+    // carry-propagate limbs 0..3 into limb 4 (limb 4 keeps the excess), limbs in [0, 2^32)
     HMPC_HD void r26_normalize(u32 (&x)[5])
     {
         x[1] += x[0] >> 26;
@@ -110,10 +119,10 @@ namespace hmpc_amd
         x[3] &= m26;
     }
 
-    // value -> congruent value < 2q, limbs < 2^28, for any limbs < 2^31
-    HMPC_HD void r26_fold(u32 (&x)[5], p26 const& P)
+    // the fold's reduction of a normalized value with x_4 in [0, 2^31 + 2^5): result in [0, 2q),
+    // limbs < 2^28
+    HMPC_HD void r26_fold_top(u32 (&x)[5], p26 const& P)
     {
-        r26_normalize(x);  // x_4 < 2^31 + 2^5
         u32 h = x[4] >> 20;
         x[4] &= (1u << 20) - 1u;
         u64 hc = static_cast<u64>(h) * P.c;  // < 2^12 * 2^24
@@ -126,10 +135,39 @@ namespace hmpc_amd
         x[4] += P.qb[4];
     }
 
-    // Montgomery product a b 2^-130 mod q (lazy: < 2q, normalized limbs)
-    HMPC_HD void r26_mont(u32 (&r)[5], u32 const (&a)[5], u32 const (&b)[5], p26 const& P)
+    // value -> congruent value < 2q, limbs < 2^28, for any limbs < 2^31
+    HMPC_HD void r26_fold(u32 (&x)[5], p26 const& P)
     {
-        u64 T[9];
+        r26_normalize(x);  // x_4 < 2^31 + 2^5
+        r26_fold_top(x, P);
+    }
+
+    // signed-digit value (|x_j| < 2^31 - 2^26 for j < 4, |x_4| < 2^29, |value| < 64 q) -> congruent
+    // value in [0, 2q), limbs < 2^28: 64 q = 2^130 + 64 c is added first (limb 4 += 2^26, 64 c into
+    // limbs 0 and 1), which makes the value positive; the signed carry chain then leaves digits
+    // 0..3 in [0, 2^26) and limb 4 in [0, 2^28)
+    HMPC_HD void r26_fold_signed(u32 (&x)[5], p26 const& P)
+    {
+        x[0] += P.b64[0];
+        x[1] += P.b64[1];
+        x[4] += 1u << 26;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            x[j + 1] += static_cast<u32>(static_cast<int>(x[j]) >> 26);
+            x[j] &= m26;
+        }
+        r26_fold_top(x, P);
+    }
+
+    // Montgomery product a b 2^-130 mod q. Signed: a has signed digits (|a_j| < 2^31), else limbs
+    // in [0, 2^31); b has limbs in [0, 2^27). Result digits 0..3 in [0, 2^26); the top limb is
+    // signed for a signed a, < 2^22 otherwise (bounds in the header).
+    template<bool Signed>
+    HMPC_HD void r26_mont_t(u32 (&r)[5], u32 const (&a)[5], u32 const (&b)[5], p26 const& P)
+    {
+        using acc = typename std::conditional<Signed, long long, u64>::type;
+        acc T[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             T[k] = 0;
@@ -137,22 +175,30 @@ namespace hmpc_amd
         for (int i = 0; i < 5; ++i)
 #pragma unroll
             for (int j = 0; j < 5; ++j)
-                T[i + j] = static_cast<u64>(a[i]) * b[j] + T[i + j];
+            {
+                if constexpr (Signed)
+                    T[i + j] = static_cast<long long>(static_cast<int>(a[i])) * static_cast<long long>(static_cast<int>(b[j])) +
+                               T[i + j];
+                else
+                    T[i + j] = static_cast<u64>(a[i]) * b[j] + T[i + j];
+            }
         // q in radix 2^26: limb 0 = c, limb 4 = 2^20, others 0. Only the last quotient digit is
         // masked to 26 bits: with m_0..m_3 < 2^32 the quotient M = sum m_i 2^(26 i) stays below
-        // 2^130 + 2^111, so the result (T + M q) / 2^130 keeps its bound (< 2q (1 + 2^-19),
-        // normalized limbs, r_4 < 2^22) and each round saves the mask
+        // 2^130 + 2^111, so the result (T + M q) / 2^130 keeps its bound and each round saves the
+        // mask. For a signed a, m q is added as unsigned 64-bit products onto the two's-complement
+        // column sums (exact: no sum leaves (-2^62, 2^62)); the low 26 bits of T_i + m_i c are
+        // zero, so the (arithmetic) shift carries exactly.
 #pragma unroll
         for (int i = 0; i < 5; ++i)
         {
             u32 m = static_cast<u32>(T[i]) * P.n26;
             if (i == 4)
                 m &= m26;
-            T[i] = static_cast<u64>(m) * P.c + T[i];
-            T[i + 4] = static_cast<u64>(m) * P.t20 + T[i + 4];
+            T[i] = static_cast<acc>(static_cast<u64>(m) * P.c + static_cast<u64>(T[i]));
+            T[i + 4] = static_cast<acc>(static_cast<u64>(m) * P.t20 + static_cast<u64>(T[i + 4]));
             T[i + 1] += T[i] >> 26;
         }
-        u64 c = T[5];
+        acc c = T[5];
         r[0] = static_cast<u32>(c) & m26;
         c = T[6] + (c >> 26);
         r[1] = static_cast<u32>(c) & m26;
@@ -161,6 +207,32 @@ namespace hmpc_amd
         c = T[8] + (c >> 26);
         r[3] = static_cast<u32>(c) & m26;
         r[4] = static_cast<u32>(c >> 26);
+    }
+
+    // non-negative operands (inverse transforms, point-wise products of folded values)
+    HMPC_HD void r26_mont(u32 (&r)[5], u32 const (&a)[5], u32 const (&b)[5], p26 const& P)
+    {
+        r26_mont_t<false>(r, a, b, P);
+    }
+
+    // a signed-digit first operand (forward-transform values)
+    HMPC_HD void r26_mont_signed(u32 (&r)[5], u32 const (&a)[5], u32 const (&b)[5], p26 const& P)
+    {
+        r26_mont_t<true>(r, a, b, P);
+    }
+
+    // forward (Cooley-Tukey) butterfly on signed digits: (x, y) -> (x + t, x - t), t = mont(y, w)
+    HMPC_HD void r26_forward(u32 (&x)[5], u32 (&y)[5], u32 const (&w)[5], p26 const& P)
+    {
+        u32 t[5];
+        r26_mont_signed(t, y, w, P);
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+        {
+            u32 xl = x[l];
+            x[l] = xl + t[l];
+            y[l] = xl - t[l];
+        }
     }
 
     // signed integer z -> a value congruent to z R mod q (R = 2^128) with limbs

@@ -1,10 +1,12 @@
 // Host build of the device arithmetic in hmpc_amd/csrc/r26.hpp, driven by tests/test_r26_host.py.
 // Reads commands from stdin, prints results; python checks them with big integers.
-//   M c a0..a4 b0..b4      -> r26_mont limbs
+//   M c a0..a4 b0..b4      -> r26_mont_signed limbs (a: signed 32-bit limbs; the result's top limb signed)
+//   U c a0..a4 b0..b4      -> r26_mont limbs (non-negative a)
 //   F c x0..x4             -> r26_fold limbs
+//   G c x0..x4             -> r26_fold_signed limbs (x: signed 32-bit limbs)
 //   C c x0..x4             -> r26_canonical 32-bit words
 //   S c z                  -> r26_from_signed limbs (z a signed 64-bit integer)
-//   B c stages x0..x4 y0..y4 w0..w4 (per stage: w limbs)  -> forward butterfly chain
+//   B c stages x0..x4 y0..y4 w0..w4 (per stage: w limbs)  -> forward butterfly chain (signed limbs)
 #include "../../hmpc_amd/csrc/r26.hpp"
 
 #include <cstdio>
@@ -31,10 +33,35 @@ int main()
         if (cmd == "M")
         {
             u32 a[5], b[5], r[5];
+            for (auto& v : a)
+            {
+                long long z;
+                std::cin >> z;
+                v = static_cast<u32>(z);
+            }
+            for (auto& v : b) std::cin >> v;
+            r26_mont_signed(r, a, b, P);
+            printf("%u %u %u %u %d\n", r[0], r[1], r[2], r[3], static_cast<int>(r[4]));
+        }
+        else if (cmd == "U")
+        {
+            u32 a[5], b[5], r[5];
             for (auto& v : a) std::cin >> v;
             for (auto& v : b) std::cin >> v;
             r26_mont(r, a, b, P);
             printf("%u %u %u %u %u\n", r[0], r[1], r[2], r[3], r[4]);
+        }
+        else if (cmd == "G")
+        {
+            u32 x[5];
+            for (auto& v : x)
+            {
+                long long z;
+                std::cin >> z;
+                v = static_cast<u32>(z);
+            }
+            r26_fold_signed(x, P);
+            printf("%u %u %u %u %u\n", x[0], x[1], x[2], x[3], x[4]);
         }
         else if (cmd == "F")
         {
@@ -60,7 +87,7 @@ int main()
         }
         else if (cmd == "B" || cmd == "I")
         {
-            // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t + 3q), t = mont(y, w),
+            // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t), t = mont(y, w) (signed digits),
             // or inverse (x + y, mont(x - y + 32q, w)) with x + y folded on request. Per stage: the
             // twiddle limbs, then for "B" a selector of the next stage's pair (0: (x', y'),
             // 1: (y', y'), 2: (x', x')), for "I" the fold flag and a selector (0: (x', y'),
@@ -68,8 +95,13 @@ int main()
             int stages;
             std::cin >> stages;
             u32 x[5], y[5];
-            for (auto& v : x) std::cin >> v;
-            for (auto& v : y) std::cin >> v;
+            for (auto* p : {&x, &y})
+                for (auto& v : *p)
+                {
+                    long long z;
+                    std::cin >> z;
+                    v = static_cast<u32>(z);
+                }
             for (int s = 0; s < stages; ++s)
             {
                 u32 w[5];
@@ -78,14 +110,7 @@ int main()
                 {
                     int sel;
                     std::cin >> sel;
-                    u32 t[5];
-                    r26_mont(t, y, w, P);
-                    for (int l = 0; l < 5; ++l)
-                    {
-                        u32 xl = x[l];
-                        x[l] = xl + t[l];
-                        y[l] = xl + (P.k3[l] - t[l]);
-                    }
+                    r26_forward(x, y, w, P);
                     for (int l = 0; l < 5; ++l)
                     {
                         if (sel == 1)
@@ -115,8 +140,12 @@ int main()
                             x[l] = y[l];
                     }
                 }
-                printf("%u %u %u %u %u %u %u %u %u %u%s", x[0], x[1], x[2], x[3], x[4], y[0], y[1], y[2], y[3], y[4],
-                       s + 1 < stages ? " " : "\n");
+                // forward values are signed digits, inverse ones are non-negative (either prints
+                // right as a signed 32-bit limb: inverse limbs stay below 2^31)
+                for (int l = 0; l < 5; ++l)
+                    printf("%d ", static_cast<int>(x[l]));
+                for (int l = 0; l < 5; ++l)
+                    printf("%d%s", static_cast<int>(y[l]), l < 4 ? " " : (s + 1 < stages ? " " : "\n"));
             }
         }
         fflush(stdout);

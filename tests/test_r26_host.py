@@ -1,9 +1,10 @@
 """CPU checks of the radix-2^26 device arithmetic (hmpc_amd/csrc/r26.hpp) built for the host.
 
 The same header is compiled into the gfx950 kernels; here it runs on the CPU against python
-big integers, including the lazy bounds the fast transform relies on: forward stages add at
-most 3q and at most one input-limb bound per limb, so 19 stages from inputs < 4q keep every mont
-input below 2^130 with limbs < 2^32; inverse stages stay below 2q.
+big integers, including the lazy bounds the fast transform relies on: forward stages work on
+signed digits and add less than 2q (1 + 2^-18) in magnitude and at most one digit bound per limb,
+so 29 stages from inputs in [0, q) keep every mont input within 64 q with limbs below 2^31 in
+magnitude; inverse stages stay below 2q.
 """
 
 import os
@@ -62,21 +63,54 @@ def redundant(v, rng, max_bits):
     return l
 
 
+def signed_redundant(v, rng, max_abs):
+    """a random signed-digit representation of v (any sign): limbs |x_j| < max_abs"""
+    l = [(v >> (26 * j)) & M26 for j in range(4)] + [v >> 104]
+    for j in range(1, 5):
+        # borrow k from limb j into limb j - 1 (k < 0 lends the other way), keeping the bound
+        for _ in range(8):
+            k = rng.randrange(-15, 16)
+            a, b = l[j - 1] + (k << 26), l[j] - k
+            if abs(a) < max_abs and abs(b) < max_abs:
+                l[j - 1], l[j] = a, b
+                break
+    assert value(l) == v
+    return l
+
+
 @pytest.mark.parametrize("q", PRIMES)
 def test_mont_fold_canonical(tool, q):
     c = q - (1 << 124)
     rng = random.Random(q & 0xFFFF)
     R = 1 << 130
-    for _ in range(400):
-        a = rng.randrange(50 * q)
+    for it in range(600):
         b = rng.randrange(q)
-        la = redundant(a, rng, 31)
         lb = limbs(b)
+        if it % 2 == 0:
+            a = rng.randrange(50 * q)
+            la = redundant(a, rng, 31)
+        else:  # a signed-digit operand, as the forward transform's values
+            a = rng.randrange(-50 * q, 50 * q)
+            la = signed_redundant(a, rng, 1 << 31)
         r = run(tool, f"M {c} " + " ".join(map(str, la + lb)))
         v = value(r)
         assert v % q == a * b * pow(R, -1, q) % q
-        # quotient digits m_0..m_3 are not masked: bound 2q (1 + 2^-19)
-        assert v < 2 * q + (q >> 18) and all(x < (1 << 26) for x in r[:4]) and r[4] < (1 << 22)
+        if min(la) >= 0:  # the unsigned product (inverse transforms) gives the same limbs
+            assert run(tool, f"U {c} " + " ".join(map(str, la + lb))) == r
+        # quotient digits m_0..m_3 are not masked: bound 2q (1 + 2^-19); digits 0..3 in [0, 2^26)
+        assert all(0 <= x < (1 << 26) for x in r[:4]) and abs(r[4]) < (1 << 22)
+        if a >= 0:
+            assert 0 <= v < 2 * q + (q >> 18)
+        else:
+            assert -q < v < 2 * q + (q >> 18)
+        # the signed fold of a signed-digit value
+        y = rng.randrange(-63 * q, 63 * q)
+        ly = signed_redundant(y, rng, (1 << 31) - (1 << 26))
+        if abs(ly[4]) < (1 << 29):
+            g = run(tool, f"G {c} " + " ".join(map(str, ly)))
+            assert value(g) % q == y % q and 0 <= value(g) < 2 * q and all(0 <= v < (1 << 28) for v in g)
+            w = run(tool, f"C {c} " + " ".join(map(str, g)))
+            assert sum(v << (32 * i) for i, v in enumerate(w)) == y % q
         x = rng.randrange(1 << 135)
         lx = redundant(x, rng, 31) if x < (1 << 133) else limbs(x)
         if max(lx) >= (1 << 31):
@@ -102,32 +136,34 @@ def test_from_signed(tool, q):
         assert v <= q and x[0] < (1 << 27) and all(l < (1 << 26) for l in x[1:4]) and x[4] < (1 << 20)
 
 
-LIMB_STEP = (1 << 27, 1 << 26, 1 << 26, 1 << 26, 1 << 22)  # per-stage limb growth bound (and input bound)
+LIMB_IN = (1 << 27, 1 << 26, 1 << 26, 1 << 26, 1 << 22)    # forward input limbs (and from_signed's)
+LIMB_STEP = (1 << 26, 1 << 26, 1 << 26, 1 << 26, 1 << 22)  # per-stage limb growth bound (digits of t)
 
 
-def _signed_sample(q, rng):
-    """a from_signed-shaped input: z R mod q for |z| small, negatives as 4q - t (limbs of k4 - t)"""
-    v = rng.randrange(2 * q)
+def _forward_input(q, rng):
+    """a forward-transform input: canonical limbs of v < q, or from_signed's shape q - t (limbs
+    qb - t, t < 2^92)"""
     if rng.random() < 0.5:
-        return limbs(v)
+        return limbs(rng.randrange(q))
     c = q - (1 << 124)
-    k4 = [4 * c + (1 << 26), (1 << 26) - 1, (1 << 26) - 1, (1 << 26) - 1, (1 << 22) - 1]
-    t = limbs(v)
-    return [k - x for k, x in zip(k4, t)]
+    qb = [c + (1 << 26), (1 << 26) - 1, (1 << 26) - 1, (1 << 26) - 1, (1 << 20) - 1]
+    t = limbs(rng.randrange(1 << 92))
+    return [k - x for k, x in zip(qb, t)]
 
 
 @pytest.mark.parametrize("q", PRIMES)
-@pytest.mark.parametrize("stages", [12, 16, 19])
+@pytest.mark.parametrize("stages", [12, 17, 29])
 def test_forward_chain_bounds(tool, q, stages):
-    """Lazy forward stages from inputs < 4q (canonical or from_signed-shaped), each stage keeping
-    either output pair or a doubled output (the +3q branch twice, the worst value growth): after
-    s stages every value is < 4q + 3q s with limbs < (s + 1) x LIMB_STEP, every mont input is
-    < 2^130, and the results are exact mod q."""
+    """Lazy forward stages on signed digits from inputs in [0, q) (canonical or from_signed-shaped),
+    each stage keeping either output pair or a doubled output: after s stages every value is
+    within q + 2q (1 + 2^-18) s in magnitude with limbs below LIMB_IN + s x LIMB_STEP in
+    magnitude, so every mont input is within 64 q with limbs below 2^31; the results are exact
+    mod q, and the last ones fold (signed) to [0, 2q) with limbs < 2^28."""
     c = q - (1 << 124)
     rng = random.Random(7 + stages)
     R = 1 << 130
     for it in range(150):
-        lx, ly = _signed_sample(q, rng), _signed_sample(q, rng)
+        lx, ly = _forward_input(q, rng), _forward_input(q, rng)
         ws = [rng.randrange(q) if rng.random() < 0.8 else rng.randrange(4) for _ in range(stages)]
         sels = [1 if it % 3 == 0 else rng.randrange(3) for _ in range(stages)]
         line = f"B {c} {stages} " + " ".join(map(str, lx + ly)) + " " + \
@@ -135,9 +171,9 @@ def test_forward_chain_bounds(tool, q, stages):
         out = run(tool, line)
         assert len(out) == 10 * stages
         ex, ey = value(lx) % q, value(ly) % q
-        prev_y = value(ly)
+        prev_y = ly
         for s, (w, sel) in enumerate(zip(ws, sels)):
-            assert prev_y < R, "mont input >= 2^130"
+            assert abs(value(prev_y)) < 64 * q and max(abs(v) for v in prev_y) < (1 << 31), "mont input"
             t = ey * w * pow(R, -1, q) % q
             ex, ey = (ex + t) % q, (ex - t) % q
             if sel == 1:
@@ -148,13 +184,14 @@ def test_forward_chain_bounds(tool, q, stages):
             X, Y = value(st[:5]), value(st[5:])
             assert X % q == ex and Y % q == ey, f"stage {s}"
             for v in (X, Y):
-                assert v < 4 * q + 3 * q * (s + 1)
+                assert abs(v) < q + (2 * q + (q >> 17)) * (s + 1)
             for half in (st[:5], st[5:]):
-                assert all(half[j] < (s + 2) * LIMB_STEP[j] for j in range(5)), f"limbs at stage {s}"
-            prev_y = Y
-        # a forward output folds (point-wise products, stores) to < 2q with limbs < 2^28
-        f = run(tool, f"F {c} " + " ".join(map(str, out[-5:])))
-        assert value(f) % q == ey and value(f) < 2 * q and all(v < (1 << 28) for v in f)
+                assert all(abs(half[j]) < LIMB_IN[j] + (s + 1) * LIMB_STEP[j] for j in range(5)), f"limbs at stage {s}"
+            prev_y = st[5:]
+        # a forward output folds (point-wise products, stores) to [0, 2q) with limbs < 2^28
+        for half in (out[-10:-5], out[-5:]):
+            f = run(tool, f"G {c} " + " ".join(map(str, half)))
+            assert value(f) % q == value(half) % q and 0 <= value(f) < 2 * q and all(0 <= v < (1 << 28) for v in f)
 
 
 @pytest.mark.parametrize("q", PRIMES)
