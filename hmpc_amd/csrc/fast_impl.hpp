@@ -141,7 +141,6 @@ namespace hmpc_amd
                 w[4] = tw[8 * static_cast<std::size_t>(idx) + 4];
             }
 
-            // (x, y) -> (x + t, x - t + 3q), t = mont(y, w)
             // (x, y) -> (x + t, x - t), t = mont(y, w), on signed digits
             __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
             {
