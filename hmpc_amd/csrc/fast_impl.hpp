@@ -147,34 +147,17 @@ namespace hmpc_amd
                 r26_forward(x, y, w, P);
             }
 
-            // (x, y) -> (x + y, mont(x - y + 32q, w)), x + y folded when `fold` (odd stages)
+            // (x, y) -> (x + y, mont(x - y, w)) on signed digits, x + y folded when `fold` (odd stages)
             __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P,
                                                        bool fold)
             {
-                u32 d[rl];
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                {
-                    d[l] = x[l] + (P.k32[l] - y[l]);
-                    x[l] = x[l] + y[l];
-                }
-                if (fold)
-                    r26_fold(x, P);
-                r26_mont(y, d, w, P);
+                r26_inverse(x, y, w, P, fold);
             }
 
             __device__ __forceinline__ static void inv_last(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w0)[rl],
                                                             u32 const (&w1)[rl], params const& P)
             {
-                u32 s[rl], d[rl];
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                {
-                    d[l] = x[l] + (P.k32[l] - y[l]);
-                    s[l] = x[l] + y[l];
-                }
-                r26_mont(x, s, w0, P);
-                r26_mont(y, d, w1, P);
+                r26_inverse_last(x, y, w0, w1, P);
             }
 
             // point-wise product of two forward results a R, b R (signed digits) -> a b R (R' / R = 4

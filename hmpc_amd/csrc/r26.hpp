@@ -30,12 +30,12 @@
 //               with limbs < 2^28, for any limbs < 2^31. fold_signed adds 64 q first, so that a
 //               signed-digit value (|value| < 64 q, |x_j| < 2^31 - 2^26, |x_4| < 2^29) folds the
 //               same way
-//   inverse butterfly (x + y, mont(x - y + K32, w)), with x + y folded at odd stages of the
-//               transform (inputs are products or canonical data, never signed): at odd stages
-//               the inputs are < 4q (1 + 2^-19) with limbs < 2^29 (one unfolded sum of folded
-//               values), at even stages < 2q (1 + 2^-19) with limbs < 2^28; from products (< 8q,
-//               limbs < 2^28 after the x4 correction) stage 0 gives < 16q, limbs < 2^29. Every
-//               mont input stays < 48q with limbs < 2^31.
+//   inverse butterfly (x + y, mont(x - y, w)) on signed digits, x + y folded (signed) at odd
+//               stages of the transform: even-stage inputs are folded values or mont outputs
+//               (within (-q, 2q (1 + 2^-19)), |limbs| < 2^28), odd-stage inputs one unfolded sum of
+//               two (within 4q (1 + 2^-19), |limbs| < 2^29); from products (< 8q, limbs < 2^28
+//               after the x4 correction) stage 0 gives < 16q, and stage 1's sums stay within 32q
+//               with |limbs| < 2^30. Every mont input stays within 32q with |limbs| < 2^30.
 #pragma once
 
 #include "modarith.hpp"
@@ -50,9 +50,6 @@ namespace hmpc_amd
     {
         u32 c;         // q = 2^124 + c
         u32 n26;       // -q^-1 mod 2^26
-        u32 k4[5];     // 4q with limbs k4_0 >= 2^26, k4_{1..3} = 2^26 - 1 (>= mont output limbs)
-        u32 k32[5];    // 32q with limbs >= 12 2^26 - 12 > 2^29 (>= the limbs of a folded value and
-                       // of one unfolded sum of two)
         u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
         u32 b64[2];    // 64 c as two radix-2^26 limbs (with 2^26 in limb 4: 64 q, the signed fold's bias)
         u32 q32[4];    // q as 32-bit limbs (final canonical subtraction)
@@ -69,14 +66,9 @@ namespace hmpc_amd
         for (int i = 0; i < 6; ++i)
             inv *= 2u - c * inv;  // c^-1 mod 2^32 (q = c mod 2^32)
         R.n26 = (0u - inv) & m26;
-        u32 const k4[5] = {4 * c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 22) - 1};
-        u32 const k32[5] = {32 * c + 12 * (1u << 26), 12 * (1u << 26) - 12, 12 * (1u << 26) - 12, 12 * (1u << 26) - 12,
-                            (1u << 25) - 12};
         u32 const qb[5] = {c + (1u << 26), (1u << 26) - 1, (1u << 26) - 1, (1u << 26) - 1, (1u << 20) - 1};
         for (int i = 0; i < 5; ++i)
         {
-            R.k4[i] = k4[i];
-            R.k32[i] = k32[i];
             R.qb[i] = qb[i];
         }
         R.b64[0] = (64u * c) & m26;
@@ -233,6 +225,36 @@ namespace hmpc_amd
             x[l] = xl + t[l];
             y[l] = xl - t[l];
         }
+    }
+
+    // inverse (Gentleman-Sande) butterfly on signed digits: (x, y) -> (x + y, mont(x - y, w)), the
+    // sum folded (to [0, 2q)) when `fold`
+    HMPC_HD void r26_inverse(u32 (&x)[5], u32 (&y)[5], u32 const (&w)[5], p26 const& P, bool fold)
+    {
+        u32 d[5];
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+        {
+            d[l] = x[l] - y[l];
+            x[l] = x[l] + y[l];
+        }
+        if (fold)
+            r26_fold_signed(x, P);
+        r26_mont_signed(y, d, w, P);
+    }
+
+    // the inverse's last stage with its scaling: (x, y) -> (mont(x + y, w0), mont(x - y, w1))
+    HMPC_HD void r26_inverse_last(u32 (&x)[5], u32 (&y)[5], u32 const (&w0)[5], u32 const (&w1)[5], p26 const& P)
+    {
+        u32 s[5], d[5];
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+        {
+            d[l] = x[l] - y[l];
+            s[l] = x[l] + y[l];
+        }
+        r26_mont_signed(x, s, w0, P);
+        r26_mont_signed(y, d, w1, P);
     }
 
     // signed integer z -> a value congruent to z R mod q (R = 2^128) with limbs

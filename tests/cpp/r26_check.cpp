@@ -88,7 +88,7 @@ int main()
         else if (cmd == "B" || cmd == "I")
         {
             // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t), t = mont(y, w) (signed digits),
-            // or inverse (x + y, mont(x - y + 32q, w)) with x + y folded on request. Per stage: the
+            // or inverse (x + y, mont(x - y, w)) with x + y folded on request (signed digits). Per stage: the
             // twiddle limbs, then for "B" a selector of the next stage's pair (0: (x', y'),
             // 1: (y', y'), 2: (x', x')), for "I" the fold flag and a selector (0: (x', y'),
             // 1: (x', x'), 2: (y', y')); prints the pair after every stage
@@ -123,15 +123,7 @@ int main()
                 {
                     int fold, sel;
                     std::cin >> fold >> sel;
-                    u32 d[5];
-                    for (int l = 0; l < 5; ++l)
-                    {
-                        d[l] = x[l] + (P.k32[l] - y[l]);
-                        x[l] = x[l] + y[l];
-                    }
-                    if (fold)
-                        r26_fold(x, P);
-                    r26_mont(y, d, w, P);
+                    r26_inverse(x, y, w, P, fold != 0);
                     for (int l = 0; l < 5; ++l)
                     {
                         if (sel == 1)
@@ -140,8 +132,7 @@ int main()
                             x[l] = y[l];
                     }
                 }
-                // forward values are signed digits, inverse ones are non-negative (either prints
-                // right as a signed 32-bit limb: inverse limbs stay below 2^31)
+                // signed digits: every limb prints as a signed 32-bit value
                 for (int l = 0; l < 5; ++l)
                     printf("%d ", static_cast<int>(x[l]));
                 for (int l = 0; l < 5; ++l)

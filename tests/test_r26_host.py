@@ -197,16 +197,14 @@ def test_forward_chain_bounds(tool, q, stages):
 @pytest.mark.parametrize("q", PRIMES)
 @pytest.mark.parametrize("stages", [12, 16])
 def test_inverse_chain_bounds(tool, q, stages):
-    """Lazy inverse stages, x + y folded at odd stages only, from product-shaped inputs (< 8q,
-    limbs < 2^28), each stage keeping either output pair or a doubled output: every mont input
-    x - y + 32q stays < 48q with limbs < 2^31 (no limb of y exceeds 32q's), odd-stage inputs
-    stay < 4q (1 + 2^-19) with limbs < 2^29, the folded outputs < 2q with limbs < 2^28, and the
-    results are exact mod q."""
+    """Lazy inverse stages on signed digits, x + y folded at odd stages only, from product-shaped
+    inputs (< 8q, limbs < 2^28), each stage keeping either output pair or a doubled output: every
+    mont input x - y stays within 32q with limbs below 2^30, odd-stage inputs within 4q (1 + 2^-18)
+    (32q after stage 0) with limbs below 2^30, the folded outputs in [0, 2q) with limbs < 2^28, the
+    mont outputs within (-q, 2q (1 + 2^-19)), and the results are exact mod q."""
     c = q - (1 << 124)
     rng = random.Random(11 + stages)
     R = 1 << 130
-    k32 = [32 * c + 12 * (1 << 26), 12 * (1 << 26) - 12, 12 * (1 << 26) - 12, 12 * (1 << 26) - 12, (1 << 25) - 12]
-    assert value(k32) == 32 * q
     for it in range(150):
         x, y = rng.randrange(8 * q), rng.randrange(8 * q)
         lx, ly = redundant(x, rng, 28), redundant(y, rng, 28)
@@ -220,12 +218,11 @@ def test_inverse_chain_bounds(tool, q, stages):
         ex, ey = x % q, y % q
         px, py = lx, ly
         for s, (w, f, sel) in enumerate(zip(ws, folds, sels)):
-            d = [a + (k - b) for a, k, b in zip(px, k32, py)]
-            assert all(0 <= k - b for k, b in zip(k32, py)), f"limb of y above 32q's at stage {s}"
-            assert value(d) < 48 * q and max(d) < (1 << 31), f"mont input at stage {s}"
+            d = [a - b for a, b in zip(px, py)]
+            assert abs(value(d)) < 32 * q and max(abs(v) for v in d) < (1 << 30), f"mont input at stage {s}"
             if f:  # odd stage: inputs are one unfolded sum (or the stage-0 sum of products)
                 bound = 16 * q if s == 1 else 4 * q + (q >> 17)
-                assert value(px) < bound and value(py) < bound and max(px + py) < (1 << 29) + (1 << 27)
+                assert abs(value(px)) < bound and abs(value(py)) < bound and max(abs(v) for v in px + py) < (1 << 30)
             ex, ey = (ex + ey) % q, (ex - ey) * w * pow(R, -1, q) % q
             if sel == 1:
                 ey = ex
@@ -235,7 +232,7 @@ def test_inverse_chain_bounds(tool, q, stages):
             X, Y = value(st[:5]), value(st[5:])
             assert X % q == ex and Y % q == ey, f"stage {s}"
             if f and sel != 2:
-                assert X < 2 * q and max(st[:5]) < (1 << 28)
+                assert 0 <= X < 2 * q and all(0 <= v < (1 << 28) for v in st[:5])
+            if sel != 1:
+                assert -q < Y < 2 * q + (q >> 18) and all(0 <= v < (1 << 26) for v in st[5:9])
             px, py = st[:5], st[5:]
-
-
