@@ -7,9 +7,9 @@
 // (acc = a * b + acc: one 64-bit mad, no carries), and additions are limb-wise.
 //
 // Representation: x = sum_j x_j 2^(26 j), j = 0..4, limbs x_j held in 32 bits (redundant). The
-// forward transform's values are signed digits: two's-complement limbs |x_j| < 2^31, so that
-// its butterfly (x + t, x - t) needs no added multiple of q; every other value has limbs in
-// [0, 2^31). Internal Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in
+// transforms' values are signed digits: two's-complement limbs |x_j| < 2^31, so that their
+// butterflies (x + t, x - t) and (x + y, x - y) need no added multiple of q; point-wise products
+// fold their operands to non-negative limbs first. Internal Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in
 // hmpc's representation a*R mod q (R = 2^128); twiddles are stored as w * 2^130 mod q so that
 // mont(a R, w 2^130) = (a w) R. A product of two data values is corrected by 4 = R' / R.
 //
