@@ -1384,6 +1384,7 @@ namespace
         int elems = 0;
         u32 type = 0;
         int log_n = 0;  // a degree of the chunk (the arithmetic parameters do not depend on it)
+        bool uniform = false;  // every job of the chunk has degree 2^log_n
         std::size_t job_off = 0, njobs = 0;
         // into the map: forward column pass, row pass, inverse column pass (products)
         std::size_t cols_off = 0, cols_blocks = 0, rows_off = 0, rows_blocks = 0, icols_off = 0, icols_blocks = 0;
@@ -1399,17 +1400,25 @@ namespace
 
         bool empty() const { return jobs.empty(); }
 
-        // in: (modulus, elements per thread, job); order is kept within a group
-        void build(std::vector<std::tuple<hmpc_modulus, int, job_desc>> const& in)
+        // in: (modulus, elements per thread, job); order is kept within a group. by_degree: a group
+        // (and so each chunk) holds one degree, and its launches run the kernels with the pass
+        // lengths fixed at compile time. That pays where launches on several streams fill each
+        // other's tails (the server's slots: peak 7,300 -> 8,200 requests/s); a batch on one stream
+        // keeps degrees mixed, so that each pass is one launch (its uneven batch: 3.6 against
+        // 3.8 ms split by degree).
+        void build(std::vector<std::tuple<hmpc_modulus, int, job_desc>> const& in, bool by_degree)
         {
             jobs.clear();
             map.clear();
             chunks.clear();
             ws_words = 0;
-            std::vector<std::tuple<hmpc_modulus, int, u32>> groups;
+            std::vector<std::tuple<hmpc_modulus, int, u32, u32>> groups;
+            auto key = [by_degree](std::tuple<hmpc_modulus, int, job_desc> const& t) {
+                return std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type, by_degree ? std::get<2>(t).log_n : 0u);
+            };
             for (auto const& t : in)
             {
-                auto g = std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type);
+                auto g = key(t);
                 if (std::find(groups.begin(), groups.end(), g) == groups.end())
                     groups.push_back(g);
             }
@@ -1429,7 +1438,7 @@ namespace
                 };
                 for (auto const& t : in)
                 {
-                    if (std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type) != g)
+                    if (key(t) != g)
                         continue;
                     job_desc J = std::get<2>(t);
                     u64 words = streams * static_cast<u64>(ws_limbs(mod->limbs)) << J.log_n;
@@ -1441,6 +1450,7 @@ namespace
                         c.elems = std::get<1>(g);
                         c.type = type;
                         c.log_n = static_cast<int>(J.log_n);
+                        c.uniform = by_degree;
                         c.job_off = jobs.size();
                     }
                     J.ws_off = c.ws_words;
@@ -1479,10 +1489,10 @@ namespace
                 T.elems = c.elems;
                 auto n = [](std::size_t v) { return static_cast<u32>(v); };
                 if (c.type == job_bgv)
-                    HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, d_jobs + c.job_off, d_map + c.cols_off, n(c.cols_blocks),
-                                                   d_map + c.rows_off, n(c.rows_blocks), ws, s));
+                    HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off,
+                                                   n(c.cols_blocks), d_map + c.rows_off, n(c.rows_blocks), ws, s));
                 else
-                    HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, d_jobs + c.job_off, d_map + c.cols_off, n(c.cols_blocks),
+                    HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off, n(c.cols_blocks),
                                                        d_map + c.rows_off, n(c.rows_blocks), d_map + c.icols_off,
                                                        n(c.icols_blocks), ws, s));
             }
@@ -1842,7 +1852,7 @@ struct hmpc_batch_s
             c.counter_off = off;
             off += counter_bytes;
         }
-        tiles.build(tiled);
+        tiles.build(tiled, false);
         tile_jobs_off = off;
         off += (tiles.jobs.size() * sizeof(job_desc) + 255) / 256 * 256;
         tile_map_off = off;
@@ -2626,7 +2636,7 @@ struct hmpc_server_s
             slot_words[c] = w;
             need = std::max<std::size_t>(need, static_cast<std::size_t>(grids[c]) * w);
         }
-        s.tiles.build(tiled);
+        s.tiles.build(tiled, true);
         need = std::max<std::size_t>(need, s.tiles.ws_words);
         if (s.map_cap < s.tiles.map.size())
         {

@@ -515,7 +515,9 @@ namespace hmpc_amd
         // (one ciphertext each): block b of a launch works on job map[b] >> 10, local tile
         // map[b] & 1023 (at most 3 x 256 tiles per job, at N = 2^20). Each job keeps its three
         // column results in its own workspace region [3][wsl][N] at ws + ws_off.
-        template<typename A, int E>
+        // S > 0: every job of the table has this column length (the pass length as a compile-time
+        // constant, as in the uniform launches); likewise the row length of the row kernels below
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_cols_table(job_desc const* __restrict__ jobs,
                                                                                     u32 const* __restrict__ map,
                                                                                     u32* __restrict__ ws,
@@ -531,14 +533,14 @@ namespace hmpc_amd
             std::size_t N = std::size_t{1} << J.log_n;
             u32* dst = ws + J.ws_off + which * A::wsl * N;
             if (which == 0)
-                bgv_column_tile<A, E, 0>(J, lds, dst, gc, ct, N, 0, P);
+                bgv_column_tile<A, E, 0, S>(J, lds, dst, gc, ct, N, 0, P);
             else if (which == 1)
-                bgv_column_tile<A, E, 1>(J, lds, dst, gc, ct, N, 0, P);
+                bgv_column_tile<A, E, 1, S>(J, lds, dst, gc, ct, N, 0, P);
             else
-                bgv_column_tile<A, E, 2>(J, lds, dst, gc, ct, N, 0, P);
+                bgv_column_tile<A, E, 2, S>(J, lds, dst, gc, ct, N, 0, P);
         }
 
-        template<typename A, int E>
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows_table(job_desc const* __restrict__ jobs,
                                                                                     u32 const* __restrict__ map,
                                                                                     u32* __restrict__ ws,
@@ -557,6 +559,11 @@ namespace hmpc_amd
             u32 hi0 = job_rows_tile(gr, rt, N, 0, td, is);
             job_rows_tile(gr, rt, J.in_count, J.in_off, td, ik);
             job_rows_tile(gr, rt, J.out_count, J.out_off, td, io);
+            if constexpr (S > 0)
+            {
+                td.s = S;
+                td.log_seqs = tile_log - S;
+            }
             u32 base0 = (1u << gr.log_n1) + hi0;
             u32 x[E][A::rl];
             tile_load<A, T, true>(lds, su, is);
@@ -574,24 +581,38 @@ namespace hmpc_amd
             }
         }
 
+        // log_n > 0: every job of the table has degree 2^log_n (pass lengths at compile time where
+        // they are specialized); 0: mixed degrees
         template<typename A>
-        hipError_t launch_bgv_table(int elems, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
+        hipError_t launch_bgv_table(int elems, int log_n, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
                                     u32 const* map_rows, u32 rows_blocks, u32* ws, typename A::params const& P, hipStream_t s)
         {
             return with_elems<A>(elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
                 constexpr std::size_t bytes = tile_bytes<A>();
+                pass_geom gc = geom_for(log_n, false);
+                pass_geom gr = geom_for(log_n, true);
+                bool const uniform = log_n > tile_log;
+                auto cols_kernel = uniform && gc.log_n1 == 7 ? k_bgv_cols_table<A, E, 7> : k_bgv_cols_table<A, E, 0>;
+                auto rows_kernel = k_bgv_rows_table<A, E, 0>;
+                switch (uniform ? gr.log_n2 : 0)
+                {
+                case 7: rows_kernel = k_bgv_rows_table<A, E, 7>; break;
+                case 8: rows_kernel = k_bgv_rows_table<A, E, 8>; break;
+                case 9: rows_kernel = k_bgv_rows_table<A, E, 9>; break;
+                case 10: rows_kernel = k_bgv_rows_table<A, E, 10>; break;
+                default: break;
+                }
                 hipError_t e = hipSuccess;
-                if ((e = allow_lds(k_bgv_cols_table<A, E>, bytes)) != hipSuccess ||
-                    (e = allow_lds(k_bgv_rows_table<A, E>, bytes)) != hipSuccess)
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast bgv columns");
-                    k_bgv_cols_table<A, E><<<cols_blocks, tile / E, bytes, s>>>(jobs, map_cols, ws, P);
+                    cols_kernel<<<cols_blocks, tile / E, bytes, s>>>(jobs, map_cols, ws, P);
                 }
                 {
                     probe_scope ps(s, "fast bgv rows");
-                    k_bgv_rows_table<A, E><<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
+                    rows_kernel<<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
                 }
                 return hipGetLastError();
             });
@@ -602,7 +623,7 @@ namespace hmpc_amd
         // (forward, product, inverse), inverse columns with the final scaling. Job j keeps
         // NTT(a), NTT(b) and the row results in its own workspace region [2][wsl][N] at
         // ws + ws_off. Blocks find their job and tile through map as above.
-        template<typename A, int E>
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E) k_polymul_cols_table(job_desc const* __restrict__ jobs,
                                                                          u32 const* __restrict__ map, u32* __restrict__ ws,
                                                                          typename A::params P)
@@ -620,6 +641,13 @@ namespace hmpc_amd
             io_desc ii, is;
             job_cols_tile(gc, ct, J.in_count, J.in_off, td, ii);
             job_cols_tile(gc, ct, N, 0, td, is);
+            if constexpr (S > 0)
+            {
+                td.s = S;
+                td.log_seqs = tile_log - S;
+                ii.log_seqs = tile_log - S;
+                is.log_seqs = tile_log - S;
+            }
             tile_load<A, T>(lds, which ? J.in1 : J.in0, ii);
             __syncthreads();
             u32 x[E][A::rl];
@@ -627,8 +655,8 @@ namespace hmpc_amd
             tile_store<A, T, true>(lds, ws + J.ws_off + which * A::wsl * N, is, ws_fold_columns(static_cast<int>(J.log_n)), P);
         }
 
-        template<typename A, int E>
-        __global__ void __launch_bounds__(tile / E) k_polymul_rows_table(job_desc const* __restrict__ jobs,
+        template<typename A, int E, int S = 0>
+        __global__ void __launch_bounds__(tile / E, (E == 8 && S > 0) ? 4 : 1) k_polymul_rows_table(job_desc const* __restrict__ jobs,
                                                                          u32 const* __restrict__ map, u32* __restrict__ ws,
                                                                          typename A::params P)
         {
@@ -645,24 +673,31 @@ namespace hmpc_amd
             tile_desc td;
             io_desc is;
             u32 hi0 = job_rows_tile(gr, rt, N, 0, td, is);
+            if constexpr (S > 0)
+            {
+                td.s = S;
+                td.log_seqs = tile_log - S;
+            }
             u32 base0 = (1u << gr.log_n1) + hi0;
+            // wave-local rows as in the uniform row kernel
+            constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
             u32 x[E][A::rl];
-            tile_load<A, T, true>(lds, sa, is);
-            __syncthreads();
-            tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-            tile_store<A, T, true>(lds, sa, is, false, P);
-            __syncthreads();
-            tile_load<A, T, true>(lds, sb, is);
-            __syncthreads();
-            tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+            tile_load<A, T, true, WL>(lds, sa, is);
+            tile_sync<WL>();
+            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+            tile_store<A, T, true, WL>(lds, sa, is, false, P);
+            tile_sync<WL>();
+            tile_load<A, T, true, WL>(lds, sb, is);
+            tile_sync<WL>();
+            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
             with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
             dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
-            __syncthreads();
-            tile_rounds<A, E, true>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
-            tile_store<A, T, true>(lds, sa, is, false, P);
+            tile_sync<WL>();
+            tile_rounds<A, E, true, WL>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
+            tile_store<A, T, true, WL>(lds, sa, is, false, P);
         }
 
-        template<typename A, int E>
+        template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_icols_table(job_desc const* __restrict__ jobs,
                                                                                           u32 const* __restrict__ map,
                                                                                           u32 const* __restrict__ ws,
@@ -679,6 +714,13 @@ namespace hmpc_amd
             io_desc is, io;
             job_cols_tile(gc, ct, N, 0, td, is);
             job_cols_tile(gc, ct, J.out_count, J.out_off, td, io);
+            if constexpr (S > 0)
+            {
+                td.s = S;
+                td.log_seqs = tile_log - S;
+                is.log_seqs = tile_log - S;
+                io.log_seqs = tile_log - S;
+            }
             tile_load<A, T, true>(lds, ws + J.ws_off, is);
             __syncthreads();
             u32 x[E][A::rl];
@@ -687,29 +729,44 @@ namespace hmpc_amd
         }
 
         template<typename A>
-        hipError_t launch_polymul_table(int elems, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
+        hipError_t launch_polymul_table(int elems, int log_n, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
                                         u32 const* map_rows, u32 rows_blocks, u32 const* map_icols, u32 icols_blocks,
                                         u32* ws, typename A::params const& P, hipStream_t s)
         {
             return with_elems<A>(elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
                 constexpr std::size_t bytes = tile_bytes<A>();
+                using AR = typename rows_arith<A>::type;
+                pass_geom gc = geom_for(log_n, false);
+                pass_geom gr = geom_for(log_n, true);
+                bool const uniform = log_n > tile_log;
+                bool const c7 = uniform && gc.log_n1 == 7;
+                auto cols_kernel = c7 ? k_polymul_cols_table<A, E, 7> : k_polymul_cols_table<A, E, 0>;
+                auto icols_kernel = c7 ? k_polymul_icols_table<A, E, 7> : k_polymul_icols_table<A, E, 0>;
+                auto rows_kernel = k_polymul_rows_table<A, E, 0>;
+                switch (uniform ? gr.log_n2 : 0)
+                {
+                case 7: rows_kernel = k_polymul_rows_table<AR, E, 7>; break;
+                case 8: rows_kernel = k_polymul_rows_table<AR, E, 8>; break;
+                case 9: rows_kernel = k_polymul_rows_table<AR, E, 9>; break;
+                case 10: rows_kernel = k_polymul_rows_table<AR, E, 10>; break;
+                default: break;
+                }
                 hipError_t e = hipSuccess;
-                if ((e = allow_lds(k_polymul_cols_table<A, E>, bytes)) != hipSuccess ||
-                    (e = allow_lds(k_polymul_rows_table<A, E>, bytes)) != hipSuccess ||
-                    (e = allow_lds(k_polymul_icols_table<A, E>, bytes)) != hipSuccess)
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess ||
+                    (e = allow_lds(icols_kernel, bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast polymul forward columns");
-                    k_polymul_cols_table<A, E><<<cols_blocks, tile / E, bytes, s>>>(jobs, map_cols, ws, P);
+                    cols_kernel<<<cols_blocks, tile / E, bytes, s>>>(jobs, map_cols, ws, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
-                    k_polymul_rows_table<A, E><<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
+                    rows_kernel<<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");
-                    k_polymul_icols_table<A, E><<<icols_blocks, tile / E, bytes, s>>>(jobs, map_icols, ws, P);
+                    icols_kernel<<<icols_blocks, tile / E, bytes, s>>>(jobs, map_icols, ws, P);
                 }
                 return hipGetLastError();
             });

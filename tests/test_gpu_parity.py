@@ -357,7 +357,8 @@ def test_autotune_installs_a_correct_schedule(ctx):
 # uneven job list: products and encryptions, N from 2^12 to 2^16, both fast moduli
 BATCH_JOBS = [("mul", Q64, 12), ("bgv", Q128A, 13), ("mul", Q128A, 16), ("bgv", Q64, 14), ("mul", Q128A, 12),
               ("bgv", Q128A, 16), ("mul", Q64, 15), ("bgv", Q64, 12), ("mul", Q128A, 14), ("mul", Q64, 16),
-              ("bgv", Q128A, 12), ("mul", Q128A, 13), ("bgv", Q128A, 14), ("bgv", Q64, 16), ("bgv", Q128A, 13)]
+              ("bgv", Q128A, 12), ("mul", Q128A, 13), ("bgv", Q128A, 14), ("bgv", Q64, 16), ("bgv", Q128A, 13),
+              ("mul", Q128A, 15), ("bgv", Q128A, 15), ("mul", Q128A, 16), ("mul", Q128A, 17), ("bgv", Q128A, 17)]
 
 
 @pytest.mark.parametrize("structure", ["tiles", "jobs"])

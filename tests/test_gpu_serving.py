@@ -162,3 +162,48 @@ def test_large_request_spans_tile_chunks(env):
     assert np.array_equal(c1.to_numpy((gm.limbs, k * N)), gm.download(d1, k * N)), "c1"
     srv.release(r)
     srv.close()
+
+
+def test_mixed_degrees_share_launches(env):
+    """Encryptions at N = 2^15 and 2^16 and products at N = 2^15, 2^16 and 2^17 submitted together:
+    the server splits its tile launches by degree (row kernels for rows of 2^8, 2^9 and 2^10
+    with the pass lengths at compile time); every result equals the direct batched call with the
+    same inputs (and, for encryptions, the same nonce)."""
+    ctx, gm, om, t_mont, keys = env
+    pm = oracle.int_to_limbs(t_mont, gm.limbs)
+    srv = hmpc_amd.Server(ctx, budget_bytes=8 << 30, slots=2)
+    rng_key = {}
+    for log_n in (15, 16):
+        N = 1 << log_n
+        A = gm.upload(om.sample_uniform(KEY, 60 + log_n, N))
+        B = gm.upload(om.sample_uniform(KEY, 70 + log_n, N))
+        rng_key[log_n] = (A, B, srv.add_key(gm, log_n, A, B, pm))
+    say("serving: mixed degrees 2^15..2^17")
+    enc = []
+    for i, (log_n, k) in enumerate([(15, 3), (16, 2), (15, 5), (16, 1)]):
+        N = 1 << log_n
+        msgs = ctx.upload(oracle.sample_words(KEY, 100 + i, k * N))
+        nonce = [9000 + 1000 * i, 0, 0]
+        enc.append((log_n, k, msgs, nonce, srv.submit_encrypt(rng_key[log_n][2], k, msgs.ptr, nonce)))
+    mul = []
+    for i, (log_n, n) in enumerate([(15, 3), (16, 2), (17, 1)]):
+        N = 1 << log_n
+        a = gm.upload(om.sample_uniform(KEY, 120 + i, n * N))
+        b = gm.upload(om.sample_uniform(KEY, 130 + i, n * N))
+        mul.append((log_n, n, a, b, srv.submit_polymul(gm, log_n, n, a.ptr, b.ptr)))
+    for *_, rid in enc + mul:
+        srv.wait(rid)
+    for log_n, k, msgs, nonce, rid in enc:
+        N = 1 << log_n
+        c0, c1 = srv.result(rid)
+        ctx.set_rng(KEY, nonce)
+        d0, d1 = gm.bgv_encrypt(log_n, k, rng_key[log_n][0], rng_key[log_n][1], msgs, pm)
+        assert np.array_equal(c0.to_numpy((gm.limbs, k * N)), gm.download(d0, k * N)), ("c0", log_n, k)
+        assert np.array_equal(c1.to_numpy((gm.limbs, k * N)), gm.download(d1, k * N)), ("c1", log_n, k)
+        srv.release(rid)
+    for log_n, n, a, b, rid in mul:
+        N = 1 << log_n
+        p0, _ = srv.result(rid)
+        assert np.array_equal(p0.to_numpy((gm.limbs, n * N)), gm.download(gm.polymul(log_n, n, a, b), n * N)), log_n
+        srv.release(rid)
+    srv.close()
