@@ -532,25 +532,32 @@ namespace hmpc_amd
             u32 ct = local - which * gc.blocks_per_poly;
             std::size_t N = std::size_t{1} << J.log_n;
             u32* dst = ws + J.ws_off + which * A::wsl * N;
-            if (which == 0)
-                bgv_column_tile<A, E, 0, S>(J, lds, dst, gc, ct, N, 0, P);
-            else if (which == 1)
-                bgv_column_tile<A, E, 1, S>(J, lds, dst, gc, ct, N, 0, P);
+            // mixed degrees (S = 0): the column length 2^7 (N >= 2^14) at compile time too
+            if (S == 7 || (S == 0 && gc.log_n1 == 7))
+            {
+                if (which == 0)
+                    bgv_column_tile<A, E, 0, 7>(J, lds, dst, gc, ct, N, 0, P);
+                else if (which == 1)
+                    bgv_column_tile<A, E, 1, 7>(J, lds, dst, gc, ct, N, 0, P);
+                else
+                    bgv_column_tile<A, E, 2, 7>(J, lds, dst, gc, ct, N, 0, P);
+            }
             else
-                bgv_column_tile<A, E, 2, S>(J, lds, dst, gc, ct, N, 0, P);
+            {
+                if (which == 0)
+                    bgv_column_tile<A, E, 0, 0>(J, lds, dst, gc, ct, N, 0, P);
+                else if (which == 1)
+                    bgv_column_tile<A, E, 1, 0>(J, lds, dst, gc, ct, N, 0, P);
+                else
+                    bgv_column_tile<A, E, 2, 0>(J, lds, dst, gc, ct, N, 0, P);
+            }
         }
 
-        template<typename A, int E, int S = 0>
-        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows_table(job_desc const* __restrict__ jobs,
-                                                                                    u32 const* __restrict__ map,
-                                                                                    u32* __restrict__ ws,
-                                                                                    typename A::params P)
+        template<typename A, int E, int S>
+        __device__ __forceinline__ void bgv_rows_table_tile(job_desc const& J, u32* lds, u32* ws, u32 rt,
+                                                            typename A::params const& P)
         {
-            extern __shared__ __align__(16) u32 lds[];
             constexpr int T = tile / E;
-            u32 m = map[blockIdx.x];
-            job_desc J = jobs[m >> 10];
-            u32 rt = m & 1023u;
             pass_geom gr = geom_for(static_cast<int>(J.log_n), true);
             std::size_t N = std::size_t{1} << J.log_n;
             u32* su = ws + J.ws_off;
@@ -578,6 +585,33 @@ namespace hmpc_amd
                 tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
                 bool first = which == 1;
                 bgv_emit<A, E>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+            }
+        }
+
+        // S > 0: every job has this row length; S = 0: mixed degrees, each block branching (uniformly)
+        // to the row length of its own job (2^7..2^10 at compile time, others at run time)
+        template<typename A, int E, int S = 0>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows_table(job_desc const* __restrict__ jobs,
+                                                                                    u32 const* __restrict__ map,
+                                                                                    u32* __restrict__ ws,
+                                                                                    typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            u32 rt = m & 1023u;
+            if constexpr (S > 0)
+                bgv_rows_table_tile<A, E, S>(J, lds, ws, rt, P);
+            else
+            {
+                switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
+                {
+                case 7: bgv_rows_table_tile<A, E, 7>(J, lds, ws, rt, P); break;
+                case 8: bgv_rows_table_tile<A, E, 8>(J, lds, ws, rt, P); break;
+                case 9: bgv_rows_table_tile<A, E, 9>(J, lds, ws, rt, P); break;
+                case 10: bgv_rows_table_tile<A, E, 10>(J, lds, ws, rt, P); break;
+                default: bgv_rows_table_tile<A, E, 0>(J, lds, ws, rt, P); break;
+                }
             }
         }
 
@@ -623,17 +657,11 @@ namespace hmpc_amd
         // (forward, product, inverse), inverse columns with the final scaling. Job j keeps
         // NTT(a), NTT(b) and the row results in its own workspace region [2][wsl][N] at
         // ws + ws_off. Blocks find their job and tile through map as above.
-        template<typename A, int E, int S = 0>
-        __global__ void __launch_bounds__(tile / E) k_polymul_cols_table(job_desc const* __restrict__ jobs,
-                                                                         u32 const* __restrict__ map, u32* __restrict__ ws,
-                                                                         typename A::params P)
+        template<typename A, int E, int S>
+        __device__ __forceinline__ void polymul_cols_table_tile(job_desc const& J, u32* lds, u32* ws, u32 local,
+                                                                pass_geom const& gc, typename A::params const& P)
         {
-            extern __shared__ __align__(16) u32 lds[];
             constexpr int T = tile / E;
-            u32 m = map[blockIdx.x];
-            job_desc J = jobs[m >> 10];
-            u32 local = m & 1023u;
-            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
             u32 which = local >= gc.blocks_per_poly ? 1u : 0u;
             u32 ct = local - which * gc.blocks_per_poly;
             std::size_t N = std::size_t{1} << J.log_n;
@@ -655,17 +683,29 @@ namespace hmpc_amd
             tile_store<A, T, true>(lds, ws + J.ws_off + which * A::wsl * N, is, ws_fold_columns(static_cast<int>(J.log_n)), P);
         }
 
+        // S > 0: every job has this column length; S = 0: mixed degrees, the column length 2^7
+        // (N >= 2^14) at compile time too
         template<typename A, int E, int S = 0>
-        __global__ void __launch_bounds__(tile / E, (E == 8 && S > 0) ? 4 : 1) k_polymul_rows_table(job_desc const* __restrict__ jobs,
+        __global__ void __launch_bounds__(tile / E) k_polymul_cols_table(job_desc const* __restrict__ jobs,
                                                                          u32 const* __restrict__ map, u32* __restrict__ ws,
                                                                          typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
-            constexpr int T = tile / E;
-            constexpr int WM = shape<E>::wmax;
             u32 m = map[blockIdx.x];
             job_desc J = jobs[m >> 10];
-            u32 rt = m & 1023u;
+            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
+            if (S == 7 || (S == 0 && gc.log_n1 == 7))
+                polymul_cols_table_tile<A, E, 7>(J, lds, ws, m & 1023u, gc, P);
+            else
+                polymul_cols_table_tile<A, E, 0>(J, lds, ws, m & 1023u, gc, P);
+        }
+
+        template<typename A, int E, int S>
+        __device__ __forceinline__ void polymul_rows_table_tile(job_desc const& J, u32* lds, u32* ws, u32 rt,
+                                                                typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            constexpr int WM = shape<E>::wmax;
             pass_geom gr = geom_for(static_cast<int>(J.log_n), true);
             std::size_t N = std::size_t{1} << J.log_n;
             u32* sa = ws + J.ws_off;
@@ -697,18 +737,39 @@ namespace hmpc_amd
             tile_store<A, T, true, WL>(lds, sa, is, false, P);
         }
 
+        // S > 0: every job has this row length (A then the row policy); S = 0: mixed degrees, each
+        // block branching (uniformly) to its own job's row length, 2^7..2^10 with the row policy
         template<typename A, int E, int S = 0>
-        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_icols_table(job_desc const* __restrict__ jobs,
-                                                                                          u32 const* __restrict__ map,
-                                                                                          u32 const* __restrict__ ws,
-                                                                                          typename A::params P)
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_rows_table(job_desc const* __restrict__ jobs,
+                                                                                        u32 const* __restrict__ map,
+                                                                                        u32* __restrict__ ws,
+                                                                                        typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
-            constexpr int T = tile / E;
+            using AR = typename rows_arith<A>::type;
             u32 m = map[blockIdx.x];
             job_desc J = jobs[m >> 10];
-            u32 ct = m & 1023u;
-            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
+            u32 rt = m & 1023u;
+            if constexpr (S > 0)
+                polymul_rows_table_tile<A, E, S>(J, lds, ws, rt, P);
+            else
+            {
+                switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
+                {
+                case 7: polymul_rows_table_tile<AR, E, 7>(J, lds, ws, rt, P); break;
+                case 8: polymul_rows_table_tile<AR, E, 8>(J, lds, ws, rt, P); break;
+                case 9: polymul_rows_table_tile<AR, E, 9>(J, lds, ws, rt, P); break;
+                case 10: polymul_rows_table_tile<AR, E, 10>(J, lds, ws, rt, P); break;
+                default: polymul_rows_table_tile<A, E, 0>(J, lds, ws, rt, P); break;
+                }
+            }
+        }
+
+        template<typename A, int E, int S>
+        __device__ __forceinline__ void polymul_icols_table_tile(job_desc const& J, u32* lds, u32 const* ws, u32 ct,
+                                                                 pass_geom const& gc, typename A::params const& P)
+        {
+            constexpr int T = tile / E;
             std::size_t N = std::size_t{1} << J.log_n;
             tile_desc td;
             io_desc is, io;
@@ -726,6 +787,23 @@ namespace hmpc_amd
             u32 x[E][A::rl];
             tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false, gc.log_n2);
             tile_store<A, T>(lds, J.out0, io, true, P);
+        }
+
+        // S as for the forward column kernel
+        template<typename A, int E, int S = 0>
+        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_icols_table(job_desc const* __restrict__ jobs,
+                                                                                          u32 const* __restrict__ map,
+                                                                                          u32 const* __restrict__ ws,
+                                                                                          typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            u32 m = map[blockIdx.x];
+            job_desc J = jobs[m >> 10];
+            pass_geom gc = geom_for(static_cast<int>(J.log_n), false);
+            if (S == 7 || (S == 0 && gc.log_n1 == 7))
+                polymul_icols_table_tile<A, E, 7>(J, lds, ws, m & 1023u, gc, P);
+            else
+                polymul_icols_table_tile<A, E, 0>(J, lds, ws, m & 1023u, gc, P);
         }
 
         template<typename A>
