@@ -287,8 +287,9 @@ struct hmpc_modulus_s
 
     int default_elems(int) const { return limbs == 4 ? 4 : 16; }
     // elements per thread for one call of n transforms: the tuned schedule, else the default,
-    // except that a Goldilocks batch too small to give every CU a tile runs twice the threads per
-    // tile (8 elements: one N = 4096 NTT -> INTT in 31.5 against 41.1 us at 16)
+    // except that a Goldilocks batch too small to give every CU a tile runs more threads per tile
+    // (whole tiles: 4 elements, 1024 threads; two passes: 8). One N = 4096 NTT -> INTT took 41.1 us
+    // at 16 and 31.5 us at 8 elements per thread.
     int call_elems(int log_n, std::size_t n) const
     {
         auto e = elems.find(log_n);
@@ -297,7 +298,7 @@ struct hmpc_modulus_s
         int const lt = 12;  // log2 of the fast tile
         std::size_t tiles = log_n <= lt ? (n + (std::size_t{1} << (lt - log_n)) - 1) >> (lt - log_n) : n << (log_n - lt);
         if (limbs == 2 && tiles < static_cast<std::size_t>(ctx->props.multiProcessorCount))
-            return 8;
+            return log_n <= lt ? 4 : 8;
         return default_elems(log_n);
     }
     // tuned choice, else tiles for batches smaller than the device's resident workgroups

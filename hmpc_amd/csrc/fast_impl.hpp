@@ -1160,10 +1160,45 @@ namespace hmpc_amd
             return e;
         }
 
+        // whole-polynomial tiles (N <= 4096)
+        template<typename A, int E>
+        hipError_t launch_whole_tiles(transform_args const& T, typename A::params const& P, std::size_t n, bool inverse,
+                                      u32 const* in, u32* out, hipStream_t s)
+        {
+            constexpr int TH = tile / E;
+            constexpr std::size_t bytes = tile_bytes<A>();
+            u32 const* tw = inverse ? T.inv_fast : T.fwd_fast;
+            hipError_t e = hipSuccess;
+            pass_geom G = geom_for(T.log_n, true);
+            unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
+            if (!inverse)
+            {
+                if ((e = allow_lds(k_pass<A, E, 4>, bytes)) != hipSuccess)
+                    return e;
+                probe_scope ps(s, "fast ntt forward (whole tile)");
+                k_pass<A, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+            }
+            else
+            {
+                if ((e = allow_lds(k_pass<A, E, 5>, bytes)) != hipSuccess)
+                    return e;
+                probe_scope ps(s, "fast ntt inverse (whole tile)");
+                k_pass<A, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
+            }
+            return hipGetLastError();
+        }
+
         template<typename A>
         hipError_t launch_ntt(transform_args const& T, typename A::params const& P, std::size_t n, bool inverse,
                               u32 const* in, u32* out, hipStream_t s)
         {
+            // the latency schedule of small whole-tile Goldilocks calls: 4 elements per thread
+            // (1024 threads per tile)
+            if constexpr (A::rl == 2)
+            {
+                if (T.elems == 4 && T.log_n <= tile_log)
+                    return launch_whole_tiles<A, 4>(T, P, n, inverse, in, out, s);
+            }
             return with_elems<A>(T.elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
                 constexpr int TH = tile / E;
@@ -1171,29 +1206,7 @@ namespace hmpc_amd
                 u32 const* tw = inverse ? T.inv_fast : T.fwd_fast;
                 hipError_t e = hipSuccess;
                 if (T.log_n <= tile_log)
-                {
-                    pass_geom G = geom_for(T.log_n, true);
-                    unsigned grid = static_cast<unsigned>((n + (std::size_t{1} << G.log_seqs) - 1) >> G.log_seqs);
-                    if (!inverse)
-                    {
-                        if ((e = allow_lds(k_pass<A, E, 4>, bytes)) != hipSuccess)
-                            return e;
-                        {
-                            probe_scope ps(s, "fast ntt forward (whole tile)");
-                            k_pass<A, E, 4><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
-                        }
-                    }
-                    else
-                    {
-                        if ((e = allow_lds(k_pass<A, E, 5>, bytes)) != hipSuccess)
-                            return e;
-                        {
-                            probe_scope ps(s, "fast ntt inverse (whole tile)");
-                            k_pass<A, E, 5><<<grid, TH, bytes, s>>>(in, out, tw, G, n, P);
-                        }
-                    }
-                    return hipGetLastError();
-                }
+                    return launch_whole_tiles<A, E>(T, P, n, inverse, in, out, s);
                 pass_geom gc = geom_for(T.log_n, false);
                 pass_geom gr = geom_for(T.log_n, true);
                 unsigned grid_c = static_cast<unsigned>(n * gc.blocks_per_poly);
