@@ -108,7 +108,9 @@ int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
  * job kernel in which each workgroup runs whole products / encryptions; 0: one launch per
  * transform pass; -1: unchanged; -2: the default, which is the persistent kernel for uniform
  * products and the tile launches in job batches and serving). Chosen by hmpc_autotune; results
- * are identical. */
+ * are identical. Without a set value, an hmpc_ntt call on 2 limbs too small to give every
+ * compute unit a tile runs more threads per tile (4 elements per thread for N <= 4096, 8 above)
+ * for latency; hmpc_modulus_get_schedule reports the set or default value. */
 int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int persistent);
 /* Schedule of fast-path BGV encryption for N = 2^log_n: elements per thread (0 = default) and, for
  * N > 4096, tiles = 1 (a column launch and a row launch over all ciphertexts' tiles), 0 (the
