@@ -141,6 +141,21 @@ namespace hmpc_amd
                 w[4] = tw[8 * static_cast<std::size_t>(idx) + 4];
             }
 
+            // the same entry from a 32-bit byte offset off the uniform table base (tables stay far
+            // below 4 GiB): one shift per twiddle instead of two 64-bit address computations. The
+            // row kernels use it (product rows 2.3% faster); the column kernels ran 3-4% slower.
+            __device__ __forceinline__ static void tw_get_offset32(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
+            {
+                char const* base = reinterpret_cast<char const*>(tw);
+                u32 const off = idx << 5;
+                uint4 v = *reinterpret_cast<uint4 const*>(base + off);
+                w[0] = v.x;
+                w[1] = v.y;
+                w[2] = v.z;
+                w[3] = v.w;
+                w[4] = *reinterpret_cast<u32 const*>(base + off + 16u);
+            }
+
             // (x, y) -> (x + t, x - t), t = mont(y, w), on signed digits
             __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
             {
@@ -210,19 +225,9 @@ namespace hmpc_amd
         {
             static constexpr bool wave_units = true;  // wave-local row tiles (rows of <= 64 E elements)
 
-            // entry idx of the 8-word twiddle table from a 32-bit byte offset off the uniform table
-            // base (tables stay far below 4 GiB): one shift per twiddle instead of two 64-bit address
-            // computations. 2.7% faster in the row kernel; the column kernels ran slower with it.
             __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
             {
-                char const* base = reinterpret_cast<char const*>(tw);
-                u32 const off = idx << 5;
-                uint4 v = *reinterpret_cast<uint4 const*>(base + off);
-                w[0] = v.x;
-                w[1] = v.y;
-                w[2] = v.z;
-                w[3] = v.w;
-                w[4] = *reinterpret_cast<u32 const*>(base + off + 16u);
+                tw_get_offset32(w, tw, idx);
             }
 
             __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 3) & 63) ^ ((e >> 10) & 15); }
@@ -244,6 +249,15 @@ namespace hmpc_amd
             }
         };
 
+        // the BGV row kernels: twiddles from 32-bit offsets as in the product's row kernel
+        struct arith_r26_bgv_rows : arith_r26
+        {
+            __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
+            {
+                tw_get_offset32(w, tw, idx);
+            }
+        };
+
         // the arithmetic of the product's row kernel: the radix-2^26 rows are wave-local in the row
         // kernels' LDS slot order (twiddles stage by stage: with signed-digit forward butterflies
         // this ran 2.4% faster than whole-round prefetch)
@@ -256,6 +270,17 @@ namespace hmpc_amd
         struct rows_arith<arith_r26>
         {
             using type = arith_r26_rows;
+        };
+
+        template<typename A>
+        struct bgv_rows_arith
+        {
+            using type = A;
+        };
+        template<>
+        struct bgv_rows_arith<arith_r26>
+        {
+            using type = arith_r26_bgv_rows;
         };
 
         struct arith_gold

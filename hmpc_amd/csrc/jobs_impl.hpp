@@ -487,13 +487,14 @@ namespace hmpc_amd
                 hipError_t e = hipSuccess;
                 // column length 2^7 from N = 2^14 on; row lengths 2^7 .. 2^10 at N = 2^14 .. 2^17
                 auto cols_kernel = gc.log_n1 == 7 ? k_bgv_cols<A, E, 7> : k_bgv_cols<A, E, 0>;
+                using AB = typename bgv_rows_arith<A>::type;
                 auto rows_kernel = k_bgv_rows<A, E, 0>;
                 switch (gr.log_n2)
                 {
-                case 7: rows_kernel = k_bgv_rows<A, E, 7>; break;
-                case 8: rows_kernel = k_bgv_rows<A, E, 8>; break;
-                case 9: rows_kernel = k_bgv_rows<A, E, 9>; break;
-                case 10: rows_kernel = k_bgv_rows<A, E, 10>; break;
+                case 7: rows_kernel = k_bgv_rows<AB, E, 7>; break;
+                case 8: rows_kernel = k_bgv_rows<AB, E, 8>; break;
+                case 9: rows_kernel = k_bgv_rows<AB, E, 9>; break;
+                case 10: rows_kernel = k_bgv_rows<AB, E, 10>; break;
                 default: break;
                 }
                 if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess)
