@@ -598,19 +598,20 @@ namespace hmpc_amd
                                                                                     typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
+            using AB = typename bgv_rows_arith<A>::type;
             u32 m = map[blockIdx.x];
             job_desc J = jobs[m >> 10];
             u32 rt = m & 1023u;
             if constexpr (S > 0)
-                bgv_rows_table_tile<A, E, S>(J, lds, ws, rt, P);
+                bgv_rows_table_tile<AB, E, S>(J, lds, ws, rt, P);
             else
             {
                 switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
                 {
-                case 7: bgv_rows_table_tile<A, E, 7>(J, lds, ws, rt, P); break;
-                case 8: bgv_rows_table_tile<A, E, 8>(J, lds, ws, rt, P); break;
-                case 9: bgv_rows_table_tile<A, E, 9>(J, lds, ws, rt, P); break;
-                case 10: bgv_rows_table_tile<A, E, 10>(J, lds, ws, rt, P); break;
+                case 7: bgv_rows_table_tile<AB, E, 7>(J, lds, ws, rt, P); break;
+                case 8: bgv_rows_table_tile<AB, E, 8>(J, lds, ws, rt, P); break;
+                case 9: bgv_rows_table_tile<AB, E, 9>(J, lds, ws, rt, P); break;
+                case 10: bgv_rows_table_tile<AB, E, 10>(J, lds, ws, rt, P); break;
                 default: bgv_rows_table_tile<A, E, 0>(J, lds, ws, rt, P); break;
                 }
             }
