@@ -14,8 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmpc_hip.so")
-SOURCES = ["kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip", "kernels_fast_l4.hip", "kernels_jobs_l2.hip",
-           "kernels_jobs_l4.hip", "kernels_ext.hip", "api.hip"]
+# the job kernels (jobs_impl.hpp) are instantiated in one unit per launch family and limb count, so
+# that the slowest units compile in parallel
+SOURCES = ["kernels_jobs_l2.hip", "kernels_jobs_l2_e16.hip", "kernels_jobs_l2_bgv.hip", "kernels_jobs_l2_mul.hip", "kernels_jobs_l4.hip",
+           "kernels_jobs_l4_bgv.hip", "kernels_jobs_l4_mul.hip", "kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip",
+           "kernels_fast_l4.hip", "kernels_ext.hip", "api.hip"]
 ARCH = os.environ.get("HMPC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -30,8 +33,22 @@ CFLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _up_to_date(src: str, obj: str) -> bool:
+    """True when obj is newer than the source and every header of csrc/ and include/ (a coarse rule
+    for the incremental developer build; build() itself always recompiles)."""
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    deps.append(os.path.join(os.path.dirname(HERE), "include", "hmpc_hip.h"))
+    return all(os.path.getmtime(d) < t for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and _up_to_date(src, obj):
+        return obj
     cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     t0 = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,10 +60,13 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every unit (incremental=True skips units whose object is newer than all sources) and
+    link hmpc_amd/libhmpc_hip.so."""
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    workers = max(1, min(len(SOURCES), os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(lambda src: _compile(src, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -77,4 +97,4 @@ def build_facade_test() -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)

@@ -28,6 +28,20 @@ namespace hmpc_amd
 {
     namespace fast
     {
+        // kernel-argument constants of the Goldilocks policy
+        inline fast_params gold_params(transform_args const& T)
+        {
+            fast_params P{};
+            for (int l = 0; l < max_limbs; ++l)
+            {
+                P.q[l] = T.P.q[l];
+                P.q2[l] = T.q2[l];
+                P.r2[l] = T.r2[l];
+            }
+            P.nprime = T.P.nprime;
+            return P;
+        }
+
         // ------------------------------------------------------------------ tile placement
         __device__ __forceinline__ void job_cols_tile(pass_geom const& G, u32 t, std::size_t count, std::size_t off,
                                                       tile_desc& td, io_desc& io)
@@ -852,16 +866,38 @@ namespace hmpc_amd
             });
         }
 
+        // one elements-per-thread variant (the Goldilocks variants compile in separate units)
+        template<typename A, int E>
+        hipError_t jobs_occupancy_e(int* blocks_per_cu)
+        {
+            constexpr std::size_t bytes = tile_bytes<A>();
+            hipError_t e = allow_lds(k_jobs<A, E>, bytes);
+            if (e != hipSuccess)
+                return e;
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_jobs<A, E>, tile / E, bytes);
+        }
+
+        template<typename A, int E>
+        hipError_t launch_jobs_e(job_desc const* table, job_desc const& base, u32 njobs, unsigned grid, u32* counter,
+                                 u32* scratch, u64 slot_words, typename A::params const& P, hipStream_t s)
+        {
+            constexpr std::size_t bytes = tile_bytes<A>();
+            hipError_t e = allow_lds(k_jobs<A, E>, bytes);
+            if (e != hipSuccess)
+                return e;
+            k_reset_counter<A><<<1, 64, 0, s>>>(counter);
+            {
+                probe_scope ps(s, "fast persistent jobs");
+                k_jobs<A, E><<<grid, tile / E, bytes, s>>>(table, base, njobs, counter, scratch, slot_words, P);
+            }
+            return hipGetLastError();
+        }
+
         template<typename A>
         hipError_t jobs_occupancy(int elems, int* blocks_per_cu)
         {
             return with_elems<A>(elems, [&](auto EC) -> hipError_t {
-                constexpr int E = decltype(EC)::value;
-                constexpr std::size_t bytes = tile_bytes<A>();
-                hipError_t e = allow_lds(k_jobs<A, E>, bytes);
-                if (e != hipSuccess)
-                    return e;
-                return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_jobs<A, E>, tile / E, bytes);
+                return jobs_occupancy_e<A, decltype(EC)::value>(blocks_per_cu);
             });
         }
 
@@ -870,17 +906,7 @@ namespace hmpc_amd
                                u32* counter, u32* scratch, u64 slot_words, typename A::params const& P, hipStream_t s)
         {
             return with_elems<A>(elems, [&](auto EC) -> hipError_t {
-                constexpr int E = decltype(EC)::value;
-                constexpr std::size_t bytes = tile_bytes<A>();
-                hipError_t e = allow_lds(k_jobs<A, E>, bytes);
-                if (e != hipSuccess)
-                    return e;
-                k_reset_counter<A><<<1, 64, 0, s>>>(counter);
-                {
-                    probe_scope ps(s, "fast persistent jobs");
-                    k_jobs<A, E><<<grid, tile / E, bytes, s>>>(table, base, njobs, counter, scratch, slot_words, P);
-                }
-                return hipGetLastError();
+                return launch_jobs_e<A, decltype(EC)::value>(table, base, njobs, grid, counter, scratch, slot_words, P, s);
             });
         }
     }
