@@ -71,6 +71,12 @@ def lib():
                                          i, i, i, _u32p, _u32p, i]
         L.oracle_bgv_decrypt.argtypes = [vp, i, sz, _u32p, _u32p, _u32p, _u32p, i]
         L.oracle_schoolbook_negacyclic.argtypes = [vp, i, _u32p, _u32p, _u32p]
+        L.oracle_cast.argtypes = [vp, vp, sz, _u32p, _u32p]
+        L.oracle_from_signed_bigint.argtypes = [vp, _u32p, i, i, _u32p]
+        L.oracle_sample_capability.argtypes = [vp, _u32p, _u32p, i, i, i, i, sz, _u32p]
+        L.oracle_bgv_encrypt_plaintext.argtypes = [vp, vp, i, sz, _u32p, _u32p, _u32p, i, _u32p, _u32p, i, i, i, i,
+                                                   _u32p, _u32p, i]
+        L.oracle_bgv_decrypt_plaintext.argtypes = [vp, vp, i, sz, _u32p, _u32p, _u32p, i, _u32p, i]
         L.oracle_polymul.restype = ctypes.c_double
         L.oracle_polymul.argtypes = [vp, i, sz, _u32p, _u32p, _u32p, i]
         _lib = L
@@ -272,6 +278,61 @@ class Modulus:
             raise ValueError("oracle_bgv_decrypt failed")
         return out
 
+    # -- cross-modulus cast, capability samplers, BGV with an Rp plaintext
+    def cast_to(self, other: "Modulus", x: np.ndarray) -> np.ndarray:
+        """mod<other>(mod<self>) element-wise (centred lift, hmpc/ints/mod.hpp:259-307)."""
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        assert x.shape[0] == self.limbs
+        out = np.empty((other.limbs, x.shape[1]), np.uint32)
+        lib().oracle_cast(self._h, other._h, x.shape[1], _p(x), _p(out))
+        return out
+
+    def from_signed_bigint(self, value: int, bits: int) -> int:
+        """mod(sbigint<bits>) of a python int in [-2^(bits-1), 2^(bits-1))."""
+        words = (bits + 31) // 32
+        w = int_to_limbs(value % (1 << (32 * words)), words)
+        out = np.zeros(self.limbs, np.uint32)
+        lib().oracle_from_signed_bigint(self._h, _p(w), words, bits, _p(out))
+        return limbs_to_int(out)
+
+    DISTRIBUTIONS = {"uniform": 0, "centered_binomial": 1, "binomial": 2, "drown_signed": 3, "drown_unsigned": 4}
+    LAYOUTS = {"element": 0, "ntt": 1, "intt": 2}
+
+    def sample_capability(self, key, nonce, dist: str, bits: int, layout: str, log_n: int, n: int) -> np.ndarray:
+        k = key_array(key)
+        nz = np.array(nonce, dtype=np.uint32)
+        out = np.empty((self.limbs, n << log_n), np.uint32)
+        rc = lib().oracle_sample_capability(self._h, _p(k), _p(nz), self.DISTRIBUTIONS[dist], bits, self.LAYOUTS[layout],
+                                            log_n, n, _p(out))
+        if rc:
+            raise ValueError("oracle_sample_capability failed")
+        return out
+
+    def bgv_encrypt_plaintext(self, pmod: "Modulus", log_n, pk_a, pk_b, plaintext, plaintext_ntt: bool, key, nonce,
+                              bits_u=2, v_dist="centered_binomial", v_bits=40, bits_w=40, threads: int = 1):
+        n = plaintext.shape[1] >> log_n
+        k = key_array(key)
+        nz = np.array(nonce, dtype=np.uint32).copy()
+        x = np.ascontiguousarray(plaintext, dtype=np.uint32)
+        c0 = np.empty((self.limbs, n << log_n), np.uint32)
+        c1 = np.empty_like(c0)
+        rc = lib().oracle_bgv_encrypt_plaintext(self._h, pmod._h, log_n, n, _p(np.ascontiguousarray(pk_a)),
+                                                _p(np.ascontiguousarray(pk_b)), _p(x), int(plaintext_ntt), _p(k), _p(nz),
+                                                bits_u, self.DISTRIBUTIONS[v_dist], v_bits, bits_w, _p(c0), _p(c1), threads)
+        if rc:
+            raise ValueError("oracle_bgv_encrypt_plaintext failed")
+        return c0, c1, nz
+
+    def bgv_decrypt_plaintext(self, pmod: "Modulus", log_n, s_ntt, c0, c1, plaintext_ntt: bool, threads: int = 1):
+        n = c0.shape[1] >> log_n
+        out = np.empty((pmod.limbs, n << log_n), np.uint32)
+        rc = lib().oracle_bgv_decrypt_plaintext(self._h, pmod._h, log_n, n, _p(np.ascontiguousarray(s_ntt)),
+                                                _p(np.ascontiguousarray(c0)), _p(np.ascontiguousarray(c1)),
+                                                int(plaintext_ntt), _p(out), threads)
+        if rc:
+            raise ValueError("oracle_bgv_decrypt_plaintext failed")
+        return out
+
     def schoolbook_negacyclic(self, log_n, a, b) -> np.ndarray:
         out = np.empty((self.limbs, 1 << log_n), np.uint32)
         lib().oracle_schoolbook_negacyclic(self._h, log_n, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)), _p(out))
@@ -318,6 +379,12 @@ def cipher_decrypt(words: np.ndarray, key, nonce, counter: int = 0) -> np.ndarra
     out = np.empty((limbs, count), np.uint32)
     lib().oracle_cipher_decrypt(limbs, count, _p(k), _p(nz), counter, _p(words), _p(out))
     return out
+
+
+def drown_bits(bound: int, signed: bool, security: int = 80) -> int:
+    """Width of drown_uniform's draw: bit_width((bound - 1) << security) + signed
+    (hmpc/random/uniform.hpp:34-47)."""
+    return ((bound - 1) << security).bit_length() + int(bool(signed))
 
 
 def sample_words(key, offset: int, count: int) -> np.ndarray:

@@ -22,6 +22,15 @@
 //                             hmpc/expr/number_theoretic_transform.hpp:320-333
 //   bgv encrypt / decrypt     decrypt round trip (tests/expr/crypto/lhe/enc.cpp:65-72 shape),
 //                             linear-algebra recomputation in python big ints
+//   cast (centred lift)       the cross-modulus KATs of tests/ints/mod.cpp:60-78 (q = 99 against the
+//                             150-bit p) and a python big-int closed form (signed centred value
+//                             mod Q, Montgomery form) over edge values in both directions
+//   capability samplers       python recomputation from ChaCha words (textbook ChaCha, tests/bigref.py)
+//                             per layout and distribution; signed conversions against the
+//                             KATs of tests/ints/mod.cpp:50-56 and closed forms
+//   bgv with an Rp plaintext  python big-int recomputation at small N of c0 = A U + p V + M and
+//                             c1 = B U + p W from independently computed samples and closed-form
+//                             DFTs, and decryption of hmpc's own test case (enc.cpp:15-72)
 //   cipher_encrypt / decrypt  RFC 8439 section 2.4.2 vector held in tests/crypto/cipher.cpp:8-36,
 //                             the round trip of tests/expr/crypto/cipher.cpp, and the continuous
 //                             keystream of chacha20_stream (one stream across hmpc's batches)
@@ -447,7 +456,7 @@ namespace oracle
     // hmpc/random/binomial.hpp:30-123: popcount of `bits` random bits minus bits/2.
     inline long long centered_binomial(number_generator& g, int bits)
     {
-        u32 w[8];
+        u32 w[8] = {};
         g.uniform_bits(w, bits);
         int words = (bits + 31) / 32;
         long long pop = 0;
@@ -613,6 +622,237 @@ namespace oracle
     {
         for (int l = 0; l < L; ++l)
             planes[static_cast<size_t>(l) * count + index] = x[l];
+    }
+
+    // ---------------------------------------------------------------- cross-modulus cast
+    // value mod q (value of value_limbs limbs, q of L limbs) by binary long division
+    inline void bigint_mod(u32* result, u32 const* value, int value_limbs, u32 const* q, int L)
+    {
+        u32 r[max_limbs + 1] = {};
+        u32 qext[max_limbs + 1] = {};
+        std::copy(q, q + L, qext);
+        for (int bit = 32 * value_limbs - 1; bit >= 0; --bit)
+        {
+            u32 d[max_limbs + 1] = {};
+            add_limbs(d, r, r, L + 1);
+            d[0] |= (value[bit / 32] >> (bit % 32)) & 1u;
+            u32 s[max_limbs + 1];
+            u32 underflow = sub_limbs(s, d, qext, L + 1);
+            std::copy(underflow ? d : s, (underflow ? d : s) + L + 1, r);
+        }
+        std::copy(r, r + L, result);
+    }
+
+    // hmpc/ints/mod.hpp:146-158 (offset_to<Other>): (modulus - Other mod modulus) mod modulus
+    inline void offset_to(u32* result, u32 const* other, int other_limbs, modulus const& m)
+    {
+        int L = m.limbs;
+        u32 r[max_limbs];
+        bigint_mod(r, other, other_limbs, m.q, L);
+        if (mod_is_zero(r, L))
+            std::fill(result, result + L, 0u);
+        else
+            sub_limbs(result, m.q, r, L);
+    }
+
+    // limbs needed for an integer of `bits` bits (at least one)
+    inline int limbs_for_bits(int bits) { return bits <= 0 ? 1 : (bits + 31) / 32; }
+
+    // hmpc/ints/mod.hpp:259-307 (mod<Q>(mod<P>)): the canonical integer v of x in `from`; values
+    // above floor(P / 2) are shifted by an offset congruent to -P mod Q (Q > P: Q - P; otherwise
+    // offset_to<P>), then the integer enters `to` as an unsigned value (one REDC, or two for a value
+    // wider than to's R, :165-189).
+    inline void mod_cast(u32* result, u32 const* x, modulus const& from, modulus const& to)
+    {
+        int Lf = from.limbs, Lt = to.limbs;
+        int W = std::max(Lf, Lt) + 1;
+        u32 v[max_limbs + 1] = {};
+        mod_to_unsigned(v, x, from);
+        u32 half[max_limbs + 1] = {};
+        for (int l = 0; l < Lf; ++l)
+            half[l] = (from.q[l] >> 1) | (l + 1 < Lf ? from.q[l + 1] << 31 : 0u);
+        bool above_half = !greater_equal(half, Lf, v, Lf);
+        u32 offset[max_limbs + 1] = {};
+        u32 normalized[max_limbs + 2] = {};
+        if (!greater_equal(from.q, Lf, to.q, Lt))  // to.q > from.q
+        {
+            u32 fq[max_limbs + 1] = {};
+            std::copy(from.q, from.q + Lf, fq);
+            sub_limbs(offset, to.q, fq, Lt);
+            if (!above_half)
+                std::fill(offset, offset + Lt, 0u);
+            add_limbs(normalized, v, offset, Lt);
+            mod_from_unsigned(result, normalized, Lt, to);
+        }
+        else
+        {
+            offset_to(offset, from.q, Lf, to);
+            // max value P + offset - 1 fixes the width of the normalized integer
+            u32 top[max_limbs + 2] = {};
+            u32 fq[max_limbs + 2] = {};
+            std::copy(from.q, from.q + Lf, fq);
+            add_limbs(top, fq, offset, W);
+            u32 one[max_limbs + 2] = {1};
+            sub_limbs(top, top, one, W);
+            int width = limbs_for_bits(bit_width(top, W));
+            if (!above_half)
+                std::fill(offset, offset + Lt, 0u);
+            add_limbs(normalized, v, offset, W);
+            mod_from_unsigned(result, normalized, width, to);
+        }
+    }
+
+    // ---------------------------------------------------------------- capability samplers
+    enum distribution
+    {
+        dist_uniform = 0,         // hmpc/expr/random/uniform.hpp:41-45 -> hmpc/random/uniform.hpp:61-66
+        dist_centered_binomial = 1,  // hmpc/expr/random/binomial.hpp (centred), hmpc/random/binomial.hpp:88-128
+        dist_binomial = 2,        // hmpc/expr/random/binomial.hpp:42-46 (uncentred), hmpc/random/binomial.hpp:30-58
+        dist_drown_signed = 3,    // hmpc/expr/random/uniform.hpp:91-95 with_sign, hmpc/random/uniform.hpp:34-47
+        dist_drown_unsigned = 4,  // the same without sign
+    };
+
+    // hmpc/ints/mod.hpp:228-253 (mod(sbigint<Bits>) for Bits >= bit_size) and :219-226 (Bits <
+    // bit_size): a signed integer of `bits` bits in two's complement over `words` words
+    inline void mod_from_signed_bigint(u32* result, u32 const* value, int words, int bits, modulus const& m)
+    {
+        int L = m.limbs;
+        bool negative = (value[(bits - 1) / 32] >> ((bits - 1) % 32)) & 1u;
+        if (bits < m.bit_size)
+        {
+            // q masked by the sign plus the (sign-extended) value, in bit_size bits
+            u32 ext[max_limbs] = {};
+            for (int l = 0; l < L; ++l)
+                ext[l] = l < words ? value[l] : (negative ? 0xffffffffu : 0u);
+            u32 qm[max_limbs] = {};
+            for (int l = 0; l < L; ++l)
+                qm[l] = negative ? m.q[l] : 0u;
+            u32 v[max_limbs];
+            add_limbs(v, ext, qm, L);
+            if (m.bit_size % 32)
+                v[L - 1] &= (1u << (m.bit_size % 32)) - 1u;
+            mod_from_unsigned(result, v, L, m);
+            return;
+        }
+        // M = 2^(bits - 1) + offset_to<2^(bits - 1)>: the next multiple of q at or above 2^(bits - 1)
+        int W = std::max(words, L) + 1;
+        u32 pow2[max_limbs + 2] = {};
+        pow2[(bits - 1) / 32] = 1u << ((bits - 1) % 32);
+        u32 off[max_limbs] = {};
+        offset_to(off, pow2, W, m);
+        u32 M[max_limbs + 2] = {}, offw[max_limbs + 2] = {};
+        std::copy(off, off + L, offw);
+        add_limbs(M, pow2, offw, W);
+        u32 top[max_limbs + 2] = {}, one[max_limbs + 2] = {1};
+        sub_limbs(top, M, one, W);
+        int width = limbs_for_bits(bit_width(top, W));
+        u32 ext[max_limbs + 2] = {};
+        for (int l = 0; l < W; ++l)
+            ext[l] = l < words ? value[l] : (negative ? 0xffffffffu : 0u);
+        if (!negative)
+            std::fill(M, M + W, 0u);
+        u32 normalized[max_limbs + 2] = {};
+        add_limbs(normalized, ext, M, W);  // value + M for negative values (mod 2^(32 W))
+        mod_from_unsigned(result, normalized, width, m);
+    }
+
+    // one draw of `dist` from the generator, in Montgomery form
+    inline void sample_one(u32* out, number_generator& g, int dist, int bits, modulus const& m)
+    {
+        switch (dist)
+        {
+        case dist_uniform:
+            uniform_mod(out, g, m);
+            return;
+        case dist_centered_binomial:
+            mod_from_small_signed(out, centered_binomial(g, bits), m);
+            return;
+        case dist_binomial:
+        {
+            // popcount of `bits` random bits as an unsigned integer (hmpc/random/binomial.hpp:30-58)
+            u32 w[16] = {};
+            g.uniform_bits(w, bits);
+            u32 pop = 0;
+            for (int i = 0; i < (bits + 31) / 32; ++i)
+                pop += static_cast<u32>(__builtin_popcount(w[i]));
+            mod_from_unsigned(out, &pop, 1, m);
+            return;
+        }
+        case dist_drown_signed:
+        case dist_drown_unsigned:
+        {
+            // a bigint of `bits` bits: ceil(bits / 32) words, the top one masked (unsigned) or
+            // sign-extended from bit bits - 1 (signed), hmpc/core/bit_span.hpp:180-200
+            int words = (bits + 31) / 32;
+            u32 w[16] = {};
+            for (int i = 0; i < words; ++i)
+                w[i] = g.next();
+            int top = bits % 32;
+            if (dist == dist_drown_unsigned)
+            {
+                if (top)
+                    w[words - 1] &= (1u << top) - 1u;
+                mod_from_unsigned(out, w, words, m);
+            }
+            else
+            {
+                if (top)
+                {
+                    u32 sign = (w[words - 1] >> (top - 1)) & 1u;
+                    u32 mask = (1u << top) - 1u;
+                    w[words - 1] = sign ? (w[words - 1] | ~mask) : (w[words - 1] & mask);
+                }
+                mod_from_signed_bigint(out, w, words, bits, m);
+            }
+            return;
+        }
+        default:
+            std::fill(out, out + m.limbs, 0u);
+        }
+    }
+
+    enum capability_layout
+    {
+        layout_element = 0,  // element-wise kernel: one stream per element (hmpc/comp/queue.hpp:387-412)
+        layout_ntt = 1,      // forward NTT stage 0: pair (t, t + N/2) from the stream of index t
+        layout_intt = 2,     // inverse NTT stage 0: pair (2t, 2t + 1) from the stream of index 2t
+    };
+
+    // Capability-path samples of n polynomials of N = 2^log_n coefficients (coefficient domain, in
+    // the order the consuming kernel reads them). The capability of linear index i is the ChaCha20
+    // stream (key, kernel nonce, counter = low32(i)) (hmpc/comp/queue.hpp:185-218).
+    //   element: coefficient e draws from the stream of e (hmpc/comp/queue.hpp:400-409)
+    //   ntt:     t < N/2 of polynomial b: stream b N + t gives coefficient t, then t + N/2
+    //            (hmpc/expr/number_theoretic_transform.hpp:320-329)
+    //   intt:    stream b N + 2t gives coefficient 2t, then 2t + 1 (:480-489)
+    inline void sample_capability(modulus const& m, u32 const* key, u32 const* nonce, int dist, int bits, int layout,
+                                  int log_n, size_t n, u32* out)
+    {
+        int L = m.limbs;
+        size_t N = size_t{1} << log_n, total = n * N;
+        u32 x[max_limbs];
+        if (layout == layout_element)
+        {
+            for (size_t e = 0; e < total; ++e)
+            {
+                number_generator g(key, nonce, static_cast<u32>(e));
+                sample_one(x, g, dist, bits, m);
+                store(out, total, e, x, L);
+            }
+            return;
+        }
+        size_t half = N / 2;
+        for (size_t b = 0; b < n; ++b)
+            for (size_t t = 0; t < half; ++t)
+            {
+                size_t first = layout == layout_ntt ? b * N + t : b * N + 2 * t;
+                size_t second = layout == layout_ntt ? first + half : first + 1;
+                number_generator g(key, nonce, static_cast<u32>(first));
+                sample_one(x, g, dist, bits, m);
+                store(out, total, first, x, L);
+                sample_one(x, g, dist, bits, m);
+                store(out, total, second, x, L);
+            }
     }
 
     // ---------------------------------------------------------------- NTT
@@ -1138,6 +1378,140 @@ extern "C"
         }
         for (size_t k = 0; k < N; ++k)
             store(out, N, k, &acc[k * L], L);
+        return 0;
+    }
+
+    // ------------------------------------------------------------------ cast, capability samplers
+    void oracle_cast(void* from, void* to, size_t count, u32 const* in, u32* out)
+    {
+        auto& f = *static_cast<modulus*>(from);
+        auto& t = *static_cast<modulus*>(to);
+        for (size_t e = 0; e < count; ++e)
+        {
+            u32 x[max_limbs], y[max_limbs];
+            load(x, in, count, e, f.limbs);
+            mod_cast(y, x, f, t);
+            store(out, count, e, y, t.limbs);
+        }
+    }
+
+    void oracle_from_signed_bigint(void* mp, u32 const* value, int words, int bits, u32* out)
+    {
+        mod_from_signed_bigint(out, value, words, bits, *static_cast<modulus*>(mp));
+    }
+
+    int oracle_sample_capability(void* mp, u32 const* key, u32 const* nonce, int dist, int bits, int layout, int log_n,
+                                 size_t n, u32* out)
+    {
+        if (layout != layout_element && log_n < 1)
+            return 1;
+        sample_capability(*static_cast<modulus*>(mp), key, nonce, dist, bits, layout, log_n, n, out);
+        return 0;
+    }
+
+    // ------------------------------------------------------------------ BGV with an Rp plaintext
+    // hmpc/expr/crypto/lhe/enc.hpp:19-69 for a plaintext over mod p (mp):
+    //   message = NTT_q(cast<mod q>(INTT_p(x)))  (NTT-form plaintext; coefficient form: no INTT_p)
+    //   p = the plaintext modulus as a mod q constant (:59-63, constant_cast = mod q from unsigned)
+    //   u = NTT(CBD(bits_u)), v = NTT(v_dist(v_bits)), w = NTT(CBD(bits_w)) on the capability path
+    //   (layout ntt), nonces nonce, nonce + nN, nonce + 2nN in hmpc's cache order (u, v, w; the
+    //   message transforms draw no randomness); v_dist = centred binomial is randomness()
+    //   (randomness.hpp:80-110), drown_signed is drowning_randomness() (:112-143)
+    //   c0 = A u + v p + message, c1 = B u + w p
+    int oracle_bgv_encrypt_plaintext(void* mqp, void* mpp, int log_n, size_t n, u32 const* pk_a, u32 const* pk_b,
+                                     u32 const* plaintext, int plaintext_ntt, u32 const* key, u32* nonce, int bits_u,
+                                     int v_dist, int v_bits, int bits_w, u32* c0, u32* c1, int threads)
+    {
+        auto& mq = *static_cast<modulus*>(mqp);
+        auto& mp = *static_cast<modulus*>(mpp);
+        int L = mq.limbs, Lp = mp.limbs;
+        size_t N = size_t{1} << log_n, total = n * N;
+        std::vector<u32> fr;
+        if (!make_roots(fr, mq, log_n, false))
+            return 1;
+        std::vector<u32> xp(total * Lp), tmp(total * L), msg(total * L), u(total * L), v(total * L), w(total * L);
+        if (plaintext_ntt)
+        {
+            std::vector<u32> ipr;
+            if (!make_roots(ipr, mp, log_n, true))
+                return 2;
+            ntt_inverse(mp, log_n, n, plaintext, xp.data(), ipr, threads);
+        }
+        else
+            std::copy(plaintext, plaintext + total * Lp, xp.begin());
+        oracle_cast(mpp, mqp, total, xp.data(), tmp.data());
+        ntt_forward(mq, log_n, n, tmp.data(), msg.data(), fr, threads);
+        u32 p_const[max_limbs];
+        mod_from_unsigned(p_const, mp.q, Lp, mq);
+        sample_capability(mq, key, nonce, dist_centered_binomial, bits_u, layout_ntt, log_n, n, tmp.data());
+        nonce_add(nonce, total);
+        ntt_forward(mq, log_n, n, tmp.data(), u.data(), fr, threads);
+        sample_capability(mq, key, nonce, v_dist, v_bits, layout_ntt, log_n, n, tmp.data());
+        nonce_add(nonce, total);
+        ntt_forward(mq, log_n, n, tmp.data(), v.data(), fr, threads);
+        sample_capability(mq, key, nonce, dist_centered_binomial, bits_w, layout_ntt, log_n, n, tmp.data());
+        nonce_add(nonce, total);
+        ntt_forward(mq, log_n, n, tmp.data(), w.data(), fr, threads);
+        parallel_for(total, threads, [&](size_t st, size_t e) {
+            for (size_t i = st; i < e; ++i)
+            {
+                u32 a[max_limbs], b[max_limbs], uu[max_limbs], x[max_limbs], t1[max_limbs], t2[max_limbs];
+                load(a, pk_a, N, i % N, L);
+                load(b, pk_b, N, i % N, L);
+                load(uu, u.data(), total, i, L);
+                mod_mul(t1, a, uu, mq);
+                load(x, v.data(), total, i, L);
+                mod_mul(t2, x, p_const, mq);
+                mod_add(t1, t1, t2, mq);
+                load(x, msg.data(), total, i, L);
+                mod_add(t1, t1, x, mq);
+                store(c0, total, i, t1, L);
+                mod_mul(t1, b, uu, mq);
+                load(x, w.data(), total, i, L);
+                mod_mul(t2, x, p_const, mq);
+                mod_add(t1, t1, t2, mq);
+                store(c1, total, i, t1, L);
+            }
+        });
+        return 0;
+    }
+
+    // hmpc/expr/crypto/lhe/dec.hpp:13-45: x = INTT_q(c0 - s c1), cast to mod p (centred lift), then
+    // NTT_p for an NTT-form plaintext
+    int oracle_bgv_decrypt_plaintext(void* mqp, void* mpp, int log_n, size_t n, u32 const* s_ntt, u32 const* c0,
+                                     u32 const* c1, int plaintext_ntt, u32* out, int threads)
+    {
+        auto& mq = *static_cast<modulus*>(mqp);
+        auto& mp = *static_cast<modulus*>(mpp);
+        int L = mq.limbs, Lp = mp.limbs;
+        size_t N = size_t{1} << log_n, total = n * N;
+        std::vector<u32> ir;
+        if (!make_roots(ir, mq, log_n, true))
+            return 1;
+        std::vector<u32> x(total * L), y(total * L), z(total * Lp);
+        parallel_for(total, threads, [&](size_t st, size_t e) {
+            for (size_t i = st; i < e; ++i)
+            {
+                u32 a[max_limbs], b[max_limbs], s[max_limbs], t[max_limbs];
+                load(a, c0, total, i, L);
+                load(b, c1, total, i, L);
+                load(s, s_ntt, N, i % N, L);
+                mod_mul(t, s, b, mq);
+                mod_sub(t, a, t, mq);
+                store(x.data(), total, i, t, L);
+            }
+        });
+        ntt_inverse(mq, log_n, n, x.data(), y.data(), ir, threads);
+        if (plaintext_ntt)
+        {
+            std::vector<u32> fpr;
+            if (!make_roots(fpr, mp, log_n, false))
+                return 2;
+            oracle_cast(mqp, mpp, total, y.data(), z.data());
+            ntt_forward(mp, log_n, n, z.data(), out, fpr, threads);
+        }
+        else
+            oracle_cast(mqp, mpp, total, y.data(), out);
         return 0;
     }
 

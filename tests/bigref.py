@@ -107,3 +107,87 @@ def schoolbook_negacyclic(a, b, q):
             else:
                 c[k - N] -= a[i] * b[j]
     return [x % q for x in c]
+
+
+class Stream:
+    """hmpc's number_generator: 32-bit words of the ChaCha20 stream (key, nonce, counter), one block
+    after another (the counter advances per block)."""
+
+    def __init__(self, key, nonce, counter):
+        self.key, self.nonce, self.counter = list(key) + [0] * (8 - len(key)), list(nonce), counter
+        self.buf = []
+
+    def next(self):
+        if not self.buf:
+            self.buf = chacha20_block(self.key, self.counter & 0xFFFFFFFF, self.nonce)
+            self.counter += 1
+        return self.buf.pop(0)
+
+    def bits(self, bits):
+        words = (bits + 31) // 32
+        v = sum(self.next() << (32 * i) for i in range(words))
+        return v & ((1 << bits) - 1)
+
+
+def sample_mont(st, dist, bits, q):
+    """One capability draw of a distribution, as the stored Montgomery word value (R = 2^(32 L))."""
+    L = limbs_of(q)
+    R = 1 << (32 * L)
+    if dist == "uniform":
+        b = (q - 1).bit_length() + 80
+        W = (b + 31) // 32
+        v = st.bits(b)
+        return v * pow(1 << (32 * max(L, W)), -1, q) % q
+    if dist == "centered_binomial":
+        val = bin(st.bits(bits)).count("1") - bits // 2
+    elif dist == "binomial":
+        val = bin(st.bits(bits)).count("1")
+    elif dist == "drown_unsigned":
+        words = (bits + 31) // 32
+        val = sum(st.next() << (32 * i) for i in range(words)) & ((1 << bits) - 1)
+    elif dist == "drown_signed":
+        words = (bits + 31) // 32
+        raw = sum(st.next() << (32 * i) for i in range(words)) & ((1 << bits) - 1)
+        val = raw - (1 << bits) if raw >> (bits - 1) else raw
+    else:
+        raise ValueError(dist)
+    return val % q * R % q
+
+
+def capability_samples(key, nonce, dist, bits, layout, log_n, n, q):
+    """Capability-path samples (stored Montgomery values) of n polynomials of N coefficients:
+    element: coefficient e from the stream with counter e; ntt: the stream of b N + t gives t, then
+    t + N/2; intt: the stream of b N + 2t gives 2t, then 2t + 1."""
+    N = 1 << log_n
+    out = [0] * (n * N)
+    if layout == "element":
+        for e in range(n * N):
+            out[e] = sample_mont(Stream(key, nonce, e), dist, bits, q)
+        return out
+    for b in range(n):
+        for t in range(N // 2):
+            first = b * N + t if layout == "ntt" else b * N + 2 * t
+            second = first + N // 2 if layout == "ntt" else first + 1
+            st = Stream(key, nonce, first)
+            out[first] = sample_mont(st, dist, bits, q)
+            out[second] = sample_mont(st, dist, bits, q)
+    return out
+
+
+def inverse_negacyclic_dft(A, phi, q):
+    """a_i = N^-1 sum_j A_j phi^(-(2 brv(j) + 1) i): the inverse of negacyclic_dft."""
+    N = len(A)
+    bits = N.bit_length() - 1
+    ninv = pow(N, -1, q)
+    out = []
+    for i in range(N):
+        acc = 0
+        for j in range(N):
+            acc += A[j] * pow(phi, -(2 * brv(j, bits) + 1) * i, q)
+        out.append(acc * ninv % q)
+    return out
+
+
+def centred(v, p):
+    """The centred representative of v in [0, p): values above floor(p / 2) are v - p."""
+    return v - p if v > p // 2 else v

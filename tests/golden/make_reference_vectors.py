@@ -8,6 +8,9 @@ Sources (read as text, nothing is compiled or run):
   tests/ints/num/theory/root_of_unity.cpp:8-9   p = 1601, n = 64
   tests/crypto/cipher.cpp:8-36         RFC 8439 section 2.4.2 encryption (key, nonce, counter 1, 114 bytes)
   tests/expr/crypto/cipher.cpp:8-30    tensor cipher inputs: ubigint<127> x_i = i, shape {4, 2}, key {42}, nonce {1}
+  tests/ints/mod.cpp:50-78             signed constructors and the cross-modulus constructors (q = 99)
+  tests/expr/crypto/lhe/enc.cpp:15-72  linear BGV: p, q, N, shape, a = b = iota (raw words), s = one, x = iota
+  examples/bgv.cpp:25-29               the BGV example's q, N and plaintext modulus p
 
 Usage: python tests/golden/make_reference_vectors.py [/root/reference]
 """
@@ -90,8 +93,34 @@ def mod_vectors():
     r_mod_p = int(re.search(r"reduced_auxiliary_modulus == 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
     m512 = int(re.search(r"fivetwelve_montgomery = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
     big = int(re.search(r"mod_p\(0x(2faeadbe7a0195c011ac195ad10269830e80010+2a)_int\)\)", src).group(1), 16)
+    neg_big = int(re.search(r"mod_p\(-0x(2faeadbe7a0195c011ac195ad10269830e80010+2a)_int\)\) == 0x([0-9a-f']+)_int",
+                            src).group(2).replace("'", ""), 16)
+    modulo = section(src, 'SECTION("Modulo")')
+    small_q = int(re.search(r"constexpr auto q = (\d+)_int", src).group(1))
+    hi = int(re.search(r"i <= (\d+)", modulo).group(1))
     return {"p": p, "R_mod_p": r_mod_p, "mont_512": m512, "big_unsigned": big, "big_unsigned_value": 42,
-            "neg_one_value": p - 1}
+            "neg_one_value": p - 1, "neg_big_unsigned": -big, "neg_big_unsigned_value": neg_big,
+            "cross": {"q": small_q, "n_max": hi,
+                      "rules": "mod_p(mod_q(n)) == n, mod_q(mod_p(n)) == n, mod_p(mod_q(-n)) == p - n and "
+                               "mod_q(mod_p(-n)) == q - n for 0 < n <= n_max (0 for n = 0)"}}
+
+
+def lhe_vectors():
+    src = open(os.path.join(ROOT, "tests/expr/crypto/lhe/enc.cpp")).read()
+    p = int(re.search(r"constexpr auto p = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    q = int(re.search(r"constexpr auto q = 0x([0-9a-f']+)_int", src).group(1).replace("'", ""), 16)
+    log_n = int(re.search(r"N = hmpc::size\{1\} << (\d+)", src).group(1))
+    shape = [int(v) for v in re.search(r"auto shape = hmpc::shape\{(\d+), (\d+)\}", src).groups()]
+    ex = open(os.path.join(ROOT, "examples/bgv.cpp")).read()
+    eq = int(re.search(r"constexpr auto q = 0x([0-9a-f']+)_int", ex).group(1).replace("'", ""), 16)
+    elog = int(re.search(r"N = 1 << (\d+)", ex).group(1))
+    ep = int(re.search(r"mod_q\{0x([0-9a-f']+)_int\}", ex).group(1).replace("'", ""), 16)
+    return {
+        "enc_test": {"p": p, "q": q, "log_n": log_n, "shape": shape, "plaintext": "ntt",
+                     "a": "raw word i (bit_copy)", "b": "a", "s": "one (R mod q)", "x": "raw word i (bit_copy)",
+                     "check": "dec<plaintext>(s, enc(key{a, b}, x, randomness(shape))) == x"},
+        "bgv_example": {"q": eq, "log_n": elog, "p": ep},
+    }
 
 
 def poly_vectors():
@@ -120,6 +149,7 @@ def main():
         "poly_mod": poly_vectors(),
         "root_of_unity": root_vectors(),
         "cipher": cipher_vectors(),
+        "lhe": lhe_vectors(),
     }
 
     def hexify(o):
@@ -127,7 +157,7 @@ def main():
             return {k: hexify(v) for k, v in o.items()}
         if isinstance(o, list):
             return [hexify(v) for v in o]
-        if isinstance(o, int) and o >= 1 << 32:
+        if isinstance(o, int) and abs(o) >= 1 << 32:
             return hex(o)
         return o
 

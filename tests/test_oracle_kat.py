@@ -12,11 +12,17 @@ import numpy as np
 import pytest
 
 import oracle
-from bigref import (P150, Q64, Q128A, Q128B, brv, chacha20_block, is_probable_prime, limbs_of,
-                    negacyclic_dft, quarter_round, schoolbook_negacyclic, stream_words, words_to_int)
+from bigref import (P150, Q64, Q128A, Q128B, Stream, brv, capability_samples, centred, chacha20_block,
+                    inverse_negacyclic_dft, is_probable_prime, limbs_of, negacyclic_dft, quarter_round, sample_mont,
+                    schoolbook_negacyclic, stream_words, words_to_int)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 VEC = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+# moduli of hmpc's linear-BGV test (tests/expr/crypto/lhe/enc.cpp:16-17) and of its BGV example
+# (examples/bgv.cpp:25-29)
+P64_ENC = int(VEC["lhe"]["enc_test"]["p"], 16)
+Q279 = int(VEC["lhe"]["enc_test"]["q"], 16)
+P64_EXAMPLE = int(VEC["lhe"]["bgv_example"]["p"], 16)
 
 
 def _int(x):
@@ -359,6 +365,252 @@ def test_bgv_encrypt_decrypt_roundtrip(q, log_n):
     tcol = oracle.ints_to_planar([t_mont] * (n * N), m.limbs)
     want = m.pointwise("add", m.pointwise("mul", Bb, u), m.pointwise("mul", w, tcol))
     assert np.array_equal(c1, want)
+    # c0 = A*u + t*v + NTT(m) with v drawn independently at nonce nN
+    v = m.ntt(m.sample_cbd_capability(enc_key, [n * N, 0, 0], 40, log_n, n), log_n)
+    msg = m.ntt(oracle.ints_to_planar([m.from_unsigned(int(w_)) for w_ in msgs], m.limbs), log_n)
+    Ab = np.tile(A, (1, n))
+    want0 = m.pointwise("add", m.pointwise("add", m.pointwise("mul", Ab, u), m.pointwise("mul", v, tcol)), msg)
+    assert np.array_equal(c0, want0)
+
+
+@pytest.mark.parametrize("q", [Q128B, Q64])
+def test_bgv_u32_messages_closed_form(q):
+    """The 32-bit-message encryption (examples/bgv.cpp:66-69 lift, t = 2^32 as a mod q constant) against
+    python big ints: c0 = A U + t V + DFT(m), c1 = B U + t W with U, V, W from the textbook ChaCha
+    words in the ntt layout (nonces 0, nN, 2nN)."""
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    log_n, n = 3, 2
+    N = 1 << log_n
+    rng = random.Random(11)
+    A = [rng.randrange(q) for _ in range(N)]
+    B = [rng.randrange(q) for _ in range(N)]
+    msgs = np.array([rng.getrandbits(32) for _ in range(n * N)], dtype=np.uint32)
+    t = (1 << 32) % q
+    key = [42, 0, 0, 0, 0, 0, 0, 1]
+    c0, c1, _ = m.bgv_encrypt(log_n, oracle.ints_to_planar([a * R % q for a in A], m.limbs),
+                              oracle.ints_to_planar([b * R % q for b in B], m.limbs), msgs, t * R % q, key, [0, 0, 0])
+    phi = python_root_search(q, log_n + 1)[0] * pow(R, -1, q) % q
+    def dft_samples(k, bits):
+        mont = capability_samples(key, [k * n * N, 0, 0], "centered_binomial", bits, "ntt", log_n, n, q)
+        plain = [v_ * pow(R, -1, q) % q for v_ in mont]
+        return [negacyclic_dft(plain[b * N:(b + 1) * N], phi, q) for b in range(n)]
+    U, V, W = dft_samples(0, 2), dft_samples(1, 40), dft_samples(2, 40)
+    Rinv = pow(R, -1, q)
+    g0 = [v_ * Rinv % q for v_ in oracle.planar_to_ints(c0)]
+    g1 = [v_ * Rinv % q for v_ in oracle.planar_to_ints(c1)]
+    for b in range(n):
+        M = negacyclic_dft([int(w_) % q for w_ in msgs[b * N:(b + 1) * N]], phi, q)
+        for j in range(N):
+            assert g0[b * N + j] == (A[j] * U[b][j] + t * V[b][j] + M[j]) % q
+            assert g1[b * N + j] == (B[j] * U[b][j] + t * W[b][j]) % q
+
+
+# ------------------------------------------------------------------ signed and cross-modulus constructors
+def test_signed_constructor_kats():
+    """tests/ints/mod.cpp:50-56: mod_p(-1) and mod_p(-(p * 2^128 + 42)) (a signed value wider than p,
+    the 'larger signed' branch of hmpc/ints/mod.hpp:228-253)."""
+    v = VEC["mod"]
+    p = _int(v["p"])
+    m = oracle.Modulus(p)
+    big = _int(v["big_unsigned"])
+    bits = big.bit_length() + 1
+    assert m.to_unsigned(m.from_signed_bigint(-big, bits)) == _int(v["neg_big_unsigned_value"])
+    assert m.to_unsigned(m.from_signed_bigint(-1, 2)) == _int(v["neg_one_value"])
+    assert m.to_unsigned(m.from_signed_bigint(big, bits)) == v["big_unsigned_value"]
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A, P150, Q279])
+@pytest.mark.parametrize("bits", [2, 7, 64, 65, 120, 150, 151, 200, 231])
+def test_signed_bigint_closed_form(q, bits):
+    """mod(sbigint<bits>) = (value mod q) R mod q for both branches (bits < and >= bit_size)."""
+    m = oracle.Modulus(q)
+    R = 1 << (32 * m.limbs)
+    rng = random.Random(bits * 7 + q % 97)
+    lo, hi = -(1 << (bits - 1)), (1 << (bits - 1)) - 1
+    for value in [lo, hi, -1, 0, 1] + [rng.randint(lo, hi) for _ in range(20)]:
+        assert m.from_signed_bigint(value, bits) == value % q * R % q, (bits, value)
+
+
+def test_cross_modulus_kats():
+    """tests/ints/mod.cpp:60-78: q = 99 against the 150-bit p in both directions, n = 0..49, with
+    the negative values (-n) lifted centred."""
+    v = VEC["mod"]
+    p = _int(v["p"])
+    c = v["cross"]
+    q = c["q"]
+    mp, mq = oracle.Modulus(p), oracle.Modulus(q)
+    ns = list(range(c["n_max"] + 1))
+    xq = oracle.ints_to_planar([mq.from_unsigned(n) for n in ns], mq.limbs)
+    xp = oracle.ints_to_planar([mp.from_unsigned(n) for n in ns], mp.limbs)
+    nq = oracle.ints_to_planar([mq.from_small_signed(-n) for n in ns], mq.limbs)
+    np_ = oracle.ints_to_planar([mp.from_small_signed(-n) for n in ns], mp.limbs)
+    got_pq = [mp.to_unsigned(x) for x in oracle.planar_to_ints(mq.cast_to(mp, xq))]
+    got_qp = [mq.to_unsigned(x) for x in oracle.planar_to_ints(mp.cast_to(mq, xp))]
+    got_pnq = [mp.to_unsigned(x) for x in oracle.planar_to_ints(mq.cast_to(mp, nq))]
+    got_qnp = [mq.to_unsigned(x) for x in oracle.planar_to_ints(mp.cast_to(mq, np_))]
+    assert got_pq == ns and got_qp == ns
+    assert got_pnq == [p - n if n else 0 for n in ns]
+    assert got_qnp == [q - n if n else 0 for n in ns]
+
+
+CAST_PAIRS = [(P150, Q128A), (Q128A, P150), (Q279, P64_ENC), (P64_ENC, Q279), (Q64, Q128B), (Q128B, Q64),
+              (P150, P64_EXAMPLE), (99, P150)]
+
+
+@pytest.mark.parametrize("src,dst", CAST_PAIRS)
+def test_cast_closed_form(src, dst):
+    """mod<Q>(mod<P> x) = (centred(v) mod Q) R_Q mod Q with v = x R_P^-1 mod P, at the edges of the
+    centring (0, (P-1)/2, (P+1)/2, P-1) and random values."""
+    ms, md = oracle.Modulus(src), oracle.Modulus(dst)
+    Rs, Rd = 1 << (32 * ms.limbs), 1 << (32 * md.limbs)
+    rng = random.Random(src % 1009 + dst % 1013)
+    vals = [0, 1, (src - 1) // 2, (src + 1) // 2, src - 1] + [rng.randrange(src) for _ in range(40)]
+    x = oracle.ints_to_planar([v * Rs % src for v in vals], ms.limbs)
+    got = oracle.planar_to_ints(ms.cast_to(md, x))
+    assert got == [centred(v, src) % dst * Rd % dst for v in vals]
+
+
+# ------------------------------------------------------------------ capability samplers, every layout
+CAP_CASES = [("uniform", 0), ("centered_binomial", 2), ("centered_binomial", 40), ("binomial", 20),
+             ("drown_signed", oracle.drown_bits(1 << 64, True)), ("drown_unsigned", oracle.drown_bits(12345, False)),
+             ("drown_signed", 300)]
+
+
+@pytest.mark.parametrize("q", [Q64, Q128A, P150, Q279])
+@pytest.mark.parametrize("layout", ["element", "ntt", "intt"])
+@pytest.mark.parametrize("dist,bits", CAP_CASES)
+def test_capability_samplers_closed_form(q, layout, dist, bits):
+    """Every (distribution, layout) of the capability path recomputed from the textbook ChaCha20 words
+    (hmpc/comp/queue.hpp:185-218, :400-409; hmpc/expr/number_theoretic_transform.hpp:320-329, 480-489)."""
+    m = oracle.Modulus(q)
+    key, nonce = [42, 0, 0, 0, 0, 0, 0, 1], [3, 1, 0]
+    log_n, n = 3, 2
+    got = oracle.planar_to_ints(m.sample_capability(key, nonce, dist, bits, layout, log_n, n))
+    assert got == capability_samples(key, nonce, dist, bits, layout, log_n, n, q)
+
+
+def test_capability_ntt_layout_equals_the_fused_cbd_path():
+    m = oracle.Modulus(Q128A)
+    key, nonce = [42, 0, 0, 0, 0, 0, 0, 1], [9, 0, 0]
+    a = m.sample_capability(key, nonce, "centered_binomial", 40, "ntt", 5, 3)
+    assert np.array_equal(a, m.sample_cbd_capability(key, nonce, 40, 5, 3))
+
+
+def test_drown_bits_closed_form():
+    # bit_width((bound - 1) << 80) + signed (hmpc/random/uniform.hpp:34-47)
+    assert oracle.drown_bits(1 << 64, True) == 64 + 80 + 1
+    assert oracle.drown_bits(1 << 64, False) == 64 + 80
+    assert oracle.drown_bits(2, True) == 82
+    assert oracle.drown_bits(5, False, security=40) == 43
+
+
+# ------------------------------------------------------------------ BGV with an Rp plaintext
+def _python_bgv(q, p, log_n, A, B, x_mont_p, plaintext_ntt, key, nonce, bits_u, v_dist, v_bits, bits_w):
+    """hmpc/expr/crypto/lhe/enc.hpp:19-69 in python big ints from the mathematics: plaintext values
+    (Montgomery mod p) -> coefficients mod p (inverse negacyclic DFT for an NTT-form plaintext) ->
+    centred lift to q -> DFT; u, v, w from the textbook ChaCha words (ntt layout, nonces nonce,
+    nonce + nN, nonce + 2nN) -> DFT; c0 = A u + p v + M, c1 = B u + p w. Plain values in, plain out."""
+    N = 1 << log_n
+    n = len(x_mont_p) // N
+    Lq, Lp = limbs_of(q), limbs_of(p)
+    Rq, Rp = 1 << (32 * Lq), 1 << (32 * Lp)
+    phi_q = python_root_search(q, log_n + 1)[0] * pow(Rq, -1, q) % q
+    xs = [v * pow(Rp, -1, p) % p for v in x_mont_p]
+    nonce_int = nonce[0] | nonce[1] << 32 | nonce[2] << 64
+    def nonce_words(k):
+        v = nonce_int + k * n * N
+        return [v & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF, (v >> 64) & 0xFFFFFFFF]
+    def ntt_samples(k, dist, bits):
+        mont = capability_samples(key, nonce_words(k), dist, bits, "ntt", log_n, n, q)
+        plain = [v * pow(Rq, -1, q) % q for v in mont]
+        return [negacyclic_dft(plain[b * N:(b + 1) * N], phi_q, q) for b in range(n)]
+    U = ntt_samples(0, "centered_binomial", bits_u)
+    V = ntt_samples(1, v_dist, v_bits)
+    W = ntt_samples(2, "centered_binomial", bits_w)
+    c0, c1 = [], []
+    for b in range(n):
+        coeffs = xs[b * N:(b + 1) * N]
+        if plaintext_ntt:
+            phi_p = python_root_search(p, log_n + 1)[0] * pow(Rp, -1, p) % p
+            coeffs = inverse_negacyclic_dft(coeffs, phi_p, p)
+        M = negacyclic_dft([centred(c, p) % q for c in coeffs], phi_q, q)
+        for j in range(N):
+            c0.append((A[j] * U[b][j] + p * V[b][j] + M[j]) % q)
+            c1.append((B[j] * U[b][j] + p * W[b][j]) % q)
+    return c0, c1
+
+
+@pytest.mark.parametrize("q,p,plaintext_ntt,v_dist,v_bits", [
+    (P150, P64_EXAMPLE, True, "centered_binomial", 40),
+    (P150, P64_EXAMPLE, False, "centered_binomial", 40),
+    (Q279, P64_ENC, True, "centered_binomial", 40),
+    (Q128A, (1 << 64) - (1 << 32) + 1, True, "drown_signed", oracle.drown_bits(1 << 20, True)),
+])
+def test_bgv_plaintext_closed_form(q, p, plaintext_ntt, v_dist, v_bits):
+    """The oracle's Rp-plaintext encryption against the python big-int statement: pins the message path
+    (INTT_p, centred lift, NTT_q), the p v term, the capability layout and the nonce order."""
+    mq, mp = oracle.Modulus(q), oracle.Modulus(p)
+    log_n, n = 3, 2
+    N = 1 << log_n
+    Rq = 1 << (32 * mq.limbs)
+    rng = random.Random(q % 997)
+    A = [rng.randrange(q) for _ in range(N)]
+    B = [rng.randrange(q) for _ in range(N)]
+    x = [rng.randrange(p) for _ in range(n * N)]
+    key, nonce = [42, 0, 0, 0, 0, 0, 0, 1], [7, 0, 0]
+    pa = oracle.ints_to_planar([a * Rq % q for a in A], mq.limbs)
+    pb = oracle.ints_to_planar([b * Rq % q for b in B], mq.limbs)
+    px = oracle.ints_to_planar(x, mp.limbs)
+    c0, c1, nz = mq.bgv_encrypt_plaintext(mp, log_n, pa, pb, px, plaintext_ntt, key, nonce, 2, v_dist, v_bits, 40)
+    assert nz.tolist() == [7 + 3 * n * N, 0, 0]
+    w0, w1 = _python_bgv(q, p, log_n, A, B, x, plaintext_ntt, key, nonce, 2, v_dist, v_bits, 40)
+    Rinv = pow(Rq, -1, q)
+    assert [v * Rinv % q for v in oracle.planar_to_ints(c0)] == w0
+    assert [v * Rinv % q for v in oracle.planar_to_ints(c1)] == w1
+
+
+def _iota_planar(count, limbs):
+    out = np.zeros((limbs, count), np.uint32)
+    out[0] = np.arange(count, dtype=np.uint32)
+    return out
+
+
+def test_reference_linear_bgv_case():
+    """hmpc's own test, tests/expr/crypto/lhe/enc.cpp:15-72, at its size: q (279 bits, 9 limbs),
+    p = 0x8822d80623320001, N = 2^16, shape {4, 2}, key a = b = raw words i, s = one, NTT-form
+    plaintext x = raw words i; dec<plaintext>(s, enc(key, x, randomness)) == x."""
+    t = VEC["lhe"]["enc_test"]
+    q, p, log_n = int(t["q"], 16), int(t["p"], 16), t["log_n"]
+    n = t["shape"][0] * t["shape"][1]
+    N = 1 << log_n
+    mq, mp = oracle.Modulus(q), oracle.Modulus(p)
+    assert (mq.limbs, mp.limbs) == (9, 2)
+    a = _iota_planar(N, mq.limbs)
+    s = oracle.ints_to_planar([mq.from_unsigned(1)] * N, mq.limbs)
+    x = _iota_planar(n * N, mp.limbs)
+    c0, c1, nz = mq.bgv_encrypt_plaintext(mp, log_n, a, a, x, True, [1, 2, 3, 4, 5, 6, 7, 8], [0, 0, 0], threads=8)
+    assert nz.tolist() == [3 * n * N, 0, 0]
+    y = mq.bgv_decrypt_plaintext(mp, log_n, s, c0, c1, True, threads=8)
+    assert np.array_equal(y, x)
+
+
+@pytest.mark.parametrize("plaintext_ntt", [True, False])
+def test_bgv_example_modulus_roundtrip(plaintext_ntt):
+    """examples/bgv.cpp:25-29: q = the 150-bit prime, N = 2^14, plaintext modulus p =
+    0x89d046e0ae640001; keys A = B s + p e (project keygen); decryption returns the plaintext."""
+    e = VEC["lhe"]["bgv_example"]
+    q, p, log_n = int(e["q"], 16), int(e["p"], 16), e["log_n"]
+    mq, mp = oracle.Modulus(q), oracle.Modulus(p)
+    N = 1 << log_n
+    n = 2
+    key = [42]
+    p_mont = mq.from_unsigned(p)
+    A, B, s = _keygen(mq, log_n, key, p_mont)
+    x = mp.sample_uniform(key, 5, n * N)
+    c0, c1, _ = mq.bgv_encrypt_plaintext(mp, log_n, A, B, x, plaintext_ntt, [42, 0, 0, 0, 0, 0, 0, 1], [0, 0, 0],
+                                         threads=8)
+    assert np.array_equal(mq.bgv_decrypt_plaintext(mp, log_n, s, c0, c1, plaintext_ntt, threads=8), x)
 
 
 # ---------------------------------------------------------------- tensor cipher
