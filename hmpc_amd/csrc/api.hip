@@ -1715,6 +1715,296 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
     return HMPC_OK;
 }
 
+// ------------------------------------------------------------------ general plaintext path
+namespace
+{
+    // plain 2^k mod q (k doublings of 1)
+    void pow2_mod(u32* x, int k, u32 const* q, int L)
+    {
+        std::fill(x, x + L, 0u);
+        x[0] = 1u;
+        for (int i = 0; i < k; ++i)
+            double_mod(x, q, L);
+    }
+
+    // plain 2^-k mod q for odd q (k halvings of 1: odd x -> (x + q) / 2)
+    void half_pow2_mod(u32* x, int k, u32 const* q, int L)
+    {
+        u32 t[max_limbs + 1] = {1u};
+        for (int i = 0; i < k; ++i)
+        {
+            if (t[0] & 1u)
+            {
+                u32 c = 0;
+                for (int l = 0; l < L; ++l)
+                    t[l] = addc(t[l], q[l], c, c);
+                t[L] = c;
+            }
+            for (int l = 0; l < L; ++l)
+                t[l] = (t[l] >> 1) | (t[l + 1] << 31);
+            t[L] = 0;
+        }
+        std::copy(t, t + L, x);
+    }
+
+    // constants of the general path's kernels for one modulus
+    struct lhe_consts
+    {
+        u32 c32[max_limbs] = {};   // 2^32 R mod q
+        u32 kuni[max_limbs] = {};  // 2^(-32 W) mod q for the uniform sampler's W words
+        int uniform_words = 0;
+        int uniform_bits = 0;
+    };
+
+    lhe_consts make_lhe_consts(hmpc_modulus_s const& m)
+    {
+        lhe_consts K;
+        int L = m.limbs;
+        host_limbs(L, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            u32 x[LL], r2v[LL], c[LL];
+            pow2_mod(x, 32, m.P.q, LL);
+            std::copy(m.r2, m.r2 + LL, r2v);
+            mod_mul<LL>(c, x, r2v, m.P);
+            std::copy(c, c + LL, K.c32);
+            return 0;
+        });
+        K.uniform_bits = m.bit_size + 80;
+        K.uniform_words = (K.uniform_bits + 31) / 32;
+        half_pow2_mod(K.kuni, 32 * K.uniform_words, m.P.q, L);
+        return K;
+    }
+
+    src_modulus make_src(hmpc_modulus_s const& m)
+    {
+        src_modulus S{};
+        S.limbs = m.limbs;
+        S.nprime = m.P.nprime;
+        for (int l = 0; l < m.limbs; ++l)
+            S.q[l] = m.P.q[l];
+        for (int l = 0; l < max_limbs; ++l)
+            S.half[l] = (S.q[l] >> 1) | (l + 1 < max_limbs ? S.q[l + 1] << 31 : 0u);
+        return S;
+    }
+
+    // (integer of `words` words mod q) R mod q on the host (the plaintext modulus as a mod q constant)
+    void host_mont_from_words(u32* out, u32 const* w, int words, hmpc_modulus_s const& m, lhe_consts const& K)
+    {
+        host_limbs(m.limbs, [&](auto LC) {
+            constexpr int LL = decltype(LC)::value;
+            u32 r[LL] = {}, x[LL], r2v[LL], c32[LL];
+            std::copy(m.r2, m.r2 + LL, r2v);
+            std::copy(K.c32, K.c32 + LL, c32);
+            for (int i = words - 1; i >= 0; --i)
+            {
+                u32 t[LL], d[LL];
+                mod_mul<LL>(t, r, c32, m.P);
+                for (int l = 0; l < LL; ++l)
+                    x[l] = l == 0 ? w[i] : 0u;
+                mod_mul<LL>(d, x, r2v, m.P);
+                mod_add<LL>(r, t, d, m.P);
+            }
+            std::copy(r, r + LL, out);
+            return 0;
+        });
+    }
+
+    int check_cap(int dist, int bits, int layout, int log_n, size_t n)
+    {
+        if (dist < cap_uniform || dist > cap_drown_unsigned)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "unknown distribution");
+        if (dist == cap_centered_binomial && !valid_bits(bits))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "centred binomial: bits = 4 * variance must be even and in [2, 64]");
+        if (dist == cap_binomial && (bits < 1 || bits > 256))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "binomial: bits must be in [1, 256]");
+        if ((dist == cap_drown_signed && (bits < 2 || bits > 32 * cap_max_words)) ||
+            (dist == cap_drown_unsigned && (bits < 1 || bits > 32 * cap_max_words)))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "drowning: bits must be in [1 (2 signed), 384]");
+        if (layout < cap_layout_element || layout > cap_layout_intt)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "unknown layout");
+        if (log_n < (layout == cap_layout_element ? 0 : 1) || log_n > 30)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n out of range");
+        if (n > (std::size_t{1} << 40) >> log_n)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "tensor too large");
+        return HMPC_OK;
+    }
+
+    // samples of the context's capability (key, current nonce) into out; advances the nonce by n N
+    int sample_capability(hmpc_context ctx, hmpc_modulus_s& m, int dist, int bits, int layout, int log_n, size_t n,
+                          u32* out, hipStream_t s)
+    {
+        lhe_consts K = make_lhe_consts(m);
+        if (dist == cap_uniform && K.uniform_words > cap_max_words)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "uniform: modulus too wide for the sampler");
+        cap_args C{};
+        C.key = make_key(ctx->key);
+        std::copy(ctx->nonce, ctx->nonce + 3, C.nonce);
+        C.dist = dist;
+        C.bits = bits;
+        C.layout = layout;
+        C.log_n = log_n;
+        C.total = static_cast<u64>(n) << log_n;
+        C.uniform_words = K.uniform_words;
+        C.uniform_bits = K.uniform_bits;
+        HMPC_HIP(launch_sample_capability(m.limbs, m.P, m.r2, K.c32, K.kuni, C, out, s));
+        nonce_add(ctx->nonce, C.total);
+        return HMPC_OK;
+    }
+}
+
+int hmpc_cast(hmpc_modulus from, hmpc_modulus to, size_t count, const uint32_t* in, uint32_t* out, void* stream)
+{
+    if (!from || !to || (count && (!in || !out)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (from->ctx != to->ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "cast: both moduli must belong to one context");
+    if (count == 0)
+        return HMPC_OK;
+    device_guard g(to->ctx->device);
+    lhe_consts K = make_lhe_consts(*to);
+    HMPC_HIP(launch_cast(make_src(*from), to->limbs, to->P, to->r2, K.c32, count, in, out, as_stream(stream)));
+    return HMPC_OK;
+}
+
+int hmpc_drown_bits(const uint32_t* bound, int bound_words, int is_signed, int security, int* bits)
+{
+    if (!bound || !bits || bound_words <= 0 || bound_words > 64 || security < 0 || security > 256)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bad drowning bound arguments");
+    std::vector<u32> bm1(bound, bound + bound_words);
+    if (bit_width(bm1.data(), bound_words) == 0)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "drowning bound must be positive");
+    u32 br = 1;
+    for (auto& w : bm1)
+        w = subb(w, 0u, br, br);
+    int width = bit_width(bm1.data(), bound_words);
+    int b = (width ? width + security : 0) + (is_signed ? 1 : 0);
+    if (b < 1 || b > 32 * cap_max_words)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "drowning draw wider than 384 bits");
+    *bits = b;
+    return HMPC_OK;
+}
+
+int hmpc_sample_capability(hmpc_context ctx, hmpc_modulus mod, int distribution, int bits, int layout, int log_n, size_t n,
+                           uint32_t* out, void* stream)
+{
+    if (!ctx || !mod || (n && !out))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (mod->ctx != ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "modulus of another context");
+    int rc = check_cap(distribution, bits, layout, log_n, n);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    return sample_capability(ctx, *mod, distribution, bits, layout, log_n, n, out, as_stream(stream));
+}
+
+int hmpc_bgv_encrypt_plaintext(hmpc_context ctx, hmpc_modulus mod_q, hmpc_modulus mod_p, int log_n, size_t n,
+                               const uint32_t* pk_a, const uint32_t* pk_b, const uint32_t* plaintext, int plaintext_ntt,
+                               int bits_u, int v_distribution, int bits_v, int bits_w, uint32_t* c0, uint32_t* c1,
+                               void* stream)
+{
+    if (!ctx || !mod_q || !mod_p || (n && (!pk_a || !pk_b || !plaintext || !c0 || !c1)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (mod_q->ctx != ctx || mod_p->ctx != ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "moduli of another context");
+    if (!valid_bits(bits_u) || !valid_bits(bits_w))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    if (v_distribution != cap_centered_binomial && v_distribution != cap_drown_signed)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "v: centred binomial (randomness) or signed drowning (drowning_randomness)");
+    int rc = check_cap(v_distribution, bits_v, cap_layout_ntt, log_n, n);
+    if (rc)
+        return rc;
+    transform_args Tq, Tp;
+    if ((rc = mod_q->args(log_n, Tq)) != 0)
+        return rc;
+    if (plaintext_ntt && (rc = mod_p->args(log_n, Tp)) != 0)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    device_guard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    std::size_t count = n << log_n;
+    std::size_t Lq = static_cast<std::size_t>(mod_q->limbs), Lp = static_cast<std::size_t>(mod_p->limbs);
+    void* w = nullptr;
+    if ((rc = ctx->workspace(s, (4 * Lq + Lp) * count * 4, &w)) != 0)
+        return rc;
+    u32* U = static_cast<u32*>(w);
+    u32* V = U + Lq * count;
+    u32* W = V + Lq * count;
+    u32* M = W + Lq * count;
+    u32* X = M + Lq * count;
+    // message = NTT_q(cast<mod q>(INTT_p(plaintext))) (hmpc/expr/crypto/lhe/enc.hpp:43-57)
+    u32 const* coeffs = plaintext;
+    if (plaintext_ntt)
+    {
+        HMPC_HIP(dispatch_ntt(*mod_p, Tp, n, true, plaintext, X, s));
+        coeffs = X;
+    }
+    lhe_consts Kq = make_lhe_consts(*mod_q);
+    HMPC_HIP(launch_cast(make_src(*mod_p), mod_q->limbs, mod_q->P, mod_q->r2, Kq.c32, count, coeffs, M, s));
+    HMPC_HIP(dispatch_ntt(*mod_q, Tq, n, false, M, M, s));
+    // u, v, w on the capability stream in hmpc's cache order (nonces nonce, + nN, + 2nN), each
+    // sampled in the forward transform's stage-0 layout
+    HMPC_HIP(launch_sample_cbd_ntt(mod_q->limbs, Tq, make_key(ctx->key), ctx->nonce, bits_u, n, U, s));
+    nonce_add(ctx->nonce, count);
+    if (v_distribution == cap_centered_binomial)
+    {
+        HMPC_HIP(launch_sample_cbd_ntt(mod_q->limbs, Tq, make_key(ctx->key), ctx->nonce, bits_v, n, V, s));
+        nonce_add(ctx->nonce, count);
+    }
+    else
+    {
+        if ((rc = sample_capability(ctx, *mod_q, v_distribution, bits_v, cap_layout_ntt, log_n, n, V, s)) != 0)
+            return rc;
+        HMPC_HIP(dispatch_ntt(*mod_q, Tq, n, false, V, V, s));
+    }
+    HMPC_HIP(launch_sample_cbd_ntt(mod_q->limbs, Tq, make_key(ctx->key), ctx->nonce, bits_w, n, W, s));
+    nonce_add(ctx->nonce, count);
+    // p as a mod q constant (hmpc/expr/crypto/lhe/enc.hpp:59-63)
+    u32 p_const[max_limbs] = {};
+    host_mont_from_words(p_const, mod_p->P.q, mod_p->limbs, *mod_q, Kq);
+    HMPC_HIP(launch_bgv_combine(mod_q->limbs, mod_q->P, count, std::size_t{1} << log_n, pk_a, pk_b, U, V, W, M, p_const, c0, c1,
+                                s));
+    return HMPC_OK;
+}
+
+int hmpc_bgv_decrypt_plaintext(hmpc_modulus mod_q, hmpc_modulus mod_p, int log_n, size_t n, const uint32_t* s_ntt,
+                               const uint32_t* c0, const uint32_t* c1, int plaintext_ntt, uint32_t* plaintext, void* stream)
+{
+    if (!mod_q || !mod_p || (n && (!s_ntt || !c0 || !c1 || !plaintext)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (mod_q->ctx != mod_p->ctx)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "moduli of another context");
+    transform_args Tq, Tp;
+    int rc = mod_q->args(log_n, Tq);
+    if (rc)
+        return rc;
+    if (plaintext_ntt && (rc = mod_p->args(log_n, Tp)) != 0)
+        return rc;
+    if (n == 0)
+        return HMPC_OK;
+    hmpc_context ctx = mod_q->ctx;
+    device_guard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    std::size_t count = n << log_n;
+    void* w = nullptr;
+    if ((rc = ctx->workspace(s, count * mod_q->limbs * 4, &w)) != 0)
+        return rc;
+    u32* x = static_cast<u32*>(w);
+    int L = mod_q->limbs;
+    // hmpc/expr/crypto/lhe/dec.hpp:29-44: INTT(c0 - s c1), cast to mod p, NTT_p for NTT-form plaintexts
+    HMPC_HIP(launch_pointwise(L, mod_q->P, HMPC_MUL, count, c1, s_ntt, std::size_t{1} << log_n, x, s));
+    HMPC_HIP(launch_pointwise(L, mod_q->P, HMPC_SUB, count, c0, x, count, x, s));
+    HMPC_HIP(dispatch_ntt(*mod_q, Tq, n, true, x, x, s));
+    lhe_consts Kp = make_lhe_consts(*mod_p);
+    HMPC_HIP(launch_cast(make_src(*mod_q), mod_p->limbs, mod_p->P, mod_p->r2, Kp.c32, count, x, plaintext, s));
+    if (plaintext_ntt)
+        HMPC_HIP(dispatch_ntt(*mod_p, Tp, n, false, plaintext, plaintext, s));
+    return HMPC_OK;
+}
+
 // ------------------------------------------------------------------ kernel timing
 int hmpc_context_set_timing(hmpc_context ctx, int enabled)
 {

@@ -173,4 +173,53 @@ namespace hmpc_amd
     hipError_t launch_sample_cbd_capability(int L, mod_params const& P, u32 const* r2, chacha_key const& key,
                                             u32 const* nonce, int bits, int log_n, std::size_t n, u32* out,
                                             std::size_t first_poly, hipStream_t s);
+
+    // ---------------------------------------------------------------- general plaintext path (kernels_lhe.hip)
+    // source modulus of a cross-modulus conversion (q, floor(q / 2), N' padded with zero limbs)
+    struct src_modulus
+    {
+        u32 q[max_limbs];
+        u32 half[max_limbs];
+        u32 nprime;
+        int limbs;
+    };
+    // out (target modulus, L limbs, constants r2 = R^2 mod q and c32 = 2^32 R mod q) = centred lift of
+    // in (source modulus S), limb-planar, `count` elements
+    hipError_t launch_cast(src_modulus const& S, int L, mod_params const& P, u32 const* r2, u32 const* c32, std::size_t count,
+                           u32 const* in, u32* out, hipStream_t s);
+
+    enum cap_distribution : int
+    {
+        cap_uniform = 0,
+        cap_centered_binomial = 1,
+        cap_binomial = 2,
+        cap_drown_signed = 3,
+        cap_drown_unsigned = 4,
+    };
+    enum cap_layout : int
+    {
+        cap_layout_element = 0,  // one stream per element (element-wise kernels)
+        cap_layout_ntt = 1,      // stream of index b N + t: coefficients t, t + N/2 (forward stage 0)
+        cap_layout_intt = 2,     // stream of index b N + 2t: coefficients 2t, 2t + 1 (inverse stage 0)
+    };
+    constexpr int cap_max_words = 12;  // widest draw: 384 bits
+    struct cap_args
+    {
+        chacha_key key;
+        u32 nonce[3];
+        int dist;
+        int bits;           // centred binomial: 4 x variance (<= 64); binomial: count (<= 256); drowning: width
+        int layout;
+        int log_n;          // pair layouts: polynomials of 2^log_n coefficients
+        u64 total;          // elements
+        int uniform_words;  // uniform<mod q>: ceil((bit_size + 80) / 32) words of bit_size + 80 bits
+        int uniform_bits;
+    };
+    // c32 = 2^32 R mod q, kuni = 2^(-32 uniform_words) mod q (plain)
+    hipError_t launch_sample_capability(int L, mod_params const& P, u32 const* r2, u32 const* c32, u32 const* kuni,
+                                        cap_args const& C, u32* out, hipStream_t s);
+    // c0 = A u + p v + M, c1 = B u + p w with A, B broadcast over the polynomials (period N)
+    hipError_t launch_bgv_combine(int L, mod_params const& P, std::size_t count, std::size_t N, u32 const* pk_a, u32 const* pk_b,
+                                  u32 const* U, u32 const* V, u32 const* W, u32 const* M, u32 const* p_const, u32* c0, u32* c1,
+                                  hipStream_t s);
 }

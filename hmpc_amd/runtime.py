@@ -75,6 +75,11 @@ SIGNATURES = {
     "hmpc_bgv_encrypt": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
     "hmpc_bgv_encrypt_plain": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
     "hmpc_bgv_decrypt": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "hmpc_cast": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
+    "hmpc_sample_capability": (_i, [_vp, _vp, _i, _i, _i, _i, _sz, _vp, _vp]),
+    "hmpc_drown_bits": (_i, [_u32p, _i, _i, _i, ctypes.POINTER(_i)]),
+    "hmpc_bgv_encrypt_plaintext": (_i, [_vp, _vp, _vp, _i, _sz, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "hmpc_bgv_decrypt_plaintext": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _i, _vp, _vp]),
     "hmpc_batch_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "hmpc_batch_destroy": (_i, [_vp]),
     "hmpc_batch_add_polymul": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
@@ -577,6 +582,49 @@ class Modulus:
         out = out or Buffer(self.ctx, 4 * (n << log_n))
         check(_lib.hmpc_bgv_decrypt(self.handle, log_n, n, s_ntt.ptr, c0.ptr, c1.ptr, out.ptr, _sp(stream)))
         return out
+
+    # -- hmpc's general plaintext path
+    DISTRIBUTIONS = {"uniform": 0, "centered_binomial": 1, "binomial": 2, "drown_signed": 3, "drown_unsigned": 4}
+    LAYOUTS = {"element": 0, "ntt": 1, "intt": 2}
+
+    def cast(self, to: "Modulus", count, x: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
+        """mod<to>(mod<self>) element-wise with the centred lift (hmpc_cast)."""
+        out = out or to.alloc(count)
+        check(_lib.hmpc_cast(self.handle, to.handle, count, x.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def sample_capability(self, dist: str, bits: int, layout: str, log_n: int, n: int, out: Buffer | None = None,
+                          stream=None) -> Buffer:
+        out = out or self.alloc(n << log_n)
+        check(_lib.hmpc_sample_capability(self.ctx.handle, self.handle, self.DISTRIBUTIONS[dist], bits, self.LAYOUTS[layout],
+                                          log_n, n, out.ptr, _sp(stream)))
+        return out
+
+    def bgv_encrypt_plaintext(self, pmod: "Modulus", log_n, n, pk_a: Buffer, pk_b: Buffer, plaintext: Buffer,
+                              plaintext_ntt: bool, bits_u=2, v_dist="centered_binomial", bits_v=40, bits_w=40,
+                              c0: Buffer | None = None, c1: Buffer | None = None, stream=None):
+        c0 = c0 or self.alloc(n << log_n)
+        c1 = c1 or self.alloc(n << log_n)
+        check(_lib.hmpc_bgv_encrypt_plaintext(self.ctx.handle, self.handle, pmod.handle, log_n, n, pk_a.ptr, pk_b.ptr,
+                                              plaintext.ptr, int(plaintext_ntt), bits_u, self.DISTRIBUTIONS[v_dist], bits_v,
+                                              bits_w, c0.ptr, c1.ptr, _sp(stream)))
+        return c0, c1
+
+    def bgv_decrypt_plaintext(self, pmod: "Modulus", log_n, n, s_ntt: Buffer, c0: Buffer, c1: Buffer, plaintext_ntt: bool,
+                              out: Buffer | None = None, stream=None) -> Buffer:
+        out = out or pmod.alloc(n << log_n)
+        check(_lib.hmpc_bgv_decrypt_plaintext(self.handle, pmod.handle, log_n, n, s_ntt.ptr, c0.ptr, c1.ptr,
+                                              int(plaintext_ntt), out.ptr, _sp(stream)))
+        return out
+
+
+def drown_bits(bound: int, signed: bool, security: int = 80) -> int:
+    """hmpc_drown_bits: width of drown_uniform's draw for a bound."""
+    words = max(1, (int(bound).bit_length() + 31) // 32)
+    b = _arr([(int(bound) >> (32 * i)) & 0xFFFFFFFF for i in range(words)], words)
+    out = ctypes.c_int(0)
+    check(load().hmpc_drown_bits(_p(b), words, int(bool(signed)), security, ctypes.byref(out)))
+    return out.value
 
 
 class Batch:

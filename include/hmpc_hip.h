@@ -248,6 +248,64 @@ int hmpc_bgv_encrypt_plain(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t
 int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_ntt, const uint32_t* c0,
                      const uint32_t* c1, uint32_t* messages, void* stream);
 
+/* ------------------------------------------------------------------ general plaintext path
+ * Cross-modulus conversion: hmpc::expr::cast<mod<Q>> of a mod<P> tensor (hmpc/expr/cast.hpp:42-45)
+ * with the centred lift of hmpc/ints/mod.hpp:259-307: the canonical integer v of each element of
+ * `from` is taken as v - P when v > floor(P / 2), then reduced into `to` (Montgomery form).
+ * Limb-planar in ([L_from][count]) and out ([L_to][count]); `in` may equal `out`. */
+int hmpc_cast(hmpc_modulus from, hmpc_modulus to, size_t count, const uint32_t* in, uint32_t* out, void* stream);
+
+/* Capability-path sampling (hmpc/expr/random/uniform.hpp, hmpc/expr/random/binomial.hpp drawn through
+ * the queue's random-number capability, hmpc/comp/queue.hpp:185-253): the context's key and current
+ * nonce, then the nonce advances by n N (N = 2^log_n). The capability of linear index i is the
+ * ChaCha20 stream (key, nonce, counter = low32(i)). Output: n polynomials of N coefficients mod q
+ * in coefficient form, in the order the consuming kernel reads them:
+ *   HMPC_LAYOUT_ELEMENT  element-wise kernels: coefficient e from stream e (hmpc/comp/queue.hpp:400-409;
+ *                        log_n may be 0 for plain tensors of n elements)
+ *   HMPC_LAYOUT_NTT      a following forward NTT: stream b N + t gives t, then t + N/2
+ *                        (hmpc/expr/number_theoretic_transform.hpp:320-329)
+ *   HMPC_LAYOUT_INTT     a following inverse NTT: stream b N + 2t gives 2t, then 2t + 1 (:480-489)
+ * Distributions (bits):
+ *   HMPC_UNIFORM            uniform<mod q> with 80-bit drowning (hmpc/random/uniform.hpp:61-66; bits unused)
+ *   HMPC_CENTERED_BINOMIAL  bits = 4 x variance, even, 2..64 (hmpc/random/binomial.hpp:88-128)
+ *   HMPC_BINOMIAL           popcount of bits = 1..256 random bits (hmpc/random/binomial.hpp:30-58)
+ *   HMPC_DROWN_SIGNED / _UNSIGNED  drown_uniform of width bits = 1..384 (hmpc/random/uniform.hpp:34-47;
+ *                        hmpc_drown_bits gives the width for a bound) */
+typedef enum hmpc_distribution {
+    HMPC_UNIFORM = 0,
+    HMPC_CENTERED_BINOMIAL = 1,
+    HMPC_BINOMIAL = 2,
+    HMPC_DROWN_SIGNED = 3,
+    HMPC_DROWN_UNSIGNED = 4
+} hmpc_distribution;
+typedef enum hmpc_layout { HMPC_LAYOUT_ELEMENT = 0, HMPC_LAYOUT_NTT = 1, HMPC_LAYOUT_INTT = 2 } hmpc_layout;
+int hmpc_sample_capability(hmpc_context ctx, hmpc_modulus mod, int distribution, int bits, int layout, int log_n, size_t n,
+                           uint32_t* out, void* stream);
+/* width of drown_uniform's draw for a bound (bound_words u32 words, little-endian):
+ * bit_width((bound - 1) << security) + is_signed (hmpc/random/uniform.hpp:39-41; hmpc's default
+ * statistical security is 80, hmpc/config.hpp:102) */
+int hmpc_drown_bits(const uint32_t* bound, int bound_words, int is_signed, int security, int* bits);
+
+/* BGV public-key encryption of a plaintext over mod p (hmpc/expr/crypto/lhe/enc.hpp:19-69):
+ *   message = NTT_q(cast<mod q>(INTT_p(x))) for an NTT-form plaintext (plaintext_ntt = 1), or
+ *             NTT_q(cast<mod q>(x)) for a coefficient-form one (centred lift, hmpc_cast)
+ *   u = NTT(CBD(bits_u)), v = NTT(v_distribution(bits_v)), w = NTT(CBD(bits_w)) on the context's
+ *   capability stream with nonces nonce, nonce + nN, nonce + 2nN (hmpc's cache order); the nonce
+ *   advances by 3nN. v_distribution = HMPC_CENTERED_BINOMIAL is lhe::randomness() (hmpc's default
+ *   bits 2, 40, 40; randomness.hpp:80-110); HMPC_DROWN_SIGNED with bits_v = hmpc_drown_bits(bound,
+ *   signed, 80) is lhe::drowning_randomness(bound) (randomness.hpp:112-143)
+ *   c0 = A u + v p + message,  c1 = B u + w p  (p = the plaintext modulus as a mod q constant)
+ * pk_a, pk_b: [L_q][N] (NTT form); plaintext: [L_p][n N] mod p; c0, c1: [L_q][n N]. Any odd moduli
+ * of 1..9 limbs with 2N | q - 1 (and 2N | p - 1 for an NTT-form plaintext). */
+int hmpc_bgv_encrypt_plaintext(hmpc_context ctx, hmpc_modulus mod_q, hmpc_modulus mod_p, int log_n, size_t n,
+                               const uint32_t* pk_a, const uint32_t* pk_b, const uint32_t* plaintext, int plaintext_ntt,
+                               int bits_u, int v_distribution, int bits_v, int bits_w, uint32_t* c0, uint32_t* c1,
+                               void* stream);
+/* dec<plaintext> (hmpc/expr/crypto/lhe/dec.hpp:13-45): x = INTT_q(c0 - s c1), cast to mod p with the
+ * centred lift, then NTT_p for an NTT-form plaintext (plaintext_ntt = 1). plaintext: [L_p][n N]. */
+int hmpc_bgv_decrypt_plaintext(hmpc_modulus mod_q, hmpc_modulus mod_p, int log_n, size_t n, const uint32_t* s_ntt,
+                               const uint32_t* c0, const uint32_t* c1, int plaintext_ntt, uint32_t* plaintext, void* stream);
+
 /* ------------------------------------------------------------------ job batches
  * Many independent jobs of uneven degree (2^12 .. 2^16 and beyond) and modulus in persistent
  * launches: one launch per (modulus, schedule) class, in which each workgroup claims jobs
