@@ -12,16 +12,24 @@
 //   hmpc::expr::tensor(t)                             hmpc_amd::expr::tensor(t)
 //   number_theoretic_transform(e), inverse_...(e)     same names (hmpc/expr/number_theoretic_transform.hpp)
 //   e1 + e2, e1 - e2, e1 * e2, e * constant           same operators (hmpc/expr/binary_expression.hpp)
-//   random::centered_binomial (capability)            expr::random::centered_binomial(m, bits, n, log_n)
+//   random::{uniform, centered_binomial, binomial,    expr::random::{uniform, centered_binomial, binomial,
+//     drown_signed/unsigned_uniform} (capability)       drown_signed/unsigned_uniform}(m, [bits,] n, log_n)
 //   random::number_generator(key, index) + uniform    expr::random::uniform(m, key, offset, n, log_n)
-//   crypto::lhe::enc(key, m, randomness()) / dec      expr::crypto::lhe::enc(...) / dec(...)
+//   cast<T>(e) across moduli                          expr::cast(e, modulus)  (centred lift)
+//   crypto::lhe::enc(key, x, randomness()) / dec<T>   expr::crypto::lhe::enc(...) / dec(...)
 //   q(exprs...) -> tensors, q.wait(), q.info()        same (hmpc/comp/queue.hpp:416-597)
 //
 // Evaluation keeps hmpc's algebraic rewrites (hmpc/expr/number_theoretic_transform.hpp:558-583):
-// NTT(INTT(e)) and INTT(NTT(e)) are e. It fuses INTT(NTT(a) o NTT(b)) into the batched product
-// kernel and NTT(capability CBD) into the fused sampler; everything else runs as one C-ABI call per
-// node. Tensors share ownership of their device buffer (copying aliases, as in hmpc). Errors throw
-// hmpc_amd::error (hmpc's FFI wrapper throws on non-ok codes, hmpc/net/ffi.hpp:14).
+// NTT(INTT(e)) and INTT(NTT(e)) are e. q(e1, e2, ...) plans its kernels as hmpc's execution cache
+// does (hmpc/expr/cache.hpp:85-253): a cacheable node (transform, reduction, capability sampler) is
+// materialised when it is complex, top-level, reached on more than one path or through a broadcast;
+// every other node is fused into the kernel of its nearest materialised ancestor; kernels run in
+// hmpc's post-order over the arguments. Capability samplers therefore draw on the same nonces and
+// in the same stream layouts as in hmpc (element-wise: one stream per element; under a forward or
+// inverse transform: the stage-0 pair layouts). Each node is evaluated once per call. INTT(NTT(a) o
+// NTT(b)) runs as the batched product kernel and NTT(capability CBD) as the fused sampler, with the
+// same results. Tensors share ownership of their device buffer (copying aliases, as in hmpc).
+// Errors throw hmpc_amd::error (hmpc's FFI wrapper throws on non-ok codes, hmpc/net/ffi.hpp:14).
 //
 // Header only, C++17; link with -lhmpc_hip.
 #pragma once
@@ -32,11 +40,15 @@
 #include <array>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <tuple>
 #include <type_traits>
+#include <unordered_map>
+#include <unordered_set>
 #include <utility>
 #include <vector>
 
@@ -168,7 +180,9 @@ namespace hmpc_amd
             matvec,         // hmpc::expr::matrix_vector_product
             uniform,        // explicit-key uniform<mod q>
             cbd,            // explicit-key centred binomial
-            cbd_capability, // queue-capability centred binomial (only under an NTT)
+            capability,     // queue-capability sampler (distribution `dist`, width `bits`)
+            cast,           // cast to the modulus `mod` (centred lift)
+            cache,          // the wrapper of a top-level expression that is not cacheable itself
         };
 
         struct node
@@ -177,10 +191,11 @@ namespace hmpc_amd
             comp::tensor value;            // leaf
             expression a, b;               // operands
             std::vector<std::uint32_t> c;  // scalar (Montgomery words)
-            comp::modulus mod;             // samplers
+            comp::modulus mod;             // samplers, cast target
             std::array<std::uint32_t, 8> key{};
             std::uint64_t offset = 0;
             int bits = 0;
+            int dist = HMPC_CENTERED_BINOMIAL;
             std::size_t n = 0;
             int log_n = 0;
         };
@@ -239,6 +254,17 @@ namespace hmpc_amd
             return make(std::move(n));
         }
 
+        // hmpc::expr::cast<mod<Q>> of a mod<P> expression (hmpc/expr/cast.hpp): the centred lift of
+        // hmpc/ints/mod.hpp:259-307
+        inline expression cast(expression x, comp::modulus to)
+        {
+            node n;
+            n.kind = op::cast;
+            n.a = std::move(x);
+            n.mod = std::move(to);
+            return make(std::move(n));
+        }
+
         // hmpc/expr/reduce.hpp:264-274: sum / product of every element, a one-element tensor
         inline expression sum(expression x)
         {
@@ -293,22 +319,55 @@ namespace hmpc_amd
                 return make(std::move(e));
             }
 
-            // queue-capability centred binomial (hmpc/expr/random/binomial.hpp); consumed by an NTT
-            inline expression centered_binomial(comp::modulus m, int bits, std::size_t n, int log_n)
+            // queue-capability samplers (hmpc/expr/random/uniform.hpp, hmpc/expr/random/binomial.hpp):
+            // n polynomials of N = 2^log_n coefficients (log_n = 0: n elements)
+            inline expression capability(comp::modulus m, int dist, int bits, std::size_t n, int log_n)
             {
                 node e;
-                e.kind = op::cbd_capability;
+                e.kind = op::capability;
                 e.mod = std::move(m);
+                e.dist = dist;
                 e.bits = bits;
                 e.n = n;
                 e.log_n = log_n;
                 return make(std::move(e));
             }
+            inline expression uniform(comp::modulus m, std::size_t n, int log_n)
+            {
+                return capability(std::move(m), HMPC_UNIFORM, 0, n, log_n);
+            }
+            // bits = 4 x variance
+            inline expression centered_binomial(comp::modulus m, int bits, std::size_t n, int log_n)
+            {
+                return capability(std::move(m), HMPC_CENTERED_BINOMIAL, bits, n, log_n);
+            }
+            // popcount of `count` random bits
+            inline expression binomial(comp::modulus m, int count, std::size_t n, int log_n)
+            {
+                return capability(std::move(m), HMPC_BINOMIAL, count, n, log_n);
+            }
+            // drown_uniform of a bound given as words: width bit_width((bound - 1) << security) + signed
+            inline int drown_bits(std::vector<std::uint32_t> const& bound, bool is_signed, int security = 80)
+            {
+                int b = 0;
+                check(hmpc_drown_bits(bound.data(), static_cast<int>(bound.size()), is_signed ? 1 : 0, security, &b));
+                return b;
+            }
+            inline expression drown_signed_uniform(comp::modulus m, std::vector<std::uint32_t> const& bound, std::size_t n,
+                                                   int log_n, int security = 80)
+            {
+                return capability(std::move(m), HMPC_DROWN_SIGNED, drown_bits(bound, true, security), n, log_n);
+            }
+            inline expression drown_unsigned_uniform(comp::modulus m, std::vector<std::uint32_t> const& bound, std::size_t n,
+                                                     int log_n, int security = 80)
+            {
+                return capability(std::move(m), HMPC_DROWN_UNSIGNED, drown_bits(bound, false, security), n, log_n);
+            }
         }
 
         namespace crypto::lhe
         {
-            // hmpc::comp::crypto::lhe::key: A = B s + t e and B, both in NTT form, shape {} (one polynomial)
+            // hmpc::comp::crypto::lhe::key: A = B s + p e and B, both in NTT form, shape {} (one polynomial)
             struct key
             {
                 comp::tensor A, B;
@@ -317,28 +376,52 @@ namespace hmpc_amd
             {
                 comp::tensor c0, c1;
             };
-            // default randomness of hmpc/expr/crypto/lhe/randomness.hpp:80 (variances 1/2, 10, 10)
+            // hmpc/expr/crypto/lhe/randomness.hpp: randomness() draws u, v, w centred binomial with
+            // variances 1/2, 10, 10 (bits 2, 40, 40, :80-110); drowning_randomness(bound) draws v as a
+            // signed drowning uniform (:112-143)
             struct randomness
             {
                 int bits_u = 2, bits_v = 40, bits_w = 40;
+                int v_distribution = HMPC_CENTERED_BINOMIAL;
             };
+            inline randomness drowning_randomness(std::vector<std::uint32_t> const& bound, int security = 80)
+            {
+                randomness r;
+                r.v_distribution = HMPC_DROWN_SIGNED;
+                r.bits_v = random::drown_bits(bound, true, security);
+                return r;
+            }
+            // 32-bit messages (representation::words) with p = t as a Montgomery constant, or a
+            // plaintext tensor over its own modulus p (coefficient or NTT representation)
             struct enc_expression
             {
                 key k;
-                comp::tensor messages;  // representation::words, n N
-                std::vector<std::uint32_t> plaintext_modulus;  // t as a Montgomery constant
+                comp::tensor messages;
+                std::vector<std::uint32_t> plaintext_modulus;  // words path: t as a Montgomery constant
                 randomness r;
             };
             struct dec_expression
             {
                 comp::tensor s;  // secret in NTT form
                 ciphertext ct;
+                comp::modulus plaintext_modulus;  // unset: 32-bit words (t = 2^32)
+                comp::representation rep = comp::representation::words;
             };
             inline enc_expression enc(key k, comp::tensor messages, std::vector<std::uint32_t> t_mont, randomness r = {})
             {
                 return {std::move(k), std::move(messages), std::move(t_mont), r};
             }
-            inline dec_expression dec(comp::tensor s, ciphertext ct) { return {std::move(s), std::move(ct)}; }
+            // hmpc/expr/crypto/lhe/enc.hpp:19-69: a plaintext tensor over mod p
+            inline enc_expression enc(key k, comp::tensor plaintext, randomness r = {})
+            {
+                return {std::move(k), std::move(plaintext), {}, r};
+            }
+            inline dec_expression dec(comp::tensor s, ciphertext ct) { return {std::move(s), std::move(ct), {}, comp::representation::words}; }
+            // dec<plaintext> (hmpc/expr/crypto/lhe/dec.hpp:13-45): the plaintext's modulus and representation
+            inline dec_expression dec(comp::tensor s, ciphertext ct, comp::modulus p, comp::representation rep)
+            {
+                return {std::move(s), std::move(ct), std::move(p), rep};
+            }
 
             // ciphertext_expression operators (hmpc/expr/crypto/lhe/ciphertext.hpp:48-81): with another
             // ciphertext (+, -), a plaintext polynomial in NTT form broadcast over the batch (*), or a
@@ -424,17 +507,38 @@ namespace hmpc_amd
                 return n;
             }
 
-            // q(expr): evaluates one expression into a new tensor
-            tensor operator()(expr::expression const& e) { return eval(e); }
+            // q(expr): one expression into a new tensor
+            tensor operator()(expr::expression const& e) { return std::move(eval({e})[0]); }
+            // q(e1, e2, ...): one execution plan over all of them (shared nodes evaluated once, kernels in
+            // hmpc's cache order), a tuple of new tensors
+            template<typename... E>
+            auto operator()(E const&... es)
+                -> std::enable_if_t<(sizeof...(E) > 1) && (std::is_convertible_v<E const&, expr::expression> && ...),
+                                    std::tuple<decltype((void)std::declval<E>(), tensor())...>>
+            {
+                std::vector<tensor> out = eval({expr::expression(es)...});
+                return unpack<sizeof...(E)>(out, std::make_index_sequence<sizeof...(E)>{});
+            }
             // q(enc(...)): one BGV encryption call; the capability nonce advances by 3 n N
             expr::crypto::lhe::ciphertext operator()(expr::crypto::lhe::enc_expression const& e)
             {
                 auto const& m = e.k.A.mod();
                 tensor c0(*this, m, e.messages.batch(), e.messages.log_n(), representation::number_theoretic_transform);
                 tensor c1(*this, m, e.messages.batch(), e.messages.log_n(), representation::number_theoretic_transform);
-                check(hmpc_bgv_encrypt(ctx_, m.handle(), e.messages.log_n(), e.messages.batch(), e.k.A.data(), e.k.B.data(),
-                                       e.messages.data(), e.plaintext_modulus.data(), e.r.bits_u, e.r.bits_v, e.r.bits_w,
-                                       c0.data(), c1.data(), nullptr));
+                if (e.messages.rep() == representation::words)
+                    check(hmpc_bgv_encrypt(ctx_, m.handle(), e.messages.log_n(), e.messages.batch(), e.k.A.data(), e.k.B.data(),
+                                           e.messages.data(), e.plaintext_modulus.data(), e.r.bits_u, e.r.bits_v, e.r.bits_w,
+                                           c0.data(), c1.data(), nullptr));
+                else
+                {
+                    if (e.r.v_distribution == HMPC_CENTERED_BINOMIAL && e.messages.mod() == m)
+                        throw error(HMPC_ERR_INVALID_ARGUMENT, "enc: the plaintext needs its own modulus p");
+                    bool ntt_form = e.messages.rep() == representation::number_theoretic_transform;
+                    check(hmpc_bgv_encrypt_plaintext(ctx_, m.handle(), e.messages.mod().handle(), e.messages.log_n(),
+                                                     e.messages.batch(), e.k.A.data(), e.k.B.data(), e.messages.data(),
+                                                     ntt_form ? 1 : 0, e.r.bits_u, e.r.v_distribution, e.r.bits_v, e.r.bits_w,
+                                                     c0.data(), c1.data(), nullptr));
+                }
                 return {c0, c1};
             }
             // q(ct op ...): a new ciphertext, both parts in one launch
@@ -473,26 +577,35 @@ namespace hmpc_amd
                                           nullptr));
                 return t;
             }
-            // q(dec(...)): plaintext words
+            // q(dec(...)): plaintext words (t = 2^32), or dec<plaintext> over the plaintext's modulus
             tensor operator()(expr::crypto::lhe::dec_expression const& e)
             {
                 auto const& m = e.ct.c0.mod();
-                tensor out(*this, m, e.ct.c0.batch(), e.ct.c0.log_n(), representation::words);
-                check(hmpc_bgv_decrypt(m.handle(), e.ct.c0.log_n(), e.ct.c0.batch(), e.s.data(), e.ct.c0.data(),
-                                       e.ct.c1.data(), out.data(), nullptr));
+                if (e.rep == representation::words)
+                {
+                    tensor out(*this, m, e.ct.c0.batch(), e.ct.c0.log_n(), representation::words);
+                    check(hmpc_bgv_decrypt(m.handle(), e.ct.c0.log_n(), e.ct.c0.batch(), e.s.data(), e.ct.c0.data(),
+                                           e.ct.c1.data(), out.data(), nullptr));
+                    return out;
+                }
+                tensor out(*this, e.plaintext_modulus, e.ct.c0.batch(), e.ct.c0.log_n(), e.rep);
+                check(hmpc_bgv_decrypt_plaintext(m.handle(), e.plaintext_modulus.handle(), e.ct.c0.log_n(), e.ct.c0.batch(),
+                                                 e.s.data(), e.ct.c0.data(), e.ct.c1.data(),
+                                                 e.rep == representation::number_theoretic_transform ? 1 : 0, out.data(),
+                                                 nullptr));
                 return out;
-            }
-            // several expressions, evaluated in argument order (hmpc's cache order for independent roots)
-            template<typename... E>
-            auto operator()(E const&... es) -> std::enable_if_t<(sizeof...(E) > 1), std::tuple<decltype((*this)(es))...>>
-            {
-                return std::tuple<decltype((*this)(es))...>{(*this)(es)...};
             }
 
         private:
             hmpc_context ctx_ = nullptr;
 
-            tensor eval(expr::expression const& e);
+            std::vector<tensor> eval(std::vector<expr::expression> const& roots);
+
+            template<std::size_t K, std::size_t... I>
+            static auto unpack(std::vector<tensor>& v, std::index_sequence<I...>)
+            {
+                return std::make_tuple(std::move(v[I])...);
+            }
         };
 
         inline modulus::modulus(queue& q, std::vector<std::uint32_t> const& q_words) : state_(std::make_shared<state>())
@@ -584,119 +697,486 @@ namespace hmpc_amd
             return tensor(q, m, n, log_n, r);
         }
 
-        inline tensor queue::eval(expr::expression const& e)
+
+        // ---------------------------------------------------------------- execution plan
+        // A restatement of hmpc's execution cache (hmpc/expr/cache.hpp:85-253) over expression nodes
+        // (node identity stands for hmpc's expression type tags):
+        //   trace:  for every cacheable node, the distinct paths from its nearest cacheable ancestor
+        //           (cache.hpp:86-163); a path through a broadcast (operands of different sizes) is a
+        //           "multiple" access (binary_expression.hpp access())
+        //   cache:  post-order over the top-level expressions in argument order; a cacheable node is
+        //           an entry when it is top-level, complex, traced more than once or through a broadcast,
+        //           else it is fused into its ancestor's kernel (cache.hpp:187-231)
+        //   run:    entries in order, one kernel each; a capability sampler fused into a kernel draws
+        //           with that kernel's nonce (the queue nonce then advances by the kernel's size,
+        //           queue.hpp:242-253) in its layout: element-wise, forward-NTT pairs, inverse-NTT pairs
+        namespace detail
         {
+            using expr::node;
             using expr::op;
-            switch (e->kind)
+
+            struct path
             {
-            case op::leaf:
-                return e->value;
-            case op::uniform:
-            {
-                tensor t(*this, e->mod, e->n, e->log_n, representation::coefficient);
-                check(hmpc_sample_uniform(e->mod.handle(), e->key.data(), e->offset, t.size(), t.data(), nullptr));
-                return t;
-            }
-            case op::cbd:
-            {
-                tensor t(*this, e->mod, e->n, e->log_n, representation::coefficient);
-                check(hmpc_sample_cbd(e->mod.handle(), e->key.data(), e->offset, e->bits, t.size(), t.data(), nullptr));
-                return t;
-            }
-            case op::cbd_capability:
-                throw error(HMPC_ERR_INVALID_ARGUMENT, "capability samples are produced under a number_theoretic_transform");
-            case op::ntt:
-            {
-                if (e->a->kind == op::cbd_capability)
+                std::vector<std::pair<node const*, int>> steps;
+                bool multiple = false;
+                bool operator<(path const& o) const
                 {
-                    auto const& s = *e->a;
-                    tensor t(*this, s.mod, s.n, s.log_n, representation::number_theoretic_transform);
-                    check(hmpc_sample_cbd_ntt(ctx_, s.mod.handle(), s.bits, s.log_n, s.n, t.data(), nullptr));
+                    return std::tie(steps, multiple) < std::tie(o.steps, o.multiple);
+                }
+            };
+
+            struct shape_info
+            {
+                modulus mod;
+                std::size_t n = 0;
+                int log_n = 0;
+                representation rep = representation::coefficient;
+                std::size_t size() const { return n << log_n; }
+            };
+
+            enum class layout
+            {
+                element = HMPC_LAYOUT_ELEMENT,
+                ntt = HMPC_LAYOUT_NTT,
+                intt = HMPC_LAYOUT_INTT,
+            };
+
+            inline bool cacheable(node const* e)
+            {
+                switch (e->kind)
+                {
+                case op::ntt:
+                case op::intt:
+                case op::sum:
+                case op::product:
+                case op::capability:
+                case op::cache:
+                    return true;
+                default:
+                    return false;
+                }
+            }
+
+            inline bool complex(node const* e)
+            {
+                return e->kind == op::ntt || e->kind == op::intt || e->kind == op::sum || e->kind == op::product;
+            }
+
+            inline int arity(node const* e)
+            {
+                if (e->b)
+                    return 2;
+                return e->a ? 1 : 0;
+            }
+
+            inline node const* child(node const* e, int i) { return (i == 0 ? e->a : e->b).get(); }
+
+            class plan
+            {
+            public:
+                explicit plan(queue& q) : q_(q) {}
+
+                std::vector<tensor> run(std::vector<expr::expression> const& roots)
+                {
+                    std::vector<node const*> tops;
+                    for (auto const& r : roots)
+                        tops.push_back(top_level(r));
+                    for (node const* t : tops)
+                        find(t);
+                    for (node const* t : tops)
+                        cache(t, true);
+                    mark_fused_products(tops);
+                    for (node const* e : order_)
+                        if (!elided_.count(e))
+                            results_.emplace(e, kernel(e));
+                    std::vector<tensor> out;
+                    for (node const* t : tops)
+                        out.push_back(results_.at(t));
+                    return out;
+                }
+
+                std::vector<node const*> const& order() const { return order_; }
+
+            private:
+                queue& q_;
+                std::vector<std::unique_ptr<node>> wrappers_;
+                std::map<node const*, node const*> wrapper_of_;
+                std::unordered_map<node const*, std::set<path>> traces_;
+                std::unordered_set<node const*> found_;
+                std::vector<node const*> order_;
+                std::unordered_set<node const*> in_order_;
+                std::unordered_set<node const*> elided_, fused_products_;
+                std::unordered_map<node const*, tensor> results_;
+                std::unordered_map<node const*, shape_info> shapes_;
+
+                // the top-level node: cacheable expressions stand for themselves, others are wrapped
+                // (hmpc::expr::cache, cache.hpp:71-82; one wrapper per distinct expression)
+                node const* top_level(expr::expression const& r)
+                {
+                    node const* e = r.get();
+                    if (cacheable(e))
+                        return e;
+                    auto it = wrapper_of_.find(e);
+                    if (it != wrapper_of_.end())
+                        return it->second;
+                    auto w = std::make_unique<node>();
+                    w->kind = op::cache;
+                    w->a = r;
+                    node const* wp = w.get();
+                    wrappers_.push_back(std::move(w));
+                    wrapper_of_.emplace(e, wp);
+                    return wp;
+                }
+
+                shape_info const& shape(node const* e)
+                {
+                    auto it = shapes_.find(e);
+                    if (it != shapes_.end())
+                        return it->second;
+                    shape_info s;
+                    switch (e->kind)
+                    {
+                    case op::leaf:
+                        s = {e->value.mod(), e->value.batch(), e->value.log_n(), e->value.rep()};
+                        break;
+                    case op::uniform:
+                    case op::cbd:
+                    case op::capability:
+                        s = {e->mod, e->n, e->log_n, representation::coefficient};
+                        break;
+                    case op::ntt:
+                        s = shape(e->a.get());
+                        s.rep = representation::number_theoretic_transform;
+                        break;
+                    case op::intt:
+                        s = shape(e->a.get());
+                        s.rep = representation::coefficient;
+                        break;
+                    case op::cast:
+                        s = shape(e->a.get());
+                        s.mod = e->mod;
+                        break;
+                    case op::scalar_mul:
+                    case op::cache:
+                        s = shape(e->a.get());
+                        break;
+                    case op::sum:
+                    case op::product:
+                        s = shape(e->a.get());
+                        s.n = 1;
+                        s.log_n = 0;
+                        break;
+                    case op::matvec:
+                    {
+                        shape_info const& m = shape(e->a.get());
+                        s = m;
+                        int log_rows = 0;
+                        while ((std::size_t{1} << log_rows) < m.n)
+                            ++log_rows;
+                        s.n = 1;
+                        s.log_n = log_rows;
+                        break;
+                    }
+                    default:  // add, sub, mul: the larger operand's shape (broadcast)
+                    {
+                        shape_info const& a = shape(e->a.get());
+                        shape_info const& b = shape(e->b.get());
+                        s = a.size() >= b.size() ? a : b;
+                        break;
+                    }
+                    }
+                    return shapes_.emplace(e, s).first->second;
+                }
+
+                // an operand reached through a broadcast is read for several output elements
+                bool multiple_access(node const* e)
+                {
+                    if (e->kind != op::add && e->kind != op::sub && e->kind != op::mul)
+                        return false;
+                    return shape(e->a.get()).size() != shape(e->b.get()).size();
+                }
+
+                void trace(node const* x, path p)
+                {
+                    if (cacheable(x))
+                    {
+                        traces_[x].insert(std::move(p));
+                        return;
+                    }
+                    for (int i = 0; i < arity(x); ++i)
+                    {
+                        path q = p;
+                        q.steps.emplace_back(x, i);
+                        q.multiple = q.multiple || multiple_access(x);
+                        trace(child(x, i), std::move(q));
+                    }
+                }
+
+                // cache.hpp:113-147 (find_next_expression_to_trace)
+                void find(node const* e)
+                {
+                    if (!found_.insert(e).second)
+                        return;
+                    if (cacheable(e))
+                        for (int i = 0; i < arity(e); ++i)
+                        {
+                            path p;
+                            p.steps.emplace_back(e, i);
+                            trace(child(e, i), std::move(p));
+                        }
+                    for (int i = 0; i < arity(e); ++i)
+                        find(child(e, i));
+                }
+
+                void insert(node const* e)
+                {
+                    if (in_order_.insert(e).second)
+                        order_.push_back(e);
+                }
+
+                // cache.hpp:187-231 (cache_expression)
+                void cache(node const* e, bool top)
+                {
+                    auto traverse = [&] {
+                        for (int i = 0; i < arity(e); ++i)
+                            cache(child(e, i), false);
+                    };
+                    if (!cacheable(e))
+                    {
+                        traverse();
+                        return;
+                    }
+                    auto const& tr = traces_[e];
+                    bool fused = !top && !complex(e) && tr.size() == 1 && !tr.begin()->multiple;
+                    traverse();
+                    if (!fused)
+                        insert(e);
+                }
+
+                bool entry(node const* e) const { return in_order_.count(e) != 0; }
+
+                // does the kernel subtree of x (stopping at entries) draw on the capability?
+                bool draws(node const* x) const
+                {
+                    if (entry(x))
+                        return false;
+                    if (x->kind == op::capability)
+                        return true;
+                    for (int i = 0; i < arity(x); ++i)
+                        if (draws(child(x, i)))
+                            return true;
+                    return false;
+                }
+
+                // INTT(NTT(a) o NTT(b)) whose two forward transforms are used there only, are not
+                // top-level and draw no randomness: one batched product kernel (identical results; the
+                // transforms of a and b are not materialised)
+                void mark_fused_products(std::vector<node const*> const& tops)
+                {
+                    for (node const* e : order_)
+                    {
+                        if (e->kind != op::intt || e->a->kind != op::mul)
+                            continue;
+                        node const* x = e->a->a.get();
+                        node const* y = e->a->b.get();
+                        if (x->kind != op::ntt || y->kind != op::ntt || x == y)
+                            continue;
+                        auto single = [&](node const* t) {
+                            auto const& tr = traces_[t];
+                            return tr.size() == 1 && !tr.begin()->multiple && !draws(t->a.get()) &&
+                                   std::find(tops.begin(), tops.end(), t) == tops.end();
+                        };
+                        shape_info const& sx = shape(x->a.get());
+                        shape_info const& sy = shape(y->a.get());
+                        if (single(x) && single(y) && sx.mod == sy.mod && sx.size() == sy.size())
+                        {
+                            elided_.insert(x);
+                            elided_.insert(y);
+                            fused_products_.insert(e);
+                        }
+                    }
+                }
+
+                // one capability sampler of shape s drawn in layout `lay` at the queue's nonce
+                tensor sample(node const* x, layout lay)
+                {
+                    tensor t = make(shape(x));
+                    check(hmpc_sample_capability(q_.handle(), x->mod.handle(), x->dist, x->bits, static_cast<int>(lay), x->log_n,
+                                                 x->n, t.data(), nullptr));
                     return t;
                 }
-                tensor x = eval(e->a);
-                tensor t(*this, x.mod(), x.batch(), x.log_n(), representation::number_theoretic_transform);
-                check(hmpc_ntt(x.mod().handle(), x.log_n(), x.batch(), 0, x.data(), t.data(), nullptr));
-                return t;
-            }
-            case op::intt:
-            {
-                // fused batched product: INTT(NTT(a) o NTT(b))
-                auto const& inner = *e->a;
-                if (inner.kind == op::mul && inner.a->kind == op::ntt && inner.b->kind == op::ntt)
+
+                tensor make(shape_info const& s) { return tensor(q_, s.mod, s.n, s.log_n, s.rep); }
+
+                // the value of x inside the current kernel (x fused into it), reading entries' results
+                tensor value(node const* x, layout lay, int& capability_draws)
                 {
-                    tensor a = eval(inner.a->a);
-                    tensor b = eval(inner.b->a);
-                    if (a.mod() == b.mod() && a.size() == b.size())
+                    if (entry(x))
+                        return results_.at(x);
+                    shape_info const& s = shape(x);
+                    switch (x->kind)
                     {
-                        tensor t(*this, a.mod(), a.batch(), a.log_n(), representation::coefficient);
-                        check(hmpc_polymul(a.mod().handle(), a.log_n(), a.batch(), a.data(), b.data(), t.data(), nullptr));
+                    case op::leaf:
+                        return x->value;
+                    case op::uniform:
+                    {
+                        tensor t = make(s);
+                        check(hmpc_sample_uniform(s.mod.handle(), x->key.data(), x->offset, t.size(), t.data(), nullptr));
                         return t;
                     }
-                }
-                tensor x = eval(e->a);
-                tensor t(*this, x.mod(), x.batch(), x.log_n(), representation::coefficient);
-                check(hmpc_ntt(x.mod().handle(), x.log_n(), x.batch(), 1, x.data(), t.data(), nullptr));
-                return t;
-            }
-            case op::add:
-            case op::sub:
-            case op::mul:
-            {
-                // a * b + c in one kernel, as hmpc fuses an element-wise tree into one (queue.hpp:387-412)
-                if (e->kind == op::add && e->a->kind == op::mul)
-                {
-                    tensor x = eval(e->a->a);
-                    tensor y = eval(e->a->b);
-                    tensor z = eval(e->b);
-                    if (x.mod() == y.mod() && x.mod() == z.mod() && x.size() == y.size() && x.size() == z.size())
+                    case op::cbd:
                     {
-                        tensor t(*this, x.mod(), x.batch(), x.log_n(), x.rep());
-                        check(hmpc_fma(x.mod().handle(), x.size(), x.data(), y.data(), z.data(), t.data(), nullptr));
+                        tensor t = make(s);
+                        check(hmpc_sample_cbd(s.mod.handle(), x->key.data(), x->offset, x->bits, t.size(), t.data(), nullptr));
                         return t;
                     }
+                    case op::capability:
+                    {
+                        // one capability per kernel element: a second sampler in the same kernel would
+                        // continue the same stream, which this interface does not restate
+                        if (++capability_draws > 1)
+                            throw error(HMPC_ERR_INVALID_ARGUMENT, "two capability samplers fused into one kernel");
+                        return sample(x, lay);
+                    }
+                    case op::cast:
+                    {
+                        tensor a = value(x->a.get(), lay, capability_draws);
+                        tensor t = make(s);
+                        check(hmpc_cast(a.mod().handle(), s.mod.handle(), a.size(), a.data(), t.data(), nullptr));
+                        return t;
+                    }
+                    case op::scalar_mul:
+                    {
+                        tensor a = value(x->a.get(), lay, capability_draws);
+                        tensor t = make(s);
+                        check(hmpc_pointwise_scalar(a.mod().handle(), HMPC_MUL, a.size(), a.data(), x->c.data(), t.data(), nullptr));
+                        return t;
+                    }
+                    case op::matvec:
+                    {
+                        tensor m = value(x->a.get(), lay, capability_draws);
+                        tensor v = value(x->b.get(), lay, capability_draws);
+                        std::size_t rows = m.batch(), cols = m.size() / m.batch();
+                        if (!(m.mod() == v.mod()) || v.size() != cols || (std::size_t{1} << s.log_n) != rows)
+                            throw error(HMPC_ERR_INVALID_ARGUMENT, "matrix_vector_product: one modulus, v of M's row length, 2^k rows");
+                        tensor t = make(s);
+                        check(hmpc_matvec(m.mod().handle(), 1, rows, cols, m.data(), 0, v.data(), 0, t.data(), nullptr));
+                        return t;
+                    }
+                    case op::add:
+                    case op::sub:
+                    case op::mul:
+                    {
+                        // a * b + c of one shape: one fused kernel (hmpc fuses the whole element-wise tree)
+                        if (x->kind == op::add && x->a->kind == op::mul && !entry(x->a.get()))
+                        {
+                            node const* m = x->a.get();
+                            if (shape(m->a.get()).size() == s.size() && shape(m->b.get()).size() == s.size() &&
+                                shape(x->b.get()).size() == s.size())
+                            {
+                                tensor a = value(m->a.get(), lay, capability_draws);
+                                tensor b = value(m->b.get(), lay, capability_draws);
+                                tensor c = value(x->b.get(), lay, capability_draws);
+                                if (a.mod() == b.mod() && a.mod() == c.mod())
+                                {
+                                    tensor t = make(s);
+                                    check(hmpc_fma(a.mod().handle(), a.size(), a.data(), b.data(), c.data(), t.data(), nullptr));
+                                    return t;
+                                }
+                                throw error(HMPC_ERR_INVALID_ARGUMENT, "operands need one modulus");
+                            }
+                        }
+                        tensor a = value(x->a.get(), lay, capability_draws);
+                        tensor b = value(x->b.get(), lay, capability_draws);
+                        int o = x->kind == op::add ? HMPC_ADD : (x->kind == op::sub ? HMPC_SUB : HMPC_MUL);
+                        if (a.size() < b.size() && o != HMPC_SUB)
+                            std::swap(a, b);  // commutative: broadcast the smaller operand
+                        if (!(a.mod() == b.mod()) || b.size() == 0 || a.size() % b.size() != 0)
+                            throw error(HMPC_ERR_INVALID_ARGUMENT, "operands need one modulus and a broadcastable shape");
+                        tensor t = make(s);
+                        check(hmpc_pointwise(a.mod().handle(), o, a.size(), a.data(), b.data(), b.size(), t.data(), nullptr));
+                        return t;
+                    }
+                    default:
+                        throw error(HMPC_ERR_INVALID_ARGUMENT, "a complex expression was not planned as a kernel");
+                    }
                 }
-                tensor a = eval(e->a);
-                tensor b = eval(e->b);
-                if (!(a.mod() == b.mod()) || b.size() == 0 || a.size() % b.size() != 0)
-                    throw error(HMPC_ERR_INVALID_ARGUMENT, "operands need one modulus and a broadcastable shape");
-                int o = e->kind == op::add ? HMPC_ADD : (e->kind == op::sub ? HMPC_SUB : HMPC_MUL);
-                tensor t(*this, a.mod(), a.batch(), a.log_n(), a.rep());
-                check(hmpc_pointwise(a.mod().handle(), o, a.size(), a.data(), b.data(), b.size(), t.data(), nullptr));
-                return t;
-            }
-            case op::scalar_mul:
-            {
-                tensor a = eval(e->a);
-                tensor t(*this, a.mod(), a.batch(), a.log_n(), a.rep());
-                check(hmpc_pointwise_scalar(a.mod().handle(), HMPC_MUL, a.size(), a.data(), e->c.data(), t.data(), nullptr));
-                return t;
-            }
-            case op::sum:
-            case op::product:
-            {
-                tensor a = eval(e->a);
-                tensor t(*this, a.mod(), 1, 0, a.rep());
-                check(hmpc_reduce(a.mod().handle(), e->kind == op::sum ? HMPC_ADD : HMPC_MUL, a.size(), a.data(), t.data(),
-                                  nullptr));
-                return t;
-            }
-            case op::matvec:
-            {
-                tensor m = eval(e->a);
-                tensor v = eval(e->b);
-                std::size_t rows = m.batch(), cols = m.size() / m.batch();
-                int log_rows = 0;
-                while ((std::size_t{1} << log_rows) < rows)
-                    ++log_rows;
-                if (!(m.mod() == v.mod()) || v.size() != cols || (std::size_t{1} << log_rows) != rows)
-                    throw error(HMPC_ERR_INVALID_ARGUMENT, "matrix_vector_product: one modulus, v of M's row length, 2^k rows");
-                tensor t(*this, m.mod(), 1, log_rows, m.rep());
-                check(hmpc_matvec(m.mod().handle(), 1, rows, cols, m.data(), 0, v.data(), 0, t.data(), nullptr));
-                return t;
-            }
-            }
-            throw error(HMPC_ERR_INVALID_ARGUMENT, "unknown expression");
+
+                // one entry of the plan: its own kernel over its fused subtree
+                tensor kernel(node const* e)
+                {
+                    int draws_in_kernel = 0;
+                    shape_info const& s = shape(e);
+                    switch (e->kind)
+                    {
+                    case op::cache:
+                    {
+                        // hmpc materialises a top-level expression into a new tensor
+                        tensor v = value(e->a.get(), layout::element, draws_in_kernel);
+                        if (e->a->kind == op::leaf)
+                        {
+                            tensor t = make(s);
+                            check(hmpc_memcpy_d2d(q_.handle(), t.data(), v.data(), v.bytes(), nullptr));
+                            return t;
+                        }
+                        return v;
+                    }
+                    case op::capability:
+                        // materialised (shared or reached through a broadcast): an element-wise kernel
+                        return sample(e, layout::element);
+                    case op::ntt:
+                    {
+                        node const* inner = e->a.get();
+                        if (inner->kind == op::capability && !entry(inner) && inner->dist == HMPC_CENTERED_BINOMIAL)
+                        {
+                            // sampling fused with the transform's first stages (same nonce and layout)
+                            tensor t = make(s);
+                            check(hmpc_sample_cbd_ntt(q_.handle(), inner->mod.handle(), inner->bits, inner->log_n, inner->n,
+                                                      t.data(), nullptr));
+                            return t;
+                        }
+                        tensor x = value(inner, layout::ntt, draws_in_kernel);
+                        tensor t = make(s);
+                        check(hmpc_ntt(x.mod().handle(), x.log_n(), x.batch(), 0, x.data(), t.data(), nullptr));
+                        return t;
+                    }
+                    case op::intt:
+                    {
+                        if (fused_products_.count(e))
+                        {
+                            tensor a = value(e->a->a->a.get(), layout::element, draws_in_kernel);
+                            tensor b = value(e->a->b->a.get(), layout::element, draws_in_kernel);
+                            tensor t = make(s);
+                            check(hmpc_polymul(a.mod().handle(), a.log_n(), a.batch(), a.data(), b.data(), t.data(), nullptr));
+                            return t;
+                        }
+                        tensor x = value(e->a.get(), layout::intt, draws_in_kernel);
+                        tensor t = make(s);
+                        check(hmpc_ntt(x.mod().handle(), x.log_n(), x.batch(), 1, x.data(), t.data(), nullptr));
+                        return t;
+                    }
+                    case op::sum:
+                    case op::product:
+                    {
+                        if (draws(e->a.get()))
+                            throw error(HMPC_ERR_INVALID_ARGUMENT, "capability samplers inside a reduction are not supported");
+                        tensor a = value(e->a.get(), layout::element, draws_in_kernel);
+                        tensor t = make(s);
+                        check(hmpc_reduce(a.mod().handle(), e->kind == op::sum ? HMPC_ADD : HMPC_MUL, a.size(), a.data(), t.data(),
+                                          nullptr));
+                        return t;
+                    }
+                    default:
+                        throw error(HMPC_ERR_INVALID_ARGUMENT, "unknown plan entry");
+                    }
+                }
+            };
+        }
+
+        inline std::vector<tensor> queue::eval(std::vector<expr::expression> const& roots)
+        {
+            detail::plan p(*this);
+            return p.run(roots);
         }
     }
 }

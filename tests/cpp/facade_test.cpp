@@ -1,6 +1,7 @@
 // Uses the C++ interface (include/hmpc_amd/comp.hpp) the way hmpc code uses hmpc::comp: a queue,
 // tensors, NTT expressions with hmpc's rewrites, explicit-key samplers, lhe::enc / dec, ciphertext
-// operators, the tensor cipher, sum and matrix_vector_product. Writes
+// operators, the tensor cipher, sum and matrix_vector_product, capability samplers planned as hmpc's
+// execution cache plans them (examples/bgv.cpp with a shared u), casts and BGV with an Rp plaintext. Writes
 // its results (limb-planar u32 words) to the file named by argv[1] for tests/test_gpu_facade.py,
 // which recomputes them with the oracle. Exit code 0 when its own checks pass.
 #include "../../include/hmpc_amd/comp.hpp"
@@ -33,6 +34,7 @@ int main(int argc, char** argv)
         return fail("usage: facade_test OUTPUT");
     try
     {
+        std::vector<std::vector<std::uint32_t>> put_later;
         std::array<std::uint32_t, 8> key{42, 0, 0, 0, 0, 0, 0, 0};
         comp::queue q(0, key);
         // q = 2^124 + 2^20 + 2^18 + 1
@@ -49,10 +51,13 @@ int main(int argc, char** argv)
         auto c2 = q(inverse_number_theoretic_transform(tensor(p)));
         if (c.to_host() != c2.to_host())
             return fail("fused product differs from the node-by-node product");
-        // hmpc's rewrite: NTT(INTT(x)) is x itself, no kernel
-        auto same = q(number_theoretic_transform(inverse_number_theoretic_transform(tensor(a))));
-        if (same.data() != a.data())
+        // hmpc's rewrite: NTT(INTT(x)) is x itself, no kernel (the top-level tensor is copied out)
+        auto rewritten = number_theoretic_transform(inverse_number_theoretic_transform(tensor(a)));
+        if (rewritten->kind != op::leaf)
             return fail("NTT(INTT(x)) was not rewritten to x");
+        auto same = q(rewritten);
+        if (same.data() == a.data() || same.to_host() != a.to_host())
+            return fail("a top-level tensor is not materialised into a new tensor");
         // BGV with a hand-built key, as in tests/expr/crypto/lhe/enc.cpp:35-47
         auto B = q(random::uniform(m, key, 10, 1, log_n));
         auto s = q(number_theoretic_transform(random::centered_binomial(m, key, 11, 2, 1, log_n)));
@@ -97,6 +102,50 @@ int main(int argc, char** argv)
         auto M = q(random::uniform(m, key, 20, 4, log_n));
         auto v = q(random::uniform(m, key, 21, 1, log_n));
         auto Mv = q(matrix_vector_product(tensor(M), tensor(v)));
+        // ---- execution plan: examples/bgv.cpp:80-90 with the shared capability u. The cache order is
+        // u, v, (c0), w, (c1): u, v, w draw on nonces n0, n0 + kN, n0 + 2kN
+        auto nonce_bgv = q.nonce();
+        auto mq = comp::make_tensor(q, m, k, log_n);
+        check(hmpc_from_u32(m.handle(), mq.size(), msgs.data(), mq.data(), nullptr));
+        auto mh = q(number_theoretic_transform(tensor(mq)));
+        {
+            auto u = number_theoretic_transform(random::centered_binomial(m, 2, k, log_n));
+            auto v = number_theoretic_transform(random::centered_binomial(m, 40, k, log_n));
+            auto w = number_theoretic_transform(random::centered_binomial(m, 40, k, log_n));
+            auto [e0, e1] = q(tensor(A) * u + v * t + tensor(mh), tensor(B) * u + w * t);
+            auto after = q.nonce();
+            if (after[0] != nonce_bgv[0] + 3u * (k << log_n))
+                return fail("shared-u BGV expression: nonce did not advance by 3 n N");
+            auto dec_e = q(crypto::lhe::dec(s, {e0, e1})).to_host();
+            if (dec_e != words)
+                return fail("shared-u BGV expression does not decrypt");
+            put_later.push_back(e0.to_host());
+            put_later.push_back(e1.to_host());
+        }
+        // element-wise capability samplers: top-level (its own kernel) and fused into a sum
+        auto nonce_elem = q.nonce();
+        auto ru = q(random::uniform(m, n, log_n));
+        auto rs = q(random::centered_binomial(m, 40, n, log_n) + tensor(a));
+        // under an inverse transform (pairs 2t, 2t + 1) and a drowning sampler under a forward one
+        auto ri = q(inverse_number_theoretic_transform(random::centered_binomial(m, 40, n, log_n)));
+        auto rd = q(number_theoretic_transform(random::drown_signed_uniform(m, {2u}, n, log_n)));
+        // a sampler shared by two results: materialised once (element layout), one nonce
+        auto shared = random::centered_binomial(m, 2, n, log_n);
+        auto [sh0, sh1] = q(shared + tensor(a), shared * tensor(b));
+        auto nonce_end = q.nonce();
+        if (nonce_end[0] != nonce_elem[0] + 5u * (n << log_n))
+            return fail("capability samplers: five kernels should advance the nonce by 5 n N");
+        // cast and BGV with an Rp plaintext (NTT form) over the Goldilocks prime
+        comp::modulus mp(q, {0x00000001u, 0xffffffffu});
+        auto xp = q(random::uniform(mp, key, 30, k, log_n));
+        auto xq = q(cast(tensor(xp), m));
+        auto Ap = q(tensor(B) * tensor(s) + tensor(e) * m.from_unsigned(0xffffffff00000001ull));
+        auto xp_ntt = q(number_theoretic_transform(tensor(xp)));
+        auto nonce_plain = q.nonce();
+        auto ctp = q(crypto::lhe::enc({Ap, B}, xp_ntt));
+        auto back = q(crypto::lhe::dec(s, ctp, mp, comp::representation::number_theoretic_transform));
+        if (back.to_host() != xp_ntt.to_host())
+            return fail("dec<plaintext>(enc(x)) differs from x");
         q.wait();
         std::ofstream f(argv[1], std::ios::binary);
         put(f, a.to_host());
@@ -116,6 +165,23 @@ int main(int argc, char** argv)
         put(f, y.to_host());
         put(f, total.to_host());
         put(f, Mv.to_host());
+        put(f, std::vector<std::uint32_t>(nonce_bgv.begin(), nonce_bgv.end()));
+        put(f, mh.to_host());
+        for (auto const& v : put_later)
+            put(f, v);
+        put(f, std::vector<std::uint32_t>(nonce_elem.begin(), nonce_elem.end()));
+        put(f, ru.to_host());
+        put(f, rs.to_host());
+        put(f, ri.to_host());
+        put(f, rd.to_host());
+        put(f, sh0.to_host());
+        put(f, sh1.to_host());
+        put(f, xp.to_host());
+        put(f, xq.to_host());
+        put(f, Ap.to_host());
+        put(f, std::vector<std::uint32_t>(nonce_plain.begin(), nonce_plain.end()));
+        put(f, ctp.c0.to_host());
+        put(f, ctp.c1.to_host());
         std::printf("facade ok\n");
         return 0;
     }

@@ -38,7 +38,8 @@ def test_cpp_facade_matches_oracle(tmp_path):
     sys.stdout.flush()
     r = subprocess.run([BIN, str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    a, b, c, t, B, s, e, A, words, c0, c1, words2, s0, s1, y, total, Mv = _read(out)
+    (a, b, c, t, B, s, e, A, words, c0, c1, words2, s0, s1, y, total, Mv, nonce_bgv, mh, e0, e1, nonce_elem, ru, rs, ri,
+     rd, sh0, sh1, xp, xq, Ap, nonce_plain, p0, p1) = _read(out)
     om = oracle.Modulus(Q128A)
     L, log_n, n, k = 4, 12, 3, 2
     N = 1 << log_n
@@ -74,3 +75,32 @@ def test_cpp_facade_matches_oracle(tmp_path):
     got = oracle.planar_to_ints(Mv.reshape(L, 4))
     for r in range(4):
         assert got[r] == sum(M[r * N + j] * v[j] for j in range(N)) * Rinv % q
+    # examples/bgv.cpp:80-90 as expressions with a shared capability u: the same ciphertexts as one
+    # enc call on the recorded nonce (u, v, w on nonces n0, n0 + kN, n0 + 2kN)
+    want_mh = om.ntt(oracle.ints_to_planar([om.from_unsigned(int(w_)) for w_ in words], L), log_n)
+    assert np.array_equal(mh, want_mh.reshape(-1))
+    x0, x1, _ = om.bgv_encrypt(log_n, oA, oB, words, t_mont, KEY, nonce_bgv.tolist(), threads=8)
+    assert np.array_equal(e0, x0.reshape(-1)) and np.array_equal(e1, x1.reshape(-1))
+    # capability samplers in the planned kernels: one nonce per kernel, in order
+    nz = int(nonce_elem[0])
+    cnt = n * N
+    def cap(dist, bits, layout, k):
+        return om.sample_capability(KEY, [nz + k * cnt, 0, 0], dist, bits, layout, log_n, n)
+    assert np.array_equal(ru, cap("uniform", 0, "element", 0).reshape(-1))
+    assert np.array_equal(rs, om.pointwise("add", cap("centered_binomial", 40, "element", 1), pa).reshape(-1))
+    assert np.array_equal(ri, om.ntt(cap("centered_binomial", 40, "intt", 2), log_n, inverse=True).reshape(-1))
+    assert np.array_equal(rd, om.ntt(cap("drown_signed", oracle.drown_bits(2, True), "ntt", 3), log_n).reshape(-1))
+    shared = cap("centered_binomial", 2, "element", 4)
+    assert np.array_equal(sh0, om.pointwise("add", shared, pa).reshape(-1))
+    assert np.array_equal(sh1, om.pointwise("mul", shared, pb).reshape(-1))
+    # cast to q and BGV with an NTT-form plaintext over the Goldilocks prime
+    gold = oracle.Modulus((1 << 64) - (1 << 32) + 1)
+    oxp = gold.sample_uniform(KEY, 30, k * N)
+    assert np.array_equal(xp, oxp.reshape(-1))
+    assert np.array_equal(xq, gold.cast_to(om, oxp).reshape(-1))
+    pcol = oracle.ints_to_planar([om.from_unsigned((1 << 64) - (1 << 32) + 1)] * N, L)
+    oAp = om.pointwise("add", om.pointwise("mul", oB, os_), om.pointwise("mul", oe, pcol))
+    assert np.array_equal(Ap, oAp.reshape(-1))
+    w0_, w1_, _ = om.bgv_encrypt_plaintext(gold, log_n, oAp, oB, gold.ntt(oxp, log_n), True, KEY, nonce_plain.tolist(),
+                                           threads=8)
+    assert np.array_equal(p0, w0_.reshape(-1)) and np.array_equal(p1, w1_.reshape(-1))
