@@ -5,6 +5,7 @@
 // same limb templates the kernels use.
 #include "kernels.hpp"
 #include "fastmath.hpp"
+#include "server_pool.hpp"
 
 #include "../../include/hmpc_hip.h"
 
@@ -285,7 +286,12 @@ struct hmpc_modulus_s
     std::map<int, int> bgv_passes; // fast BGV, N > 4096: 1 = two launches over tiles, 0 = persistent jobs (tuner)
     int occupancy[17] = {};   // persistent job kernel: workgroups per CU by elements per thread
 
-    int default_elems(int) const { return limbs == 4 ? 4 : 16; }
+    // per-pass kernels (transforms, products): 8 elements per thread at 4 limbs, 16 at 2; the
+    // persistent job kernel: 4 at 4 limbs (it holds one workgroup per CU). Measured on MI355X over
+    // N = 2^12..2^17 with 2^25 coefficients per call: products at 4 limbs take 2.42-2.95 ms at 8
+    // against 2.46-3.34 ms at 4 per pass, and 2.65-3.71 ms on the persistent kernel.
+    int default_elems(int) const { return limbs == 4 ? 8 : 16; }
+    int jobs_default_elems() const { return limbs == 4 ? 4 : 16; }
     // elements per thread for one call of n transforms: the tuned schedule, else the default,
     // except that a Goldilocks batch too small to give every CU a tile runs more threads per tile
     // (whole tiles: 4 elements, 1024 threads; two passes: 8). One N = 4096 NTT -> INTT took 41.1 us
@@ -342,7 +348,9 @@ struct hmpc_modulus_s
     int polymul_elems_for(int log_n) const
     {
         auto e = elems.find(log_n);
-        return e != elems.end() ? e->second : default_elems(log_n);
+        if (e != elems.end())
+            return e->second;
+        return table_polymul(log_n) ? default_elems(log_n) : jobs_default_elems();
     }
     // tile launches over a job table for this job, and with which elements per thread
     bool table_job(job_desc const& J, int* e) const
@@ -352,11 +360,11 @@ struct hmpc_modulus_s
         *e = bgv ? table_elems_for(log_n) : polymul_elems_for(log_n);
         return bgv ? table_tiles(log_n) : table_polymul(log_n);
     }
-    // uniform products: the persistent job kernel unless the passes are chosen
+    // uniform products: one launch per pass unless the persistent job kernel is chosen
     bool persistent(int log_n) const
     {
         auto p = passes.find(log_n);
-        return p == passes.end() || p->second == 0;
+        return p != passes.end() && p->second == 0;
     }
 
     ~hmpc_modulus_s()
@@ -1394,9 +1402,20 @@ namespace
     // of one stream run in order and share one workspace.
     constexpr u64 tile_ws_cap_words = u64{1} << 27;  // 512 MiB
 
+    // a job of a tile launch with the arithmetic parameters of its modulus (the job itself carries its
+    // degree's twiddle tables), captured by the caller
+    struct tile_input
+    {
+        hmpc_modulus mod;
+        int elems;
+        job_desc job;
+        transform_args T;
+    };
+
     struct tile_chunk
     {
         hmpc_modulus mod = nullptr;
+        transform_args T{};
         int elems = 0;
         u32 type = 0;
         int log_n = 0;  // a degree of the chunk (the arithmetic parameters do not depend on it)
@@ -1422,15 +1441,15 @@ namespace
         // other's tails (the server's slots: peak 7,300 -> 8,200 requests/s); a batch on one stream
         // keeps degrees mixed, so that each pass is one launch (its uneven batch: 3.6 against
         // 3.8 ms split by degree).
-        void build(std::vector<std::tuple<hmpc_modulus, int, job_desc>> const& in, bool by_degree)
+        void build(std::vector<tile_input> const& in, bool by_degree)
         {
             jobs.clear();
             map.clear();
             chunks.clear();
             ws_words = 0;
             std::vector<std::tuple<hmpc_modulus, int, u32, u32>> groups;
-            auto key = [by_degree](std::tuple<hmpc_modulus, int, job_desc> const& t) {
-                return std::make_tuple(std::get<0>(t), std::get<1>(t), std::get<2>(t).type, by_degree ? std::get<2>(t).log_n : 0u);
+            auto key = [by_degree](tile_input const& t) {
+                return std::make_tuple(t.mod, t.elems, t.job.type, by_degree ? t.job.log_n : 0u);
             };
             for (auto const& t : in)
             {
@@ -1456,13 +1475,14 @@ namespace
                 {
                     if (key(t) != g)
                         continue;
-                    job_desc J = std::get<2>(t);
+                    job_desc J = t.job;
                     u64 words = streams * static_cast<u64>(ws_limbs(mod->limbs)) << J.log_n;
                     if (c.njobs && (c.ws_words + words > tile_ws_cap_words || c.njobs == (1u << 22)))
                         close();
                     if (!c.njobs)
                     {
                         c.mod = mod;
+                        c.T = t.T;
                         c.elems = std::get<1>(g);
                         c.type = type;
                         c.log_n = static_cast<int>(J.log_n);
@@ -1498,10 +1518,7 @@ namespace
         {
             for (auto const& c : chunks)
             {
-                transform_args T;
-                int rc = c.mod->args(c.log_n, T);
-                if (rc)
-                    return rc;
+                transform_args T = c.T;
                 T.elems = c.elems;
                 auto n = [](std::size_t v) { return static_cast<u32>(v); };
                 if (c.type == job_bgv)
@@ -1536,7 +1553,11 @@ int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const
     void* w = nullptr;
     bool fast = mod->fast(log_n);
     if (fast && mod->persistent(log_n))
+    {
+        auto e = mod->elems.find(log_n);
+        T.elems = e != mod->elems.end() ? e->second : mod->jobs_default_elems();
         return run_uniform_jobs(*mod, T, polymul_job(T, log_n, a, b, out, static_cast<u64>(n) << log_n, 0), n, s);
+    }
     std::size_t words = fast ? fast_polymul_workspace_words(mod->limbs, log_n, n) : polymul_workspace_words(mod->limbs, log_n, n);
     if (words)
     {
@@ -2110,19 +2131,21 @@ struct hmpc_batch_s
         device_guard g(ctx->device);
         std::map<std::pair<hmpc_modulus, int>, std::size_t> index;
         std::vector<std::vector<entry const*>> members;
-        std::vector<std::tuple<hmpc_modulus, int, job_desc>> tiled;
+        std::vector<tile_input> tiled;
         for (auto const& e : entries)
         {
             int log_n = static_cast<int>(e.d.log_n);
             int te = 0;
             if (e.mod->table_job(e.d, &te))
             {
-                tiled.emplace_back(e.mod, te, e.d);
+                transform_args T;
+                int rc = e.mod->args(log_n, T);
+                if (rc)
+                    return rc;
+                tiled.push_back({e.mod, te, e.d, T});
                 continue;
             }
-            auto it = e.mod->elems.find(log_n);
-            int elems = e.d.type == job_bgv ? e.mod->bgv_elems_for(log_n)
-                                            : (it != e.mod->elems.end() ? it->second : e.mod->default_elems(log_n));
+            int elems = e.d.type == job_bgv ? e.mod->bgv_elems_for(log_n) : e.mod->polymul_elems_for(log_n);
             auto key = std::make_pair(e.mod, elems);
             auto f = index.find(key);
             if (f == index.end())
@@ -2585,60 +2608,8 @@ namespace
 {
     using server_clock = std::chrono::steady_clock;
 
-    // request outputs: power-of-two blocks (>= 1 MiB) carved from 1 GiB chunks and recycled
-    struct block_pool
-    {
-        std::vector<void*> chunks;
-        char* cur = nullptr;
-        std::size_t left = 0;
-        std::map<std::size_t, std::vector<void*>> free_lists;
-
-        static std::size_t class_of(std::size_t bytes)
-        {
-            std::size_t c = std::size_t{1} << 20;
-            while (c < bytes)
-                c <<= 1;
-            return c;
-        }
-
-        hipError_t take(std::size_t cls, void** out)
-        {
-            auto& fl = free_lists[cls];
-            if (!fl.empty())
-            {
-                *out = fl.back();
-                fl.pop_back();
-                return hipSuccess;
-            }
-            if (left < cls)
-            {
-                std::size_t bytes = std::max(std::size_t{1} << 30, cls);
-                void* p = nullptr;
-                hipError_t e = hipMalloc(&p, bytes);
-                if (e != hipSuccess)
-                    return e;
-                chunks.push_back(p);
-                cur = static_cast<char*>(p);
-                left = bytes;
-            }
-            *out = cur;
-            cur += cls;
-            left -= cls;
-            return hipSuccess;
-        }
-
-        void give(std::size_t cls, void* p) { free_lists[cls].push_back(p); }
-
-        void release_all()
-        {
-            for (void* c : chunks)
-                (void)hipFree(c);
-            chunks.clear();
-            free_lists.clear();
-            cur = nullptr;
-            left = 0;
-        }
-    };
+    int pool_alloc(void*, std::size_t bytes, void** out) { return hipMalloc(out, bytes) == hipSuccess ? 0 : 1; }
+    void pool_free(void*, void* p) { (void)hipFree(p); }
 
     struct server_key
     {
@@ -2658,10 +2629,19 @@ namespace
         req_failed = 3,
     };
 
+    // A request carries everything the worker needs from its modulus, resolved on the client's thread
+    // at submission (arithmetic parameters and tables, launch structure, elements per thread,
+    // occupancy): the worker never reads or writes a modulus's state, which the client may change
+    // meanwhile (new degrees, schedules).
     struct server_request
     {
         int kind = 0;  // 0 encrypt, 1 product
         hmpc_modulus mod = nullptr;
+        transform_args T{};
+        bool tile = false;   // tile launches over a job table (two-pass degrees)
+        int elems = 0;       // elements per thread of its launch
+        int occupancy = 0;   // persistent kernel: workgroups per CU at `elems`
+        std::string err;     // why it failed
         int log_n = 0;
         std::size_t n = 0;  // ciphertexts or pairs
         int key = -1;
@@ -2707,6 +2687,22 @@ namespace
     }
 
     constexpr int max_classes_per_launch = 16;
+
+    // workgroups per CU of the persistent kernel at `elems` for this modulus (computed once)
+    int jobs_occupancy(hmpc_modulus_s& m, int elems, int* occ_out)
+    {
+        int& occ = m.occupancy[elems];
+        if (occ == 0)
+        {
+            device_guard g(m.ctx->device);
+            hipError_t e = fast_jobs_occupancy(m.limbs, elems, &occ);
+            if (e != hipSuccess)
+                return hip_fail(e, "fast_jobs_occupancy");
+            occ = std::max(occ, 1);
+        }
+        *occ_out = occ;
+        return HMPC_OK;
+    }
 }
 
 struct hmpc_server_s
@@ -2727,7 +2723,6 @@ struct hmpc_server_s
     bool stop = false;
     u64 next_id = 1;
     std::vector<std::pair<u64, double>> completed;  // finished, not yet reported: (request, latency us)
-    std::string error;
     u64 launches = 0, jobs_run = 0;
 
     bool idle() const
@@ -2771,7 +2766,10 @@ struct hmpc_server_s
             {
                 // shutting down: queued requests are dropped, running ones drained
                 for (u64 id : queue)
+                {
                     reqs[id].state = req_failed;
+                    reqs[id].err = "server stopped";
+                }
                 queue.clear();
                 if (idle())
                     break;
@@ -2787,43 +2785,65 @@ struct hmpc_server_s
         }
     }
 
-    // moves queued requests that fit into slot s and launches them; false if nothing fit
+    // a request that cannot run: its outputs go back to the pool, the client sees it failed with `why`
+    void fail_request(u64 id, std::string const& why)
+    {
+        auto& r = reqs[id];
+        if (r.out)
+        {
+            pool.give(r.cls, r.out);
+            in_use -= r.bytes;
+            r.out = nullptr;
+        }
+        r.state = req_failed;
+        r.err = why;
+        completed.push_back({id, -1.0});
+    }
+
+    // moves queued requests that fit into slot s (arrival order: the head waits for memory) and
+    // launches them; false if nothing was launched
     bool admit(server_slot& s)
     {
         std::vector<u64> take;
-        std::size_t jobs = 0;
+        launch_admission adm;
+        adm.max_jobs = max_jobs;
+        adm.max_classes = max_classes_per_launch;
+        bool failed = false;
         while (!queue.empty())
         {
-            auto& r = reqs[queue.front()];
-            if (!take.empty() && jobs + r.n > max_jobs)
+            u64 id = queue.front();
+            auto& r = reqs[id];
+            if (!adm.fits(r.n, r.mod, r.elems, !r.tile))
                 break;
             if (in_use + r.bytes > budget)
-                break;  // arrival order: the head waits for memory
-            void* out = nullptr;
-            if (pool.take(r.cls, &out) != hipSuccess)
-            {
-                error = "out of device memory for request outputs";
                 break;
+            void* out = nullptr;
+            int took = pool.take(r.cls, &out);
+            if (took != block_pool::take_ok)
+            {
+                // this request alone fails; the server and the requests behind it go on
+                queue.pop_front();
+                fail_request(id, took == block_pool::take_out_of_memory ? "device out of memory for the request's outputs"
+                                                                        : "request outputs exceed the memory budget");
+                failed = true;
+                continue;
             }
             r.out = out;
             in_use += r.bytes;
             r.state = req_running;
-            take.push_back(queue.front());
-            jobs += r.n;
+            take.push_back(id);
+            adm.add(r.n, r.mod, r.elems, !r.tile);
             queue.pop_front();
         }
+        if (failed)
+            done_cv.notify_all();
         if (take.empty())
             return false;
-        if (launch(s, take, jobs) != HMPC_OK)
+        if (launch(s, take, adm.jobs) != HMPC_OK)
         {
-            if (error.empty())
-                error = last_error;
+            std::string why = last_error;
             for (u64 id : take)
-            {
-                auto& r = reqs[id];
-                r.state = req_failed;
-                completed.push_back({id, -1.0});
-            }
+                fail_request(id, why);
             done_cv.notify_all();
             return false;
         }
@@ -2831,48 +2851,38 @@ struct hmpc_server_s
     }
 
     // Requests at two-pass degrees run as tile launches over a table of all their ciphertexts
-    // or pairs; N <= 4096 (and degrees tuned to it) as persistent launches per class.
+    // or pairs; N <= 4096 (and degrees tuned to it) as persistent launches per class. Uses only what
+    // the requests captured at submission.
     int launch(server_slot& s, std::vector<u64> const& take, std::size_t jobs)
     {
         struct cls_info
         {
             hmpc_modulus mod;
             int elems;
-            int log_n;
+            int occupancy;
+            transform_args T;
             std::vector<std::pair<u64, job_desc>> jobs;  // (cost, job)
         };
         std::vector<cls_info> classes;
-        std::vector<std::tuple<hmpc_modulus, int, job_desc>> tiled;
+        std::vector<tile_input> tiled;
         for (u64 id : take)
         {
             auto& r = reqs[id];
-            transform_args T;
-            int rc = r.mod->args(r.log_n, T);
-            if (rc)
-            {
-                error = last_error;
-                return rc;
-            }
-            bool tile = r.kind == 0 ? r.mod->table_tiles(r.log_n) : r.mod->table_polymul(r.log_n);
-            int elems = r.kind == 0 ? r.mod->bgv_elems_for(r.log_n) : r.mod->polymul_elems_for(r.log_n);
+            transform_args const& T = r.T;
             auto it = classes.end();
-            if (!tile)
+            if (!r.tile)
             {
-                it = std::find_if(classes.begin(), classes.end(), [&](cls_info const& c) { return c.mod == r.mod && c.elems == elems; });
+                it = std::find_if(classes.begin(), classes.end(), [&](cls_info const& c) { return c.mod == r.mod && c.elems == r.elems; });
                 if (it == classes.end())
                 {
-                    if (classes.size() == max_classes_per_launch)
-                    {
-                        error = "too many kernel classes in one launch";
-                        return HMPC_ERR_INVALID_ARGUMENT;
-                    }
-                    classes.push_back({r.mod, elems, r.log_n, {}});
+                    classes.push_back({r.mod, r.elems, r.occupancy, T, {}});
                     it = classes.end() - 1;
                 }
             }
             u64 N = u64{1} << r.log_n;
             u64 count = static_cast<u64>(r.n) << r.log_n;
             auto* out = static_cast<u32*>(r.out);
+            int limbs = r.T.P.limbs;
             if (r.kind == 0)
             {
                 server_key const& k = keys[static_cast<std::size_t>(r.key)];
@@ -2886,13 +2896,13 @@ struct hmpc_server_s
                 B.bits_u = k.bits[0];
                 B.bits_v = k.bits[1];
                 B.bits_w = k.bits[2];
-                u32* c1 = out + static_cast<std::size_t>(r.mod->limbs) * count;
+                u32* c1 = out + static_cast<std::size_t>(limbs) * count;
                 for (std::size_t i = 0; i < r.n; ++i)
                 {
                     job_desc J = bgv_job(T, B, k.p_int, r.log_n, k.pk_a, k.pk_b, r.messages, out, c1, count, i * N,
                                          static_cast<u32>(i));
-                    if (tile)
-                        tiled.emplace_back(r.mod, r.mod->table_elems_for(r.log_n), J);
+                    if (r.tile)
+                        tiled.push_back({r.mod, r.elems, J, T});
                     else
                         it->jobs.push_back({4 * N * r.log_n, J});
                 }
@@ -2902,8 +2912,8 @@ struct hmpc_server_s
                 for (std::size_t i = 0; i < r.n; ++i)
                 {
                     job_desc J = polymul_job(T, r.log_n, r.a, r.b, out, count, i * N);
-                    if (tile)
-                        tiled.emplace_back(r.mod, r.mod->polymul_elems_for(r.log_n), J);
+                    if (r.tile)
+                        tiled.push_back({r.mod, r.elems, J, T});
                     else
                         it->jobs.push_back({3 * N * r.log_n, J});
                 }
@@ -2932,13 +2942,9 @@ struct hmpc_server_s
             std::stable_sort(cl.jobs.begin(), cl.jobs.end(), [](auto const& x, auto const& y) { return x.first > y.first; });
             u64 w = 0;
             for (auto const& j : cl.jobs)
-                w = std::max(w, job_scratch_words(j.second.type, static_cast<int>(j.second.log_n), cl.mod->limbs));
-            int rc = jobs_grid(*cl.mod, cl.elems, cl.jobs.size(), &grids[c]);
-            if (rc)
-            {
-                error = last_error;
-                return rc;
-            }
+                w = std::max(w, job_scratch_words(j.second.type, static_cast<int>(j.second.log_n), cl.T.P.limbs));
+            std::size_t g = static_cast<std::size_t>(cl.occupancy) * static_cast<std::size_t>(ctx->props.multiProcessorCount);
+            grids[c] = static_cast<unsigned>(std::max<std::size_t>(1, std::min(g, cl.jobs.size())));
             slot_words[c] = w;
             need = std::max<std::size_t>(need, static_cast<std::size_t>(grids[c]) * w);
         }
@@ -2979,21 +2985,15 @@ struct hmpc_server_s
             HMPC_HIP(hipMemcpyAsync(s.d_map, s.h_map, s.tiles.map.size() * 4, hipMemcpyHostToDevice, s.stream));
             int rc = s.tiles.launch(s.d_table + tile_off, s.d_map, s.scratch, s.stream);
             if (rc)
-            {
-                error = last_error;
                 return rc;
-            }
         }
         off = 0;
         for (std::size_t c = 0; c < classes.size(); ++c)
         {
             auto& cl = classes[c];
-            transform_args T;
-            int rc = cl.mod->args(cl.log_n, T);
-            if (rc)
-                return rc;
+            transform_args T = cl.T;
             T.elems = cl.elems;
-            HMPC_HIP(launch_fast_jobs(cl.mod->limbs, T, s.d_table + off, job_desc{}, static_cast<u32>(cl.jobs.size()), grids[c],
+            HMPC_HIP(launch_fast_jobs(T.P.limbs, T, s.d_table + off, job_desc{}, static_cast<u32>(cl.jobs.size()), grids[c],
                                       s.counters + 64 * c, s.scratch, slot_words[c], s.stream));
             off += cl.jobs.size();
         }
@@ -3017,6 +3017,9 @@ int hmpc_server_create(hmpc_context ctx, size_t budget_bytes, int slots, hmpc_se
     auto srv = std::make_unique<hmpc_server_s>();
     srv->ctx = ctx;
     srv->budget = budget_bytes;
+    srv->pool.alloc = pool_alloc;
+    srv->pool.release = pool_free;
+    srv->pool.cap = budget_bytes;
     std::copy(ctx->key, ctx->key + 8, srv->key.k);
     for (int i = 0; i < slots; ++i)
     {
@@ -3061,6 +3064,9 @@ int hmpc_server_destroy(hmpc_server srv)
         if (s->scratch)
             (void)hipFree(s->scratch);
     }
+    for (auto& [id, r] : srv->reqs)
+        if (r.out)
+            srv->pool.give(r.cls, r.out);
     srv->pool.release_all();
     delete srv;
     return HMPC_OK;
@@ -3089,14 +3095,35 @@ int hmpc_server_add_key(hmpc_server srv, hmpc_modulus mod, int log_n, const uint
 
 namespace
 {
+    // the request's modulus-dependent launch parameters, resolved on the client's thread
+    int resolve_request(server_request& r)
+    {
+        int rc = r.mod->args(r.log_n, r.T);
+        if (rc)
+            return rc;
+        if (r.kind == 0)
+        {
+            r.tile = r.mod->table_tiles(r.log_n);
+            r.elems = r.tile ? r.mod->table_elems_for(r.log_n) : r.mod->bgv_elems_for(r.log_n);
+        }
+        else
+        {
+            r.tile = r.mod->table_polymul(r.log_n);
+            r.elems = r.mod->polymul_elems_for(r.log_n);
+        }
+        r.T.elems = r.elems;
+        return r.tile ? HMPC_OK : jobs_occupancy(*r.mod, r.elems, &r.occupancy);
+    }
+
     int server_enqueue(hmpc_server srv, server_request r, uint64_t* request)
     {
         if (r.bytes > srv->budget)
             return fail(HMPC_ERR_BUDGET, "request outputs exceed the server's memory budget");
+        int rc = resolve_request(r);
+        if (rc)
+            return rc;
         {
             std::lock_guard<std::mutex> lk(srv->m);
-            if (!srv->error.empty())
-                return fail(HMPC_ERR_DEVICE, "server: " + srv->error);
             u64 id = srv->next_id++;
             r.t_submit = server_clock::now();
             srv->reqs.emplace(id, r);
@@ -3170,7 +3197,7 @@ int hmpc_server_poll(hmpc_server srv, uint64_t request, int* state, double* late
         *latency_us = it->second.state == req_done
                           ? std::chrono::duration<double, std::micro>(it->second.t_done - it->second.t_submit).count()
                           : 0.0;
-    return it->second.state == req_failed ? fail(HMPC_ERR_DEVICE, "server: " + srv->error) : HMPC_OK;
+    return it->second.state == req_failed ? fail(HMPC_ERR_DEVICE, "server: " + it->second.err) : HMPC_OK;
 }
 
 int hmpc_server_wait(hmpc_server srv, uint64_t request)
@@ -3186,7 +3213,12 @@ int hmpc_server_wait(hmpc_server srv, uint64_t request)
         if (it->second.state == req_done)
             return HMPC_OK;
         if (it->second.state == req_failed)
-            return fail(HMPC_ERR_DEVICE, "server: " + srv->error);
+            return fail(HMPC_ERR_DEVICE, "server: " + it->second.err);
+        // nothing runs and the head of the queue waits for memory that only a release frees: waiting
+        // would not end
+        if (it->second.state == req_queued && srv->idle() && !srv->queue.empty() &&
+            srv->in_use + srv->reqs[srv->queue.front()].bytes > srv->budget)
+            return fail(HMPC_ERR_BUDGET, "server: queued behind outputs that are not released (hmpc_server_release)");
         srv->done_cv.wait_for(lk, std::chrono::milliseconds(10));
     }
 }

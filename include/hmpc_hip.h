@@ -106,8 +106,9 @@ int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
 /* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 limbs, 8 or
  * 16 for 2 limbs; 0 restores the default) and launch structure (persistent = 1: one persistent
  * job kernel in which each workgroup runs whole products / encryptions; 0: one launch per
- * transform pass; -1: unchanged; -2: the default, which is the persistent kernel for uniform
- * products and the tile launches in job batches and serving). Chosen by hmpc_autotune; results
+ * transform pass; -1: unchanged; -2: the default, which is one launch per pass for uniform products
+ * and the tile launches in job batches and serving; the default elements per thread are 8 at 4 limbs
+ * and 16 at 2 per pass, 4 at 4 limbs on the persistent kernel). Chosen by hmpc_autotune; results
  * are identical. Without a set value, an hmpc_ntt call on 2 limbs too small to give every
  * compute unit a tile runs more threads per tile (4 elements per thread for N <= 4096, 8 above)
  * for latency; hmpc_modulus_get_schedule reports the set or default value. */
@@ -337,17 +338,25 @@ int hmpc_batch_info(hmpc_batch batch, size_t* jobs, size_t* classes, size_t* dev
 
 /* ------------------------------------------------------------------ serving
  * A request loop on a native worker thread that admits encryption and product requests as they
- * arrive (continuous batching): whenever one of `slots` streams is free, queued requests whose
- * outputs fit the HBM budget are packed, in arrival order, into one persistent-kernel launch on
- * it; launches on different slots overlap on the device. Keys, twiddles, messages and outputs
- * stay in HBM. Outputs are allocated by the server (counted against `budget_bytes`) and held
- * until hmpc_server_release. A request larger than the budget fails with HMPC_ERR_BUDGET;
- * others wait for memory. Randomness: the server's ChaCha key is the context key at creation;
- * an encryption request of k messages with nonce n gives exactly the ciphertexts of
- * hmpc_bgv_encrypt(n = k) on a context with that key and nonce (u, v, w on nonces n, n + kN,
- * n + 2kN), whatever else shares the GPU. Input buffers must stay valid until the request is
- * done. Thread safety: one client thread; the calls below lock the server. No counterpart in
- * hmpc, whose queue executes synchronously per call (hmpc/comp/queue.hpp:416-529). */
+ * arrive (continuous batching). Whenever one of `slots` streams is free, queued requests whose
+ * outputs fit the HBM budget are taken in arrival order into one set of launches on it: tile
+ * launches over a table of every ciphertext or product pair at two-pass degrees (N > 4096, split by
+ * degree, so that each runs the kernels with its pass lengths fixed at compile time), and one
+ * persistent launch per (modulus, elements per thread) for N <= 4096 and degrees tuned to the
+ * persistent kernel (at most 16 such classes per set; a request that would add a 17th waits for the
+ * next). Launches on different slots overlap on the device. Keys, twiddles, messages and outputs
+ * stay in HBM. Outputs are allocated by the server from a pool whose device memory stays within
+ * `budget_bytes` and are held until hmpc_server_release. A request larger than the budget fails with
+ * HMPC_ERR_BUDGET at submission; others wait for memory. A request whose launch or output
+ * allocation fails fails alone (hmpc_server_poll / _wait report why); later requests run.
+ * Randomness: the server's ChaCha key is the context key at creation; an encryption request of k
+ * messages with nonce n gives exactly the ciphertexts of hmpc_bgv_encrypt(n = k) on a context with
+ * that key and nonce (u, v, w on nonces n, n + kN, n + 2kN), whatever else shares the GPU. Input
+ * buffers must stay valid until the request is done, and moduli and keys until the server is
+ * destroyed. A request's launch parameters (tables, schedule) are taken from its modulus at
+ * submission, on the client's thread: the worker never reads a modulus's state, so the client may
+ * use the modulus meanwhile. Thread safety: one client thread; the calls below lock the server. No
+ * counterpart in hmpc, whose queue executes synchronously per call (hmpc/comp/queue.hpp:416-529). */
 typedef struct hmpc_server_s* hmpc_server;
 int hmpc_server_create(hmpc_context ctx, size_t budget_bytes, int slots, hmpc_server* out);
 /* drops queued requests, drains running ones, frees all server memory */
@@ -363,6 +372,8 @@ int hmpc_server_submit_polymul(hmpc_server server, hmpc_modulus mod, int log_n, 
                                const uint32_t* b, uint64_t* request);
 /* state 0 queued, 1 running, 2 done, 3 failed; latency = submission to completion (us, when done) */
 int hmpc_server_poll(hmpc_server server, uint64_t request, int* state, double* latency_us);
+/* blocks until the request is done or failed; fails with HMPC_ERR_BUDGET instead of waiting forever
+ * when nothing runs and the queue's head waits for memory held by unreleased outputs */
 int hmpc_server_wait(hmpc_server server, uint64_t request);
 /* device pointers of a finished request's outputs (c1 is NULL for products), valid until release */
 int hmpc_server_result(hmpc_server server, uint64_t request, uint32_t** c0, uint32_t** c1);

@@ -338,6 +338,17 @@ def test_autotune_bgv_keeps_results_and_rng(ctx):
     assert np.array_equal(gm.download(c0, n * N), want0) and np.array_equal(gm.download(c1, n * N), want1)
 
 
+def test_default_schedule_is_one_launch_per_pass(ctx):
+    """Untuned products run one launch per pass (the faster structure at every measured degree) at 8
+    elements per thread for 4 limbs and 16 for 2."""
+    for q, elems in ((Q128A, 8), (Q64, 16)):
+        gm, _ = mods(ctx, q)
+        say(f"default schedule q={hex(q)[:12]}")
+        gm.set_schedule(16, 0, -2)
+        sched = gm.schedule(16)
+        assert sched["fast"] and not sched["persistent"] and sched["elems_per_thread"] == elems
+
+
 def test_autotune_installs_a_correct_schedule(ctx):
     gm, om = mods(ctx, Q128A)
     log_n, n = 14, 8
