@@ -26,6 +26,7 @@
 #include "kernels.hpp"
 #include "fastmath.hpp"
 #include "r26.hpp"
+#include "tile_geometry.hpp"
 
 #include <mutex>
 #include <type_traits>
@@ -36,9 +37,6 @@ namespace hmpc_amd
 {
     namespace fast
     {
-        constexpr int tile_log = 12;
-        constexpr int tile = 1 << tile_log;
-
         // Thread index as an opaque value per use. Every per-thread LDS / twiddle address below is
         // a function of it; kept opaque, that arithmetic is recomputed where it is used (a few
         // integer ops) instead of being hoisted out of the job kernel's tile loops and held in
@@ -49,8 +47,6 @@ namespace hmpc_amd
             asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(static_cast<int>(threadIdx.x)));
             return t;
         }
-
-        constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
         // ------------------------------------------------------------------ arithmetic policies
         // r26: registers and LDS hold 5 limbs of 26 bits per element; global memory keeps hmpc's
@@ -394,82 +390,6 @@ namespace hmpc_amd
             }
         };
 
-        // ------------------------------------------------------------------ tile geometry
-        struct tile_desc
-        {
-            int s;         // log2 of the sub-transform length
-            int log_seqs;  // s + log_seqs = 12
-            bool rows;     // row layout (c << s) + j, else column layout (j << log_seqs) + c
-        };
-
-        struct unit_t
-        {
-            u32 c, F, Lo;
-        };
-
-        __device__ __forceinline__ unit_t unit_of(tile_desc const& td, int a, int w, bool inv, int u)
-        {
-            int fbits = td.s - w;
-            u32 f, c;
-            if (td.rows)
-            {
-                f = static_cast<u32>(u) & ((1u << fbits) - 1u);
-                c = static_cast<u32>(u) >> fbits;
-            }
-            else
-            {
-                c = static_cast<u32>(u) & ((1u << td.log_seqs) - 1u);
-                f = static_cast<u32>(u) >> td.log_seqs;
-            }
-            unit_t r;
-            r.c = c;
-            if (!inv)
-            {
-                int lo_bits = td.s - a - w;
-                r.F = f >> lo_bits;
-                r.Lo = f & ((1u << lo_bits) - 1u);
-            }
-            else
-            {
-                r.F = f >> a;
-                r.Lo = f & ((1u << a) - 1u);
-            }
-            return r;
-        }
-
-        __device__ __forceinline__ u32 j_of(unit_t const& un, u32 k, int a, int w, int s, bool inv)
-        {
-            return inv ? ((un.F << (a + w)) | (k << a) | un.Lo) : ((un.F << (s - a)) | (k << (s - a - w)) | un.Lo);
-        }
-
-        __device__ __forceinline__ int elem_of(tile_desc const& td, u32 c, u32 j)
-        {
-            return td.rows ? static_cast<int>((c << td.s) + j) : static_cast<int>((j << td.log_seqs) + c);
-        }
-
-        // round schedule: forward full rounds first, inverse partial round first
-        template<int WMAX>
-        __device__ __forceinline__ int round_count(int s) { return (s + WMAX - 1) / WMAX; }
-        template<int WMAX>
-        __device__ __forceinline__ int round_start(int s, int r, bool inv)
-        {
-            int part = s % WMAX;
-            if (!inv || part == 0)
-                return WMAX * r;
-            return r == 0 ? 0 : part + WMAX * (r - 1);
-        }
-        template<int WMAX>
-        __host__ __device__ __forceinline__ int round_width(int s, int r, bool inv)
-        {
-            int part = s % WMAX;
-            int n = (s + WMAX - 1) / WMAX;
-            if (part == 0)
-                return WMAX;
-            if (!inv)
-                return r == n - 1 ? part : WMAX;
-            return r == 0 ? part : WMAX;
-        }
-
         template<int E>
         struct shape
         {
@@ -477,22 +397,11 @@ namespace hmpc_amd
             static constexpr int wmax = ilog2(E);
         };
 
-        // Unit g of the thread in a round of width W: tid + (tile / E) g, or with the policy's
-        // wave_units the wave-major order in which each wave owns the 64 E / 2^W consecutive units
-        // [wave 64 E / 2^W, (wave + 1) 64 E / 2^W), lane-consecutive within each g. In a row-layout
-        // tile whose rows are at most 64 E elements long every round of a wave then touches only
-        // the wave's own rows (tile elements [wave 64 E, (wave + 1) 64 E)). Both orders are the same
-        // for full-width rounds (2^W = E, one unit per thread).
+        // unit g of this thread in a round of width W, in the policy's unit order (unit_index_at)
         template<typename A, int E, int W>
         __device__ __forceinline__ int unit_index(int g)
         {
-            int const t = tid_x();
-            if constexpr ((1 << W) == E)
-                return t;
-            else if constexpr (A::wave_units)
-                return ((((t >> 6) * (E >> W)) + g) << 6) | (t & 63);
-            else
-                return t + shape<E>::threads * g;
+            return unit_index_at<E, W, A::wave_units>(tid_x(), g);
         }
 
         // Wave-local tiles (every exchange of a wave stays within its own tile elements) replace
@@ -568,11 +477,7 @@ namespace hmpc_amd
                 u32 base = base0 + un.c * base_step;
 #pragma unroll
                 for (int m = 0; m < count; ++m)
-                {
-                    u32 idx = Inv ? base * h + ((un.F << (W - II - 1)) | static_cast<u32>(m))
-                                  : (base << i) + ((un.F << II) | static_cast<u32>(m));
-                    A::tw_get(tw_regs[g][first + m], tw, idx);
-                }
+                    A::tw_get(tw_regs[g][first + m], tw, tw_index<Inv, W, II>(base, h, i, un.F, m));
             }
         }
 
@@ -776,25 +681,6 @@ namespace hmpc_amd
             }
         }
 
-        // global (limb-planar) <-> LDS tile. Element e of the tile maps to global index
-        // base + e (rows) or base + (e >> log_seqs) * stride + (e & (C - 1)) (columns).
-        struct io_desc
-        {
-            std::size_t base;   // global element index of tile element 0
-            std::size_t count;  // plane size (elements per limb plane)
-            u32 stride;         // column layout: distance between rows (N2)
-            int log_seqs;
-            bool rows;
-            int valid;          // tile elements backed by real data (tail tiles)
-        };
-
-        __device__ __forceinline__ std::size_t gidx(io_desc const& io, int e)
-        {
-            if (io.rows)
-                return io.base + e;
-            return io.base + static_cast<std::size_t>(e >> io.log_seqs) * io.stride + (e & ((1 << io.log_seqs) - 1));
-        }
-
         // Ws: src is a private workspace (A::wsl limb planes, raw format) rather than data in the
         // hmpc layout. WL: each wave moves its own elements wave 64 E + 64 i + lane (E = tile / T),
         // the ones its rounds work on in a wave-local tile.
@@ -843,94 +729,6 @@ namespace hmpc_amd
                 else
                     A::store_global(dst, io.count, gidx(io, e), x, canonical, P);
             }
-        }
-
-        // The lazy forward transform runs up to 29 stages without reductions (r26.hpp); a
-        // forward column result of a transform longer than 18 stages is folded before it goes to
-        // the workspace. (No prime 2^124 + c with c < 2^24 has 2^19 | q - 1, so radix-2^26
-        // transforms stop at N = 2^17 and the fold is a guard only.)
-        __host__ __device__ __forceinline__ bool ws_fold_columns(int log_n) { return log_n > 18; }
-
-        struct pass_geom
-        {
-            int log_n, log_n1, log_n2;
-            int log_seqs;  // sequences per tile (columns, rows or polynomials)
-            bool single;
-            u32 blocks_per_poly;
-        };
-
-        __device__ __forceinline__ void cols_setup(pass_geom const& G, u32 bid, std::size_t count, tile_desc& td, io_desc& io)
-        {
-            u32 p = bid / G.blocks_per_poly;
-            u32 cb = bid - p * G.blocks_per_poly;
-            td = tile_desc{G.log_n1, G.log_seqs, false};
-            io.base = (static_cast<std::size_t>(p) << G.log_n) + (static_cast<std::size_t>(cb) << G.log_seqs);
-            io.count = count;
-            io.stride = 1u << G.log_n2;
-            io.log_seqs = G.log_seqs;
-            io.rows = false;
-            io.valid = tile;
-        }
-
-        __device__ __forceinline__ u32 rows_setup(pass_geom const& G, u32 bid, std::size_t count, tile_desc& td, io_desc& io)
-        {
-            u32 p = bid / G.blocks_per_poly;
-            u32 rb = bid - p * G.blocks_per_poly;
-            u32 hi0 = rb << G.log_seqs;
-            td = tile_desc{G.log_n2, G.log_seqs, true};
-            io.base = (static_cast<std::size_t>(p) << G.log_n) + (static_cast<std::size_t>(hi0) << G.log_n2);
-            io.count = count;
-            io.stride = 0;
-            io.log_seqs = G.log_seqs;
-            io.rows = true;
-            io.valid = tile;
-            return hi0;
-        }
-
-        __device__ __forceinline__ void single_setup(pass_geom const& G, u32 bid, std::size_t n, tile_desc& td, io_desc& io)
-        {
-            td = tile_desc{G.log_n, G.log_seqs, true};
-            std::size_t first = static_cast<std::size_t>(bid) << G.log_seqs;
-            std::size_t polys = n - first < (std::size_t{1} << G.log_seqs) ? n - first : (std::size_t{1} << G.log_seqs);
-            io.base = first << G.log_n;
-            io.count = n << G.log_n;
-            io.stride = 0;
-            io.log_seqs = G.log_seqs;
-            io.rows = true;
-            io.valid = static_cast<int>(polys << G.log_n);
-        }
-
-        __host__ __device__ inline pass_geom geom_for(int log_n, bool rows_pass)
-        {
-            pass_geom G{};
-            G.log_n = log_n;
-            if (log_n <= tile_log)
-            {
-                G.single = true;
-                G.log_n1 = log_n;
-                G.log_n2 = 0;
-                G.log_seqs = tile_log - log_n;
-                G.blocks_per_poly = 1;
-                return G;
-            }
-            G.single = false;
-            // column tiles of at least 32 columns: each tile row is then 128 contiguous bytes per
-            // limb plane (whole cache lines; 16-column tiles fetched every line twice)
-            G.log_n1 = log_n / 2 < 7 ? log_n / 2 : 7;
-            if (log_n - G.log_n1 > tile_log)
-                G.log_n1 = log_n - tile_log;
-            G.log_n2 = log_n - G.log_n1;
-            if (rows_pass)
-            {
-                G.log_seqs = tile_log - G.log_n2;
-                G.blocks_per_poly = 1u << (G.log_n1 - G.log_seqs);
-            }
-            else
-            {
-                G.log_seqs = tile_log - G.log_n1;
-                G.blocks_per_poly = 1u << (G.log_n2 - G.log_seqs);
-            }
-            return G;
         }
 
         template<typename A>

@@ -42,34 +42,6 @@ namespace hmpc_amd
             return P;
         }
 
-        // ------------------------------------------------------------------ tile placement
-        __device__ __forceinline__ void job_cols_tile(pass_geom const& G, u32 t, std::size_t count, std::size_t off,
-                                                      tile_desc& td, io_desc& io)
-        {
-            cols_setup(G, t, count, td, io);
-            io.base += off;
-        }
-
-        __device__ __forceinline__ u32 job_rows_tile(pass_geom const& G, u32 t, std::size_t count, std::size_t off,
-                                                     tile_desc& td, io_desc& io)
-        {
-            u32 hi0 = rows_setup(G, t, count, td, io);
-            io.base += off;
-            return hi0;
-        }
-
-        __device__ __forceinline__ void job_single_tile(int log_n, std::size_t count, std::size_t off, tile_desc& td,
-                                                        io_desc& io)
-        {
-            td = tile_desc{log_n, tile_log - log_n, true};
-            io.base = off;
-            io.count = count;
-            io.stride = 0;
-            io.log_seqs = tile_log - log_n;
-            io.rows = true;
-            io.valid = 1 << log_n;
-        }
-
         template<typename A, int E>
         __device__ __forceinline__ void scatter_last(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td)
         {
@@ -106,29 +78,15 @@ namespace hmpc_amd
             constexpr int which = Which;
             int const log_n = static_cast<int>(J.log_n);
             u32 const half = 1u << (log_n - 1);
-            int const half_rows = cols ? (1 << (log_n1 - 1)) : 0;
-            int const pairs = cols ? (half_rows << log_seqs) : static_cast<int>(half);
+            int const pairs = sample_pairs(cols, log_n, log_n1, log_seqs);
             int const bits = J.bits[which];
             u32 const* nonce = J.nonce[which];
             long long const p = static_cast<long long>(J.p_int);
             for (int e = tid_x(); e < pairs; e += T)
             {
-                u32 k;
-                int elo, ehi;
-                if (cols)
-                {
-                    int r = e >> log_seqs;
-                    int c = e & ((1 << log_seqs) - 1);
-                    k = (static_cast<u32>(r) << (log_n - log_n1)) + col0 + static_cast<u32>(c);
-                    elo = e;
-                    ehi = e + (half_rows << log_seqs);
-                }
-                else
-                {
-                    k = static_cast<u32>(e);
-                    elo = e;
-                    ehi = e + static_cast<int>(half);
-                }
+                sample_slot sl = sample_slot_of(cols, e, log_n, log_n1, log_seqs, col0);
+                u32 const k = sl.k;
+                int const elo = sl.elo, ehi = sl.ehi;
                 u32 blk[16];
                 chacha20_block(blk, J.key, static_cast<u32>((static_cast<u64>(J.poly) << log_n) + k), nonce[0], nonce[1],
                                nonce[2]);

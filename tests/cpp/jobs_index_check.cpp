@@ -1,0 +1,372 @@
+// CPU check of the persistent job kernel's indexing (k_jobs in hmpc_amd/csrc/jobs_impl.hpp): the
+// access sequence of run_polymul_job and run_bgv_job is replayed for every thread of the workgroup
+// with the kernel's own geometry code (hmpc_amd/csrc/tile_geometry.hpp), for both job types, both
+// limb policies (4 limbs: 5 workspace planes; 2 limbs: 2), every elements-per-thread choice and
+// N = 2^2 .. 2^17, and every index is checked:
+//   operands, messages   within the job's own polynomial [off, off + N) of their planes
+//   public key           within its N elements
+//   outputs              within the job's polynomial of the output planes
+//   scratch              within the job's slot (job_scratch_words), planes of N elements
+//   twiddles             below N;  LDS elements below the 4096-element tile
+// It also counts the second-operand reads that the `valid` guard of with_operands skips for
+// whole-tile jobs of degree N < 4096: without that guard those reads would fall outside the N-element
+// key and scratch planes. Prints "ok" and the counts, or the first failure.
+#include "../../hmpc_amd/csrc/jobs.hpp"
+#include "../../hmpc_amd/csrc/tile_geometry.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+using namespace hmpc_amd;
+using namespace hmpc_amd::fast;
+
+namespace
+{
+    long long checks = 0, guarded = 0;
+    int failures = 0;
+
+    void require(bool ok, std::string const& what)
+    {
+        ++checks;
+        if (!ok && failures++ < 20)
+            std::printf("FAILED: %s\n", what.c_str());
+    }
+
+    struct job_case
+    {
+        int log_n;
+        int E;
+        int wsl;          // workspace planes per element
+        int io_limbs;     // limbs in global memory (2 or 4)
+        bool wave_units;  // unit order of the policy (false for the job kernel's policies)
+        std::size_t N, in_count, in_off, out_count, out_off, msg_off;
+        u64 slot_words;
+    };
+
+    // a buffer region: plane size `count`, allowed element range [lo, hi), planes at word offset
+    // `plane0` (scratch: the slot holds `slot_words`)
+    struct region
+    {
+        char const* name;
+        std::size_t count, lo, hi;
+        u64 plane0;
+        int planes;
+        u64 slot_words;  // 0: not a scratch region
+    };
+
+    void access(job_case const& J, region const& r, std::size_t idx, char const* where)
+    {
+        require(idx >= r.lo && idx < r.hi && idx < r.count,
+                std::string(where) + ": " + r.name + " index " + std::to_string(idx) + " outside [" + std::to_string(r.lo) +
+                    ", " + std::to_string(r.hi) + ") N=2^" + std::to_string(J.log_n) + " E=" + std::to_string(J.E));
+        if (r.slot_words)
+            require(r.plane0 + static_cast<u64>(r.planes - 1) * r.count + idx < r.slot_words,
+                    std::string(where) + ": scratch word outside the slot, N=2^" + std::to_string(J.log_n));
+    }
+
+    // tile_load / tile_store of the job kernel (no wave-local order): every thread t moves tile
+    // elements t, t + T, ... below io.valid
+    void tile_io(job_case const& J, io_desc const& io, region const& r, char const* where)
+    {
+        int const T = tile / J.E;
+        for (int t = 0; t < T; ++t)
+            for (int e = t; e < tile; e += T)
+                if (e < io.valid)
+                    access(J, r, gidx(io, e), where);
+    }
+
+    template<int E, int W>
+    void round_accesses(job_case const& J, tile_desc const& td, int a, bool inv, u32 base0, u32 base_step, bool scaled)
+    {
+        constexpr int K = 1 << W;
+        int const T = tile / E;
+        for (int t = 0; t < T; ++t)
+            for (int g = 0; g < E / K; ++g)
+            {
+                int u = J.wave_units ? unit_index_at<E, W, true>(t, g) : unit_index_at<E, W, false>(t, g);
+                unit_t un = unit_of(td, a, W, inv, u);
+                for (int k = 0; k < K; ++k)
+                {
+                    int el = elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, td.s, inv));
+                    require(el >= 0 && el < tile, "LDS element outside the tile");
+                }
+                u32 base = base0 + un.c * base_step;
+                for (int ii = 0; ii < W; ++ii)
+                {
+                    if (scaled && ii == W - 1)
+                        continue;  // the scaled final stage reads twiddles 0 and 1
+                    int i = a + ii;
+                    u32 h = 1u << (td.s - i - 1);
+                    int count = inv ? (1 << (W - ii - 1)) : (1 << ii);
+                    for (int m = 0; m < count; ++m)
+                    {
+                        u32 idx = 0;
+                        switch (ii)
+                        {
+                        case 0: idx = inv ? tw_index<true, W, 0>(base, h, i, un.F, m) : tw_index<false, W, 0>(base, h, i, un.F, m); break;
+                        case 1: if constexpr (W > 1) idx = inv ? tw_index<true, W, 1>(base, h, i, un.F, m) : tw_index<false, W, 1>(base, h, i, un.F, m); break;
+                        case 2: if constexpr (W > 2) idx = inv ? tw_index<true, W, 2>(base, h, i, un.F, m) : tw_index<false, W, 2>(base, h, i, un.F, m); break;
+                        default: if constexpr (W > 3) idx = inv ? tw_index<true, W, 3>(base, h, i, un.F, m) : tw_index<false, W, 3>(base, h, i, un.F, m); break;
+                        }
+                        require(idx < J.N, "twiddle index " + std::to_string(idx) + " >= N=2^" + std::to_string(J.log_n));
+                    }
+                }
+            }
+    }
+
+    template<int E>
+    void rounds(job_case const& J, tile_desc const& td, bool inv, u32 base0, u32 base_step, bool final_scale, int first_round)
+    {
+        constexpr int WM = ilog2(E);
+        int n = round_count<WM>(td.s);
+        for (int r = first_round; r < n; ++r)
+        {
+            int a = round_start<WM>(td.s, r, inv);
+            int w = round_width<WM>(td.s, r, inv);
+            bool scaled = inv && final_scale && a + w == td.s;
+            switch (w)
+            {
+            case 1: round_accesses<E, 1>(J, td, a, inv, base0, base_step, scaled); break;
+            case 2: if constexpr (WM >= 2) round_accesses<E, 2>(J, td, a, inv, base0, base_step, scaled); break;
+            case 3: if constexpr (WM >= 3) round_accesses<E, 3>(J, td, a, inv, base0, base_step, scaled); break;
+            case 4: if constexpr (WM >= 4) round_accesses<E, 4>(J, td, a, inv, base0, base_step, scaled); break;
+            default: require(false, "round width");
+            }
+        }
+    }
+
+    // with_operands after the forward's last round: src (and key) at base + e0 + k for e0 + k < valid
+    template<int E, int W>
+    void operands_w(job_case const& J, int valid, region const& src, std::size_t src_base, region const* key, std::size_t key_base)
+    {
+        constexpr int K = 1 << W;
+        int const T = tile / E;
+        for (int t = 0; t < T; ++t)
+            for (int g = 0; g < E / K; ++g)
+            {
+                int e0 = (J.wave_units ? unit_index_at<E, W, true>(t, g) : unit_index_at<E, W, false>(t, g)) << W;
+                for (int k = 0; k < K; ++k)
+                {
+                    if (e0 + k >= valid)
+                    {
+                        // skipped by the guard: this read would lie beyond the plane's N elements
+                        if (src_base + static_cast<std::size_t>(e0 + k) >= src.hi)
+                            ++guarded;
+                        continue;
+                    }
+                    access(J, src, src_base + static_cast<std::size_t>(e0 + k), "second operand");
+                    if (key)
+                        access(J, *key, key_base + static_cast<std::size_t>(e0 + k), "public key");
+                }
+            }
+    }
+
+    template<int E>
+    void operands(job_case const& J, tile_desc const& td, int valid, region const& src, std::size_t src_base, region const* key,
+                  std::size_t key_base)
+    {
+        constexpr int WM = ilog2(E);
+        int w = round_width<WM>(td.s, round_count<WM>(td.s) - 1, false);
+        switch (w)
+        {
+        case 1: operands_w<E, 1>(J, valid, src, src_base, key, key_base); break;
+        case 2: if constexpr (WM >= 2) operands_w<E, 2>(J, valid, src, src_base, key, key_base); break;
+        case 3: if constexpr (WM >= 3) operands_w<E, 3>(J, valid, src, src_base, key, key_base); break;
+        case 4: if constexpr (WM >= 4) operands_w<E, 4>(J, valid, src, src_base, key, key_base); break;
+        default: require(false, "operand round width");
+        }
+    }
+
+    void sample(job_case const& J, bool cols, int log_n1, int log_seqs, u32 col0, region const& msg)
+    {
+        std::size_t half = J.N / 2;
+        int pairs = sample_pairs(cols, J.log_n, log_n1, log_seqs);
+        for (int e = 0; e < pairs; ++e)
+        {
+            sample_slot sl = sample_slot_of(cols, e, J.log_n, log_n1, log_seqs, col0);
+            require(sl.k < half, "sampled pair index >= N/2");
+            require(sl.elo >= 0 && sl.elo < tile && sl.ehi >= 0 && sl.ehi < tile, "sampled LDS element outside the tile");
+            access(J, msg, J.msg_off + sl.k, "message");
+            access(J, msg, J.msg_off + sl.k + half, "message");
+        }
+    }
+
+    region scratch_plane(job_case const& J, int which)
+    {
+        return {"scratch", J.N, 0, J.N, static_cast<u64>(which) * J.wsl * J.N, J.wsl, J.slot_words};
+    }
+
+    template<int E>
+    void polymul_job(job_case const& J)
+    {
+        region in{"operand", J.in_count, J.in_off, J.in_off + J.N, 0, J.io_limbs, 0};
+        region out{"output", J.out_count, J.out_off, J.out_off + J.N, 0, J.io_limbs, 0};
+        int const log_n = J.log_n;
+        if (log_n <= tile_log)
+        {
+            tile_desc td;
+            io_desc ia, io, is;
+            job_single_tile(log_n, J.in_count, J.in_off, td, ia);
+            job_single_tile(log_n, J.out_count, J.out_off, td, io);
+            job_single_tile(log_n, J.N, 0, td, is);
+            region sa = scratch_plane(J, 0);
+            tile_io(J, ia, in, "load a");
+            rounds<E>(J, td, false, 1u, 0u, false, 0);
+            tile_io(J, is, sa, "store NTT(a)");
+            tile_io(J, ia, in, "load b");
+            rounds<E>(J, td, false, 1u, 0u, false, 0);
+            operands<E>(J, td, is.valid, sa, 0, nullptr, 0);
+            rounds<E>(J, td, true, 1u, 0u, true, 0);
+            tile_io(J, io, out, "store product");
+            return;
+        }
+        pass_geom const gc = geom_for(log_n, false);
+        pass_geom const gr = geom_for(log_n, true);
+        region sa = scratch_plane(J, 0), sb = scratch_plane(J, 1);
+        for (u32 t = 0; t < 2 * gc.blocks_per_poly; ++t)
+        {
+            u32 which = t >= gc.blocks_per_poly ? 1u : 0u;
+            u32 ct = t - which * gc.blocks_per_poly;
+            tile_desc td;
+            io_desc ii, is;
+            job_cols_tile(gc, ct, J.in_count, J.in_off, td, ii);
+            job_cols_tile(gc, ct, J.N, 0, td, is);
+            tile_io(J, ii, in, "column load");
+            rounds<E>(J, td, false, 1u, 0u, false, 0);
+            tile_io(J, is, which ? sb : sa, "column store");
+        }
+        for (u32 t = 0; t < gr.blocks_per_poly; ++t)
+        {
+            tile_desc td;
+            io_desc is;
+            u32 hi0 = job_rows_tile(gr, t, J.N, 0, td, is);
+            u32 base0 = (1u << gr.log_n1) + hi0;
+            tile_io(J, is, sa, "row load a");
+            rounds<E>(J, td, false, base0, 1u, false, 0);
+            tile_io(J, is, sa, "row store a");
+            tile_io(J, is, sb, "row load b");
+            rounds<E>(J, td, false, base0, 1u, false, 0);
+            operands<E>(J, td, is.valid, sa, is.base, nullptr, 0);
+            rounds<E>(J, td, true, base0, 1u, false, 0);
+            tile_io(J, is, sa, "row store product");
+        }
+        for (u32 t = 0; t < gc.blocks_per_poly; ++t)
+        {
+            tile_desc td;
+            io_desc is, io;
+            job_cols_tile(gc, t, J.N, 0, td, is);
+            job_cols_tile(gc, t, J.out_count, J.out_off, td, io);
+            tile_io(J, is, sa, "inverse column load");
+            rounds<E>(J, td, true, 1u, 0u, true, 0);
+            tile_io(J, io, out, "inverse column store");
+        }
+    }
+
+    template<int E>
+    void bgv_job(job_case const& J)
+    {
+        region key{"public key", J.N, 0, J.N, 0, J.io_limbs, 0};
+        region out{"ciphertext", J.out_count, J.out_off, J.out_off + J.N, 0, J.io_limbs, 0};
+        region msg{"message", J.out_count, J.msg_off, J.msg_off + J.N, 0, 1, 0};
+        int const log_n = J.log_n;
+        if (log_n <= tile_log)
+        {
+            tile_desc td;
+            io_desc ik, io, is;
+            job_single_tile(log_n, J.N, 0, td, ik);
+            job_single_tile(log_n, J.out_count, J.out_off, td, io);
+            job_single_tile(log_n, J.N, 0, td, is);
+            region su = scratch_plane(J, 0);
+            sample(J, false, 0, 0, 0u, msg);
+            rounds<E>(J, td, false, 1u, 0u, false, 0);
+            tile_io(J, is, su, "store NTT(u)");
+            for (int which = 1; which < 3; ++which)
+            {
+                sample(J, false, 0, 0, 0u, msg);
+                rounds<E>(J, td, false, 1u, 0u, false, 0);
+                operands<E>(J, td, ik.valid, su, 0, &key, ik.base);
+                tile_io(J, io, out, "store c");
+            }
+            return;
+        }
+        pass_geom const gc = geom_for(log_n, false);
+        pass_geom const gr = geom_for(log_n, true);
+        for (int which = 0; which < 3; ++which)
+        {
+            region dst = scratch_plane(J, which);
+            for (u32 ct = 0; ct < gc.blocks_per_poly; ++ct)
+            {
+                tile_desc td;
+                io_desc is;
+                job_cols_tile(gc, ct, J.N, 0, td, is);
+                sample(J, true, gc.log_n1, gc.log_seqs, ct << gc.log_seqs, msg);
+                rounds<E>(J, td, false, 1u, 0u, false, 0);
+                tile_io(J, is, dst, "column store");
+            }
+        }
+        region su = scratch_plane(J, 0);
+        for (u32 t = 0; t < gr.blocks_per_poly; ++t)
+        {
+            tile_desc td;
+            io_desc is, ik, io;
+            u32 hi0 = job_rows_tile(gr, t, J.N, 0, td, is);
+            job_rows_tile(gr, t, J.N, 0, td, ik);
+            job_rows_tile(gr, t, J.out_count, J.out_off, td, io);
+            u32 base0 = (1u << gr.log_n1) + hi0;
+            tile_io(J, is, su, "row load u");
+            rounds<E>(J, td, false, base0, 1u, false, 0);
+            tile_io(J, is, su, "row store u");
+            for (int which = 1; which < 3; ++which)
+            {
+                tile_io(J, is, scratch_plane(J, which), "row load x");
+                rounds<E>(J, td, false, base0, 1u, false, 0);
+                operands<E>(J, td, ik.valid, su, is.base, &key, ik.base);
+                tile_io(J, io, out, "row store c");
+            }
+        }
+    }
+
+    template<int E>
+    void run_case(int log_n, int io_limbs)
+    {
+        std::size_t const N = std::size_t{1} << log_n;
+        for (u32 type : {job_polymul, job_bgv})
+        {
+            job_case J{};
+            J.log_n = log_n;
+            J.E = E;
+            J.io_limbs = io_limbs;
+            J.wsl = ws_limbs(io_limbs);
+            J.wave_units = false;
+            J.N = N;
+            // the middle job of three: neighbours on both sides
+            J.in_count = type == job_bgv ? N : 3 * N;
+            J.in_off = type == job_bgv ? 0 : N;
+            J.out_count = 3 * N;
+            J.out_off = N;
+            J.msg_off = N;
+            J.slot_words = job_scratch_words(type, log_n, io_limbs);
+            if (type == job_polymul)
+                polymul_job<E>(J);
+            else
+                bgv_job<E>(J);
+        }
+    }
+}
+
+int main()
+{
+    for (int log_n = 2; log_n <= 17; ++log_n)
+    {
+        // 4 limbs (radix-2^26 policy): 4 or 8 elements per thread; 2 limbs (Goldilocks): 8 or 16
+        run_case<4>(log_n, 4);
+        run_case<8>(log_n, 4);
+        run_case<8>(log_n, 2);
+        run_case<16>(log_n, 2);
+    }
+    if (guarded == 0)
+        require(false, "expected second-operand reads beyond N for whole-tile jobs of N < 4096");
+    std::printf("%s checks=%lld guarded=%lld\n", failures ? "FAILED" : "ok", checks, guarded);
+    return failures ? 1 : 0;
+}
