@@ -270,6 +270,7 @@ struct table_pair
 struct hmpc_modulus_s
 {
     hmpc_context ctx = nullptr;
+    int device = 0;  // the context's device, kept for destruction in any order with the context
     int limbs = 0;
     int bit_size = 0;
     mod_params P{};
@@ -509,9 +510,15 @@ namespace
         pow_mod<L>(r, x, e, L, one, m.P);
     }
 
+    // makes `device` current; a failure is consumed here, so that it is not reported later as the
+    // error of an unrelated launch (hipGetLastError after a launch)
     struct device_guard
     {
-        explicit device_guard(int device) { (void)hipSetDevice(device); }
+        explicit device_guard(int device)
+        {
+            if (hipSetDevice(device) != hipSuccess)
+                (void)hipGetLastError();
+        }
     };
 
     // installs the context's timing probe for the duration of one entry point
@@ -538,7 +545,19 @@ int hmpc_modulus_s::prepare(int log_n)
         return HMPC_OK;
     device_guard g(ctx->device);
     table_pair t;
-    bool ok = host_limbs(limbs, [&](auto LC) {
+    auto release = [&]() {
+        for (u32* p : {t.fwd, t.inv, t.fwd_em, t.inv_em})
+            if (p)
+                (void)hipFree(p);
+        if (t.fast_owned)
+            for (u32* p : {t.fwd_fast, t.inv_fast})
+                if (p)
+                    (void)hipFree(p);
+    };
+    // the root search fails for a modulus without a 2N-th root; any later failure is the device's
+    char const* where = nullptr;
+    hipError_t err = hipSuccess;
+    bool root = host_limbs(limbs, [&](auto LC) {
         constexpr int LL = decltype(LC)::value;
         u32 phi[LL], iphi[LL], nval[LL] = {}, nm[LL], ninv[LL], r2v[LL];
         if (!root_search<LL>(*this, log_n + 1, phi))
@@ -551,45 +570,48 @@ int hmpc_modulus_s::prepare(int log_n)
         invert_mont<LL>(ninv, nm, *this);
         std::copy(phi, phi + LL, t.root);
         std::copy(iphi, iphi + LL, t.inverse_root);
+        auto step = [&](hipError_t e, char const* w) {
+            if (err == hipSuccess && e != hipSuccess)
+            {
+                err = e;
+                where = w;
+            }
+            return err == hipSuccess;
+        };
         std::size_t bytes = (std::size_t{4} * LL) << log_n;
         for (u32** p : {&t.fwd, &t.inv, &t.fwd_em, &t.inv_em})
-            if (hipMalloc(p, bytes) != hipSuccess)
-                return false;
-        if (launch_make_twiddles(LL, t.fwd, log_n, phi, ninv, one, false, P, nullptr) != hipSuccess)
-            return false;
-        if (launch_make_twiddles(LL, t.inv, log_n, iphi, ninv, one, true, P, nullptr) != hipSuccess)
-            return false;
-        if (launch_to_element_major(LL, t.fwd, t.fwd_em, std::size_t{1} << log_n, nullptr) != hipSuccess)
-            return false;
-        if (launch_to_element_major(LL, t.inv, t.inv_em, std::size_t{1} << log_n, nullptr) != hipSuccess)
-            return false;
+            if (!step(hipMalloc(p, bytes), "hipMalloc (twiddle tables)"))
+                return true;
+        if (!step(launch_make_twiddles(LL, t.fwd, log_n, phi, ninv, one, false, P, nullptr), "twiddle kernel") ||
+            !step(launch_make_twiddles(LL, t.inv, log_n, iphi, ninv, one, true, P, nullptr), "twiddle kernel") ||
+            !step(launch_to_element_major(LL, t.fwd, t.fwd_em, std::size_t{1} << log_n, nullptr), "table relayout") ||
+            !step(launch_to_element_major(LL, t.inv, t.inv_em, std::size_t{1} << log_n, nullptr), "table relayout"))
+            return true;
         if (form == form_r26)
         {
             std::size_t fb = (std::size_t{4} * 8) << log_n;
-            if (hipMalloc(&t.fwd_fast, fb) != hipSuccess || hipMalloc(&t.inv_fast, fb) != hipSuccess)
-                return false;
+            if (!step(hipMalloc(&t.fwd_fast, fb), "hipMalloc (radix-2^26 tables)") ||
+                !step(hipMalloc(&t.inv_fast, fb), "hipMalloc (radix-2^26 tables)"))
+                return true;
             t.fast_owned = true;
-            if (launch_r26_table(t.fwd, t.fwd_fast, std::size_t{1} << log_n, r26, nullptr) != hipSuccess ||
-                launch_r26_table(t.inv, t.inv_fast, std::size_t{1} << log_n, r26, nullptr) != hipSuccess)
-                return false;
+            if (!step(launch_r26_table(t.fwd, t.fwd_fast, std::size_t{1} << log_n, r26, nullptr), "radix-2^26 table") ||
+                !step(launch_r26_table(t.inv, t.inv_fast, std::size_t{1} << log_n, r26, nullptr), "radix-2^26 table"))
+                return true;
         }
         else
         {
             t.fwd_fast = t.fwd_em;
             t.inv_fast = t.inv_em;
         }
-        return hipDeviceSynchronize() == hipSuccess;
+        step(hipStreamSynchronize(nullptr), "hipStreamSynchronize after the table kernels");
+        return true;
     });
-    if (!ok)
-    {
-        for (u32* p : {t.fwd, t.inv, t.fwd_em, t.inv_em})
-            if (p)
-                (void)hipFree(p);
-        if (t.fast_owned)
-            for (u32* p : {t.fwd_fast, t.inv_fast})
-                if (p)
-                    (void)hipFree(p);
+    if (!root)
         return fail(HMPC_ERR_INVALID_MODULUS, "no primitive 2N-th root of unity for this modulus (2N must divide q - 1)");
+    if (err != hipSuccess)
+    {
+        release();
+        return hip_fail(err, where);
     }
     tables[log_n] = t;
     return HMPC_OK;
@@ -823,6 +845,7 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
         return fail(HMPC_ERR_INVALID_MODULUS, "modulus wider than 288 bits is not supported");
     auto m = std::make_unique<hmpc_modulus_s>();
     m->ctx = ctx;
+    m->device = ctx->device;
     m->limbs = L;
     m->bit_size = bit_size;
     m->P.limbs = L;
@@ -876,7 +899,7 @@ int hmpc_modulus_destroy(hmpc_modulus mod)
 {
     if (mod)
     {
-        device_guard g(mod->ctx->device);
+        device_guard g(mod->device);
         (void)hipDeviceSynchronize();
         delete mod;
     }
@@ -1737,6 +1760,8 @@ int hmpc_bgv_decrypt(hmpc_modulus mod, int log_n, size_t n, const uint32_t* s_nt
 }
 
 // ------------------------------------------------------------------ general plaintext path
+}  // extern "C"
+
 namespace
 {
     // plain 2^k mod q (k doublings of 1)
@@ -1872,6 +1897,8 @@ namespace
         return HMPC_OK;
     }
 }
+
+extern "C" {
 
 int hmpc_cast(hmpc_modulus from, hmpc_modulus to, size_t count, const uint32_t* in, uint32_t* out, void* stream)
 {
@@ -2083,6 +2110,7 @@ struct hmpc_batch_s
     };
 
     hmpc_context ctx = nullptr;
+    int device = 0;  // kept for destruction in any order with the context
     std::vector<entry> entries;
     std::vector<job_class> classes;
     tile_plan tiles;               // jobs of two-pass degrees
@@ -2115,7 +2143,7 @@ struct hmpc_batch_s
 
     ~hmpc_batch_s()
     {
-        device_guard g(ctx->device);
+        device_guard g(device);
         (void)hipDeviceSynchronize();
         release();
         if (capture_stream)
@@ -2241,6 +2269,7 @@ int hmpc_batch_create(hmpc_context ctx, hmpc_batch* out)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
     auto b = std::make_unique<hmpc_batch_s>();
     b->ctx = ctx;
+    b->device = ctx->device;
     *out = b.release();
     return HMPC_OK;
 }
@@ -2708,6 +2737,7 @@ namespace
 struct hmpc_server_s
 {
     hmpc_context ctx = nullptr;
+    int device = 0;  // kept for destruction in any order with the context
     std::size_t budget = 0;
     std::size_t in_use = 0;
     std::size_t max_jobs = 4096;
@@ -2736,7 +2766,7 @@ struct hmpc_server_s
     // worker: completions, then admission into free slots
     void run()
     {
-        (void)hipSetDevice(ctx->device);
+        device_guard g(device);
         std::unique_lock<std::mutex> lk(m);
         for (;;)
         {
@@ -3016,6 +3046,7 @@ int hmpc_server_create(hmpc_context ctx, size_t budget_bytes, int slots, hmpc_se
     device_guard g(ctx->device);
     auto srv = std::make_unique<hmpc_server_s>();
     srv->ctx = ctx;
+    srv->device = ctx->device;
     srv->budget = budget_bytes;
     srv->pool.alloc = pool_alloc;
     srv->pool.release = pool_free;
@@ -3046,7 +3077,7 @@ int hmpc_server_destroy(hmpc_server srv)
     srv->work_cv.notify_all();
     if (srv->worker.joinable())
         srv->worker.join();
-    device_guard g(srv->ctx->device);
+    device_guard g(srv->device);
     for (auto& s : srv->slots)
     {
         (void)hipStreamSynchronize(s->stream);

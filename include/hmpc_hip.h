@@ -96,6 +96,7 @@ int hmpc_stream_synchronize(hmpc_context ctx, void* stream);
  * (hmpc/expr/number_theoretic_transform.hpp:35-88), cached like the queue's extra tensors
  * (hmpc/comp/queue.hpp:364-381). q must be odd with bit_width(q - 1) <= 288. */
 int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_modulus* out);
+/* A modulus (like a batch or a server) keeps its device: it may be destroyed after its context. */
 int hmpc_modulus_destroy(hmpc_modulus mod);
 /* limbs L, bit size of q - 1, and constants (each array L words, may be NULL) */
 int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* nprime, uint32_t* r_mod_q, uint32_t* r2_mod_q);
