@@ -214,6 +214,42 @@ namespace
     };
 }
 
+// ------------------------------------------------------------------ allocation-free workspace
+// Workspace of hmpc_polymul_noalloc: a file-scope device array of a fixed size. Every GPU the library
+// runs on holds its own copy from module load until the process ends. In host code `scratch` is only
+// a stand-in for the symbol: its device address on each GPU comes from hipGetSymbolAddress, taken once
+// per device when the first context on it is created (with that device current), and only that
+// address is passed to kernels. Calls that overlap on one GPU share (and race on) the array.
+namespace
+{
+    constexpr std::size_t noalloc_scratch_bytes = std::size_t{64} << 20;
+    constexpr int max_devices = 64;
+    std::mutex noalloc_mutex;
+    void* noalloc_address[max_devices] = {};
+
+    // the array's address on `device` (current), resolved once
+    int noalloc_resolve(int device);
+}
+static __device__ __align__(256) u32 scratch[noalloc_scratch_bytes / 4];
+
+namespace
+{
+    int noalloc_resolve(int device)
+    {
+        if (device < 0 || device >= max_devices)
+            return fail(HMPC_ERR_INVALID_ARGUMENT, "device ordinal beyond the allocation-free workspace table");
+        std::lock_guard<std::mutex> lock(noalloc_mutex);
+        if (noalloc_address[device])
+            return HMPC_OK;
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        noalloc_address[device] = ptr;
+        return HMPC_OK;
+    }
+}
+
 struct hmpc_context_s
 {
     int device = 0;
@@ -634,6 +670,9 @@ int hmpc_context_create(int device, const uint32_t* key, hmpc_context* out)
     ctx->device = device;
     HMPC_HIP(hipSetDevice(device));
     HMPC_HIP(hipGetDeviceProperties(&ctx->props, device));
+    int rc = noalloc_resolve(device);
+    if (rc)
+        return rc;
     if (key)
         std::copy(key, key + 8, ctx->key);
     else
@@ -1592,6 +1631,48 @@ int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const
         HMPC_HIP(launch_fast_polymul(mod->limbs, T, n, a, b, out, static_cast<u32*>(w), s));
     else
         HMPC_HIP(launch_polymul(mod->limbs, T, n, a, b, out, static_cast<u32*>(w), s));
+    return HMPC_OK;
+}
+
+// hmpc_polymul's per-pass kernels with their workspace in the file-scope array (no allocation: the
+// tables of N must be prepared, and the workspace must fit the array). The workspace needs no
+// clearing: every word the kernels read was written by the same call before.
+int hmpc_polymul_noalloc(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                         void* stream)
+{
+    if (!mod || (n && (!a || !b || !out)))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
+    if (!supported_log_n(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
+    if (!mod->tables.count(log_n))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "hmpc_polymul_noalloc: prepare the tables first (hmpc_modulus_prepare)");
+    if (n == 0)
+        return HMPC_OK;
+    if (n > 0x7fffffffu)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
+    bool fast = mod->fast(log_n);
+    std::size_t words = fast ? fast_polymul_workspace_words(mod->limbs, log_n, n) : polymul_workspace_words(mod->limbs, log_n, n);
+    if (words * 4 > noalloc_scratch_bytes)
+        return fail(HMPC_ERR_INVALID_ARGUMENT, "hmpc_polymul_noalloc: the workspace of " + std::to_string(words * 4) +
+                                                   " bytes exceeds the fixed " + std::to_string(noalloc_scratch_bytes) +
+                                                   " bytes (fewer pairs per call)");
+    transform_args T;
+    int rc = mod->args(log_n, T);  // tables present: no allocation
+    if (rc)
+        return rc;
+    device_guard g(mod->ctx->device);
+    u32* w = static_cast<u32*>(noalloc_address[mod->ctx->device]);
+    if (!w)
+        return fail(HMPC_ERR_DEVICE, "allocation-free workspace not resolved for this device");
+    hipStream_t s = as_stream(stream);
+    if (fast)
+    {
+        auto e = mod->elems.find(log_n);
+        T.elems = e != mod->elems.end() ? e->second : mod->default_elems(log_n);
+        HMPC_HIP(launch_fast_polymul(mod->limbs, T, n, a, b, out, w, s));
+    }
+    else
+        HMPC_HIP(launch_polymul(mod->limbs, T, n, a, b, out, w, s));
     return HMPC_OK;
 }
 

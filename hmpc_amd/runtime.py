@@ -72,6 +72,7 @@ SIGNATURES = {
     "hmpc_sample_words": (_i, [_vp, _u32p, _u64, _sz, _vp, _vp]),
     "hmpc_sample_cbd_ntt": (_i, [_vp, _vp, _i, _i, _sz, _vp, _vp]),
     "hmpc_polymul": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp]),
+    "hmpc_polymul_noalloc": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp]),
     "hmpc_bgv_encrypt": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
     "hmpc_bgv_encrypt_plain": (_i, [_vp, _vp, _i, _sz, _vp, _vp, _vp, _u32p, _i, _i, _i, _vp, _vp, _vp]),
     "hmpc_bgv_decrypt": (_i, [_vp, _i, _sz, _vp, _vp, _vp, _vp, _vp]),
@@ -566,6 +567,11 @@ class Modulus:
     def polymul(self, log_n, n, a: Buffer, b: Buffer, out: Buffer | None = None, stream=None) -> Buffer:
         out = out or self.alloc(n << log_n)
         check(_lib.hmpc_polymul(self.handle, log_n, n, a.ptr, b.ptr, out.ptr, _sp(stream)))
+        return out
+
+    def polymul_noalloc(self, log_n, n, a: Buffer, b: Buffer, out: Buffer, stream=None) -> Buffer:
+        """hmpc_polymul_noalloc: the caller passes every buffer; the library allocates nothing."""
+        check(_lib.hmpc_polymul_noalloc(self.handle, log_n, n, a.ptr, b.ptr, out.ptr, _sp(stream)))
         return out
 
     def bgv_encrypt(self, log_n, n, pk_a: Buffer, pk_b: Buffer, messages: Buffer, p_mont: np.ndarray,

@@ -228,6 +228,16 @@ int hmpc_sample_cbd_ntt(hmpc_context ctx, hmpc_modulus mod, int bits, int log_n,
  * Batched negacyclic products out_i = INTT(NTT(a_i) o NTT(b_i)), i < n, in one fused pass
  * sequence (LDS-resident stages, no per-stage launches). */
 int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out, void* stream);
+/* hmpc_polymul without any allocation: no counterpart in hmpc. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid allocation). The
+ * same results through the per-pass kernels, with their workspace (2 x 5 (4 limbs) or 2 x L words per
+ * coefficient for N > 4096; none for N <= 4096) in a fixed 64 MiB device array that the library holds
+ * on every GPU it runs on until the process ends. Fails with HMPC_ERR_INVALID_ARGUMENT when the
+ * workspace would exceed 64 MiB (e.g. more than 12 pairs at N = 2^16 with 4 limbs) or when the tables
+ * of N were not prepared (hmpc_modulus_prepare). Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams or threads) race. */
+int hmpc_polymul_noalloc(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                         void* stream);
 
 /* BGV public-key encryption of n messages (hmpc/expr/crypto/lhe/enc.hpp:19-69 with the
  * default randomness of hmpc/expr/crypto/lhe/randomness.hpp:80-110):
