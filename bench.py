@@ -137,8 +137,18 @@ def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
     pc0, pc1 = mod.alloc(n_plain * N), mod.alloc(n_plain * N)
     pms, _, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n_plain, A, B, msgs, t_mont, c0=pc0, c1=pc1,
                                                                plain=True, stream=stream), 2, 1)
+    # the fused kernels launched once per ciphertext (hmpc_bgv_encrypt with n = 1, one call each)
+    views = [(hm.View(ctx, msgs.ptr + 4 * i * N), hm.View(ctx, pc0.ptr + 16 * i * N), hm.View(ctx, pc1.ptr + 16 * i * N))
+             for i in range(n_plain)]
+
+    def one_by_one():
+        for m_i, c0_i, c1_i in views:
+            mod.bgv_encrypt(log_n, 1, A, B, m_i, t_mont, c0=c0_i, c1=c1_i, stream=stream)
+
+    fms, _, _ = time_steps(ctx, stream, one_by_one, 3, 1)
     return {"n": n, "ms": ms, "per": per, "enc_per_s": n / (ms / 1e3), "decrypt_ok": ok, "schedule": tuned,
-            "kernels": kernels, "plain_enc_per_s": n_plain / (pms / 1e3), "plain_sample": n_plain}
+            "kernels": kernels, "plain_enc_per_s": n_plain / (pms / 1e3), "plain_sample": n_plain,
+            "per_ciphertext_fused_enc_per_s": n_plain / (fms / 1e3)}
 
 
 def bench_c1(hm, ctx, stream, iters=200):
@@ -240,11 +250,14 @@ def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
             "device_bytes": info["device_bytes"], "schedules": tuned}
 
 
-def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0)):
+def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0),
+             operating_requests=100000):
     """Serving: 16 resident BGV keys (q128a, N = 2^13..2^16, four keys per N); each request
     encrypts k ~ U[1, 64] messages under a uniformly chosen key. Peak = closed-loop throughput with
     every request queued at once; then open-loop Poisson arrivals at each fraction of the peak
-    rate. Latency runs from the scheduled arrival to completion (client lateness included)."""
+    rate (`requests` per point: the sweep is reduced below the configured 10^5 unless asked), and
+    the configured 10^5 requests at the operating point (the highest load served at >= 80% of the
+    peak). Latency runs from the scheduled arrival to completion (client lateness included)."""
     mod = hm.Modulus(ctx, hm.Q128A)
     t_mont = mod.mont(1 << 32)
     log(f"[bench] C5 serving: {requests} requests per load point")
@@ -314,6 +327,18 @@ def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0
                        "p99_ms": percentile(lat, 99)})
     ok = [p for p in points if p["achieved_req_per_s"] >= 0.8 * peak_rps]
     op = max(ok, key=lambda p: p["fraction"]) if ok else points[-1]
+    # the configured request count (10^5) at the operating point's load
+    full = None
+    if operating_requests:
+        rate = op["fraction"] * peak_rps
+        reqs = draw(operating_requests)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+        log(f"[bench] C5 operating point: {operating_requests} requests at {rate:.0f} req/s")
+        wall, lat = run(reqs, arrivals)
+        full = {"requests": operating_requests, "fraction": op["fraction"], "offered_req_per_s": rate,
+                "achieved_req_per_s": len(reqs) / max(wall, arrivals[-1]),
+                "ct_per_s": sum(k for _, k in reqs) / wall, "p50_ms": percentile(lat, 50),
+                "p99_ms": percentile(lat, 99)}
     st = srv.stats()
     srv.close()
     # plain path: the same kind of requests, one per call through hmpc's launch structure
@@ -328,9 +353,98 @@ def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0
         n_ct += k
     stream.synchronize()
     plain_ct = n_ct / (time.perf_counter() - t0)
-    return {"requests_per_point": requests, "peak_req_per_s": peak_rps, "peak_ct_per_s": cts / (len(reqs) / peak_rps),
-            "operating_point": op, "points": points, "plain_ct_per_s": plain_ct, "plain_sample_requests": len(sample),
+    return {"requests_per_point": requests, "reduced_sweep": requests < 100000, "configured_requests_per_point": 100000,
+            "peak_req_per_s": peak_rps, "peak_ct_per_s": cts / (len(reqs) / peak_rps),
+            "operating_point": op, "operating_point_full": full, "points": points, "plain_ct_per_s": plain_ct,
+            "plain_sample_requests": len(sample),
             "launches": st["launches"], "budget_bytes": 200 << 30, "slots": 3}
+
+
+# hmpc's examples/ntt.cpp:22-23 (p, N = 2^20, n = 10) and examples/bgv.cpp:25-29 (q, N = 2^14, p, n = 100), and
+# the size of tests/expr/crypto/lhe/enc.cpp (q of 279 bits, p = 0x8822d80623320001, N = 2^16, shape {4, 2})
+P_NTT_EXAMPLE = 0x18F25CD9A75CCBD9C146D4ABAEC00001
+Q_BGV_EXAMPLE = 0x2FAEADBE7A0195C011AC195AD10269830E8001
+P_BGV_EXAMPLE = 0x89D046E0AE640001
+Q_ENC_TEST = 0x591F5B834C0D961F67343BCC8902BDEDA2771F54306FF151162FF8D2B40F4194DC0001
+P_ENC_TEST = 0x8822D80623320001
+
+
+def bench_examples(hm, ctx, stream, reps=5):
+    """hmpc's own example workloads on their moduli, with argc = 1:
+    - examples/ntt.cpp: forward NTT of n = 10 polynomials of N = 2^20 over the 125-bit p, inputs
+      argc + N n - i; timed: the transform; checked by the inverse transform;
+    - examples/bgv.cpp: encryption of n = 100 messages at N = 2^14 over the 150-bit q with keys
+      NTT(argc + n + i), NTT(argc + N - i), messages argc + n - i and the constant p; timed: the
+      samplers, transforms and combination, which here also transform the messages (hmpc transforms
+      them before its clock starts);
+    - tests/expr/crypto/lhe/enc.cpp at its size (q of 279 bits, p = 0x8822d80623320001, N = 2^16,
+      8 messages) through the plaintext path: encryption and dec<plaintext>; checked: dec(enc(x)) == x."""
+    out = {}
+    # examples/ntt.cpp
+    p = hm.Modulus(ctx, P_NTT_EXAMPLE)
+    log_n, n = 20, 10
+    N = 1 << log_n
+    p.prepare(log_n)
+    words = ((1 + N * n - np.arange(N * n, dtype=np.int64)) & 0xFFFFFFFF).astype(np.uint32)
+    w = ctx.upload(words)
+    x = p.from_u32(N * n, w)
+    y, z = p.alloc(N * n), p.alloc(N * n)
+    ms, _, _ = time_steps(ctx, stream, lambda: p.ntt(log_n, n, x, out=y, stream=stream), reps, 1)
+    p.ntt(log_n, n, y, out=z, inverse=True, stream=stream)
+    stream.synchronize()
+    ok = bool(np.array_equal(p.download(z, N * n), p.download(x, N * n)))
+    out["ntt_example"] = {"source": "examples/ntt.cpp:22-23", "q_bits": P_NTT_EXAMPLE.bit_length(), "limbs": p.limbs,
+                          "N": N, "n": n, "ms": ms, "gcoeff_per_s": n * N / (ms / 1e3) / 1e9,
+                          "fast_kernels": p.schedule(log_n)["fast"], "roundtrip_ok": ok}
+    del x, y, z, w
+    # examples/bgv.cpp
+    q = hm.Modulus(ctx, Q_BGV_EXAMPLE)
+    log_n, n = 14, 100
+    N = 1 << log_n
+    q.prepare(log_n)
+    argc = 1
+    a = q.from_u32(N, ctx.upload(((argc + n + np.arange(N)) & 0xFFFFFFFF).astype(np.uint32)))
+    b = q.from_u32(N, ctx.upload(((argc + N - np.arange(N)) & 0xFFFFFFFF).astype(np.uint32)))
+    ka = q.ntt(log_n, 1, a)
+    kb = q.ntt(log_n, 1, b)
+    msgs = ctx.upload(((argc + n - np.arange(N * n, dtype=np.int64)) & 0xFFFFFFFF).astype(np.uint32))
+    p_mont = q.mont(P_BGV_EXAMPLE)
+    c0, c1 = q.alloc(n * N), q.alloc(n * N)
+    ctx.set_rng(ENC_KEY, [0, 0, 0])
+    ms, _, _ = time_steps(ctx, stream, lambda: q.bgv_encrypt(log_n, n, kb, ka, msgs, p_mont, c0=c0, c1=c1,
+                                                             stream=stream), reps, 1)
+    out["bgv_example"] = {"source": "examples/bgv.cpp:25-29", "q_bits": Q_BGV_EXAMPLE.bit_length(), "limbs": q.limbs,
+                          "N": N, "n": n, "ms": ms, "enc_per_s": n / (ms / 1e3),
+                          "fast_kernels": q.schedule(log_n)["fast"]}
+    del c0, c1, msgs
+    # tests/expr/crypto/lhe/enc.cpp at its size, through the plaintext path
+    qq, pp = hm.Modulus(ctx, Q_ENC_TEST), hm.Modulus(ctx, P_ENC_TEST)
+    log_n, n = 16, 8
+    N = 1 << log_n
+    qq.prepare(log_n)
+    pp.prepare(log_n)
+    iota_q = np.zeros((qq.limbs, N), np.uint32)
+    iota_q[0] = np.arange(N, dtype=np.uint32)
+    ka = qq.upload(iota_q)
+    one = np.zeros((qq.limbs, N), np.uint32)
+    for l_, wv in enumerate(qq.mont(1)):
+        one[l_] = wv
+    s = qq.upload(one)
+    xi = np.zeros((pp.limbs, n * N), np.uint32)
+    xi[0] = np.arange(n * N, dtype=np.uint32)
+    xd = pp.upload(xi)
+    c0, c1 = qq.alloc(n * N), qq.alloc(n * N)
+    ctx.set_rng(ENC_KEY, [0, 0, 0])
+    ms, _, _ = time_steps(ctx, stream, lambda: qq.bgv_encrypt_plaintext(pp, log_n, n, ka, ka, xd, True, c0=c0, c1=c1,
+                                                                        stream=stream), reps, 1)
+    dec = pp.alloc(n * N)
+    dms, _, _ = time_steps(ctx, stream, lambda: qq.bgv_decrypt_plaintext(pp, log_n, n, s, c0, c1, True, out=dec,
+                                                                         stream=stream), reps, 1)
+    ok = bool(np.array_equal(pp.download(dec, n * N), xi))
+    out["lhe_enc_test"] = {"source": "tests/expr/crypto/lhe/enc.cpp:15-72", "q_bits": Q_ENC_TEST.bit_length(),
+                           "q_limbs": qq.limbs, "p_limbs": pp.limbs, "N": N, "n": n, "enc_ms": ms,
+                           "enc_per_s": n / (ms / 1e3), "dec_ms": dms, "dec_ok": ok}
+    return out
 
 
 def bench_copy(hm, ctx, stream, nbytes=1 << 30, reps=5):
@@ -420,7 +534,8 @@ def cpu_baseline(pm, seconds_target=10.0):
     the same inputs (the first pairs of the batch)."""
     import oracle
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    # the GPU's share of the host: 16 threads per GPU, whatever the machine holds in total
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
     om = oracle.Modulus(pm["mod"].q)
     log_n = pm["log_n"]
     N = 1 << log_n
@@ -438,7 +553,39 @@ def cpu_baseline(pm, seconds_target=10.0):
     want = pm["out"].to_numpy((om.limbs, count))[:, : pairs * N]
     return {"value": 3 * pairs * N / secs / 1e9, "unit": "Gcoeff/s", "cores": threads, "kind": "port",
             "sample": f"{pairs} of {n_total} pairs (N=2^{log_n}, q128a), {secs:.2f} s wall",
-            "matches_gpu": bool(np.array_equal(got, want))}
+            "matches_gpu": bool(np.array_equal(got, want)), "cpu_model": cpu_model(),
+            "host_logical_cpus": os.cpu_count(), "cpus_available": len(os.sched_getaffinity(0))}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for row in f:
+                if row.startswith("model name"):
+                    return row.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def roofline(kname, kavg, launches, kbytes, achieved, traffic, valu, step_gbs):
+    """The bound of the dominant kernel. Its algorithmic HBM rate (bytes per launch / live average
+    duration) against 8 TB/s is always reported; when the committed counter summary holds the
+    kernel's VALU instruction count and that count needs a larger share of the duration at the
+    engine clock than the HBM bytes do, the kernel is VALU-issue bound and the line says so: the
+    achieved rate is then wave instructions per second against one wave instruction per SIMD
+    every 4 cycles, at the clock measured under the counters."""
+    hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
+    common = {"traffic": traffic, "kernel": kname, "kernel_avg_ms": kavg, "launches": launches,
+              "algorithmic_bytes_per_launch": kbytes, "step_hbm_gbs": step_gbs, "valu_issue": valu}
+    if valu is not None:
+        ghz = valu.get("clock_ghz_measured") or VALU_CLOCK_GHZ
+        frac = valu.get("frac_at_measured_clock", valu["frac"])
+        if frac > hbm["frac"]:
+            peak = valu["issue_rate_per_s"] / VALU_CLOCK_GHZ * ghz / 1e9
+            return {"bound": "valu", "achieved": valu["wave_instructions"] / (kavg / 1e3) / 1e9, "peak": peak,
+                    "unit": "Gwave-instructions/s", "frac": frac, "clock_ghz": ghz, "hbm": hbm, **common}
+    return {"bound": "hbm", **hbm, **common}
 
 
 def traffic_for(kernel, workload):
@@ -488,9 +635,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--only", default="", help="comma list of sections to run: c1,c2,c3,c4,c5,ext,copy (default all)")
+    ap.add_argument("--only", default="",
+                    help="comma list of sections to run: c1,c2,c3,c4,c5,ext,examples,copy (default all)")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests per load point of the serving sweep (the full workload uses 100000)")
+    ap.add_argument("--serving-operating-requests", type=int, default=100000,
+                    help="requests of the run at the serving operating point (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the workloads shard by independent batch elements); "
@@ -499,7 +649,7 @@ def main():
 
     import hmpc_amd as hm
 
-    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "c5", "ext", "copy"}
+    sections = set(args.only.split(",")) if args.only else {"c1", "c2", "c3", "c4", "c5", "ext", "examples", "copy"}
     ctx = hm.Context(0, key=KEY)
     stream = ctx.stream()
     info = ctx.info()
@@ -530,12 +680,9 @@ def main():
                        "schedule": pm["schedule"]},
             "plain_path": {"ms_per_step": pm["plain_ms"], "value": coeffs / (pm["plain_ms"] / 1e3) / 1e9,
                            "speedup_fused": pm["plain_ms"] / pm["ms"]},
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(kname, "c2"),
-                         "kernel": kname, "kernel_avg_ms": kavg, "launches": klaunch,
-                         "algorithmic_bytes_per_launch": kbytes,
-                         "step_hbm_gbs": 3 * unit_bytes / (pm["ms"] / 1e3) / 1e9,
-                         "valu_issue": valu_issue_for(kname, "c2", kavg, info["compute_units"])},
+            "roofline": roofline(kname, kavg, klaunch, kbytes, achieved, traffic_for(kname, "c2"),
+                                 valu_issue_for(kname, "c2", kavg, info["compute_units"]),
+                                 3 * unit_bytes / (pm["ms"] / 1e3) / 1e9),
             "kernels_ms": {k: v[1] for k, v in pm["kernels"].items()},
             "int_rate": {"modmul_per_s": modmuls / (pm["ms"] / 1e3), "modmuls_per_step": modmuls},
         })
@@ -545,16 +692,22 @@ def main():
                        "enc_per_s": bg["enc_per_s"], "ms_per_batch": bg["ms"],
                        "p50_ms": percentile(bg["per"], 50), "p99_ms": percentile(bg["per"], 99),
                        "plain_enc_per_s": bg["plain_enc_per_s"], "plain_sample": bg["plain_sample"],
-                       "batched_vs_plain": bg["enc_per_s"] / bg["plain_enc_per_s"], "decrypt_ok": bg["decrypt_ok"],
+                       "batched_vs_plain": bg["enc_per_s"] / bg["plain_enc_per_s"],
+                       "per_ciphertext_fused_enc_per_s": bg["per_ciphertext_fused_enc_per_s"],
+                       "batched_vs_per_ciphertext_fused": bg["enc_per_s"] / bg["per_ciphertext_fused_enc_per_s"],
+                       "decrypt_ok": bg["decrypt_ok"],
                        "schedule": bg["schedule"], "kernels_ms": {k: v[1] for k, v in bg["kernels"].items()}}
     if "c4" in sections:
         line["uneven_batch"] = bench_c4(hm, ctx, stream)
     if "c5" in sections:
-        line["serving"] = bench_c5(hm, ctx, stream, requests=args.serving_requests)
+        line["serving"] = bench_c5(hm, ctx, stream, requests=args.serving_requests,
+                                   operating_requests=args.serving_operating_requests)
     if "c1" in sections:
         line["c1_latency"] = bench_c1(hm, ctx, stream)
     if "ext" in sections:
         line["elementwise"] = bench_ext(hm, ctx, stream)
+    if "examples" in sections:
+        line["hmpc_examples"] = bench_examples(hm, ctx, stream)
     if "copy" in sections:
         line["copy_reference"] = bench_copy(hm, ctx, stream)
     if pm is not None and not args.no_cpu_baseline:
