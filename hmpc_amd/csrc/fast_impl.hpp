@@ -48,6 +48,20 @@ namespace hmpc_amd
             return t;
         }
 
+        // Global accesses of the policies: `base` points at element 0 of a tile (or of an operand's
+        // elements) in limb plane 0, planes `count` words apart, and `off` is the element's offset
+        // from it (below 2^28 for every tile geometry). The plane pointers are uniform, so the
+        // accesses address a scalar base plus a 32-bit vector byte offset shared by all limbs.
+        __device__ __forceinline__ u32 ld_off(u32 const* __restrict__ p, u32 off)
+        {
+            return *reinterpret_cast<u32 const*>(reinterpret_cast<char const*>(p) + (off << 2));
+        }
+
+        __device__ __forceinline__ void st_off(u32* __restrict__ p, u32 off, u32 v)
+        {
+            *reinterpret_cast<u32*>(reinterpret_cast<char*>(p) + (off << 2)) = v;
+        }
+
         // ------------------------------------------------------------------ arithmetic policies
         // r26: registers and LDS hold 5 limbs of 26 bits per element; global memory keeps hmpc's
         // 4 x 32-bit limb-planar layout (converted once on load and once on store per kernel).
@@ -62,13 +76,13 @@ namespace hmpc_amd
             using params = p26;
 
             __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
-                                                               std::size_t g)
+                                                               u32 g)
             {
-                r26_unpack(x, src[g], src[count + g], src[2 * count + g], src[3 * count + g]);
+                r26_unpack(x, ld_off(src, g), ld_off(src + count, g), ld_off(src + 2 * count, g), ld_off(src + 3 * count, g));
             }
 
             // canonical: final output; otherwise a lazy value < 2q for the next pass
-            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, std::size_t g,
+            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, u32 g,
                                                                 u32 const (&xin)[rl], bool canonical, params const& P)
             {
                 u32 x[rl] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
@@ -83,7 +97,7 @@ namespace hmpc_amd
                 }
 #pragma unroll
                 for (int l = 0; l < 4; ++l)
-                    dst[l * count + g] = w[l];
+                    st_off(dst + l * count, g, w[l]);
             }
 
             // private workspaces between passes hold the raw redundant limbs (no reduction, no
@@ -91,14 +105,14 @@ namespace hmpc_amd
             // forward column result is folded first only when the whole transform is longer
             // than the lazy forward bound allows (fold = true).
             __device__ __forceinline__ static void load_ws(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
-                                                           std::size_t g)
+                                                           u32 g)
             {
 #pragma unroll
                 for (int l = 0; l < rl; ++l)
-                    x[l] = src[l * count + g];
+                    x[l] = ld_off(src + l * count, g);
             }
 
-            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, std::size_t g,
+            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, u32 g,
                                                             u32 const (&xin)[rl], bool fold, params const& P)
             {
                 u32 x[rl] = {xin[0], xin[1], xin[2], xin[3], xin[4]};
@@ -106,26 +120,30 @@ namespace hmpc_amd
                     r26_fold_signed(x, P);
 #pragma unroll
                 for (int l = 0; l < rl; ++l)
-                    dst[l * count + g] = x[l];
+                    st_off(dst + l * count, g, x[l]);
             }
 
-            __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 4) & 15); }
+            // LDS slot of tile element e (an XOR swizzle: linear over GF(2), so the slot of e | d for
+            // disjoint bits is phys(e) ^ phys(d))
+            __device__ __forceinline__ static int phys(int e) { return swizzle_16(e); }
 
-            __device__ __forceinline__ static void lds_get(u32 (&x)[rl], u32 const* lds, int e)
+            __device__ __forceinline__ static void lds_get_at(u32 (&x)[rl], u32 const* lds, int slot)
             {
-                int p = phys(e) * rl;
+                int p = slot * rl;
 #pragma unroll
                 for (int l = 0; l < rl; ++l)
                     x[l] = lds[p + l];
             }
 
-            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl])
+            __device__ __forceinline__ static void lds_put_at(u32* lds, int slot, u32 const (&x)[rl])
             {
-                int p = phys(e) * rl;
+                int p = slot * rl;
 #pragma unroll
                 for (int l = 0; l < rl; ++l)
                     lds[p + l] = x[l];
             }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
 
             __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
             {
@@ -171,17 +189,11 @@ namespace hmpc_amd
                 r26_inverse_last(x, y, w0, w1, P);
             }
 
-            // point-wise product of two forward results a R, b R (signed digits) -> a b R (R' / R = 4
-            // correction)
+            // point-wise product of two forward results a R, b R (signed digits) -> a b R, only b
+            // folded (bounds with r26_point)
             __device__ __forceinline__ static void point(u32 (&a)[rl], u32 (&b)[rl], params const& P)
             {
-                r26_fold_signed(a, P);
-                r26_fold_signed(b, P);
-                u32 t[rl];
-                r26_mont(t, a, b, P);
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                    a[l] = t[l] << 2;
+                r26_point(a, b, P);
             }
 
             // x = k R + x for k canonical (limbs < 2^26) and u, x forward-transform outputs (signed
@@ -226,23 +238,9 @@ namespace hmpc_amd
                 tw_get_offset32(w, tw, idx);
             }
 
-            __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 3) & 63) ^ ((e >> 10) & 15); }
+            __device__ __forceinline__ static int phys(int e) { return swizzle_rows(e); }
 
-            __device__ __forceinline__ static void lds_get(u32 (&x)[rl], u32 const* lds, int e)
-            {
-                int p = phys(e) * rl;
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                    x[l] = lds[p + l];
-            }
-
-            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl])
-            {
-                int p = phys(e) * rl;
-#pragma unroll
-                for (int l = 0; l < rl; ++l)
-                    lds[p + l] = x[l];
-            }
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
         };
 
         // the BGV row kernels: twiddles from 32-bit offsets as in the product's row kernel
@@ -291,45 +289,47 @@ namespace hmpc_amd
             using ops = fast_ops<2, form_goldilocks>;
 
             __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
-                                                               std::size_t g)
+                                                               u32 g)
             {
-                x[0] = src[g];
-                x[1] = src[count + g];
+                x[0] = ld_off(src, g);
+                x[1] = ld_off(src + count, g);
             }
 
-            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, std::size_t g,
+            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, u32 g,
                                                                 u32 const (&x)[rl], bool, params const&)
             {
-                dst[g] = x[0];
-                dst[count + g] = x[1];
+                st_off(dst, g, x[0]);
+                st_off(dst + count, g, x[1]);
             }
 
             // values are canonical throughout: workspaces use the global format
             __device__ __forceinline__ static void load_ws(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
-                                                           std::size_t g)
+                                                           u32 g)
             {
                 load_global(x, src, count, g);
             }
 
-            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, std::size_t g,
+            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, u32 g,
                                                             u32 const (&x)[rl], bool, params const& P)
             {
                 store_global(dst, count, g, x, false, P);
             }
 
-            __device__ __forceinline__ static int phys(int e) { return e ^ ((e >> 4) & 15); }
+            __device__ __forceinline__ static int phys(int e) { return swizzle_16(e); }
 
-            __device__ __forceinline__ static void lds_get(u32 (&x)[rl], u32 const* lds, int e)
+            __device__ __forceinline__ static void lds_get_at(u32 (&x)[rl], u32 const* lds, int slot)
             {
-                uint2 v = reinterpret_cast<uint2 const*>(lds)[phys(e)];
+                uint2 v = reinterpret_cast<uint2 const*>(lds)[slot];
                 x[0] = v.x;
                 x[1] = v.y;
             }
 
-            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl])
+            __device__ __forceinline__ static void lds_put_at(u32* lds, int slot, u32 const (&x)[rl])
             {
-                reinterpret_cast<uint2*>(lds)[phys(e)] = make_uint2(x[0], x[1]);
+                reinterpret_cast<uint2*>(lds)[slot] = make_uint2(x[0], x[1]);
             }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
 
             __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
             {
@@ -424,6 +424,18 @@ namespace hmpc_amd
                 __syncthreads();
         }
 
+        // The elements of a unit are elem_of(unit, k = 0) | unit_offset(k) with disjoint bits, so
+        // their LDS slots are the unit's slot XOR a slot offset that depends on k and the round only
+        // (uniform over the workgroup, a compile-time constant for compile-time row lengths): the
+        // swizzle is evaluated once per unit instead of once per element. (Pinning the offset in a
+        // scalar register with readfirstlane cut the column kernels' VALU count further but ran
+        // their forward pass 3% slower.)
+        template<typename A>
+        __device__ __forceinline__ int slot_offset(int d)
+        {
+            return A::phys(d);
+        }
+
         template<typename A, int E, int W>
         __device__ __forceinline__ void gather(u32 (&x)[E][A::rl], u32 const* lds, tile_desc const& td, int a, bool inv)
         {
@@ -432,9 +444,10 @@ namespace hmpc_amd
             for (int g = 0; g < E / K; ++g)
             {
                 unit_t un = unit_of(td, a, W, inv, unit_index<A, E, W>(g));
+                int const slot = A::phys(elem_of(td, un.c, j_of(un, 0, a, W, td.s, inv)));
 #pragma unroll
                 for (int k = 0; k < K; ++k)
-                    A::lds_get(x[g * K + k], lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)));
+                    A::lds_get_at(x[g * K + k], lds, slot ^ slot_offset<A>(unit_offset(td, a, W, inv, k)));
             }
         }
 
@@ -446,9 +459,10 @@ namespace hmpc_amd
             for (int g = 0; g < E / K; ++g)
             {
                 unit_t un = unit_of(td, a, W, inv, unit_index<A, E, W>(g));
+                int const slot = A::phys(elem_of(td, un.c, j_of(un, 0, a, W, td.s, inv)));
 #pragma unroll
                 for (int k = 0; k < K; ++k)
-                    A::lds_put(lds, elem_of(td, un.c, j_of(un, k, a, W, td.s, inv)), x[g * K + k]);
+                    A::lds_put_at(lds, slot ^ slot_offset<A>(unit_offset(td, a, W, inv, k)), x[g * K + k]);
             }
         }
 
@@ -461,34 +475,41 @@ namespace hmpc_amd
             return inv ? ((1 << W) - (1 << (W - ii)) + (k >> (ii + 1))) : ((1 << ii) - 1 + (k >> (W - ii)));
         }
 
-        template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void prefetch_stage(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], tile_desc const& td,
-                                                       int a, u32 base0, u32 base_step, u32 const* __restrict__ tw)
+        // per unit g of the thread: the round's twiddle base (tw_round_base), from which every
+        // twiddle index of the round is one shift and a constant
+        template<typename A, int E, bool Inv, int W>
+        __device__ __forceinline__ void round_bases(u32 (&xr)[E >> W], tile_desc const& td, int a, u32 base0,
+                                                    u32 base_step)
         {
-            constexpr int K = 1 << W;
-            constexpr int count = Inv ? (1 << (W - II - 1)) : (1 << II);
-            constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
-            int const i = a + II;
-            u32 const h = 1u << (td.s - i - 1);
 #pragma unroll
-            for (int g = 0; g < E / K; ++g)
+            for (int g = 0; g < (E >> W); ++g)
             {
                 unit_t un = unit_of(td, a, W, Inv, unit_index<A, E, W>(g));
-                u32 base = base0 + un.c * base_step;
-#pragma unroll
-                for (int m = 0; m < count; ++m)
-                    A::tw_get(tw_regs[g][first + m], tw, tw_index<Inv, W, II>(base, h, i, un.F, m));
+                xr[g] = tw_round_base<Inv, W>(base0 + un.c * base_step, td.s, a, un.F);
             }
         }
 
         template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void prefetch_upto(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], tile_desc const& td,
-                                                      int a, u32 base0, u32 base_step, u32 const* __restrict__ tw)
+        __device__ __forceinline__ void prefetch_stage(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], u32 const (&xr)[E >> W],
+                                                       u32 const* __restrict__ tw)
+        {
+            constexpr int count = Inv ? (1 << (W - II - 1)) : (1 << II);
+            constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
+#pragma unroll
+            for (int g = 0; g < (E >> W); ++g)
+#pragma unroll
+                for (int m = 0; m < count; ++m)
+                    A::tw_get(tw_regs[g][first + m], tw, tw_index_round<Inv, W, II>(xr[g], m));
+        }
+
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void prefetch_upto(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], u32 const (&xr)[E >> W],
+                                                      u32 const* __restrict__ tw)
         {
             if constexpr (II < W - 1)
             {
-                prefetch_stage<A, E, Inv, W, II>(tw_regs, td, a, base0, base_step, tw);
-                prefetch_upto<A, E, Inv, W, II + 1>(tw_regs, td, a, base0, base_step, tw);
+                prefetch_stage<A, E, Inv, W, II>(tw_regs, xr, tw);
+                prefetch_upto<A, E, Inv, W, II + 1>(tw_regs, xr, tw);
             }
         }
 
@@ -536,15 +557,15 @@ namespace hmpc_amd
         // stages II .. W-2 of a round, each preceded by the loads of its own twiddles
         template<typename A, int E, bool Inv, int W, int II>
         __device__ __forceinline__ void stages_prefetched(u32 (&x)[E][A::rl], u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
-                                                          tile_desc const& td, int a, u32 base0, u32 base_step,
-                                                          u32 const* __restrict__ tw, typename A::params const& P, int g0)
+                                                          u32 const (&xr)[E >> W], u32 const* __restrict__ tw,
+                                                          typename A::params const& P, int g0)
         {
             if constexpr (II < W - 1)
             {
                 u32 const z[A::rl] = {};
-                prefetch_stage<A, E, Inv, W, II>(tw_regs, td, a, base0, base_step, tw);
+                prefetch_stage<A, E, Inv, W, II>(tw_regs, xr, tw);
                 stage<A, E, Inv, W, II, false>(x, tw_regs, z, z, P, g0);
-                stages_prefetched<A, E, Inv, W, II + 1>(x, tw_regs, td, a, base0, base_step, tw, P, g0);
+                stages_prefetched<A, E, Inv, W, II + 1>(x, tw_regs, xr, tw, P, g0);
             }
         }
 
@@ -561,11 +582,13 @@ namespace hmpc_amd
             bool const scaled = Inv && final_scale && (a + W == td.s);
             u32 w0[A::rl] = {}, w1[A::rl] = {};
             int const g0 = off + a;
+            u32 xr[E >> W];
+            round_bases<A, E, Inv, W>(xr, td, a, base0, base_step);
             if constexpr (A::stage_prefetch)
             {
                 // twiddles loaded stage by stage: at most one stage's twiddles live (fewer VGPRs,
                 // the latency is covered by the other workgroup on the CU)
-                stages_prefetched<A, E, Inv, W, 0>(x, tw_regs, td, a, base0, base_step, tw, P, g0);
+                stages_prefetched<A, E, Inv, W, 0>(x, tw_regs, xr, tw, P, g0);
                 if (scaled)
                 {
                     A::tw_get(w0, tw, 0);
@@ -574,19 +597,19 @@ namespace hmpc_amd
                 }
                 else
                 {
-                    prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, td, a, base0, base_step, tw);
+                    prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, xr, tw);
                     stage<A, E, Inv, W, W - 1, false>(x, tw_regs, w0, w1, P, g0);
                 }
                 return;
             }
-            prefetch_upto<A, E, Inv, W, 0>(tw_regs, td, a, base0, base_step, tw);
+            prefetch_upto<A, E, Inv, W, 0>(tw_regs, xr, tw);
             if (scaled)
             {
                 A::tw_get(w0, tw, 0);
                 A::tw_get(w1, tw, 1);
             }
             else
-                prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, td, a, base0, base_step, tw);
+                prefetch_stage<A, E, Inv, W, W - 1>(tw_regs, xr, tw);
             stages_upto<A, E, Inv, W, 0>(x, tw_regs, w0, w1, P, g0);
             if (scaled)
                 stage<A, E, Inv, W, W - 1, true>(x, tw_regs, w0, w1, P, g0);
@@ -687,19 +710,41 @@ namespace hmpc_amd
         template<typename A, int T, bool Ws = false, bool WL = false>
         __device__ __forceinline__ void tile_load(u32* lds, u32 const* __restrict__ src, io_desc const& io)
         {
+            constexpr int E = tile / T;
+            constexpr int step = WL ? 64 : T;
             int const t = tid_x();
-            int const e0 = WL ? (t & ~63) * (tile / T) + (t & 63) : t;
-            int const end = WL ? e0 + tile / T * 64 : tile;
-#pragma unroll 4
-            for (int e = e0; e < end; e += WL ? 64 : T)
+            int const e0 = WL ? (t & ~63) * E + (t & 63) : t;
+            // e = e0 + d with d a multiple of the step above e0's bits: slot phys(e0) ^ phys(d)
+            int const slot0 = A::phys(e0);
+            u32 const* const src_t = src + io.base;
+            // gidx_rel is additive over elements with disjoint bits, so element e0 + i step is at the
+            // thread's offset off0 from a base that moves by a uniform amount per i
+            u32 const off0 = gidx_rel(io, e0);
+            if (io.valid >= tile)
+            {
+                // a whole tile (every tile but the last of a batch of small polynomials): no checks
+#pragma unroll
+                for (int i = 0; i < E; ++i)
+                {
+                    u32 x[A::rl];
+                    if constexpr (Ws)
+                        A::load_ws(x, src_t + gidx_rel(io, i * step), io.count, off0);
+                    else
+                        A::load_global(x, src_t + gidx_rel(io, i * step), io.count, off0);
+                    A::lds_put_at(lds, slot0 ^ slot_offset<A>(i * step), x);
+                }
+                return;
+            }
+#pragma unroll
+            for (int i = 0; i < E; ++i)
             {
                 u32 x[A::rl];
-                if (e < io.valid)
+                if (e0 + i * step < io.valid)
                 {
                     if constexpr (Ws)
-                        A::load_ws(x, src, io.count, gidx(io, e));
+                        A::load_ws(x, src_t + gidx_rel(io, i * step), io.count, off0);
                     else
-                        A::load_global(x, src, io.count, gidx(io, e));
+                        A::load_global(x, src_t + gidx_rel(io, i * step), io.count, off0);
                 }
                 else
                 {
@@ -707,7 +752,7 @@ namespace hmpc_amd
                     for (int l = 0; l < A::rl; ++l)
                         x[l] = 0;
                 }
-                A::lds_put(lds, e, x);
+                A::lds_put_at(lds, slot0 ^ slot_offset<A>(i * step), x);
             }
         }
 
@@ -716,18 +761,38 @@ namespace hmpc_amd
         __device__ __forceinline__ void tile_store(u32 const* lds, u32* __restrict__ dst, io_desc const& io, bool canonical,
                                                    typename A::params const& P)
         {
+            constexpr int E = tile / T;
+            constexpr int step = WL ? 64 : T;
             int const t = tid_x();
-            int const e0 = WL ? (t & ~63) * (tile / T) + (t & 63) : t;
-            int const end = WL ? (e0 + tile / T * 64 < io.valid ? e0 + tile / T * 64 : io.valid) : io.valid;
-#pragma unroll 4
-            for (int e = e0; e < end; e += WL ? 64 : T)
+            int const e0 = WL ? (t & ~63) * E + (t & 63) : t;
+            int const slot0 = A::phys(e0);
+            u32* const dst_t = dst + io.base;
+            u32 const off0 = gidx_rel(io, e0);
+            if (io.valid >= tile)
             {
+#pragma unroll
+                for (int i = 0; i < E; ++i)
+                {
+                    u32 x[A::rl];
+                    A::lds_get_at(x, lds, slot0 ^ slot_offset<A>(i * step));
+                    if constexpr (Ws)
+                        A::store_ws(dst_t + gidx_rel(io, i * step), io.count, off0, x, canonical, P);
+                    else
+                        A::store_global(dst_t + gidx_rel(io, i * step), io.count, off0, x, canonical, P);
+                }
+                return;
+            }
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+            {
+                if (e0 + i * step >= io.valid)
+                    continue;
                 u32 x[A::rl];
-                A::lds_get(x, lds, e);
+                A::lds_get_at(x, lds, slot0 ^ slot_offset<A>(i * step));
                 if constexpr (Ws)
-                    A::store_ws(dst, io.count, gidx(io, e), x, canonical, P);
+                    A::store_ws(dst_t + gidx_rel(io, i * step), io.count, off0, x, canonical, P);
                 else
-                    A::store_global(dst, io.count, gidx(io, e), x, canonical, P);
+                    A::store_global(dst_t + gidx_rel(io, i * step), io.count, off0, x, canonical, P);
             }
         }
 
@@ -760,7 +825,7 @@ namespace hmpc_amd
                     if (e0 + k >= valid)
                         continue;
                     u32 v[A::rl];
-                    A::load_ws(v, src, src_count, src_base + e0 + k);
+                    A::load_ws(v, src + src_base, src_count, static_cast<u32>(e0 + k));
                     if constexpr (Mode == 0)
                     {
                         A::point(v, x[g * K + k], P);
@@ -771,7 +836,7 @@ namespace hmpc_amd
                     else
                     {
                         u32 kv[A::rl];
-                        A::load_global(kv, key, key_count, key_base + e0 + k);
+                        A::load_global(kv, key + key_base, key_count, static_cast<u32>(e0 + k));
                         A::point_add(x[g * K + k], v, kv, P);
                     }
                 }

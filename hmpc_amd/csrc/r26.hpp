@@ -9,7 +9,7 @@
 // Representation: x = sum_j x_j 2^(26 j), j = 0..4, limbs x_j held in 32 bits (redundant). The
 // transforms' values are signed digits: two's-complement limbs |x_j| < 2^31, so that their
 // butterflies (x + t, x - t) and (x + y, x - y) need no added multiple of q; point-wise products
-// fold their operands to non-negative limbs first. Internal Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in
+// fold one operand to non-negative limbs first. Internal Montgomery radix R' = 2^130: mont(a, b) = a b 2^-130 mod q. Data stays in
 // hmpc's representation a*R mod q (R = 2^128); twiddles are stored as w * 2^130 mod q so that
 // mont(a R, w 2^130) = (a w) R. A product of two data values is corrected by 4 = R' / R.
 //
@@ -18,7 +18,9 @@
 //               below 2^61 in magnitude; the result (T + M q) / 2^130 has digits r_0..r_3 in
 //               [0, 2^26) and a signed top limb, and lies in (-|a| q / 2^130, |a| q / 2^130 +
 //               q (1 + 2^-19)): within (-q, 2q (1 + 2^-19)) for |a| < 64 q, in [0, 2q (1 + 2^-19))
-//               with r_4 < 2^22 for a >= 0, v(a) v(b) < 2^130 q
+//               with r_4 < 2^22 for a >= 0, v(a) v(b) < 2^130 q. With b in [0, 2q) and limbs < 2^28
+//               (a folded value) and |a_j| < 2^30.2 the sums stay below 2^61 as well and the result
+//               lies in (-2|a| q / 2^130, 2|a| q / 2^130 + q (1 + 2^-19))
 //   forward butterfly (x + t, x - t), t = mont(y, w): |value| grows by < 2q (1 + 2^-18) per stage,
 //               limbs 0..3 by < 2^26 and limb 4 by < 2^22. From inputs in [0, q) with limbs
 //               (< 2^27, < 2^26, < 2^26, < 2^26, < 2^22) (canonical data, or from_signed
@@ -33,9 +35,11 @@
 //   inverse butterfly (x + y, mont(x - y, w)) on signed digits, x + y folded (signed) at odd
 //               stages of the transform: even-stage inputs are folded values or mont outputs
 //               (within (-q, 2q (1 + 2^-19)), |limbs| < 2^28), odd-stage inputs one unfolded sum of
-//               two (within 4q (1 + 2^-19), |limbs| < 2^29); from products (< 8q, limbs < 2^28
-//               after the x4 correction) stage 0 gives < 16q, and stage 1's sums stay within 32q
-//               with |limbs| < 2^30. Every mont input stays within 32q with |limbs| < 2^30.
+//               two (within 4q (1 + 2^-19), |limbs| < 2^29); from point-wise products (a forward
+//               result, |a| < 35.01q with |limbs| < 2^30.2 after at most 17 stages, times a folded
+//               one: within (-4.4q, 8.4q), digits 0..3 < 2^28 and |top limb| < 2^24 after the x4
+//               correction) stage 0 gives < 16.8q, and stage 1's sums stay within 33.6q with
+//               |limbs| < 2^30. Every mont input stays within 33.6q with |limbs| < 2^30.
 #pragma once
 
 #include "modarith.hpp"
@@ -255,6 +259,20 @@ namespace hmpc_amd
         }
         r26_mont_signed(x, s, w0, P);
         r26_mont_signed(y, d, w1, P);
+    }
+
+    // point-wise product of two forward-transform values a R, b R (signed digits) -> a b R: b is
+    // folded to [0, 2q), a stays the signed operand, and the Montgomery product is corrected by
+    // R' / R = 4. For |a| < 35.01 q with |a_j| < 2^30.2 (forward outputs of up to 17 stages) the
+    // result lies in (-4.4 q, 8.4 q) with digits 0..3 in [0, 2^28) and |r_4| < 2^24.
+    HMPC_HD void r26_point(u32 (&a)[5], u32 (&b)[5], p26 const& P)
+    {
+        r26_fold_signed(b, P);
+        u32 t[5];
+        r26_mont_signed(t, a, b, P);
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+            a[l] = t[l] << 2;
     }
 
     // signed integer z -> a value congruent to z R mod q (R = 2^128) with limbs

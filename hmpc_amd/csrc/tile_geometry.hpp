@@ -71,6 +71,17 @@ namespace hmpc_amd
             return td.rows ? static_cast<int>((c << td.s) + j) : static_cast<int>((j << td.log_seqs) + c);
         }
 
+        // LDS slot swizzles of the fast kernels' tiles (XOR of shifted bit fields: linear over GF(2))
+        HMPC_HD int swizzle_16(int e) { return e ^ ((e >> 4) & 15); }
+        HMPC_HD int swizzle_rows(int e) { return e ^ ((e >> 3) & 63) ^ ((e >> 10) & 15); }
+
+        // element k of a unit relative to its element 0: elem_of(un.c, j_of(un, k)) equals
+        // elem_of(un.c, j_of(un, 0)) | unit_offset(k), the two with disjoint bits
+        HMPC_HD int unit_offset(tile_desc const& td, int a, int w, bool inv, u32 k)
+        {
+            return elem_of(td, 0u, k << (inv ? a : td.s - a - w));
+        }
+
         // round schedule: forward full rounds first, inverse partial round first
         template<int WMAX>
         HMPC_HD int round_count(int s) { return (s + WMAX - 1) / WMAX; }
@@ -106,12 +117,16 @@ namespace hmpc_amd
             int valid;          // tile elements backed by real data (tail tiles)
         };
 
-        HMPC_HD std::size_t gidx(io_desc const& io, int e)
+        // offset of tile element e from the tile's element 0 (io.base): below 2^12 for row tiles,
+        // below 2^(12 - log_seqs) stride for column tiles (2^20 at N = 2^20)
+        HMPC_HD u32 gidx_rel(io_desc const& io, int e)
         {
             if (io.rows)
-                return io.base + e;
-            return io.base + static_cast<std::size_t>(e >> io.log_seqs) * io.stride + (e & ((1 << io.log_seqs) - 1));
+                return static_cast<u32>(e);
+            return static_cast<u32>(e >> io.log_seqs) * io.stride + static_cast<u32>(e & ((1 << io.log_seqs) - 1));
         }
+
+        HMPC_HD std::size_t gidx(io_desc const& io, int e) { return io.base + gidx_rel(io, e); }
 
         // The lazy forward transform runs up to 29 stages without reductions (r26.hpp); a
         // forward column result of a transform longer than 18 stages is folded before it goes to
@@ -252,6 +267,21 @@ namespace hmpc_amd
         HMPC_HD u32 tw_index(u32 base, u32 h, int i, u32 F, int m)
         {
             return Inv ? base * h + ((F << (W - II - 1)) | static_cast<u32>(m)) : (base << i) + ((F << II) | static_cast<u32>(m));
+        }
+
+        // The same indices from one value per unit and round: every twiddle of a round of width W
+        // starting at stage a is (X << shift(II)) + m, X = (base << a) + F forward and
+        // (base << (s - a - W)) + F inverse, shift(II) = II forward and W - II - 1 inverse.
+        template<bool Inv, int W>
+        HMPC_HD u32 tw_round_base(u32 base, int s, int a, u32 F)
+        {
+            return Inv ? (base << (s - a - W)) + F : (base << a) + F;
+        }
+
+        template<bool Inv, int W, int II>
+        HMPC_HD u32 tw_index_round(u32 X, int m)
+        {
+            return (X << (Inv ? W - II - 1 : II)) + static_cast<u32>(m);
         }
 
         // capability sampling into a tile: pair e of the tile's pairs draws coefficients k and

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 using namespace hmpc_amd;
 using namespace hmpc_amd::fast;
@@ -72,8 +73,13 @@ namespace
         int const T = tile / J.E;
         for (int t = 0; t < T; ++t)
             for (int e = t; e < tile; e += T)
+            {
+                require((t & (e - t)) == 0, "tile_load/tile_store element is not e0 | d");
+                require(gidx_rel(io, e) < (1u << 28), "tile element offset beyond 2^28");
+                require(gidx_rel(io, e) == gidx_rel(io, t) + gidx_rel(io, e - t), "tile offset not additive");
                 if (e < io.valid)
                     access(J, r, gidx(io, e), where);
+            }
     }
 
     template<int E, int W>
@@ -86,10 +92,13 @@ namespace
             {
                 int u = J.wave_units ? unit_index_at<E, W, true>(t, g) : unit_index_at<E, W, false>(t, g);
                 unit_t un = unit_of(td, a, W, inv, u);
+                int const el0 = elem_of(td, un.c, j_of(un, 0u, a, W, td.s, inv));
                 for (int k = 0; k < K; ++k)
                 {
                     int el = elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, td.s, inv));
                     require(el >= 0 && el < tile, "LDS element outside the tile");
+                    int d = unit_offset(td, a, W, inv, static_cast<u32>(k));
+                    require((el0 & d) == 0 && (el0 | d) == el, "unit element is not element 0 | unit_offset(k)");
                 }
                 u32 base = base0 + un.c * base_step;
                 for (int ii = 0; ii < W; ++ii)
@@ -110,6 +119,16 @@ namespace
                         default: if constexpr (W > 3) idx = inv ? tw_index<true, W, 3>(base, h, i, un.F, m) : tw_index<false, W, 3>(base, h, i, un.F, m); break;
                         }
                         require(idx < J.N, "twiddle index " + std::to_string(idx) + " >= N=2^" + std::to_string(J.log_n));
+                        u32 X = inv ? tw_round_base<true, W>(base, td.s, a, un.F) : tw_round_base<false, W>(base, td.s, a, un.F);
+                        u32 idx2 = 0;
+                        switch (ii)
+                        {
+                        case 0: idx2 = inv ? tw_index_round<true, W, 0>(X, m) : tw_index_round<false, W, 0>(X, m); break;
+                        case 1: if constexpr (W > 1) idx2 = inv ? tw_index_round<true, W, 1>(X, m) : tw_index_round<false, W, 1>(X, m); break;
+                        case 2: if constexpr (W > 2) idx2 = inv ? tw_index_round<true, W, 2>(X, m) : tw_index_round<false, W, 2>(X, m); break;
+                        default: if constexpr (W > 3) idx2 = inv ? tw_index_round<true, W, 3>(X, m) : tw_index_round<false, W, 3>(X, m); break;
+                        }
+                        require(idx2 == idx, "round-base twiddle index differs");
                     }
                 }
             }
@@ -355,8 +374,127 @@ namespace
     }
 }
 
+// The kernels address LDS as swizzle(e0) ^ swizzle(d) for elements e0 | d with disjoint bits:
+// every swizzle is checked linear on all such pairs of the tile and a permutation of the tile's
+// slots, and the unit decomposition (unit element k = element 0 | unit_offset(k)) is checked for
+// every tile geometry, round and unit order, and the wave-local tile_load / tile_store order.
+template<int W, int II>
+void tw_round_stage(bool inv, u32 base, int s, int a, u32 F)
+{
+    if constexpr (II < W)
+    {
+        int i = a + II;
+        u32 h = 1u << (s - i - 1);
+        int count = inv ? (1 << (W - II - 1)) : (1 << II);
+        u32 X = inv ? tw_round_base<true, W>(base, s, a, F) : tw_round_base<false, W>(base, s, a, F);
+        for (int m = 0; m < count; ++m)
+        {
+            u32 want = inv ? tw_index<true, W, II>(base, h, i, F, m) : tw_index<false, W, II>(base, h, i, F, m);
+            u32 got = inv ? tw_index_round<true, W, II>(X, m) : tw_index_round<false, W, II>(X, m);
+            require(got == want, "round-base twiddle index differs");
+        }
+        tw_round_stage<W, II + 1>(inv, base, s, a, F);
+    }
+}
+
+template<int W>
+void tw_round_check(bool inv, u32 base, int s, int a, u32 F)
+{
+    tw_round_stage<W, 0>(inv, base, s, a, F);
+}
+
+template<int E, int W>
+void unit_offsets_all()
+{
+    for (int s = W; s <= tile_log; ++s)
+        for (int rows = 0; rows < 2; ++rows)
+            for (int inv = 0; inv < 2; ++inv)
+                for (int a = 0; a + W <= s; ++a)
+                    for (int wave = 0; wave < 2; ++wave)
+                    {
+                        tile_desc td{s, tile_log - s, rows != 0};
+                        for (int t = 0; t < tile / E; ++t)
+                            for (int g = 0; g < E >> W; ++g)
+                            {
+                                int u = wave ? unit_index_at<E, W, true>(t, g) : unit_index_at<E, W, false>(t, g);
+                                unit_t un = unit_of(td, a, W, inv != 0, u);
+                                int el0 = elem_of(td, un.c, j_of(un, 0u, a, W, s, inv != 0));
+                                for (int k = 0; k < (1 << W); ++k)
+                                {
+                                    int el = elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, s, inv != 0));
+                                    int d = unit_offset(td, a, W, inv != 0, static_cast<u32>(k));
+                                    require((el0 & d) == 0 && (el0 | d) == el && el < tile,
+                                            "unit offset s=" + std::to_string(s) + " a=" + std::to_string(a) +
+                                                " W=" + std::to_string(W) + " E=" + std::to_string(E));
+                                }
+                                // twiddle indices: whole-polynomial / column base 1, row bases N1 + row
+                                for (u32 base : {1u, 128u + un.c, 256u + 3u * un.c})
+                                    tw_round_check<W>(inv != 0, base, s, a, un.F);
+                            }
+                    }
+}
+
+template<int E>
+void wave_local_io()
+{
+    int const T = tile / E;
+    for (int t = 0; t < T; ++t)
+    {
+        int e0 = (t & ~63) * E + (t & 63);
+        for (int e = e0; e < e0 + E * 64; e += 64)
+        {
+            require((e0 & (e - e0)) == 0 && e < tile, "wave-local tile_load element is not e0 | d");
+            // offsets of row tiles and of column tiles of every width
+            for (int ls = 0; ls <= 10; ++ls)
+            {
+                io_desc io{0, 0, 1u << 13, ls, ls == 0, tile};
+                require(gidx_rel(io, e) == gidx_rel(io, e0) + gidx_rel(io, e - e0), "wave-local tile offset not additive");
+            }
+        }
+    }
+}
+
+void swizzles()
+{
+    int (*const fns[2])(int) = {swizzle_16, swizzle_rows};
+    for (auto f : fns)
+    {
+        std::vector<char> seen(tile, 0);
+        for (int e = 0; e < tile; ++e)
+        {
+            int p = f(e);
+            require(p >= 0 && p < tile && !seen[p], "swizzle is not a permutation of the tile");
+            if (p >= 0 && p < tile)
+                seen[p] = 1;
+            // every d whose bits are disjoint from e
+            int const free = (tile - 1) & ~e;
+            for (int d = free;; d = (d - 1) & free)
+            {
+                if (f(e | d) != (f(e) ^ f(d)) && failures++ < 20)
+                    std::printf("FAILED: swizzle not linear at e=%d d=%d\n", e, d);
+                ++checks;
+                if (d == 0)
+                    break;
+            }
+        }
+    }
+}
+
 int main()
 {
+    swizzles();
+    unit_offsets_all<4, 1>();
+    unit_offsets_all<4, 2>();
+    unit_offsets_all<8, 1>();
+    unit_offsets_all<8, 2>();
+    unit_offsets_all<8, 3>();
+    unit_offsets_all<16, 1>();
+    unit_offsets_all<16, 2>();
+    unit_offsets_all<16, 3>();
+    unit_offsets_all<16, 4>();
+    wave_local_io<4>();
+    wave_local_io<8>();
+    wave_local_io<16>();
     for (int log_n = 2; log_n <= 17; ++log_n)
     {
         // 4 limbs (radix-2^26 policy): 4 or 8 elements per thread; 2 limbs (Goldilocks): 8 or 16

@@ -6,6 +6,7 @@
 //   G c x0..x4             -> r26_fold_signed limbs (x: signed 32-bit limbs)
 //   C c x0..x4             -> r26_canonical 32-bit words
 //   S c z                  -> r26_from_signed limbs (z a signed 64-bit integer)
+//   P c a0..a4 b0..b4      -> r26_point limbs (a, b: signed 32-bit limbs; the result's top limb signed)
 //   B c stages x0..x4 y0..y4 w0..w4 (per stage: w limbs)  -> forward butterfly chain (signed limbs)
 #include "../../hmpc_amd/csrc/r26.hpp"
 
@@ -42,6 +43,20 @@ int main()
             for (auto& v : b) std::cin >> v;
             r26_mont_signed(r, a, b, P);
             printf("%u %u %u %u %d\n", r[0], r[1], r[2], r[3], static_cast<int>(r[4]));
+        }
+        else if (cmd == "P")
+        {
+            u32 a[5], b[5];
+            for (auto* p : {&a, &b})
+                for (auto& v : *p)
+                {
+                    long long z;
+                    std::cin >> z;
+                    v = static_cast<u32>(z);
+                }
+            r26_point(a, b, P);
+            printf("%d %d %d %d %d\n", static_cast<int>(a[0]), static_cast<int>(a[1]), static_cast<int>(a[2]),
+                   static_cast<int>(a[3]), static_cast<int>(a[4]));
         }
         else if (cmd == "U")
         {

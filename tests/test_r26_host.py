@@ -194,20 +194,57 @@ def test_forward_chain_bounds(tool, q, stages):
             assert value(f) % q == value(half) % q and 0 <= value(f) < 2 * q and all(0 <= v < (1 << 28) for v in f)
 
 
+def _forward_output(q, rng, extreme):
+    """a forward-transform output after 17 stages: |value| < 35.01 q, |limbs| < 19 2^26 (limb 4:
+    < 2^22 + 17 2^22); extreme: magnitudes near those bounds"""
+    bound = q + (2 * q + (q >> 17)) * 17
+    v = (bound - 1 - rng.randrange(q >> 8)) * rng.choice((-1, 1)) if extreme else rng.randrange(-bound + 1, bound)
+    for _ in range(200):
+        l = signed_redundant(v, rng, 19 << 26)
+        if all(abs(l[j]) < LIMB_IN[j] + 17 * LIMB_STEP[j] for j in range(5)):
+            return l
+    return None
+
+
+def _product(tool, q, rng, extreme=False):
+    """a point-wise product of two forward outputs through r26_point, with its exact value checked"""
+    c = q - (1 << 124)
+    while True:
+        la, lb = _forward_output(q, rng, extreme), _forward_output(q, rng, extreme)
+        if la is not None and lb is not None:
+            break
+    r = run(tool, f"P {c} " + " ".join(map(str, la + lb)))
+    assert value(r) % q == value(la) * value(lb) * pow(1 << 128, -1, q) % q
+    return r
+
+
+@pytest.mark.parametrize("q", PRIMES)
+def test_point_product_bounds(tool, q):
+    """r26_point on forward outputs of up to 17 stages (b folded, a the signed operand): exact
+    a b R^-1 mod q, within (-4.4q, 8.4q), digits 0..3 in [0, 2^28) and a signed top limb below
+    2^24 in magnitude; operands near the magnitude bounds included."""
+    rng = random.Random(q & 0xFFF)
+    for it in range(400):
+        r = _product(tool, q, rng, extreme=it % 2 == 0)
+        assert -44 * q < 10 * value(r) < 84 * q
+        assert all(0 <= v < (1 << 28) for v in r[:4]) and abs(r[4]) < (1 << 24)
+
+
 @pytest.mark.parametrize("q", PRIMES)
 @pytest.mark.parametrize("stages", [12, 16])
 def test_inverse_chain_bounds(tool, q, stages):
-    """Lazy inverse stages on signed digits, x + y folded at odd stages only, from product-shaped
-    inputs (< 8q, limbs < 2^28), each stage keeping either output pair or a doubled output: every
-    mont input x - y stays within 32q with limbs below 2^30, odd-stage inputs within 4q (1 + 2^-18)
-    (32q after stage 0) with limbs below 2^30, the folded outputs in [0, 2q) with limbs < 2^28, the
-    mont outputs within (-q, 2q (1 + 2^-19)), and the results are exact mod q."""
+    """Lazy inverse stages on signed digits, x + y folded at odd stages only, from point-wise
+    products (r26_point of forward outputs: within (-4.4q, 8.4q), digits < 2^28), each stage keeping
+    either output pair or a doubled output: every mont input x - y stays within 33.6q with limbs
+    below 2^30, odd-stage inputs within 4q (1 + 2^-18) (16.8q after stage 0) with limbs below 2^30,
+    the folded outputs in [0, 2q) with limbs < 2^28, the mont outputs within (-q, 2q (1 + 2^-19)),
+    and the results are exact mod q."""
     c = q - (1 << 124)
     rng = random.Random(11 + stages)
     R = 1 << 130
     for it in range(150):
-        x, y = rng.randrange(8 * q), rng.randrange(8 * q)
-        lx, ly = redundant(x, rng, 28), redundant(y, rng, 28)
+        lx, ly = _product(tool, q, rng, extreme=it % 2 == 0), _product(tool, q, rng, extreme=it % 3 == 0)
+        x, y = value(lx), value(ly)
         ws = [rng.randrange(q) for _ in range(stages)]
         sels = [1 if it % 3 == 0 else rng.randrange(3) for _ in range(stages)]
         folds = [s & 1 for s in range(stages)]
@@ -219,9 +256,9 @@ def test_inverse_chain_bounds(tool, q, stages):
         px, py = lx, ly
         for s, (w, f, sel) in enumerate(zip(ws, folds, sels)):
             d = [a - b for a, b in zip(px, py)]
-            assert abs(value(d)) < 32 * q and max(abs(v) for v in d) < (1 << 30), f"mont input at stage {s}"
+            assert 10 * abs(value(d)) < 336 * q and max(abs(v) for v in d) < (1 << 30), f"mont input at stage {s}"
             if f:  # odd stage: inputs are one unfolded sum (or the stage-0 sum of products)
-                bound = 16 * q if s == 1 else 4 * q + (q >> 17)
+                bound = 168 * q // 10 if s == 1 else 4 * q + (q >> 17)
                 assert abs(value(px)) < bound and abs(value(py)) < bound and max(abs(v) for v in px + py) < (1 << 30)
             ex, ey = (ex + ey) % q, (ex - ey) * w * pow(R, -1, q) % q
             if sel == 1:
