@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libhmpc_hip.so")
 # that the slowest units compile in parallel
 SOURCES = ["kernels_jobs_l2.hip", "kernels_jobs_l2_e16.hip", "kernels_jobs_l2_bgv.hip", "kernels_jobs_l2_mul.hip", "kernels_jobs_l4.hip",
            "kernels_jobs_l4_bgv.hip", "kernels_jobs_l4_mul.hip", "kernels.hip", "kernels_fast.hip", "kernels_fast_l2.hip",
-           "kernels_fast_l4.hip", "kernels_ext.hip", "kernels_lhe.hip", "api.hip"]
+           "kernels_fast_l4.hip", "kernels_fast_c4.hip", "kernels_fast_c5.hip", "kernels_ext.hip", "kernels_lhe.hip", "api.hip"]
 ARCH = os.environ.get("HMPC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

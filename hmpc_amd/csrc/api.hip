@@ -327,7 +327,7 @@ struct hmpc_modulus_s
     // persistent job kernel: 4 at 4 limbs (it holds one workgroup per CU). Measured on MI355X over
     // N = 2^12..2^17 with 2^25 coefficients per call: products at 4 limbs take 2.42-2.95 ms at 8
     // against 2.46-3.34 ms at 4 per pass, and 2.65-3.71 ms on the persistent kernel.
-    int default_elems(int) const { return limbs == 4 ? 8 : 16; }
+    int default_elems(int) const { return limbs >= 4 ? 8 : 16; }
     int jobs_default_elems() const { return limbs == 4 ? 4 : 16; }
     // elements per thread for one call of n transforms: the tuned schedule, else the default,
     // except that a Goldilocks batch too small to give every CU a tile runs more threads per tile
@@ -418,7 +418,11 @@ struct hmpc_modulus_s
         }
     }
 
+    // fast(): the special-form moduli, served by every fast kernel (job kernels, BGV, serving);
+    // fast_transform(): transforms and products on the fast tile kernels, which also take any
+    // 4- or 5-limb modulus with 4q < R
     bool fast(int log_n) const { return variant == 0 && fast_path_supported(limbs, form, log_n); }
+    bool fast_transform(int log_n) const { return variant == 0 && fast_transform_supported(limbs, form, log_n); }
 
     int prepare(int log_n);
 
@@ -452,7 +456,7 @@ namespace
     hipError_t dispatch_ntt(hmpc_modulus_s const& m, transform_args const& T, std::size_t n, bool inverse, u32 const* in,
                             u32* out, hipStream_t s)
     {
-        if (m.fast(T.log_n))
+        if (m.fast_transform(T.log_n))
             return launch_fast_ntt(m.limbs, T, n, inverse, in, out, s);
         return launch_ntt_fused(m.limbs, T, n, inverse, in, out, s);
     }
@@ -974,10 +978,12 @@ int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread,
 {
     if (!mod || log_n < 2 || log_n > 20)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
-    bool ok = elems_per_thread == 0 || (mod->limbs == 4 && (elems_per_thread == 4 || elems_per_thread == 8)) ||
+    bool ok = elems_per_thread == 0 ||
+              ((mod->limbs == 4 || mod->limbs == 5) && (elems_per_thread == 4 || elems_per_thread == 8)) ||
               (mod->limbs == 2 && (elems_per_thread == 8 || elems_per_thread == 16));
     if (!ok)
-        return fail(HMPC_ERR_INVALID_ARGUMENT, "elements per thread: 4 or 8 (4 limbs), 8 or 16 (2 limbs), 0 = default");
+        return fail(HMPC_ERR_INVALID_ARGUMENT,
+                    "elements per thread: 4 or 8 (4 and 5 limbs), 8 or 16 (2 limbs), 0 = default");
     if (persistent < -2 || persistent > 1)
         return fail(HMPC_ERR_INVALID_ARGUMENT,
                     "persistent: 1 (job kernel), 0 (one launch per pass), -1 (unchanged), -2 (default)");
@@ -996,10 +1002,12 @@ int hmpc_modulus_set_bgv_schedule(hmpc_modulus mod, int log_n, int elems_per_thr
 {
     if (!mod || log_n < 2 || log_n > 20)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "log_n must be in [2, 20]");
-    bool ok = elems_per_thread == 0 || (mod->limbs == 4 && (elems_per_thread == 4 || elems_per_thread == 8)) ||
+    bool ok = elems_per_thread == 0 ||
+              ((mod->limbs == 4 || mod->limbs == 5) && (elems_per_thread == 4 || elems_per_thread == 8)) ||
               (mod->limbs == 2 && (elems_per_thread == 8 || elems_per_thread == 16));
     if (!ok)
-        return fail(HMPC_ERR_INVALID_ARGUMENT, "elements per thread: 4 or 8 (4 limbs), 8 or 16 (2 limbs), 0 = default");
+        return fail(HMPC_ERR_INVALID_ARGUMENT,
+                    "elements per thread: 4 or 8 (4 and 5 limbs), 8 or 16 (2 limbs), 0 = default");
     if (tiles < -1 || tiles > 1)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "tiles: 1 (two launches over tiles), 0 (persistent jobs), -1 (default)");
     if (elems_per_thread == 0)
@@ -1023,7 +1031,7 @@ int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread
     if (persistent)
         *persistent = mod->persistent(log_n) ? 1 : 0;
     if (fast)
-        *fast = mod->fast(log_n) ? 1 : 0;
+        *fast = mod->fast_transform(log_n) ? 1 : 0;
     return HMPC_OK;
 }
 
@@ -1613,8 +1621,8 @@ int hmpc_polymul(hmpc_modulus mod, int log_n, size_t n, const uint32_t* a, const
     probe_guard pg(mod->ctx);
     hipStream_t s = as_stream(stream);
     void* w = nullptr;
-    bool fast = mod->fast(log_n);
-    if (fast && mod->persistent(log_n))
+    bool fast = mod->fast_transform(log_n);
+    if (mod->fast(log_n) && mod->persistent(log_n))
     {
         auto e = mod->elems.find(log_n);
         T.elems = e != mod->elems.end() ? e->second : mod->jobs_default_elems();
@@ -1650,7 +1658,7 @@ int hmpc_polymul_noalloc(hmpc_modulus mod, int log_n, size_t n, const uint32_t* 
         return HMPC_OK;
     if (n > 0x7fffffffu)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
-    bool fast = mod->fast(log_n);
+    bool fast = mod->fast_transform(log_n);
     std::size_t words = fast ? fast_polymul_workspace_words(mod->limbs, log_n, n) : polymul_workspace_words(mod->limbs, log_n, n);
     if (words * 4 > noalloc_scratch_bytes)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "hmpc_polymul_noalloc: the workspace of " + std::to_string(words * 4) +

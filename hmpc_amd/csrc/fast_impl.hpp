@@ -390,6 +390,143 @@ namespace hmpc_amd
             }
         };
 
+        // Any other modulus with 4q < R (R = 2^(32 L)) at 4 or 5 limbs, e.g. the 125-bit prime of
+        // hmpc's examples/ntt.cpp and the 150-bit prime of examples/bgv.cpp: CIOS Montgomery
+        // products on 32-bit limbs (fastmath.hpp, the lazy form) with Harvey butterflies, values in
+        // [0, 4q) through forward transforms and [0, 2q) through inverse ones. Registers, LDS,
+        // workspaces and global memory all hold hmpc's L x 32-bit limbs; twiddles come from the
+        // element-major tables ([N][L] words).
+        template<int L>
+        struct arith_cios
+        {
+            static constexpr int rl = L;
+            static constexpr int iol = L;
+            static constexpr int wsl = L;
+            static constexpr int tws = L;
+            static constexpr bool stage_prefetch = true;
+            static constexpr bool wave_units = false;
+            using params = fast_params;
+            using ops = fast_ops<L, form_lazy>;
+
+            __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
+                                                               u32 g)
+            {
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    x[l] = ld_off(src + l * count, g);
+            }
+
+            // canonical: [0, 4q) -> [0, q); otherwise the lazy value itself (the next pass accepts
+            // [0, 4q) forward and [0, 2q) inverse, which is what each pass leaves)
+            __device__ __forceinline__ static void store_global(u32* __restrict__ dst, std::size_t count, u32 g,
+                                                                u32 const (&xin)[rl], bool canonical, params const& P)
+            {
+                u32 x[rl];
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    x[l] = xin[l];
+                if (canonical)
+                    ops::canonical(x, P);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    st_off(dst + l * count, g, x[l]);
+            }
+
+            __device__ __forceinline__ static void load_ws(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
+                                                           u32 g)
+            {
+                load_global(x, src, count, g);
+            }
+
+            __device__ __forceinline__ static void store_ws(u32* __restrict__ dst, std::size_t count, u32 g,
+                                                            u32 const (&x)[rl], bool, params const& P)
+            {
+                store_global(dst, count, g, x, false, P);
+            }
+
+            __device__ __forceinline__ static int phys(int e) { return swizzle_16(e); }
+
+            __device__ __forceinline__ static void lds_get_at(u32 (&x)[rl], u32 const* lds, int slot)
+            {
+                if constexpr (L == 4)
+                {
+                    uint4 v = reinterpret_cast<uint4 const*>(lds)[slot];
+                    x[0] = v.x;
+                    x[1] = v.y;
+                    x[2] = v.z;
+                    x[3] = v.w;
+                }
+                else
+                {
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        x[l] = lds[slot * L + l];
+                }
+            }
+
+            __device__ __forceinline__ static void lds_put_at(u32* lds, int slot, u32 const (&x)[rl])
+            {
+                if constexpr (L == 4)
+                    reinterpret_cast<uint4*>(lds)[slot] = make_uint4(x[0], x[1], x[2], x[3]);
+                else
+                {
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        lds[slot * L + l] = x[l];
+                }
+            }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
+
+            __device__ __forceinline__ static void tw_get(u32 (&w)[rl], u32 const* __restrict__ tw, u32 idx)
+            {
+                if constexpr (L == 4)
+                {
+                    uint4 v = reinterpret_cast<uint4 const*>(tw)[idx];
+                    w[0] = v.x;
+                    w[1] = v.y;
+                    w[2] = v.z;
+                    w[3] = v.w;
+                }
+                else
+                {
+#pragma unroll
+                    for (int l = 0; l < L; ++l)
+                        w[l] = tw[static_cast<std::size_t>(idx) * L + l];
+                }
+            }
+
+            __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P)
+            {
+                ops::fwd(x, y, w, P);
+            }
+
+            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[rl], params const& P,
+                                                       bool)
+            {
+                ops::inv(x, y, w, P);
+            }
+
+            __device__ __forceinline__ static void inv_last(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w0)[rl],
+                                                            u32 const (&w1)[rl], params const& P)
+            {
+                ops::inv_last(x, y, w0, w1, P);
+            }
+
+            // a b R from forward results a R, b R in [0, 4q): both halved to [0, 2q), so that
+            // a b < 4 q^2 and the lazy product lies in [0, 2q), the inverse transform's input range
+            __device__ __forceinline__ static void point(u32 (&a)[rl], u32 (&b)[rl], params const& P)
+            {
+                ops::half_reduce(a, P);
+                ops::half_reduce(b, P);
+                u32 t[rl];
+                ops::mul(t, a, b, P);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    a[l] = t[l];
+            }
+        };
+
         template<int E>
         struct shape
         {
@@ -1026,10 +1163,24 @@ namespace hmpc_amd
         }
 
         // ------------------------------------------------------------------ host launchers
+        // kernel-argument constants of the 32-bit-limb Montgomery policies (Goldilocks, CIOS)
+        inline fast_params montgomery_params(transform_args const& T)
+        {
+            fast_params P{};
+            for (int l = 0; l < max_limbs; ++l)
+            {
+                P.q[l] = T.P.q[l];
+                P.q2[l] = T.q2[l];
+                P.r2[l] = T.r2[l];
+            }
+            P.nprime = T.P.nprime;
+            return P;
+        }
+
         template<typename A, typename F>
         hipError_t with_elems(int elems, F&& f)
         {
-            if constexpr (A::rl == 5)
+            if constexpr (A::rl >= 4)
             {
                 if (elems == 4)
                     return f(std::integral_constant<int, 4>{});

@@ -28,20 +28,6 @@ namespace hmpc_amd
 {
     namespace fast
     {
-        // kernel-argument constants of the Goldilocks policy
-        inline fast_params gold_params(transform_args const& T)
-        {
-            fast_params P{};
-            for (int l = 0; l < max_limbs; ++l)
-            {
-                P.q[l] = T.P.q[l];
-                P.q2[l] = T.q2[l];
-                P.r2[l] = T.r2[l];
-            }
-            P.nprime = T.P.nprime;
-            return P;
-        }
-
         template<typename A, int E>
         __device__ __forceinline__ void scatter_last(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td)
         {

@@ -74,6 +74,9 @@ namespace hmpc_amd
 
     // fast path (L = 2, 4): register-resident radix-16 rounds, see kernels_fast.hip
     bool fast_path_supported(int L, int form, int log_n);
+    // the fast tile kernels of transforms and products: the special-form moduli above, and any
+    // modulus of 4 or 5 limbs with 4q < R (the CIOS policy)
+    bool fast_transform_supported(int L, int form, int log_n);
     hipError_t launch_fast_ntt(int L, transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
                                hipStream_t s);
     std::size_t fast_polymul_workspace_words(int L, int log_n, std::size_t n);

@@ -14,6 +14,14 @@ namespace hmpc_amd
                                       u32* workspace, hipStream_t s);
     hipError_t launch_fast_polymul_l4(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                       u32* workspace, hipStream_t s);
+    hipError_t launch_fast_ntt_c4(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                                  hipStream_t s);
+    hipError_t launch_fast_ntt_c5(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
+                                  hipStream_t s);
+    hipError_t launch_fast_polymul_c4(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                      u32* workspace, hipStream_t s);
+    hipError_t launch_fast_polymul_c5(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
+                                      u32* workspace, hipStream_t s);
 
     hipError_t fast_jobs_occupancy_l2(int elems, int* blocks_per_cu);
     hipError_t fast_jobs_occupancy_l4(int elems, int* blocks_per_cu);
@@ -45,13 +53,25 @@ namespace hmpc_amd
         return special && log_n >= 2 && log_n <= 20;
     }
 
+    namespace
+    {
+        bool cios_form(int L, int form) { return (L == 4 || L == 5) && (form == form_lazy || form == form_sparse); }
+    }
+
+    bool fast_transform_supported(int L, int form, int log_n)
+    {
+        return fast_path_supported(L, form, log_n) || (cios_form(L, form) && log_n >= 2 && log_n <= 20);
+    }
+
     hipError_t launch_fast_ntt(int L, transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
                                hipStream_t s)
     {
-        if (L == 4)
+        if (L == 4 && T.form == form_r26)
             return launch_fast_ntt_l4(T, n, inverse, in, out, s);
-        if (L == 2)
+        if (L == 2 && T.form == form_goldilocks)
             return launch_fast_ntt_l2(T, n, inverse, in, out, s);
+        if (cios_form(L, T.form))
+            return L == 4 ? launch_fast_ntt_c4(T, n, inverse, in, out, s) : launch_fast_ntt_c5(T, n, inverse, in, out, s);
         return hipErrorInvalidValue;
     }
 
@@ -63,10 +83,13 @@ namespace hmpc_amd
     hipError_t launch_fast_polymul(int L, transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                    u32* workspace, hipStream_t s)
     {
-        if (L == 4)
+        if (L == 4 && T.form == form_r26)
             return launch_fast_polymul_l4(T, n, a, b, out, workspace, s);
-        if (L == 2)
+        if (L == 2 && T.form == form_goldilocks)
             return launch_fast_polymul_l2(T, n, a, b, out, workspace, s);
+        if (cios_form(L, T.form))
+            return L == 4 ? launch_fast_polymul_c4(T, n, a, b, out, workspace, s)
+                          : launch_fast_polymul_c5(T, n, a, b, out, workspace, s);
         return hipErrorInvalidValue;
     }
 

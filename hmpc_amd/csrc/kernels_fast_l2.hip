@@ -3,31 +3,15 @@
 
 namespace hmpc_amd
 {
-    namespace
-    {
-        fast_params gold_params(transform_args const& T)
-        {
-            fast_params P{};
-            for (int l = 0; l < max_limbs; ++l)
-            {
-                P.q[l] = T.P.q[l];
-                P.q2[l] = T.q2[l];
-                P.r2[l] = T.r2[l];
-            }
-            P.nprime = T.P.nprime;
-            return P;
-        }
-    }
-
     hipError_t launch_fast_ntt_l2(transform_args const& T, std::size_t n, bool inverse, u32 const* in, u32* out,
                                   hipStream_t s)
     {
-        return fast::launch_ntt<fast::arith_gold>(T, gold_params(T), n, inverse, in, out, s);
+        return fast::launch_ntt<fast::arith_gold>(T, fast::montgomery_params(T), n, inverse, in, out, s);
     }
 
     hipError_t launch_fast_polymul_l2(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                       u32* workspace, hipStream_t s)
     {
-        return fast::launch_polymul<fast::arith_gold>(T, gold_params(T), n, a, b, out, workspace, s);
+        return fast::launch_polymul<fast::arith_gold>(T, fast::montgomery_params(T), n, a, b, out, workspace, s);
     }
 }

@@ -21,7 +21,7 @@ namespace hmpc_amd
     {
         if (T.elems == 8)
             return fast::launch_jobs_e<fast::arith_gold, 8>(table, base, njobs, grid, counter, scratch, slot_words,
-                                                            fast::gold_params(T), s);
-        return launch_fast_jobs_l2_e16(table, base, njobs, grid, counter, scratch, slot_words, fast::gold_params(T), s);
+                                                            fast::montgomery_params(T), s);
+        return launch_fast_jobs_l2_e16(table, base, njobs, grid, counter, scratch, slot_words, fast::montgomery_params(T), s);
     }
 }

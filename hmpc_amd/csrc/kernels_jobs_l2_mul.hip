@@ -8,6 +8,6 @@ namespace hmpc_amd
                                              u32 icols_blocks, u32* ws, hipStream_t s)
     {
         return fast::launch_polymul_table<fast::arith_gold>(T.elems, uniform_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, map_icols,
-                                                        icols_blocks, ws, fast::gold_params(T), s);
+                                                        icols_blocks, ws, fast::montgomery_params(T), s);
     }
 }

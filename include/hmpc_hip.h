@@ -100,12 +100,16 @@ int hmpc_modulus_create(hmpc_context ctx, const uint32_t* q, int q_words, hmpc_m
 int hmpc_modulus_destroy(hmpc_modulus mod);
 /* limbs L, bit size of q - 1, and constants (each array L words, may be NULL) */
 int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* nprime, uint32_t* r_mod_q, uint32_t* r2_mod_q);
-/* Kernel family for this modulus: 0 (default) = register-resident radix-16 rounds where
- * supported (L = 2 or 4; Goldilocks / sparse / lazy reduction chosen from q), else the
- * portable LDS radix-2 kernels; 1 = always the portable kernels. Results are identical. */
+/* Kernel family for this modulus: 0 (default) = the fast tile kernels (register-resident rounds
+ * between LDS exchanges) where supported, else the portable LDS radix-2 kernels; 1 = always the
+ * portable kernels. Results are identical. The fast kernels serve every operation for
+ * q = 2^124 + c (c < 2^24, radix-2^26 arithmetic) and Goldilocks 2^64 - 2^32 + 1, and transforms
+ * and products (hmpc_ntt, hmpc_polymul) for any 4- or 5-limb q with 4q < 2^(32 L) (CIOS Montgomery
+ * products, lazy butterflies), e.g. the 125-bit prime of hmpc's examples/ntt.cpp and the 150-bit
+ * prime of examples/bgv.cpp. */
 int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
-/* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 limbs, 8 or
- * 16 for 2 limbs; 0 restores the default) and launch structure (persistent = 1: one persistent
+/* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 and 5 limbs, 8
+ * or 16 for 2 limbs; 0 restores the default) and launch structure (persistent = 1: one persistent
  * job kernel in which each workgroup runs whole products / encryptions; 0: one launch per
  * transform pass; -1: unchanged; -2: the default, which is one launch per pass for uniform products
  * and the tile launches in job batches and serving; the default elements per thread are 8 at 4 limbs
@@ -119,7 +123,7 @@ int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread,
  * persistent job kernel, one workgroup per ciphertext) or -1 (default: tiles below 256
  * ciphertexts). Chosen by hmpc_autotune_bgv; results are identical. */
 int hmpc_modulus_set_bgv_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int tiles);
-/* Current schedule and whether the fast kernels serve this (modulus, N) at all. */
+/* Current schedule and whether the fast kernels serve this (modulus, N)'s transforms and products. */
 int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* persistent, int* fast);
 /* Autotuner: times batched products of `pairs` synthetic polynomial pairs of N = 2^log_n under
  * every schedule of the fast kernels, installs the fastest with hmpc_modulus_set_schedule and

@@ -1,7 +1,12 @@
-"""CPU check of the persistent job kernel's indexing (tests/cpp/jobs_index_check.cpp): every operand,
-key, message, twiddle, scratch and LDS index of both job types stays within the job's polynomial, its
-scratch slot and the tile, for N = 2^2..2^17, 2 and 4 limbs and every elements-per-thread choice,
-with the kernels' own geometry code (hmpc_amd/csrc/tile_geometry.hpp) compiled for the host."""
+"""CPU check of the fast kernels' indexing (tests/cpp/jobs_index_check.cpp), with the kernels' own
+geometry code (hmpc_amd/csrc/tile_geometry.hpp) compiled for the host:
+- the persistent job kernel: every operand, key, message, twiddle, scratch and LDS index of both
+  job types stays within the job's polynomial, its scratch slot and the tile, for N = 2^2..2^17, 2
+  and 4 limbs and every elements-per-thread choice;
+- the address decompositions every tile kernel relies on: the LDS swizzles are permutations and
+  linear over disjoint bits, each unit's elements are element 0 | unit_offset(k), the per-round
+  twiddle base gives the per-stage indices, and tile offsets are additive, for every tile geometry,
+  round, unit order and elements-per-thread choice."""
 
 import os
 import subprocess
