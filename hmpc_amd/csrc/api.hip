@@ -1494,6 +1494,8 @@ namespace
         // into the map: forward column pass, row pass, inverse column pass (products)
         std::size_t cols_off = 0, cols_blocks = 0, rows_off = 0, rows_blocks = 0, icols_off = 0, icols_blocks = 0;
         u64 ws_words = 0;
+        u64 ws_base = 0;  // the chunk's workspace offset (words): 0 with a shared workspace
+        u64 cost = 0;     // sum of N log N over its jobs (scheduling weight)
     };
 
     struct tile_plan
@@ -1508,10 +1510,11 @@ namespace
         // in: (modulus, elements per thread, job); order is kept within a group. by_degree: a group
         // (and so each chunk) holds one degree, and its launches run the kernels with the pass
         // lengths fixed at compile time. That pays where launches on several streams fill each
-        // other's tails (the server's slots: peak 7,300 -> 8,200 requests/s); a batch on one stream
-        // keeps degrees mixed, so that each pass is one launch (its uneven batch: 3.6 against
-        // 3.8 ms split by degree).
-        void build(std::vector<tile_input> const& in, bool by_degree)
+        // other's tails (the server's slots: peak 7,300 -> 8,200 requests/s); a batch keeps
+        // degrees mixed, so that each pass is one launch (its uneven batch: 3.6 against 3.8 ms
+        // split by degree). separate: every chunk gets a workspace of its own (ws_words is then the
+        // sum, not the largest), so that chunks may run concurrently.
+        void build(std::vector<tile_input> const& in, bool by_degree, bool separate = false)
         {
             jobs.clear();
             map.clear();
@@ -1536,7 +1539,13 @@ namespace
                 auto close = [&]() {
                     if (c.njobs)
                     {
-                        ws_words = std::max(ws_words, c.ws_words);
+                        if (separate)
+                        {
+                            c.ws_base = ws_words;
+                            ws_words += c.ws_words;
+                        }
+                        else
+                            ws_words = std::max(ws_words, c.ws_words);
                         chunks.push_back(c);
                     }
                     c = tile_chunk{};
@@ -1561,6 +1570,7 @@ namespace
                     }
                     J.ws_off = c.ws_words;
                     c.ws_words += words;
+                    c.cost += static_cast<u64>(J.log_n) << J.log_n;
                     c.njobs += 1;
                     jobs.push_back(J);
                 }
@@ -1583,21 +1593,31 @@ namespace
             }
         }
 
-        // device copies of `jobs` and `map` at d_jobs / d_map; ws holds ws_words
+        // the launches of one chunk; device copies of `jobs` and `map` at d_jobs / d_map; ws holds
+        // ws_words
+        int launch_chunk(tile_chunk const& c, job_desc const* d_jobs, u32 const* d_map, u32* ws, hipStream_t s) const
+        {
+            transform_args T = c.T;
+            T.elems = c.elems;
+            auto n = [](std::size_t v) { return static_cast<u32>(v); };
+            u32* w = ws + c.ws_base;
+            if (c.type == job_bgv)
+                HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off,
+                                               n(c.cols_blocks), d_map + c.rows_off, n(c.rows_blocks), w, s));
+            else
+                HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off,
+                                                   n(c.cols_blocks), d_map + c.rows_off, n(c.rows_blocks), d_map + c.icols_off,
+                                                   n(c.icols_blocks), w, s));
+            return HMPC_OK;
+        }
+
         int launch(job_desc const* d_jobs, u32 const* d_map, u32* ws, hipStream_t s) const
         {
             for (auto const& c : chunks)
             {
-                transform_args T = c.T;
-                T.elems = c.elems;
-                auto n = [](std::size_t v) { return static_cast<u32>(v); };
-                if (c.type == job_bgv)
-                    HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off,
-                                                   n(c.cols_blocks), d_map + c.rows_off, n(c.rows_blocks), ws, s));
-                else
-                    HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off, n(c.cols_blocks),
-                                                       d_map + c.rows_off, n(c.rows_blocks), d_map + c.icols_off,
-                                                       n(c.icols_blocks), ws, s));
+                int rc = launch_chunk(c, d_jobs, d_map, ws, s);
+                if (rc)
+                    return rc;
             }
             return HMPC_OK;
         }
@@ -2194,9 +2214,18 @@ struct hmpc_batch_s
         std::vector<job_desc> jobs;   // longest first
         std::size_t table_off = 0;    // byte offsets into mem
         std::size_t counter_off = 0;
+        std::size_t scratch_off = 0;  // the class's own scratch (byte offset into mem)
         unsigned grid = 0;
         u64 slot_words = 0;
+        u64 cost = 0;
     };
+    // work item of a launch: a tile chunk (index >= 0) or a persistent class (-1 - index)
+    struct lane_item
+    {
+        int index;
+        u64 cost;
+    };
+    static constexpr int aux_lanes = 3;  // streams beside the caller's (4 hardware queues per process)
 
     hmpc_context ctx = nullptr;
     int device = 0;  // kept for destruction in any order with the context
@@ -2205,13 +2234,17 @@ struct hmpc_batch_s
     tile_plan tiles;               // jobs of two-pass degrees
     std::size_t tile_jobs_off = 0; // byte offsets into mem
     std::size_t tile_map_off = 0;
+    std::size_t tile_ws_off = 0;
     bool built = false;
     void* mem = nullptr;
     std::size_t mem_bytes = 0;
-    std::size_t scratch_off = 0;
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<std::vector<lane_item>> lanes;  // lane 0: the launch stream, then aux streams
+    hipStream_t aux[aux_lanes] = {};
+    hipEvent_t fork_event = nullptr;
+    hipEvent_t join_events[aux_lanes] = {};
 
     void release()
     {
@@ -2227,6 +2260,7 @@ struct hmpc_batch_s
         mem_bytes = 0;
         classes.clear();
         tiles = tile_plan{};
+        lanes.clear();
         built = false;
     }
 
@@ -2237,12 +2271,24 @@ struct hmpc_batch_s
         release();
         if (capture_stream)
             (void)hipStreamDestroy(capture_stream);
+        for (int i = 0; i < aux_lanes; ++i)
+        {
+            if (aux[i])
+                (void)hipStreamDestroy(aux[i]);
+            if (join_events[i])
+                (void)hipEventDestroy(join_events[i]);
+        }
+        if (fork_event)
+            (void)hipEventDestroy(fork_event);
     }
 
     // Jobs of two-pass degrees go to tile launches over job tables (unless the tuner chose the
     // persistent kernel for their degree); the others are grouped by (modulus, elements per
-    // thread), each class ordered longest job first (LPT) for the persistent kernel. Sizes the
-    // grids and the shared scratch, uploads the tables.
+    // thread), each class ordered longest job first (LPT) for the persistent kernel. Every tile
+    // chunk and every class gets its own workspace, so that they are independent of each other:
+    // they are spread over the launch stream and up to three more (longest first onto the least
+    // loaded), where small launches (a class's persistent kernel holds about one workgroup per CU)
+    // and the tails of large ones overlap. Sizes the grids and the workspaces, uploads the tables.
     int build()
     {
         device_guard g(ctx->device);
@@ -2278,7 +2324,6 @@ struct hmpc_batch_s
             members[f->second].push_back(&e);
         }
         std::size_t off = 0;
-        u64 scratch_words = 0;
         for (std::size_t k = 0; k < classes.size(); ++k)
         {
             auto& c = classes[k];
@@ -2288,24 +2333,53 @@ struct hmpc_batch_s
             {
                 c.jobs.push_back(e->d);
                 c.slot_words = std::max(c.slot_words, job_scratch_words(e->d.type, static_cast<int>(e->d.log_n), c.mod->limbs));
+                c.cost += static_cast<u64>(e->d.log_n) << e->d.log_n;
             }
             int rc = jobs_grid(*c.mod, c.elems, c.jobs.size(), &c.grid);
             if (rc)
                 return rc;
-            scratch_words = std::max(scratch_words, static_cast<u64>(c.grid) * c.slot_words);
             c.table_off = off;
             off += (c.jobs.size() * sizeof(job_desc) + 255) / 256 * 256;
             c.counter_off = off;
             off += counter_bytes;
         }
-        tiles.build(tiled, false);
+        tiles.build(tiled, false, true);
         tile_jobs_off = off;
         off += (tiles.jobs.size() * sizeof(job_desc) + 255) / 256 * 256;
         tile_map_off = off;
         off += (tiles.map.size() * 4 + 255) / 256 * 256;
-        scratch_words = std::max(scratch_words, tiles.ws_words);
-        scratch_off = off;
-        mem_bytes = off + scratch_words * 4;
+        tile_ws_off = off;
+        off += tiles.ws_words * 4;
+        for (auto& c : classes)
+        {
+            c.scratch_off = off;
+            off += (static_cast<u64>(c.grid) * c.slot_words * 4 + 255) / 256 * 256;
+        }
+        mem_bytes = off;
+        // longest work item first onto the least loaded lane
+        std::vector<lane_item> items;
+        for (std::size_t i = 0; i < tiles.chunks.size(); ++i)
+            items.push_back({static_cast<int>(i), tiles.chunks[i].cost});
+        for (std::size_t k = 0; k < classes.size(); ++k)
+            items.push_back({-1 - static_cast<int>(k), classes[k].cost});
+        std::stable_sort(items.begin(), items.end(), [](lane_item const& x, lane_item const& y) { return x.cost > y.cost; });
+        lanes.assign(std::min<std::size_t>(items.size(), 1 + aux_lanes), {});
+        std::vector<u64> load(lanes.size(), 0);
+        for (auto const& it : items)
+        {
+            std::size_t l = static_cast<std::size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+            lanes[l].push_back(it);
+            load[l] += it.cost;
+        }
+        for (std::size_t l = 1; l < lanes.size(); ++l)
+        {
+            if (!aux[l - 1])
+                HMPC_HIP(hipStreamCreateWithFlags(&aux[l - 1], hipStreamNonBlocking));
+            if (!join_events[l - 1])
+                HMPC_HIP(hipEventCreateWithFlags(&join_events[l - 1], hipEventDisableTiming));
+        }
+        if (lanes.size() > 1 && !fork_event)
+            HMPC_HIP(hipEventCreateWithFlags(&fork_event, hipEventDisableTiming));
         HMPC_HIP(hipMalloc(&mem, mem_bytes));
         for (auto const& c : classes)
             HMPC_HIP(hipMemcpy(static_cast<char*>(mem) + c.table_off, c.jobs.data(), c.jobs.size() * sizeof(job_desc),
@@ -2321,30 +2395,50 @@ struct hmpc_batch_s
         return HMPC_OK;
     }
 
-    // the tile launches, then one persistent launch per class, in sequence on `s` (all share the
-    // scratch)
+    int launch_item(lane_item const& it, hipStream_t s)
+    {
+        char* base = static_cast<char*>(mem);
+        if (it.index >= 0)
+            return tiles.launch_chunk(tiles.chunks[static_cast<std::size_t>(it.index)],
+                                      reinterpret_cast<job_desc const*>(base + tile_jobs_off),
+                                      reinterpret_cast<u32 const*>(base + tile_map_off), reinterpret_cast<u32*>(base + tile_ws_off),
+                                      s);
+        auto const& c = classes[static_cast<std::size_t>(-1 - it.index)];
+        transform_args T;
+        int rc = c.mod->args(c.log_n, T);
+        if (rc)
+            return rc;
+        T.elems = c.elems;
+        HMPC_HIP(launch_fast_jobs(c.mod->limbs, T, reinterpret_cast<job_desc const*>(base + c.table_off), job_desc{},
+                                  static_cast<u32>(c.jobs.size()), c.grid, reinterpret_cast<u32*>(base + c.counter_off),
+                                  reinterpret_cast<u32*>(base + c.scratch_off), c.slot_words, s));
+        return HMPC_OK;
+    }
+
+    // lane 0 on `s`, the others on the auxiliary streams between a fork and a join event on `s`
+    // (captured as branches of the graph when `s` is being captured)
     int launch(hipStream_t s)
     {
-        if (!tiles.empty())
+        if (lanes.size() > 1)
         {
-            int rc = tiles.launch(reinterpret_cast<job_desc const*>(static_cast<char*>(mem) + tile_jobs_off),
-                                  reinterpret_cast<u32 const*>(static_cast<char*>(mem) + tile_map_off),
-                                  reinterpret_cast<u32*>(static_cast<char*>(mem) + scratch_off), s);
-            if (rc)
-                return rc;
+            HMPC_HIP(hipEventRecord(fork_event, s));
+            for (std::size_t l = 1; l < lanes.size(); ++l)
+                HMPC_HIP(hipStreamWaitEvent(aux[l - 1], fork_event, 0));
         }
-        for (auto const& c : classes)
+        for (std::size_t l = 0; l < lanes.size(); ++l)
         {
-            transform_args T;
-            int rc = c.mod->args(c.log_n, T);
-            if (rc)
-                return rc;
-            T.elems = c.elems;
-            auto* table = reinterpret_cast<job_desc const*>(static_cast<char*>(mem) + c.table_off);
-            auto* counter = reinterpret_cast<u32*>(static_cast<char*>(mem) + c.counter_off);
-            auto* scratch = reinterpret_cast<u32*>(static_cast<char*>(mem) + scratch_off);
-            HMPC_HIP(launch_fast_jobs(c.mod->limbs, T, table, job_desc{}, static_cast<u32>(c.jobs.size()), c.grid, counter,
-                                      scratch, c.slot_words, s));
+            hipStream_t ls = l == 0 ? s : aux[l - 1];
+            for (auto const& it : lanes[l])
+            {
+                int rc = launch_item(it, ls);
+                if (rc)
+                    return rc;
+            }
+            if (l > 0)
+            {
+                HMPC_HIP(hipEventRecord(join_events[l - 1], ls));
+                HMPC_HIP(hipStreamWaitEvent(s, join_events[l - 1], 0));
+            }
         }
         return HMPC_OK;
     }
