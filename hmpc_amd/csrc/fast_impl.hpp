@@ -650,8 +650,8 @@ namespace hmpc_amd
             }
         }
 
-        // g0: index of the round's first stage within the whole inverse transform (its parity
-        // selects the stages that fold)
+        // g0: index of the round's first stage within the whole inverse transform (it selects the
+        // stages that fold: r26_inverse_fold_stage)
         template<typename A, int E, bool Inv, int W, int II, bool Last>
         __device__ __forceinline__ void stage(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
                                               u32 const (&w0)[A::rl], u32 const (&w1)[A::rl], typename A::params const& P,
@@ -659,7 +659,7 @@ namespace hmpc_amd
         {
             constexpr int K = 1 << W;
             constexpr int lb = Inv ? II : (W - 1 - II);
-            bool const fold = ((g0 + II) & 1) != 0;
+            bool const fold = r26_inverse_fold_stage(g0 + II);
 #pragma unroll
             for (int g = 0; g < E / K; ++g)
             {

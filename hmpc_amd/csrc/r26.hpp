@@ -32,12 +32,14 @@
 //               with limbs < 2^28, for any limbs < 2^31. fold_signed adds 64 q first, so that a
 //               signed-digit value (|value| < 64 q, |x_j| < 2^31 - 2^26, |x_4| < 2^29) folds the
 //               same way
-//   inverse butterfly (x + y, mont(x - y, w)) on signed digits, x + y folded (signed) at odd
-//               stages of the transform: even-stage inputs are folded values or mont outputs
-//               (within (-q, 2q (1 + 2^-19)), |limbs| < 2^28), odd-stage inputs one unfolded sum of
-//               two (within 4q (1 + 2^-19), |limbs| < 2^29); from point-wise products (a forward
-//               result, |a| < 35.01q with |limbs| < 2^30.2 after at most 17 stages, times a folded
-//               one: within (-4.4q, 8.4q), digits 0..3 < 2^28 and |top limb| < 2^24 after the x4
+//   inverse butterfly (x + y, mont(x - y, w)) on signed digits, x + y folded (signed) at stages
+//               1, 4, 7, ... of the transform (r26_inverse_fold_stage). Folded values are in [0, 2q)
+//               with limbs < 2^27.2 and mont outputs within (-q, 2q (1 + 2^-19)) with digits 0..3
+//               < 2^26, so the sums of the two stages after a fold stay within 4.2q and 8.4q with
+//               limbs < 2^28.2 and < 2^29.2, and the third stage's sum (< 16.8q, limbs < 2^30.2)
+//               meets the signed fold's bounds; from point-wise products (a forward result,
+//               |a| < 35.01q with |limbs| < 2^30.2 after at most 17 stages, times a folded one:
+//               within (-4.4q, 8.4q), digits 0..3 < 2^28 and |top limb| < 2^24 after the x4
 //               correction) stage 0 gives < 16.8q, and stage 1's sums stay within 33.6q with
 //               |limbs| < 2^30. Every mont input stays within 33.6q with |limbs| < 2^30.
 #pragma once
@@ -246,6 +248,9 @@ namespace hmpc_amd
             r26_fold_signed(x, P);
         r26_mont_signed(y, d, w, P);
     }
+
+    // the inverse stages whose sums x + y are folded (bounds in the header): 1, 4, 7, ...
+    HMPC_HD bool r26_inverse_fold_stage(int i) { return i % 3 == 1; }
 
     // the inverse's last stage with its scaling: (x, y) -> (mont(x + y, w0), mont(x - y, w1))
     HMPC_HD void r26_inverse_last(u32 (&x)[5], u32 (&y)[5], u32 const (&w0)[5], u32 const (&w1)[5], p26 const& P)

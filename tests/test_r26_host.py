@@ -231,14 +231,15 @@ def test_point_product_bounds(tool, q):
 
 
 @pytest.mark.parametrize("q", PRIMES)
-@pytest.mark.parametrize("stages", [12, 16])
+@pytest.mark.parametrize("stages", [12, 16, 17])
 def test_inverse_chain_bounds(tool, q, stages):
-    """Lazy inverse stages on signed digits, x + y folded at odd stages only, from point-wise
-    products (r26_point of forward outputs: within (-4.4q, 8.4q), digits < 2^28), each stage keeping
-    either output pair or a doubled output: every mont input x - y stays within 33.6q with limbs
-    below 2^30, odd-stage inputs within 4q (1 + 2^-18) (16.8q after stage 0) with limbs below 2^30,
-    the folded outputs in [0, 2q) with limbs < 2^28, the mont outputs within (-q, 2q (1 + 2^-19)),
-    and the results are exact mod q."""
+    """Lazy inverse stages on signed digits, x + y folded at stages 1, 4, 7, ... only (the kernels'
+    schedule, r26_inverse_fold_stage), from point-wise products (r26_point of forward outputs:
+    within (-4.4q, 8.4q), digits < 2^28), each stage keeping either output pair or a doubled output:
+    every mont input x - y stays within 33.6q with limbs below 2^30, fold-stage inputs within 8.4q
+    (1 + 2^-16) (16.8q at stage 1) with limbs below 2^29.2 (2^30 at stage 1), the folded outputs in
+    [0, 2q) with limbs < 2^28, the mont outputs within (-q, 2q (1 + 2^-19)), and the results are
+    exact mod q."""
     c = q - (1 << 124)
     rng = random.Random(11 + stages)
     R = 1 << 130
@@ -247,7 +248,7 @@ def test_inverse_chain_bounds(tool, q, stages):
         x, y = value(lx), value(ly)
         ws = [rng.randrange(q) for _ in range(stages)]
         sels = [1 if it % 3 == 0 else rng.randrange(3) for _ in range(stages)]
-        folds = [s & 1 for s in range(stages)]
+        folds = [int(s % 3 == 1) for s in range(stages)]
         line = f"I {c} {stages} " + " ".join(map(str, lx + ly)) + " " + \
                " ".join(" ".join(map(str, limbs(w))) + f" {f} {sel}" for w, f, sel in zip(ws, folds, sels))
         out = run(tool, line)
@@ -257,9 +258,10 @@ def test_inverse_chain_bounds(tool, q, stages):
         for s, (w, f, sel) in enumerate(zip(ws, folds, sels)):
             d = [a - b for a, b in zip(px, py)]
             assert 10 * abs(value(d)) < 336 * q and max(abs(v) for v in d) < (1 << 30), f"mont input at stage {s}"
-            if f:  # odd stage: inputs are one unfolded sum (or the stage-0 sum of products)
-                bound = 168 * q // 10 if s == 1 else 4 * q + (q >> 17)
-                assert abs(value(px)) < bound and abs(value(py)) < bound and max(abs(v) for v in px + py) < (1 << 30)
+            if f:  # fold stage: inputs are sums of two stages since the last fold (of products at stage 1)
+                bound = 168 * q // 10 if s == 1 else 84 * q // 10 + (q >> 16)
+                limb = (1 << 30) if s == 1 else int(2 ** 29.2)
+                assert abs(value(px)) < bound and abs(value(py)) < bound and max(abs(v) for v in px + py) < limb
             ex, ey = (ex + ey) % q, (ex - ey) * w * pow(R, -1, q) % q
             if sel == 1:
                 ey = ex
