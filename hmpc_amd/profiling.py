@@ -15,6 +15,11 @@
         clock each kernel ran at (busy cycles per XCD / dispatch duration). Several runs (databases of
         different counters) merge into one entry per kernel in --out.
 
+    python -m hmpc_amd.profiling overlap RESULTS.db [--gap-us 20] [--min-dispatches 20]
+        busy periods of a kernel trace (dispatches separated by less than the gap): per period the
+        dispatch count, the streams and queues used, the sum of kernel durations, the wall span
+        from first start to last end, and their ratio (above 1 where kernels ran concurrently)
+
 The traffic file maps bench kernel names (the names of the library's launch probes) to bytes
 per launch; bench.py reports it as roofline.traffic.
 """
@@ -166,6 +171,30 @@ def cmd_counters(args):
             f.write("\n")
 
 
+def cmd_overlap(args):
+    cur = sqlite3.connect(args.db).cursor()
+    rows = sorted(cur.execute("select start, end, stream_id, queue_id from kernels").fetchall())
+    periods, cur_p, last_end = [], [], None
+    for start, end, stream, queue in rows:
+        if cur_p and start - last_end > args.gap_us * 1e3:
+            periods.append(cur_p)
+            cur_p = []
+        cur_p.append((start, end, stream, queue))
+        last_end = end if last_end is None or not cur_p[:-1] else max(last_end, end)
+    if cur_p:
+        periods.append(cur_p)
+    print(f"{'period':>6} {'dispatches':>10} {'streams':>7} {'queues':>6} {'busy ms':>9} {'span ms':>9} {'busy/span':>9}")
+    k = 0
+    for p in periods:
+        if len(p) < args.min_dispatches:
+            continue
+        busy = sum(e - s for s, e, _, _ in p) / 1e6
+        span = (max(e for _, e, _, _ in p) - p[0][0]) / 1e6
+        print(f"{k:>6} {len(p):>10} {len({x[2] for x in p}):>7} {len({x[3] for x in p}):>6} {busy:>9.3f} {span:>9.3f} "
+              f"{busy / span if span else 0:>9.2f}")
+        k += 1
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -185,8 +214,12 @@ def main(argv=None):
     c.add_argument("--clock-ghz", type=float, default=2.4)
     c.add_argument("--xcds", type=int, default=8)
     c.add_argument("--out", default="")
+    o = sub.add_parser("overlap")
+    o.add_argument("db")
+    o.add_argument("--gap-us", type=float, default=20.0)
+    o.add_argument("--min-dispatches", type=int, default=20)
     args = ap.parse_args(argv)
-    {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters}[args.cmd](args)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "counters": cmd_counters, "overlap": cmd_overlap}[args.cmd](args)
 
 
 if __name__ == "__main__":
