@@ -405,7 +405,7 @@ def bench_examples(hm, ctx, stream, reps=5):
     ok = bool(np.array_equal(p.download(z, N * n), p.download(x, N * n)))
     out["ntt_example"] = {"source": "examples/ntt.cpp:22-23", "q_bits": P_NTT_EXAMPLE.bit_length(), "limbs": p.limbs,
                           "N": N, "n": n, "ms": ms, "gcoeff_per_s": n * N / (ms / 1e3) / 1e9,
-                          "fast_kernels": p.schedule(log_n)["fast"], "roundtrip_ok": ok}
+                          "fast_transform_kernels": p.schedule(log_n)["fast"], "roundtrip_ok": ok}
     del x, y, z, w
     # examples/bgv.cpp
     q = hm.Modulus(ctx, Q_BGV_EXAMPLE)
@@ -425,7 +425,7 @@ def bench_examples(hm, ctx, stream, reps=5):
                                                              stream=stream), reps, 1)
     out["bgv_example"] = {"source": "examples/bgv.cpp:25-29", "q_bits": Q_BGV_EXAMPLE.bit_length(), "limbs": q.limbs,
                           "N": N, "n": n, "ms": ms, "enc_per_s": n / (ms / 1e3),
-                          "fast_kernels": q.schedule(log_n)["fast"]}
+                          "fast_transform_kernels": q.schedule(log_n)["fast"]}
     del c0, c1, msgs
     # tests/expr/crypto/lhe/enc.cpp at its size, through the plaintext path
     qq, pp = hm.Modulus(ctx, Q_ENC_TEST), hm.Modulus(ctx, P_ENC_TEST)
