@@ -1773,6 +1773,22 @@ int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, co
         nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
         return HMPC_OK;
     }
+    if (mod->fast_transform(log_n) && log_n > 12)
+    {
+        // 4- or 5-limb moduli with 4q < R (the CIOS policy): the two tile launches with any
+        // plaintext modulus, taken as the Montgomery constant p_mont
+        auto e = mod->bgv_elems.find(log_n);
+        T.elems = e != mod->bgv_elems.end() ? e->second : 8;
+        job_desc J = bgv_job(T, B, 0, log_n, pk_a, pk_b, messages, c0, c1, static_cast<u64>(n) << log_n, 0, 0);
+        std::copy(p_mont, p_mont + mod->limbs, J.p_mont);
+        void* w = nullptr;
+        rc = ctx->workspace(s, 3 * static_cast<std::size_t>(ws_limbs(mod->limbs)) * (n << log_n) * 4, &w);
+        if (rc)
+            return rc;
+        HMPC_HIP(launch_fast_bgv_passes(mod->limbs, T, J, n, static_cast<u32*>(w), s));
+        nonce_add(ctx->nonce, 3 * (static_cast<u64>(n) << log_n));
+        return HMPC_OK;
+    }
     void* w = nullptr;
     rc = ctx->workspace(s, bgv_workspace_words(mod->limbs, log_n, n) * 4, &w);
     if (rc)

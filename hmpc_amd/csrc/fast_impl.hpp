@@ -73,6 +73,7 @@ namespace hmpc_amd
             static constexpr int tws = 8;   // twiddle table stride in words (5 used)
             static constexpr bool stage_prefetch = true;
             static constexpr bool wave_units = false;  // unit order (unit_index)
+            static constexpr bool mont_plaintext = false;  // BGV: p v + m formed as a signed 64-bit integer
             using params = p26;
 
             __device__ __forceinline__ static void load_global(u32 (&x)[rl], u32 const* __restrict__ src, std::size_t count,
@@ -285,6 +286,7 @@ namespace hmpc_amd
             static constexpr int tws = 2;
             static constexpr bool stage_prefetch = false;
             static constexpr bool wave_units = false;
+            static constexpr bool mont_plaintext = false;
             using params = fast_params;
             using ops = fast_ops<2, form_goldilocks>;
 
@@ -405,6 +407,9 @@ namespace hmpc_amd
             static constexpr int tws = L;
             static constexpr bool stage_prefetch = true;
             static constexpr bool wave_units = false;
+            // BGV sampling with any plaintext modulus (given as a Montgomery constant):
+            // from_signed, mul_const and add_signed below
+            static constexpr bool mont_plaintext = true;
             using params = fast_params;
             using ops = fast_ops<L, form_lazy>;
 
@@ -524,6 +529,72 @@ namespace hmpc_amd
 #pragma unroll
                 for (int l = 0; l < L; ++l)
                     a[l] = t[l];
+            }
+
+            // x = k u + x for k canonical and u, x forward results in [0, 4q): u and x halved to
+            // [0, 2q), k u < 2 q^2 gives a lazy product in [0, 2q), the sum lies in [0, 4q) (the
+            // canonical store's input range)
+            __device__ __forceinline__ static void point_add(u32 (&x)[rl], u32 const (&u)[rl], u32 const (&k)[rl],
+                                                             params const& P)
+            {
+                u32 uh[rl], t[rl], xh[rl];
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                {
+                    uh[l] = u[l];
+                    xh[l] = x[l];
+                }
+                ops::half_reduce(uh, P);
+                ops::half_reduce(xh, P);
+                ops::mul(t, k, uh, P);
+                add_plain<L>(x, xh, t);
+            }
+
+            // signed integer |z| < 2^63 -> z R mod q in [0, 2q]: |z| < q, so the lazy Montgomery
+            // product with R^2 is |z| R in [0, 2q); a negative z takes 2q minus it
+            __device__ __forceinline__ static void from_signed(u32 (&x)[rl], long long z, params const& P)
+            {
+                u64 az = z < 0 ? static_cast<u64>(0) - static_cast<u64>(z) : static_cast<u64>(z);
+                u32 a[rl] = {}, r2[rl], t[rl];
+                a[0] = static_cast<u32>(az);
+                a[1] = static_cast<u32>(az >> 32);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    r2[l] = P.r2[l];
+                ops::mul(t, a, r2, P);
+                u32 n[rl];
+                u32 br = 0;
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    n[l] = subb(P.q2[l], t[l], br, br);
+                bool const neg = z < 0;
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    x[l] = neg ? n[l] : t[l];
+            }
+
+            // x (in [0, 2q]) -> x c R^-1 for a canonical constant c: in [0, 2q)
+            __device__ __forceinline__ static void mul_const(u32 (&x)[rl], u32 const* c, params const& P)
+            {
+                u32 cc[rl], t[rl];
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    cc[l] = c[l];
+                ops::mul(t, x, cc, P);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    x[l] = t[l];
+            }
+
+            // x (in [0, 2q)) + m R for an unsigned 32-bit m: in [0, 4q)
+            __device__ __forceinline__ static void add_signed(u32 (&x)[rl], u32 m, params const& P)
+            {
+                u32 y[rl], s[rl];
+                from_signed(y, static_cast<long long>(m), P);
+                add_plain<L>(s, x, y);
+#pragma unroll
+                for (int l = 0; l < L; ++l)
+                    x[l] = s[l];
             }
         };
 

@@ -26,6 +26,7 @@ namespace hmpc_amd
         chacha_key key;    // bgv: queue key
         u32 nonce[3][3];   // bgv: nonces of the u, v, w streams
         u64 p_int;         // bgv: plaintext modulus as an integer (< 2^40)
+        u32 p_mont[max_limbs];  // bgv on the CIOS policy: the plaintext modulus as a Montgomery constant
         u32 const* twf;    // forward / inverse twiddles of the policy's table format for log_n
         u32 const* twi;
         // polymul: in0 = a, in1 = b (planes of in_count, polynomial at element in_off)

@@ -78,19 +78,38 @@ namespace hmpc_amd
                                nonce[2]);
                 long long zl = centered_binomial_words(blk[0], blk[1], bits);
                 long long zh = bits <= 32 ? centered_binomial_words(blk[1], 0u, bits) : centered_binomial_words(blk[2], blk[3], bits);
-                if constexpr (which != 0)
-                {
-                    zl *= p;
-                    zh *= p;
-                }
-                if constexpr (which == 1)
-                {
-                    zl += J.msg[J.msg_off + k];
-                    zh += J.msg[J.msg_off + k + half];
-                }
                 u32 xl[A::rl], xh[A::rl];
-                A::from_signed(xl, zl, P);
-                A::from_signed(xh, zh, P);
+                if constexpr (A::mont_plaintext)
+                {
+                    // any plaintext modulus: p v + m = mont(v R, p R) + m R (p as a Montgomery constant)
+                    A::from_signed(xl, zl, P);
+                    A::from_signed(xh, zh, P);
+                    if constexpr (which != 0)
+                    {
+                        A::mul_const(xl, J.p_mont, P);
+                        A::mul_const(xh, J.p_mont, P);
+                    }
+                    if constexpr (which == 1)
+                    {
+                        A::add_signed(xl, J.msg[J.msg_off + k], P);
+                        A::add_signed(xh, J.msg[J.msg_off + k + half], P);
+                    }
+                }
+                else
+                {
+                    if constexpr (which != 0)
+                    {
+                        zl *= p;
+                        zh *= p;
+                    }
+                    if constexpr (which == 1)
+                    {
+                        zl += J.msg[J.msg_off + k];
+                        zh += J.msg[J.msg_off + k + half];
+                    }
+                    A::from_signed(xl, zl, P);
+                    A::from_signed(xh, zh, P);
+                }
                 A::lds_put(lds, elo, xl);
                 A::lds_put(lds, ehi, xh);
             }

@@ -22,6 +22,8 @@ namespace hmpc_amd
                                       u32* workspace, hipStream_t s);
     hipError_t launch_fast_polymul_c5(transform_args const& T, std::size_t n, u32 const* a, u32 const* b, u32* out,
                                       u32* workspace, hipStream_t s);
+    hipError_t launch_fast_bgv_passes_c4(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
+    hipError_t launch_fast_bgv_passes_c5(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
 
     hipError_t fast_jobs_occupancy_l2(int elems, int* blocks_per_cu);
     hipError_t fast_jobs_occupancy_l4(int elems, int* blocks_per_cu);
@@ -114,10 +116,12 @@ namespace hmpc_amd
 
     hipError_t launch_fast_bgv_passes(int L, transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s)
     {
-        if (L == 4)
+        if (L == 4 && T.form == form_r26)
             return launch_fast_bgv_passes_l4(T, base, n, ws, s);
-        if (L == 2)
+        if (L == 2 && T.form == form_goldilocks)
             return launch_fast_bgv_passes_l2(T, base, n, ws, s);
+        if (cios_form(L, T.form))
+            return L == 4 ? launch_fast_bgv_passes_c4(T, base, n, ws, s) : launch_fast_bgv_passes_c5(T, base, n, ws, s);
         return hipErrorInvalidValue;
     }
 

@@ -1,6 +1,7 @@
-// Fast transform kernels instantiated for the 5-limb CIOS policy (any modulus with 4q < 2^160,
-// code in fast_impl.hpp): transforms and products.
-#include "fast_impl.hpp"
+// Fast tile kernels instantiated for the 5-limb CIOS policy (any modulus with 4q < 2^160, code
+// in fast_impl.hpp and jobs_impl.hpp): transforms, products, and BGV encryption over tiles with the
+// plaintext modulus as a Montgomery constant.
+#include "jobs_impl.hpp"
 
 namespace hmpc_amd
 {
@@ -14,5 +15,10 @@ namespace hmpc_amd
                                       u32* workspace, hipStream_t s)
     {
         return fast::launch_polymul<fast::arith_cios<5>>(T, fast::montgomery_params(T), n, a, b, out, workspace, s);
+    }
+
+    hipError_t launch_fast_bgv_passes_c5(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s)
+    {
+        return fast::launch_bgv_passes<fast::arith_cios<5>>(T.elems, base, n, ws, fast::montgomery_params(T), s);
     }
 }
