@@ -103,10 +103,10 @@ int hmpc_modulus_info(hmpc_modulus mod, int* limbs, int* bit_size, uint32_t* npr
 /* Kernel family for this modulus: 0 (default) = the fast tile kernels (register-resident rounds
  * between LDS exchanges) where supported, else the portable LDS radix-2 kernels; 1 = always the
  * portable kernels. Results are identical. The fast kernels serve every operation for
- * q = 2^124 + c (c < 2^24, radix-2^26 arithmetic) and Goldilocks 2^64 - 2^32 + 1, and transforms
- * and products (hmpc_ntt, hmpc_polymul) for any 4- or 5-limb q with 4q < 2^(32 L) (CIOS Montgomery
- * products, lazy butterflies), e.g. the 125-bit prime of hmpc's examples/ntt.cpp and the 150-bit
- * prime of examples/bgv.cpp. */
+ * q = 2^124 + c (c < 2^24, radix-2^26 arithmetic) and Goldilocks 2^64 - 2^32 + 1; for any 4- or
+ * 5-limb q with 4q < 2^(32 L) (CIOS Montgomery products, lazy butterflies), e.g. the 125-bit prime of
+ * hmpc's examples/ntt.cpp and the 150-bit prime of examples/bgv.cpp, they serve transforms and
+ * products (hmpc_ntt, hmpc_polymul) and hmpc_bgv_encrypt at N > 4096 (any plaintext modulus). */
 int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
 /* Schedule of the fast kernels for N = 2^log_n: elements per thread (4 or 8 for 4 and 5 limbs, 8
  * or 16 for 2 limbs; 0 restores the default) and launch structure (persistent = 1: one persistent
@@ -250,7 +250,10 @@ int hmpc_polymul_noalloc(hmpc_modulus mod, int log_n, size_t n, const uint32_t* 
  *            nonce, nonce + nN, nonce + 2nN; the context nonce advances by 3nN
  *   c0 = A*NTT(u) + NTT(v)*p + NTT(m),  c1 = B*NTT(u) + NTT(w)*p
  * pk_a, pk_b: [L][N] in NTT form; messages: n*N u32 words lifted as unsigned values;
- * p_mont: plaintext modulus as a Montgomery constant (L host words); c0, c1: [L][n*N]. */
+ * p_mont: plaintext modulus as a Montgomery constant (L host words); c0, c1: [L][n*N].
+ * Kernels: for q = 2^124 + c or Goldilocks with p < 2^40, fused launches over tiles or the
+ * persistent job kernel (hmpc_modulus_set_bgv_schedule); for other 4- and 5-limb q with 4q < R at
+ * N > 4096, the tile launches with p as a Montgomery constant; otherwise the portable kernels. */
 int hmpc_bgv_encrypt(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, const uint32_t* pk_a,
                      const uint32_t* pk_b, const uint32_t* messages, const uint32_t* p_mont, int bits_u,
                      int bits_v, int bits_w, uint32_t* c0, uint32_t* c1, void* stream);
