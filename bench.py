@@ -103,12 +103,29 @@ def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
     return res
 
 
-def keygen(mod, log_n, t_mont):
-    """pk = (A, B) with A = B s + t e in NTT form (project-defined key generation: hmpc has none)"""
+# capability-stream width of the uniform ternary draw (HMPC_BITS_TERNARY in include/hmpc_hip.h)
+BITS_TERNARY = 0
+# C3's stated randomness: s and u uniform ternary, errors centred binomial with eta = 10 (20 bits)
+C3_BITS = (BITS_TERNARY, 20, 20)
+# hmpc's lhe::randomness(): u, v, w centred binomial with variances 1/2, 10, 10 (2, 40, 40 bits)
+HMPC_BITS = (2, 40, 40)
+
+
+def keygen(mod, log_n, t_mont, ternary_s=False, e_bits=40):
+    """pk = (A, B) with A = B s + t e in NTT form (project-defined key generation: hmpc has none).
+    s is centred binomial with variance 1/2 (hmpc's u distribution) or, with ternary_s, uniform
+    ternary from the capability stream at nonce {1, 0, 0}; e is centred binomial of e_bits bits."""
     N = 1 << log_n
     B = mod.sample_uniform(KEY, 0, N)
-    s = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 1, 2, N))
-    e = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 2, 40, N))
+    if ternary_s:
+        rng = mod.ctx.rng
+        mod.ctx.set_rng(KEY, [1, 0, 0])
+        s_coeff = mod.sample_capability("centered_binomial", BITS_TERNARY, "element", log_n, 1)
+        mod.ctx.set_rng(*rng)
+    else:
+        s_coeff = mod.sample_cbd(KEY, 1, 2, N)
+    s = mod.ntt(log_n, 1, s_coeff)
+    e = mod.ntt(log_n, 1, mod.sample_cbd(KEY, 2, e_bits, N))
     A = mod.pointwise(mod.ADD, N, mod.pointwise(mod.MUL, N, B, s), mod.pointwise_scalar(mod.MUL, N, e, t_mont))
     return A, B, s
 
@@ -119,36 +136,46 @@ def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
     N = 1 << log_n
     log(f"[bench] C3 BGV: {n} ciphertexts, N=2^{log_n}")
     t_mont = mod.mont(1 << 32)
-    A, B, s = keygen(mod, log_n, t_mont)
+    A, B, s = keygen(mod, log_n, t_mont, ternary_s=True, e_bits=20)
     msgs = ctx.sample_words(KEY, 3, n * N)
     c0, c1 = mod.alloc(n * N), mod.alloc(n * N)
     tuned = mod.autotune_bgv(log_n, 128, stream=stream)  # 128 < resident workgroups: its launches have their own grid
     log(f"[bench] autotuned BGV schedule: {tuned}")
-    ctx.set_rng(ENC_KEY, [0, 0, 0])
-    ctx.set_timing(True)
-    ms, per, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n, A, B, msgs, t_mont, c0=c0, c1=c1, stream=stream),
-                            steps, warmup)
-    kernels = ctx.kernel_times()
-    ctx.set_timing(False)
-    # decrypt-and-compare on the last batch (size-independent property)
-    dec = mod.bgv_decrypt(log_n, n, s, c0, c1, stream=stream)
-    stream.synchronize()
-    ok = bool(np.array_equal(dec.to_numpy(n * N), msgs.to_numpy(n * N)))
+
+    def timed_batch(A, B, s, bits):
+        ctx.set_rng(ENC_KEY, [0, 0, 0])
+        ctx.set_timing(True)
+        ms, per, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n, A, B, msgs, t_mont, bits=bits, c0=c0, c1=c1,
+                                                                     stream=stream), steps, warmup)
+        kernels = ctx.kernel_times()
+        ctx.set_timing(False)
+        # decrypt-and-compare on the last batch (size-independent property)
+        dec = mod.bgv_decrypt(log_n, n, s, c0, c1, stream=stream)
+        stream.synchronize()
+        return ms, per, kernels, bool(np.array_equal(dec.to_numpy(n * N), msgs.to_numpy(n * N)))
+
+    # hmpc's randomness first, with its own key (s of variance 1/2, e of 40 bits)
+    hA, hB, hs = keygen(mod, log_n, t_mont)
+    hms, hper, _, hok = timed_batch(hA, hB, hs, HMPC_BITS)
+    ms, per, kernels, ok = timed_batch(A, B, s, C3_BITS)
     pc0, pc1 = mod.alloc(n_plain * N), mod.alloc(n_plain * N)
-    pms, _, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n_plain, A, B, msgs, t_mont, c0=pc0, c1=pc1,
-                                                               plain=True, stream=stream), 2, 1)
+    pms, _, _ = time_steps(ctx, stream, lambda: mod.bgv_encrypt(log_n, n_plain, A, B, msgs, t_mont, bits=C3_BITS, c0=pc0,
+                                                               c1=pc1, plain=True, stream=stream), 2, 1)
     # the fused kernels launched once per ciphertext (hmpc_bgv_encrypt with n = 1, one call each)
     views = [(hm.View(ctx, msgs.ptr + 4 * i * N), hm.View(ctx, pc0.ptr + 16 * i * N), hm.View(ctx, pc1.ptr + 16 * i * N))
              for i in range(n_plain)]
 
     def one_by_one():
         for m_i, c0_i, c1_i in views:
-            mod.bgv_encrypt(log_n, 1, A, B, m_i, t_mont, c0=c0_i, c1=c1_i, stream=stream)
+            mod.bgv_encrypt(log_n, 1, A, B, m_i, t_mont, bits=C3_BITS, c0=c0_i, c1=c1_i, stream=stream)
 
     fms, _, _ = time_steps(ctx, stream, one_by_one, 3, 1)
     return {"n": n, "ms": ms, "per": per, "enc_per_s": n / (ms / 1e3), "decrypt_ok": ok, "schedule": tuned,
             "kernels": kernels, "plain_enc_per_s": n_plain / (pms / 1e3), "plain_sample": n_plain,
-            "per_ciphertext_fused_enc_per_s": n_plain / (fms / 1e3)}
+            "per_ciphertext_fused_enc_per_s": n_plain / (fms / 1e3),
+            "hmpc_randomness": {"bits_u_v_w": list(HMPC_BITS), "s": "centred binomial, variance 1/2",
+                                "e_bits": 40, "enc_per_s": n / (hms / 1e3), "ms_per_batch": hms,
+                                "p50_ms": percentile(hper, 50), "p99_ms": percentile(hper, 99), "decrypt_ok": hok}}
 
 
 def bench_c1(hm, ctx, stream, iters=200):
@@ -699,13 +726,15 @@ def main():
     if "c3" in sections:
         bg = bench_bgv(hm, ctx, stream, max(3, args.steps // 2), args.warmup)
         line["bgv"] = {"workload": "BGV pk encryption, 1024 ciphertexts, N=2^14, q=0x1...78001 (4 limbs), t=2^32",
+                       "randomness": {"s": "uniform ternary", "u": "uniform ternary",
+                                      "e_e0_e1": "centred binomial eta=10 (20 bits)", "bits_u_v_w": list(C3_BITS)},
                        "enc_per_s": bg["enc_per_s"], "ms_per_batch": bg["ms"],
                        "p50_ms": percentile(bg["per"], 50), "p99_ms": percentile(bg["per"], 99),
                        "plain_enc_per_s": bg["plain_enc_per_s"], "plain_sample": bg["plain_sample"],
                        "batched_vs_plain": bg["enc_per_s"] / bg["plain_enc_per_s"],
                        "per_ciphertext_fused_enc_per_s": bg["per_ciphertext_fused_enc_per_s"],
                        "batched_vs_per_ciphertext_fused": bg["enc_per_s"] / bg["per_ciphertext_fused_enc_per_s"],
-                       "decrypt_ok": bg["decrypt_ok"],
+                       "decrypt_ok": bg["decrypt_ok"], "hmpc_randomness": bg["hmpc_randomness"],
                        "schedule": bg["schedule"], "kernels_ms": {k: v[1] for k, v in bg["kernels"].items()}}
     if "c4" in sections:
         line["uneven_batch"] = bench_c4(hm, ctx, stream)

@@ -6,7 +6,7 @@ bindings (``runtime.py``).
 """
 
 from .runtime import (Batch, Buffer, Context, Event, HmpcError, HostBuffer, Modulus, Server, Stream, View, LIB_PATH,  # noqa: F401
-                      SIGNATURES,
+                      BITS_TERNARY, SIGNATURES,
                       int_to_words, load, words_to_int)
 
 Q64 = (1 << 64) - (1 << 32) + 1

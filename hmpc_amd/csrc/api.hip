@@ -1288,6 +1288,9 @@ namespace
     }
 
     bool valid_bits(int bits) { return bits >= 2 && bits <= 64 && bits % 2 == 0; }
+    // capability-stream draws also take width 0, the uniform ternary draw
+    bool valid_cap_bits(int bits) { return bits == ternary_bits || valid_bits(bits); }
+    constexpr char const* cap_bits_message = "bits = 4 * variance must be even and in [2, 64], or 0 (uniform ternary)";
 }
 
 int hmpc_sample_uniform(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream)
@@ -1330,8 +1333,8 @@ int hmpc_sample_cbd_ntt(hmpc_context ctx, hmpc_modulus mod, int bits, int log_n,
 {
     if (!ctx || !mod || (n && !out))
         return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
-    if (!valid_bits(bits))
-        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    if (!valid_cap_bits(bits))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, cap_bits_message);
     transform_args T;
     int rc = mod->args(log_n, T);
     if (rc)
@@ -1711,8 +1714,8 @@ namespace
     {
         if (!ctx || !mod || !p_mont || (n && (!pk_a || !pk_b || !messages || !c0 || !c1)))
             return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
-        if (!valid_bits(bu) || !valid_bits(bv) || !valid_bits(bw))
-            return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+        if (!valid_cap_bits(bu) || !valid_cap_bits(bv) || !valid_cap_bits(bw))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, cap_bits_message);
         if (n > 0x7fffffffu)
             return fail(HMPC_ERR_INVALID_ARGUMENT, "batch too large");
         return HMPC_OK;
@@ -1984,8 +1987,8 @@ namespace
     {
         if (dist < cap_uniform || dist > cap_drown_unsigned)
             return fail(HMPC_ERR_INVALID_ARGUMENT, "unknown distribution");
-        if (dist == cap_centered_binomial && !valid_bits(bits))
-            return fail(HMPC_ERR_INVALID_ARGUMENT, "centred binomial: bits = 4 * variance must be even and in [2, 64]");
+        if (dist == cap_centered_binomial && !valid_cap_bits(bits))
+            return fail(HMPC_ERR_INVALID_ARGUMENT, cap_bits_message);
         if (dist == cap_binomial && (bits < 1 || bits > 256))
             return fail(HMPC_ERR_INVALID_ARGUMENT, "binomial: bits must be in [1, 256]");
         if ((dist == cap_drown_signed && (bits < 2 || bits > 32 * cap_max_words)) ||
@@ -2082,8 +2085,8 @@ int hmpc_bgv_encrypt_plaintext(hmpc_context ctx, hmpc_modulus mod_q, hmpc_modulu
         return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument");
     if (mod_q->ctx != ctx || mod_p->ctx != ctx)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "moduli of another context");
-    if (!valid_bits(bits_u) || !valid_bits(bits_w))
-        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    if (!valid_cap_bits(bits_u) || !valid_cap_bits(bits_w))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, cap_bits_message);
     if (v_distribution != cap_centered_binomial && v_distribution != cap_drown_signed)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "v: centred binomial (randomness) or signed drowning (drowning_randomness)");
     int rc = check_cap(v_distribution, bits_v, cap_layout_ntt, log_n, n);
@@ -3307,8 +3310,8 @@ int hmpc_server_add_key(hmpc_server srv, hmpc_modulus mod, int log_n, const uint
 {
     if (!srv || !mod || !pk_a || !pk_b || !p_mont || !key_id || mod->ctx != srv->ctx)
         return fail(HMPC_ERR_INVALID_ARGUMENT, "null argument or foreign modulus");
-    if (!valid_bits(bits_u) || !valid_bits(bits_v) || !valid_bits(bits_w))
-        return fail(HMPC_ERR_INVALID_ARGUMENT, "bits = 4 * variance must be even and in [2, 64]");
+    if (!valid_cap_bits(bits_u) || !valid_cap_bits(bits_v) || !valid_cap_bits(bits_w))
+        return fail(HMPC_ERR_INVALID_ARGUMENT, cap_bits_message);
     int rc = mod->prepare(log_n);
     if (rc)
         return rc;

@@ -89,6 +89,28 @@ namespace hmpc_amd
         return __builtin_popcount(m0) + __builtin_popcount(m1) - bits / 2;
     }
 
+    // Width 0 on a capability stream selects the uniform ternary draw, which hmpc does not have:
+    // t = floor(3 w / 2^64) - 1 for the 64-bit word w = w1 : w0. Each of -1, 0, 1 takes
+    // floor or ceil of 2^64 / 3 values of w, so the draw is within 2^-63 of uniform.
+    constexpr int ternary_bits = 0;
+
+    HMPC_HD int ternary_words(u32 w0, u32 w1)
+    {
+        u64 w = (static_cast<u64>(w1) << 32) | w0;
+        u64 twice = w << 1;
+        int hi = static_cast<int>(w >> 63) + ((twice + w) < twice ? 1 : 0);  // high word of 2w + w
+        return hi - 1;
+    }
+
+    // whether a capability draw of this width takes two words (ternary and widths above 32)
+    HMPC_HD bool capability_two_words(int bits) { return bits == ternary_bits || bits > 32; }
+
+    // one capability draw: the centred binomial of `bits` bits, or the ternary draw at width 0
+    HMPC_HD int capability_small_words(u32 w0, u32 w1, int bits)
+    {
+        return bits == ternary_bits ? ternary_words(w0, w1) : centered_binomial_words(w0, w1, bits);
+    }
+
     // Montgomery form of a small signed value v (|v| < 2^31): (v mod q) * R mod q.
     // hmpc/ints/mod.hpp:219-226: add q when negative, then multiply by R^2 and reduce.
     template<int L>

@@ -76,8 +76,9 @@ namespace hmpc_amd
                 u32 blk[16];
                 chacha20_block(blk, J.key, static_cast<u32>((static_cast<u64>(J.poly) << log_n) + k), nonce[0], nonce[1],
                                nonce[2]);
-                long long zl = centered_binomial_words(blk[0], blk[1], bits);
-                long long zh = bits <= 32 ? centered_binomial_words(blk[1], 0u, bits) : centered_binomial_words(blk[2], blk[3], bits);
+                long long zl = capability_small_words(blk[0], blk[1], bits);
+                long long zh = capability_two_words(bits) ? capability_small_words(blk[2], blk[3], bits)
+                                                          : centered_binomial_words(blk[1], 0u, bits);
                 u32 xl[A::rl], xh[A::rl];
                 if constexpr (A::mont_plaintext)
                 {

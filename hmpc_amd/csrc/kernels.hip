@@ -1027,8 +1027,9 @@ namespace hmpc_amd
         u32 counter = static_cast<u32>((poly << log_n) + j);
         u32 blk[16];
         chacha20_block(blk, cs.key, counter, cs.nonce[0], cs.nonce[1], cs.nonce[2]);
-        int a = centered_binomial_words(blk[0], blk[1], cs.bits);
-        int b = cs.bits <= 32 ? centered_binomial_words(blk[1], 0u, cs.bits) : centered_binomial_words(blk[2], blk[3], cs.bits);
+        int a = capability_small_words(blk[0], blk[1], cs.bits);
+        int b = capability_two_words(cs.bits) ? capability_small_words(blk[2], blk[3], cs.bits)
+                                              : centered_binomial_words(blk[1], 0u, cs.bits);
         mod_from_small_signed<L>(lo, a, r2, P);
         mod_from_small_signed<L>(hi, b, r2, P);
     }

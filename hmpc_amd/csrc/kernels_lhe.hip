@@ -220,8 +220,8 @@ namespace hmpc_amd
             case cap_centered_binomial:
             {
                 u32 w0 = g.next();
-                u32 w1 = C.bits > 32 ? g.next() : 0u;
-                mod_from_small_signed<L>(r, centered_binomial_words(w0, w1, C.bits), r2.v, P);
+                u32 w1 = capability_two_words(C.bits) ? g.next() : 0u;
+                mod_from_small_signed<L>(r, capability_small_words(w0, w1, C.bits), r2.v, P);
                 return;
             }
             case cap_binomial:

@@ -14,6 +14,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhmpc_hip.so")
+# capability-stream width that draws uniform ternary values (HMPC_BITS_TERNARY)
+BITS_TERNARY = 0
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p

@@ -223,6 +223,14 @@ int hmpc_cipher_decrypt(hmpc_context ctx, int limbs, size_t count, const uint32_
 int hmpc_sample_uniform(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream);
 int hmpc_sample_cbd(hmpc_modulus mod, const uint32_t* key8, uint64_t offset, int bits, size_t count, uint32_t* out, void* stream);
 int hmpc_sample_words(hmpc_context ctx, const uint32_t* key8, uint64_t offset, size_t count, uint32_t* out, void* stream);
+/* Width of a capability-stream draw that selects the uniform ternary distribution over {-1, 0, 1}
+ * instead of a centred binomial. hmpc has no such sampler: this is the "ternary uniform" secret and
+ * ephemeral key of the common BGV parameter sets. Each draw takes two words w0, w1 of the stream
+ * (as a centred binomial of more than 32 bits does) and gives floor(3 (w1 2^32 + w0) / 2^64) - 1,
+ * within 2^-63 of uniform. Accepted wherever a capability width is (hmpc_sample_cbd_ntt, the BGV
+ * entry points' bits_u / bits_v / bits_w, HMPC_CENTERED_BINOMIAL in hmpc_sample_capability);
+ * explicit-key samplers (hmpc_sample_cbd) follow hmpc and reject it. */
+#define HMPC_BITS_TERNARY 0
 /* Capability-path centred binomial consumed by a forward NTT (hmpc/comp/queue.hpp:185-253,
  * hmpc/expr/number_theoretic_transform.hpp:320-333): uses the context's key and current
  * nonce, then advances the nonce by n * N. Output is NTT(sample) (evaluation domain). */
@@ -246,7 +254,8 @@ int hmpc_polymul_noalloc(hmpc_modulus mod, int log_n, size_t n, const uint32_t* 
 /* BGV public-key encryption of n messages (hmpc/expr/crypto/lhe/enc.hpp:19-69 with the
  * default randomness of hmpc/expr/crypto/lhe/randomness.hpp:80-110):
  *   u, v, w  centred binomial with bits_u/v/w random bits (hmpc's defaults: 2, 40, 40 —
- *            variances 1/2, 10, 10), drawn on the context's capability stream with nonces
+ *            variances 1/2, 10, 10; width HMPC_BITS_TERNARY draws uniform ternary values;
+ *            eta = 10 in the usual sense is 20 bits), drawn on the context's capability stream with nonces
  *            nonce, nonce + nN, nonce + 2nN; the context nonce advances by 3nN
  *   c0 = A*NTT(u) + NTT(v)*p + NTT(m),  c1 = B*NTT(u) + NTT(w)*p
  * pk_a, pk_b: [L][N] in NTT form; messages: n*N u32 words lifted as unsigned values;
@@ -286,7 +295,8 @@ int hmpc_cast(hmpc_modulus from, hmpc_modulus to, size_t count, const uint32_t* 
  *   HMPC_LAYOUT_INTT     a following inverse NTT: stream b N + 2t gives 2t, then 2t + 1 (:480-489)
  * Distributions (bits):
  *   HMPC_UNIFORM            uniform<mod q> with 80-bit drowning (hmpc/random/uniform.hpp:61-66; bits unused)
- *   HMPC_CENTERED_BINOMIAL  bits = 4 x variance, even, 2..64 (hmpc/random/binomial.hpp:88-128)
+ *   HMPC_CENTERED_BINOMIAL  bits = 4 x variance, even, 2..64 (hmpc/random/binomial.hpp:88-128), or
+ *                        HMPC_BITS_TERNARY for uniform ternary values (not in hmpc)
  *   HMPC_BINOMIAL           popcount of bits = 1..256 random bits (hmpc/random/binomial.hpp:30-58)
  *   HMPC_DROWN_SIGNED / _UNSIGNED  drown_uniform of width bits = 1..384 (hmpc/random/uniform.hpp:34-47;
  *                        hmpc_drown_bits gives the width for a bound) */

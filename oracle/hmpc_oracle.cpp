@@ -465,6 +465,18 @@ namespace oracle
         return pop - bits / 2;
     }
 
+    // A capability draw: the centred binomial above, or at width 0 the uniform ternary draw that
+    // hmpc does not have (the C3 workload's "ternary uniform" secret and ephemeral key): two
+    // words w0, w1 as w = w1 * 2^32 + w0, then floor(3 w / 2^64) - 1, within 2^-63 of uniform.
+    inline long long capability_small(number_generator& g, int bits)
+    {
+        if (bits != 0)
+            return centered_binomial(g, bits);
+        unsigned __int128 w = g.next();
+        w |= static_cast<unsigned __int128>(g.next()) << 32;
+        return static_cast<long long>((w * 3) >> 64) - 1;
+    }
+
     // hmpc/random/uniform.hpp:34-66 + hmpc/ints/mod.hpp:210-215:
     // draw bit_size + 80 bits, then REDC with r = max(L, W) rounds.
     inline void uniform_mod(u32* out, number_generator& g, modulus const& m)
@@ -765,7 +777,7 @@ namespace oracle
             uniform_mod(out, g, m);
             return;
         case dist_centered_binomial:
-            mod_from_small_signed(out, centered_binomial(g, bits), m);
+            mod_from_small_signed(out, capability_small(g, bits), m);
             return;
         case dist_binomial:
         {
@@ -1235,9 +1247,9 @@ extern "C"
             {
                 number_generator g(key, nonce, static_cast<u32>(i * N + tid));
                 u32 x[max_limbs];
-                mod_from_small_signed(x, centered_binomial(g, bits), m);
+                mod_from_small_signed(x, capability_small(g, bits), m);
                 store(out, total, i * N + tid, x, m.limbs);
-                mod_from_small_signed(x, centered_binomial(g, bits), m);
+                mod_from_small_signed(x, capability_small(g, bits), m);
                 store(out, total, i * N + tid + half, x, m.limbs);
             }
         return 0;

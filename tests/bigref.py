@@ -138,7 +138,10 @@ def sample_mont(st, dist, bits, q):
         W = (b + 31) // 32
         v = st.bits(b)
         return v * pow(1 << (32 * max(L, W)), -1, q) % q
-    if dist == "centered_binomial":
+    if dist == "centered_binomial" and bits == 0:
+        # uniform ternary (not in hmpc): floor(3 w / 2^64) - 1 for the next 64 bits w
+        val = (3 * st.bits(64) >> 64) - 1
+    elif dist == "centered_binomial":
         val = bin(st.bits(bits)).count("1") - bits // 2
     elif dist == "binomial":
         val = bin(st.bits(bits)).count("1")

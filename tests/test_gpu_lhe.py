@@ -74,6 +74,7 @@ def test_cast_matches_oracle(ctx, src, dst):
 
 
 SAMPLER_CASES = [("uniform", 0), ("centered_binomial", 2), ("centered_binomial", 40), ("centered_binomial", 64),
+                 ("centered_binomial", hmpc_amd.BITS_TERNARY),
                  ("binomial", 20), ("binomial", 200), ("drown_signed", oracle.drown_bits(1 << 64, True)),
                  ("drown_unsigned", oracle.drown_bits(12345, False)), ("drown_signed", 300)]
 

@@ -169,7 +169,8 @@ def test_from_u32_and_to_unsigned(ctx):
     assert oracle.planar_to_ints(back) == [int(w) for w in words]
 
 
-@pytest.mark.parametrize("q,log_n,n,bits", [(Q64, 12, 3, 40), (Q128B, 14, 2, 2), (Q128A, 6, 3, 64), (P150, 10, 2, 40)])
+@pytest.mark.parametrize("q,log_n,n,bits", [(Q64, 12, 3, 40), (Q128B, 14, 2, 2), (Q128A, 6, 3, 64), (P150, 10, 2, 40),
+                                            (Q128B, 12, 2, hmpc_amd.BITS_TERNARY)])
 def test_capability_cbd_ntt(ctx, q, log_n, n, bits):
     gm, om = mods(ctx, q)
     say(f"cbd_ntt q={hex(q)[:12]} log_n={log_n} n={n} bits={bits}")
@@ -223,6 +224,42 @@ def test_bgv_encrypt_matches_oracle_and_decrypts(ctx, q, log_n, n):
     dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
     assert np.array_equal(dec, msgs)
     assert np.array_equal(dec, om.bgv_decrypt(log_n, s, want0, want1))
+
+
+# The common BGV parameter set: s and u uniform ternary, errors centred binomial with eta = 10 (20
+# bits): on the tile launches, the job kernel, the portable kernels and the plain path.
+@pytest.mark.parametrize("q,log_n,n", [(Q128B, 14, 3), (Q64, 13, 2), (Q128A, 12, 2), (P150, 13, 2)])
+def test_bgv_ternary_eta10_randomness(ctx, q, log_n, n):
+    gm, om = mods(ctx, q)
+    say(f"bgv ternary/eta10 q={hex(q)[:12]} log_n={log_n} n={n}")
+    N = 1 << log_n
+    bits = (hmpc_amd.BITS_TERNARY, 20, 20)
+    t_mont = om.from_unsigned(1 << 32)
+    B = om.sample_uniform(KEY, 0, N)
+    s = om.ntt(om.sample_capability(KEY, [1, 0, 0], "centered_binomial", hmpc_amd.BITS_TERNARY, "element", log_n, 1), log_n)
+    e = om.ntt(om.sample_cbd(KEY, 2, 20, N), log_n)
+    A = om.pointwise("add", om.pointwise("mul", B, s), om.pointwise("mul", e, oracle.ints_to_planar([t_mont] * N, om.limbs)))
+    msgs = oracle.sample_words(KEY, 3, n * N)
+    enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
+    want0, want1, nonce = om.bgv_encrypt(log_n, A, B, msgs, t_mont, enc_key, [0, 0, 0], bits=bits, threads=8)
+    dA, dB, dm = gm.upload(A), gm.upload(B), ctx.upload(msgs)
+    pm = oracle.int_to_limbs(t_mont, gm.limbs)
+    gp, _ = mods(ctx, q, 1)
+    fast_r26_gold = q in (Q64, Q128A, Q128B)
+    modes = ([("jobs", 8)] if fast_r26_gold else []) + [("tiles", 8), ("portable", 0), ("plain", 0)]
+    for mode, elems in modes:
+        say(f"  bgv mode={mode} elems={elems}")
+        if mode in ("jobs", "tiles"):
+            gm.set_bgv_schedule(log_n, elems, 1 if mode == "tiles" else 0)
+        ctx.set_rng(enc_key, [0, 0, 0])
+        g = gp if mode == "portable" else gm
+        c0, c1 = g.bgv_encrypt(log_n, n, dA, dB, dm, pm, bits=bits, plain=mode == "plain")
+        assert np.array_equal(gm.download(c0, n * N), want0), f"c0 mode={mode}"
+        assert np.array_equal(gm.download(c1, n * N), want1), f"c1 mode={mode}"
+        assert ctx.rng[1].tolist() == nonce.tolist()
+    gm.set_bgv_schedule(log_n, 0, -1)
+    dec = gm.bgv_decrypt(log_n, n, gm.upload(s), c0, c1).to_numpy(n * N)
+    assert np.array_equal(dec, msgs)
 
 
 def test_empty_inputs_are_no_ops(ctx):

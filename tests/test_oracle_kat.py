@@ -373,11 +373,13 @@ def test_bgv_encrypt_decrypt_roundtrip(q, log_n):
     assert np.array_equal(c0, want0)
 
 
+@pytest.mark.parametrize("bits", [(2, 40, 40), (0, 20, 20)])
 @pytest.mark.parametrize("q", [Q128B, Q64])
-def test_bgv_u32_messages_closed_form(q):
+def test_bgv_u32_messages_closed_form(q, bits):
     """The 32-bit-message encryption (examples/bgv.cpp:66-69 lift, t = 2^32 as a mod q constant) against
     python big ints: c0 = A U + t V + DFT(m), c1 = B U + t W with U, V, W from the textbook ChaCha
-    words in the ntt layout (nonces 0, nN, 2nN)."""
+    words in the ntt layout (nonces 0, nN, 2nN), with hmpc's randomness (2, 40, 40 bits) and with a
+    uniform ternary u and 20-bit (eta = 10) v, w."""
     m = oracle.Modulus(q)
     R = 1 << (32 * m.limbs)
     log_n, n = 3, 2
@@ -389,13 +391,14 @@ def test_bgv_u32_messages_closed_form(q):
     t = (1 << 32) % q
     key = [42, 0, 0, 0, 0, 0, 0, 1]
     c0, c1, _ = m.bgv_encrypt(log_n, oracle.ints_to_planar([a * R % q for a in A], m.limbs),
-                              oracle.ints_to_planar([b * R % q for b in B], m.limbs), msgs, t * R % q, key, [0, 0, 0])
+                              oracle.ints_to_planar([b * R % q for b in B], m.limbs), msgs, t * R % q, key, [0, 0, 0],
+                              bits=bits)
     phi = python_root_search(q, log_n + 1)[0] * pow(R, -1, q) % q
     def dft_samples(k, bits):
         mont = capability_samples(key, [k * n * N, 0, 0], "centered_binomial", bits, "ntt", log_n, n, q)
         plain = [v_ * pow(R, -1, q) % q for v_ in mont]
         return [negacyclic_dft(plain[b * N:(b + 1) * N], phi, q) for b in range(n)]
-    U, V, W = dft_samples(0, 2), dft_samples(1, 40), dft_samples(2, 40)
+    U, V, W = dft_samples(0, bits[0]), dft_samples(1, bits[1]), dft_samples(2, bits[2])
     Rinv = pow(R, -1, q)
     g0 = [v_ * Rinv % q for v_ in oracle.planar_to_ints(c0)]
     g1 = [v_ * Rinv % q for v_ in oracle.planar_to_ints(c1)]
@@ -472,7 +475,7 @@ def test_cast_closed_form(src, dst):
 
 
 # ------------------------------------------------------------------ capability samplers, every layout
-CAP_CASES = [("uniform", 0), ("centered_binomial", 2), ("centered_binomial", 40), ("binomial", 20),
+CAP_CASES = [("uniform", 0), ("centered_binomial", 2), ("centered_binomial", 40), ("centered_binomial", 0), ("binomial", 20),
              ("drown_signed", oracle.drown_bits(1 << 64, True)), ("drown_unsigned", oracle.drown_bits(12345, False)),
              ("drown_signed", 300)]
 
