@@ -75,6 +75,7 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
     build_facade_test()
+    build_hmpc_source_test()
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -94,6 +95,23 @@ def build_facade_test() -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("facade test build failed")
     return FACADE_TEST_BIN
+
+
+HMPC_SOURCE_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "hmpc_source_test.cpp")
+HMPC_SOURCE_TEST_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "hmpc_source_test")
+
+
+def build_hmpc_source_test() -> str:
+    """The C++20 program written in hmpc's spelling over include/hmpc_amd/hmpc.hpp (run by
+    tests/test_gpu_hmpc_source.py)."""
+    root = os.path.dirname(HERE)
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{os.path.join(root, 'include')}", HMPC_SOURCE_TEST_SRC,
+           f"-L{HERE}", "-lhmpc_hip", "-Wl,-rpath,$ORIGIN/../../hmpc_amd", "-o", HMPC_SOURCE_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hmpc source test build failed")
+    return HMPC_SOURCE_TEST_BIN
 
 
 if __name__ == "__main__":

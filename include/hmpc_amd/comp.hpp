@@ -509,6 +509,8 @@ namespace hmpc_amd
 
             // q(expr): one expression into a new tensor
             tensor operator()(expr::expression const& e) { return std::move(eval({e})[0]); }
+            // q(e1, e2, ...) over a list known at run time: one execution plan, one tensor per root
+            std::vector<tensor> evaluate(std::vector<expr::expression> const& roots) { return eval(roots); }
             // q(e1, e2, ...): one execution plan over all of them (shared nodes evaluated once, kernels in
             // hmpc's cache order), a tuple of new tensors
             template<typename... E>
