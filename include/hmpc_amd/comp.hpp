@@ -265,6 +265,16 @@ namespace hmpc_amd
             return make(std::move(n));
         }
 
+        // hmpc::expr::cache(e) (hmpc/expr/cache.hpp:71-82): e may be materialised on its own; the plan
+        // decides (an entry when top-level, reached on several paths or through a broadcast)
+        inline expression cache(expression x)
+        {
+            node n;
+            n.kind = op::cache;
+            n.a = std::move(x);
+            return make(std::move(n));
+        }
+
         // hmpc/expr/reduce.hpp:264-274: sum / product of every element, a one-element tensor
         inline expression sum(expression x)
         {
@@ -511,6 +521,15 @@ namespace hmpc_amd
             tensor operator()(expr::expression const& e) { return std::move(eval({e})[0]); }
             // q(e1, e2, ...) over a list known at run time: one execution plan, one tensor per root
             std::vector<tensor> evaluate(std::vector<expr::expression> const& roots) { return eval(roots); }
+            // hmpc::expr::generate_execution_cache(e1, e2, ...) (hmpc/expr/cache.hpp:233-253): the
+            // kernels q(e1, e2, ...) would run, in order, without running them
+            struct cache_entry
+            {
+                expr::op kind;
+                std::size_t n;  // polynomials (or elements)
+                int log_n;
+            };
+            std::vector<cache_entry> execution_cache(std::vector<expr::expression> const& roots);
             // q(e1, e2, ...): one execution plan over all of them (shared nodes evaluated once, kernels in
             // hmpc's cache order), a tuple of new tensors
             template<typename... E>
@@ -799,6 +818,24 @@ namespace hmpc_amd
 
                 std::vector<node const*> const& order() const { return order_; }
 
+                // hmpc's execution cache of the roots (cache.hpp:233-253) without running it: the
+                // entries in execution order with their kinds and shapes (before this library's fused
+                // products, which hmpc does not have)
+                std::vector<std::pair<op, shape_info>> analyze(std::vector<expr::expression> const& roots)
+                {
+                    std::vector<node const*> tops;
+                    for (auto const& r : roots)
+                        tops.push_back(top_level(r));
+                    for (node const* t : tops)
+                        find(t);
+                    for (node const* t : tops)
+                        cache(t, true);
+                    std::vector<std::pair<op, shape_info>> out;
+                    for (node const* e : order_)
+                        out.emplace_back(e->kind, shape(e));
+                    return out;
+                }
+
             private:
                 queue& q_;
                 std::vector<std::unique_ptr<node>> wrappers_;
@@ -1020,6 +1057,9 @@ namespace hmpc_amd
                     {
                     case op::leaf:
                         return x->value;
+                    case op::cache:
+                        // a cache node fused into its ancestor's kernel: its operand's value
+                        return value(x->a.get(), lay, capability_draws);
                     case op::uniform:
                     {
                         tensor t = make(s);
@@ -1179,6 +1219,15 @@ namespace hmpc_amd
         {
             detail::plan p(*this);
             return p.run(roots);
+        }
+
+        inline std::vector<queue::cache_entry> queue::execution_cache(std::vector<expr::expression> const& roots)
+        {
+            detail::plan p(*this);
+            std::vector<cache_entry> out;
+            for (auto const& [kind, s] : p.analyze(roots))
+                out.push_back({kind, s.n, s.log_n});
+            return out;
         }
     }
 }

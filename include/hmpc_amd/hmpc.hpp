@@ -756,6 +756,7 @@ namespace hmpc
                 capability,
                 cast,
                 unsqueeze,
+                cache,
             };
 
             struct node
@@ -854,6 +855,13 @@ namespace hmpc
             n->shape = e.shape();
             n->q = hmpc::detail::value_info<Target>::element_type::modulus_words();
             return {std::move(n)};
+        }
+
+        // hmpc/expr/cache.hpp:71-82: e may be materialised on its own (the plan decides)
+        template<typename T>
+        expression<T> cache(expression<T> const& e)
+        {
+            return detail::unary<T>(detail::kind::cache, e.node, e.shape());
         }
 
         // hmpc/expr/unsqueeze.hpp: a new axis of extent 1 at `axis` (negative: from the end)
@@ -1133,6 +1141,34 @@ namespace hmpc
                 return it->second;
             }
 
+            // hmpc::expr::generate_execution_cache(e1, e2, ...) (hmpc/expr/cache.hpp:233-253), which
+            // needs the queue here to register moduli and bind tensors: the kernels queue(e1, e2, ...)
+            // would run, in order, without running them; entry i has shape {polynomials}
+            struct execution_cache_entry
+            {
+                hmpc::size polynomials;
+                hmpc::shape shape() const { return hmpc::shape{polynomials}; }
+            };
+            struct execution_cache_result
+            {
+                hmpc::size size = 0;
+                std::vector<execution_cache_entry> entries;
+                template<typename T, T I>
+                execution_cache_entry const& get(constant<T, I>) const { return entries.at(static_cast<hmpc::size>(I)); }
+            };
+            template<typename... E>
+            execution_cache_result execution_cache(expr::expression<E> const&... es)
+            {
+                std::vector<hmpc_amd::expr::expression> roots;
+                memo_map memo;
+                (roots.push_back(translate(es.node.get(), memo)), ...);
+                execution_cache_result r;
+                for (auto const& e : rq_.execution_cache(roots))
+                    r.entries.push_back({e.n});
+                r.size = r.entries.size();
+                return r;
+            }
+
             // queue(e): the value; queue(e1, e2, ...): a tuple. Expression arguments share one execution
             // plan (hmpc's cache order and nonces); enc, ciphertext and dec arguments run after it, in
             // argument order.
@@ -1231,6 +1267,9 @@ namespace hmpc
                 case kind::unsqueeze:
                     // extents only: the same values
                     r = translate(n->a.get(), memo);
+                    break;
+                case kind::cache:
+                    r = hmpc_amd::expr::cache(translate(n->a.get(), memo));
                     break;
                 }
                 memo.emplace(n, r);

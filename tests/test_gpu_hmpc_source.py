@@ -9,6 +9,10 @@ hmpc's own code; every tensor it writes is recomputed here with the oracle, bit 
   bgv example   examples/bgv.cpp:25-90 with n = 4 (150-bit q, p = 0x89d046e0ae640001 as a constant,
                 u shared by c0 and c1)
   ntt example   examples/ntt.cpp:22-55 with n = 2 (125-bit p, N = 2^20)
+
+The program also asserts, on the planner, the execution-cache sizes of hmpc's cache test
+(tests/expr/cache.cpp:13-57) and that INTT(NTT(x)) and dec(enc(x)) return their inputs; it exits
+non-zero when any of those fails.
 """
 
 import os
