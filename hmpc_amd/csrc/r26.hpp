@@ -161,50 +161,66 @@ namespace hmpc_amd
     // Montgomery product a b 2^-130 mod q. Signed: a has signed digits (|a_j| < 2^31), else limbs
     // in [0, 2^31); b has limbs in [0, 2^27). Result digits 0..3 in [0, 2^26); the top limb is
     // signed for a signed a, < 2^22 otherwise (bounds in the header).
+    //
+    // Product scanning: column k (weight 2^(26 k)) is accumulated after column k - 1 is finished,
+    // starting from that column's carry, so every carry enters as the addend of a 64-bit mad
+    // instead of a separate 64-bit add: 25 product mads, per reduction round one v_mul_lo_u32, the
+    // m c mad and one 64-bit shift, and per result limb a mask and a shift (54 instructions against
+    // 62 when all columns are summed first and carried afterwards; the results are identical).
+    // An empty register constraint on a column sum: the compiler cannot reassociate the sum
+    // across it, so each accumulation stays a mad whose addend is the running sum (left free, it
+    // moves the carry to the end of the column as a separate 64-bit add). No-op on the host.
+    template<typename T>
+    HMPC_HD void r26_pin(T& t)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm("" : "+v"(t));
+#else
+        (void)t;
+#endif
+    }
+
     template<bool Signed>
     HMPC_HD void r26_mont_t(u32 (&r)[5], u32 const (&a)[5], u32 const (&b)[5], p26 const& P)
     {
         using acc = typename std::conditional<Signed, long long, u64>::type;
-        acc T[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            T[k] = 0;
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-            {
-                if constexpr (Signed)
-                    T[i + j] = static_cast<long long>(static_cast<int>(a[i])) * static_cast<long long>(static_cast<int>(b[j])) +
-                               T[i + j];
-                else
-                    T[i + j] = static_cast<u64>(a[i]) * b[j] + T[i + j];
-            }
         // q in radix 2^26: limb 0 = c, limb 4 = 2^20, others 0. Only the last quotient digit is
         // masked to 26 bits: with m_0..m_3 < 2^32 the quotient M = sum m_i 2^(26 i) stays below
         // 2^130 + 2^111, so the result (T + M q) / 2^130 keeps its bound and each round saves the
         // mask. For a signed a, m q is added as unsigned 64-bit products onto the two's-complement
-        // column sums (exact: no sum leaves (-2^62, 2^62)); the low 26 bits of T_i + m_i c are
-        // zero, so the (arithmetic) shift carries exactly.
+        // column sums (exact: no sum leaves (-2^62, 2^62)); the low 26 bits of T_k + m_k c are
+        // zero, so the (arithmetic) shift carries exactly. Column k >= 4 also takes m_(k-4) 2^20,
+        // the top limb of m_(k-4) q.
+        u32 m[5];
+        acc carry = 0;
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int k = 0; k < 9; ++k)
         {
-            u32 m = static_cast<u32>(T[i]) * P.n26;
-            if (i == 4)
-                m &= m26;
-            T[i] = static_cast<acc>(static_cast<u64>(m) * P.c + static_cast<u64>(T[i]));
-            T[i + 4] = static_cast<acc>(static_cast<u64>(m) * P.t20 + static_cast<u64>(T[i + 4]));
-            T[i + 1] += T[i] >> 26;
+            acc t = carry;
+#pragma unroll
+            for (int i = (k > 4 ? k - 4 : 0); i <= (k < 4 ? k : 4); ++i)
+            {
+                if constexpr (Signed)
+                    t = static_cast<long long>(static_cast<int>(a[i])) * static_cast<long long>(static_cast<int>(b[k - i])) + t;
+                else
+                    t = static_cast<u64>(a[i]) * b[k - i] + t;
+                r26_pin(t);
+            }
+            if (k >= 4)
+                t = static_cast<acc>(static_cast<u64>(m[k - 4]) * P.t20 + static_cast<u64>(t));
+            if (k < 5)
+            {
+                u32 mk = static_cast<u32>(t) * P.n26;
+                if (k == 4)
+                    mk &= m26;
+                m[k] = mk;
+                t = static_cast<acc>(static_cast<u64>(mk) * P.c + static_cast<u64>(t));
+            }
+            else
+                r[k - 5] = static_cast<u32>(t) & m26;
+            carry = t >> 26;
         }
-        acc c = T[5];
-        r[0] = static_cast<u32>(c) & m26;
-        c = T[6] + (c >> 26);
-        r[1] = static_cast<u32>(c) & m26;
-        c = T[7] + (c >> 26);
-        r[2] = static_cast<u32>(c) & m26;
-        c = T[8] + (c >> 26);
-        r[3] = static_cast<u32>(c) & m26;
-        r[4] = static_cast<u32>(c >> 26);
+        r[4] = static_cast<u32>(carry);
     }
 
     // non-negative operands (inverse transforms, point-wise products of folded values)
