@@ -28,6 +28,7 @@
 #include "r26.hpp"
 #include "tile_geometry.hpp"
 
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 #include <utility>
@@ -1233,7 +1234,302 @@ namespace hmpc_amd
             tile_store<A, T, true>(lds, which ? wb : wa, io, ws_fold_columns(G.log_n), P);
         }
 
+        // ------------------------------------------------------------------ streaming column passes
+        // The column passes of the radix-2^26 product (columns of 2^7, 32 columns per tile, 8
+        // elements per thread) as persistent workgroups that walk the tiles t = blockIdx.x,
+        // blockIdx.x + gridDim.x, ... and load tile t + gridDim.x into registers while tile t is
+        // transformed. With one tile per workgroup the two workgroups of a CU load, transform and
+        // store in step, so memory and VALU time add up (0.52 + 0.53 ms for the forward pass of 512
+        // pairs at N = 2^16, 0.85-0.89 ms together); streamed, they overlap (0.67 ms, same results).
+        //
+        // The prefetch only overlaps if no later vector load waits behind it (vector loads complete
+        // in issue order), so these passes take no twiddle from a vector load inside the loop:
+        //   - full-width rounds (3 stages) have one twiddle per stage and wave in this geometry
+        //     (forward stages 0-5, inverse stages 1-6: the unit's top bits F are 0 or the wave
+        //     index), loaded with scalar loads from lane 0's index;
+        //   - the width-1 round (forward stage 6, inverse stage 0) has two twiddles per wave (lanes
+        //     0-31 and 32-63) that do not depend on the tile (base 1): the forward pass loads them
+        //     once per workgroup and keeps them in VGPRs, the inverse pass (whose 5-word prefetch
+        //     leaves no room for them) takes both with scalar loads and selects per lane.
+        // tests/cpp/jobs_index_check.cpp checks the per-wave uniformity on the CPU.
+        // Round-0 elements are loaded straight into registers and the last round's stored straight
+        // from them (each wave access is two 128-byte column segments per limb plane, as in
+        // tile_load), so LDS carries only the two exchanges between rounds.
+
+        // a twiddle whose index is the same on every lane of the wave
+        template<typename A>
+        __device__ __forceinline__ void tw_get_wave(u32 (&w)[A::rl], u32 const* __restrict__ tw, u32 idx)
+        {
+            A::tw_get(w, tw, __builtin_amdgcn_readfirstlane(idx));
+        }
+
+        // a twiddle whose index takes at most two values on the wave, lane 0's and lane 63's
+        // (two scalar loads and a per-lane select)
+        template<typename A>
+        __device__ __forceinline__ void tw_get_wave2(u32 (&w)[A::rl], u32 const* __restrict__ tw, u32 idx)
+        {
+            u32 const i0 = __builtin_amdgcn_readfirstlane(idx);
+            u32 const i1 = __builtin_amdgcn_readlane(idx, 63);
+            u32 w0[A::rl], w1[A::rl];
+            A::tw_get(w0, tw, i0);
+            A::tw_get(w1, tw, i1);
+            bool const first = idx == i0;
+#pragma unroll
+            for (int l = 0; l < A::rl; ++l)
+                w[l] = first ? w0[l] : w1[l];
+        }
+
+        // stages II.. of a full-width round (one unit per thread) with per-wave twiddles; with
+        // `scaled` the inverse's last stage is the scaled final stage (twiddles 0 and 1)
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void stages_wave(u32 (&x)[E][A::rl], u32 (&twr)[1][(1 << W) - 1][A::rl], u32 X,
+                                                    u32 const* __restrict__ tw, typename A::params const& P, int g0,
+                                                    bool scaled)
+        {
+            if constexpr (II < W)
+            {
+                constexpr int count = Inv ? (1 << (W - II - 1)) : (1 << II);
+                constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
+                if (Inv && II == W - 1 && scaled)
+                {
+                    u32 w0[A::rl], w1[A::rl];
+                    tw_get_wave<A>(w0, tw, 0u);
+                    tw_get_wave<A>(w1, tw, 1u);
+                    stage<A, E, Inv, W, II, true>(x, twr, w0, w1, P, g0);
+                }
+                else
+                {
+                    u32 const z[A::rl] = {};
+#pragma unroll
+                    for (int m = 0; m < count; ++m)
+                        tw_get_wave<A>(twr[0][first + m], tw, tw_index_round<Inv, W, II>(X, m));
+                    stage<A, E, Inv, W, II, false>(x, twr, z, z, P, g0);
+                }
+                stages_wave<A, E, Inv, W, II + 1>(x, twr, X, tw, P, g0, scaled);
+            }
+        }
+
+        template<typename A, int E, bool Inv, int W>
+        __device__ __forceinline__ void round_wave(u32 (&x)[E][A::rl], tile_desc const& td, int a, u32 const* __restrict__ tw,
+                                                   typename A::params const& P, int g0, bool scaled)
+        {
+            static_assert((1 << W) == E, "full-width round");
+            u32 twr[1][(1 << W) - 1][A::rl];
+            u32 xr[1];
+            round_bases<A, E, Inv, W>(xr, td, a, 1u, 0u);
+            stages_wave<A, E, Inv, W, 0>(x, twr, xr[0], tw, P, g0, scaled);
+        }
+
+        // tile element of element k of this thread's unit g in round (a, W)
+        template<typename A, int E, int W>
+        __device__ __forceinline__ int unit_elem(tile_desc const& td, int a, bool inv, int g, int k)
+        {
+            unit_t un = unit_of(td, a, W, inv, unit_index<A, E, W>(g));
+            return elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, td.s, inv));
+        }
+
+        // the width-1 round's per-lane twiddles, loaded once; the empty asm uses make the loads
+        // complete before the tile loop (inside it they would wait behind the prefetch)
+        template<typename A, int E, bool Inv>
+        __device__ __forceinline__ void resident_w1(u32 (&tw1)[E / 2][1][A::rl], tile_desc const& td, int a,
+                                                    u32 const* __restrict__ tw)
+        {
+            u32 xr[E / 2];
+            round_bases<A, E, Inv, 1>(xr, td, a, 1u, 0u);
+#pragma unroll
+            for (int g = 0; g < E / 2; ++g)
+                A::tw_get(tw1[g][0], tw, tw_index_round<Inv, 1, 0>(xr[g], 0));
+#pragma unroll
+            for (int g = 0; g < E / 2; ++g)
+#pragma unroll
+                for (int l = 0; l < A::rl; ++l)
+                    asm volatile("" ::"v"(tw1[g][0][l]));
+        }
+
+        // forward column pass of both product inputs, tile t = 2 * column tile + operand (as
+        // k_polymul_cols), rounds (stages 0-2) (3-5) (6)
+        template<typename A, int E, int S>
+        __global__ void __launch_bounds__(tile / E, 4) k_polymul_cols_stream(u32 const* __restrict__ a, u32 const* __restrict__ b,
+                                                                             u32* __restrict__ wa, u32* __restrict__ wb,
+                                                                             u32 const* __restrict__ tw, pass_geom G, u64 n,
+                                                                             typename A::params P, u32 tiles)
+        {
+            static_assert(E == 8 && S == 7, "streaming column geometry: 2^7-point columns, 8 elements per thread");
+            extern __shared__ __align__(16) u32 lds[];
+            std::size_t const count = static_cast<std::size_t>(n) << G.log_n;
+            tile_desc const td{S, tile_log - S, false};
+            u32 tw1[E / 2][1][A::rl];
+            resident_w1<A, E, false>(tw1, td, 6, tw);
+            bool const fold = ws_fold_columns(G.log_n);
+            io_desc io;
+            tile_desc tdd;
+            u32 t = blockIdx.x;
+            if (t >= tiles)
+                return;
+            cols_setup(G, t >> 1, count, tdd, io);
+            // round 0: one unit per thread, element k at e0 + unit_offset(k) (disjoint bits, so the
+            // global offsets add: gidx_rel is additive over them)
+            int const e_in = unit_elem<A, E, 3>(td, 0, false, 0, 0);
+            u32 const o_in = gidx_rel(io, e_in);
+            u32 raw[E][4];
+            {
+                u32 const* src = ((t & 1u) ? b : a) + io.base;
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+#pragma unroll
+                    for (int l = 0; l < 4; ++l)
+                        raw[k][l] = ld_off(src + l * count + gidx_rel(io, unit_offset(td, 0, 3, false, k)), o_in);
+            }
+            // last round (stage 6): units g = 0..3 of two elements
+            int const e_out = unit_elem<A, E, 1>(td, 6, false, 0, 0);
+            u32 const o_out = gidx_rel(io, e_out);
+            for (;;)
+            {
+                u32* const dst = ((t & 1u) ? wb : wa) + io.base;
+                u32 x[E][A::rl];
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                    r26_unpack(x[k], raw[k][0], raw[k][1], raw[k][2], raw[k][3]);
+                u32 const tn = t + gridDim.x;
+                io_desc ion;
+                if (tn < tiles)
+                {
+                    cols_setup(G, tn >> 1, count, tdd, ion);
+                    u32 const* src = ((tn & 1u) ? b : a) + ion.base;
+#pragma unroll
+                    for (int k = 0; k < E; ++k)
+#pragma unroll
+                        for (int l = 0; l < 4; ++l)
+                            raw[k][l] = ld_off(src + l * count + gidx_rel(ion, unit_offset(td, 0, 3, false, k)), o_in);
+                }
+                round_wave<A, E, false, 3>(x, td, 0, tw, P, 0, false);
+                __syncthreads();  // the previous tile's last gather is done
+                scatter<A, E, 3>(x, lds, td, 0, false);
+                __syncthreads();
+                gather<A, E, 3>(x, lds, td, 3, false);
+                round_wave<A, E, false, 3>(x, td, 3, tw, P, 0, false);
+                scatter<A, E, 3>(x, lds, td, 3, false);
+                __syncthreads();
+                gather<A, E, 1>(x, lds, td, 6, false);
+                {
+                    u32 const z[A::rl] = {};
+                    stage<A, E, false, 1, 0, false>(x, tw1, z, z, P, 0);
+                }
+#pragma unroll
+                for (int g = 0; g < E / 2; ++g)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                    {
+                        int const d = unit_elem<A, E, 1>(td, 6, false, g, k) - e_out;
+                        A::store_ws(dst + gidx_rel(io, d), count, o_out, x[2 * g + k], fold, P);
+                    }
+                if (tn >= tiles)
+                    break;
+                t = tn;
+                io = ion;
+            }
+        }
+
+        // inverse column pass of the product from the row pass's workspace (raw limbs), with the
+        // N^-1 scaling and canonical output (as k_pass<A, E, 3, true, 7>): rounds (stage 0)
+        // (1-3) (4-6); `off` = log N2, the inverse stages the row pass ran before
+        template<typename A, int E, int S>
+        __global__ void __launch_bounds__(tile / E, 4) k_icols_stream(u32 const* __restrict__ in, u32* __restrict__ out,
+                                                                      u32 const* __restrict__ tw, pass_geom G, u64 n,
+                                                                      typename A::params P, u32 tiles)
+        {
+            static_assert(E == 8 && S == 7, "streaming column geometry: 2^7-point columns, 8 elements per thread");
+            extern __shared__ __align__(16) u32 lds[];
+            std::size_t const count = static_cast<std::size_t>(n) << G.log_n;
+            tile_desc const td{S, tile_log - S, false};
+            int const off = G.log_n2;
+            io_desc io;
+            tile_desc tdd;
+            u32 t = blockIdx.x;
+            if (t >= tiles)
+                return;
+            cols_setup(G, t, count, tdd, io);
+            // round 0 (stage 0): units g = 0..3 of two elements
+            int const e_in = unit_elem<A, E, 1>(td, 0, true, 0, 0);
+            u32 const o_in = gidx_rel(io, e_in);
+            u32 raw[E][A::wsl];
+#pragma unroll
+            for (int g = 0; g < E / 2; ++g)
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    A::load_ws(raw[2 * g + k], in + io.base + gidx_rel(io, unit_elem<A, E, 1>(td, 0, true, g, k) - e_in), count,
+                               o_in);
+            // last round (stages 4-6): one unit per thread
+            int const e_out = unit_elem<A, E, 3>(td, 4, true, 0, 0);
+            u32 const o_out = gidx_rel(io, e_out);
+            for (;;)
+            {
+                u32* const dst = out + io.base;
+                u32 x[E][A::rl];
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+#pragma unroll
+                    for (int l = 0; l < A::rl; ++l)
+                        x[k][l] = raw[k][l];
+                u32 const tn = t + gridDim.x;
+                io_desc ion;
+                if (tn < tiles)
+                {
+                    cols_setup(G, tn, count, tdd, ion);
+#pragma unroll
+                    for (int g = 0; g < E / 2; ++g)
+#pragma unroll
+                        for (int k = 0; k < 2; ++k)
+                            A::load_ws(raw[2 * g + k], in + ion.base + gidx_rel(ion, unit_elem<A, E, 1>(td, 0, true, g, k) - e_in),
+                                       count, o_in);
+                }
+                {
+                    // stage 0: two twiddles per wave and unit (lanes 0-31 and 32-63)
+                    u32 const z[A::rl] = {};
+                    u32 tw1[E / 2][1][A::rl];
+                    u32 xr[E / 2];
+                    round_bases<A, E, true, 1>(xr, td, 0, 1u, 0u);
+#pragma unroll
+                    for (int g = 0; g < E / 2; ++g)
+                        tw_get_wave2<A>(tw1[g][0], tw, tw_index_round<true, 1, 0>(xr[g], 0));
+                    stage<A, E, true, 1, 0, false>(x, tw1, z, z, P, off);
+                }
+                __syncthreads();  // the previous tile's last gather is done
+                scatter<A, E, 1>(x, lds, td, 0, true);
+                __syncthreads();
+                gather<A, E, 3>(x, lds, td, 1, true);
+                round_wave<A, E, true, 3>(x, td, 1, tw, P, off + 1, false);
+                scatter<A, E, 3>(x, lds, td, 1, true);
+                __syncthreads();
+                gather<A, E, 3>(x, lds, td, 4, true);
+                round_wave<A, E, true, 3>(x, td, 4, tw, P, off + 4, true);
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                {
+                    int const d = unit_elem<A, E, 3>(td, 4, true, 0, k) - e_out;
+                    A::store_global(dst + gidx_rel(io, d), count, o_out, x[k], true, P);
+                }
+                if (tn >= tiles)
+                    break;
+                t = tn;
+                io = ion;
+            }
+        }
+
         // ------------------------------------------------------------------ host launchers
+        // compute units of the current device (cached per device; 0 if the query fails)
+        inline int device_cus()
+        {
+            static std::atomic<int> cache[64];
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+                return 0;
+            int v = cache[dev].load(std::memory_order_relaxed);
+            if (v == 0 && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+                cache[dev].store(v, std::memory_order_relaxed);
+            return v > 0 ? v : 0;
+        }
+
         // kernel-argument constants of the 32-bit-limb Montgomery policies (Goldilocks, CIOS)
         inline fast_params montgomery_params(transform_args const& T)
         {
@@ -1411,9 +1707,34 @@ namespace hmpc_amd
                 }
                 if ((e = allow_lds(rows_kernel, bytes)) != hipSuccess)
                     return e;
+                // the radix-2^26 column passes stream their tiles through two resident workgroups
+                // per CU (k_polymul_cols_stream, k_icols_stream)
+                bool stream_cols = false;
+                unsigned grid_s = 0;
+                if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                {
+                    int const cus = device_cus();
+                    if (gc.log_n1 == 7 && cus > 0)
+                    {
+                        if ((e = allow_lds(k_polymul_cols_stream<A, E, 7>, bytes)) != hipSuccess ||
+                            (e = allow_lds(k_icols_stream<A, E, 7>, bytes)) != hipSuccess)
+                            return e;
+                        stream_cols = true;
+                        grid_s = 2u * static_cast<unsigned>(cus);
+                    }
+                }
                 {
                     probe_scope ps(s, "fast polymul forward columns");
-                    cols_kernel<<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
+                    if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                    {
+                        if (stream_cols)
+                            k_polymul_cols_stream<A, E, 7><<<grid_s < 2 * grid_c ? grid_s : 2 * grid_c, TH, bytes, s>>>(
+                                a, b, wa, wb, T.fwd_fast, gc, n, P, 2 * grid_c);
+                        else
+                            cols_kernel<<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
+                    }
+                    else
+                        cols_kernel<<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
@@ -1421,7 +1742,16 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");
-                    icols_kernel<<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
+                    if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                    {
+                        if (stream_cols)
+                            k_icols_stream<A, E, 7><<<grid_s < grid_c ? grid_s : grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n,
+                                                                                                      P, grid_c);
+                        else
+                            icols_kernel<<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
+                    }
+                    else
+                        icols_kernel<<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
                 }
                 return hipGetLastError();
             });

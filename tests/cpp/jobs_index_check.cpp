@@ -480,9 +480,101 @@ void swizzles()
     }
 }
 
+// The streaming column passes of the radix-2^26 product (k_polymul_cols_stream, k_icols_stream in
+// hmpc_amd/csrc/fast_impl.hpp): 2^7-point columns, 32 columns per tile, 8 elements per thread, 512
+// threads. Their twiddles come from scalar loads at lane 0's index (full-width rounds: forward
+// stages 0-5, inverse stages 1-6), which needs the index to be the same on every lane of a wave,
+// and from a lane-0 / lane-63 pair (the inverse's stage 0), which needs at most those two values
+// per wave; their round-0 loads and last-round stores address element e0 + d as the offset of e0
+// plus the offset of d. Checked for every thread, unit and element at N = 2^14 .. 2^17 (every
+// degree whose column length is 2^7 in the radix-2^26 product), together with: each pass reads and
+// writes every tile element exactly once, inside the tile's polynomial.
+template<int W>
+int unit_elem_at(tile_desc const& td, int a, bool inv, int t, int g, int k)
+{
+    unit_t un = unit_of(td, a, W, inv, unit_index_at<8, W, false>(t, g));
+    return elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, td.s, inv));
+}
+
+template<int W>
+u32 round_x(tile_desc const& td, int a, bool inv, int t, int g)
+{
+    unit_t un = unit_of(td, a, W, inv, unit_index_at<8, W, false>(t, g));
+    return inv ? tw_round_base<true, W>(1u, td.s, a, un.F) : tw_round_base<false, W>(1u, td.s, a, un.F);
+}
+
+template<int W>
+void elements_once(tile_desc const& td, int a, bool inv, io_desc const& io, std::size_t poly_lo, std::size_t N, char const* what)
+{
+    std::vector<int> seen(tile, 0);
+    for (int t = 0; t < 512; ++t)
+    {
+        int const e0 = unit_elem_at<W>(td, a, inv, t, 0, 0);
+        for (int g = 0; g < (8 >> W); ++g)
+            for (int k = 0; k < (1 << W); ++k)
+            {
+                int const e = unit_elem_at<W>(td, a, inv, t, g, k);
+                int const d = e - e0;
+                require(e >= 0 && e < tile && d >= 0 && (e0 & d) == 0, std::string(what) + ": element is not e0 | d");
+                require(gidx_rel(io, e) == gidx_rel(io, e0) + gidx_rel(io, d), std::string(what) + ": offset not additive");
+                std::size_t const gi = io.base + gidx_rel(io, e);
+                require(gi >= poly_lo && gi < poly_lo + N, std::string(what) + ": element outside its polynomial");
+                if (e >= 0 && e < tile)
+                    ++seen[e];
+            }
+    }
+    for (int e = 0; e < tile; ++e)
+        require(seen[e] == 1, std::string(what) + ": tile element not moved exactly once");
+}
+
+void streaming_columns()
+{
+    for (int log_n = 14; log_n <= 17; ++log_n)
+    {
+        pass_geom const G = geom_for(log_n, false);
+        require(G.log_n1 == 7 && G.log_seqs == 5, "streaming columns need 2^7-point columns, 32 per tile");
+        tile_desc const td{7, tile_log - 7, false};
+        std::size_t const N = std::size_t{1} << log_n, n = 2, count = n * N;
+        for (u32 ct = 0; ct < n * G.blocks_per_poly; ++ct)
+        {
+            tile_desc tdd;
+            io_desc io;
+            cols_setup(G, ct, count, tdd, io);
+            require(tdd.s == td.s && tdd.log_seqs == td.log_seqs && !tdd.rows, "column tile geometry");
+            std::size_t const poly_lo = (ct / G.blocks_per_poly) * N;
+            elements_once<3>(td, 0, false, io, poly_lo, N, "forward round-0 loads");
+            elements_once<1>(td, 6, false, io, poly_lo, N, "forward last-round stores");
+            elements_once<1>(td, 0, true, io, poly_lo, N, "inverse round-0 loads");
+            elements_once<3>(td, 4, true, io, poly_lo, N, "inverse last-round stores");
+        }
+        for (int w = 0; w < 8; ++w)
+            for (int lane = 0; lane < 64; ++lane)
+            {
+                int const t = 64 * w + lane;
+                // per-wave uniform rounds (the per-stage indices are functions of X)
+                require(round_x<3>(td, 0, false, t, 0) == round_x<3>(td, 0, false, 64 * w, 0), "forward stages 0-2 not wave-uniform");
+                require(round_x<3>(td, 3, false, t, 0) == round_x<3>(td, 3, false, 64 * w, 0), "forward stages 3-5 not wave-uniform");
+                require(round_x<3>(td, 1, true, t, 0) == round_x<3>(td, 1, true, 64 * w, 0), "inverse stages 1-3 not wave-uniform");
+                require(round_x<3>(td, 4, true, t, 0) == round_x<3>(td, 4, true, 64 * w, 0), "inverse stages 4-6 not wave-uniform");
+                for (int g = 0; g < 4; ++g)
+                {
+                    // the inverse's stage 0: lane 0's or lane 63's index
+                    u32 const x = round_x<1>(td, 0, true, t, g);
+                    require(x == round_x<1>(td, 0, true, 64 * w, g) || x == round_x<1>(td, 0, true, 64 * w + 63, g),
+                            "inverse stage 0 has more than lane 0's and lane 63's twiddles");
+                    // twiddle indices stay inside the 2^7 column entries
+                    require(round_x<1>(td, 6, false, t, g) < 128u && x < 128u, "width-1 round twiddle index");
+                }
+                require(tw_index_round<false, 3, 2>(round_x<3>(td, 3, false, t, 0), 3) < 128u, "forward twiddle index");
+                require(tw_index_round<true, 3, 0>(round_x<3>(td, 1, true, t, 0), 3) < 128u, "inverse twiddle index");
+            }
+    }
+}
+
 int main()
 {
     swizzles();
+    streaming_columns();
     unit_offsets_all<4, 1>();
     unit_offsets_all<4, 2>();
     unit_offsets_all<8, 1>();

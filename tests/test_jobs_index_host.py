@@ -6,7 +6,10 @@ geometry code (hmpc_amd/csrc/tile_geometry.hpp) compiled for the host:
 - the address decompositions every tile kernel relies on: the LDS swizzles are permutations and
   linear over disjoint bits, each unit's elements are element 0 | unit_offset(k), the per-round
   twiddle base gives the per-stage indices, and tile offsets are additive, for every tile geometry,
-  round, unit order and elements-per-thread choice."""
+  round, unit order and elements-per-thread choice;
+- the streaming column passes of the radix-2^26 product: scalar-loaded twiddle indices are the same
+  on every lane of a wave (or lane 0's / lane 63's where two are taken), and their direct round-0
+  loads and last-round stores move every tile element once, inside its polynomial, at N = 2^14..2^17."""
 
 import os
 import subprocess
