@@ -49,6 +49,18 @@ namespace hmpc_amd
             return t;
         }
 
+        // The thread's index in its 4096-element tile: the thread index, or for one-wave workgroups
+        // (wave_blocks) the lane plus 64 times the wave's place in its tile (blockIdx.x modulo the
+        // tile's waves)
+        template<typename A, int E>
+        __device__ __forceinline__ int tile_thread()
+        {
+            if constexpr (A::wave_blocks)
+                return tid_x() + static_cast<int>((blockIdx.x % static_cast<u32>(tile / E / 64)) << 6);
+            else
+                return tid_x();
+        }
+
         // Global accesses of the policies: `base` points at element 0 of a tile (or of an operand's
         // elements) in limb plane 0, planes `count` words apart, and `off` is the element's offset
         // from it (below 2^28 for every tile geometry). The plane pointers are uniform, so the
@@ -74,6 +86,7 @@ namespace hmpc_amd
             static constexpr int tws = 8;   // twiddle table stride in words (5 used)
             static constexpr bool stage_prefetch = true;
             static constexpr bool wave_units = false;  // unit order (unit_index)
+            static constexpr bool wave_blocks = false;  // one wave per workgroup (tile_thread)
             static constexpr bool mont_plaintext = false;  // BGV: p v + m formed as a signed 64-bit integer
             using params = p26;
 
@@ -245,6 +258,23 @@ namespace hmpc_amd
             __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
         };
 
+        // The product's row kernel with one wave per workgroup: its rows are wave-local, so each
+        // wave-sized block of a tile (64 E elements, wave b % (tile / 64 E) of tile b / (tile / 64 E))
+        // runs as a workgroup of its own with a 64 E-slot LDS block (10 KiB at E = 8). Workgroups of
+        // eight waves load, transform and store in step; single waves start and end independently, so
+        // the memory phases of some overlap the arithmetic of others. A wave's tile elements share
+        // their bits above log2(64 E), which the slot order keeps (its XOR terms reach bit 5 only), so
+        // the slot of element e is swizzle_rows(e) modulo 64 E: a one-to-one map of the wave's elements
+        // onto its block, still XOR-linear.
+        struct arith_r26_rows1 : arith_r26_rows
+        {
+            static constexpr bool wave_blocks = true;
+
+            __device__ __forceinline__ static int phys(int e) { return swizzle_rows(e) & 511; }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
+        };
+
         // the BGV row kernels: twiddles from 32-bit offsets as in the product's row kernel
         struct arith_r26_bgv_rows : arith_r26
         {
@@ -287,6 +317,7 @@ namespace hmpc_amd
             static constexpr int tws = 2;
             static constexpr bool stage_prefetch = false;
             static constexpr bool wave_units = false;
+            static constexpr bool wave_blocks = false;
             static constexpr bool mont_plaintext = false;
             using params = fast_params;
             using ops = fast_ops<2, form_goldilocks>;
@@ -408,6 +439,7 @@ namespace hmpc_amd
             static constexpr int tws = L;
             static constexpr bool stage_prefetch = true;
             static constexpr bool wave_units = false;
+            static constexpr bool wave_blocks = false;
             // BGV sampling with any plaintext modulus (given as a Montgomery constant):
             // from_signed, mul_const and add_signed below
             static constexpr bool mont_plaintext = true;
@@ -610,7 +642,7 @@ namespace hmpc_amd
         template<typename A, int E, int W>
         __device__ __forceinline__ int unit_index(int g)
         {
-            return unit_index_at<E, W, A::wave_units>(tid_x(), g);
+            return unit_index_at<E, W, A::wave_units>(tile_thread<A, E>(), g);
         }
 
         // Wave-local tiles (every exchange of a wave stays within its own tile elements) replace
@@ -921,7 +953,7 @@ namespace hmpc_amd
         {
             constexpr int E = tile / T;
             constexpr int step = WL ? 64 : T;
-            int const t = tid_x();
+            int const t = tile_thread<A, E>();
             int const e0 = WL ? (t & ~63) * E + (t & 63) : t;
             // e = e0 + d with d a multiple of the step above e0's bits: slot phys(e0) ^ phys(d)
             int const slot0 = A::phys(e0);
@@ -972,7 +1004,7 @@ namespace hmpc_amd
         {
             constexpr int E = tile / T;
             constexpr int step = WL ? 64 : T;
-            int const t = tid_x();
+            int const t = tile_thread<A, E>();
             int const e0 = WL ? (t & ~63) * E + (t & 63) : t;
             int const slot0 = A::phys(e0);
             u32* const dst_t = dst + io.base;
@@ -1152,7 +1184,11 @@ namespace hmpc_amd
             }
             else
             {
-                u32 hi0 = rows_setup(G, blockIdx.x, count, td, io);
+                // one-wave workgroups: block b serves wave b % (tile / 64 E) of tile b / (tile / 64 E)
+                static_assert(!A::wave_blocks || (E == 8 && S > 0 && (1 << S) <= 64 * E),
+                              "one-wave workgroups need wave-local rows at 8 elements per thread");
+                u32 const tb = A::wave_blocks ? blockIdx.x / static_cast<u32>(tile / E / 64) : blockIdx.x;
+                u32 hi0 = rows_setup(G, tb, count, td, io);
                 if constexpr (S > 0)
                 {
                     td.s = S;
@@ -1705,7 +1741,22 @@ namespace hmpc_amd
                 case 10: rows_kernel = k_polymul<AR, E, false, 10>; break;
                 default: break;
                 }
-                if ((e = allow_lds(rows_kernel, bytes)) != hipSuccess)
+                // radix-2^26 rows of up to 64 E elements: one wave per workgroup (arith_r26_rows1)
+                unsigned rows_grid = grid_r, rows_threads = TH;
+                std::size_t rows_bytes = bytes;
+                if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                {
+                    if (gr.log_n2 >= 7 && gr.log_n2 <= 9)
+                    {
+                        rows_kernel = gr.log_n2 == 7   ? k_polymul<arith_r26_rows1, E, false, 7>
+                                      : gr.log_n2 == 8 ? k_polymul<arith_r26_rows1, E, false, 8>
+                                                       : k_polymul<arith_r26_rows1, E, false, 9>;
+                        rows_grid = grid_r * static_cast<unsigned>(tile / E / 64);
+                        rows_threads = 64;
+                        rows_bytes = bytes / static_cast<std::size_t>(tile / E / 64);
+                    }
+                }
+                if ((e = allow_lds(rows_kernel, rows_bytes)) != hipSuccess)
                     return e;
                 // the radix-2^26 column passes stream their tiles through two resident workgroups
                 // per CU (k_polymul_cols_stream, k_icols_stream)
@@ -1738,7 +1789,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
-                    rows_kernel<<<grid_r, TH, bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
+                    rows_kernel<<<rows_grid, rows_threads, rows_bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");

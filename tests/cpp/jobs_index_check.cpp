@@ -571,10 +571,66 @@ void streaming_columns()
     }
 }
 
+// The product's row kernel with one wave per workgroup (arith_r26_rows1, 8 elements per thread,
+// rows of 2^7 .. 2^9): wave w of a tile may touch only the tile elements [512 w, 512 w + 512) in every
+// round of the forward and inverse row transforms and in its loads and stores, and its LDS slot
+// swizzle_rows(e) & 511 must map those 512 elements one-to-one onto the 512 slots of its block,
+// XOR-linearly (the kernels address slot(e0 | d) as slot(e0) ^ slot(d)).
+template<int W>
+void rows1_rounds(int s)
+{
+    tile_desc td{s, tile_log - s, true};
+    for (int inv = 0; inv < 2; ++inv)
+        for (int a = 0; a + W <= s; ++a)
+            for (int t = 0; t < 512; ++t)
+                for (int g = 0; g < (8 >> W); ++g)
+                {
+                    unit_t un = unit_of(td, a, W, inv != 0, unit_index_at<8, W, true>(t, g));
+                    for (int k = 0; k < (1 << W); ++k)
+                    {
+                        int e = elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, s, inv != 0));
+                        require(e >> 9 == t >> 6, "one-wave rows: a round reaches another wave's elements, s=" + std::to_string(s));
+                    }
+                }
+}
+
+void rows1_blocks()
+{
+    for (int w = 0; w < 8; ++w)
+    {
+        std::vector<int> seen(512, 0);
+        for (int e = 512 * w; e < 512 * w + 512; ++e)
+        {
+            int const p = swizzle_rows(e) & 511;
+            require(!seen[p], "one-wave rows: slot map is not one-to-one on the wave's block");
+            seen[p] = 1;
+            int const free = 511 & ~e;
+            for (int d = free;; d = (d - 1) & free)
+            {
+                require((swizzle_rows(e | d) & 511) == ((swizzle_rows(e) & 511) ^ (swizzle_rows(d) & 511)),
+                        "one-wave rows: slot map not XOR-linear");
+                if (d == 0)
+                    break;
+            }
+        }
+    }
+    // wave-local tile loads and stores: thread t moves elements e0 + 64 i, e0 = (t & ~63) * 8 + (t & 63)
+    for (int t = 0; t < 512; ++t)
+        for (int i = 0; i < 8; ++i)
+            require(((t & ~63) * 8 + (t & 63) + 64 * i) >> 9 == t >> 6, "one-wave rows: tile load leaves the wave's block");
+    for (int s = 7; s <= 9; ++s)
+    {
+        rows1_rounds<1>(s);
+        rows1_rounds<2>(s);
+        rows1_rounds<3>(s);
+    }
+}
+
 int main()
 {
     swizzles();
     streaming_columns();
+    rows1_blocks();
     unit_offsets_all<4, 1>();
     unit_offsets_all<4, 2>();
     unit_offsets_all<8, 1>();
