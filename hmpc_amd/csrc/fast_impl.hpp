@@ -284,6 +284,17 @@ namespace hmpc_amd
             }
         };
 
+        // the BGV row kernel with one wave per workgroup (as arith_r26_rows1 for the product)
+        struct arith_r26_bgv_rows1 : arith_r26_bgv_rows
+        {
+            static constexpr bool wave_units = true;
+            static constexpr bool wave_blocks = true;
+
+            __device__ __forceinline__ static int phys(int e) { return swizzle_rows(e) & 511; }
+
+            __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
+        };
+
         // the arithmetic of the product's row kernel: the radix-2^26 rows are wave-local in the row
         // kernels' LDS slot order (twiddles stage by stage: with signed-digit forward butterflies
         // this ran 2.4% faster than whole-round prefetch)

@@ -413,11 +413,15 @@ namespace hmpc_amd
             extern __shared__ __align__(16) u32 lds[];
             constexpr int T = tile / E;
             std::size_t count = static_cast<std::size_t>(n) << base.log_n;
-            u32 p = blockIdx.x / gr.blocks_per_poly;
-            u32 rt = blockIdx.x - p * gr.blocks_per_poly;
+            // one-wave workgroups: block b serves wave b % (tile / 64 E) of tile b / (tile / 64 E)
+            static_assert(!A::wave_blocks || (E == 8 && S > 0 && (1 << S) <= 64 * E),
+                          "one-wave workgroups need wave-local rows at 8 elements per thread");
+            u32 const tb = A::wave_blocks ? blockIdx.x / static_cast<u32>(tile / E / 64) : blockIdx.x;
+            u32 p = tb / gr.blocks_per_poly;
+            u32 rt = tb - p * gr.blocks_per_poly;
             tile_desc td;
             io_desc is, ik, io;
-            u32 hi0 = rows_setup(gr, blockIdx.x, count, td, is);  // workspace planes of n N
+            u32 hi0 = rows_setup(gr, tb, count, td, is);  // workspace planes of n N
             job_rows_tile(gr, rt, base.in_count, base.in_off, td, ik);  // public key (one polynomial)
             if constexpr (S > 0)
             {
@@ -475,7 +479,22 @@ namespace hmpc_amd
                 case 10: rows_kernel = k_bgv_rows<AB, E, 10>; break;
                 default: break;
                 }
-                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess)
+                // radix-2^26 rows of up to 64 E elements: one wave per workgroup (arith_r26_bgv_rows1)
+                unsigned rows_grid = static_cast<unsigned>(n * gr.blocks_per_poly), rows_threads = tile / E;
+                std::size_t rows_bytes = bytes;
+                if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                {
+                    if (gr.log_n2 >= 7 && gr.log_n2 <= 9)
+                    {
+                        rows_kernel = gr.log_n2 == 7   ? k_bgv_rows<arith_r26_bgv_rows1, E, 7>
+                                      : gr.log_n2 == 8 ? k_bgv_rows<arith_r26_bgv_rows1, E, 8>
+                                                       : k_bgv_rows<arith_r26_bgv_rows1, E, 9>;
+                        rows_grid *= static_cast<unsigned>(tile / E / 64);
+                        rows_threads = 64;
+                        rows_bytes = bytes / static_cast<std::size_t>(tile / E / 64);
+                    }
+                }
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, rows_bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast bgv columns");
@@ -483,7 +502,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast bgv rows");
-                    rows_kernel<<<static_cast<unsigned>(n * gr.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gr, n, P);
+                    rows_kernel<<<rows_grid, rows_threads, rows_bytes, s>>>(base, ws, gr, n, P);
                 }
                 return hipGetLastError();
             });
