@@ -1493,6 +1493,7 @@ namespace
         u32 type = 0;
         int log_n = 0;  // a degree of the chunk (the arithmetic parameters do not depend on it)
         bool uniform = false;  // every job of the chunk has degree 2^log_n
+        int max_log_n = 0;     // the largest degree of the chunk
         std::size_t job_off = 0, njobs = 0;
         // into the map: forward column pass, row pass, inverse column pass (products)
         std::size_t cols_off = 0, cols_blocks = 0, rows_off = 0, rows_blocks = 0, icols_off = 0, icols_blocks = 0;
@@ -1573,6 +1574,7 @@ namespace
                     }
                     J.ws_off = c.ws_words;
                     c.ws_words += words;
+                    c.max_log_n = std::max(c.max_log_n, static_cast<int>(J.log_n));
                     c.cost += static_cast<u64>(J.log_n) << J.log_n;
                     c.njobs += 1;
                     jobs.push_back(J);
@@ -1605,10 +1607,10 @@ namespace
             auto n = [](std::size_t v) { return static_cast<u32>(v); };
             u32* w = ws + c.ws_base;
             if (c.type == job_bgv)
-                HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off,
+                HMPC_HIP(launch_fast_bgv_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, c.max_log_n, d_jobs + c.job_off, d_map + c.cols_off,
                                                n(c.cols_blocks), d_map + c.rows_off, n(c.rows_blocks), w, s));
             else
-                HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, d_jobs + c.job_off, d_map + c.cols_off,
+                HMPC_HIP(launch_fast_polymul_table(c.mod->limbs, T, c.uniform ? c.log_n : 0, c.max_log_n, d_jobs + c.job_off, d_map + c.cols_off,
                                                    n(c.cols_blocks), d_map + c.rows_off, n(c.rows_blocks), d_map + c.icols_off,
                                                    n(c.icols_blocks), w, s));
             return HMPC_OK;

@@ -571,36 +571,75 @@ namespace hmpc_amd
             }
             u32 base0 = (1u << gr.log_n1) + hi0;
             u32 x[E][A::rl];
-            tile_load<A, T, true>(lds, su, is);
-            __syncthreads();
-            tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-            tile_store<A, T, true>(lds, su, is, false, P);
-            __syncthreads();
+            // wave-local rows (the one-wave policy) need no workgroup barrier
+            constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
+            tile_load<A, T, true, WL>(lds, su, is);
+            tile_sync<WL>();
+            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+            tile_store<A, T, true, WL>(lds, su, is, false, P);
+            tile_sync<WL>();
             for (int which = 1; which < 3; ++which)
             {
-                tile_load<A, T, true>(lds, su + which * A::wsl * N, is);
-                __syncthreads();
-                tile_rounds<A, E, false>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                tile_load<A, T, true, WL>(lds, su + which * A::wsl * N, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
                 bool first = which == 1;
-                bgv_emit<A, E>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+                bgv_emit<A, E, WL>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
             }
+        }
+
+        // the row policies of the table kernels: with W1 (radix-2^26 at 8 elements per thread, rows
+        // of at most 2^9 elements) one wave per workgroup, block b serving wave b % 8 of map entry
+        // b / 8
+        template<typename A, bool W1>
+        struct table_rows_policy
+        {
+            using polymul = typename rows_arith<A>::type;
+            using bgv = typename bgv_rows_arith<A>::type;
+        };
+        template<>
+        struct table_rows_policy<arith_r26, true>
+        {
+            using polymul = arith_r26_rows1;
+            using bgv = arith_r26_bgv_rows1;
+        };
+
+        // whether the row launch of a table whose largest degree is 2^max_log_n runs one wave per
+        // workgroup: every job's rows must be wave-local (two-pass degrees up to 2^16 at E = 8)
+        template<typename A, int E>
+        bool table_rows_one_wave(int max_log_n)
+        {
+            return std::is_same<A, arith_r26>::value && E == 8 && max_log_n > tile_log &&
+                   geom_for(max_log_n, true).log_n2 <= 9;
         }
 
         // S > 0: every job has this row length; S = 0: mixed degrees, each block branching (uniformly)
         // to the row length of its own job (2^7..2^10 at compile time, others at run time)
-        template<typename A, int E, int S = 0>
+        // W1: one wave per workgroup (table_rows_one_wave holds for the table: every row length is
+        // 2^7 .. 2^9, the only cases the W1 instantiation handles)
+        template<typename A, int E, int S = 0, bool W1 = false>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows_table(job_desc const* __restrict__ jobs,
                                                                                     u32 const* __restrict__ map,
                                                                                     u32* __restrict__ ws,
                                                                                     typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
-            using AB = typename bgv_rows_arith<A>::type;
-            u32 m = map[blockIdx.x];
+            using AB = typename table_rows_policy<A, W1>::bgv;
+            u32 m = map[W1 ? blockIdx.x / static_cast<u32>(tile / E / 64) : blockIdx.x];
             job_desc J = jobs[m >> 10];
             u32 rt = m & 1023u;
             if constexpr (S > 0)
                 bgv_rows_table_tile<AB, E, S>(J, lds, ws, rt, P);
+            else if constexpr (W1)
+            {
+                switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
+                {
+                case 7: bgv_rows_table_tile<AB, E, 7>(J, lds, ws, rt, P); break;
+                case 8: bgv_rows_table_tile<AB, E, 8>(J, lds, ws, rt, P); break;
+                case 9: bgv_rows_table_tile<AB, E, 9>(J, lds, ws, rt, P); break;
+                default: break;
+                }
+            }
             else
             {
                 switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
@@ -617,8 +656,9 @@ namespace hmpc_amd
         // log_n > 0: every job of the table has degree 2^log_n (pass lengths at compile time where
         // they are specialized); 0: mixed degrees
         template<typename A>
-        hipError_t launch_bgv_table(int elems, int log_n, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
-                                    u32 const* map_rows, u32 rows_blocks, u32* ws, typename A::params const& P, hipStream_t s)
+        hipError_t launch_bgv_table(int elems, int log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
+                                    u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, typename A::params const& P,
+                                    hipStream_t s)
         {
             return with_elems<A>(elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
@@ -636,8 +676,26 @@ namespace hmpc_amd
                 case 10: rows_kernel = k_bgv_rows_table<A, E, 10>; break;
                 default: break;
                 }
+                unsigned rows_grid = rows_blocks, rows_threads = tile / E;
+                std::size_t rows_bytes = bytes;
+                if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                {
+                    if (table_rows_one_wave<A, E>(max_log_n))
+                    {
+                        switch (uniform ? gr.log_n2 : 0)
+                        {
+                        case 7: rows_kernel = k_bgv_rows_table<A, E, 7, true>; break;
+                        case 8: rows_kernel = k_bgv_rows_table<A, E, 8, true>; break;
+                        case 9: rows_kernel = k_bgv_rows_table<A, E, 9, true>; break;
+                        default: rows_kernel = k_bgv_rows_table<A, E, 0, true>; break;
+                        }
+                        rows_grid = rows_blocks * static_cast<unsigned>(tile / E / 64);
+                        rows_threads = 64;
+                        rows_bytes = bytes / static_cast<std::size_t>(tile / E / 64);
+                    }
+                }
                 hipError_t e = hipSuccess;
-                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess)
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, rows_bytes)) != hipSuccess)
                     return e;
                 {
                     probe_scope ps(s, "fast bgv columns");
@@ -645,7 +703,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast bgv rows");
-                    rows_kernel<<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
+                    rows_kernel<<<rows_grid, rows_threads, rows_bytes, s>>>(jobs, map_rows, ws, P);
                 }
                 return hipGetLastError();
             });
@@ -738,19 +796,32 @@ namespace hmpc_amd
 
         // S > 0: every job has this row length (A then the row policy); S = 0: mixed degrees, each
         // block branching (uniformly) to its own job's row length, 2^7..2^10 with the row policy
-        template<typename A, int E, int S = 0>
+        // W1: one wave per workgroup, as k_bgv_rows_table
+        template<typename A, int E, int S = 0, bool W1 = false>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_rows_table(job_desc const* __restrict__ jobs,
                                                                                         u32 const* __restrict__ map,
                                                                                         u32* __restrict__ ws,
                                                                                         typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
-            using AR = typename rows_arith<A>::type;
-            u32 m = map[blockIdx.x];
+            using AR = typename table_rows_policy<A, W1>::polymul;
+            u32 m = map[W1 ? blockIdx.x / static_cast<u32>(tile / E / 64) : blockIdx.x];
             job_desc J = jobs[m >> 10];
             u32 rt = m & 1023u;
-            if constexpr (S > 0)
+            if constexpr (S > 0 && W1)
+                polymul_rows_table_tile<AR, E, S>(J, lds, ws, rt, P);
+            else if constexpr (S > 0)
                 polymul_rows_table_tile<A, E, S>(J, lds, ws, rt, P);
+            else if constexpr (W1)
+            {
+                switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
+                {
+                case 7: polymul_rows_table_tile<AR, E, 7>(J, lds, ws, rt, P); break;
+                case 8: polymul_rows_table_tile<AR, E, 8>(J, lds, ws, rt, P); break;
+                case 9: polymul_rows_table_tile<AR, E, 9>(J, lds, ws, rt, P); break;
+                default: break;
+                }
+            }
             else
             {
                 switch (geom_for(static_cast<int>(J.log_n), true).log_n2)
@@ -806,9 +877,9 @@ namespace hmpc_amd
         }
 
         template<typename A>
-        hipError_t launch_polymul_table(int elems, int log_n, job_desc const* jobs, u32 const* map_cols, u32 cols_blocks,
-                                        u32 const* map_rows, u32 rows_blocks, u32 const* map_icols, u32 icols_blocks,
-                                        u32* ws, typename A::params const& P, hipStream_t s)
+        hipError_t launch_polymul_table(int elems, int log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
+                                        u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
+                                        u32 icols_blocks, u32* ws, typename A::params const& P, hipStream_t s)
         {
             return with_elems<A>(elems, [&](auto EC) -> hipError_t {
                 constexpr int E = decltype(EC)::value;
@@ -829,8 +900,26 @@ namespace hmpc_amd
                 case 10: rows_kernel = k_polymul_rows_table<AR, E, 10>; break;
                 default: break;
                 }
+                unsigned rows_grid = rows_blocks, rows_threads = tile / E;
+                std::size_t rows_bytes = bytes;
+                if constexpr (std::is_same<A, arith_r26>::value && E == 8)
+                {
+                    if (table_rows_one_wave<A, E>(max_log_n))
+                    {
+                        switch (uniform ? gr.log_n2 : 0)
+                        {
+                        case 7: rows_kernel = k_polymul_rows_table<A, E, 7, true>; break;
+                        case 8: rows_kernel = k_polymul_rows_table<A, E, 8, true>; break;
+                        case 9: rows_kernel = k_polymul_rows_table<A, E, 9, true>; break;
+                        default: rows_kernel = k_polymul_rows_table<A, E, 0, true>; break;
+                        }
+                        rows_grid = rows_blocks * static_cast<unsigned>(tile / E / 64);
+                        rows_threads = 64;
+                        rows_bytes = bytes / static_cast<std::size_t>(tile / E / 64);
+                    }
+                }
                 hipError_t e = hipSuccess;
-                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, bytes)) != hipSuccess ||
+                if ((e = allow_lds(cols_kernel, bytes)) != hipSuccess || (e = allow_lds(rows_kernel, rows_bytes)) != hipSuccess ||
                     (e = allow_lds(icols_kernel, bytes)) != hipSuccess)
                     return e;
                 {
@@ -839,7 +928,7 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
-                    rows_kernel<<<rows_blocks, tile / E, bytes, s>>>(jobs, map_rows, ws, P);
+                    rows_kernel<<<rows_grid, rows_threads, rows_bytes, s>>>(jobs, map_rows, ws, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");

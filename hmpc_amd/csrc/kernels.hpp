@@ -93,13 +93,15 @@ namespace hmpc_amd
     // the same over a table of BGV jobs of two-pass degrees (one ciphertext each): block b runs
     // job map[b] >> 10, local tile map[b] & 1023; job j's column results live at
     // ws + jobs[j].ws_off ([3][ws_limbs(L)][N]). uniform_log_n > 0: every job has degree
-    // 2^uniform_log_n (kernels with the pass lengths at compile time); 0: mixed degrees
-    hipError_t launch_fast_bgv_table(int L, transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    // 2^uniform_log_n (kernels with the pass lengths at compile time); 0: mixed degrees. max_log_n:
+    // the largest degree of the table (up to 2^16 the radix-2^26 row launch runs one wave per
+    // workgroup)
+    hipError_t launch_fast_bgv_table(int L, transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                      u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s);
     // the two-pass product over a table of jobs of two-pass degrees (one pair each): forward
     // columns (2 x column tiles per job), rows, inverse columns; job j's workspace region is
     // ws + jobs[j].ws_off ([2][ws_limbs(L)][N]); maps and uniform_log_n as above
-    hipError_t launch_fast_polymul_table(int L, transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_polymul_table(int L, transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                          u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
                                          u32 icols_blocks, u32* ws, hipStream_t s);
     // tiles per polynomial of the column (rows = false) or row pass at N = 2^log_n > 4096

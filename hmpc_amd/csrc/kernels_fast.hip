@@ -35,15 +35,15 @@ namespace hmpc_amd
     hipError_t launch_fast_bgv_passes_l2(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
     hipError_t launch_fast_bgv_passes_l4(transform_args const& T, job_desc const& base, u64 n, u32* ws, hipStream_t s);
 
-    hipError_t launch_fast_bgv_table_l2(transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_bgv_table_l2(transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                         u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s);
-    hipError_t launch_fast_bgv_table_l4(transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_bgv_table_l4(transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                         u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s);
 
-    hipError_t launch_fast_polymul_table_l2(transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_polymul_table_l2(transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                             u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
                                             u32 icols_blocks, u32* ws, hipStream_t s);
-    hipError_t launch_fast_polymul_table_l4(transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_polymul_table_l4(transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                             u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
                                             u32 icols_blocks, u32* ws, hipStream_t s);
 
@@ -125,25 +125,25 @@ namespace hmpc_amd
         return hipErrorInvalidValue;
     }
 
-    hipError_t launch_fast_bgv_table(int L, transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_bgv_table(int L, transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                      u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s)
     {
         if (L == 4)
-            return launch_fast_bgv_table_l4(T, uniform_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, ws, s);
+            return launch_fast_bgv_table_l4(T, uniform_log_n, max_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, ws, s);
         if (L == 2)
-            return launch_fast_bgv_table_l2(T, uniform_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, ws, s);
+            return launch_fast_bgv_table_l2(T, uniform_log_n, max_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, ws, s);
         return hipErrorInvalidValue;
     }
 
-    hipError_t launch_fast_polymul_table(int L, transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_polymul_table(int L, transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                          u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32 const* map_icols,
                                          u32 icols_blocks, u32* ws, hipStream_t s)
     {
         if (L == 4)
-            return launch_fast_polymul_table_l4(T, uniform_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, map_icols, icols_blocks,
+            return launch_fast_polymul_table_l4(T, uniform_log_n, max_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, map_icols, icols_blocks,
                                                 ws, s);
         if (L == 2)
-            return launch_fast_polymul_table_l2(T, uniform_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, map_icols, icols_blocks,
+            return launch_fast_polymul_table_l2(T, uniform_log_n, max_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, map_icols, icols_blocks,
                                                 ws, s);
         return hipErrorInvalidValue;
     }

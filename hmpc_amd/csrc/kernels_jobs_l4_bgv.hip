@@ -9,10 +9,10 @@ namespace hmpc_amd
         return fast::launch_bgv_passes<fast::arith_r26>(T.elems, base, n, ws, T.r26, s);
     }
 
-    hipError_t launch_fast_bgv_table_l4(transform_args const& T, int uniform_log_n, job_desc const* jobs, u32 const* map_cols,
+    hipError_t launch_fast_bgv_table_l4(transform_args const& T, int uniform_log_n, int max_log_n, job_desc const* jobs, u32 const* map_cols,
                                          u32 cols_blocks, u32 const* map_rows, u32 rows_blocks, u32* ws, hipStream_t s)
     {
-        return fast::launch_bgv_table<fast::arith_r26>(T.elems, uniform_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, ws,
+        return fast::launch_bgv_table<fast::arith_r26>(T.elems, uniform_log_n, max_log_n, jobs, map_cols, cols_blocks, map_rows, rows_blocks, ws,
                                                     T.r26, s);
     }
 }
