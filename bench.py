@@ -352,6 +352,7 @@ def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0
     cts = sum(k for _, k in reqs)
     wall, _ = run(reqs, [0.0] * len(reqs))
     peak_rps = len(reqs) / wall
+    peak_cps = cts / wall
     points = []
     for f in fractions:
         rate = f * peak_rps
@@ -391,7 +392,7 @@ def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0
     stream.synchronize()
     plain_ct = n_ct / (time.perf_counter() - t0)
     return {"requests_per_point": requests, "reduced_sweep": requests < 100000, "configured_requests_per_point": 100000,
-            "peak_req_per_s": peak_rps, "peak_ct_per_s": cts / (len(reqs) / peak_rps),
+            "peak_req_per_s": peak_rps, "peak_ct_per_s": peak_cps,
             "operating_point": op, "operating_point_full": full, "points": points, "plain_ct_per_s": plain_ct,
             "plain_sample_requests": len(sample),
             "launches": st["launches"], "budget_bytes": 200 << 30, "slots": 3}

@@ -35,9 +35,10 @@ import sys
 # launch-probe name (bench.py) -> kernel symbol prefix of the workload's dispatches
 WORKLOAD_KERNELS = {
     "c2": {
-        "fast polymul forward columns": "hmpc_amd::fast::k_polymul_cols<",
+        # the streaming column passes (persistent grids: the timed batch is the longest size class)
+        "fast polymul forward columns": "hmpc_amd::fast::k_polymul_cols_stream<",
         "fast polymul rows": "hmpc_amd::fast::k_polymul<",        # the two-pass product (N > 4096)
-        "fast polymul inverse columns": "hmpc_amd::fast::k_pass<",  # the only per-pass transform of the step
+        "fast polymul inverse columns": "hmpc_amd::fast::k_icols_stream<",
     },
     "c3": {
         "fast bgv columns": "hmpc_amd::fast::k_bgv_cols<",
@@ -52,33 +53,60 @@ def _short(name: str) -> str:
     return name[5:] if name.startswith("void ") else name
 
 
+def _size_classes(durations):
+    """Split one kernel's dispatches of one grid into classes of similar duration (a new class where
+    a duration exceeds twice the shortest of the current class): persistent kernels launch the same
+    grid for batches of different size (the bench's timed batch and the autotuner's trial batch)."""
+    classes = []
+    for d in sorted(durations):
+        if classes and d <= 2.0 * classes[-1][0]:
+            classes[-1].append(d)
+        else:
+            classes.append([d])
+    return classes
+
+
 def kernel_rows(db_path: str):
     cur = sqlite3.connect(db_path).cursor()
     rows = cur.execute("select name, grid_x, workgroup_x, duration from kernels").fetchall()
-    groups = collections.defaultdict(list)
+    raw = collections.defaultdict(list)
     for name, grid, wg, dur in rows:
-        groups[(_short(name), int(grid), int(wg))].append(float(dur))
+        raw[(_short(name), int(grid), int(wg))].append(float(dur))
+    groups = {}
+    for (name, grid, wg), durs in raw.items():
+        classes = _size_classes(durs)
+        for i, c in enumerate(classes):
+            groups[(name if len(classes) == 1 else f"{name} [size class {i}]", grid, wg)] = c
     return groups
+
+
+def _longest_class(records):
+    """records: (duration, value) per dispatch -> the values of the longest size class"""
+    if not records:
+        return []
+    top = _size_classes([d for d, _ in records])[-1]
+    return [v for d, v in records if d >= top[0]]
 
 
 def counter_rows(db_path: str, counter: str):
+    """per (kernel, grid): the counter's value per dispatch of the longest size class"""
     cur = sqlite3.connect(db_path).cursor()
-    rows = cur.execute("select kernel_name, grid_size, value from counters_collection where counter_name = ?",
+    rows = cur.execute("select kernel_name, grid_size, value, start, end from counters_collection where counter_name = ?",
                        (counter,)).fetchall()
     groups = collections.defaultdict(list)
-    for name, grid, value in rows:
-        groups[(_short(name), int(grid))].append(float(value))
-    return groups
+    for name, grid, value, start, end in rows:
+        groups[(_short(name), int(grid))].append((float(end - start), float(value)))
+    return {k: _longest_class(v) for k, v in groups.items()}
 
 
 def cmd_kernels(args):
     groups = kernel_rows(args.db)
     wanted = [s for s in args.symbols.split(",") if s] if args.symbols else None
-    print(f"{'kernel':<78} {'grid':>10} {'wg':>5} {'calls':>6} {'avg us':>10} {'min us':>10} {'max us':>10}")
+    print(f"{'kernel':<96} {'grid':>10} {'wg':>5} {'calls':>6} {'avg us':>10} {'min us':>10} {'max us':>10}")
     for (name, grid, wg), d in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
         if wanted and not any(name.startswith(w) for w in wanted):
             continue
-        print(f"{name[:78]:<78} {grid:>10} {wg:>5} {len(d):>6} {sum(d) / len(d) / 1e3:>10.1f} {min(d) / 1e3:>10.1f} "
+        print(f"{name[:96]:<96} {grid:>10} {wg:>5} {len(d):>6} {sum(d) / len(d) / 1e3:>10.1f} {min(d) / 1e3:>10.1f} "
               f"{max(d) / 1e3:>10.1f}")
 
 
@@ -130,12 +158,14 @@ def cmd_traffic(args):
 def cmd_counters(args):
     cur = sqlite3.connect(args.db).cursor()
     rows = cur.execute("select kernel_name, grid_size, counter_name, value, start, end from counters_collection").fetchall()
-    groups = collections.defaultdict(lambda: collections.defaultdict(list))
+    # per (kernel, grid) and counter: (duration, value) per dispatch, then the longest size class
+    recs = collections.defaultdict(lambda: collections.defaultdict(list))
     for name, grid, counter, value, start, end in rows:
-        groups[(_short(name), int(grid))][counter].append(float(value))
+        recs[(_short(name), int(grid))][counter].append((float(end - start), float(value)))
         if counter == "GRBM_GUI_ACTIVE":
             # GPU busy cycles over the dispatch: with its duration, the engine clock it ran at
-            groups[(_short(name), int(grid))]["duration_ns"].append(float(end - start))
+            recs[(_short(name), int(grid))]["duration_ns"].append((float(end - start), float(end - start)))
+    groups = {k: {c: _longest_class(v) for c, v in cs.items()} for k, cs in recs.items()}
     issue = args.cus * 4 * args.clock_ghz * 1e9 / 4  # wave instructions per second
     entries = {}
     print(f"VALU issue rate: {args.cus} CUs x 4 SIMDs x {args.clock_ghz} GHz / 4 cycles = {issue / 1e9:.1f} G wave-instr/s")
