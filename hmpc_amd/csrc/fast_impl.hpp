@@ -1119,6 +1119,53 @@ namespace hmpc_amd
             }
         }
 
+        // x += key o u for the register operand u (the same unit order as x) and the key read per
+        // element (BGV rows with NTT(u) kept in registers)
+        template<typename A, int E, int W>
+        __device__ __forceinline__ void with_key_regs(u32 (&x)[E][A::rl], u32 const (&u)[E][A::rl], int valid,
+                                                      u32 const* __restrict__ key, std::size_t key_count, std::size_t key_base,
+                                                      typename A::params const& P)
+        {
+            constexpr int K = 1 << W;
+#pragma unroll
+            for (int g = 0; g < E / K; ++g)
+            {
+                int const e0 = unit_index<A, E, W>(g) << W;
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    if (e0 + k >= valid)
+                        continue;
+                    u32 kv[A::rl];
+                    A::load_global(kv, key + key_base, key_count, static_cast<u32>(e0 + k));
+                    A::point_add(x[g * K + k], u[g * K + k], kv, P);
+                }
+            }
+        }
+
+        template<typename A, int E>
+        __device__ __forceinline__ void with_key_regs_last(u32 (&x)[E][A::rl], u32 const (&u)[E][A::rl], tile_desc const& td,
+                                                           int valid, u32 const* key, std::size_t key_count, std::size_t key_base,
+                                                           typename A::params const& P)
+        {
+            constexpr int WM = shape<E>::wmax;
+            int w = round_width<WM>(td.s, round_count<WM>(td.s) - 1, false);
+            if (w == WM)
+                with_key_regs<A, E, WM>(x, u, valid, key, key_count, key_base, P);
+            else if (WM > 1 && w == 1)
+                with_key_regs<A, E, 1>(x, u, valid, key, key_count, key_base, P);
+            else if constexpr (WM > 2)
+            {
+                if (w == 2)
+                    with_key_regs<A, E, 2>(x, u, valid, key, key_count, key_base, P);
+                else if constexpr (WM > 3)
+                {
+                    if (w == 3)
+                        with_key_regs<A, E, 3>(x, u, valid, key, key_count, key_base, P);
+                }
+            }
+        }
+
         // ------------------------------------------------------------------ kernels
         // mode: 0 forward column pass, 1 forward row pass (final), 2 inverse row pass,
         //       3 inverse column pass (final), 4 forward whole-poly, 5 inverse whole-poly

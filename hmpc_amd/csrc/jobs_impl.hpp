@@ -225,6 +225,32 @@ namespace hmpc_amd
             tile_sync<WL>();
         }
 
+        // the same with NTT(u) in registers (the unit order of x)
+        template<typename A, int E, bool WL = false>
+        __device__ __forceinline__ void bgv_emit_regs(u32 (&x)[E][A::rl], u32 const (&u)[E][A::rl], u32* lds, tile_desc const& td,
+                                                      u32 const* key, io_desc const& ik, u32* out, io_desc const& io,
+                                                      typename A::params const& P)
+        {
+            constexpr int T = tile / E;
+            with_key_regs_last<A, E>(x, u, td, ik.valid, key, ik.count, ik.base, P);
+            scatter_last<A, E>(x, lds, td);
+            tile_sync<WL>();
+            tile_store<A, T, false, WL>(lds, out, io, true, P);
+            tile_sync<WL>();
+        }
+
+        // The one-wave BGV row passes keep NTT(u) in registers while they transform X0 and X1 (40 more
+        // VGPRs): no write-back of U's rows and no re-read per ciphertext output (1.0 of 2.5 GB per
+        // batch of 1024 at N = 2^14; BGV rows 0.72 -> 0.66 ms). Their launch bounds allow 3 waves per
+        // SIMD, so that rows of 2^7 fit without spilling (134 VGPRs; as fast as 128 with 14 spilled).
+        // The other row kernels, whose mixed-length variants would spill heavily, read U back from the
+        // workspace.
+        template<typename A>
+        constexpr bool bgv_u_in_registers()
+        {
+            return A::wave_blocks;
+        }
+
         template<typename A, int E, int Which>
         __device__ __forceinline__ void bgv_columns(job_desc const& J, u32* lds, u32* dst, pass_geom const& gc,
                                                     u32 (&x)[E][A::rl], typename A::params const& P)
@@ -407,7 +433,7 @@ namespace hmpc_amd
         }
 
         template<typename A, int E, int S = 0>
-        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows(job_desc base, u32* __restrict__ ws,
+        __global__ void __launch_bounds__(A::wave_blocks ? 64 : tile / E, A::wave_blocks ? 3 : (E == 8 ? 4 : 1)) k_bgv_rows(job_desc base, u32* __restrict__ ws,
                                                                               pass_geom gr, u64 n, typename A::params P)
         {
             extern __shared__ __align__(16) u32 lds[];
@@ -437,20 +463,40 @@ namespace hmpc_amd
             // with the wave-major unit order rows of at most 64 E elements are wave-local: no
             // workgroup barrier
             constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
-            // NTT(u) rows back into the workspace, read per element below
-            tile_load<A, T, true, WL>(lds, su, is);
-            tile_sync<WL>();
-            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, false);
-            tile_store<A, T, true, WL>(lds, su, is, false, P);
-            tile_sync<WL>();
-            for (int which = 1; which < 3; ++which)
+            if constexpr (bgv_u_in_registers<A>())
             {
-                tile_load<A, T, true, WL>(lds, ws + which * A::wsl * count, is);
+                // NTT(u) stays in registers (the last round's unit order, shared by X0 and X1)
+                u32 u[E][A::rl];
+                tile_load<A, T, true, WL>(lds, su, is);
                 tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
-                bool first = which == 1;
-                bgv_emit<A, E, WL>(x, lds, td, su, count, is.base, first ? base.in0 : base.in1, ik,
-                                   first ? base.out0 : base.out1, io, P);
+                tile_rounds<A, E, false, WL>(u, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                tile_sync<WL>();
+                for (int which = 1; which < 3; ++which)
+                {
+                    tile_load<A, T, true, WL>(lds, ws + which * A::wsl * count, is);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                    bool first = which == 1;
+                    bgv_emit_regs<A, E, WL>(x, u, lds, td, first ? base.in0 : base.in1, ik, first ? base.out0 : base.out1, io, P);
+                }
+            }
+            else
+            {
+                // NTT(u) rows back into the workspace, read per element below
+                tile_load<A, T, true, WL>(lds, su, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, false);
+                tile_store<A, T, true, WL>(lds, su, is, false, P);
+                tile_sync<WL>();
+                for (int which = 1; which < 3; ++which)
+                {
+                    tile_load<A, T, true, WL>(lds, ws + which * A::wsl * count, is);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                    bool first = which == 1;
+                    bgv_emit<A, E, WL>(x, lds, td, su, count, is.base, first ? base.in0 : base.in1, ik,
+                                       first ? base.out0 : base.out1, io, P);
+                }
             }
         }
 
@@ -573,18 +619,37 @@ namespace hmpc_amd
             u32 x[E][A::rl];
             // wave-local rows (the one-wave policy) need no workgroup barrier
             constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
-            tile_load<A, T, true, WL>(lds, su, is);
-            tile_sync<WL>();
-            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-            tile_store<A, T, true, WL>(lds, su, is, false, P);
-            tile_sync<WL>();
-            for (int which = 1; which < 3; ++which)
+            if constexpr (bgv_u_in_registers<A>())
             {
-                tile_load<A, T, true, WL>(lds, su + which * A::wsl * N, is);
+                u32 u[E][A::rl];
+                tile_load<A, T, true, WL>(lds, su, is);
                 tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
-                bool first = which == 1;
-                bgv_emit<A, E, WL>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+                tile_rounds<A, E, false, WL>(u, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                tile_sync<WL>();
+                for (int which = 1; which < 3; ++which)
+                {
+                    tile_load<A, T, true, WL>(lds, su + which * A::wsl * N, is);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                    bool first = which == 1;
+                    bgv_emit_regs<A, E, WL>(x, u, lds, td, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+                }
+            }
+            else
+            {
+                tile_load<A, T, true, WL>(lds, su, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+                tile_store<A, T, true, WL>(lds, su, is, false, P);
+                tile_sync<WL>();
+                for (int which = 1; which < 3; ++which)
+                {
+                    tile_load<A, T, true, WL>(lds, su + which * A::wsl * N, is);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                    bool first = which == 1;
+                    bgv_emit<A, E, WL>(x, lds, td, su, N, is.base, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
+                }
             }
         }
 
@@ -618,7 +683,7 @@ namespace hmpc_amd
         // W1: one wave per workgroup (table_rows_one_wave holds for the table: every row length is
         // 2^7 .. 2^9, the only cases the W1 instantiation handles)
         template<typename A, int E, int S = 0, bool W1 = false>
-        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_rows_table(job_desc const* __restrict__ jobs,
+        __global__ void __launch_bounds__(W1 ? 64 : tile / E, W1 ? 3 : (E == 8 ? 4 : 1)) k_bgv_rows_table(job_desc const* __restrict__ jobs,
                                                                                     u32 const* __restrict__ map,
                                                                                     u32* __restrict__ ws,
                                                                                     typename A::params P)

@@ -113,6 +113,9 @@ def test_poly_mod_kat_on_gpu(ctx):
 
 POLYMUL_CASES = [(Q64, 4, 3), (Q64, 10, 4), (Q64, 12, 3), (Q64, 13, 2), (Q128A, 12, 2), (Q128A, 15, 2), (Q128A, 16, 2),
                  (Q128A, 17, 1),
+                 # more column tiles than the streaming column passes' 512 resident workgroups, not a
+                 # multiple of them (1280 and 264 forward tiles: workgroups walk 2 or 3 tiles, or 1)
+                 (Q128A, 16, 40), (Q128B, 14, 33),
                  (Q64, 18, 1),
                  (Q128A, 5, 7), (Q128B, 14, 2), (Q128B, 13, 3), (Q_GENERIC4, 12, 2), (Q_GENERIC4, 14, 1),
                  (Q_LAZY4, 13, 2), (Q_LAZY2, 12, 2), (Q_GENERIC2, 13, 2), (P150, 9, 2)]
