@@ -1224,7 +1224,7 @@ namespace hmpc_amd
         // structure and every LDS address are resolved at compile time (and only the round widths
         // this length uses are emitted)
         template<typename A, int E, bool Single, int S = 0>
-        __global__ void __launch_bounds__(tile / E, (E == 8 && !Single) ? 4 : 1) k_polymul(u32 const* a, u32 const* __restrict__ b,
+        __global__ void __launch_bounds__(A::wave_blocks ? 64 : tile / E, A::wave_blocks ? 3 : ((E == 8 && !Single) ? 4 : 1)) k_polymul(u32 const* a, u32 const* __restrict__ b,
                                                               u32* out, u32 const* __restrict__ twf,
                                                               u32 const* __restrict__ twi, pass_geom G, u64 n,
                                                               typename A::params P)
@@ -1285,15 +1285,38 @@ namespace hmpc_amd
                 // own, with no workgroup barrier.
                 constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
                 u32 x[E][A::rl];
-                tile_load<A, T, true, WL>(lds, a, io);
-                tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, false);
-                tile_store<A, T, true, WL>(lds, out, io, false, P);
-                tile_sync<WL>();
-                tile_load<A, T, true, WL>(lds, b, io);
-                tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
-                with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
+                if constexpr (A::wave_blocks)
+                {
+                    // one-wave rows: NTT(a) stays in registers next to NTT(b)
+                    u32 xa[E][A::rl];
+                    tile_load<A, T, true, WL>(lds, a, io);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
+                    tile_sync<WL>();
+                    tile_load<A, T, true, WL>(lds, b, io);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
+#pragma unroll
+                    for (int k = 0; k < E; ++k)
+                    {
+                        A::point(xa[k], x[k], P);
+#pragma unroll
+                        for (int l = 0; l < A::rl; ++l)
+                            x[k][l] = xa[k][l];
+                    }
+                }
+                else
+                {
+                    tile_load<A, T, true, WL>(lds, a, io);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, false);
+                    tile_store<A, T, true, WL>(lds, out, io, false, P);
+                    tile_sync<WL>();
+                    tile_load<A, T, true, WL>(lds, b, io);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
+                    with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
+                }
                 dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
                 tile_sync<WL>();
                 tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false);

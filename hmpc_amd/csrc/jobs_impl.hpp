@@ -844,15 +844,38 @@ namespace hmpc_amd
             // wave-local rows as in the uniform row kernel
             constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
             u32 x[E][A::rl];
-            tile_load<A, T, true, WL>(lds, sa, is);
-            tile_sync<WL>();
-            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
-            tile_store<A, T, true, WL>(lds, sa, is, false, P);
-            tile_sync<WL>();
-            tile_load<A, T, true, WL>(lds, sb, is);
-            tile_sync<WL>();
-            tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
-            with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
+            if constexpr (A::wave_blocks)
+            {
+                // one-wave rows: NTT(a) stays in registers next to NTT(b), as in the uniform kernel
+                u32 xa[E][A::rl];
+                tile_load<A, T, true, WL>(lds, sa, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(xa, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                tile_sync<WL>();
+                tile_load<A, T, true, WL>(lds, sb, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                {
+                    A::point(xa[k], x[k], P);
+#pragma unroll
+                    for (int l = 0; l < A::rl; ++l)
+                        x[k][l] = xa[k][l];
+                }
+            }
+            else
+            {
+                tile_load<A, T, true, WL>(lds, sa, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, false);
+                tile_store<A, T, true, WL>(lds, sa, is, false, P);
+                tile_sync<WL>();
+                tile_load<A, T, true, WL>(lds, sb, is);
+                tile_sync<WL>();
+                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                with_operands_last<A, E, 0>(x, td, is.valid, sa, N, is.base, nullptr, 0, 0, P);
+            }
             dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
             tile_sync<WL>();
             tile_rounds<A, E, true, WL>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
@@ -863,7 +886,7 @@ namespace hmpc_amd
         // block branching (uniformly) to its own job's row length, 2^7..2^10 with the row policy
         // W1: one wave per workgroup, as k_bgv_rows_table
         template<typename A, int E, int S = 0, bool W1 = false>
-        __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_polymul_rows_table(job_desc const* __restrict__ jobs,
+        __global__ void __launch_bounds__(W1 ? 64 : tile / E, W1 ? 3 : (E == 8 ? 4 : 1)) k_polymul_rows_table(job_desc const* __restrict__ jobs,
                                                                                         u32 const* __restrict__ map,
                                                                                         u32* __restrict__ ws,
                                                                                         typename A::params P)
