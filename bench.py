@@ -178,7 +178,7 @@ def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
                                 "p50_ms": percentile(hper, 50), "p99_ms": percentile(hper, 99), "decrypt_ok": hok}}
 
 
-def bench_c1(hm, ctx, stream, iters=200):
+def bench_c1(hm, ctx, stream, iters=1000):
     mod = hm.Modulus(ctx, hm.Q64)
     mod.prepare(12)
     x = mod.sample_uniform(KEY, 0, 4096)
