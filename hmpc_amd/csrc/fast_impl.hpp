@@ -1361,17 +1361,20 @@ namespace hmpc_amd
         //
         // The prefetch only overlaps if no later vector load waits behind it (vector loads complete
         // in issue order), so these passes take no twiddle from a vector load inside the loop:
-        //   - full-width rounds (3 stages) have one twiddle per stage and wave in this geometry
-        //     (forward stages 0-5, inverse stages 1-6: the unit's top bits F are 0 or the wave
-        //     index), loaded with scalar loads from lane 0's index;
-        //   - the width-1 round (forward stage 6, inverse stage 0) has two twiddles per wave (lanes
-        //     0-31 and 32-63) that do not depend on the tile (base 1): the forward pass loads them
-        //     once per workgroup and keeps them in VGPRs, the inverse pass (whose 5-word prefetch
-        //     leaves no room for them) takes both with scalar loads and selects per lane.
-        // tests/cpp/jobs_index_check.cpp checks the per-wave uniformity on the CPU.
-        // Round-0 elements are loaded straight into registers and the last round's stored straight
+        //   - the 3-stage rounds have one twiddle per stage and wave in this geometry (forward
+        //     stages 0-5, inverse stages 1-6: the unit's top bits F are 0 or the wave index), loaded
+        //     with scalar loads from lane 0's index;
+        //   - forward stage 6 and inverse stage 0 run in registers in the pair layout
+        //     (cols_pair_row: one v_permlane32_swap per word and pair from or to the 3-stage
+        //     layout instead of an LDS exchange and a barrier); their twiddles (two per wave and
+        //     pair, not depending on the tile) are loaded once per workgroup and kept in VGPRs
+        //     (forward), or taken with two scalar loads and a per-lane select (inverse, whose 5-word
+        //     prefetch leaves no room).
+        // tests/cpp/jobs_index_check.cpp checks the per-wave uniformity, the pair layout and that each
+        // pass moves every tile element once, inside its polynomial. Round-0 (forward) and pair-layout
+        // (inverse) elements are loaded straight into registers and the last round's stored straight
         // from them (each wave access is two 128-byte column segments per limb plane, as in
-        // tile_load), so LDS carries only the two exchanges between rounds.
+        // tile_load), so LDS carries a single exchange per tile.
 
         // a twiddle whose index is the same on every lane of the wave
         template<typename A>
@@ -1445,26 +1448,26 @@ namespace hmpc_amd
             return elem_of(td, un.c, j_of(un, static_cast<u32>(k), a, W, td.s, inv));
         }
 
-        // the width-1 round's per-lane twiddles, loaded once; the empty asm uses make the loads
-        // complete before the tile loop (inside it they would wait behind the prefetch)
-        template<typename A, int E, bool Inv>
-        __device__ __forceinline__ void resident_w1(u32 (&tw1)[E / 2][1][A::rl], tile_desc const& td, int a,
-                                                    u32 const* __restrict__ tw)
+        // pair layout <-> 3-stage-round layout of the streaming column passes (cols_pair_row): for
+        // each pair m, the lower half-wave's element 2m + 1 and the upper half's element 2m change
+        // places (v_permlane32_swap gives (a_lo | b_lo, a_hi | b_hi) for (a, b)); its own inverse
+        template<typename A, int E>
+        __device__ __forceinline__ void cols_pair_swap(u32 (&x)[E][A::rl])
         {
-            u32 xr[E / 2];
-            round_bases<A, E, Inv, 1>(xr, td, a, 1u, 0u);
 #pragma unroll
-            for (int g = 0; g < E / 2; ++g)
-                A::tw_get(tw1[g][0], tw, tw_index_round<Inv, 1, 0>(xr[g], 0));
-#pragma unroll
-            for (int g = 0; g < E / 2; ++g)
+            for (int m = 0; m < E / 2; ++m)
 #pragma unroll
                 for (int l = 0; l < A::rl; ++l)
-                    asm volatile("" ::"v"(tw1[g][0][l]));
+                {
+                    auto r = __builtin_amdgcn_permlane32_swap(x[2 * m][l], x[2 * m + 1][l], false, false);
+                    x[2 * m][l] = r[0];
+                    x[2 * m + 1][l] = r[1];
+                }
         }
 
         // forward column pass of both product inputs, tile t = 2 * column tile + operand (as
-        // k_polymul_cols), rounds (stages 0-2) (3-5) (6)
+        // k_polymul_cols): rounds (stages 0-2) and (3-5) through LDS, stage 6 in registers after
+        // the half-wave exchange into the pair layout
         template<typename A, int E, int S>
         __global__ void __launch_bounds__(tile / E, 4) k_polymul_cols_stream(u32 const* __restrict__ a, u32 const* __restrict__ b,
                                                                              u32* __restrict__ wa, u32* __restrict__ wb,
@@ -1475,8 +1478,19 @@ namespace hmpc_amd
             extern __shared__ __align__(16) u32 lds[];
             std::size_t const count = static_cast<std::size_t>(n) << G.log_n;
             tile_desc const td{S, tile_log - S, false};
-            u32 tw1[E / 2][1][A::rl];
-            resident_w1<A, E, false>(tw1, td, 6, tw);
+            int const t0 = tid_x();
+            // stage 6: the pairs' twiddles do not depend on the tile (two per wave and pair), loaded
+            // once; the empty asm uses make the loads complete before the tile loop (inside it they
+            // would wait behind the prefetch)
+            u32 tw6[E / 2][1][A::rl];
+#pragma unroll
+            for (int m = 0; m < E / 2; ++m)
+                A::tw_get(tw6[m][0], tw, cols_pair_twiddle(t0, m));
+#pragma unroll
+            for (int m = 0; m < E / 2; ++m)
+#pragma unroll
+                for (int l = 0; l < A::rl; ++l)
+                    asm volatile("" ::"v"(tw6[m][0][l]));
             bool const fold = ws_fold_columns(G.log_n);
             io_desc io;
             tile_desc tdd;
@@ -1497,8 +1511,8 @@ namespace hmpc_amd
                     for (int l = 0; l < 4; ++l)
                         raw[k][l] = ld_off(src + l * count + gidx_rel(io, unit_offset(td, 0, 3, false, k)), o_in);
             }
-            // last round (stage 6): units g = 0..3 of two elements
-            int const e_out = unit_elem<A, E, 1>(td, 6, false, 0, 0);
+            // stores in the pair layout: element (m, k) at e_out + the rows' offset
+            int const e_out = cols_pair_elem(t0, 0, 0);
             u32 const o_out = gidx_rel(io, e_out);
             for (;;)
             {
@@ -1520,26 +1534,21 @@ namespace hmpc_amd
                             raw[k][l] = ld_off(src + l * count + gidx_rel(ion, unit_offset(td, 0, 3, false, k)), o_in);
                 }
                 round_wave<A, E, false, 3>(x, td, 0, tw, P, 0, false);
-                __syncthreads();  // the previous tile's last gather is done
+                __syncthreads();  // the previous tile's gather is done
                 scatter<A, E, 3>(x, lds, td, 0, false);
                 __syncthreads();
                 gather<A, E, 3>(x, lds, td, 3, false);
                 round_wave<A, E, false, 3>(x, td, 3, tw, P, 0, false);
-                scatter<A, E, 3>(x, lds, td, 3, false);
-                __syncthreads();
-                gather<A, E, 1>(x, lds, td, 6, false);
+                cols_pair_swap<A, E>(x);
                 {
                     u32 const z[A::rl] = {};
-                    stage<A, E, false, 1, 0, false>(x, tw1, z, z, P, 0);
+                    stage<A, E, false, 1, 0, false>(x, tw6, z, z, P, 0);
                 }
 #pragma unroll
-                for (int g = 0; g < E / 2; ++g)
+                for (int m = 0; m < E / 2; ++m)
 #pragma unroll
                     for (int k = 0; k < 2; ++k)
-                    {
-                        int const d = unit_elem<A, E, 1>(td, 6, false, g, k) - e_out;
-                        A::store_ws(dst + gidx_rel(io, d), count, o_out, x[2 * g + k], fold, P);
-                    }
+                        A::store_ws(dst + gidx_rel(io, cols_pair_elem(t0, m, k) - e_out), count, o_out, x[2 * m + k], fold, P);
                 if (tn >= tiles)
                     break;
                 t = tn;
@@ -1548,8 +1557,9 @@ namespace hmpc_amd
         }
 
         // inverse column pass of the product from the row pass's workspace (raw limbs), with the
-        // N^-1 scaling and canonical output (as k_pass<A, E, 3, true, 7>): rounds (stage 0)
-        // (1-3) (4-6); `off` = log N2, the inverse stages the row pass ran before
+        // N^-1 scaling and canonical output (as k_pass<A, E, 3, true, 7>): stage 0 in registers in
+        // the pair layout, the half-wave exchange, then rounds (1-3) and (4-6); `off` = log N2, the
+        // inverse stages the row pass ran before
         template<typename A, int E, int S>
         __global__ void __launch_bounds__(tile / E, 4) k_icols_stream(u32 const* __restrict__ in, u32* __restrict__ out,
                                                                       u32 const* __restrict__ tw, pass_geom G, u64 n,
@@ -1560,22 +1570,22 @@ namespace hmpc_amd
             std::size_t const count = static_cast<std::size_t>(n) << G.log_n;
             tile_desc const td{S, tile_log - S, false};
             int const off = G.log_n2;
+            int const t0 = tid_x();
             io_desc io;
             tile_desc tdd;
             u32 t = blockIdx.x;
             if (t >= tiles)
                 return;
             cols_setup(G, t, count, tdd, io);
-            // round 0 (stage 0): units g = 0..3 of two elements
-            int const e_in = unit_elem<A, E, 1>(td, 0, true, 0, 0);
+            // loads in the pair layout
+            int const e_in = cols_pair_elem(t0, 0, 0);
             u32 const o_in = gidx_rel(io, e_in);
             u32 raw[E][A::wsl];
 #pragma unroll
-            for (int g = 0; g < E / 2; ++g)
+            for (int m = 0; m < E / 2; ++m)
 #pragma unroll
                 for (int k = 0; k < 2; ++k)
-                    A::load_ws(raw[2 * g + k], in + io.base + gidx_rel(io, unit_elem<A, E, 1>(td, 0, true, g, k) - e_in), count,
-                               o_in);
+                    A::load_ws(raw[2 * m + k], in + io.base + gidx_rel(io, cols_pair_elem(t0, m, k) - e_in), count, o_in);
             // last round (stages 4-6): one unit per thread
             int const e_out = unit_elem<A, E, 3>(td, 4, true, 0, 0);
             u32 const o_out = gidx_rel(io, e_out);
@@ -1594,28 +1604,23 @@ namespace hmpc_amd
                 {
                     cols_setup(G, tn, count, tdd, ion);
 #pragma unroll
-                    for (int g = 0; g < E / 2; ++g)
+                    for (int m = 0; m < E / 2; ++m)
 #pragma unroll
                         for (int k = 0; k < 2; ++k)
-                            A::load_ws(raw[2 * g + k], in + ion.base + gidx_rel(ion, unit_elem<A, E, 1>(td, 0, true, g, k) - e_in),
-                                       count, o_in);
+                            A::load_ws(raw[2 * m + k], in + ion.base + gidx_rel(ion, cols_pair_elem(t0, m, k) - e_in), count, o_in);
                 }
                 {
-                    // stage 0: two twiddles per wave and unit (lanes 0-31 and 32-63)
+                    // stage 0: two twiddles per wave and pair (lanes 0-31 and 32-63)
                     u32 const z[A::rl] = {};
                     u32 tw1[E / 2][1][A::rl];
-                    u32 xr[E / 2];
-                    round_bases<A, E, true, 1>(xr, td, 0, 1u, 0u);
 #pragma unroll
-                    for (int g = 0; g < E / 2; ++g)
-                        tw_get_wave2<A>(tw1[g][0], tw, tw_index_round<true, 1, 0>(xr[g], 0));
+                    for (int m = 0; m < E / 2; ++m)
+                        tw_get_wave2<A>(tw1[m][0], tw, cols_pair_twiddle(t0, m));
                     stage<A, E, true, 1, 0, false>(x, tw1, z, z, P, off);
                 }
-                __syncthreads();  // the previous tile's last gather is done
-                scatter<A, E, 1>(x, lds, td, 0, true);
-                __syncthreads();
-                gather<A, E, 3>(x, lds, td, 1, true);
+                cols_pair_swap<A, E>(x);
                 round_wave<A, E, true, 3>(x, td, 1, tw, P, off + 1, false);
+                __syncthreads();  // the previous tile's gather is done
                 scatter<A, E, 3>(x, lds, td, 1, true);
                 __syncthreads();
                 gather<A, E, 3>(x, lds, td, 4, true);

@@ -134,6 +134,19 @@ namespace hmpc_amd
         // transforms stop at N = 2^17 and the fold is a guard only.)
         __host__ HMPC_HD bool ws_fold_columns(int log_n) { return log_n > 18; }
 
+        // The pair layout of the streaming column passes (2^7-point columns, 32 columns per tile, 8
+        // elements per thread, 512 threads): thread t holds the four pairs (j, j + 1) of rows
+        // j = 16 (t >> 6) + 4 m + 2 h, m = 0..3, h = (t >> 5) & 1, of column t & 31. In the columns'
+        // 3-stage rounds starting at stage 3 (forward) or 1 (inverse) thread t holds rows
+        // 16 (t >> 6) + 2 k + h, k = 0..7: the two layouts differ by exchanging the odd elements of
+        // the lower half-wave with the even elements of the upper half (one v_permlane32_swap per
+        // word and pair), and the pairs are the partners of forward stage 6 and inverse stage 0,
+        // whose twiddle is entry 2^6 + (j >> 1) of the column table.
+        HMPC_HD int cols_pair_row(int t, int m, int k) { return 16 * (t >> 6) + 4 * m + 2 * ((t >> 5) & 1) + k; }
+        HMPC_HD u32 cols_pair_twiddle(int t, int m) { return static_cast<u32>(64 + (cols_pair_row(t, m, 0) >> 1)); }
+        // tile element of pair m, element k of thread t
+        HMPC_HD int cols_pair_elem(int t, int m, int k) { return (cols_pair_row(t, m, k) << (tile_log - 7)) + (t & 31); }
+
         struct pass_geom
         {
             int log_n, log_n1, log_n2;

@@ -482,11 +482,11 @@ void swizzles()
 
 // The streaming column passes of the radix-2^26 product (k_polymul_cols_stream, k_icols_stream in
 // hmpc_amd/csrc/fast_impl.hpp): 2^7-point columns, 32 columns per tile, 8 elements per thread, 512
-// threads. Their twiddles come from scalar loads at lane 0's index (full-width rounds: forward
-// stages 0-5, inverse stages 1-6), which needs the index to be the same on every lane of a wave,
-// and from a lane-0 / lane-63 pair (the inverse's stage 0), which needs at most those two values
-// per wave; their round-0 loads and last-round stores address element e0 + d as the offset of e0
-// plus the offset of d. Checked for every thread, unit and element at N = 2^14 .. 2^17 (every
+// threads. Their twiddles come from scalar loads at lane 0's index (3-stage rounds: forward stages
+// 0-5, inverse stages 1-6), which needs the index to be the same on every lane of a wave, and from a
+// lane-0 / lane-63 pair (the pair layout of forward stage 6 and inverse stage 0), which needs at most
+// those two values per wave; their direct loads and stores address element e0 + d as the offset of
+// e0 plus the offset of d. Checked for every thread, unit and element at N = 2^14 .. 2^17 (every
 // degree whose column length is 2^7 in the radix-2^26 product), together with: each pass reads and
 // writes every tile element exactly once, inside the tile's polynomial.
 template<int W>
@@ -527,8 +527,68 @@ void elements_once(tile_desc const& td, int a, bool inv, io_desc const& io, std:
         require(seen[e] == 1, std::string(what) + ": tile element not moved exactly once");
 }
 
+// the pair layout (cols_pair_row): every tile element once, inside its polynomial, offsets additive
+void pairs_once(io_desc const& io, std::size_t poly_lo, std::size_t N, char const* what)
+{
+    std::vector<int> seen(tile, 0);
+    for (int t = 0; t < 512; ++t)
+    {
+        int const e0 = cols_pair_elem(t, 0, 0);
+        for (int m = 0; m < 4; ++m)
+            for (int k = 0; k < 2; ++k)
+            {
+                int const e = cols_pair_elem(t, m, k);
+                int const d = e - e0;
+                require(e >= 0 && e < tile && d >= 0 && (e0 & d) == 0, std::string(what) + ": element is not e0 | d");
+                require(gidx_rel(io, e) == gidx_rel(io, e0) + gidx_rel(io, d), std::string(what) + ": offset not additive");
+                std::size_t const gi = io.base + gidx_rel(io, e);
+                require(gi >= poly_lo && gi < poly_lo + N, std::string(what) + ": element outside its polynomial");
+                if (e >= 0 && e < tile)
+                    ++seen[e];
+            }
+    }
+    for (int e = 0; e < tile; ++e)
+        require(seen[e] == 1, std::string(what) + ": tile element not moved exactly once");
+}
+
+// The 3-stage rounds next to the pair layout (forward stages 3-5, inverse stages 1-3) hold rows
+// 16 (t >> 6) + 2 k + ((t >> 5) & 1) of column t & 31; the half-wave exchange (element 2m + 1 of the
+// lower half-wave <-> element 2m of the upper half, as v_permlane32_swap does) turns that layout into
+// the pair layout and back, the pairs are the partners of forward stage 6 / inverse stage 0 (rows j
+// and j + 1, j even), and their twiddle is the generic stage index.
+void pair_exchange()
+{
+    tile_desc const td{7, tile_log - 7, false};
+    auto round_row = [&](int t, int k, bool inv) {
+        unit_t un = unit_of(td, inv ? 1 : 3, 3, inv, unit_index_at<8, 3, false>(t, 0));
+        require(static_cast<int>(un.c) == (t & 31), "3-stage round: unit column is not t & 31");
+        return static_cast<int>(j_of(un, static_cast<u32>(k), inv ? 1 : 3, 3, 7, inv));
+    };
+    for (int inv = 0; inv < 2; ++inv)
+        for (int t = 0; t < 512; ++t)
+        {
+            bool const upper = (t & 32) != 0;
+            for (int m = 0; m < 4; ++m)
+            {
+                // what the exchange leaves in this lane's elements 2m, 2m + 1
+                int const lo = upper ? t - 32 : t, hi = lo + 32;
+                int const after0 = upper ? round_row(lo, 2 * m + 1, inv != 0) : round_row(lo, 2 * m, inv != 0);
+                int const after1 = upper ? round_row(hi, 2 * m + 1, inv != 0) : round_row(hi, 2 * m, inv != 0);
+                require(after0 == cols_pair_row(t, m, 0) && after1 == cols_pair_row(t, m, 1),
+                        "half-wave exchange does not give the pair layout");
+                int const j = cols_pair_row(t, m, 0);
+                require((j & 1) == 0 && cols_pair_row(t, m, 1) == j + 1, "pair is not (j, j + 1), j even");
+                // forward stage 6 (base 1, F = j >> 1) and inverse stage 0 use the same table entry
+                u32 const fwd = tw_index<false, 1, 0>(1u, 1u, 6, static_cast<u32>(j >> 1), 0);
+                u32 const inv0 = tw_index<true, 1, 0>(1u, 1u << 6, 0, static_cast<u32>(j >> 1), 0);
+                require(cols_pair_twiddle(t, m) == fwd && cols_pair_twiddle(t, m) == inv0, "pair twiddle index");
+            }
+        }
+}
+
 void streaming_columns()
 {
+    pair_exchange();
     for (int log_n = 14; log_n <= 17; ++log_n)
     {
         pass_geom const G = geom_for(log_n, false);
@@ -543,8 +603,7 @@ void streaming_columns()
             require(tdd.s == td.s && tdd.log_seqs == td.log_seqs && !tdd.rows, "column tile geometry");
             std::size_t const poly_lo = (ct / G.blocks_per_poly) * N;
             elements_once<3>(td, 0, false, io, poly_lo, N, "forward round-0 loads");
-            elements_once<1>(td, 6, false, io, poly_lo, N, "forward last-round stores");
-            elements_once<1>(td, 0, true, io, poly_lo, N, "inverse round-0 loads");
+            pairs_once(io, poly_lo, N, "forward stage-6 stores / inverse stage-0 loads (pair layout)");
             elements_once<3>(td, 4, true, io, poly_lo, N, "inverse last-round stores");
         }
         for (int w = 0; w < 8; ++w)
@@ -556,14 +615,13 @@ void streaming_columns()
                 require(round_x<3>(td, 3, false, t, 0) == round_x<3>(td, 3, false, 64 * w, 0), "forward stages 3-5 not wave-uniform");
                 require(round_x<3>(td, 1, true, t, 0) == round_x<3>(td, 1, true, 64 * w, 0), "inverse stages 1-3 not wave-uniform");
                 require(round_x<3>(td, 4, true, t, 0) == round_x<3>(td, 4, true, 64 * w, 0), "inverse stages 4-6 not wave-uniform");
-                for (int g = 0; g < 4; ++g)
+                for (int m = 0; m < 4; ++m)
                 {
-                    // the inverse's stage 0: lane 0's or lane 63's index
-                    u32 const x = round_x<1>(td, 0, true, t, g);
-                    require(x == round_x<1>(td, 0, true, 64 * w, g) || x == round_x<1>(td, 0, true, 64 * w + 63, g),
-                            "inverse stage 0 has more than lane 0's and lane 63's twiddles");
-                    // twiddle indices stay inside the 2^7 column entries
-                    require(round_x<1>(td, 6, false, t, g) < 128u && x < 128u, "width-1 round twiddle index");
+                    // the pair layout's twiddles: lane 0's or lane 63's index, inside the 2^7 entries
+                    u32 const x = cols_pair_twiddle(t, m);
+                    require(x == cols_pair_twiddle(64 * w, m) || x == cols_pair_twiddle(64 * w + 63, m),
+                            "pair twiddles other than lane 0's and lane 63's");
+                    require(x < 128u, "pair twiddle index");
                 }
                 require(tw_index_round<false, 3, 2>(round_x<3>(td, 3, false, t, 0), 3) < 128u, "forward twiddle index");
                 require(tw_index_round<true, 3, 0>(round_x<3>(td, 1, true, t, 0), 3) < 128u, "inverse twiddle index");
