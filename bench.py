@@ -69,6 +69,23 @@ KERNEL_PASSES = {
 }
 
 
+# --no-autotune: every modulus keeps the library's default schedule (profiling runs: the tuner times
+# its trials, and under a profiler those timings do not reflect the device)
+AUTOTUNE = [True]
+
+
+def tune(mod, log_n, trials, stream):
+    if AUTOTUNE[0]:
+        return mod.autotune(log_n, trials, stream=stream)
+    return dict(mod.schedule(log_n), default=True)
+
+
+def tune_bgv(mod, log_n, trials, stream):
+    if AUTOTUNE[0]:
+        return mod.autotune_bgv(log_n, trials, stream=stream)
+    return {"default": True}
+
+
 def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
     mod = hm.Modulus(ctx, hm.Q128A)
     mod.prepare(log_n)
@@ -77,7 +94,7 @@ def bench_polymul(hm, ctx, stream, steps, warmup, n=512, log_n=16):
     a = mod.sample_uniform(KEY, 0, count)
     b = mod.sample_uniform(KEY, 1, count)
     out = mod.alloc(count)
-    tuned = mod.autotune(log_n, 64, stream=stream)
+    tuned = tune(mod, log_n, 64, stream)
     log(f"[bench] autotuned schedule: {tuned}")
     ctx.set_timing(True)
     ms, per, wall = time_steps(ctx, stream, lambda: mod.polymul(log_n, n, a, b, out, stream=stream), steps, warmup)
@@ -139,7 +156,7 @@ def bench_bgv(hm, ctx, stream, steps, warmup, n=1024, log_n=14, n_plain=32):
     A, B, s = keygen(mod, log_n, t_mont, ternary_s=True, e_bits=20)
     msgs = ctx.sample_words(KEY, 3, n * N)
     c0, c1 = mod.alloc(n * N), mod.alloc(n * N)
-    tuned = mod.autotune_bgv(log_n, 128, stream=stream)  # 128 < resident workgroups: its launches have their own grid
+    tuned = tune_bgv(mod, log_n, 128, stream)  # 128 < resident workgroups: its launches have their own grid
     log(f"[bench] autotuned BGV schedule: {tuned}")
 
     def timed_batch(A, B, s, bits):
@@ -211,8 +228,8 @@ def bench_c4(hm, ctx, stream, jobs=2000, seed=4):
         for ln in range(12, 17):
             m.prepare(ln)
             # trial sizes near the batch's per-class job counts (about 2000 / 20 classes)
-            tuned[f"{'q64' if qi == 0 else 'q128a'}/2^{ln}"] = {"polymul": m.autotune(ln, 64, stream=stream),
-                                                               "bgv": m.autotune_bgv(ln, 128, stream=stream)}
+            tuned[f"{'q64' if qi == 0 else 'q128a'}/2^{ln}"] = {"polymul": tune(m, ln, 64, stream),
+                                                               "bgv": tune_bgv(m, ln, 128, stream)}
             t_mont = m.mont(1 << 32)
             A, B, _ = keygen(m, ln, t_mont)
             keys[(qi, ln)] = (A, B, t_mont)
@@ -301,7 +318,7 @@ def bench_c5(hm, ctx, stream, requests=2000, seed=5, fractions=(0.1, 0.2, 0.3, 0
     keys = []
     for ln in (13, 14, 15, 16):
         mod.prepare(ln)
-        mod.autotune_bgv(ln, 64, stream=stream)
+        tune_bgv(mod, ln, 64, stream)
     srv = hm.Server(ctx, budget_bytes=200 << 30, slots=3)
     pk = {}
     for i in range(16):
@@ -553,7 +570,7 @@ def bench_ext(hm, ctx, stream, reps=5):
     msgs_h.array()[:] = ctx.sample_words(KEY, 3, n * N).to_numpy(n * N)
     c0_h, c1_h = ctx.host_buffer(16 * n * N), ctx.host_buffer(16 * n * N)
     msgs_d, c0, c1 = ctx.buffer(4 * n * N), bmod.alloc(n * N), bmod.alloc(n * N)
-    bmod.autotune_bgv(log_n, 128, stream=stream)
+    tune_bgv(bmod, log_n, 128, stream)
 
     def host_step():
         msgs_h.to_device(msgs_d, stream=stream)
@@ -675,11 +692,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: c1,c2,c3,c4,c5,ext,examples,copy (default all)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default schedules (for profiling runs)")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests per load point of the serving sweep (the full workload uses 100000)")
     ap.add_argument("--serving-operating-requests", type=int, default=100000,
                     help="requests of the run at the serving operating point (0: none)")
     args = ap.parse_args()
+    AUTOTUNE[0] = not args.no_autotune
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the workloads shard by independent batch elements); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)

@@ -344,20 +344,28 @@ struct hmpc_modulus_s
             return log_n <= lt ? 4 : 8;
         return default_elems(log_n);
     }
-    // tuned choice, else tiles for batches smaller than the device's resident workgroups
+    // tuned choice, else the tile launches (at 1024 ciphertexts of N = 2^14 they take 1.78 ms
+    // against 2.3 ms for the persistent kernel, and they are the tuner's choice at every batch size
+    // measured)
     bool bgv_tiles(int log_n, std::size_t n) const
     {
+        (void)n;
         if (log_n <= 12)
             return false;
         auto p = bgv_passes.find(log_n);
         if (p != bgv_passes.end())
             return p->second != 0;
-        return n < 256;
+        return true;
     }
+    // tuned choice, else 8 elements per thread for the tile launches (N > 4096: the tuner's choice
+    // there, and what the one-wave row kernels of the radix-2^26 moduli run at) and 4 (4 limbs) or 8
+    // for the persistent kernel (N <= 4096)
     int bgv_elems_for(int log_n) const
     {
         auto e = bgv_elems.find(log_n);
-        return e != bgv_elems.end() ? e->second : (limbs == 4 ? 4 : 8);
+        if (e != bgv_elems.end())
+            return e->second;
+        return log_n > 12 || limbs != 4 ? 8 : 4;
     }
     // tile launches over job tables (batches, serving): every two-pass BGV job unless the tuner
     // chose the persistent kernel for its degree; 8 elements per thread unless tuned

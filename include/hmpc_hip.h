@@ -120,8 +120,8 @@ int hmpc_modulus_set_kernels(hmpc_modulus mod, int variant);
 int hmpc_modulus_set_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int persistent);
 /* Schedule of fast-path BGV encryption for N = 2^log_n: elements per thread (0 = default) and, for
  * N > 4096, tiles = 1 (a column launch and a row launch over all ciphertexts' tiles), 0 (the
- * persistent job kernel, one workgroup per ciphertext) or -1 (default: tiles below 256
- * ciphertexts). Chosen by hmpc_autotune_bgv; results are identical. */
+ * persistent job kernel, one workgroup per ciphertext) or -1 (default: tiles, at 8 elements per
+ * thread). Chosen by hmpc_autotune_bgv; results are identical. */
 int hmpc_modulus_set_bgv_schedule(hmpc_modulus mod, int log_n, int elems_per_thread, int tiles);
 /* Current schedule and whether the fast kernels serve this (modulus, N)'s transforms and products. */
 int hmpc_modulus_get_schedule(hmpc_modulus mod, int log_n, int* elems_per_thread, int* persistent, int* fast);
@@ -136,7 +136,8 @@ int hmpc_autotune(hmpc_modulus mod, int log_n, size_t pairs, void* stream, int* 
  * column launch and a row launch over every ciphertext's tiles; 0: the persistent job kernel, one
  * workgroup per ciphertext) and installs the fastest for hmpc_bgv_encrypt (batch and serving jobs
  * use the persistent kernel with the chosen elements per thread). Untuned, hmpc_bgv_encrypt uses
- * tiles for batches below 256 ciphertexts. The context's RNG state is restored afterwards.
+ * the tile launches at 8 elements per thread for N > 4096. The context's RNG state is restored
+ * afterwards.
  * Blocking. */
 int hmpc_autotune_bgv(hmpc_context ctx, hmpc_modulus mod, int log_n, size_t n, void* stream, int* elems_per_thread,
                       int* tiles, float* ms);
