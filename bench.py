@@ -629,9 +629,10 @@ def roofline(kname, kavg, launches, kbytes, achieved, traffic, valu, step_gbs):
     kernel's VALU instruction count and that count needs a larger share of the duration than the
     HBM bytes do, the kernel is VALU-issue bound and the line says so: the achieved rate is then
     wave instructions per second against the peak of one wave instruction per SIMD every 4 cycles
-    at the 2.4 GHz peak engine clock (`frac`). The same at the engine clock measured under the
-    counters (in the counter run, not this one) is reported beside it (`frac_at_counter_clock`):
-    the integer load holds the clock near 2 GHz."""
+    at the 2.4 GHz peak engine clock (`frac`). Beside it: the engine clock measured under the
+    counters (in the counter run, on its box, not this one) and the time the instructions take at
+    that clock, to compare with the kernel's duration: the integer load holds the clock near
+    2 GHz."""
     hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
     common = {"traffic": traffic, "kernel": kname, "kernel_avg_ms": kavg, "launches": launches,
               "algorithmic_bytes_per_launch": kbytes, "step_hbm_gbs": step_gbs, "valu_issue": valu}
@@ -642,7 +643,7 @@ def roofline(kname, kavg, launches, kbytes, achieved, traffic, valu, step_gbs):
                "clock_ghz": VALU_CLOCK_GHZ, "hbm": hbm, **common}
         if valu.get("clock_ghz_measured"):
             out["counter_clock_ghz"] = valu["clock_ghz_measured"]
-            out["frac_at_counter_clock"] = valu["frac_at_measured_clock"]
+            out["issue_ms_at_counter_clock"] = valu["issue_ms_at_measured_clock"]
         return out
     return {"bound": "hbm", **hbm, **common}
 
