@@ -412,13 +412,19 @@ BATCH_JOBS = [("mul", Q64, 12), ("bgv", Q128A, 13), ("mul", Q128A, 16), ("bgv", 
               ("mul", Q128A, 15), ("bgv", Q128A, 15), ("mul", Q128A, 16), ("mul", Q128A, 17), ("bgv", Q128A, 17)]
 
 
-@pytest.mark.parametrize("structure", ["tiles", "jobs"])
-def test_batch_of_uneven_jobs_matches_oracle(ctx, structure):
+# the same without the jobs of 2^17: the q128a chunks' largest degree is then 2^16, so their row
+# launches run one wave per workgroup over mixed row lengths (with 2^17 jobs they run eight)
+BATCH_JOBS_16 = [j for j in BATCH_JOBS if j[2] <= 16]
+
+
+@pytest.mark.parametrize("structure,jobs", [("tiles", "to17"), ("jobs", "to17"), ("tiles", "to16")])
+def test_batch_of_uneven_jobs_matches_oracle(ctx, structure, jobs):
     """Persistent job kernels (one per modulus class) and, for jobs at N > 4096, tile launches
     over job tables (or, when so scheduled, the persistent kernel) over an uneven job list, eager
     and replayed from a hipGraph three times with host work between replays; every job's output
     equals the oracle's, and each encryption consumes the nonces a queue call would."""
-    two_pass = sorted({(kind, q, log_n) for kind, q, log_n in BATCH_JOBS if log_n > 12})
+    job_list = BATCH_JOBS if jobs == "to17" else BATCH_JOBS_16
+    two_pass = sorted({(kind, q, log_n) for kind, q, log_n in job_list if log_n > 12})
     if structure == "jobs":
         for kind, q, log_n in two_pass:
             gm = mods(ctx, q)[0]
@@ -427,7 +433,7 @@ def test_batch_of_uneven_jobs_matches_oracle(ctx, structure):
             else:
                 gm.set_schedule(log_n, 0, 1)
     try:
-        info = _run_uneven_batch(ctx)
+        info = _run_uneven_batch(ctx, job_list)
         tiled = len({(kind, q) for kind, q, _ in two_pass})
         assert info["classes"] >= (tiled if structure == "tiles" else 2)
     finally:
@@ -439,13 +445,13 @@ def test_batch_of_uneven_jobs_matches_oracle(ctx, structure):
                 gm.set_schedule(log_n, 0, -2)
 
 
-def _run_uneven_batch(ctx):
+def _run_uneven_batch(ctx, job_list):
     enc_key = [42, 0, 0, 0, 0, 0, 0, 1]
     ctx.set_rng(enc_key, [0, 0, 0])
     batch = hmpc_amd.Batch(ctx)
     checks = []
     nonce = np.zeros(3, np.uint32)
-    for i, (kind, q, log_n) in enumerate(BATCH_JOBS):
+    for i, (kind, q, log_n) in enumerate(job_list):
         gm, om = mods(ctx, q)
         N = 1 << log_n
         if kind == "mul":
@@ -466,7 +472,7 @@ def _run_uneven_batch(ctx):
             checks.append((gm, N, [(c0, want0), (c1, want1)]))
     assert ctx.rng[1].tolist() == list(nonce)
     for run, graph in enumerate((False, True, True, True)):
-        say(f"batch run={run} graph={graph} jobs={len(BATCH_JOBS)}")
+        say(f"batch run={run} graph={graph} jobs={len(job_list)}")
         for gm, N, outs in checks:
             for buf, _ in outs:  # eager work between replays: clear every output
                 gm.pointwise_scalar(gm.MUL, N, buf, np.zeros(gm.limbs, np.uint32), out=buf)
@@ -475,7 +481,7 @@ def _run_uneven_batch(ctx):
         ctx.wait()
         for k, (gm, N, outs) in enumerate(checks):
             for buf, want in outs:
-                assert np.array_equal(gm.download(buf, N), want), f"job {k} {BATCH_JOBS[k]} run {run}"
+                assert np.array_equal(gm.download(buf, N), want), f"job {k} {job_list[k]} run {run}"
     info = batch.info()
-    assert info["jobs"] == len(BATCH_JOBS) and info["classes"] >= 2
+    assert info["jobs"] == len(job_list) and info["classes"] >= 2
     return info
