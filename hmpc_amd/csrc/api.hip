@@ -298,6 +298,8 @@ struct table_pair
     u32* inv_em = nullptr;
     u32* fwd_fast = nullptr;  // r26 tables (owned) or aliases of the em tables
     u32* inv_fast = nullptr;
+    u32* fwd_tw = nullptr;    // r26: expanded twiddles of the first entries (owned with the r26 tables)
+    u32* inv_tw = nullptr;
     bool fast_owned = false;
     u32 root[max_limbs] = {};
     u32 inverse_root[max_limbs] = {};
@@ -420,7 +422,7 @@ struct hmpc_modulus_s
                 if (p)
                     (void)hipFree(p);
             if (t.fast_owned)
-                for (u32* p : {t.fwd_fast, t.inv_fast})
+                for (u32* p : {t.fwd_fast, t.inv_fast, t.fwd_tw, t.inv_tw})
                     if (p)
                         (void)hipFree(p);
         }
@@ -450,6 +452,8 @@ struct hmpc_modulus_s
         T.inv_em = t.inv_em;
         T.fwd_fast = t.fwd_fast;
         T.inv_fast = t.inv_fast;
+        T.fwd_tw = t.fwd_tw;
+        T.inv_tw = t.inv_tw;
         T.r26 = r26;
         std::copy(q2, q2 + max_limbs, T.q2);
         T.form = form;
@@ -598,7 +602,7 @@ int hmpc_modulus_s::prepare(int log_n)
             if (p)
                 (void)hipFree(p);
         if (t.fast_owned)
-            for (u32* p : {t.fwd_fast, t.inv_fast})
+            for (u32* p : {t.fwd_fast, t.inv_fast, t.fwd_tw, t.inv_tw})
                 if (p)
                     (void)hipFree(p);
     };
@@ -644,6 +648,13 @@ int hmpc_modulus_s::prepare(int log_n)
             t.fast_owned = true;
             if (!step(launch_r26_table(t.fwd, t.fwd_fast, std::size_t{1} << log_n, r26, nullptr), "radix-2^26 table") ||
                 !step(launch_r26_table(t.inv, t.inv_fast, std::size_t{1} << log_n, r26, nullptr), "radix-2^26 table"))
+                return true;
+            std::size_t const xn = std::min(r26_tw_entries, std::size_t{1} << log_n);
+            std::size_t const xb = std::size_t{4} * r26_tw_stride * xn;
+            if (!step(hipMalloc(&t.fwd_tw, xb), "hipMalloc (expanded twiddles)") ||
+                !step(hipMalloc(&t.inv_tw, xb), "hipMalloc (expanded twiddles)") ||
+                !step(launch_r26_tw_table(t.fwd_fast, t.fwd_tw, xn, r26, nullptr), "expanded twiddle table") ||
+                !step(launch_r26_tw_table(t.inv_fast, t.inv_tw, xn, r26, nullptr), "expanded twiddle table"))
                 return true;
         }
         else

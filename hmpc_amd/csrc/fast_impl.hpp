@@ -81,6 +81,7 @@ namespace hmpc_amd
         struct arith_r26
         {
             static constexpr int rl = 5;    // limbs per element in registers and LDS
+            static constexpr int twl = rl;  // words per twiddle in registers
             static constexpr int iol = 4;   // limbs per element in global memory (hmpc layout)
             static constexpr int wsl = 5;   // limbs per element in private workspaces (raw limbs)
             static constexpr int tws = 8;   // twiddle table stride in words (5 used)
@@ -239,6 +240,46 @@ namespace hmpc_amd
             static constexpr bool stage_prefetch = false;
         };
 
+        // the streaming column passes' 3-stage rounds: wave-uniform twiddles in the expanded form
+        // of r26_mont_tw (25 words per entry of the expanded tables, through scalar loads, the mads
+        // taking them from SGPRs): 38 instead of 54 instructions per product. Elements, LDS and
+        // global accesses as arith_r26.
+        struct arith_r26_tw : arith_r26
+        {
+            static constexpr int twl = r26_tw_words;
+
+            __device__ __forceinline__ static void tw_get(u32 (&w)[twl], u32 const* __restrict__ tw, u32 idx)
+            {
+                u32 const* p = tw + static_cast<std::size_t>(idx) * r26_tw_stride;
+#pragma unroll
+                for (int l = 0; l < twl; ++l)
+                    w[l] = p[l];
+            }
+
+            __device__ __forceinline__ static void fwd(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[twl], params const& P)
+            {
+                r26_forward_tw(x, y, w, P);
+            }
+
+            __device__ __forceinline__ static void inv(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w)[twl], params const& P,
+                                                       bool fold)
+            {
+                r26_inverse_tw(x, y, w, P, fold);
+            }
+
+            // the scaled last stage keeps mont with the stored twiddles (V_4 = w 2^130 of entries 0
+            // and 1; the loads of V_0..V_3 fall away): its two twiddles are the same for every tile,
+            // and held across the tile loop they take 10 SGPRs instead of 50, which the other
+            // stages' twiddles need (with 50 the inverse pass spilled SGPRs to VGPR lanes)
+            __device__ __forceinline__ static void inv_last(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w0)[twl],
+                                                            u32 const (&w1)[twl], params const& P)
+            {
+                u32 const s0[rl] = {w0[20], w0[21], w0[22], w0[23], w0[24]};
+                u32 const s1[rl] = {w1[20], w1[21], w1[22], w1[23], w1[24]};
+                r26_inverse_last(x, y, s0, s1, P);
+            }
+        };
+
         // The LDS slot order of the row kernels (rows of 2^7 .. 2^10 elements). With 64 banks of 4
         // bytes, element slots of 5 words and E = 8 every gather and scatter of every round of these
         // rows, and the 64 consecutive elements of a tile load or store, reach 64 distinct banks
@@ -323,6 +364,7 @@ namespace hmpc_amd
         struct arith_gold
         {
             static constexpr int rl = 2;
+            static constexpr int twl = rl;
             static constexpr int iol = 2;
             static constexpr int wsl = 2;
             static constexpr int tws = 2;
@@ -445,6 +487,7 @@ namespace hmpc_amd
         struct arith_cios
         {
             static constexpr int rl = L;
+            static constexpr int twl = rl;
             static constexpr int iol = L;
             static constexpr int wsl = L;
             static constexpr int tws = L;
@@ -742,7 +785,7 @@ namespace hmpc_amd
         }
 
         template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void prefetch_stage(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], u32 const (&xr)[E >> W],
+        __device__ __forceinline__ void prefetch_stage(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::twl], u32 const (&xr)[E >> W],
                                                        u32 const* __restrict__ tw)
         {
             constexpr int count = Inv ? (1 << (W - II - 1)) : (1 << II);
@@ -755,7 +798,7 @@ namespace hmpc_amd
         }
 
         template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void prefetch_upto(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl], u32 const (&xr)[E >> W],
+        __device__ __forceinline__ void prefetch_upto(u32 (&tw_regs)[E >> W][(1 << W) - 1][A::twl], u32 const (&xr)[E >> W],
                                                       u32 const* __restrict__ tw)
         {
             if constexpr (II < W - 1)
@@ -768,8 +811,8 @@ namespace hmpc_amd
         // g0: index of the round's first stage within the whole inverse transform (it selects the
         // stages that fold: r26_inverse_fold_stage)
         template<typename A, int E, bool Inv, int W, int II, bool Last>
-        __device__ __forceinline__ void stage(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
-                                              u32 const (&w0)[A::rl], u32 const (&w1)[A::rl], typename A::params const& P,
+        __device__ __forceinline__ void stage(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::twl],
+                                              u32 const (&w0)[A::twl], u32 const (&w1)[A::twl], typename A::params const& P,
                                               int g0)
         {
             constexpr int K = 1 << W;
@@ -795,8 +838,8 @@ namespace hmpc_amd
         }
 
         template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void stages_upto(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
-                                                    u32 const (&w0)[A::rl], u32 const (&w1)[A::rl],
+        __device__ __forceinline__ void stages_upto(u32 (&x)[E][A::rl], u32 const (&tw_regs)[E >> W][(1 << W) - 1][A::twl],
+                                                    u32 const (&w0)[A::twl], u32 const (&w1)[A::twl],
                                                     typename A::params const& P, int g0)
         {
             if constexpr (II < W - 1)
@@ -808,13 +851,13 @@ namespace hmpc_amd
 
         // stages II .. W-2 of a round, each preceded by the loads of its own twiddles
         template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void stages_prefetched(u32 (&x)[E][A::rl], u32 (&tw_regs)[E >> W][(1 << W) - 1][A::rl],
+        __device__ __forceinline__ void stages_prefetched(u32 (&x)[E][A::rl], u32 (&tw_regs)[E >> W][(1 << W) - 1][A::twl],
                                                           u32 const (&xr)[E >> W], u32 const* __restrict__ tw,
                                                           typename A::params const& P, int g0)
         {
             if constexpr (II < W - 1)
             {
-                u32 const z[A::rl] = {};
+                u32 const z[A::twl] = {};
                 prefetch_stage<A, E, Inv, W, II>(tw_regs, xr, tw);
                 stage<A, E, Inv, W, II, false>(x, tw_regs, z, z, P, g0);
                 stages_prefetched<A, E, Inv, W, II + 1>(x, tw_regs, xr, tw, P, g0);
@@ -830,9 +873,9 @@ namespace hmpc_amd
                                                       u32 base_step, u32 const* __restrict__ tw,
                                                       typename A::params const& P, bool final_scale, int off)
         {
-            u32 tw_regs[E >> W][(1 << W) - 1][A::rl];
+            u32 tw_regs[E >> W][(1 << W) - 1][A::twl];
             bool const scaled = Inv && final_scale && (a + W == td.s);
-            u32 w0[A::rl] = {}, w1[A::rl] = {};
+            u32 w0[A::twl] = {}, w1[A::twl] = {};
             int const g0 = off + a;
             u32 xr[E >> W];
             round_bases<A, E, Inv, W>(xr, td, a, base0, base_step);
@@ -1378,7 +1421,7 @@ namespace hmpc_amd
 
         // a twiddle whose index is the same on every lane of the wave
         template<typename A>
-        __device__ __forceinline__ void tw_get_wave(u32 (&w)[A::rl], u32 const* __restrict__ tw, u32 idx)
+        __device__ __forceinline__ void tw_get_wave(u32 (&w)[A::twl], u32 const* __restrict__ tw, u32 idx)
         {
             A::tw_get(w, tw, __builtin_amdgcn_readfirstlane(idx));
         }
@@ -1386,23 +1429,76 @@ namespace hmpc_amd
         // a twiddle whose index takes at most two values on the wave, lane 0's and lane 63's
         // (two scalar loads and a per-lane select)
         template<typename A>
-        __device__ __forceinline__ void tw_get_wave2(u32 (&w)[A::rl], u32 const* __restrict__ tw, u32 idx)
+        __device__ __forceinline__ void tw_get_wave2(u32 (&w)[A::twl], u32 const* __restrict__ tw, u32 idx)
         {
             u32 const i0 = __builtin_amdgcn_readfirstlane(idx);
             u32 const i1 = __builtin_amdgcn_readlane(idx, 63);
-            u32 w0[A::rl], w1[A::rl];
+            u32 w0[A::twl], w1[A::twl];
             A::tw_get(w0, tw, i0);
             A::tw_get(w1, tw, i1);
             bool const first = idx == i0;
 #pragma unroll
-            for (int l = 0; l < A::rl; ++l)
+            for (int l = 0; l < A::twl; ++l)
                 w[l] = first ? w0[l] : w1[l];
+        }
+
+        // The expanded twiddles of the columns' rounds are the same for every tile, and hoisted out
+        // of the tile loop they would hold 25 SGPRs each for the whole loop (150 in the forward
+        // pass) and spill to VGPR lanes, one v_readlane per use. Adding an opaque zero (an empty
+        // asm) to the table pointer at the top of each iteration keeps each stage's scalar loads in
+        // the loop (scalar-cache hits), so its twiddles live only through that stage. The pointer
+        // stays an offset of the kernel's restrict argument: an opaque pointer would lose the
+        // no-clobber proof that lets uniform loads be scalar loads.
+        __device__ __forceinline__ u32 const* opaque_table(u32 const* tw)
+        {
+            u32 z = 0;
+            asm volatile("" : "+s"(z));
+            return tw + z;
+        }
+
+        // stage II of a full-width round with expanded twiddles (A::twl > A::rl), each loaded just
+        // before the first butterfly that uses it. Loaded a stage at a time they held up to four
+        // expanded twiddles (100 SGPRs) and spilled SGPRs to VGPR lanes; here an empty asm makes the
+        // next load's index depend on the butterfly just done (through an SGPR that stays zero), so
+        // one expanded twiddle is live at a time. Butterfly k uses entry m = slot(k) - first of the
+        // stage, which does not decrease with k.
+        template<typename A, int E, bool Inv, int W, int II>
+        __device__ __forceinline__ void stage_each(u32 (&x)[E][A::rl], u32 X, u32 const* __restrict__ tw,
+                                                   typename A::params const& P, int g0)
+        {
+            static_assert((1 << W) == E, "full-width round");
+            constexpr int K = 1 << W;
+            constexpr int lb = Inv ? II : (W - 1 - II);
+            constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
+            bool const fold = r26_inverse_fold_stage(g0 + II);
+            u32 dep = 0;
+            u32 w[A::twl];
+            int loaded = -1;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+            {
+                if (k & (1 << lb))
+                    continue;
+                int const m = tw_slot<W>(Inv, II, k) - first;
+                if (m != loaded)
+                {
+                    tw_get_wave<A>(w, tw, tw_index_round<Inv, W, II>(X, m) + dep);
+                    loaded = m;
+                }
+                u32 (&lo)[A::rl] = x[k];
+                u32 (&hi)[A::rl] = x[k | (1 << lb)];
+                if constexpr (!Inv)
+                    A::fwd(lo, hi, w, P);
+                else
+                    A::inv(lo, hi, w, P, fold);
+                asm volatile("" : "+s"(dep) : "v"(hi[4]));
+            }
         }
 
         // stages II.. of a full-width round (one unit per thread) with per-wave twiddles; with
         // `scaled` the inverse's last stage is the scaled final stage (twiddles 0 and 1)
         template<typename A, int E, bool Inv, int W, int II>
-        __device__ __forceinline__ void stages_wave(u32 (&x)[E][A::rl], u32 (&twr)[1][(1 << W) - 1][A::rl], u32 X,
+        __device__ __forceinline__ void stages_wave(u32 (&x)[E][A::rl], u32 (&twr)[1][(1 << W) - 1][A::twl], u32 X,
                                                     u32 const* __restrict__ tw, typename A::params const& P, int g0,
                                                     bool scaled)
         {
@@ -1412,14 +1508,16 @@ namespace hmpc_amd
                 constexpr int first = Inv ? ((1 << W) - (1 << (W - II))) : ((1 << II) - 1);
                 if (Inv && II == W - 1 && scaled)
                 {
-                    u32 w0[A::rl], w1[A::rl];
+                    u32 w0[A::twl], w1[A::twl];
                     tw_get_wave<A>(w0, tw, 0u);
                     tw_get_wave<A>(w1, tw, 1u);
                     stage<A, E, Inv, W, II, true>(x, twr, w0, w1, P, g0);
                 }
+                else if constexpr (A::twl > A::rl)
+                    stage_each<A, E, Inv, W, II>(x, X, tw, P, g0);
                 else
                 {
-                    u32 const z[A::rl] = {};
+                    u32 const z[A::twl] = {};
 #pragma unroll
                     for (int m = 0; m < count; ++m)
                         tw_get_wave<A>(twr[0][first + m], tw, tw_index_round<Inv, W, II>(X, m));
@@ -1434,7 +1532,7 @@ namespace hmpc_amd
                                                    typename A::params const& P, int g0, bool scaled)
         {
             static_assert((1 << W) == E, "full-width round");
-            u32 twr[1][(1 << W) - 1][A::rl];
+            u32 twr[1][(1 << W) - 1][A::twl];
             u32 xr[1];
             round_bases<A, E, Inv, W>(xr, td, a, 1u, 0u);
             stages_wave<A, E, Inv, W, 0>(x, twr, xr[0], tw, P, g0, scaled);
@@ -1471,10 +1569,12 @@ namespace hmpc_amd
         template<typename A, int E, int S>
         __global__ void __launch_bounds__(tile / E, 4) k_polymul_cols_stream(u32 const* __restrict__ a, u32 const* __restrict__ b,
                                                                              u32* __restrict__ wa, u32* __restrict__ wb,
-                                                                             u32 const* __restrict__ tw, pass_geom G, u64 n,
-                                                                             typename A::params P, u32 tiles)
+                                                                             u32 const* __restrict__ tw, u32 const* __restrict__ twx,
+                                                                             pass_geom G, u64 n, typename A::params P, u32 tiles)
         {
             static_assert(E == 8 && S == 7, "streaming column geometry: 2^7-point columns, 8 elements per thread");
+            static_assert(std::is_same<A, arith_r26>::value, "the rounds take expanded twiddles (arith_r26_tw)");
+            using AT = arith_r26_tw;
             extern __shared__ __align__(16) u32 lds[];
             std::size_t const count = static_cast<std::size_t>(n) << G.log_n;
             tile_desc const td{S, tile_log - S, false};
@@ -1533,15 +1633,16 @@ namespace hmpc_amd
                         for (int l = 0; l < 4; ++l)
                             raw[k][l] = ld_off(src + l * count + gidx_rel(ion, unit_offset(td, 0, 3, false, k)), o_in);
                 }
-                round_wave<A, E, false, 3>(x, td, 0, tw, P, 0, false);
+                u32 const* const twt = opaque_table(twx);
+                round_wave<AT, E, false, 3>(x, td, 0, twt, P, 0, false);
                 __syncthreads();  // the previous tile's gather is done
                 scatter<A, E, 3>(x, lds, td, 0, false);
                 __syncthreads();
                 gather<A, E, 3>(x, lds, td, 3, false);
-                round_wave<A, E, false, 3>(x, td, 3, tw, P, 0, false);
+                round_wave<AT, E, false, 3>(x, td, 3, twt, P, 0, false);
                 cols_pair_swap<A, E>(x);
                 {
-                    u32 const z[A::rl] = {};
+                    u32 const z[A::twl] = {};
                     stage<A, E, false, 1, 0, false>(x, tw6, z, z, P, 0);
                 }
 #pragma unroll
@@ -1562,10 +1663,12 @@ namespace hmpc_amd
         // inverse stages the row pass ran before
         template<typename A, int E, int S>
         __global__ void __launch_bounds__(tile / E, 4) k_icols_stream(u32 const* __restrict__ in, u32* __restrict__ out,
-                                                                      u32 const* __restrict__ tw, pass_geom G, u64 n,
-                                                                      typename A::params P, u32 tiles)
+                                                                      u32 const* __restrict__ tw, u32 const* __restrict__ twx,
+                                                                      pass_geom G, u64 n, typename A::params P, u32 tiles)
         {
             static_assert(E == 8 && S == 7, "streaming column geometry: 2^7-point columns, 8 elements per thread");
+            static_assert(std::is_same<A, arith_r26>::value, "the rounds take expanded twiddles (arith_r26_tw)");
+            using AT = arith_r26_tw;
             extern __shared__ __align__(16) u32 lds[];
             std::size_t const count = static_cast<std::size_t>(n) << G.log_n;
             tile_desc const td{S, tile_log - S, false};
@@ -1611,7 +1714,7 @@ namespace hmpc_amd
                 }
                 {
                     // stage 0: two twiddles per wave and pair (lanes 0-31 and 32-63)
-                    u32 const z[A::rl] = {};
+                    u32 const z[A::twl] = {};
                     u32 tw1[E / 2][1][A::rl];
 #pragma unroll
                     for (int m = 0; m < E / 2; ++m)
@@ -1619,12 +1722,13 @@ namespace hmpc_amd
                     stage<A, E, true, 1, 0, false>(x, tw1, z, z, P, off);
                 }
                 cols_pair_swap<A, E>(x);
-                round_wave<A, E, true, 3>(x, td, 1, tw, P, off + 1, false);
+                u32 const* const twt = opaque_table(twx);
+                round_wave<AT, E, true, 3>(x, td, 1, twt, P, off + 1, false);
                 __syncthreads();  // the previous tile's gather is done
                 scatter<A, E, 3>(x, lds, td, 1, true);
                 __syncthreads();
                 gather<A, E, 3>(x, lds, td, 4, true);
-                round_wave<A, E, true, 3>(x, td, 4, tw, P, off + 4, true);
+                round_wave<AT, E, true, 3>(x, td, 4, twt, P, off + 4, true);
 #pragma unroll
                 for (int k = 0; k < E; ++k)
                 {
@@ -1851,7 +1955,7 @@ namespace hmpc_amd
                 if constexpr (std::is_same<A, arith_r26>::value && E == 8)
                 {
                     int const cus = device_cus();
-                    if (gc.log_n1 == 7 && cus > 0)
+                    if (gc.log_n1 == 7 && cus > 0 && T.fwd_tw && T.inv_tw)
                     {
                         if ((e = allow_lds(k_polymul_cols_stream<A, E, 7>, bytes)) != hipSuccess ||
                             (e = allow_lds(k_icols_stream<A, E, 7>, bytes)) != hipSuccess)
@@ -1866,7 +1970,7 @@ namespace hmpc_amd
                     {
                         if (stream_cols)
                             k_polymul_cols_stream<A, E, 7><<<grid_s < 2 * grid_c ? grid_s : 2 * grid_c, TH, bytes, s>>>(
-                                a, b, wa, wb, T.fwd_fast, gc, n, P, 2 * grid_c);
+                                a, b, wa, wb, T.fwd_fast, T.fwd_tw, gc, n, P, 2 * grid_c);
                         else
                             cols_kernel<<<2 * grid_c, TH, bytes, s>>>(a, b, wa, wb, T.fwd_fast, gc, n, P);
                     }
@@ -1882,8 +1986,8 @@ namespace hmpc_amd
                     if constexpr (std::is_same<A, arith_r26>::value && E == 8)
                     {
                         if (stream_cols)
-                            k_icols_stream<A, E, 7><<<grid_s < grid_c ? grid_s : grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n,
-                                                                                                      P, grid_c);
+                            k_icols_stream<A, E, 7><<<grid_s < grid_c ? grid_s : grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, T.inv_tw,
+                                                                                                      gc, n, P, grid_c);
                         else
                             icols_kernel<<<grid_c, TH, bytes, s>>>(wa, out, T.inv_fast, gc, n, P);
                     }

@@ -201,6 +201,30 @@ namespace hmpc_amd
         return hipGetLastError();
     }
 
+    // the first `count` entries of a radix-2^26 table (stride 8) in the expanded form of
+    // r26_mont_tw (25 words per entry, stride r26_tw_stride, the rest zero)
+    __global__ void __launch_bounds__(block_threads) k_r26_tw_table(u32 const* __restrict__ fast, u32* __restrict__ out, u32 count,
+                                                                    p26 P)
+    {
+        u32 e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= count)
+            return;
+        u32 W[5], V[r26_tw_words];
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+            W[l] = fast[8 * e + l];
+        r26_expand_twiddle(V, W, P);
+#pragma unroll
+        for (int l = 0; l < r26_tw_stride; ++l)
+            out[r26_tw_stride * e + l] = l < r26_tw_words ? V[l] : 0u;
+    }
+
+    hipError_t launch_r26_tw_table(u32 const* fast, u32* out, std::size_t count, p26 const& P, hipStream_t s)
+    {
+        k_r26_tw_table<<<grid_for(count), block_threads, 0, s>>>(fast, out, static_cast<u32>(count), P);
+        return hipGetLastError();
+    }
+
     // ------------------------------------------------------------------ plain path (hmpc's launch structure)
     // K2 / K5: first stage, [n][N] -> transposed scratch [N][n]
     template<int L, bool Inv>

@@ -50,6 +50,8 @@ namespace hmpc_amd
         const u32* inv_em;
         const u32* fwd_fast;   // tables in the fast path's format (r26: [N][8] words)
         const u32* inv_fast;
+        const u32* fwd_tw;     // r26: the first r26_tw_entries entries of fwd_fast / inv_fast expanded
+        const u32* inv_tw;     // for r26_mont_tw (r26_tw_stride words each); null for other forms
         u32 q2[max_limbs];     // 2q
         int form;              // arith_form of the fast path
         int elems;             // fast path: elements per thread (4, 8 or 16)
@@ -71,6 +73,10 @@ namespace hmpc_amd
     hipError_t launch_to_element_major(int L, u32 const* planar, u32* em, std::size_t count, hipStream_t s);
     // [4][N] canonical table of w R -> [N][8] radix-2^26 limbs of w 2^130 mod q
     hipError_t launch_r26_table(u32 const* planar, u32* out, std::size_t count, p26 const& P, hipStream_t s);
+    // entries of the expanded twiddle tables (r26_mont_tw): the twiddles of the first 7 forward
+    // stages and of the inverse's last 7, which the streaming column passes take per wave
+    constexpr std::size_t r26_tw_entries = 128;
+    hipError_t launch_r26_tw_table(u32 const* fast, u32* out, std::size_t count, p26 const& P, hipStream_t s);
 
     // fast path (L = 2, 4): register-resident radix-16 rounds, see kernels_fast.hip
     bool fast_path_supported(int L, int form, int log_n);

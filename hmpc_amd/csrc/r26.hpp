@@ -13,25 +13,38 @@
 // hmpc's representation a*R mod q (R = 2^128); twiddles are stored as w * 2^130 mod q so that
 // mont(a R, w 2^130) = (a w) R. A product of two data values is corrected by 4 = R' / R.
 //
-// Bounds (q > 2^124, so 64 q > 2^130):
+// Bounds (q > 2^124, so 64 q > 2^130). A signed-digit value with |x_j| < 2^31 for j < 4 and
+// |x_4| < 2^30 - 2^6 lies within (|x_4| + 2^6) 2^104 < 1024 q of zero, so limb bounds bound values:
 //   mont(a, b): a signed-digit (|a_j| < 2^31), b in [0, q) with limbs < 2^27: column sums stay
 //               below 2^61 in magnitude; the result (T + M q) / 2^130 has digits r_0..r_3 in
 //               [0, 2^26) and a signed top limb, and lies in (-|a| q / 2^130, |a| q / 2^130 +
 //               q (1 + 2^-19)): within (-q, 2q (1 + 2^-19)) for |a| < 64 q, in [0, 2q (1 + 2^-19))
-//               with r_4 < 2^22 for a >= 0, v(a) v(b) < 2^130 q. With b in [0, 2q) and limbs < 2^28
+//               with r_4 < 2^22 for a >= 0, v(a) v(b) < 2^130 q; for any such a,
+//               |r_4| < (|a_4| + 2^6) / 2^6 + 2^20 + 2^2. With b in [0, 2q) and limbs < 2^28
 //               (a folded value) and |a_j| < 2^30.2 the sums stay below 2^61 as well and the result
 //               lies in (-2|a| q / 2^130, 2|a| q / 2^130 + q (1 + 2^-19))
+//   mont_tw(a, V): a times a twiddle w held in its expanded form V_i = 2^(26 (i + 1)) w mod q,
+//               i = 0..4 (canonical limbs: < 2^26, top limb <= 2^20; V_4 is the stored w 2^130):
+//               z = sum_i a_i V_i = a w 2^26 mod q collects in five columns, and one Montgomery
+//               round divides by 2^26. For |a_j| <= A < 2^31 the columns stay below 2^59 in
+//               magnitude; the result has digits 0..3 in [0, 2^26) and |r_4| < 5 A / 2^6 + 2^20
+//               + 2^8: its value is bounded by a's limbs, not by a's value.
 //   forward butterfly (x + t, x - t), t = mont(y, w): |value| grows by < 2q (1 + 2^-18) per stage,
-//               limbs 0..3 by < 2^26 and limb 4 by < 2^22. From inputs in [0, q) with limbs
-//               (< 2^27, < 2^26, < 2^26, < 2^26, < 2^22) (canonical data, or from_signed
-//               samples) S stages stay below q + 2.0001 q S in magnitude with |limbs| < (S + 2) x
-//               those: a mont input needs |value| < 64 q and |limbs| < 2^31, so up to 29 stages
-//               (a whole transform of N <= 2^29, both of its passes) run without reductions
+//               limbs 0..3 by < 2^26 and limb 4 by < 2^22 while |y| < 64 q. From inputs in [0, q)
+//               with limbs (< 2^27, < 2^26, < 2^26, < 2^26, < 2^22) (canonical data, or
+//               from_signed samples) S stages stay below q + 2.0001 q S in magnitude with |limbs| <
+//               (S + 2) x those: a mont input needs |limbs| < 2^31, so up to 29 stages (a whole
+//               transform of N <= 2^29, both of its passes) run without reductions. With
+//               t = mont_tw(y, V) limb 4 grows by < 5 A / 2^6 + 2^20 + 2^8 per stage instead
+//               (A: y's largest limb); the streaming column passes' schedule (their 3-stage
+//               rounds on mont_tw, the rest on mont) is checked stage by stage for N = 2^14..2^17
+//               in tests/test_r26_host.py: every mont and mont_tw input keeps |limbs| < 2^30.2,
+//               every folded sum |x_4| < 2^29.6
 //   fold(x):    normalize, then x_4 >= 2^20 carries h = x_4 >> 20 copies of 2^124 = q - c:
 //               replace by -h c (plus one q in borrowed digits to keep limbs >= 0); result < 2q
-//               with limbs < 2^28, for any limbs < 2^31. fold_signed adds 64 q first, so that a
-//               signed-digit value (|value| < 64 q, |x_j| < 2^31 - 2^26, |x_4| < 2^29) folds the
-//               same way
+//               with limbs < 2^28, for any limbs < 2^31. fold_signed adds 1024 q = 2^134 + 1024 c
+//               first (limb 4 += 2^30), so that a signed-digit value (|x_j| < 2^31 - 2^26 for
+//               j < 4, |x_4| < 2^30 - 2^6) folds the same way
 //   inverse butterfly (x + y, mont(x - y, w)) on signed digits, x + y folded (signed) at stages
 //               1, 4, 7, ... of the transform (r26_inverse_fold_stage). Folded values are in [0, 2q)
 //               with limbs < 2^27.2 and mont outputs within (-q, 2q (1 + 2^-19)) with digits 0..3
@@ -41,7 +54,9 @@
 //               |a| < 35.01q with |limbs| < 2^30.2 after at most 17 stages, times a folded one:
 //               within (-4.4q, 8.4q), digits 0..3 < 2^28 and |top limb| < 2^24 after the x4
 //               correction) stage 0 gives < 16.8q, and stage 1's sums stay within 33.6q with
-//               |limbs| < 2^30. Every mont input stays within 33.6q with |limbs| < 2^30.
+//               |limbs| < 2^30. Every mont input stays within 33.6q with |limbs| < 2^30. With
+//               mont_tw the top limbs of the sums grow faster (by the bound above); the fold
+//               stages stay the same and the checked schedule keeps them within the signed fold.
 #pragma once
 
 #include "modarith.hpp"
@@ -57,7 +72,7 @@ namespace hmpc_amd
         u32 c;         // q = 2^124 + c
         u32 n26;       // -q^-1 mod 2^26
         u32 qb[5];     // q with limbs qb_0 >= 2^26, qb_1 >= 2^10 (bias of the fold)
-        u32 b64[2];    // 64 c as two radix-2^26 limbs (with 2^26 in limb 4: 64 q, the signed fold's bias)
+        u32 b1024[2];  // 1024 c as two radix-2^26 limbs (with 2^30 in limb 4: 1024 q, the signed fold's bias)
         u32 q32[4];    // q as 32-bit limbs (final canonical subtraction)
         u32 rr[5];     // R 2^130 mod q (R = 2^128), normalized: mont(z, rr) = z R mod q
         u32 t20;       // 2^20 (top radix-2^26 limb of q), kept in a register so that m * 2^20 + T
@@ -77,8 +92,9 @@ namespace hmpc_amd
         {
             R.qb[i] = qb[i];
         }
-        R.b64[0] = (64u * c) & m26;
-        R.b64[1] = (64u * c) >> 26;
+        u64 const c1024 = static_cast<u64>(c) << 10;  // < 2^34
+        R.b1024[0] = static_cast<u32>(c1024) & m26;
+        R.b1024[1] = static_cast<u32>(c1024 >> 26);
         R.q32[0] = c;
         R.q32[1] = 0;
         R.q32[2] = 0;
@@ -140,15 +156,15 @@ namespace hmpc_amd
         r26_fold_top(x, P);
     }
 
-    // signed-digit value (|x_j| < 2^31 - 2^26 for j < 4, |x_4| < 2^29, |value| < 64 q) -> congruent
-    // value in [0, 2q), limbs < 2^28: 64 q = 2^130 + 64 c is added first (limb 4 += 2^26, 64 c into
+    // signed-digit value (|x_j| < 2^31 - 2^26 for j < 4, |x_4| < 2^30 - 2^6) -> congruent value in
+    // [0, 2q), limbs < 2^28: 1024 q = 2^134 + 1024 c is added first (limb 4 += 2^30, 1024 c into
     // limbs 0 and 1), which makes the value positive; the signed carry chain then leaves digits
-    // 0..3 in [0, 2^26) and limb 4 in [0, 2^28)
+    // 0..3 in [0, 2^26) and limb 4 in [0, 2^31), the range of r26_fold_top
     HMPC_HD void r26_fold_signed(u32 (&x)[5], p26 const& P)
     {
-        x[0] += P.b64[0];
-        x[1] += P.b64[1];
-        x[4] += 1u << 26;
+        x[0] += P.b1024[0];
+        x[1] += P.b1024[1];
+        x[4] += 1u << 30;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
@@ -282,6 +298,76 @@ namespace hmpc_amd
         r26_mont_signed(y, d, w1, P);
     }
 
+    // ---- products by a twiddle in expanded form
+    // V holds the twiddle w as five canonical values V_i = 2^(26 (i + 1)) w mod q, limb j of V_i at
+    // V[5 i + j] (r26_expand_twiddle). a w 2^26 = sum_i a_i V_i (mod q) then collects in columns
+    // 0..4 only: 25 mads, and a single Montgomery round (m = z_0 (-q^-1) mod 2^26, z + m q,
+    // divided by 2^26) returns a w, 38 instructions against mont's 54. The cost is the twiddle's
+    // size (25 words): for wave-uniform twiddles it comes from scalar loads and the mads take
+    // its limbs from SGPRs. Bounds in the header.
+    constexpr int r26_tw_words = 25;     // words of an expanded twiddle
+    constexpr int r26_tw_stride = 32;    // table stride in words
+
+    HMPC_HD void r26_mont_tw(u32 (&r)[5], u32 const (&a)[5], u32 const (&V)[r26_tw_words], p26 const& P)
+    {
+        long long t = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            t = static_cast<long long>(static_cast<int>(a[i])) * static_cast<long long>(static_cast<int>(V[5 * i])) + t;
+            r26_pin(t);
+        }
+        // masked to 26 bits, so that m 2^20 adds less than 2^20 to the top limb
+        u32 const m = (static_cast<u32>(t) * P.n26) & m26;
+        t = static_cast<long long>(static_cast<u64>(m) * P.c + static_cast<u64>(t));
+        long long carry = t >> 26;
+#pragma unroll
+        for (int k = 1; k < 5; ++k)
+        {
+            t = carry;
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+            {
+                t = static_cast<long long>(static_cast<int>(a[i])) * static_cast<long long>(static_cast<int>(V[5 * i + k])) + t;
+                r26_pin(t);
+            }
+            if (k == 4)
+                t = static_cast<long long>(static_cast<u64>(m) * P.t20 + static_cast<u64>(t));
+            r[k - 1] = static_cast<u32>(t) & m26;
+            carry = t >> 26;
+        }
+        r[4] = static_cast<u32>(carry);
+    }
+
+    // the butterflies with expanded twiddles (the same values as r26_forward and r26_inverse,
+    // modulo q)
+    HMPC_HD void r26_forward_tw(u32 (&x)[5], u32 (&y)[5], u32 const (&V)[r26_tw_words], p26 const& P)
+    {
+        u32 t[5];
+        r26_mont_tw(t, y, V, P);
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+        {
+            u32 xl = x[l];
+            x[l] = xl + t[l];
+            y[l] = xl - t[l];
+        }
+    }
+
+    HMPC_HD void r26_inverse_tw(u32 (&x)[5], u32 (&y)[5], u32 const (&V)[r26_tw_words], p26 const& P, bool fold)
+    {
+        u32 d[5];
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+        {
+            d[l] = x[l] - y[l];
+            x[l] = x[l] + y[l];
+        }
+        if (fold)
+            r26_fold_signed(x, P);
+        r26_mont_tw(y, d, V, P);
+    }
+
     // point-wise product of two forward-transform values a R, b R (signed digits) -> a b R: b is
     // folded to [0, 2q), a stays the signed operand, and the Montgomery product is corrected by
     // R' / R = 4. For |a| < 35.01 q with |a_j| < 2^30.2 (forward outputs of up to 17 stages) the
@@ -331,5 +417,32 @@ namespace hmpc_amd
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             w[i] = take ? d[i] : w[i];
+    }
+
+    // a twiddle table entry W = w 2^130 mod q (canonical radix-2^26 limbs) -> its expanded form
+    // for r26_mont_tw: V_i = 2^(26 (i + 1)) w mod q is mont(W, 2^(26 (i + 1))) for i < 4 (a unit
+    // limb as the second operand) and W itself for i = 4; each made canonical (fold, final
+    // subtraction, unpacked again)
+    HMPC_HD void r26_expand_twiddle(u32 (&V)[r26_tw_words], u32 const (&W)[5], p26 const& P)
+    {
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+        {
+            u32 v[5] = {W[0], W[1], W[2], W[3], W[4]};
+            if (i < 4)
+            {
+                u32 e[5] = {0u, 0u, 0u, 0u, 0u};
+                e[i + 1] = 1u;
+                r26_mont(v, W, e, P);
+            }
+            r26_fold(v, P);
+            u32 w[4];
+            r26_canonical(w, v, P);
+            u32 x[5];
+            r26_unpack(x, w[0], w[1], w[2], w[3]);
+#pragma unroll
+            for (int l = 0; l < 5; ++l)
+                V[5 * i + l] = x[l];
+        }
     }
 }

@@ -8,6 +8,10 @@
 //   S c z                  -> r26_from_signed limbs (z a signed 64-bit integer)
 //   P c a0..a4 b0..b4      -> r26_point limbs (a, b: signed 32-bit limbs; the result's top limb signed)
 //   B c stages x0..x4 y0..y4 w0..w4 (per stage: w limbs)  -> forward butterfly chain (signed limbs)
+//   T c a0..a4 W0..W4      -> r26_mont_tw limbs of a (signed limbs) and the expanded twiddle of W
+//   X c W0..W4             -> r26_expand_twiddle's 25 words
+//   BX / IX                -> B / I chains whose stages name their product: after each stage's
+//                             twiddle limbs (W, as stored) a kind, 0 = mont, 1 = mont_tw
 #include "../../hmpc_amd/csrc/r26.hpp"
 
 #include <cstdio>
@@ -58,6 +62,29 @@ int main()
             printf("%d %d %d %d %d\n", static_cast<int>(a[0]), static_cast<int>(a[1]), static_cast<int>(a[2]),
                    static_cast<int>(a[3]), static_cast<int>(a[4]));
         }
+        else if (cmd == "T" || cmd == "X")
+        {
+            u32 a[5] = {}, W[5], r[5], V[r26_tw_words];
+            if (cmd == "T")
+                for (auto& v : a)
+                {
+                    long long z;
+                    std::cin >> z;
+                    v = static_cast<u32>(z);
+                }
+            for (auto& v : W) std::cin >> v;
+            r26_expand_twiddle(V, W, P);
+            if (cmd == "X")
+            {
+                for (int i = 0; i < r26_tw_words; ++i)
+                    printf("%u%s", V[i], i + 1 < r26_tw_words ? " " : "\n");
+            }
+            else
+            {
+                r26_mont_tw(r, a, V, P);
+                printf("%u %u %u %u %d\n", r[0], r[1], r[2], r[3], static_cast<int>(r[4]));
+            }
+        }
         else if (cmd == "U")
         {
             u32 a[5], b[5], r[5];
@@ -100,7 +127,7 @@ int main()
             r26_canonical(w, x, P);
             printf("%u %u %u %u\n", w[0], w[1], w[2], w[3]);
         }
-        else if (cmd == "B" || cmd == "I")
+        else if (cmd == "B" || cmd == "I" || cmd == "BX" || cmd == "IX")
         {
             // chain of butterflies on one pair: forward (x, y) -> (x + t, x - t), t = mont(y, w) (signed digits),
             // or inverse (x + y, mont(x - y, w)) with x + y folded on request (signed digits). Per stage: the
@@ -121,11 +148,20 @@ int main()
             {
                 u32 w[5];
                 for (auto& v : w) std::cin >> v;
-                if (cmd == "B")
+                int kind = 0;
+                if (cmd.size() == 2)
+                    std::cin >> kind;
+                u32 V[r26_tw_words];
+                if (kind)
+                    r26_expand_twiddle(V, w, P);
+                if (cmd[0] == 'B')
                 {
                     int sel;
                     std::cin >> sel;
-                    r26_forward(x, y, w, P);
+                    if (kind)
+                        r26_forward_tw(x, y, V, P);
+                    else
+                        r26_forward(x, y, w, P);
                     for (int l = 0; l < 5; ++l)
                     {
                         if (sel == 1)
@@ -138,7 +174,10 @@ int main()
                 {
                     int fold, sel;
                     std::cin >> fold >> sel;
-                    r26_inverse(x, y, w, P, fold != 0);
+                    if (kind)
+                        r26_inverse_tw(x, y, V, P, fold != 0);
+                    else
+                        r26_inverse(x, y, w, P, fold != 0);
                     for (int l = 0; l < 5; ++l)
                     {
                         if (sel == 1)

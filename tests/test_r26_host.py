@@ -25,8 +25,11 @@ M26 = (1 << 26) - 1
 def tool():
     if not os.path.exists(BIN) or os.path.getmtime(BIN) < os.path.getmtime(SRC) or \
             os.path.getmtime(BIN) < os.path.getmtime(os.path.join(ROOT, "hmpc_amd", "csrc", "r26.hpp")):
-        subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", SRC, "-o", BIN],
+        # built under a name of this process and renamed, so that parallel workers do not race
+        tmp = f"{BIN}.{os.getpid()}"
+        subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", SRC, "-o", tmp],
                        check=True)
+        os.replace(tmp, BIN)
     p = subprocess.Popen([BIN], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
     yield p
     p.stdin.close()
@@ -275,3 +278,160 @@ def test_inverse_chain_bounds(tool, q, stages):
             if sel != 1:
                 assert -q < Y < 2 * q + (q >> 18) and all(0 <= v < (1 << 26) for v in st[5:9])
             px, py = st[:5], st[5:]
+
+
+# ---- products by a twiddle in expanded form (r26_mont_tw) and the streaming column schedule
+
+def _expanded(q, w):
+    """the expanded twiddle of w from its closed form: V_i = 2^(26 (i + 1)) w mod q, canonical limbs"""
+    return [limbs(w * (1 << (26 * (i + 1))) % q) for i in range(5)]
+
+
+@pytest.mark.parametrize("q", PRIMES)
+def test_expanded_twiddle_and_mont_tw(tool, q):
+    """r26_expand_twiddle from a stored entry W = w 2^130 mod q against the closed form
+    V_i = 2^(26 (i + 1)) w mod q, and r26_mont_tw against a w mod q for signed-digit a with limbs
+    up to |a_j| < 2^31 (extreme limbs included): digits 0..3 in [0, 2^26) and
+    |r_4| < 5 A / 2^6 + 2^20 + 2^8 (A = max |a_j|)."""
+    c = q - (1 << 124)
+    rng = random.Random(q & 0xFFFFF)
+    for it in range(800):
+        w = rng.randrange(q) if it % 8 else (0, 1, 2, q - 1)[(it // 8) % 4]
+        W = limbs(w * (1 << 130) % q)
+        V = run(tool, f"X {c} " + " ".join(map(str, W)))
+        assert [V[5 * i:5 * i + 5] for i in range(5)] == _expanded(q, w)
+        A = (1 << 26, 1 << 29, 1 << 30, (1 << 31) - 1)[it % 4]
+        a = [rng.choice((-A + 1, A - 1)) if rng.random() < 0.3 else rng.randrange(-A + 1, A) for _ in range(5)]
+        r = run(tool, f"T {c} " + " ".join(map(str, a + W)))
+        assert value(r) % q == value(a) * w % q
+        assert all(0 <= v < (1 << 26) for v in r[:4])
+        assert abs(r[4]) < 5 * max(abs(v) for v in a) / 64 + (1 << 20) + (1 << 8)
+
+
+@pytest.mark.parametrize("q", PRIMES)
+def test_fold_signed_whole_range(tool, q):
+    """r26_fold_signed over its whole input range, |x_j| < 2^31 - 2^26 for j < 4 and
+    |x_4| < 2^30 - 2^6 (values within 1024 q of zero), edges included: a congruent value in [0, 2q)
+    with limbs < 2^28."""
+    c = q - (1 << 124)
+    rng = random.Random(17)
+    lo_max, top_max = (1 << 31) - (1 << 26) - 1, (1 << 30) - (1 << 6) - 1
+    for _ in range(1500):
+        x = [rng.choice((-lo_max, lo_max)) if rng.random() < 0.3 else rng.randrange(-lo_max, lo_max + 1) for _ in range(4)]
+        x.append(rng.choice((-top_max, top_max)) if rng.random() < 0.3 else rng.randrange(-top_max, top_max + 1))
+        g = run(tool, f"G {c} " + " ".join(map(str, x)))
+        assert value(g) % q == value(x) % q and 0 <= value(g) < 2 * q and all(0 <= v < (1 << 28) for v in g)
+
+
+# per-operation limb bounds, as stated in r26.hpp: an element is (max |limb_j| for j < 4, |limb_4|)
+def _mont_tw_bound(x):
+    return (1 << 26, 5 * max(x) / 64 + (1 << 20) + (1 << 8))
+
+
+def _mont_bound(x):
+    return (1 << 26, (x[1] + 64) / 64 + (1 << 20) + 4)
+
+
+def _stream_kinds(rows):
+    """the product of each stage in the streaming column passes' schedule of a transform of
+    N = 2^(7 + rows): forward columns (6 rounds' stages on mont_tw, stage 6 on mont), row stages
+    (mont); inverse rows (mont), columns' stage 0 (mont), stages 1-5 (mont_tw); the scaled last
+    stage (mont) is applied separately"""
+    return ["tw"] * 6 + ["mont"] * (1 + rows), ["mont"] * (rows + 1) + ["tw"] * 5
+
+
+@pytest.mark.parametrize("rows", [7, 8, 9, 10])
+def test_stream_schedule_worst_case(rows):
+    """Worst-case limb propagation over the streaming schedule with the bounds above, from
+    canonical inputs: every product input keeps |limbs| < 2^30.2 (mont and mont_tw need < 2^31),
+    the point-wise product's operands meet its bounds, and every folded sum (stages 1, 4, 7, ...)
+    has |x_4| < 2^29.6 and |x_j| < 2^31 - 2^26 (the signed fold's range)."""
+    fwd, inv = _stream_kinds(rows)
+    x = (1 << 26, 1 << 21)
+    for k in fwd:
+        assert max(x) < 2 ** 30.2
+        t = (_mont_tw_bound if k == "tw" else _mont_bound)(x)
+        x = (x[0] + t[0], x[1] + t[1])
+    assert max(x) < 2 ** 30.2
+    # r26_point: 4 mont(a, folded b): digits < 2^28, |top| < 4 ((|a_4| + 64) / 16 + 2^20 + 2)
+    y = (1 << 28, (x[1] + 64) / 4 + (1 << 22) + 8)
+    for i, k in enumerate(inv):
+        s = (2 * y[0], 2 * y[1])  # x + y and x - y of two elements within y's bounds
+        assert max(s) < 2 ** 30.2
+        if i % 3 == 1:
+            assert s[1] < 2 ** 29.6 and s[0] < (1 << 31) - (1 << 26)
+            s = ((1 << 27) + (1 << 24), 1 << 21)  # folded: limb 0 < 2^26 + qb_0 < 2^27 + c, others < 2^27
+        t = (_mont_tw_bound if k == "tw" else _mont_bound)(s)
+        y = (max(s[0], t[0]), max(s[1], t[1]))
+    assert max(2 * y[0], 2 * y[1]) < 2 ** 30.2  # the scaled last stage's mont inputs
+
+
+def _chain(tool, q, cmd, x, y, stages, rng, rows):
+    """run a BX / IX chain on one pair; stages: list of (kind, fold); returns the per-stage outputs
+    and checks each against exact arithmetic mod q"""
+    c = q - (1 << 124)
+    R = 1 << 130
+    ws = [rng.randrange(q) for _ in stages]
+    sels = [rng.randrange(3) for _ in stages]
+    parts = []
+    for (kind, fold), w, sel in zip(stages, ws, sels):
+        extra = f" {fold}" if cmd == "IX" else ""
+        parts.append(" ".join(map(str, limbs(w))) + f" {int(kind == 'tw')}{extra} {sel}")
+    out = run(tool, f"{cmd} {c} {len(stages)} " + " ".join(map(str, x + y)) + " " + " ".join(parts))
+    assert len(out) == 10 * len(stages)
+    ex, ey = value(x) % q, value(y) % q
+    px, py = x, y
+    for s, ((kind, fold), w, sel) in enumerate(zip(stages, ws, sels)):
+        if cmd == "BX":
+            assert max(abs(v) for v in py) < 2 ** 30.2, f"product input at stage {s}"
+            t = ey * w * pow(R, -1, q) % q
+            ex, ey = (ex + t) % q, (ex - t) % q
+            if sel == 1:
+                ex = ey
+            elif sel == 2:
+                ey = ex
+        else:
+            d = [a - b for a, b in zip(px, py)]
+            assert max(abs(v) for v in d) < 2 ** 30.2, f"product input at stage {s}"
+            if fold:
+                top = abs(px[4] + py[4])
+                assert top < 2 ** 29.6 and max(abs(a + b) for a, b in zip(px[:4], py[:4])) < (1 << 31) - (1 << 26)
+            ex, ey = (ex + ey) % q, (ex - ey) * w * pow(R, -1, q) % q
+            if sel == 1:
+                ey = ex
+            elif sel == 2:
+                ex = ey
+        st = out[10 * s:10 * s + 10]
+        assert value(st[:5]) % q == ex and value(st[5:]) % q == ey, f"{cmd} stage {s}"
+        px, py = st[:5], st[5:]
+    return px, py
+
+
+@pytest.mark.parametrize("q", PRIMES)
+@pytest.mark.parametrize("rows", [7, 10])
+def test_stream_schedule_chains(tool, q, rows):
+    """The streaming schedule on pairs through the host build: forward chains (mont_tw and mont
+    stages as in the columns and rows), a point-wise product of two forward outputs, then inverse
+    chains (x + y folded at stages 1, 4, 7, ...), each stage keeping either output pair or a
+    doubled one: exact mod q at every stage, every product input below 2^30.2 per limb, every
+    folded sum within the signed fold's range, and the results fold (signed) to [0, 2q)."""
+    c = q - (1 << 124)
+    rng = random.Random(rows * 31 + (q & 0xFF))
+    fwd, inv = _stream_kinds(rows)
+    for _ in range(40):
+        outs = []
+        for _ in range(2):
+            x, y = _forward_input(q, rng), _forward_input(q, rng)
+            outs.append(_chain(tool, q, "BX", x, y, [(k, 0) for k in fwd], rng, rows))
+        # forward outputs fold (stores) and multiply point-wise
+        for half in outs[0]:
+            f = run(tool, f"G {c} " + " ".join(map(str, half)))
+            assert value(f) % q == value(half) % q and 0 <= value(f) < 2 * q
+        a, b = outs[0][0], outs[1][1]
+        pa = run(tool, f"P {c} " + " ".join(map(str, a + b)))
+        pb = run(tool, f"P {c} " + " ".join(map(str, outs[0][1] + outs[1][0])))
+        assert value(pa) % q == value(a) * value(b) * pow(1 << 128, -1, q) % q
+        x, y = _chain(tool, q, "IX", pa, pb, [(k, int(i % 3 == 1)) for i, k in enumerate(inv)], rng, rows)
+        for half in (x, y):
+            f = run(tool, f"G {c} " + " ".join(map(str, half)))
+            assert value(f) % q == value(half) % q and 0 <= value(f) < 2 * q
