@@ -240,12 +240,16 @@ namespace hmpc_amd
             static constexpr bool stage_prefetch = false;
         };
 
-        // the streaming column passes' 3-stage rounds: wave-uniform twiddles in the expanded form
-        // of r26_mont_tw (25 words per entry of the expanded tables, through scalar loads, the mads
-        // taking them from SGPRs): 38 instead of 54 instructions per product. Elements, LDS and
-        // global accesses as arith_r26.
-        struct arith_r26_tw : arith_r26
+        // Rounds whose twiddles are wave-uniform (the streaming column passes' 3-stage rounds, the
+        // one-wave row kernel's forward first and inverse last rounds): twiddles in the expanded
+        // form of r26_mont_tw (25 words per entry of the expanded tables, through scalar loads, the
+        // mads taking them from SGPRs), 38 instead of 54 instructions per product. Elements, LDS
+        // and global accesses as the base policy.
+        template<typename Base>
+        struct arith_tw : Base
         {
+            using typename Base::params;
+            static constexpr int rl = Base::rl;
             static constexpr int twl = r26_tw_words;
 
             __device__ __forceinline__ static void tw_get(u32 (&w)[twl], u32 const* __restrict__ tw, u32 idx)
@@ -269,8 +273,8 @@ namespace hmpc_amd
 
             // the scaled last stage keeps mont with the stored twiddles (V_4 = w 2^130 of entries 0
             // and 1; the loads of V_0..V_3 fall away): its two twiddles are the same for every tile,
-            // and held across the tile loop they take 10 SGPRs instead of 50, which the other
-            // stages' twiddles need (with 50 the inverse pass spilled SGPRs to VGPR lanes)
+            // and held across a tile loop they take 10 SGPRs instead of 50, which the other
+            // stages' twiddles need (with 50 the inverse column pass spilled SGPRs to VGPR lanes)
             __device__ __forceinline__ static void inv_last(u32 (&x)[rl], u32 (&y)[rl], u32 const (&w0)[twl],
                                                             u32 const (&w1)[twl], params const& P)
             {
@@ -279,6 +283,8 @@ namespace hmpc_amd
                 r26_inverse_last(x, y, s0, s1, P);
             }
         };
+
+        using arith_r26_tw = arith_tw<arith_r26>;
 
         // The LDS slot order of the row kernels (rows of 2^7 .. 2^10 elements). With 64 banks of 4
         // bytes, element slots of 5 words and E = 8 every gather and scatter of every round of these
@@ -315,6 +321,8 @@ namespace hmpc_amd
 
             __device__ __forceinline__ static void lds_put(u32* lds, int e, u32 const (&x)[rl]) { lds_put_at(lds, phys(e), x); }
         };
+
+        using arith_r26_rows1_tw = arith_tw<arith_r26_rows1>;
 
         // the BGV row kernels: twiddles from 32-bit offsets as in the product's row kernel
         struct arith_r26_bgv_rows : arith_r26
@@ -952,10 +960,11 @@ namespace hmpc_amd
         template<typename A, int E, bool Inv, bool WL = false>
         __device__ __forceinline__ void tile_rounds(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
                                                     u32 base_step, u32 const* __restrict__ tw, typename A::params const& P,
-                                                    bool final_scale, int first_round, bool keep_last, int off = 0)
+                                                    bool final_scale, int first_round, bool keep_last, int off = 0,
+                                                    int end_round = -1)
         {
             constexpr int WM = shape<E>::wmax;
-            int n = round_count<WM>(td.s);
+            int n = end_round < 0 ? round_count<WM>(td.s) : end_round;
             for (int r = first_round; r < n; ++r)
             {
                 int a = round_start<WM>(td.s, r, Inv);
@@ -1260,165 +1269,9 @@ namespace hmpc_amd
             tile_store<A, T>(lds, out, io, canonical, P);
         }
 
-        // polynomial product, whole polynomials per tile (N <= 4096) or the row pass of the
-        // two-pass product: forward(A), forward(B), A o B, inverse. The row pass works in place
-        // (a == out) and asks for 4 waves per SIMD (2 workgroups per CU with the 80 KiB tile).
-        // S > 0: the row length log2 (G.log_n2) as a compile-time constant, so that the round
-        // structure and every LDS address are resolved at compile time (and only the round widths
-        // this length uses are emitted)
-        template<typename A, int E, bool Single, int S = 0>
-        __global__ void __launch_bounds__(A::wave_blocks ? 64 : tile / E, A::wave_blocks ? 3 : ((E == 8 && !Single) ? 4 : 1)) k_polymul(u32 const* a, u32 const* __restrict__ b,
-                                                              u32* out, u32 const* __restrict__ twf,
-                                                              u32 const* __restrict__ twi, pass_geom G, u64 n,
-                                                              typename A::params P)
-        {
-            extern __shared__ __align__(16) u32 lds[];
-            constexpr int T = tile / E;
-            constexpr int WM = shape<E>::wmax;
-            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
-            tile_desc td;
-            io_desc io;
-            u32 base0 = 1, base_step = 0;
-            if constexpr (Single)
-            {
-                single_setup(G, blockIdx.x, n, td, io);
-            }
-            else
-            {
-                // one-wave workgroups: block b serves wave b % (tile / 64 E) of tile b / (tile / 64 E)
-                static_assert(!A::wave_blocks || (E == 8 && S > 0 && (1 << S) <= 64 * E),
-                              "one-wave workgroups need wave-local rows at 8 elements per thread");
-                u32 const tb = A::wave_blocks ? blockIdx.x / static_cast<u32>(tile / E / 64) : blockIdx.x;
-                u32 hi0 = rows_setup(G, tb, count, td, io);
-                if constexpr (S > 0)
-                {
-                    td.s = S;
-                    td.log_seqs = tile_log - S;
-                }
-                base0 = (1u << G.log_n1) + hi0;
-                base_step = 1;
-            }
-            int w0 = round_width<WM>(td.s, 0, true);
-            if constexpr (Single)
-            {
-                u32 xa[E][A::rl];
-                u32 xb[E][A::rl];
-                tile_load<A, T>(lds, a, io);
-                __syncthreads();
-                tile_rounds<A, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
-                __syncthreads();
-                tile_load<A, T>(lds, b, io);
-                __syncthreads();
-                tile_rounds<A, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
-#pragma unroll
-                for (int k = 0; k < E; ++k)
-                    A::point(xa[k], xb[k], P);
-                // inverse round 0 groups the same elements as the forward's last round
-                dispatch_round_regs<A, E>(w0, xa, lds, td, base0, base_step, twi, P, true);
-                __syncthreads();
-                tile_rounds<A, E, true>(xa, lds, td, base0, base_step, twi, P, true, 1, false);
-                tile_store<A, T>(lds, out, io, true, P);
-            }
-            else
-            {
-                // two-pass row tile: NTT(a) rows go back to the workspace (a == out, in place) and
-                // are read per element next to NTT(b) in registers, so one register tile is live.
-                // With the wave-major unit order, rows of at most 64 E elements (S <= 9 at E = 8) are
-                // wave-local: each wave loads, transforms, multiplies and stores whole rows of its
-                // own, with no workgroup barrier.
-                constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
-                u32 x[E][A::rl];
-                if constexpr (A::wave_blocks)
-                {
-                    // one-wave rows: NTT(a) stays in registers next to NTT(b)
-                    u32 xa[E][A::rl];
-                    tile_load<A, T, true, WL>(lds, a, io);
-                    tile_sync<WL>();
-                    tile_rounds<A, E, false, WL>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
-                    tile_sync<WL>();
-                    tile_load<A, T, true, WL>(lds, b, io);
-                    tile_sync<WL>();
-                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
-#pragma unroll
-                    for (int k = 0; k < E; ++k)
-                    {
-                        A::point(xa[k], x[k], P);
-#pragma unroll
-                        for (int l = 0; l < A::rl; ++l)
-                            x[k][l] = xa[k][l];
-                    }
-                }
-                else
-                {
-                    tile_load<A, T, true, WL>(lds, a, io);
-                    tile_sync<WL>();
-                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, false);
-                    tile_store<A, T, true, WL>(lds, out, io, false, P);
-                    tile_sync<WL>();
-                    tile_load<A, T, true, WL>(lds, b, io);
-                    tile_sync<WL>();
-                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
-                    with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
-                }
-                dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
-                tile_sync<WL>();
-                tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false);
-                tile_store<A, T, true, WL>(lds, out, io, false, P);
-            }
-        }
-
-        // forward column pass of both inputs of the two-pass product (blockIdx.x = 2 * tile + which)
-        template<typename A, int E, int S = 0>
-        __global__ void __launch_bounds__(tile / E) k_polymul_cols(u32 const* __restrict__ a, u32 const* __restrict__ b,
-                                                                   u32* __restrict__ wa, u32* __restrict__ wb,
-                                                                   u32 const* __restrict__ tw, pass_geom G, u64 n,
-                                                                   typename A::params P)
-        {
-            extern __shared__ __align__(16) u32 lds[];
-            constexpr int T = tile / E;
-            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
-            u32 which = blockIdx.x & 1u;
-            tile_desc td;
-            io_desc io;
-            cols_setup(G, blockIdx.x >> 1, count, td, io);
-            if constexpr (S > 0)
-            {
-                td.s = S;
-                td.log_seqs = tile_log - S;
-                io.log_seqs = tile_log - S;
-            }
-            tile_load<A, T>(lds, which ? b : a, io);
-            __syncthreads();
-            u32 x[E][A::rl];
-            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, tw, P, false, 0, false);
-            tile_store<A, T, true>(lds, which ? wb : wa, io, ws_fold_columns(G.log_n), P);
-        }
-
-        // ------------------------------------------------------------------ streaming column passes
-        // The column passes of the radix-2^26 product (columns of 2^7, 32 columns per tile, 8
-        // elements per thread) as persistent workgroups that walk the tiles t = blockIdx.x,
-        // blockIdx.x + gridDim.x, ... and load tile t + gridDim.x into registers while tile t is
-        // transformed. With one tile per workgroup the two workgroups of a CU load, transform and
-        // store in step, so memory and VALU time add up (0.52 + 0.53 ms for the forward pass of 512
-        // pairs at N = 2^16, 0.85-0.89 ms together); streamed, they overlap (0.67 ms, same results).
-        //
-        // The prefetch only overlaps if no later vector load waits behind it (vector loads complete
-        // in issue order), so these passes take no twiddle from a vector load inside the loop:
-        //   - the 3-stage rounds have one twiddle per stage and wave in this geometry (forward
-        //     stages 0-5, inverse stages 1-6: the unit's top bits F are 0 or the wave index), loaded
-        //     with scalar loads from lane 0's index;
-        //   - forward stage 6 and inverse stage 0 run in registers in the pair layout
-        //     (cols_pair_row: one v_permlane32_swap per word and pair from or to the 3-stage
-        //     layout instead of an LDS exchange and a barrier); their twiddles (two per wave and
-        //     pair, not depending on the tile) are loaded once per workgroup and kept in VGPRs
-        //     (forward), or taken with two scalar loads and a per-lane select (inverse, whose 5-word
-        //     prefetch leaves no room).
-        // tests/cpp/jobs_index_check.cpp checks the per-wave uniformity, the pair layout and that each
-        // pass moves every tile element once, inside its polynomial. Round-0 (forward) and pair-layout
-        // (inverse) elements are loaded straight into registers and the last round's stored straight
-        // from them (each wave access is two 128-byte column segments per limb plane, as in
-        // tile_load), so LDS carries a single exchange per tile.
-
+        // ------------------------------------------------------------------ rounds with wave-uniform twiddles
+        // (the streaming column passes and the one-wave row kernel's first forward and last
+        // inverse rounds)
         // a twiddle whose index is the same on every lane of the wave
         template<typename A>
         __device__ __forceinline__ void tw_get_wave(u32 (&w)[A::twl], u32 const* __restrict__ tw, u32 idx)
@@ -1529,14 +1382,209 @@ namespace hmpc_amd
 
         template<typename A, int E, bool Inv, int W>
         __device__ __forceinline__ void round_wave(u32 (&x)[E][A::rl], tile_desc const& td, int a, u32 const* __restrict__ tw,
-                                                   typename A::params const& P, int g0, bool scaled)
+                                                   typename A::params const& P, int g0, bool scaled, u32 base0 = 1u,
+                                                   u32 base_step = 0u)
         {
             static_assert((1 << W) == E, "full-width round");
             u32 twr[1][(1 << W) - 1][A::twl];
             u32 xr[1];
-            round_bases<A, E, Inv, W>(xr, td, a, 1u, 0u);
+            round_bases<A, E, Inv, W>(xr, td, a, base0, base_step);
             stages_wave<A, E, Inv, W, 0>(x, twr, xr[0], tw, P, g0, scaled);
         }
+
+        // polynomial product, whole polynomials per tile (N <= 4096) or the row pass of the
+        // two-pass product: forward(A), forward(B), A o B, inverse. The row pass works in place
+        // (a == out) and asks for 4 waves per SIMD (2 workgroups per CU with the 80 KiB tile).
+        // S > 0: the row length log2 (G.log_n2) as a compile-time constant, so that the round
+        // structure and every LDS address are resolved at compile time (and only the round widths
+        // this length uses are emitted)
+        // One-wave rows of 2^9 (A::wave_blocks, S = 9) hold one row per wave, so the twiddles of the
+        // forward transforms' first round and of the inverse's last round (units of stride 2^6,
+        // whose index depends on the row only) are wave-uniform: those rounds take the expanded
+        // tables (twfx, twix) through round_wave.
+        template<typename A, int E, bool Inv>
+        __device__ __forceinline__ void round_expanded(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a,
+                                                       u32 base0, u32 base_step, u32 const* __restrict__ tw,
+                                                       typename A::params const& P)
+        {
+            gather<A, E, 3>(x, lds, td, a, Inv);
+            round_wave<arith_tw<A>, E, Inv, 3>(x, td, a, opaque_table(tw), P, a, false, base0, base_step);
+            scatter<A, E, 3>(x, lds, td, a, Inv);
+        }
+
+        template<typename A, int E, bool Single, int S = 0>
+        __global__ void __launch_bounds__(A::wave_blocks ? 64 : tile / E, A::wave_blocks ? 3 : ((E == 8 && !Single) ? 4 : 1)) k_polymul(u32 const* a, u32 const* __restrict__ b,
+                                                              u32* out, u32 const* __restrict__ twf,
+                                                              u32 const* __restrict__ twi, u32 const* __restrict__ twfx,
+                                                              u32 const* __restrict__ twix, pass_geom G, u64 n,
+                                                              typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            constexpr int WM = shape<E>::wmax;
+            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
+            tile_desc td;
+            io_desc io;
+            u32 base0 = 1, base_step = 0;
+            if constexpr (Single)
+            {
+                single_setup(G, blockIdx.x, n, td, io);
+            }
+            else
+            {
+                // one-wave workgroups: block b serves wave b % (tile / 64 E) of tile b / (tile / 64 E)
+                static_assert(!A::wave_blocks || (E == 8 && S > 0 && (1 << S) <= 64 * E),
+                              "one-wave workgroups need wave-local rows at 8 elements per thread");
+                u32 const tb = A::wave_blocks ? blockIdx.x / static_cast<u32>(tile / E / 64) : blockIdx.x;
+                u32 hi0 = rows_setup(G, tb, count, td, io);
+                if constexpr (S > 0)
+                {
+                    td.s = S;
+                    td.log_seqs = tile_log - S;
+                }
+                base0 = (1u << G.log_n1) + hi0;
+                base_step = 1;
+            }
+            int w0 = round_width<WM>(td.s, 0, true);
+            if constexpr (Single)
+            {
+                u32 xa[E][A::rl];
+                u32 xb[E][A::rl];
+                tile_load<A, T>(lds, a, io);
+                __syncthreads();
+                tile_rounds<A, E, false>(xa, lds, td, base0, base_step, twf, P, false, 0, true);
+                __syncthreads();
+                tile_load<A, T>(lds, b, io);
+                __syncthreads();
+                tile_rounds<A, E, false>(xb, lds, td, base0, base_step, twf, P, false, 0, true);
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                    A::point(xa[k], xb[k], P);
+                // inverse round 0 groups the same elements as the forward's last round
+                dispatch_round_regs<A, E>(w0, xa, lds, td, base0, base_step, twi, P, true);
+                __syncthreads();
+                tile_rounds<A, E, true>(xa, lds, td, base0, base_step, twi, P, true, 1, false);
+                tile_store<A, T>(lds, out, io, true, P);
+            }
+            else
+            {
+                // two-pass row tile: NTT(a) rows go back to the workspace (a == out, in place) and
+                // are read per element next to NTT(b) in registers, so one register tile is live.
+                // With the wave-major unit order, rows of at most 64 E elements (S <= 9 at E = 8) are
+                // wave-local: each wave loads, transforms, multiplies and stores whole rows of its
+                // own, with no workgroup barrier.
+                constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
+                u32 x[E][A::rl];
+                // rows of 2^9, one per wave: the first forward and last inverse rounds on expanded twiddles
+                constexpr bool XW = A::wave_blocks && S == 9 && E == 8;
+                if constexpr (A::wave_blocks)
+                {
+                    // one-wave rows: NTT(a) stays in registers next to NTT(b)
+                    u32 xa[E][A::rl];
+                    tile_load<A, T, true, WL>(lds, a, io);
+                    tile_sync<WL>();
+                    if constexpr (XW)
+                    {
+                        round_expanded<A, E, false>(xa, lds, td, 0, base0, base_step, twfx, P);
+                        tile_sync<WL>();
+                    }
+                    tile_rounds<A, E, false, WL>(xa, lds, td, base0, base_step, twf, P, false, XW ? 1 : 0, true);
+                    tile_sync<WL>();
+                    tile_load<A, T, true, WL>(lds, b, io);
+                    tile_sync<WL>();
+                    if constexpr (XW)
+                    {
+                        round_expanded<A, E, false>(x, lds, td, 0, base0, base_step, twfx, P);
+                        tile_sync<WL>();
+                    }
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, XW ? 1 : 0, true);
+#pragma unroll
+                    for (int k = 0; k < E; ++k)
+                    {
+                        A::point(xa[k], x[k], P);
+#pragma unroll
+                        for (int l = 0; l < A::rl; ++l)
+                            x[k][l] = xa[k][l];
+                    }
+                }
+                else
+                {
+                    tile_load<A, T, true, WL>(lds, a, io);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, false);
+                    tile_store<A, T, true, WL>(lds, out, io, false, P);
+                    tile_sync<WL>();
+                    tile_load<A, T, true, WL>(lds, b, io);
+                    tile_sync<WL>();
+                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, 0, true);
+                    with_operands_last<A, E, 0>(x, td, io.valid, out, io.count, io.base, nullptr, 0, 0, P);
+                }
+                dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
+                tile_sync<WL>();
+                if constexpr (XW)
+                {
+                    // rounds (0-2) done above, (3-5) here, (6-8) on expanded twiddles
+                    tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false, 0, 2);
+                    round_expanded<A, E, true>(x, lds, td, 6, base0, base_step, twix, P);
+                    tile_sync<WL>();
+                }
+                else
+                    tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false);
+                tile_store<A, T, true, WL>(lds, out, io, false, P);
+            }
+        }
+
+        // forward column pass of both inputs of the two-pass product (blockIdx.x = 2 * tile + which)
+        template<typename A, int E, int S = 0>
+        __global__ void __launch_bounds__(tile / E) k_polymul_cols(u32 const* __restrict__ a, u32 const* __restrict__ b,
+                                                                   u32* __restrict__ wa, u32* __restrict__ wb,
+                                                                   u32 const* __restrict__ tw, pass_geom G, u64 n,
+                                                                   typename A::params P)
+        {
+            extern __shared__ __align__(16) u32 lds[];
+            constexpr int T = tile / E;
+            std::size_t count = static_cast<std::size_t>(n) << G.log_n;
+            u32 which = blockIdx.x & 1u;
+            tile_desc td;
+            io_desc io;
+            cols_setup(G, blockIdx.x >> 1, count, td, io);
+            if constexpr (S > 0)
+            {
+                td.s = S;
+                td.log_seqs = tile_log - S;
+                io.log_seqs = tile_log - S;
+            }
+            tile_load<A, T>(lds, which ? b : a, io);
+            __syncthreads();
+            u32 x[E][A::rl];
+            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, tw, P, false, 0, false);
+            tile_store<A, T, true>(lds, which ? wb : wa, io, ws_fold_columns(G.log_n), P);
+        }
+
+        // ------------------------------------------------------------------ streaming column passes
+        // The column passes of the radix-2^26 product (columns of 2^7, 32 columns per tile, 8
+        // elements per thread) as persistent workgroups that walk the tiles t = blockIdx.x,
+        // blockIdx.x + gridDim.x, ... and load tile t + gridDim.x into registers while tile t is
+        // transformed. With one tile per workgroup the two workgroups of a CU load, transform and
+        // store in step, so memory and VALU time add up (0.52 + 0.53 ms for the forward pass of 512
+        // pairs at N = 2^16, 0.85-0.89 ms together); streamed, they overlap (0.67 ms, same results).
+        //
+        // The prefetch only overlaps if no later vector load waits behind it (vector loads complete
+        // in issue order), so these passes take no twiddle from a vector load inside the loop:
+        //   - the 3-stage rounds have one twiddle per stage and wave in this geometry (forward
+        //     stages 0-5, inverse stages 1-6: the unit's top bits F are 0 or the wave index), loaded
+        //     with scalar loads from lane 0's index;
+        //   - forward stage 6 and inverse stage 0 run in registers in the pair layout
+        //     (cols_pair_row: one v_permlane32_swap per word and pair from or to the 3-stage
+        //     layout instead of an LDS exchange and a barrier); their twiddles (two per wave and
+        //     pair, not depending on the tile) are loaded once per workgroup and kept in VGPRs
+        //     (forward), or taken with two scalar loads and a per-lane select (inverse, whose 5-word
+        //     prefetch leaves no room).
+        // tests/cpp/jobs_index_check.cpp checks the per-wave uniformity, the pair layout and that each
+        // pass moves every tile element once, inside its polynomial. Round-0 (forward) and pair-layout
+        // (inverse) elements are loaded straight into registers and the last round's stored straight
+        // from them (each wave access is two 128-byte column segments per limb plane, as in
+        // tile_load), so LDS carries a single exchange per tile.
 
         // tile element of element k of this thread's unit g in round (a, W)
         template<typename A, int E, int W>
@@ -1905,7 +1953,8 @@ namespace hmpc_amd
                         return e;
                     {
                         probe_scope ps(s, "fast polymul (whole tile)");
-                        k_polymul<A, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_fast, T.inv_fast, G, n, P);
+                        k_polymul<A, E, true><<<grid, TH, bytes, s>>>(a, b, out, T.fwd_fast, T.inv_fast, T.fwd_tw, T.inv_tw,
+                                                                      G, n, P);
                     }
                     return hipGetLastError();
                 }
@@ -1936,7 +1985,10 @@ namespace hmpc_amd
                 std::size_t rows_bytes = bytes;
                 if constexpr (std::is_same<A, arith_r26>::value && E == 8)
                 {
-                    if (gr.log_n2 >= 7 && gr.log_n2 <= 9)
+                    // rows of 2^9 read the expanded tables up to entry 8 * 2^log_n1 (their first and
+                    // last rounds' indices)
+                    bool const expanded = T.fwd_tw && T.inv_tw && (std::size_t{8} << gr.log_n1) <= r26_tw_entries;
+                    if (gr.log_n2 >= 7 && gr.log_n2 <= 9 && (gr.log_n2 < 9 || expanded))
                     {
                         rows_kernel = gr.log_n2 == 7   ? k_polymul<arith_r26_rows1, E, false, 7>
                                       : gr.log_n2 == 8 ? k_polymul<arith_r26_rows1, E, false, 8>
@@ -1979,7 +2031,8 @@ namespace hmpc_amd
                 }
                 {
                     probe_scope ps(s, "fast polymul rows");
-                    rows_kernel<<<rows_grid, rows_threads, rows_bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, gr, n, P);
+                    rows_kernel<<<rows_grid, rows_threads, rows_bytes, s>>>(wa, wb, wa, T.fwd_fast, T.inv_fast, T.fwd_tw, T.inv_tw,
+                                                                            gr, n, P);
                 }
                 {
                     probe_scope ps(s, "fast polymul inverse columns");

@@ -73,9 +73,10 @@ namespace hmpc_amd
     hipError_t launch_to_element_major(int L, u32 const* planar, u32* em, std::size_t count, hipStream_t s);
     // [4][N] canonical table of w R -> [N][8] radix-2^26 limbs of w 2^130 mod q
     hipError_t launch_r26_table(u32 const* planar, u32* out, std::size_t count, p26 const& P, hipStream_t s);
-    // entries of the expanded twiddle tables (r26_mont_tw): the twiddles of the first 7 forward
-    // stages and of the inverse's last 7, which the streaming column passes take per wave
-    constexpr std::size_t r26_tw_entries = 128;
+    // entries of the expanded twiddle tables (r26_mont_tw): the twiddles of the first 10 forward
+    // stages and of the inverse's last 10, which the streaming column passes (stages 0-6) and the
+    // one-wave row kernel's wave-uniform rounds (stages 7-9 of N = 2^16) take per wave
+    constexpr std::size_t r26_tw_entries = 1024;
     hipError_t launch_r26_tw_table(u32 const* fast, u32* out, std::size_t count, p26 const& P, hipStream_t s);
 
     // fast path (L = 2, 4): register-resident radix-16 rounds, see kernels_fast.hip

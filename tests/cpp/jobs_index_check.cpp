@@ -12,6 +12,7 @@
 // whole-tile jobs of degree N < 4096: without that guard those reads would fall outside the N-element
 // key and scratch planes. Prints "ok" and the counts, or the first failure.
 #include "../../hmpc_amd/csrc/jobs.hpp"
+#include "../../hmpc_amd/csrc/kernels.hpp"
 #include "../../hmpc_amd/csrc/tile_geometry.hpp"
 
 #include <cstdio>
@@ -625,6 +626,7 @@ void streaming_columns()
                 }
                 require(tw_index_round<false, 3, 2>(round_x<3>(td, 3, false, t, 0), 3) < 128u, "forward twiddle index");
                 require(tw_index_round<true, 3, 0>(round_x<3>(td, 1, true, t, 0), 3) < 128u, "inverse twiddle index");
+                static_assert(r26_tw_entries >= 128, "the streaming columns' twiddles are expanded");
             }
     }
 }
@@ -684,11 +686,49 @@ void rows1_blocks()
     }
 }
 
+// The one-wave row kernel at rows of 2^9 takes the twiddles of the forward transforms' first
+// round (stages 0-2) and of the inverse's last round (stages 6-8) per wave, from the expanded
+// tables: every lane of a wave must have lane 0's row c and top bits F in those rounds (so the
+// round's twiddle bases agree), and for N = 2^16 (columns of 2^7: base = 2^7 + row, rows 0..127)
+// every index must lie inside the expanded tables (r26_tw_entries).
+void rows1_uniform()
+{
+    tile_desc const td{9, tile_log - 9, true};
+    for (int inv = 0; inv < 2; ++inv)
+    {
+        int const a = inv ? 6 : 0;
+        for (int t = 0; t < 512; ++t)
+        {
+            unit_t const un = unit_of(td, a, 3, inv != 0, unit_index_at<8, 3, true>(t, 0));
+            unit_t const u0 = unit_of(td, a, 3, inv != 0, unit_index_at<8, 3, true>(t & ~63, 0));
+            require(un.c == u0.c && un.F == u0.F, "one-wave rows of 2^9: a round on expanded twiddles is not wave-uniform");
+            for (u32 row = 0; row < 128; ++row)
+            {
+                u32 const base = 128u + row;
+                for (int ii = 0; ii < 3; ++ii)
+                {
+                    u32 const X = inv ? tw_round_base<true, 3>(base, 9, a, un.F) : tw_round_base<false, 3>(base, 9, a, un.F);
+                    int const count = inv ? (1 << (3 - ii - 1)) : (1 << ii);
+                    for (int m = 0; m < count; ++m)
+                    {
+                        u32 const idx = inv ? (ii == 0 ? tw_index_round<true, 3, 0>(X, m)
+                                                       : ii == 1 ? tw_index_round<true, 3, 1>(X, m) : tw_index_round<true, 3, 2>(X, m))
+                                            : (ii == 0 ? tw_index_round<false, 3, 0>(X, m)
+                                                       : ii == 1 ? tw_index_round<false, 3, 1>(X, m) : tw_index_round<false, 3, 2>(X, m));
+                        require(idx < r26_tw_entries, "one-wave rows of 2^9: twiddle index outside the expanded tables");
+                    }
+                }
+            }
+        }
+    }
+}
+
 int main()
 {
     swizzles();
     streaming_columns();
     rows1_blocks();
+    rows1_uniform();
     unit_offsets_all<4, 1>();
     unit_offsets_all<4, 2>();
     unit_offsets_all<8, 1>();
