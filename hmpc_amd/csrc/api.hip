@@ -1429,6 +1429,7 @@ namespace
         J.log_n = static_cast<u32>(log_n);
         J.twf = T.fwd_fast;
         J.twi = T.inv_fast;
+        J.twfx = T.fwd_tw;
         J.in0 = a;
         J.in1 = b;
         J.in_count = count;
@@ -1456,6 +1457,7 @@ namespace
         J.p_int = p_int;
         J.twf = T.fwd_fast;
         J.twi = T.inv_fast;
+        J.twfx = T.fwd_tw;
         J.in0 = pk_a;
         J.in1 = pk_b;
         J.in_count = u64{1} << log_n;

@@ -385,9 +385,10 @@ namespace hmpc_amd
         // small batches). Workspace: the three column-pass results [3][wsl][n N].
         //   column pass: blockIdx.x = (p * 3 + which) * tiles + t  (sample stream `which` of ciphertext p)
         //   row pass:    blockIdx.x = p * tiles + t
-        template<typename A, int E, int Which, int S = 0>
+        template<typename A, int E, int Which, int S = 0, bool X = false>
         __device__ __forceinline__ void bgv_column_tile(job_desc const& J, u32* lds, u32* dst, pass_geom const& gc, u32 ct,
-                                                        std::size_t count, u32 p, typename A::params const& P)
+                                                        std::size_t count, u32 p, typename A::params const& P,
+                                                        u32 const* __restrict__ twx = nullptr)
         {
             constexpr int T = tile / E;
             u32 x[E][A::rl];
@@ -406,13 +407,27 @@ namespace hmpc_amd
             }
             sample_tile<A, T, Which>(lds, J, true, log_n1, log_seqs, ct << log_seqs, P);
             __syncthreads();
-            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+            if constexpr (X)
+            {
+                // radix-2^26 columns of 2^7, 32 per tile: stages 0-2 and 3-5 have wave-uniform
+                // twiddles (as in the streaming column passes) and take them in expanded form from
+                // twx (a restrict kernel argument, so that the loads are scalar); stage 6 as before
+                static_assert(std::is_same<A, arith_r26>::value && E == 8 && S == 7, "expanded column rounds");
+                round_expanded<A, E, false>(x, lds, td, 0, 1u, 0u, twx, P);
+                __syncthreads();
+                round_expanded<A, E, false>(x, lds, td, 3, 1u, 0u, twx, P);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 2, false);
+            }
+            else
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
             tile_store<A, T, true>(lds, dst, is, ws_fold_columns(gc.log_n), P);
         }
 
         template<typename A, int E, int S = 0>
         __global__ void __launch_bounds__(tile / E, E == 8 ? 4 : 1) k_bgv_cols(job_desc base, u32* __restrict__ ws,
-                                                                              pass_geom gc, u64 n, typename A::params P)
+                                                                              pass_geom gc, u64 n, typename A::params P,
+                                                                              u32 const* __restrict__ twx)
         {
             extern __shared__ __align__(16) u32 lds[];
             u32 ct = blockIdx.x % gc.blocks_per_poly;
@@ -424,12 +439,13 @@ namespace hmpc_amd
             J.msg_off += static_cast<u64>(p) << J.log_n;
             std::size_t count = static_cast<std::size_t>(n) << J.log_n;
             u32* dst = ws + which * A::wsl * count;
+            constexpr bool X = std::is_same<A, arith_r26>::value && E == 8 && S == 7;
             if (which == 0)
-                bgv_column_tile<A, E, 0, S>(J, lds, dst, gc, ct, count, p, P);
+                bgv_column_tile<A, E, 0, S, X>(J, lds, dst, gc, ct, count, p, P, twx);
             else if (which == 1)
-                bgv_column_tile<A, E, 1, S>(J, lds, dst, gc, ct, count, p, P);
+                bgv_column_tile<A, E, 1, S, X>(J, lds, dst, gc, ct, count, p, P, twx);
             else
-                bgv_column_tile<A, E, 2, S>(J, lds, dst, gc, ct, count, p, P);
+                bgv_column_tile<A, E, 2, S, X>(J, lds, dst, gc, ct, count, p, P, twx);
         }
 
         template<typename A, int E, int S = 0>
@@ -514,7 +530,9 @@ namespace hmpc_amd
                 pass_geom gr = geom_for(log_n, true);
                 hipError_t e = hipSuccess;
                 // column length 2^7 from N = 2^14 on; row lengths 2^7 .. 2^10 at N = 2^14 .. 2^17
-                auto cols_kernel = gc.log_n1 == 7 ? k_bgv_cols<A, E, 7> : k_bgv_cols<A, E, 0>;
+                // the radix-2^26 columns of 2^7 take the expanded table (bgv_column_tile)
+                bool const cols7 = gc.log_n1 == 7 && (!std::is_same<A, arith_r26>::value || E != 8 || base.twfx);
+                auto cols_kernel = cols7 ? k_bgv_cols<A, E, 7> : k_bgv_cols<A, E, 0>;
                 using AB = typename bgv_rows_arith<A>::type;
                 auto rows_kernel = k_bgv_rows<A, E, 0>;
                 switch (gr.log_n2)
@@ -544,7 +562,8 @@ namespace hmpc_amd
                     return e;
                 {
                     probe_scope ps(s, "fast bgv columns");
-                    cols_kernel<<<static_cast<unsigned>(3 * n * gc.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gc, n, P);
+                    cols_kernel<<<static_cast<unsigned>(3 * n * gc.blocks_per_poly), tile / E, bytes, s>>>(base, ws, gc, n, P,
+                                                                                                        base.twfx);
                 }
                 {
                     probe_scope ps(s, "fast bgv rows");
