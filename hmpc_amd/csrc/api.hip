@@ -1430,6 +1430,7 @@ namespace
         J.twf = T.fwd_fast;
         J.twi = T.inv_fast;
         J.twfx = T.fwd_tw;
+        J.twix = T.inv_tw;
         J.in0 = a;
         J.in1 = b;
         J.in_count = count;
@@ -1458,6 +1459,7 @@ namespace
         J.twf = T.fwd_fast;
         J.twi = T.inv_fast;
         J.twfx = T.fwd_tw;
+        J.twix = T.inv_tw;
         J.in0 = pk_a;
         J.in1 = pk_b;
         J.in_count = u64{1} << log_n;

@@ -252,9 +252,13 @@ namespace hmpc_amd
             static constexpr int rl = Base::rl;
             static constexpr int twl = r26_tw_words;
 
+            // read through the constant address space: the tables are read-only in every kernel, and
+            // a uniform index then gives scalar loads also where the table pointer comes from memory
+            // (a job table), not only from a restrict kernel argument
             __device__ __forceinline__ static void tw_get(u32 (&w)[twl], u32 const* __restrict__ tw, u32 idx)
             {
-                u32 const* p = tw + static_cast<std::size_t>(idx) * r26_tw_stride;
+                using const_u32 = __attribute__((address_space(4))) u32 const;
+                const_u32* p = (const_u32*)(tw) + static_cast<std::size_t>(idx) * r26_tw_stride;  // address-space cast
 #pragma unroll
                 for (int l = 0; l < twl; ++l)
                     w[l] = p[l];
@@ -1402,13 +1406,15 @@ namespace hmpc_amd
         // forward transforms' first round and of the inverse's last round (units of stride 2^6,
         // whose index depends on the row only) are wave-uniform: those rounds take the expanded
         // tables (twfx, twix) through round_wave.
+        // off: the inverse stages before this tile's sub-transform (as in compute_round); scaled: the
+        // round ends the inverse with its scaled final stage
         template<typename A, int E, bool Inv>
         __device__ __forceinline__ void round_expanded(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, int a,
                                                        u32 base0, u32 base_step, u32 const* __restrict__ tw,
-                                                       typename A::params const& P)
+                                                       typename A::params const& P, int off = 0, bool scaled = false)
         {
             gather<A, E, 3>(x, lds, td, a, Inv);
-            round_wave<arith_tw<A>, E, Inv, 3>(x, td, a, opaque_table(tw), P, a, false, base0, base_step);
+            round_wave<arith_tw<A>, E, Inv, 3>(x, td, a, opaque_table(tw), P, off + a, scaled, base0, base_step);
             scatter<A, E, 3>(x, lds, td, a, Inv);
         }
 

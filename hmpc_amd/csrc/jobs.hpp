@@ -29,7 +29,8 @@ namespace hmpc_amd
         u32 p_mont[max_limbs];  // bgv on the CIOS policy: the plaintext modulus as a Montgomery constant
         u32 const* twf;    // forward / inverse twiddles of the policy's table format for log_n
         u32 const* twi;
-        u32 const* twfx;   // radix-2^26: the forward table's first entries in expanded form (r26_mont_tw)
+        u32 const* twfx;   // radix-2^26 (non-null there): the first entries of the forward / inverse
+        u32 const* twix;   // tables in expanded form (r26_mont_tw)
         // polymul: in0 = a, in1 = b (planes of in_count, polynomial at element in_off)
         // bgv:     in0 = public key A, in1 = B (planes of in_count = N, in_off = 0)
         u32 const* in0;

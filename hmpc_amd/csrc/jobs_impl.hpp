@@ -385,6 +385,15 @@ namespace hmpc_amd
         // small batches). Workspace: the three column-pass results [3][wsl][n N].
         //   column pass: blockIdx.x = (p * 3 + which) * tiles + t  (sample stream `which` of ciphertext p)
         //   row pass:    blockIdx.x = p * tiles + t
+        // column tiles of 2^7-point columns (32 per tile) on the radix-2^26 policy at 8 elements per
+        // thread: the forward rounds at stages 0-2 and 3-5 and the inverse rounds at 1-3 and 4-6 have
+        // wave-uniform twiddles and take them in expanded form (as the streaming column passes)
+        template<typename A, int E, int S>
+        constexpr bool cols_expanded()
+        {
+            return std::is_same<A, arith_r26>::value && E == 8 && S == 7;
+        }
+
         template<typename A, int E, int Which, int S = 0, bool X = false>
         __device__ __forceinline__ void bgv_column_tile(job_desc const& J, u32* lds, u32* dst, pass_geom const& gc, u32 ct,
                                                         std::size_t count, u32 p, typename A::params const& P,
@@ -599,11 +608,11 @@ namespace hmpc_amd
             if (S == 7 || (S == 0 && gc.log_n1 == 7))
             {
                 if (which == 0)
-                    bgv_column_tile<A, E, 0, 7>(J, lds, dst, gc, ct, N, 0, P);
+                    bgv_column_tile<A, E, 0, 7, cols_expanded<A, E, 7>()>(J, lds, dst, gc, ct, N, 0, P, J.twfx);
                 else if (which == 1)
-                    bgv_column_tile<A, E, 1, 7>(J, lds, dst, gc, ct, N, 0, P);
+                    bgv_column_tile<A, E, 1, 7, cols_expanded<A, E, 7>()>(J, lds, dst, gc, ct, N, 0, P, J.twfx);
                 else
-                    bgv_column_tile<A, E, 2, 7>(J, lds, dst, gc, ct, N, 0, P);
+                    bgv_column_tile<A, E, 2, 7, cols_expanded<A, E, 7>()>(J, lds, dst, gc, ct, N, 0, P, J.twfx);
             }
             else
             {
@@ -820,7 +829,17 @@ namespace hmpc_amd
             tile_load<A, T>(lds, which ? J.in1 : J.in0, ii);
             __syncthreads();
             u32 x[E][A::rl];
-            tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
+            if constexpr (cols_expanded<A, E, S>())
+            {
+                // stages 0-2 and 3-5 on wave-uniform expanded twiddles, stage 6 as before
+                round_expanded<A, E, false>(x, lds, td, 0, 1u, 0u, J.twfx, P);
+                __syncthreads();
+                round_expanded<A, E, false>(x, lds, td, 3, 1u, 0u, J.twfx, P);
+                __syncthreads();
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 2, false);
+            }
+            else
+                tile_rounds<A, E, false>(x, lds, td, 1u, 0u, J.twf, P, false, 0, false);
             tile_store<A, T, true>(lds, ws + J.ws_off + which * A::wsl * N, is, ws_fold_columns(static_cast<int>(J.log_n)), P);
         }
 
@@ -962,7 +981,18 @@ namespace hmpc_amd
             tile_load<A, T, true>(lds, ws + J.ws_off, is);
             __syncthreads();
             u32 x[E][A::rl];
-            tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false, gc.log_n2);
+            if constexpr (cols_expanded<A, E, S>())
+            {
+                // stage 0 as before, stages 1-3 and 4-6 (with the scaled final stage) on
+                // wave-uniform expanded twiddles
+                tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false, gc.log_n2, 1);
+                round_expanded<A, E, true>(x, lds, td, 1, 1u, 0u, J.twix, P, gc.log_n2);
+                __syncthreads();
+                round_expanded<A, E, true>(x, lds, td, 4, 1u, 0u, J.twix, P, gc.log_n2, true);
+                __syncthreads();
+            }
+            else
+                tile_rounds<A, E, true>(x, lds, td, 1u, 0u, J.twi, P, true, 0, false, gc.log_n2);
             tile_store<A, T>(lds, J.out0, io, true, P);
         }
 
