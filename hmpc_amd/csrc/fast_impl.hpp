@@ -1418,6 +1418,47 @@ namespace hmpc_amd
             scatter<A, E, 3>(x, lds, td, a, Inv);
         }
 
+        // one-wave row tiles of 2^9 (one row per wave) take their first forward and last inverse
+        // round on expanded twiddles
+        template<typename A, int E, int S>
+        constexpr bool rows_expanded()
+        {
+            return A::wave_blocks && S == 9 && E == 8;
+        }
+
+        // the forward rounds of a row tile (twx: the expanded forward table, read when
+        // rows_expanded)
+        template<typename A, int E, int S, bool WL>
+        __device__ __forceinline__ void row_rounds_forward(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
+                                                           u32 base_step, u32 const* __restrict__ tw, u32 const* __restrict__ twx,
+                                                           typename A::params const& P, bool keep_last)
+        {
+            constexpr bool XW = rows_expanded<A, E, S>();
+            if constexpr (XW)
+            {
+                round_expanded<A, E, false>(x, lds, td, 0, base0, base_step, twx, P);
+                tile_sync<WL>();
+            }
+            tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, tw, P, false, XW ? 1 : 0, keep_last);
+        }
+
+        // the inverse rounds after the first (which ran on registers, dispatch_round_regs); the last
+        // (stages 6-8) on the expanded inverse table twx when rows_expanded
+        template<typename A, int E, int S, bool WL>
+        __device__ __forceinline__ void row_rounds_inverse_rest(u32 (&x)[E][A::rl], u32* lds, tile_desc const& td, u32 base0,
+                                                                u32 base_step, u32 const* __restrict__ tw,
+                                                                u32 const* __restrict__ twx, typename A::params const& P)
+        {
+            if constexpr (rows_expanded<A, E, S>())
+            {
+                tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, tw, P, false, 1, false, 0, 2);
+                round_expanded<A, E, true>(x, lds, td, 6, base0, base_step, twx, P);
+                tile_sync<WL>();
+            }
+            else
+                tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, tw, P, false, 1, false);
+        }
+
         template<typename A, int E, bool Single, int S = 0>
         __global__ void __launch_bounds__(A::wave_blocks ? 64 : tile / E, A::wave_blocks ? 3 : ((E == 8 && !Single) ? 4 : 1)) k_polymul(u32 const* a, u32 const* __restrict__ b,
                                                               u32* out, u32 const* __restrict__ twf,
@@ -1481,29 +1522,17 @@ namespace hmpc_amd
                 // own, with no workgroup barrier.
                 constexpr bool WL = A::wave_units && S > 0 && (1 << S) <= 64 * E;
                 u32 x[E][A::rl];
-                // rows of 2^9, one per wave: the first forward and last inverse rounds on expanded twiddles
-                constexpr bool XW = A::wave_blocks && S == 9 && E == 8;
                 if constexpr (A::wave_blocks)
                 {
                     // one-wave rows: NTT(a) stays in registers next to NTT(b)
                     u32 xa[E][A::rl];
                     tile_load<A, T, true, WL>(lds, a, io);
                     tile_sync<WL>();
-                    if constexpr (XW)
-                    {
-                        round_expanded<A, E, false>(xa, lds, td, 0, base0, base_step, twfx, P);
-                        tile_sync<WL>();
-                    }
-                    tile_rounds<A, E, false, WL>(xa, lds, td, base0, base_step, twf, P, false, XW ? 1 : 0, true);
+                    row_rounds_forward<A, E, S, WL>(xa, lds, td, base0, base_step, twf, twfx, P, true);
                     tile_sync<WL>();
                     tile_load<A, T, true, WL>(lds, b, io);
                     tile_sync<WL>();
-                    if constexpr (XW)
-                    {
-                        round_expanded<A, E, false>(x, lds, td, 0, base0, base_step, twfx, P);
-                        tile_sync<WL>();
-                    }
-                    tile_rounds<A, E, false, WL>(x, lds, td, base0, base_step, twf, P, false, XW ? 1 : 0, true);
+                    row_rounds_forward<A, E, S, WL>(x, lds, td, base0, base_step, twf, twfx, P, true);
 #pragma unroll
                     for (int k = 0; k < E; ++k)
                     {
@@ -1527,15 +1556,7 @@ namespace hmpc_amd
                 }
                 dispatch_round_regs<A, E>(w0, x, lds, td, base0, base_step, twi, P, false);
                 tile_sync<WL>();
-                if constexpr (XW)
-                {
-                    // rounds (0-2) done above, (3-5) here, (6-8) on expanded twiddles
-                    tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false, 0, 2);
-                    round_expanded<A, E, true>(x, lds, td, 6, base0, base_step, twix, P);
-                    tile_sync<WL>();
-                }
-                else
-                    tile_rounds<A, E, true, WL>(x, lds, td, base0, base_step, twi, P, false, 1, false);
+                row_rounds_inverse_rest<A, E, S, WL>(x, lds, td, base0, base_step, twi, twix, P);
                 tile_store<A, T, true, WL>(lds, out, io, false, P);
             }
         }

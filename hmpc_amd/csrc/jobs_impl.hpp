@@ -494,13 +494,13 @@ namespace hmpc_amd
                 u32 u[E][A::rl];
                 tile_load<A, T, true, WL>(lds, su, is);
                 tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(u, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                row_rounds_forward<A, E, S, WL>(u, lds, td, base0, 1u, base.twf, base.twfx, P, true);
                 tile_sync<WL>();
                 for (int which = 1; which < 3; ++which)
                 {
                     tile_load<A, T, true, WL>(lds, ws + which * A::wsl * count, is);
                     tile_sync<WL>();
-                    tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, base.twf, P, false, 0, true);
+                    row_rounds_forward<A, E, S, WL>(x, lds, td, base0, 1u, base.twf, base.twfx, P, true);
                     bool first = which == 1;
                     bgv_emit_regs<A, E, WL>(x, u, lds, td, first ? base.in0 : base.in1, ik, first ? base.out0 : base.out1, io, P);
                 }
@@ -557,7 +557,9 @@ namespace hmpc_amd
                 std::size_t rows_bytes = bytes;
                 if constexpr (std::is_same<A, arith_r26>::value && E == 8)
                 {
-                    if (gr.log_n2 >= 7 && gr.log_n2 <= 9)
+                    // rows of 2^9 read the expanded forward table up to entry 8 * 2^log_n1
+                    bool const expanded = base.twfx && (std::size_t{8} << gr.log_n1) <= r26_tw_entries;
+                    if (gr.log_n2 >= 7 && gr.log_n2 <= 9 && (gr.log_n2 < 9 || expanded))
                     {
                         rows_kernel = gr.log_n2 == 7   ? k_bgv_rows<arith_r26_bgv_rows1, E, 7>
                                       : gr.log_n2 == 8 ? k_bgv_rows<arith_r26_bgv_rows1, E, 8>
@@ -652,13 +654,13 @@ namespace hmpc_amd
                 u32 u[E][A::rl];
                 tile_load<A, T, true, WL>(lds, su, is);
                 tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(u, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                row_rounds_forward<A, E, S, WL>(u, lds, td, base0, 1u, J.twf, J.twfx, P, true);
                 tile_sync<WL>();
                 for (int which = 1; which < 3; ++which)
                 {
                     tile_load<A, T, true, WL>(lds, su + which * A::wsl * N, is);
                     tile_sync<WL>();
-                    tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                    row_rounds_forward<A, E, S, WL>(x, lds, td, base0, 1u, J.twf, J.twfx, P, true);
                     bool first = which == 1;
                     bgv_emit_regs<A, E, WL>(x, u, lds, td, first ? J.in0 : J.in1, ik, first ? J.out0 : J.out1, io, P);
                 }
@@ -888,11 +890,11 @@ namespace hmpc_amd
                 u32 xa[E][A::rl];
                 tile_load<A, T, true, WL>(lds, sa, is);
                 tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(xa, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                row_rounds_forward<A, E, S, WL>(xa, lds, td, base0, 1u, J.twf, J.twfx, P, true);
                 tile_sync<WL>();
                 tile_load<A, T, true, WL>(lds, sb, is);
                 tile_sync<WL>();
-                tile_rounds<A, E, false, WL>(x, lds, td, base0, 1u, J.twf, P, false, 0, true);
+                row_rounds_forward<A, E, S, WL>(x, lds, td, base0, 1u, J.twf, J.twfx, P, true);
 #pragma unroll
                 for (int k = 0; k < E; ++k)
                 {
@@ -916,7 +918,7 @@ namespace hmpc_amd
             }
             dispatch_round_regs<A, E>(round_width<WM>(td.s, 0, true), x, lds, td, base0, 1u, J.twi, P, false);
             tile_sync<WL>();
-            tile_rounds<A, E, true, WL>(x, lds, td, base0, 1u, J.twi, P, false, 1, false);
+            row_rounds_inverse_rest<A, E, S, WL>(x, lds, td, base0, 1u, J.twi, J.twix, P);
             tile_store<A, T, true, WL>(lds, sa, is, false, P);
         }
 

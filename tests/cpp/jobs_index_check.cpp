@@ -693,6 +693,14 @@ void rows1_blocks()
 // every index must lie inside the expanded tables (r26_tw_entries).
 void rows1_uniform()
 {
+    // every degree whose rows have 2^9 elements has columns of at most 2^7: the rows' indices
+    // (base 2^log_n1 + row) stay inside the expanded tables
+    for (int log_n = 8; log_n <= 20; ++log_n)
+    {
+        pass_geom const g = geom_for(log_n, true);
+        if (g.log_n2 == 9)
+            require((std::size_t{8} << g.log_n1) <= r26_tw_entries, "rows of 2^9 beyond the expanded tables");
+    }
     tile_desc const td{9, tile_log - 9, true};
     for (int inv = 0; inv < 2; ++inv)
     {
