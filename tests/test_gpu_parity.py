@@ -332,6 +332,46 @@ def test_pointwise_ops_edge_values(ctx, q):
         assert np.array_equal(got, want), hex(c)
 
 
+def _structured_pairs(q, limbs, N):
+    """pairs of inputs at the value extremes: constants q - 1 and (q - 1) / 2, alternating q - 1 / 0
+    and q - 1 / 1, an impulse, the edge values of _edge_values cycled and reversed, and random
+    values against q - 1"""
+    def const(v):
+        return [v] * N
+
+    def alt(v, w):
+        return [v if i % 2 == 0 else w for i in range(N)]
+
+    rng = np.random.default_rng(11)
+    edges = _edge_values(q, limbs, 64, 3)
+    cyc = [edges[i % len(edges)] for i in range(N)]
+    rnd = [int.from_bytes(rng.bytes(4 * limbs), "little") % q for _ in range(N)]
+    impulse = [1] + [0] * (N - 1)
+    return [(const(q - 1), const(q - 1)), (alt(q - 1, 0), const(1)), (const((q - 1) // 2), alt(q - 1, 1)),
+            (cyc, cyc[::-1]), (impulse, const(q - 1)), (rnd, const(q - 1))]
+
+
+# the radix-2^26 lengths whose passes take expanded twiddles (streaming columns at N1 = 2^7, the
+# one-wave rows of 2^9 at N = 2^16) and their neighbours
+@pytest.mark.parametrize("q,log_n", [(Q128A, 16), (Q128A, 17), (Q128A, 15), (Q128B, 14)],
+                         ids=lambda v: hex(v)[:10] if isinstance(v, int) and v > 99 else str(v))
+def test_polymul_structured_inputs(ctx, q, log_n):
+    """Products of inputs at the value extremes through the default fast schedule, bit for bit
+    against the oracle, and the forward transform of the same inputs."""
+    gm, om = mods(ctx, q)
+    N = 1 << log_n
+    pairs = _structured_pairs(q, gm.limbs, N)
+    n = len(pairs)
+    say(f"polymul structured q={hex(q)[:12]} log_n={log_n} n={n}")
+    a = oracle.ints_to_planar([v for x, _ in pairs for v in x], gm.limbs)
+    b = oracle.ints_to_planar([v for _, y in pairs for v in y], gm.limbs)
+    want, _ = om.polymul_timed(log_n, a, b, threads=8)
+    got = gm.download(gm.polymul(log_n, n, gm.upload(a), gm.upload(b)), n * N)
+    assert np.array_equal(got, want)
+    fwd = gm.download(gm.ntt(log_n, n, gm.upload(a)), n * N)
+    assert np.array_equal(fwd, om.ntt(a, log_n, threads=8))
+
+
 FAST_CASES = [(Q128A, 4, 5), (Q128A, 12, 3), (Q128A, 13, 2), (Q128A, 15, 2), (Q128A, 16, 2), (Q128A, 17, 1), (Q128B, 14, 2),
               (Q64, 3, 9), (Q64, 12, 3), (Q64, 14, 2), (Q64, 15, 2), (Q64, 16, 1)]
 
