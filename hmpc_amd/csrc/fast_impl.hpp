@@ -1598,9 +1598,9 @@ namespace hmpc_amd
         //
         // The prefetch only overlaps if no later vector load waits behind it (vector loads complete
         // in issue order), so these passes take no twiddle from a vector load inside the loop:
-        //   - the 3-stage rounds have one twiddle per stage and wave in this geometry (forward
-        //     stages 0-5, inverse stages 1-6: the unit's top bits F are 0 or the wave index), loaded
-        //     with scalar loads from lane 0's index;
+        //   - the 3-stage rounds have wave-uniform twiddles in this geometry (forward stages 0-5,
+        //     inverse stages 1-6: the unit's top bits F are 0 or the wave index), taken in expanded
+        //     form (arith_tw, r26_mont_tw) with scalar loads from lane 0's index;
         //   - forward stage 6 and inverse stage 0 run in registers in the pair layout
         //     (cols_pair_row: one v_permlane32_swap per word and pair from or to the 3-stage
         //     layout instead of an LDS exchange and a barrier); their twiddles (two per wave and
