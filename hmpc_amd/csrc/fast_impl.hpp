@@ -342,6 +342,8 @@ namespace hmpc_amd
         {
             static constexpr bool wave_units = true;
             static constexpr bool wave_blocks = true;
+            // whole-round twiddle loads: 146 VGPRs, still 3 waves per SIMD, rows 0.66 -> 0.65 ms
+            static constexpr bool stage_prefetch = false;
 
             __device__ __forceinline__ static int phys(int e) { return swizzle_rows(e) & 511; }
 
