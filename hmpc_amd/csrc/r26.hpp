@@ -36,10 +36,11 @@
 //               (S + 2) x those: a mont input needs |limbs| < 2^31, so up to 29 stages (a whole
 //               transform of N <= 2^29, both of its passes) run without reductions. With
 //               t = mont_tw(y, V) limb 4 grows by < 5 A / 2^6 + 2^20 + 2^8 per stage instead
-//               (A: y's largest limb); the streaming column passes' schedule (their 3-stage
-//               rounds on mont_tw, the rest on mont) is checked stage by stage for N = 2^14..2^17
-//               in tests/test_r26_host.py: every mont and mont_tw input keeps |limbs| < 2^30.2,
-//               every folded sum |x_4| < 2^29.6
+//               (A: y's largest limb); the product's schedule (the streaming column passes'
+//               3-stage rounds and, at rows of 2^9, the row kernel's first forward and last
+//               inverse rounds on mont_tw, the rest on mont) is checked stage by stage for
+//               N = 2^14..2^17 in tests/test_r26_host.py: every mont and mont_tw input keeps
+//               |limbs| < 2^30.2, every folded sum |x_4| < 2^29.6
 //   fold(x):    normalize, then x_4 >= 2^20 carries h = x_4 >> 20 copies of 2^124 = q - c:
 //               replace by -h c (plus one q in borrowed digits to keep limbs >= 0); result < 2q
 //               with limbs < 2^28, for any limbs < 2^31. fold_signed adds 1024 q = 2^134 + 1024 c

@@ -333,11 +333,18 @@ def _mont_bound(x):
 
 
 def _stream_kinds(rows):
-    """the product of each stage in the streaming column passes' schedule of a transform of
-    N = 2^(7 + rows): forward columns (6 rounds' stages on mont_tw, stage 6 on mont), row stages
-    (mont); inverse rows (mont), columns' stage 0 (mont), stages 1-5 (mont_tw); the scaled last
-    stage (mont) is applied separately"""
-    return ["tw"] * 6 + ["mont"] * (1 + rows), ["mont"] * (rows + 1) + ["tw"] * 5
+    """the product of each stage in the product's schedule for N = 2^(7 + rows): forward columns
+    (stages 0-5 on mont_tw, stage 6 on mont), then the row stages: mont, except at rows of 2^9 (one
+    row per wave) the first round (row stages 0-2, wave-uniform twiddles) on mont_tw; the inverse
+    rows likewise (stages 6-8 on mont_tw at rows of 2^9), then the columns' stage 0 (mont) and
+    stages 1-5 (mont_tw); the scaled last stage (mont) is applied separately"""
+    if rows == 9:
+        fwd_rows = ["tw"] * 3 + ["mont"] * 6
+        inv_rows = ["mont"] * 6 + ["tw"] * 3
+    else:
+        fwd_rows = ["mont"] * rows
+        inv_rows = ["mont"] * rows
+    return ["tw"] * 6 + ["mont"] + fwd_rows, inv_rows + ["mont"] + ["tw"] * 5
 
 
 @pytest.mark.parametrize("rows", [7, 8, 9, 10])
@@ -408,7 +415,7 @@ def _chain(tool, q, cmd, x, y, stages, rng, rows):
 
 
 @pytest.mark.parametrize("q", PRIMES)
-@pytest.mark.parametrize("rows", [7, 10])
+@pytest.mark.parametrize("rows", [7, 9, 10])
 def test_stream_schedule_chains(tool, q, rows):
     """The streaming schedule on pairs through the host build: forward chains (mont_tw and mont
     stages as in the columns and rows), a point-wise product of two forward outputs, then inverse
