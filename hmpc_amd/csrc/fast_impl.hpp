@@ -249,6 +249,7 @@ namespace hmpc_amd
         struct arith_tw : Base
         {
             using typename Base::params;
+            static_assert(std::is_same<params, p26>::value, "expanded twiddles are a radix-2^26 form");
             static constexpr int rl = Base::rl;
             static constexpr int twl = r26_tw_words;
 
@@ -1425,7 +1426,7 @@ namespace hmpc_amd
         template<typename A, int E, int S>
         constexpr bool rows_expanded()
         {
-            return A::wave_blocks && S == 9 && E == 8;
+            return std::is_same<typename A::params, p26>::value && A::wave_blocks && S == 9 && E == 8;
         }
 
         // the forward rounds of a row tile (twx: the expanded forward table, read when
